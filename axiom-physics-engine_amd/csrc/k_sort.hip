@@ -1,0 +1,230 @@
+// Device-wide exclusive scan and stable LSD radix sort for gfx950 (wave64).
+//
+// Radix sort: 8-bit digits, 256-thread workgroups over 4096-key tiles.
+//   pass = histogram (LDS atomics) -> exclusive scan of the digit-major
+//   histogram -> stable scatter.  Stability comes from ranking keys in index
+//   order: each wave finds the lanes sharing its digit with 8 ballots
+//   (a match-any over the digit bits), ranks itself by the population count of
+//   lower peer lanes, and per-wave digit counts are prefixed in LDS.
+// Scan: per-tile reduction, a single-workgroup scan of tile sums, then a
+//   down-sweep with wave-level shuffles.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "launch.hpp"
+
+namespace axm {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Inclusive wave scan (wave64) of a u32.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += t;
+    }
+    return v;
+}
+
+// Exclusive block scan (blockDim = 256 = 4 waves); returns the block total in `total`.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t s_w[RS_THREADS / 64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; ++w) {
+        const uint32_t x = s_w[w];
+        if ((uint32_t)w < wv) base += x;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return base + inc - v;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                             uint32_t* __restrict__ sums) {
+    const uint32_t base = blockIdx.x * SC_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) {
+        const uint32_t i = base + it * SC_THREADS + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan(s, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of nb tile sums in place; sums[nb] = total.
+__global__ void __launch_bounds__(SC_THREADS) k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb) {
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += SC_THREADS) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, tot);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                           uint32_t n, const uint32_t* __restrict__ sums, uint32_t nb) {
+    // Each thread owns SC_ITEMS consecutive elements of the tile.
+    const uint32_t base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        const uint32_t i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = sums[blockIdx.x] + block_excl_scan(s, tot);
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        const uint32_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == SC_THREADS - 1) out[n] = sums[nb];
+}
+
+size_t scan_tmp_words(uint32_t n) { return (size_t)((n + SC_TILE - 1) / SC_TILE) + 1; }
+
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s) {
+    if (n == 0) {
+        hipMemsetAsync(out, 0, sizeof(uint32_t), s);
+        return;
+    }
+    const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb);
+}
+
+// ---------------------------------------------------------------- radix sort
+template <class K> __device__ __forceinline__ uint32_t digit_of(K k, int shift) { return (uint32_t)(k >> shift) & 0xFFu; }
+
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ keys, uint32_t n, int shift,
+                                                        uint32_t* __restrict__ hist, uint32_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const uint32_t i = base + it * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[digit_of(keys[i], shift)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                           K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                           int shift, const uint32_t* __restrict__ offs, uint32_t nb) {
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_wc[RS_THREADS / 64][256];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    s_off[tid] = offs[(size_t)tid * nb + blockIdx.x];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const uint32_t i = base + it * RS_THREADS + tid;
+        const bool valid = i < n;
+        K k = 0;
+        uint32_t v = 0, d = 256u;
+        if (valid) {
+            k = kin[i];
+            v = vin[i];
+            d = digit_of(k, shift);
+        }
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; ++w) s_wc[w][tid] = 0;
+        // lanes with the same digit (9 bits so that invalid lanes, d = 256, form their own group)
+        uint64_t peers = __ballot(1);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        __syncthreads();
+        if (valid && rank == 0) s_wc[wv][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = s_off[d] + rank;
+            for (uint32_t w = 0; w < wv; ++w) pos += s_wc[w][d];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; ++w) add += s_wc[w][tid];
+        s_off[tid] += add;
+        if (valid) {
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+    }
+}
+
+size_t radix_tmp_words(uint32_t n) {
+    const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return (size_t)256 * nb + 1 + scan_tmp_words(256u * nb);
+}
+
+template <class K>
+static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
+                           uint32_t* tmp, hipStream_t s) {
+    if (n == 0) return 0;
+    const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t* hist = tmp;
+    uint32_t* offs = tmp;  // scanned in place into the second half
+    uint32_t* scan_tmp = tmp + (size_t)256 * nb + 1;
+    K* ki = keys;
+    K* ko = keys_alt;
+    uint32_t* vi = vals;
+    uint32_t* vo = vals_alt;
+    int passes = 0;
+    for (int shift = 0; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, shift, hist, nb);
+        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s);
+        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, shift, offs, nb);
+        K* tk = ki; ki = ko; ko = tk;
+        uint32_t* tv = vi; vi = vo; vo = tv;
+        ++passes;
+    }
+    return passes & 1;
+}
+
+int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
+                   uint32_t* tmp, hipStream_t s) {
+    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s);
+}
+
+int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
+                   uint32_t* tmp, hipStream_t s) {
+    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s);
+}
+
+}  // namespace axm

@@ -1,0 +1,775 @@
+// gfx950 kernels of one simulation step (single world or a batch of worlds).
+//
+// Stage order per step (the reference sketches the same four stages,
+// examples/profiling_example.cpp:132-165, with semi-implicit Euler from
+// examples/body_inspector_example.cpp:48-60):
+//   k_begin            gravity/damping -> solver velocities, world inverse inertia, AABB, cell key
+//   radix sort         (key, body) stable, 8-bit digits
+//   k_cell_table       hash of cell -> (first sorted index, count)
+//   k_pair_count/emit  27-neighbour grid scan, inclusive AABB test, j > i; exclusive scan between
+//   k_narrow(+_epa)    primitive narrowphase -> <= 4 points per pair (staged per pair)
+//   k_compact          pairs with points -> manifolds (pair order, exclusive scan of the flags)
+//   k_col_*            Jones-Plassmann colouring with first-fit colours
+//   k_prep             per-point rows + warm start (previous-step impulses keyed by (a, b, feature))
+//   k_pgs              one colour of one sequential-impulse sweep
+//   k_residual         velocity-space fixed-point residual
+//   k_integrate        x += v dt, q += dt/2 (w,0) q, normalise
+// Every body-parallel kernel is one thread per body; every manifold-parallel
+// kernel one thread per manifold.  Colour batches are contiguous in the
+// solve-order arrays, so a wave of 64 manifolds of one colour reads 64
+// consecutive 16-byte rows.
+#include "dev_geom.hpp"
+#include "k_step.hpp"
+
+namespace axm {
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// ---------------------------------------------------------------- k_begin
+__global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
+                                               uint32_t* __restrict__ val) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x4 = B.x[i], q4v = B.q[i], v4 = B.vel[i], w4 = B.angv[i], ii = B.ii[i], sh = B.shape[i];
+    const uint32_t type = fbits(sh.w);
+    const bool dyn = v4.w > 0.0f;
+    f3 v = xyz(v4), w = xyz(w4);
+    if (dyn) {
+        const f3 g = {sp.gx, sp.gy, sp.gz};
+        v = (v + g * sp.dt) * sp.lin_keep;
+        w = w * sp.ang_keep;
+    }
+    B.sv[i] = f4(v, v4.w);
+    B.sw[i] = f4(w, 0.0f);
+    const m3 R = quat_to_mat(q4v);
+    const s3 I = world_inv_inertia(R, xyz(ii));
+    B.I0[i] = make_float4(I.xx, I.yy, I.zz, I.xy);
+    B.I1[i] = make_float4(I.xz, I.yz, 0.0f, 0.0f);
+    const float mg = sp.margin;
+    f3 e;
+    if (type == SHAPE_SPHERE) {
+        e = {sh.x + mg, sh.x + mg, sh.x + mg};
+    } else if (type == SHAPE_BOX) {
+        e.x = fabsf(R.r0.x) * sh.x + fabsf(R.r0.y) * sh.y + fabsf(R.r0.z) * sh.z + mg;
+        e.y = fabsf(R.r1.x) * sh.x + fabsf(R.r1.y) * sh.y + fabsf(R.r1.z) * sh.z + mg;
+        e.z = fabsf(R.r2.x) * sh.x + fabsf(R.r2.y) * sh.y + fabsf(R.r2.z) * sh.z + mg;
+    } else {
+        e.x = fabsf(R.r0.y) * sh.y + sh.x + mg;
+        e.y = fabsf(R.r1.y) * sh.y + sh.x + mg;
+        e.z = fabsf(R.r2.y) * sh.y + sh.x + mg;
+    }
+    const f3 c = xyz(x4);
+    B.amin[i] = f4(c - e, 0.0f);
+    B.amax[i] = f4(c + e, 0.0f);
+    auto cc = [&](float p, float o) -> uint32_t {
+        float f = (p - o) * sp.inv_cell;
+        f = f > 0.0f ? f : 0.0f;
+        f = f < 1023.0f ? f : 1023.0f;
+        return (uint32_t)floorf(f);
+    };
+    const uint32_t cx = cc(c.x, sp.ox), cy = cc(c.y, sp.oy), cz = cc(c.z, sp.oz);
+    key[i] = ((uint64_t)B.world[i] << 30) | morton3(cx, cy, cz);
+    val[i] = i;
+}
+
+// ---------------------------------------------------------------- cell table
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void ht_insert(CellTable t, uint64_t k, uint2 v) {
+    uint32_t s = (uint32_t)mix64(k) & t.mask;
+    for (;;) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&t.key[s], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+        if (prev == EMPTY_KEY || prev == k) {
+            t.val[s] = v;
+            return;
+        }
+        s = (s + 1) & t.mask;
+    }
+}
+__device__ __forceinline__ bool ht_find(const CellTable& t, uint64_t k, uint2& v) {
+    uint32_t s = (uint32_t)mix64(k) & t.mask;
+    for (;;) {
+        const uint64_t kk = t.key[s];
+        if (kk == k) {
+            v = t.val[s];
+            return true;
+        }
+        if (kk == EMPTY_KEY) return false;
+        s = (s + 1) & t.mask;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint64_t k = skey[s];
+    if (s > 0 && skey[s - 1] == k) return;
+    uint32_t c = 1;
+    while (s + c < n && skey[s + c] == k) ++c;
+    ht_insert(t, k, make_uint2(s, c));
+}
+
+// ---------------------------------------------------------------- pairs
+// Visits candidate partners of the body at sorted position t in the canonical
+// order: planes, then the 27 neighbour cells (dz, dy, dx in -1..1), then
+// ascending sorted position.  f(partner) is called for every emitted pair.
+template <class F>
+__device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
+                                            const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t, F&& f) {
+    const uint32_t i = sval[t];
+    const uint64_t key = skey[t];
+    const float4 mn = B.amin[i], mx = B.amax[i];
+    const bool dyn_i = B.vel[i].w > 0.0f;
+    if (dyn_i) {
+        const f3 c = xyz(B.x[i]);
+        const f3 e = xyz(mx) - c;
+        for (uint32_t k = 0; k < sp.num_planes; ++k) {
+            const float4 pl = sp.planes[k];
+            const float s = (pl.x * c.x + pl.y * c.y + pl.z * c.z) -
+                            (fabsf(pl.x) * e.x + fabsf(pl.y) * e.y + fabsf(pl.z) * e.z);
+            if (s <= pl.w) f(PLANE_BIT | k);
+        }
+    }
+    const uint32_t mort = (uint32_t)(key & 0x3FFFFFFFull);
+    const uint64_t wbits = key & ~0x3FFFFFFFull;
+    const int cx = (int)compact1by2(mort), cy = (int)compact1by2(mort >> 1), cz = (int)compact1by2(mort >> 2);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int Z = cz + dz;
+        if (Z < 0 || Z > 1023) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int Y = cy + dy;
+            if (Y < 0 || Y > 1023) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int X = cx + dx;
+                if (X < 0 || X > 1023) continue;
+                const uint64_t nk = wbits | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
+                uint2 r;
+                if (!ht_find(ct, nk, r)) continue;
+                for (uint32_t s = r.x; s < r.x + r.y; ++s) {
+                    const uint32_t j = sval[s];
+                    if (j <= i) continue;
+                    const bool dyn_j = B.vel[j].w > 0.0f;
+                    if (!dyn_i && !dyn_j) continue;
+                    const float4 mnj = B.amin[j], mxj = B.amax[j];
+                    if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
+                        mx.z >= mnj.z)
+                        f(j);
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
+                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
+                                                    uint32_t* __restrict__ cnt) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t c = 0;
+    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t) { ++c; });
+    cnt[t] = c;
+}
+
+__global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
+                                                   const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
+                                                   const uint32_t* __restrict__ off, uint32_t cap,
+                                                   uint32_t* __restrict__ pa, uint32_t* __restrict__ pb) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t o = off[t];
+    const uint32_t i = sval[t];
+    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t j) {
+        if (o < cap) {
+            pa[o] = i;
+            pb[o] = j;
+        }
+        ++o;
+    });
+}
+
+// ---------------------------------------------------------------- narrowphase
+__device__ __forceinline__ Shp load_shape(const Bodies& B, uint32_t i) {
+    Shp s;
+    const float4 sh = B.shape[i];
+    s.c = xyz(B.x[i]);
+    s.R = quat_to_mat(B.q[i]);
+    s.type = fbits(sh.w);
+    s.prm = xyz(sh);
+    return s;
+}
+
+__device__ __forceinline__ void store_pair(uint32_t p, const Man& m, uint32_t* __restrict__ p_np,
+                                           float4* __restrict__ p_pt, float4* __restrict__ p_nf) {
+    p_np[p] = m.np;
+    for (uint32_t k = 0; k < m.np; ++k) {
+        p_pt[(size_t)p * 4 + k] = f4(m.pt[k].p, m.pt[k].depth);
+        p_nf[(size_t)p * 4 + k] = f4(m.pt[k].n, __uint_as_float(m.pt[k].fid));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
+                                                const uint32_t* __restrict__ pb, uint32_t P,
+                                                uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
+                                                float4* __restrict__ p_nf, uint32_t* __restrict__ epa_list,
+                                                Counters* __restrict__ cnt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t a = pa[p], b = pb[p];
+    Man m;
+    m.np = 0;
+    const Shp A = load_shape(B, a);
+    if (b & PLANE_BIT) {
+        const float4 pl = sp.planes[b & ~PLANE_BIT];
+        collide_plane(m, A, f3{pl.x, pl.y, pl.z}, pl.w, sp.margin);
+    } else {
+        const Shp Bs = load_shape(B, b);
+        bool need_epa = false;
+        int st;
+        const bool swap = A.type > Bs.type;
+        st = swap ? collide_canon(m, Bs, A, sp.margin, need_epa) : collide_canon(m, A, Bs, sp.margin, need_epa);
+        if (need_epa) {
+            const uint32_t slot = atomicAdd(&cnt->epa_count, 1u);
+            epa_list[slot] = p;
+        } else if (swap) {
+            flip_manifold(m);
+        }
+        if (st == GEOM_GJK_FAIL) atomicAdd(&cnt->gjk_fail, 1u);
+    }
+    store_pair(p, m, p_np, p_pt, p_nf);
+}
+
+// Deep box-capsule pairs (cores intersect): EPA, one thread per queued pair.
+__global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
+                                                   const uint32_t* __restrict__ pb,
+                                                   const uint32_t* __restrict__ epa_list, uint32_t cap,
+                                                   uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
+                                                   float4* __restrict__ p_nf, Counters* __restrict__ cnt) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nq = cnt->epa_count;
+    if (q >= nq || q >= cap) return;
+    const uint32_t p = epa_list[q];
+    const uint32_t a = pa[p], b = pb[p];
+    Shp A = load_shape(B, a), Bs = load_shape(B, b);
+    const bool swap = A.type > Bs.type;
+    EpaWork W;
+    Man m;
+    m.np = 0;
+    const int st = swap ? box_capsule_epa(m, Bs, A, W) : box_capsule_epa(m, A, Bs, W);
+    if (swap) flip_manifold(m);
+    if (st == GEOM_EPA_FAIL) atomicAdd(&cnt->epa_fail, 1u);
+    store_pair(p, m, p_np, p_pt, p_nf);
+}
+
+__global__ void __launch_bounds__(256) k_flag_nonzero(const uint32_t* __restrict__ np, uint32_t P,
+                                                      uint32_t* __restrict__ flag) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) flag[p] = np[p] > 0 ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
+                                                 uint32_t P, const uint32_t* __restrict__ p_np,
+                                                 const float4* __restrict__ p_pt, const float4* __restrict__ p_nf,
+                                                 const uint32_t* __restrict__ moff, uint32_t cap,
+                                                 uint2* __restrict__ m_ab, uint32_t* __restrict__ m_np,
+                                                 float4* __restrict__ c_pt, float4* __restrict__ c_nf,
+                                                 Counters* __restrict__ cnt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const uint32_t np = p_np[p];
+    if (np == 0) return;
+    const uint32_t m = moff[p];
+    atomicAdd(&cnt->contacts, np);
+    if (m >= cap) return;
+    m_ab[m] = make_uint2(pa[p], pb[p]);
+    m_np[m] = np;
+    for (uint32_t k = 0; k < np; ++k) {
+        c_pt[(size_t)m * 4 + k] = p_pt[(size_t)p * 4 + k];
+        c_nf[(size_t)m * 4 + k] = p_nf[(size_t)p * 4 + k];
+    }
+}
+
+// ---------------------------------------------------------------- colouring
+__device__ __forceinline__ bool dyn_body(const Bodies& B, uint32_t b) {
+    return !(b & PLANE_BIT) && B.vel[b].w > 0.0f;
+}
+__device__ __forceinline__ uint32_t local_id(const Bodies& B, uint32_t b) {
+    return (b & PLANE_BIT) ? b : b - B.world_first[B.world[b]];
+}
+
+__global__ void __launch_bounds__(256) k_col_init(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
+                                                  uint32_t* __restrict__ col, uint64_t* __restrict__ prio) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const uint2 ab = m_ab[m];
+    col[m] = UNSET;
+    prio[m] = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
+}
+
+__global__ void __launch_bounds__(256) k_col_max(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
+                                                 const uint32_t* __restrict__ col, const uint64_t* __restrict__ prio,
+                                                 uint64_t* __restrict__ bmax, uint32_t round) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M || col[m] != UNSET) return;
+    const uint2 ab = m_ab[m];
+    const uint64_t t = ((uint64_t)(round + 1) << 48) | prio[m];
+    atomicMax((unsigned long long*)&bmax[ab.x], (unsigned long long)t);
+    if (dyn_body(B, ab.y)) atomicMax((unsigned long long*)&bmax[ab.y], (unsigned long long)t);
+}
+
+__global__ void __launch_bounds__(256) k_col_win(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
+                                                 uint32_t* __restrict__ col, const uint64_t* __restrict__ prio,
+                                                 const uint64_t* __restrict__ bmax, uint64_t* __restrict__ bmask,
+                                                 uint32_t round, Counters* __restrict__ cnt) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M || col[m] != UNSET) return;
+    const uint2 ab = m_ab[m];
+    const uint64_t t = ((uint64_t)(round + 1) << 48) | prio[m];
+    const bool bd = dyn_body(B, ab.y);
+    if (bmax[ab.x] != t || (bd && bmax[ab.y] != t)) return;
+    const uint64_t used = bmask[ab.x] | (bd ? bmask[ab.y] : 0ull);
+    uint32_t c;
+    if (~used == 0ull) {
+        c = 64u + round;
+    } else {
+        c = (uint32_t)__ffsll((unsigned long long)~used) - 1u;
+        bmask[ab.x] = bmask[ab.x] | (1ull << c);
+        if (bd) bmask[ab.y] = bmask[ab.y] | (1ull << c);
+    }
+    col[m] = c;
+    atomicSub(&cnt->remaining, 1u);
+    atomicMax(&cnt->max_colour, c + 1u);
+    atomicMax(&cnt->last_round, round + 1u);
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+// colour offsets: off[c] = first sorted position with colour >= c (c = 0..K)
+__global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t M, uint32_t K, uint32_t* __restrict__ off) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > K) return;
+    uint32_t lo = 0, hi = M;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (scol[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    off[c] = lo;
+}
+
+// ---------------------------------------------------------------- solver
+struct SB {
+    f3 v, w, x;
+    float im;
+    s3 I;
+    bool dyn;
+};
+
+__device__ __forceinline__ void load_sb(const Bodies& B, uint32_t i, SB& s, bool want_x) {
+    const float4 v = B.sv[i], w = B.sw[i];
+    s.v = xyz(v);
+    s.im = v.w;
+    s.w = xyz(w);
+    s.I = load_s3(B.I0[i], B.I1[i]);
+    s.dyn = v.w > 0.0f;
+    if (want_x) s.x = xyz(B.x[i]);
+}
+__device__ __forceinline__ void zero_sb(SB& s) {
+    s.v = s.w = s.x = f3{0.0f, 0.0f, 0.0f};
+    s.im = 0.0f;
+    s.I = s3{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    s.dyn = false;
+}
+
+__device__ __forceinline__ void apply_imp(SB& A, SB& Bd, f3 rA, f3 rB, f3 P) {
+    if (A.dyn) {
+        A.v = A.v - P * A.im;
+        A.w = A.w - smul(A.I, cross(rA, P));
+    }
+    if (Bd.dyn) {
+        Bd.v = Bd.v + P * Bd.im;
+        Bd.w = Bd.w + smul(Bd.I, cross(rB, P));
+    }
+}
+__device__ __forceinline__ f3 vrel(const SB& A, const SB& Bd, f3 rA, f3 rB) {
+    return (Bd.v + cross(Bd.w, rB)) - (A.v + cross(A.w, rA));
+}
+
+struct PrevCache {
+    const uint64_t* key;  // (a << 32 | b) -> previous slot
+    const uint32_t* slot;
+    uint32_t mask;
+    const float4* lam;    // [k * cap + s]
+    const uint32_t* fid;  // [k * cap + s]
+    const uint32_t* np;   // points per previous slot
+    uint32_t cap;
+    uint32_t valid;
+};
+
+__device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32_t& s) {
+    uint32_t h = (uint32_t)mix64(k) & c.mask;
+    for (;;) {
+        const uint64_t kk = c.key[h];
+        if (kk == k) {
+            s = c.slot[h];
+            return true;
+        }
+        if (kk == EMPTY_KEY) return false;
+        h = (h + 1) & c.mask;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order, uint32_t M,
+                                              uint32_t cap, const uint2* __restrict__ m_ab,
+                                              const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
+                                              const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
+                                              const float4* __restrict__ warm_in, Counters* __restrict__ cnt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const uint32_t m = order[s];
+    const uint2 ab = m_ab[m];
+    const uint32_t np = m_np[m];
+    const bool plane = (ab.y & PLANE_BIT) != 0;
+    SB A, Bd;
+    load_sb(B, ab.x, A, true);
+    if (plane) zero_sb(Bd); else load_sb(B, ab.y, Bd, true);
+    R.hdr[s] = make_uint4(ab.x, ab.y, np, m);
+    uint32_t ps = 0;
+    const bool have_prev = pc.valid && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps);
+    uint32_t hits = 0;
+    for (uint32_t k = 0; k < np; ++k) {
+        const float4 pt = c_pt[(size_t)m * 4 + k], nf = c_nf[(size_t)m * 4 + k];
+        const f3 p = xyz(pt), n = xyz(nf);
+        const float depth = pt.w;
+        const uint32_t fid = fbits(nf.w);
+        const f3 rA = p - A.x;
+        const f3 rB = plane ? f3{0.0f, 0.0f, 0.0f} : p - Bd.x;
+        f3 t1, t2;
+        tangents(n, t1, t2);
+        const f3 dirs[3] = {n, t1, t2};
+        float kk[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const f3 ra = cross(rA, dirs[d]);
+            const f3 rb = cross(rB, dirs[d]);
+            kk[d] = A.im + Bd.im + dot(ra, smul(A.I, ra)) + dot(rb, smul(Bd.I, rb));
+        }
+        const float mN = kk[0] > 0.0f ? 1.0f / kk[0] : 0.0f;
+        const float mT1 = kk[1] > 0.0f ? 1.0f / kk[1] : 0.0f;
+        const float mT2 = kk[2] > 0.0f ? 1.0f / kk[2] : 0.0f;
+        const float vn0 = dot(vrel(A, Bd, rA, rB), n);
+        float bias = 0.0f;
+        if (depth > sp.slop) {
+            bias = sp.baumgarte * (depth - sp.slop) * sp.inv_dt;
+            if (bias > sp.max_bias) bias = sp.max_bias;
+        } else if (depth < 0.0f) {
+            bias = depth * sp.inv_dt;
+        }
+        if (vn0 < -sp.rest_threshold) {
+            const float rb = -sp.restitution * vn0;
+            if (rb > bias) bias = rb;
+        }
+        const size_t o = (size_t)k * cap + s;
+        R.r0[o] = f4(rA, mN);
+        R.r1[o] = f4(rB, bias);
+        R.r2[o] = f4(n, mT1);
+        R.r3[o] = make_float4(mT2, kk[0], kk[1], kk[2]);
+        R.fid[o] = fid;
+        float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (warm_in) {
+            L = warm_in[(size_t)m * 4 + k];
+        } else if (have_prev) {
+            const uint32_t pnp = pc.np[ps];
+            for (uint32_t j = 0; j < pnp; ++j) {
+                const size_t po = (size_t)j * pc.cap + ps;
+                if (pc.fid[po] == fid) {
+                    L = pc.lam[po];
+                    ++hits;
+                    break;
+                }
+            }
+        }
+        L.w = 0.0f;
+        R.lam[o] = L;
+        R.lam0[o] = L;
+    }
+    if (hits) atomicAdd(&cnt->warm_hits, hits);
+}
+
+__device__ __forceinline__ void load_row(const SolverRows& R, size_t o, f3& rA, f3& rB, f3& n, float& mN, float& mT1,
+                                         float& mT2, float& target) {
+    const float4 a = R.r0[o], b = R.r1[o], c = R.r2[o], d = R.r3[o];
+    rA = xyz(a);
+    mN = a.w;
+    rB = xyz(b);
+    target = b.w;
+    n = xyz(c);
+    mT1 = c.w;
+    mT2 = d.x;
+}
+
+__global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows R, uint32_t s0, uint32_t s1,
+                                             uint32_t cap, uint32_t first) {
+    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    const uint4 h = R.hdr[s];
+    const uint32_t a = h.x, b = h.y, np = h.z;
+    const bool plane = (b & PLANE_BIT) != 0;
+    SB A, Bd;
+    load_sb(B, a, A, false);
+    if (plane) zero_sb(Bd); else load_sb(B, b, Bd, false);
+    const float mu = sp.friction;
+    if (first) {
+        for (uint32_t k = 0; k < np; ++k) {
+            const size_t o = (size_t)k * cap + s;
+            f3 rA, rB, n, t1, t2;
+            float mN, mT1, mT2, tg;
+            load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
+            tangents(n, t1, t2);
+            const float4 L = R.lam[o];
+            const f3 Pw = n * L.x + t1 * L.y + t2 * L.z;
+            apply_imp(A, Bd, rA, rB, Pw);
+        }
+    }
+    for (uint32_t k = 0; k < np; ++k) {
+        const size_t o = (size_t)k * cap + s;
+        f3 rA, rB, n, t1, t2;
+        float mN, mT1, mT2, tg;
+        load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
+        tangents(n, t1, t2);
+        float4 L = R.lam[o];
+        const float maxf = mu * L.x;
+        {
+            const float vt = dot(vrel(A, Bd, rA, rB), t1);
+            float nl = L.y - mT1 * vt;
+            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            const float d = nl - L.y;
+            L.y = nl;
+            apply_imp(A, Bd, rA, rB, t1 * d);
+        }
+        {
+            const float vt = dot(vrel(A, Bd, rA, rB), t2);
+            float nl = L.z - mT2 * vt;
+            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            const float d = nl - L.z;
+            L.z = nl;
+            apply_imp(A, Bd, rA, rB, t2 * d);
+        }
+        R.lam[o] = L;
+    }
+    for (uint32_t k = 0; k < np; ++k) {
+        const size_t o = (size_t)k * cap + s;
+        f3 rA, rB, n;
+        float mN, mT1, mT2, tg;
+        load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
+        float4 L = R.lam[o];
+        const float vn = dot(vrel(A, Bd, rA, rB), n);
+        float nl = L.x + mN * (tg - vn);
+        nl = nl > 0.0f ? nl : 0.0f;
+        const float d = nl - L.x;
+        L.x = nl;
+        apply_imp(A, Bd, rA, rB, n * d);
+        R.lam[o] = L;
+    }
+    if (A.dyn) {
+        B.sv[a] = f4(A.v, A.im);
+        B.sw[a] = f4(A.w, 0.0f);
+    }
+    if (Bd.dyn) {
+        B.sv[b] = f4(Bd.v, Bd.im);
+        B.sw[b] = f4(Bd.w, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, SolverRows R, uint32_t M, uint32_t cap,
+                                                  uint32_t* __restrict__ out_bits) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    float res = 0.0f;
+    if (s < M) {
+        const uint4 h = R.hdr[s];
+        const bool plane = (h.y & PLANE_BIT) != 0;
+        SB A, Bd;
+        load_sb(B, h.x, A, false);
+        if (plane) zero_sb(Bd); else load_sb(B, h.y, Bd, false);
+        const float mu = sp.friction;
+        for (uint32_t k = 0; k < h.z; ++k) {
+            const size_t o = (size_t)k * cap + s;
+            f3 rA, rB, n, t1, t2;
+            float mN, mT1, mT2, tg;
+            load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
+            const float4 kk = R.r3[o];
+            tangents(n, t1, t2);
+            const float4 L = R.lam[o];
+            const f3 vr = vrel(A, Bd, rA, rB);
+            float nl = L.x + mN * (tg - dot(vr, n));
+            nl = nl > 0.0f ? nl : 0.0f;
+            float e = fabsf(nl - L.x) * kk.y;
+            const float maxf = mu * L.x;
+            float q1 = L.y - mT1 * dot(vr, t1);
+            q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
+            const float e1 = fabsf(q1 - L.y) * kk.z;
+            float q2 = L.z - mT2 * dot(vr, t2);
+            q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
+            const float e2 = fabsf(q2 - L.z) * kk.w;
+            if (e1 > e) e = e1;
+            if (e2 > e) e = e2;
+            if (e > res) res = e;
+        }
+    }
+    // wave max, then one atomic per wave (non-negative floats order as their bits)
+    uint32_t bits = __float_as_uint(res);
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t ob = __shfl_xor(bits, o, 64);
+        bits = ob > bits ? ob : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, bits);
+}
+
+// Previous-step lookup table for the warm start: (a, b) -> slot.
+__global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, uint32_t M, uint64_t* __restrict__ key,
+                                                     uint32_t* __restrict__ slot, uint32_t mask) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const uint4 h = R.hdr[s];
+    const uint64_t k = ((uint64_t)h.x << 32) | h.y;
+    uint32_t q = (uint32_t)mix64(k) & mask;
+    for (;;) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+        if (prev == EMPTY_KEY || prev == k) {
+            slot[q] = s;
+            return;
+        }
+        q = (q + 1) & mask;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_copy_np(const SolverRows R, uint32_t M, uint32_t* __restrict__ np_out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < M) np_out[s] = R.hdr[s].z;
+}
+
+__global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 v4 = B.sv[i];
+    if (!(v4.w > 0.0f)) return;
+    const float4 w4 = B.sw[i];
+    const float4 x4 = B.x[i];
+    const float4 qv = B.q[i];
+    const f3 v = xyz(v4), w = xyz(w4);
+    const f3 x = xyz(x4) + v * sp.dt;
+    const float hdt = 0.5f * sp.dt;
+    const q4 q = {qv.x, qv.y, qv.z, qv.w};
+    const q4 dq = qmul(q4{w.x, w.y, w.z, 0.0f}, q);
+    const q4 nq = {q.x + dq.x * hdt, q.y + dq.y * hdt, q.z + dq.z * hdt, q.w + dq.w * hdt};
+    const float len = sqrtf(nq.x * nq.x + nq.y * nq.y + nq.z * nq.z + nq.w * nq.w);
+    const float inv = 1.0f / len;
+    B.x[i] = make_float4(x.x, x.y, x.z, x4.w);
+    B.q[i] = make_float4(nq.x * inv, nq.y * inv, nq.z * inv, nq.w * inv);
+    B.vel[i] = v4;
+    B.angv[i] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+// ---------------------------------------------------------------- host launchers
+static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
+
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
+}
+void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u64, grid_for(n), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, grid_for(n), dim3(256), 0, s, p, v, n);
+}
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
+}
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t);
+}
+void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt);
+}
+void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+                      const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
+                      hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb);
+}
+void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t P,
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t epa_cap, Counters* cnt,
+                   hipStream_t s) {
+    if (!P) return;
+    hipLaunchKernelGGL(k_narrow, grid_for(P), dim3(256), 0, s, B, sp, pa, pb, P, p_np, p_pt, p_nf, epa_list, cnt);
+    hipLaunchKernelGGL(k_narrow_epa, grid_for(epa_cap, 64), dim3(64), 0, s, B, sp, pa, pb, epa_list, epa_cap, p_np,
+                       p_pt, p_nf, cnt);
+}
+void launch_flag_nonzero(const uint32_t* np, uint32_t P, uint32_t* flag, hipStream_t s) {
+    if (P) hipLaunchKernelGGL(k_flag_nonzero, grid_for(P), dim3(256), 0, s, np, P, flag);
+}
+void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t P, const uint32_t* p_np, const float4* p_pt,
+                    const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab, uint32_t* m_np, float4* c_pt,
+                    float4* c_nf, Counters* cnt, hipStream_t s) {
+    if (P)
+        hipLaunchKernelGGL(k_compact, grid_for(P), dim3(256), 0, s, pa, pb, P, p_np, p_pt, p_nf, moff, cap, m_ab, m_np,
+                           c_pt, c_nf, cnt);
+}
+void launch_col_init(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, uint64_t* prio, hipStream_t s) {
+    if (M) hipLaunchKernelGGL(k_col_init, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio);
+}
+void launch_col_round(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, const uint64_t* prio,
+                      uint64_t* bmax, uint64_t* bmask, uint32_t round, Counters* cnt, hipStream_t s) {
+    if (!M) return;
+    hipLaunchKernelGGL(k_col_max, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio, bmax, round);
+    hipLaunchKernelGGL(k_col_win, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio, bmax, bmask, round, cnt);
+}
+void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* off, hipStream_t s) {
+    hipLaunchKernelGGL(k_col_offsets, grid_for(K + 1), dim3(256), 0, s, scol, M, K, off);
+}
+void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
+                 const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
+                 const PrevCacheArgs& pca, const float4* warm_in, Counters* cnt, hipStream_t s) {
+    if (!M) return;
+    PrevCache pc;
+    pc.key = pca.key;
+    pc.slot = pca.slot;
+    pc.mask = pca.mask;
+    pc.lam = pca.lam;
+    pc.fid = pca.fid;
+    pc.np = pca.np;
+    pc.cap = pca.cap;
+    pc.valid = pca.valid;
+    hipLaunchKernelGGL(k_prep, grid_for(M), dim3(256), 0, s, B, sp, order, M, cap, m_ab, m_np, c_pt, c_nf, R, pc,
+                       warm_in, cnt);
+}
+void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
+                uint32_t first, hipStream_t s) {
+    if (s1 > s0) hipLaunchKernelGGL(k_pgs, grid_for(s1 - s0), dim3(256), 0, s, B, sp, R, s0, s1, cap, first);
+}
+void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+                     uint32_t* out_bits, hipStream_t s) {
+    if (M) hipLaunchKernelGGL(k_residual, grid_for(M), dim3(256), 0, s, B, sp, R, M, cap, out_bits);
+}
+void launch_cache_build(const SolverRows& R, uint32_t M, uint64_t* key, uint32_t* slot, uint32_t mask,
+                        uint32_t* np_out, hipStream_t s) {
+    if (!M) return;
+    hipLaunchKernelGGL(k_cache_build, grid_for(M), dim3(256), 0, s, R, M, key, slot, mask);
+    hipLaunchKernelGGL(k_copy_np, grid_for(M), dim3(256), 0, s, R, M, np_out);
+}
+void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n);
+}
+
+}  // namespace axm

@@ -1,0 +1,64 @@
+// Launch wrappers of the step kernels (k_step.hip) and the solver-row layout.
+#pragma once
+#include "launch.hpp"
+
+namespace axm {
+
+// Solver rows in solve order, slot-major: element (point k, slot s) at k * cap + s.
+//   r0 = (rA, mN)  r1 = (rB, target)  r2 = (n, mT1)  r3 = (mT2, kN, kT1, kT2)
+//   lam = (lambda_n, lambda_t1, lambda_t2, -), lam0 = warm-start value, fid = feature id
+struct SolverRows {
+    uint4* hdr;  // (a, b, npts, manifold) per slot
+    float4* r0;
+    float4* r1;
+    float4* r2;
+    float4* r3;
+    float4* lam;
+    float4* lam0;
+    uint32_t* fid;
+};
+
+struct PrevCacheArgs {
+    const uint64_t* key;
+    const uint32_t* slot;
+    uint32_t mask;
+    const float4* lam;
+    const uint32_t* fid;
+    const uint32_t* np;
+    uint32_t cap;
+    uint32_t valid;
+};
+
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, hipStream_t s);
+void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, hipStream_t s);
+void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+                      const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
+                      hipStream_t s);
+void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t P,
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t epa_cap, Counters* cnt,
+                   hipStream_t s);
+void launch_flag_nonzero(const uint32_t* np, uint32_t P, uint32_t* flag, hipStream_t s);
+void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t P, const uint32_t* p_np, const float4* p_pt,
+                    const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab, uint32_t* m_np, float4* c_pt,
+                    float4* c_nf, Counters* cnt, hipStream_t s);
+void launch_col_init(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, uint64_t* prio, hipStream_t s);
+void launch_col_round(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, const uint64_t* prio,
+                      uint64_t* bmax, uint64_t* bmask, uint32_t round, Counters* cnt, hipStream_t s);
+void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* off, hipStream_t s);
+void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
+                 const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
+                 const PrevCacheArgs& pc, const float4* warm_in, Counters* cnt, hipStream_t s);
+void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
+                uint32_t first, hipStream_t s);
+void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+                     uint32_t* out_bits, hipStream_t s);
+void launch_cache_build(const SolverRows& R, uint32_t M, uint64_t* key, uint32_t* slot, uint32_t mask,
+                        uint32_t* np_out, hipStream_t s);
+void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, hipStream_t s);
+
+}  // namespace axm

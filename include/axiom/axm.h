@@ -1,0 +1,223 @@
+/*
+ * axm.h — C ABI of the MI355X rigid-body collision & contact pipeline.
+ *
+ * Plain C, plain pointers and sizes: no C++ or torch types cross this
+ * boundary.  It is the drop-in for the stages the reference reserves but never
+ * implements (src/collision and src/dynamics hold only .gitkeep; both modules
+ * are commented out in src/CMakeLists.txt:23-27).  Each entry point below cites
+ * the reference interface or convention it stands in for.
+ *
+ * Conventions
+ *   - Status codes are the numeric values of axiom::core::ErrorCode
+ *     (include/axiom/core/error_code.hpp:21-57); 0 = Success.
+ *   - Y-up, gravity (0,-9.81,0) by default (include/axiom/gui/physics_panel.hpp:35).
+ *   - Quaternions are x,y,z,w (include/axiom/math/quat.hpp:29-34).
+ *   - Shapes: 0 sphere (r), 1 box (hx,hy,hz), 2 capsule (r, half-height along
+ *     local +Y), following gui::ShapeType numbering
+ *     (include/axiom/gui/body_inspector.hpp:20-27); static half-space planes are
+ *     world colliders (debug::ShapeType::Plane, include/axiom/debug/physics_debug_draw.hpp:87-94).
+ *   - Contacts: normal from A to B, depth > 0 penetrating, position = witness
+ *     point on A, witness on B = position - normal*depth
+ *     (src/debug/physics_debug_draw.cpp:111-115).
+ *   - Host pointers are borrowed for the duration of a call.  Calls on one
+ *     handle are not thread-safe; each handle owns its own HIP stream.
+ *     Steps run asynchronously until axm_world_sync or an axm_*_get_* call.
+ */
+#ifndef AXIOM_AXM_H
+#define AXIOM_AXM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t axm_status;
+
+/* axiom::core::ErrorCode values (include/axiom/core/error_code.hpp:21-57). */
+enum {
+    AXM_SUCCESS = 0,
+    AXM_OUT_OF_MEMORY = 200,
+    AXM_NULL_POINTER = 202,
+    AXM_INVALID_SHAPE = 300,
+    AXM_GJK_FAILED_TO_CONVERGE = 301,
+    AXM_EPA_FAILED_TO_CONVERGE = 302,
+    AXM_INVALID_RIGID_BODY = 400,
+    AXM_CONSTRAINT_SOLVER_FAILURE = 401,
+    AXM_BUFFER_ALLOCATION_FAILED = 502,
+    AXM_GPU_INVALID_OPERATION = 503,
+    AXM_GPU_TIMEOUT = 504,
+    AXM_GPU_OPERATION_FAILED = 505,
+    AXM_INVALID_PARAMETER = 600,
+    AXM_OUT_OF_RANGE = 601
+};
+
+enum { AXM_SHAPE_SPHERE = 0, AXM_SHAPE_BOX = 1, AXM_SHAPE_CAPSULE = 2, AXM_SHAPE_PLANE = 3 };
+
+#define AXM_PLANE_BIT 0x80000000u /* pair partner index flag: static plane k = AXM_PLANE_BIT | k */
+#define AXM_MAX_PLANES 8
+
+enum { AXM_WARM_NONE = 0, AXM_WARM_CACHE = 1, AXM_WARM_MLP = 2 };
+
+/* Step flags. */
+enum {
+    AXM_STEP_TIMINGS = 1u << 0,  /* fill the per-stage millisecond fields (adds event syncs) */
+    AXM_STEP_RESIDUAL = 1u << 1, /* compute the residual after the last sweep */
+    AXM_STEP_SYNC = 1u << 2,     /* block until the step is complete */
+    AXM_STEP_KEEP_STAGES = 1u << 3 /* keep the solver's input velocities for axm_world_get_pre_solve_velocities */
+};
+
+/* World description.  Stands in for gui::PhysicsWorldConfig
+ * (include/axiom/gui/physics_panel.hpp:34-43) plus the solver constants the
+ * reference leaves unspecified.  The leading fields up to `record_residual`
+ * have the same layout as the CPU oracle's parameter block. */
+typedef struct axm_world_desc {
+    float gravity[3];
+    float friction;        /* Coulomb mu (pyramid) */
+    float restitution;     /* e, applied when the approach speed exceeds rest_threshold */
+    float rest_threshold;  /* m/s */
+    float baumgarte;       /* position-error feedback factor */
+    float slop;            /* m of allowed penetration */
+    float max_bias;        /* m/s cap on the position-error bias */
+    float margin;          /* m: AABB inflation and speculative contact distance */
+    float lin_damp;        /* per-step multiplicative damping (body_inspector_example.cpp:53-55) */
+    float ang_damp;
+    float cell_size;       /* m: uniform-grid cell; 0 = automatic (max body diameter) */
+    float origin[3];       /* grid origin; cells clamp to [0,1023] per axis */
+    uint32_t num_planes;
+    float planes[AXM_MAX_PLANES][4]; /* n.xyz, d: solid where dot(n,x) < d */
+    uint32_t warm_start;   /* AXM_WARM_* */
+    uint32_t record_residual; /* 1: residual after every sweep (diagnostic) */
+    /* capacities (0 = sized from the body count) */
+    uint32_t max_bodies;
+    uint32_t max_pairs;
+    uint32_t max_manifolds;
+    uint32_t auto_grow;    /* 1: grow pair/manifold buffers on overflow instead of failing */
+    float residual_tol;    /* > 0: stop sweeping once the residual <= tol (checked every residual_every sweeps) */
+    uint32_t residual_every;
+    uint32_t max_iterations; /* cap when residual_tol > 0 */
+    int32_t device;        /* HIP device ordinal */
+} axm_world_desc;
+
+/* Host-side body arrays (structure of arrays).  Field set follows
+ * gui::RigidBodyData (include/axiom/gui/body_inspector.hpp:58-90): position,
+ * rotation, linear / angular velocity, mass (as inverse mass, 0 = static),
+ * diagonal inertia (as its inverse, body frame), shape type and extents. */
+typedef struct axm_body_soa {
+    float* pos;          /* 3n */
+    float* quat;         /* 4n, x y z w */
+    float* lin_vel;      /* 3n */
+    float* ang_vel;      /* 3n */
+    float* inv_mass;     /* n */
+    float* inv_inertia;  /* 3n */
+    uint32_t* shape_type;/* n */
+    float* shape_params; /* 3n */
+    uint32_t* world;     /* n or NULL; batched worlds: world index per body (bodies of one world contiguous) */
+} axm_body_soa;
+
+/* Per-step statistics.  Mirrors gui::PhysicsWorldStats
+ * (include/axiom/gui/physics_panel.hpp:11-31) and adds the solver's own counts. */
+typedef struct axm_step_stats {
+    uint32_t pairs;
+    uint32_t manifolds;
+    uint32_t contacts;      /* contact points (PhysicsWorldStats::contactPointCount) */
+    uint32_t colours;
+    uint32_t colour_rounds;
+    uint32_t iterations;    /* sweeps actually run */
+    float residual;         /* velocity-space fixed-point residual after the last sweep */
+    float residual_first;   /* after the first sweep (when recorded) */
+    uint32_t gjk_fail;
+    uint32_t epa_fail;
+    uint32_t warm_hits;
+    uint32_t overflow;      /* bit 0 pairs, bit 1 manifolds */
+    float broadphase_ms;    /* PhysicsWorldStats::broadphaseTime */
+    float narrowphase_ms;   /* PhysicsWorldStats::narrowphaseTime */
+    float solver_ms;        /* PhysicsWorldStats::solverTime */
+    float integration_ms;   /* PhysicsWorldStats::integrationTime */
+    float total_ms;         /* PhysicsWorldStats::totalStepTime */
+} axm_step_stats;
+
+typedef struct axm_world_s* axm_world;
+
+/* Number of visible HIP devices (0 when none or when the runtime is unavailable). */
+int32_t axm_device_count(void);
+
+/* Mirrors errorCodeToString (src/core/error_code.cpp:5-65). */
+const char* axm_status_string(axm_status s);
+
+/* Defaults for a world description: PhysicsWorldConfig defaults
+ * (include/axiom/gui/physics_panel.hpp:35-42) plus solver constants. */
+void axm_world_desc_default(axm_world_desc* desc);
+
+/* Factory, as VkContext::create (include/axiom/gpu/vk_instance.hpp:34): returns
+ * BufferAllocationFailed / GPU_OPERATION_FAILED on failure. */
+axm_status axm_world_create(const axm_world_desc* desc, axm_world* out);
+axm_status axm_world_destroy(axm_world w);
+
+/* Uploads n bodies (copies in).  Invalid bodies (NaN state, negative mass,
+ * unknown shape) return InvalidRigidBody / InvalidShape and leave the world unchanged. */
+axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* bodies, uint32_t n);
+/* Copies out pos, quat, lin_vel, ang_vel (other fields ignored; NULL fields skipped). */
+axm_status axm_world_get_bodies(axm_world w, axm_body_soa* out);
+axm_status axm_world_body_count(axm_world w, uint32_t* n);
+
+/* One step of PhysicsWorld::step(dt) (examples/profiling_example.cpp:132-165):
+ * broadphase -> narrowphase -> solver (iters sweeps) -> integration. */
+axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags, axm_step_stats* stats);
+/* `steps` steps; with graph = 1 the step is captured once into a hipGraph and
+ * replayed (requires fixed capacities and residual_tol == 0). */
+axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last);
+axm_status axm_world_sync(axm_world w);
+
+/* Stage outputs of the last step (collision::ContactPoint / debug::DebugContactPoint,
+ * include/axiom/debug/physics_debug_draw.hpp:128-132).  Undersized buffers return
+ * OutOfRange with *count set to the required size. */
+axm_status axm_world_get_pairs(axm_world w, uint32_t* a, uint32_t* b, uint32_t cap, uint32_t* count);
+axm_status axm_world_get_manifolds(axm_world w, uint32_t* a, uint32_t* b, uint32_t* npts, float* pos /*4x3 per*/,
+                                   float* normal /*4x3*/, float* depth /*4*/, uint32_t* feature /*4*/, uint32_t cap,
+                                   uint32_t* count);
+/* colour per manifold (pair order), solve order, colour count. */
+axm_status axm_world_get_colours(axm_world w, uint32_t* colour, uint32_t* solve_order, uint32_t cap, uint32_t* count,
+                                 uint32_t* num_colours);
+/* impulses per manifold (pair order) x 4 points x (normal, t1, t2): initial (warm start) and final. */
+axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, uint32_t cap, uint32_t* count);
+/* residual after each sweep of the last step (needs record_residual). */
+axm_status axm_world_get_residuals(axm_world w, float* out, uint32_t cap, uint32_t* count);
+/* velocities entering the solver of the last step (after gravity). */
+axm_status axm_world_get_pre_solve_velocities(axm_world w, float* lin_vel, float* ang_vel);
+
+/* Stand-alone broadphase on the current state (no time advance). */
+axm_status axm_broadphase(axm_world w, uint32_t* count);
+/* Stand-alone broadphase + narrowphase on the current state. */
+axm_status axm_collide(axm_world w, uint32_t* manifold_count);
+
+/* Batched pairwise narrowphase: n shape pairs in, <= 4 contacts per pair out.
+ * type_b == AXM_SHAPE_PLANE: prm_b holds the plane normal and pos_b[0] its offset.
+ * Stand-in for the narrowphase dispatch the reference names (CLAUDE.md:88). */
+axm_status axm_collide_pairs(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a, const float* pos_a,
+                             const float* quat_a, const uint32_t* type_b, const float* prm_b, const float* pos_b,
+                             const float* quat_b, float margin, uint32_t* npts, float* pos, float* normal, float* depth,
+                             uint32_t* feature);
+/* Batched GJK (+ EPA when the cores overlap) on shape cores.  out: 10 floats per
+ * pair (distance or depth, pA[3], pB[3], n[3]); status per pair: 0 separated,
+ * 1 penetrating, 301 / 302 on failure (GJKFailedToConverge / EPAFailedToConverge,
+ * include/axiom/core/error_code.hpp:37-38). */
+axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a, const float* pos_a,
+                             const float* quat_a, const uint32_t* type_b, const float* prm_b, const float* pos_b,
+                             const float* quat_b, float* out, int32_t* status);
+
+/* Synthetic scenes (host only, seeded DeterministicRNG, include/axiom/math/random.hpp:19-68).
+ * config: 0 mixed 1k, 1 dense pile, 2 sparse 1M, 3 batched (n_worlds worlds).
+ * n = body count (0 = the config's default).  Call with out->pos == NULL to query
+ * the body count and the recommended world description. */
+axm_status axm_scene_generate(uint32_t config, uint64_t seed, uint32_t n, uint32_t n_worlds, axm_body_soa* out,
+                              uint32_t* count, axm_world_desc* desc);
+/* Raw DeterministicRNG stream (u32) and nextFloat(min,max) stream for a seed. */
+void axm_rng_stream(uint64_t seed, uint32_t n, uint32_t* u32_out, float lo, float hi, float* f_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXIOM_AXM_H */

@@ -1,0 +1,548 @@
+// CPU oracle — TEST INFRASTRUCTURE ONLY.
+//
+// Sequential restatement of one world step: uniform-grid broadphase, primitive
+// narrowphase, Jones-Plassmann colouring, colour-ordered PGS and semi-implicit
+// Euler.  The reference only sketches this loop
+// (examples/profiling_example.cpp:117-174: Broadphase -> Narrowphase -> Solver
+// -> Integration; semi-implicit order v += g dt then x += v dt in
+// examples/body_inspector_example.cpp:48-60; default config
+// include/axiom/gui/physics_panel.hpp:34-43), so the algorithm below is the
+// project's own specification; the HIP kernels implement the same sequence.
+//
+// Step(dt, I):
+//   1. v += g dt; v *= (1 - linDamp); w *= (1 - angDamp)            (dynamic bodies)
+//   2. AABB = shape bounds + margin; cell = clamp(floor((x - origin)/cell), 0, 1023);
+//      key = world << 30 | morton(cell); stable sort of bodies by key
+//   3. pairs, in sorted-body order t: planes first, then the 27 neighbour cells
+//      (dz, dy, dx each -1..1), then ascending sorted position; pair (i, j) kept
+//      iff j > i, not both static, AABBs overlap (inclusive)
+//   4. narrowphase per pair -> manifold of <= 4 points; manifolds = pairs with
+//      points, in pair order
+//   5. colouring (rounds r = 0,1,..): tag(m) = (r+1)<<48 | hash24(a,b)<<24 | m;
+//      every dynamic body keeps the max tag of its uncoloured manifolds; a
+//      manifold that holds the max at all its dynamic bodies takes the first
+//      colour free at both bodies (64-bit masks; 64+r when full)
+//   6. stable order of manifolds by colour = solve order
+//   7. prep per point, warm start (cache keyed by (a, b, feature id))
+//   8. I PGS sweeps, colour by colour, manifold by manifold: friction rows of
+//      all points, then normal rows (iteration 0 first applies the warm start)
+//   9. x += v dt; q += (dt/2) (w,0) q; q /= |q|
+#pragma once
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "orc_geom.hpp"
+
+namespace orc {
+
+constexpr uint32_t PLANE_BIT = 0x80000000u;
+constexpr int MAX_PLANES = 8;
+constexpr uint32_t CELL_MAX = 1023u;
+
+struct Params {
+    float gravity[3];
+    float friction, restitution, rest_threshold;
+    float baumgarte, slop, max_bias, margin;
+    float lin_damp, ang_damp;
+    float cell_size;
+    float origin[3];
+    uint32_t num_planes;
+    float planes[MAX_PLANES][4];  // n.xyz, d
+    uint32_t warm_start;          // 0 none, 1 cache
+    uint32_t record_residual;     // compute residual after every sweep
+};
+
+struct Stats {
+    uint32_t pairs, manifolds, contacts, colours, rounds, iterations;
+    float residual;      // after the last sweep (velocity units)
+    float residual_first;
+    double kinetic, potential;
+    uint32_t gjk_fail, epa_fail, warm_hits;
+};
+
+template <class T> struct World {
+    Params P;
+    uint32_t n = 0;
+    std::vector<V3<T>> x, v, w, ii, prm;
+    std::vector<Q4<T>> q;
+    std::vector<T> im;
+    std::vector<uint32_t> type, world;
+    // stage outputs of the last step
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> order;
+    std::vector<uint32_t> pair_a, pair_b;
+    std::vector<uint32_t> man_a, man_b;  // pair order
+    std::vector<Man<T>> man;
+    std::vector<uint32_t> colour;      // per manifold (pair order)
+    std::vector<uint32_t> solve_order;  // manifold ids by colour
+    std::vector<uint32_t> colour_off;   // K+1
+    std::vector<V3<T>> v_pre, w_pre;    // velocities entering the solver (after gravity)
+    std::vector<T> residual_hist;
+    // warm-start cache: (a,b) -> list of (fid, lambda)
+    struct CacheEnt {
+        uint32_t np;
+        uint32_t fid[4];
+        T lam[4][3];
+    };
+    std::map<uint64_t, CacheEnt> cache;
+    std::vector<T> lambda;  // per manifold x 4 points x 3 (solve output)
+    Stats st{};
+
+    bool dynamic(uint32_t i) const { return im[i] > T(0); }
+
+    Shape<T> shape_of(uint32_t i) const {
+        Shape<T> s;
+        s.c = x[i];
+        s.R = quat_to_mat(q[i]);
+        s.type = type[i];
+        s.prm = prm[i];
+        return s;
+    }
+
+    void aabb_ext(uint32_t i, const M3<T>& R, V3<T>& e) const {
+        const T mg = T(P.margin);
+        const V3<T> p = prm[i];
+        if (type[i] == SHAPE_SPHERE) {
+            e = {p.x + mg, p.x + mg, p.x + mg};
+        } else if (type[i] == SHAPE_BOX) {
+            e.x = std::fabs(R.r0.x) * p.x + std::fabs(R.r0.y) * p.y + std::fabs(R.r0.z) * p.z + mg;
+            e.y = std::fabs(R.r1.x) * p.x + std::fabs(R.r1.y) * p.y + std::fabs(R.r1.z) * p.z + mg;
+            e.z = std::fabs(R.r2.x) * p.x + std::fabs(R.r2.y) * p.y + std::fabs(R.r2.z) * p.z + mg;
+        } else {
+            e.x = std::fabs(R.r0.y) * p.y + p.x + mg;
+            e.y = std::fabs(R.r1.y) * p.y + p.x + mg;
+            e.z = std::fabs(R.r2.y) * p.y + p.x + mg;
+        }
+    }
+
+    static uint32_t cell_coord(T p, T o, T inv) {
+        T f = (p - o) * inv;
+        f = f > T(0) ? f : T(0);  // NaN -> 0
+        f = f < T(CELL_MAX) ? f : T(CELL_MAX);
+        return (uint32_t)std::floor(f);
+    }
+
+    // ---------------------------------------------------------------- broadphase
+    void broadphase(std::vector<V3<T>>& mn, std::vector<V3<T>>& mx) {
+        mn.resize(n);
+        mx.resize(n);
+        keys.resize(n);
+        const T inv = T(1) / T(P.cell_size);
+        const V3<T> o = {T(P.origin[0]), T(P.origin[1]), T(P.origin[2])};
+        std::vector<uint32_t> cx(n), cy(n), cz(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const M3<T> R = quat_to_mat(q[i]);
+            V3<T> e;
+            aabb_ext(i, R, e);
+            mn[i] = x[i] - e;
+            mx[i] = x[i] + e;
+            cx[i] = cell_coord(x[i].x, o.x, inv);
+            cy[i] = cell_coord(x[i].y, o.y, inv);
+            cz[i] = cell_coord(x[i].z, o.z, inv);
+            keys[i] = ((uint64_t)world[i] << 30) | morton3(cx[i], cy[i], cz[i]);
+        }
+        order.resize(n);
+        for (uint32_t i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+        std::vector<uint64_t> sk(n);
+        for (uint32_t t = 0; t < n; ++t) sk[t] = keys[order[t]];
+        pair_a.clear();
+        pair_b.clear();
+        for (uint32_t t = 0; t < n; ++t) {
+            const uint32_t i = order[t];
+            if (dynamic(i)) {
+                for (uint32_t k = 0; k < P.num_planes; ++k) {
+                    const V3<T> pn = {T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])};
+                    const V3<T> c = x[i];
+                    const V3<T> e = mx[i] - c;
+                    const T s = dot(pn, c) - (std::fabs(pn.x) * e.x + std::fabs(pn.y) * e.y + std::fabs(pn.z) * e.z);
+                    if (s <= T(P.planes[k][3])) {
+                        pair_a.push_back(i);
+                        pair_b.push_back(PLANE_BIT | k);
+                    }
+                }
+            }
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int64_t X = (int64_t)cx[i] + dx, Y = (int64_t)cy[i] + dy, Z = (int64_t)cz[i] + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X > CELL_MAX || Y > CELL_MAX || Z > CELL_MAX) continue;
+                        const uint64_t key = ((uint64_t)world[i] << 30) | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
+                        auto lo = std::lower_bound(sk.begin(), sk.end(), key);
+                        auto hi = std::upper_bound(lo, sk.end(), key);
+                        for (auto it = lo; it != hi; ++it) {
+                            const uint32_t j = order[(size_t)(it - sk.begin())];
+                            if (j <= i) continue;
+                            if (!dynamic(i) && !dynamic(j)) continue;
+                            if (mn[i].x <= mx[j].x && mx[i].x >= mn[j].x && mn[i].y <= mx[j].y && mx[i].y >= mn[j].y &&
+                                mn[i].z <= mx[j].z && mx[i].z >= mn[j].z) {
+                                pair_a.push_back(i);
+                                pair_b.push_back(j);
+                            }
+                        }
+                    }
+        }
+    }
+
+    // ---------------------------------------------------------------- narrowphase
+    void narrowphase() {
+        man_a.clear();
+        man_b.clear();
+        man.clear();
+        const T mg = T(P.margin);
+        for (size_t p = 0; p < pair_a.size(); ++p) {
+            const uint32_t a = pair_a[p], b = pair_b[p];
+            Man<T> m;
+            const Shape<T> A = shape_of(a);
+            if (b & PLANE_BIT) {
+                const uint32_t k = b & ~PLANE_BIT;
+                collide_plane(m, A, V3<T>{T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])}, T(P.planes[k][3]),
+                              mg);
+            } else {
+                const int s = collide(m, A, shape_of(b), mg);
+                if (s == GEOM_GJK_FAIL) ++st.gjk_fail;
+                if (s == GEOM_EPA_FAIL) ++st.epa_fail;
+            }
+            if (m.np > 0) {
+                man_a.push_back(a);
+                man_b.push_back(b);
+                man.push_back(m);
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- colouring
+    bool dyn_partner(uint32_t b) const { return !(b & PLANE_BIT) && dynamic(b); }
+
+    uint32_t local_id(uint32_t b, const std::vector<uint32_t>& world_first) const {
+        if (b & PLANE_BIT) return b;
+        return b - world_first[world[b]];
+    }
+
+    void colour_manifolds() {
+        const uint32_t M = (uint32_t)man.size();
+        colour.assign(M, 0xFFFFFFFFu);
+        // first body index of every world (bodies of a world are contiguous)
+        uint32_t nw = 0;
+        for (uint32_t i = 0; i < n; ++i) nw = std::max(nw, world[i] + 1);
+        std::vector<uint32_t> wf(nw + 1, 0xFFFFFFFFu);
+        for (uint32_t i = n; i-- > 0;) wf[world[i]] = i;
+        std::vector<uint64_t> prio(M);
+        for (uint32_t m = 0; m < M; ++m)
+            prio[m] = ((uint64_t)pair_hash24(local_id(man_a[m], wf), local_id(man_b[m], wf)) << 24) | (m & 0xFFFFFFu);
+        std::vector<uint64_t> bmax(n, 0);
+        std::vector<uint64_t> mask(n, 0);
+        uint32_t left = M, round = 0, K = 0;
+        while (left > 0) {
+            const uint64_t tagbase = (uint64_t)(round + 1) << 48;
+            for (uint32_t m = 0; m < M; ++m) {
+                if (colour[m] != 0xFFFFFFFFu) continue;
+                const uint64_t t = tagbase | prio[m];
+                bmax[man_a[m]] = std::max(bmax[man_a[m]], t);
+                if (dyn_partner(man_b[m])) bmax[man_b[m]] = std::max(bmax[man_b[m]], t);
+            }
+            std::vector<uint32_t> win;
+            for (uint32_t m = 0; m < M; ++m) {
+                if (colour[m] != 0xFFFFFFFFu) continue;
+                const uint64_t t = tagbase | prio[m];
+                const bool bd = dyn_partner(man_b[m]);
+                if (bmax[man_a[m]] == t && (!bd || bmax[man_b[m]] == t)) win.push_back(m);
+            }
+            for (uint32_t m : win) {
+                const bool bd = dyn_partner(man_b[m]);
+                const uint64_t used = mask[man_a[m]] | (bd ? mask[man_b[m]] : 0);
+                uint32_t c;
+                if (~used == 0) {
+                    c = 64 + round;
+                } else {
+                    c = 0;
+                    while (used & (1ull << c)) ++c;
+                    mask[man_a[m]] |= 1ull << c;
+                    if (bd) mask[man_b[m]] |= 1ull << c;
+                }
+                colour[m] = c;
+                K = std::max(K, c + 1);
+                --left;
+            }
+            ++round;
+        }
+        st.rounds = round;
+        st.colours = K;
+        colour_off.assign(K + 1, 0);
+        for (uint32_t m = 0; m < M; ++m) colour_off[colour[m] + 1]++;
+        for (uint32_t c = 0; c < K; ++c) colour_off[c + 1] += colour_off[c];
+        solve_order.resize(M);
+        std::vector<uint32_t> fill(colour_off.begin(), colour_off.end() - 1);
+        for (uint32_t m = 0; m < M; ++m) solve_order[fill[colour[m]]++] = m;
+    }
+
+    // ---------------------------------------------------------------- solver
+    struct Row {
+        V3<T> rA, rB, n, t1, t2;
+        T mN, mT1, mT2, kN, kT1, kT2, target;
+    };
+    struct SBody {
+        V3<T> v, w;
+        T im;
+        S3<T> I;
+        V3<T> x;
+        bool dyn;
+    };
+
+    static void apply(SBody& A, SBody& B, const Row& r, V3<T> P) {
+        if (A.dyn) {
+            A.v = A.v - P * A.im;
+            A.w = A.w - smul(A.I, cross(r.rA, P));
+        }
+        if (B.dyn) {
+            B.v = B.v + P * B.im;
+            B.w = B.w + smul(B.I, cross(r.rB, P));
+        }
+    }
+    static V3<T> vrel(const SBody& A, const SBody& B, const Row& r) {
+        return (B.v + cross(B.w, r.rB)) - (A.v + cross(A.w, r.rA));
+    }
+
+    void solve(T dt, uint32_t iters, const T* warm_in /* optional: M x 4 x 3 in manifold order */) {
+        const uint32_t M = (uint32_t)man.size();
+        std::vector<SBody> sb(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            SBody& b = sb[i];
+            b.v = v[i];
+            b.w = w[i];
+            b.im = im[i];
+            b.dyn = dynamic(i);
+            b.x = x[i];
+            const M3<T> R = quat_to_mat(q[i]);
+            b.I = world_inv_inertia(R, ii[i]);
+        }
+        SBody ground{};
+        ground.v = ground.w = ground.x = {T(0), T(0), T(0)};
+        ground.im = T(0);
+        ground.I = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        ground.dyn = false;
+        std::vector<Row> rows((size_t)M * 4);
+        lambda.assign((size_t)M * 12, T(0));
+        const T inv_dt = T(1) / dt;
+        const T mu = T(P.friction);
+        st.warm_hits = 0;
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint32_t a = man_a[m], b = man_b[m];
+            const SBody& A = sb[a];
+            const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+            const uint64_t ck = ((uint64_t)a << 32) | b;
+            const CacheEnt* ce = nullptr;
+            if (P.warm_start == 1) {
+                auto it = cache.find(ck);
+                if (it != cache.end()) ce = &it->second;
+            }
+            for (uint32_t k = 0; k < man[m].np; ++k) {
+                const CPt<T>& c = man[m].pt[k];
+                Row& r = rows[(size_t)m * 4 + k];
+                r.n = c.n;
+                r.rA = c.p - A.x;
+                r.rB = (b & PLANE_BIT) ? V3<T>{T(0), T(0), T(0)} : c.p - B.x;
+                tangents(c.n, r.t1, r.t2);
+                const V3<T> dirs[3] = {r.n, r.t1, r.t2};
+                T kk[3];
+                for (int d = 0; d < 3; ++d) {
+                    const V3<T> ra = cross(r.rA, dirs[d]);
+                    const V3<T> rb = cross(r.rB, dirs[d]);
+                    kk[d] = A.im + B.im + dot(ra, smul(A.I, ra)) + dot(rb, smul(B.I, rb));
+                }
+                r.kN = kk[0];
+                r.kT1 = kk[1];
+                r.kT2 = kk[2];
+                r.mN = kk[0] > T(0) ? T(1) / kk[0] : T(0);
+                r.mT1 = kk[1] > T(0) ? T(1) / kk[1] : T(0);
+                r.mT2 = kk[2] > T(0) ? T(1) / kk[2] : T(0);
+                const T vn0 = dot(vrel(A, B, r), r.n);
+                T bias = T(0);
+                if (c.depth > T(P.slop)) {
+                    bias = T(P.baumgarte) * (c.depth - T(P.slop)) * inv_dt;
+                    if (bias > T(P.max_bias)) bias = T(P.max_bias);
+                } else if (c.depth < T(0)) {
+                    bias = c.depth * inv_dt;
+                }
+                if (vn0 < -T(P.rest_threshold)) {
+                    const T rb = -T(P.restitution) * vn0;
+                    if (rb > bias) bias = rb;
+                }
+                r.target = bias;
+                T* L = &lambda[((size_t)m * 4 + k) * 3];
+                if (warm_in) {
+                    for (int d = 0; d < 3; ++d) L[d] = warm_in[((size_t)m * 4 + k) * 3 + d];
+                } else if (ce) {
+                    for (uint32_t j = 0; j < ce->np; ++j)
+                        if (ce->fid[j] == c.fid) {
+                            L[0] = ce->lam[j][0];
+                            L[1] = ce->lam[j][1];
+                            L[2] = ce->lam[j][2];
+                            ++st.warm_hits;
+                            break;
+                        }
+                }
+            }
+        }
+        residual_hist.clear();
+        for (uint32_t it = 0; it < iters; ++it) {
+            for (uint32_t s = 0; s < M; ++s) {
+                const uint32_t m = solve_order[s];
+                const uint32_t a = man_a[m], b = man_b[m];
+                SBody& A = sb[a];
+                SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+                const uint32_t np = man[m].np;
+                if (it == 0) {
+                    for (uint32_t k = 0; k < np; ++k) {
+                        const Row& r = rows[(size_t)m * 4 + k];
+                        const T* L = &lambda[((size_t)m * 4 + k) * 3];
+                        const V3<T> Pw = r.n * L[0] + r.t1 * L[1] + r.t2 * L[2];
+                        apply(A, B, r, Pw);
+                    }
+                }
+                for (uint32_t k = 0; k < np; ++k) {
+                    const Row& r = rows[(size_t)m * 4 + k];
+                    T* L = &lambda[((size_t)m * 4 + k) * 3];
+                    const T maxf = mu * L[0];
+                    {
+                        const T vt = dot(vrel(A, B, r), r.t1);
+                        T nl = L[1] - r.mT1 * vt;
+                        nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+                        const T d = nl - L[1];
+                        L[1] = nl;
+                        apply(A, B, r, r.t1 * d);
+                    }
+                    {
+                        const T vt = dot(vrel(A, B, r), r.t2);
+                        T nl = L[2] - r.mT2 * vt;
+                        nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+                        const T d = nl - L[2];
+                        L[2] = nl;
+                        apply(A, B, r, r.t2 * d);
+                    }
+                }
+                for (uint32_t k = 0; k < np; ++k) {
+                    const Row& r = rows[(size_t)m * 4 + k];
+                    T* L = &lambda[((size_t)m * 4 + k) * 3];
+                    const T vn = dot(vrel(A, B, r), r.n);
+                    T nl = L[0] + r.mN * (r.target - vn);
+                    nl = nl > T(0) ? nl : T(0);
+                    const T d = nl - L[0];
+                    L[0] = nl;
+                    apply(A, B, r, r.n * d);
+                }
+            }
+            if (P.record_residual || it + 1 == iters) {
+                T res = T(0);
+                for (uint32_t m = 0; m < M; ++m) {
+                    const uint32_t a = man_a[m], b = man_b[m];
+                    const SBody& A = sb[a];
+                    const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+                    for (uint32_t k = 0; k < man[m].np; ++k) {
+                        const Row& r = rows[(size_t)m * 4 + k];
+                        const T* L = &lambda[((size_t)m * 4 + k) * 3];
+                        const V3<T> vr = vrel(A, B, r);
+                        T nl = L[0] + r.mN * (r.target - dot(vr, r.n));
+                        nl = nl > T(0) ? nl : T(0);
+                        T e = std::fabs(nl - L[0]) * r.kN;
+                        const T maxf = mu * L[0];
+                        T t1 = L[1] - r.mT1 * dot(vr, r.t1);
+                        t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
+                        const T e1 = std::fabs(t1 - L[1]) * r.kT1;
+                        T t2 = L[2] - r.mT2 * dot(vr, r.t2);
+                        t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
+                        const T e2 = std::fabs(t2 - L[2]) * r.kT2;
+                        if (e1 > e) e = e1;
+                        if (e2 > e) e = e2;
+                        if (e > res) res = e;
+                    }
+                }
+                residual_hist.push_back(res);
+            }
+        }
+        st.iterations = iters;
+        st.residual = residual_hist.empty() ? 0.0f : (float)residual_hist.back();
+        st.residual_first = residual_hist.empty() ? 0.0f : (float)residual_hist.front();
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!sb[i].dyn) continue;
+            v[i] = sb[i].v;
+            w[i] = sb[i].w;
+        }
+        // refresh the warm-start cache
+        cache.clear();
+        for (uint32_t m = 0; m < M; ++m) {
+            CacheEnt e;
+            e.np = man[m].np;
+            for (uint32_t k = 0; k < e.np; ++k) {
+                e.fid[k] = man[m].pt[k].fid;
+                for (int d = 0; d < 3; ++d) e.lam[k][d] = lambda[((size_t)m * 4 + k) * 3 + d];
+            }
+            cache[((uint64_t)man_a[m] << 32) | man_b[m]] = e;
+        }
+    }
+
+    void integrate_velocities(T dt) {
+        const V3<T> g = {T(P.gravity[0]), T(P.gravity[1]), T(P.gravity[2])};
+        const T kl = T(1) - T(P.lin_damp), ka = T(1) - T(P.ang_damp);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!dynamic(i)) continue;
+            v[i] = (v[i] + g * dt) * kl;
+            w[i] = w[i] * ka;
+        }
+    }
+
+    void integrate_positions(T dt) {
+        const T hdt = T(0.5) * dt;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!dynamic(i)) continue;
+            x[i] = x[i] + v[i] * dt;
+            const Q4<T> wq = {w[i].x, w[i].y, w[i].z, T(0)};
+            const Q4<T> dq = qmul(wq, q[i]);
+            Q4<T> nq = {q[i].x + dq.x * hdt, q[i].y + dq.y * hdt, q[i].z + dq.z * hdt, q[i].w + dq.w * hdt};
+            const T len = std::sqrt(nq.x * nq.x + nq.y * nq.y + nq.z * nq.z + nq.w * nq.w);
+            const T inv = T(1) / len;
+            q[i] = {nq.x * inv, nq.y * inv, nq.z * inv, nq.w * inv};
+        }
+    }
+
+    void energy(double& ke, double& pe) const {
+        ke = pe = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!dynamic(i)) continue;
+            const double m = 1.0 / (double)im[i];
+            const double vx = v[i].x, vy = v[i].y, vz = v[i].z;
+            ke += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+            const M3<T> R = quat_to_mat(q[i]);
+            const V3<T> wl = mulT(R, w[i]);
+            const double I[3] = {ii[i].x > 0 ? 1.0 / (double)ii[i].x : 0.0, ii[i].y > 0 ? 1.0 / (double)ii[i].y : 0.0,
+                                 ii[i].z > 0 ? 1.0 / (double)ii[i].z : 0.0};
+            ke += 0.5 * (I[0] * (double)wl.x * (double)wl.x + I[1] * (double)wl.y * (double)wl.y +
+                         I[2] * (double)wl.z * (double)wl.z);
+            pe -= m * ((double)P.gravity[0] * (double)x[i].x + (double)P.gravity[1] * (double)x[i].y +
+                       (double)P.gravity[2] * (double)x[i].z);
+        }
+    }
+
+    void step(T dt, uint32_t iters) {
+        st = Stats{};
+        std::vector<V3<T>> mn, mx;
+        broadphase(mn, mx);
+        narrowphase();
+        integrate_velocities(dt);
+        v_pre = v;
+        w_pre = w;
+        colour_manifolds();
+        solve(dt, iters, nullptr);
+        integrate_positions(dt);
+        st.pairs = (uint32_t)pair_a.size();
+        st.manifolds = (uint32_t)man.size();
+        uint32_t c = 0;
+        for (auto& m : man) c += m.np;
+        st.contacts = c;
+        energy(st.kinetic, st.potential);
+    }
+};
+
+}  // namespace orc
