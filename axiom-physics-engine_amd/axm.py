@@ -46,7 +46,7 @@ class WorldDesc(C.Structure):
         ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("max_bodies", C.c_uint32),
         ("max_pairs", C.c_uint32), ("max_manifolds", C.c_uint32), ("auto_grow", C.c_uint32),
         ("residual_tol", C.c_float), ("residual_every", C.c_uint32), ("max_iterations", C.c_uint32),
-        ("device", C.c_int32),
+        ("device", C.c_int32), ("solver_mode", C.c_uint32),
     ]
 
 
@@ -73,10 +73,18 @@ class StepStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("pgs_launches", C.c_uint64), ("pgs_ms", C.c_double),
+                ("pgs_bytes", C.c_double), ("sweeps", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # Exported C-ABI symbols (the header's declarations).
 EXPORTS = [
     "axm_device_count", "axm_status_string", "axm_world_desc_default", "axm_world_create", "axm_world_destroy", "axm_world_set_bodies",
-    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync",
+    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_rng_stream",
@@ -108,6 +116,7 @@ def lib():
     L.axm_world_step.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
     L.axm_world_run.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(StepStats)]
     L.axm_world_sync.argtypes = [P]
+    L.axm_world_profile.argtypes = [P, C.c_int, C.POINTER(KernelProfile)]
     L.axm_world_get_pairs.argtypes = [P, u32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_manifolds.argtypes = [P, u32p, u32p, u32p, f32p, f32p, f32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_colours.argtypes = [P, u32p, u32p, C.c_uint32, u32p, u32p]
@@ -265,6 +274,12 @@ class World:
 
     def sync(self):
         _check(self._L.axm_world_sync(self._h), "axm_world_sync")
+
+    def profile(self, enable):
+        """Enables/disables sweep-kernel event timing; returns the profile accumulated so far."""
+        p = KernelProfile()
+        _check(self._L.axm_world_profile(self._h, 1 if enable else 0, C.byref(p)), "axm_world_profile")
+        return p
 
     def broadphase(self):
         c = C.c_uint32()

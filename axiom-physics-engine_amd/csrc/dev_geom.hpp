@@ -326,7 +326,17 @@ AXM_D bool simplex_closest(SV* s, int& n, float* lam, f3& v) {
     return true;
 }
 
-AXM_D GjkOut gjk(const Shp& A, const Shp& B, int max_iter = 32) {
+// GJK / EPA work in a frame centred on A and translate witnesses back.
+AXM_D void centre_on_a(const Shp& A, const Shp& B, Shp& A0, Shp& B0) {
+    A0 = A;
+    B0 = B;
+    A0.c = {0.0f, 0.0f, 0.0f};
+    B0.c = B.c - A.c;
+}
+
+AXM_D GjkOut gjk(const Shp& Aw, const Shp& Bw, int max_iter = 32) {
+    Shp A, B;
+    centre_on_a(Aw, Bw, A, B);
     GjkOut o;
     o.status = -1;
     f3 d = B.c - A.c;
@@ -339,21 +349,33 @@ AXM_D GjkOut gjk(const Shp& A, const Shp& B, int max_iter = 32) {
         const float vv = dot(v, v);
         if (vv <= 1e-12f) { o.status = 1; break; }
         const SV w = mink_support(A, B, -v);
-        if (vv - dot(v, w.w) <= 1e-6f * vv) { o.status = 0; break; }
+        if (vv - dot(v, w.w) <= 1e-5f * vv + 1e-9f) { o.status = 0; break; }
         bool dup = false;
         for (int i = 0; i < o.n; ++i)
             if (o.s[i].w.x == w.w.x && o.s[i].w.y == w.w.y && o.s[i].w.z == w.w.z) dup = true;
         if (dup) { o.status = 0; break; }
+        SV ps[4];
+        float pl[4];
+        const int pn = o.n;
+        for (int i = 0; i < pn; ++i) { ps[i] = o.s[i]; pl[i] = lam[i]; }
+        const f3 pv = v;
         o.s[o.n++] = w;
         if (!simplex_closest(o.s, o.n, lam, v)) { o.status = 1; break; }
+        if (!(dot(v, v) < vv)) {
+            o.n = pn;
+            for (int i = 0; i < pn; ++i) { o.s[i] = ps[i]; lam[i] = pl[i]; }
+            v = pv;
+            o.status = 0;
+            break;
+        }
     }
     f3 pa = {0.0f, 0.0f, 0.0f}, pb = {0.0f, 0.0f, 0.0f};
     for (int i = 0; i < o.n; ++i) {
         pa = pa + o.s[i].a * lam[i];
         pb = pb + o.s[i].b * lam[i];
     }
-    o.pA = pa;
-    o.pB = pb;
+    o.pA = pa + Aw.c;
+    o.pB = pb + Aw.c;
     o.dist = o.status == 1 ? 0.0f : sqrtf(dot(v, v));
     return o;
 }
@@ -387,13 +409,15 @@ AXM_D bool epa_make_face(EpaWork& W, int f) {
     return true;
 }
 
-AXM_D EpaOut epa(const Shp& A, const Shp& B, const GjkOut& g, EpaWork& W, int max_iter = 48) {
+AXM_D EpaOut epa(const Shp& Aw, const Shp& Bw, const GjkOut& g, EpaWork& W, int max_iter = 48) {
+    Shp A, B;
+    centre_on_a(Aw, Bw, A, B);
     int nv = 0, nf = 0;
     EpaOut o;
     o.status = -1;
     o.depth = 0.0f;
     o.n = {0.0f, 1.0f, 0.0f};
-    o.pA = o.pB = A.c;
+    o.pA = o.pB = Aw.c;
     for (int i = 0; i < g.n; ++i) W.vt[nv++] = g.s[i];
     if (nv == 1) return o;
     if (nv == 2) {
@@ -500,8 +524,8 @@ AXM_D EpaOut epa(const Shp& A, const Shp& B, const GjkOut& g, EpaWork& W, int ma
         lw = (d00 * d21 - d01 * d20) / den;
     }
     const float lu = 1.0f - lv - lw;
-    o.pA = W.vt[W.fv[best][0]].a * lu + W.vt[W.fv[best][1]].a * lv + W.vt[W.fv[best][2]].a * lw;
-    o.pB = W.vt[W.fv[best][0]].b * lu + W.vt[W.fv[best][1]].b * lv + W.vt[W.fv[best][2]].b * lw;
+    o.pA = W.vt[W.fv[best][0]].a * lu + W.vt[W.fv[best][1]].a * lv + W.vt[W.fv[best][2]].a * lw + Aw.c;
+    o.pB = W.vt[W.fv[best][0]].b * lu + W.vt[W.fv[best][1]].b * lv + W.vt[W.fv[best][2]].b * lw + Aw.c;
     o.n = W.fn[best];
     o.depth = W.fd[best];
     o.status = converged ? 0 : -1;

@@ -433,7 +433,8 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                                               uint32_t cap, const uint2* __restrict__ m_ab,
                                               const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
-                                              const float4* __restrict__ warm_in, Counters* __restrict__ cnt) {
+                                              const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
+                                              const uint64_t* __restrict__ bmask, Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     const uint32_t m = order[s];
@@ -444,6 +445,23 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
     load_sb(B, ab.x, A, true);
     if (plane) zero_sb(Bd); else load_sb(B, ab.y, Bd, true);
     R.hdr[s] = make_uint4(ab.x, ab.y, np, m);
+    {
+        // dependency words for the persistent sweep: deg << 8 | rank per dynamic body
+        const uint32_t c = col[m];
+        uint2 dp = make_uint2(UNSET, UNSET);
+        if (c < 64u) {
+            const unsigned long long below = (1ull << c) - 1ull;
+            if (A.dyn) {
+                const unsigned long long mk = bmask[ab.x];
+                dp.x = ((uint32_t)__popcll(mk) << 8) | (uint32_t)__popcll(mk & below);
+            }
+            if (Bd.dyn) {
+                const unsigned long long mk = bmask[ab.y];
+                dp.y = ((uint32_t)__popcll(mk) << 8) | (uint32_t)__popcll(mk & below);
+            }
+        }
+        R.dep[s] = dp;
+    }
     uint32_t ps = 0;
     const bool have_prev = pc.valid && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps);
     uint32_t hits = 0;
@@ -504,6 +522,12 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         R.lam0[o] = L;
     }
     if (hits) atomicAdd(&cnt->warm_hits, hits);
+    // algorithmic bytes one k_pgs visit of this manifold moves: header 16,
+    // per point rows r0..r2 + lambda read/write (80), body A read 64 (+32 written
+    // when dynamic), body B likewise unless it is a plane
+    unsigned long long bytes = 16ull + 80ull * np + 64ull + (A.dyn ? 32ull : 0ull);
+    if (!plane) bytes += 64ull + (Bd.dyn ? 32ull : 0ull);
+    atomicAdd(&cnt->pgs_bytes, bytes);
 }
 
 __device__ __forceinline__ void load_row(const SolverRows& R, size_t o, f3& rA, f3& rB, f3& n, float& mN, float& mT1,
@@ -518,16 +542,56 @@ __device__ __forceinline__ void load_row(const SolverRows& R, size_t o, f3& rA, 
     mT2 = d.x;
 }
 
-__global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows R, uint32_t s0, uint32_t s1,
-                                             uint32_t cap, uint32_t first) {
-    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= s1) return;
-    const uint4 h = R.hdr[s];
+// Memory policy of the sweep: plain accesses when every colour is its own
+// launch; agent-coherent accesses (sc1, coherent across XCD L2s) in the
+// persistent kernel, where bodies and impulses written by one workgroup are read
+// by another inside the same launch.
+template <bool COH> struct Mem {
+    static AXM_D float4 ld(const float4* p) {
+        if constexpr (COH) {
+            const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+            const unsigned long long lo = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long hi = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return make_float4(__uint_as_float((uint32_t)lo), __uint_as_float((uint32_t)(lo >> 32)),
+                               __uint_as_float((uint32_t)hi), __uint_as_float((uint32_t)(hi >> 32)));
+        } else {
+            return *p;
+        }
+    }
+    static AXM_D void st(float4* p, float4 v) {
+        if constexpr (COH) {
+            unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+            const unsigned long long lo = (unsigned long long)__float_as_uint(v.x) |
+                                          ((unsigned long long)__float_as_uint(v.y) << 32);
+            const unsigned long long hi = (unsigned long long)__float_as_uint(v.z) |
+                                          ((unsigned long long)__float_as_uint(v.w) << 32);
+            __hip_atomic_store(q, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(q + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            *p = v;
+        }
+    }
+};
+
+template <bool COH> __device__ __forceinline__ void load_sb_m(const Bodies& B, uint32_t i, SB& s) {
+    const float4 v = Mem<COH>::ld(B.sv + i), w = Mem<COH>::ld(B.sw + i);
+    s.v = xyz(v);
+    s.im = v.w;
+    s.w = xyz(w);
+    s.I = load_s3(B.I0[i], B.I1[i]);
+    s.dyn = v.w > 0.0f;
+}
+
+// One visit of the manifold in solve slot s: warm start (first sweep only),
+// friction rows of every point, then normal rows; writes the bodies back.
+template <bool COH>
+__device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp, const SolverRows& R, uint4 h,
+                                           uint32_t s, uint32_t cap, bool first) {
     const uint32_t a = h.x, b = h.y, np = h.z;
     const bool plane = (b & PLANE_BIT) != 0;
     SB A, Bd;
-    load_sb(B, a, A, false);
-    if (plane) zero_sb(Bd); else load_sb(B, b, Bd, false);
+    load_sb_m<COH>(B, a, A);
+    if (plane) zero_sb(Bd); else load_sb_m<COH>(B, b, Bd);
     const float mu = sp.friction;
     if (first) {
         for (uint32_t k = 0; k < np; ++k) {
@@ -536,18 +600,19 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
             float mN, mT1, mT2, tg;
             load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
             tangents(n, t1, t2);
-            const float4 L = R.lam[o];
+            const float4 L = Mem<COH>::ld(R.lam + o);
             const f3 Pw = n * L.x + t1 * L.y + t2 * L.z;
             apply_imp(A, Bd, rA, rB, Pw);
         }
     }
+    float4 Ls[4];
     for (uint32_t k = 0; k < np; ++k) {
         const size_t o = (size_t)k * cap + s;
         f3 rA, rB, n, t1, t2;
         float mN, mT1, mT2, tg;
         load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
         tangents(n, t1, t2);
-        float4 L = R.lam[o];
+        float4 L = Mem<COH>::ld(R.lam + o);
         const float maxf = mu * L.x;
         {
             const float vt = dot(vrel(A, Bd, rA, rB), t1);
@@ -565,29 +630,98 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
             L.z = nl;
             apply_imp(A, Bd, rA, rB, t2 * d);
         }
-        R.lam[o] = L;
+        Ls[k & 3] = L;
     }
     for (uint32_t k = 0; k < np; ++k) {
         const size_t o = (size_t)k * cap + s;
         f3 rA, rB, n;
         float mN, mT1, mT2, tg;
         load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
-        float4 L = R.lam[o];
+        float4 L = Ls[k & 3];
         const float vn = dot(vrel(A, Bd, rA, rB), n);
         float nl = L.x + mN * (tg - vn);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L.x;
         L.x = nl;
         apply_imp(A, Bd, rA, rB, n * d);
-        R.lam[o] = L;
+        Mem<COH>::st(R.lam + o, L);
     }
     if (A.dyn) {
-        B.sv[a] = f4(A.v, A.im);
-        B.sw[a] = f4(A.w, 0.0f);
+        Mem<COH>::st(B.sv + a, f4(A.v, A.im));
+        Mem<COH>::st(B.sw + a, f4(A.w, 0.0f));
     }
     if (Bd.dyn) {
-        B.sv[b] = f4(Bd.v, Bd.im);
-        B.sw[b] = f4(Bd.w, 0.0f);
+        Mem<COH>::st(B.sv + b, f4(Bd.v, Bd.im));
+        Mem<COH>::st(B.sw + b, f4(Bd.w, 0.0f));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows R, uint32_t s0, uint32_t s1,
+                                             uint32_t cap, uint32_t first) {
+    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    solve_slot<false>(B, sp, R, R.hdr[s], s, cap, first != 0);
+}
+
+// Persistent sweep kernel.  Work item t = sweep * M + slot, handed out in
+// increasing order in chunks of 64 (one wave).  An item may run once every
+// earlier visit of its dynamic bodies is complete: body x keeps a counter of
+// finished visits, and the visit of slot s in sweep k is number
+// k * deg(x) + rank(x, s) of body x, with rank = colours of x's manifolds below
+// s's colour (the per-body colour masks of the colouring).  Each body therefore
+// sees exactly the update sequence of the colour-by-colour sweep, without a
+// grid-wide barrier.  Chunks are taken in order by running waves, so the
+// smallest pending item always has its dependencies complete (no deadlock);
+// a bounded spin sets `err` instead of hanging.
+__global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, uint32_t M, uint32_t cap,
+                                                  uint32_t sweep0, uint32_t nsweeps, uint32_t* __restrict__ bdone,
+                                                  uint32_t* __restrict__ queue, uint32_t* __restrict__ err,
+                                                  uint32_t spin_limit) {
+    const uint32_t lane = __lane_id();
+    const uint64_t total = (uint64_t)nsweeps * M;
+    const uint64_t nchunks = (total + 63) / 64;
+    for (;;) {
+        uint32_t chunk = 0;
+        if (lane == 0) chunk = atomicAdd(queue, 1u);
+        chunk = __shfl(chunk, 0, 64);
+        if ((uint64_t)chunk >= nchunks) break;
+        const uint64_t t = (uint64_t)chunk * 64 + lane;
+        bool pending = t < total;
+        uint32_t sw = 0, s = 0;
+        uint4 h = make_uint4(0, 0, 0, 0);
+        uint2 dp = make_uint2(UNSET, UNSET);
+        if (pending) {
+            sw = (uint32_t)(t / M);
+            s = (uint32_t)(t - (uint64_t)sw * M);
+            h = R.hdr[s];
+            dp = R.dep[s];
+        }
+        const uint32_t tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
+        const uint32_t tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
+        uint32_t spins = 0;
+        while (__ballot(pending)) {
+            if (pending) {
+                bool ready = true;
+                if (dp.x != UNSET)
+                    ready = __hip_atomic_load(bdone + h.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tA;
+                if (ready && dp.y != UNSET)
+                    ready = __hip_atomic_load(bdone + h.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tB;
+                if (ready) {
+                    solve_slot<true>(B, sp, R, h, s, cap, sweep0 + sw == 0);
+                    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): body / impulse stores complete first
+                    if (dp.x != UNSET) __hip_atomic_fetch_add(bdone + h.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (dp.y != UNSET) __hip_atomic_fetch_add(bdone + h.y, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    pending = false;
+                } else if (++spins > spin_limit) {
+                    atomicOr(err, 1u);
+                    pending = false;
+                } else if ((spins & 1023u) == 0u &&
+                           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    pending = false;  // another wave already gave up: drain
+                }
+            }
+            if (__ballot(pending)) __builtin_amdgcn_s_sleep(1);
+        }
     }
 }
 
@@ -740,7 +874,8 @@ void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* 
 }
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
                  const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
-                 const PrevCacheArgs& pca, const float4* warm_in, Counters* cnt, hipStream_t s) {
+                 const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
+                 Counters* cnt, hipStream_t s) {
     if (!M) return;
     PrevCache pc;
     pc.key = pca.key;
@@ -752,7 +887,17 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
     pc.cap = pca.cap;
     pc.valid = pca.valid;
     hipLaunchKernelGGL(k_prep, grid_for(M), dim3(256), 0, s, B, sp, order, M, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, cnt);
+                       warm_in, col, bmask, cnt);
+}
+void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, uint32_t* queue, uint32_t* err,
+                     uint32_t spin_limit, uint32_t max_groups, hipStream_t s) {
+    if (!M || !nsweeps) return;
+    const uint64_t chunks = ((uint64_t)nsweeps * M + 63) / 64;
+    uint64_t groups = (chunks + 3) / 4;
+    if (groups > max_groups) groups = max_groups;
+    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, M, cap, sweep0, nsweeps, bdone,
+                       queue, err, spin_limit);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {

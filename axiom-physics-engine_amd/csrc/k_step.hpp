@@ -16,6 +16,7 @@ struct SolverRows {
     float4* lam;
     float4* lam0;
     uint32_t* fid;
+    uint2* dep;  // per slot: (deg << 8 | rank) of body a and body b in the colour order, UNSET if not dynamic
 };
 
 struct PrevCacheArgs {
@@ -52,7 +53,11 @@ void launch_col_round(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* 
 void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* off, hipStream_t s);
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
                  const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
-                 const PrevCacheArgs& pc, const float4* warm_in, Counters* cnt, hipStream_t s);
+                 const PrevCacheArgs& pc, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
+                 Counters* cnt, hipStream_t s);
+void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, uint32_t* queue, uint32_t* err,
+                     uint32_t spin_limit, uint32_t max_groups, hipStream_t s);
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,

@@ -70,7 +70,9 @@ struct Counters {
     uint32_t last_round;   // colouring: last round that coloured something + 1
     uint32_t epa_count;    // deep box-capsule pairs queued for EPA
     uint32_t residual_bits;
-    uint32_t pad[5];
+    uint32_t pad[1];
+    unsigned long long pgs_bytes;  // algorithmic bytes of one PGS sweep (all colours)
+    unsigned long long pad2;
 };
 
 // k_sort.hip

@@ -148,7 +148,7 @@ void gen_sparse(uint64_t seed, uint32_t n, std::vector<Body>& out) {
     uint32_t side = (uint32_t)std::ceil(std::cbrt((double)n));
     if (side < 1) side = 1;
     const float s = 400.0f / (float)side;
-    const float J = 0.3f * s;
+    const float J = 0.45f * s;
     for (uint32_t i = 0; i < n; ++i) {
         Body b{};
         b.world = 0;

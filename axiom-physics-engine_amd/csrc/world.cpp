@@ -100,6 +100,8 @@ struct axm_world_s {
 
     uint32_t* col = nullptr;
     uint64_t *prio = nullptr, *bmax = nullptr, *bmask = nullptr;
+    uint32_t* bdone = nullptr;      // persistent sweep: finished visits per body
+    uint32_t* flow_ctl = nullptr;   // persistent sweep: [0] work queue, [1] spin-limit error
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
     uint32_t* col_off = nullptr;
     uint32_t col_off_cap = 0;
@@ -121,6 +123,11 @@ struct axm_world_s {
     float4* warm_dev = nullptr;
     uint32_t warm_cap = 0;
 
+    // kernel profiling
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;
+    axm_kernel_profile prof{};
+
     // last step
     uint32_t P = 0, M = 0, K = 0, rounds = 0, iters_run = 0;
     bool have_step = false;
@@ -129,6 +136,7 @@ struct axm_world_s {
 
     ~axm_world_s() {
         if (stream) hipStreamSynchronize(stream);
+        for (hipEvent_t e : prof_ev) hipEventDestroy(e);
         mem.release_all();
         if (h_cnt) hipHostFree(h_cnt);
         if (stream) hipStreamDestroy(stream);
@@ -169,6 +177,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
     AXM_TRY(w->mem.get(w->bmax, cap));
     AXM_TRY(w->mem.get(w->bmask, cap));
+    AXM_TRY(w->mem.get(w->bdone, cap));
     w->cap_n = cap;
     return AXM_SUCCESS;
 }
@@ -192,6 +201,7 @@ void free_bodies(axm_world_s* w) {
     w->mem.release(w->scan_tmp);
     w->mem.release(w->bmax);
     w->mem.release(w->bmask);
+    w->mem.release(w->bdone);
     w->cap_n = 0;
 }
 
@@ -232,6 +242,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.lam);
     w->mem.release(R.lam0);
     w->mem.release(R.fid);
+    w->mem.release(R.dep);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -243,6 +254,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.lam0, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(R.dep, cap));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -488,7 +500,8 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
     pca.np = w->np_slot[pv];
     pca.cap = w->rcap[pv];
     if (!pca.key) pca.valid = 0;
-    launch_prep(w->B, sp, w->order, M, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, warm_in, w->cnt, s);
+    launch_prep(w->B, sp, w->order, M, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, warm_in, w->col, w->bmask,
+                w->cnt, s);
     const bool tol_mode = w->desc.residual_tol > 0.0f;
     const uint32_t max_it = tol_mode ? std::max<uint32_t>(w->desc.max_iterations, 1u) : iters;
     if (w->res_cap < max_it + 1) {
@@ -500,9 +513,52 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
     const bool want_res = (flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
     const uint32_t every = std::max<uint32_t>(w->desc.residual_every, 1u);
     uint32_t it = 0;
-    for (; it < max_it; ++it) {
-        for (uint32_t col = 0; col < K; ++col)
-            launch_pgs(w->B, sp, R, w->h_col_off[col], w->h_col_off[col + 1], rc, it == 0 ? 1u : 0u, s);
+    size_t nev = 0;
+    const bool flow = w->desc.solver_mode == 0 && K <= 64 && M > 0;
+    auto prof_mark = [&]() -> axm_status {
+        if (!w->prof_on) return AXM_SUCCESS;
+        while (w->prof_ev.size() < nev + 1) {
+            hipEvent_t e;
+            AXM_HIP(hipEventCreate(&e));
+            w->prof_ev.push_back(e);
+        }
+        AXM_HIP(hipEventRecord(w->prof_ev[nev++], s));
+        return AXM_SUCCESS;
+    };
+    if (flow) {
+        AXM_HIP(hipMemsetAsync(w->flow_ctl, 0, 16, s));
+        while (it < max_it) {
+            uint32_t seg = max_it - it;
+            if (w->desc.record_residual) seg = 1;
+            else if (tol_mode) seg = std::min(seg, every);
+            AXM_HIP(hipMemsetAsync(w->bdone, 0, sizeof(uint32_t) * n, s));
+            AXM_HIP(hipMemsetAsync(w->flow_ctl, 0, 4, s));
+            AXM_TRY(prof_mark());
+            launch_pgs_flow(w->B, sp, R, M, rc, it, seg, w->bdone, w->flow_ctl, w->flow_ctl + 1, 1u << 20, 4096u, s);
+            AXM_TRY(prof_mark());
+            it += seg;
+            const bool last = it == max_it;
+            const bool check = tol_mode && (it % every == 0);
+            if (w->desc.record_residual || check || (last && want_res)) {
+                launch_residual(w->B, sp, R, M, rc, w->res_bits + it - 1, s);
+                if (check) {
+                    AXM_HIP(hipMemcpyAsync(&w->h_cnt->residual_bits, w->res_bits + it - 1, 4, hipMemcpyDeviceToHost, s));
+                    AXM_HIP(hipStreamSynchronize(s));
+                    float res;
+                    std::memcpy(&res, &w->h_cnt->residual_bits, 4);
+                    if (res <= w->desc.residual_tol) break;
+                }
+            }
+        }
+    }
+    for (; !flow && it < max_it; ++it) {
+        for (uint32_t col = 0; col < K; ++col) {
+            const uint32_t s0 = w->h_col_off[col], s1 = w->h_col_off[col + 1];
+            if (s1 <= s0) continue;
+            AXM_TRY(prof_mark());
+            launch_pgs(w->B, sp, R, s0, s1, rc, it == 0 ? 1u : 0u, s);
+            AXM_TRY(prof_mark());
+        }
         const bool last = it + 1 == max_it;
         const bool check = tol_mode && ((it + 1) % every == 0);
         if (w->desc.record_residual || check || (last && want_res)) {
@@ -520,6 +576,21 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
         }
     }
     const uint32_t iters_run = it;
+    if (w->prof_on) {
+        AXM_HIP(hipStreamSynchronize(s));
+        double ms = 0.0;
+        for (size_t k = 0; k < nev; k += 2) {
+            float e = 0.0f;
+            AXM_HIP(hipEventElapsedTime(&e, w->prof_ev[k], w->prof_ev[k + 1]));
+            ms += e;
+        }
+        AXM_TRY(read_counters(w));
+        w->prof.steps += 1;
+        w->prof.pgs_launches += nev / 2;
+        w->prof.pgs_ms += ms;
+        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)iters_run;
+        w->prof.sweeps += iters_run;
+    }
     tm.mark();
     // warm-start table for the next step
     launch_fill_u64(w->pc_key[c], EMPTY_KEY, w->pc_mask[c] + 1, s);
@@ -538,6 +609,12 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
     if (w->rcap[w->cur] < w->cap_man) AXM_TRY(alloc_rows(w, w->cur, w->cap_man));
     const bool need_sync = st || (flags & AXM_STEP_SYNC) || want_res || tm.on;
     w->h_res.clear();
+    if (flow) {
+        uint32_t ctl[2] = {0, 0};
+        AXM_HIP(hipMemcpyAsync(ctl, w->flow_ctl, 8, hipMemcpyDeviceToHost, s));
+        AXM_HIP(hipStreamSynchronize(s));
+        if (ctl[1]) return AXM_GPU_TIMEOUT;
+    }
     if (need_sync) {
         AXM_TRY(read_counters(w));
         if (want_res && iters_run) {
@@ -604,6 +681,7 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
         return AXM_BUFFER_ALLOCATION_FAILED;
     }
     axm_status st = w->mem.get(w->cnt, 1);
+    if (st == AXM_SUCCESS) st = w->mem.get(w->flow_ctl, 4);
     if (st == AXM_SUCCESS) st = alloc_bodies(w, std::max<uint32_t>(desc->max_bodies, 16u));
     if (st == AXM_SUCCESS) st = alloc_pairs(w, std::max<uint32_t>(desc->max_pairs, 1024u));
     if (st == AXM_SUCCESS) st = alloc_manifolds(w, std::max<uint32_t>(desc->max_manifolds, 1024u));
@@ -805,6 +883,14 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
     (void)graph;
     for (uint32_t k = 0; k < steps; ++k)
         AXM_TRY(step_impl(w, dt, iters, k + 1 == steps ? AXM_STEP_SYNC : 0u, k + 1 == steps ? last : nullptr, nullptr));
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out) {
+    if (!w) return AXM_NULL_POINTER;
+    if (out) *out = w->prof;
+    w->prof = axm_kernel_profile{};
+    w->prof_on = enable != 0;
     return AXM_SUCCESS;
 }
 

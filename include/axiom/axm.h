@@ -98,6 +98,8 @@ typedef struct axm_world_desc {
     uint32_t residual_every;
     uint32_t max_iterations; /* cap when residual_tol > 0 */
     int32_t device;        /* HIP device ordinal */
+    uint32_t solver_mode;  /* 0: one persistent sweep kernel ordered by per-body counters (when <= 64 colours);
+                              1: one launch per colour per sweep */
 } axm_world_desc;
 
 /* Host-side body arrays (structure of arrays).  Field set follows
@@ -138,6 +140,16 @@ typedef struct axm_step_stats {
     float total_ms;         /* PhysicsWorldStats::totalStepTime */
 } axm_step_stats;
 
+/* Kernel-level timing of the solver sweep kernel, accumulated over steps run
+ * with profiling enabled (HIP events around every launch on the world's stream). */
+typedef struct axm_kernel_profile {
+    uint64_t steps;
+    uint64_t pgs_launches;
+    double pgs_ms;        /* summed kernel durations */
+    double pgs_bytes;     /* summed algorithmic bytes of those launches */
+    uint64_t sweeps;
+} axm_kernel_profile;
+
 typedef struct axm_world_s* axm_world;
 
 /* Number of visible HIP devices (0 when none or when the runtime is unavailable). */
@@ -169,6 +181,9 @@ axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags,
  * replayed (requires fixed capacities and residual_tol == 0). */
 axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last);
 axm_status axm_world_sync(axm_world w);
+/* Enables / disables per-launch event timing of the sweep kernel; `out`
+ * (optional) receives the accumulated profile, which is then reset. */
+axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out);
 
 /* Stage outputs of the last step (collision::ContactPoint / debug::DebugContactPoint,
  * include/axiom/debug/physics_debug_draw.hpp:128-132).  Undersized buffers return

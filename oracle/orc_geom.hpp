@@ -330,7 +330,18 @@ template <class T> inline bool simplex_closest(SV<T>* s, int& n, T* lam, V3<T>& 
     return true;
 }
 
-template <class T> inline GjkOut<T> gjk(const Shape<T>& A, const Shape<T>& B, int max_iter = 32) {
+// GJK and EPA run in a frame centred on A (coordinates of the size of the
+// shapes, not of the scene) and translate their witness points back.
+template <class T> inline void centre_on_a(const Shape<T>& A, const Shape<T>& B, Shape<T>& A0, Shape<T>& B0) {
+    A0 = A;
+    B0 = B;
+    A0.c = {T(0), T(0), T(0)};
+    B0.c = B.c - A.c;
+}
+
+template <class T> inline GjkOut<T> gjk(const Shape<T>& Aw, const Shape<T>& Bw, int max_iter = 32) {
+    Shape<T> A, B;
+    centre_on_a(Aw, Bw, A, B);
     GjkOut<T> o;
     o.status = -1;
     V3<T> d = B.c - A.c;
@@ -344,13 +355,26 @@ template <class T> inline GjkOut<T> gjk(const Shape<T>& A, const Shape<T>& B, in
         const T vv = dot(v, v);
         if (vv <= T(1e-12)) { o.status = 1; break; }
         const SV<T> w = mink_support(A, B, -v);
-        if (vv - dot(v, w.w) <= T(1e-6) * vv) { o.status = 0; break; }
+        if (vv - dot(v, w.w) <= T(1e-5) * vv + T(1e-9)) { o.status = 0; break; }
         bool dup = false;
         for (int i = 0; i < o.n; ++i)
             if (o.s[i].w.x == w.w.x && o.s[i].w.y == w.w.y && o.s[i].w.z == w.w.z) dup = true;
         if (dup) { o.status = 0; break; }
+        SV<T> ps[4];
+        T pl[4];
+        const int pn = o.n;
+        for (int i = 0; i < pn; ++i) { ps[i] = o.s[i]; pl[i] = lam[i]; }
+        const V3<T> pv = v;
         o.s[o.n++] = w;
         if (!simplex_closest(o.s, o.n, lam, v)) { o.status = 1; break; }
+        if (!(dot(v, v) < vv)) {
+            // no progress: keep the previous simplex (numerical limit reached)
+            o.n = pn;
+            for (int i = 0; i < pn; ++i) { o.s[i] = ps[i]; lam[i] = pl[i]; }
+            v = pv;
+            o.status = 0;
+            break;
+        }
     }
     o.iters = it;
     V3<T> pa = {T(0), T(0), T(0)}, pb = {T(0), T(0), T(0)};
@@ -358,8 +382,8 @@ template <class T> inline GjkOut<T> gjk(const Shape<T>& A, const Shape<T>& B, in
         pa = pa + o.s[i].a * lam[i];
         pb = pb + o.s[i].b * lam[i];
     }
-    o.pA = pa;
-    o.pB = pb;
+    o.pA = pa + Aw.c;
+    o.pB = pb + Aw.c;
     o.dist = o.status == 1 ? T(0) : std::sqrt(dot(v, v));
     return o;
 }
@@ -373,7 +397,9 @@ template <class T> struct EpaOut {
     int iters;
 };
 
-template <class T> inline EpaOut<T> epa(const Shape<T>& A, const Shape<T>& B, const GjkOut<T>& g, int max_iter = 48) {
+template <class T> inline EpaOut<T> epa(const Shape<T>& Aw, const Shape<T>& Bw, const GjkOut<T>& g, int max_iter = 48) {
+    Shape<T> A, B;
+    centre_on_a(Aw, Bw, A, B);
     constexpr int MAXV = 64, MAXF = 128;
     SV<T> vt[MAXV];
     int fv[MAXF][3];
@@ -384,7 +410,7 @@ template <class T> inline EpaOut<T> epa(const Shape<T>& A, const Shape<T>& B, co
     o.status = -1;
     o.depth = T(0);
     o.n = {T(0), T(1), T(0)};
-    o.pA = o.pB = A.c;
+    o.pA = o.pB = Aw.c;
     o.iters = 0;
     for (int i = 0; i < g.n; ++i) vt[nv++] = g.s[i];
     if (nv == 1) return o;
@@ -509,8 +535,8 @@ template <class T> inline EpaOut<T> epa(const Shape<T>& A, const Shape<T>& B, co
         lw = (d00 * d21 - d01 * d20) / den;
     }
     const T lu = T(1) - lv - lw;
-    o.pA = vt[fv[best][0]].a * lu + vt[fv[best][1]].a * lv + vt[fv[best][2]].a * lw;
-    o.pB = vt[fv[best][0]].b * lu + vt[fv[best][1]].b * lv + vt[fv[best][2]].b * lw;
+    o.pA = vt[fv[best][0]].a * lu + vt[fv[best][1]].a * lv + vt[fv[best][2]].a * lw + Aw.c;
+    o.pB = vt[fv[best][0]].b * lu + vt[fv[best][1]].b * lv + vt[fv[best][2]].b * lw + Aw.c;
     o.n = fn[best];
     o.depth = fd[best];
     o.status = converged ? 0 : -1;

@@ -113,3 +113,19 @@ def test_batched_worlds_match_solo(axm, orc, gpu):
             ws.step(DT, 10)
         g1 = ws.get_bodies()
         assert np.array_equal(g1.pos, gb.pos[sel]) and np.array_equal(g1.lin_vel, gb.lin_vel[sel]), k
+
+
+def test_colour_launch_mode_matches_persistent(axm, orc, gpu):
+    """The per-colour launch path and the persistent sweep give identical results."""
+    b, d = axm.scene(1, seed(1), 5000)
+    out = []
+    for mode in (0, 1):
+        d.solver_mode = mode
+        w = axm.World(d)
+        w.set_bodies(b)
+        for _ in range(10):
+            w.step(DT, 20)
+        g = w.get_bodies()
+        out.append((g.pos.copy(), g.lin_vel.copy(), g.ang_vel.copy()))
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
