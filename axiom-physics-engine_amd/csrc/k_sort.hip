@@ -69,7 +69,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_reduce(const uint32_t* __re
 }
 
 // Single workgroup: exclusive scan of nb tile sums in place; sums[nb] = total.
-__global__ void __launch_bounds__(SC_THREADS) k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb) {
+__global__ void __launch_bounds__(SC_THREADS) k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb,
+                                                           uint32_t* __restrict__ total_out) {
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += SC_THREADS) {
         const uint32_t i = base + threadIdx.x;
@@ -79,7 +80,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_sums(uint32_t* __restrict__
         if (i < nb) sums[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) sums[nb] = carry;
+    if (threadIdx.x == 0) {
+        sums[nb] = carry;
+        if (total_out) *total_out = carry;
+    }
 }
 
 __global__ void __launch_bounds__(SC_THREADS) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
@@ -107,23 +111,36 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_down(const uint32_t* __rest
 
 size_t scan_tmp_words(uint32_t n) { return (size_t)((n + SC_TILE - 1) / SC_TILE) + 1; }
 
-void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s) {
+__global__ void k_zero1(uint32_t* p, uint32_t* q) {
+    *p = 0;
+    if (q) *q = 0;
+}
+
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out) {
     if (n == 0) {
-        hipMemsetAsync(out, 0, sizeof(uint32_t), s);
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, out, total_out);
         return;
     }
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, tmp, nb, total_out);
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb);
 }
 
 // ---------------------------------------------------------------- radix sort
 template <class K> __device__ __forceinline__ uint32_t digit_of(K k, int shift) { return (uint32_t)(k >> shift) & 0xFFu; }
 
+__device__ __forceinline__ uint32_t eff_count(uint32_t cap, const uint32_t* n_dev) {
+    if (!n_dev) return cap;
+    const uint32_t v = *n_dev;
+    return v < cap ? v : cap;
+}
+
 template <class K>
-__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ keys, uint32_t n, int shift,
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ keys, uint32_t cap,
+                                                        const uint32_t* __restrict__ n_dev, int shift,
                                                         uint32_t* __restrict__ hist, uint32_t nb) {
+    const uint32_t n = eff_count(cap, n_dev);
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -139,8 +156,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ ke
 
 template <class K>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                           K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
-                                                           int shift, const uint32_t* __restrict__ offs, uint32_t nb) {
+                                                           K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t cap,
+                                                           const uint32_t* __restrict__ n_dev, int shift,
+                                                           const uint32_t* __restrict__ offs, uint32_t nb) {
+    const uint32_t n = eff_count(cap, n_dev);
+    if (blockIdx.x * RS_TILE >= n) return;
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_wc[RS_THREADS / 64][256];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
@@ -195,7 +215,7 @@ size_t radix_tmp_words(uint32_t n) {
 
 template <class K>
 static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                           uint32_t* tmp, hipStream_t s) {
+                           uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
     if (n == 0) return 0;
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* hist = tmp;
@@ -207,9 +227,10 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     uint32_t* vo = vals_alt;
     int passes = 0;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, shift, hist, nb);
-        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s);
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, shift, offs, nb);
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, n_dev, shift, hist, nb);
+        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s, nullptr);
+        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, n_dev, shift, offs,
+                           nb);
         K* tk = ki; ki = ko; ko = tk;
         uint32_t* tv = vi; vi = vo; vo = tv;
         ++passes;
@@ -218,13 +239,13 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s) {
-    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
+    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev);
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s) {
-    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
+    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev);
 }
 
 }  // namespace axm

@@ -83,21 +83,25 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
     if (i < n) p[i] = v;
 }
 
-__device__ __forceinline__ void ht_insert(CellTable t, uint64_t k, uint2 v) {
+// Linear-probing tables.  Every probe loop visits at most mask + 1 slots, so a
+// full table ends the loop (insert returns false, lookup misses) instead of
+// spinning; tables are sized >= 2x their maximum entry count.
+__device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v) {
     uint32_t s = (uint32_t)mix64(k) & t.mask;
-    for (;;) {
+    for (uint32_t probe = 0; probe <= t.mask; ++probe) {
         const unsigned long long prev =
             atomicCAS((unsigned long long*)&t.key[s], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
         if (prev == EMPTY_KEY || prev == k) {
             t.val[s] = v;
-            return;
+            return true;
         }
         s = (s + 1) & t.mask;
     }
+    return false;
 }
 __device__ __forceinline__ bool ht_find(const CellTable& t, uint64_t k, uint2& v) {
     uint32_t s = (uint32_t)mix64(k) & t.mask;
-    for (;;) {
+    for (uint32_t probe = 0; probe <= t.mask; ++probe) {
         const uint64_t kk = t.key[s];
         if (kk == k) {
             v = t.val[s];
@@ -106,16 +110,18 @@ __device__ __forceinline__ bool ht_find(const CellTable& t, uint64_t k, uint2& v
         if (kk == EMPTY_KEY) return false;
         s = (s + 1) & t.mask;
     }
+    return false;
 }
 
-__global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t) {
+__global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+                                                    Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint64_t k = skey[s];
     if (s > 0 && skey[s - 1] == k) return;
     uint32_t c = 1;
     while (s + c < n && skey[s + c] == k) ++c;
-    ht_insert(t, k, make_uint2(s, c));
+    if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
 }
 
 // ---------------------------------------------------------------- pairs
@@ -182,8 +188,14 @@ __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, con
 __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                    const uint32_t* __restrict__ off, uint32_t cap,
-                                                   uint32_t* __restrict__ pa, uint32_t* __restrict__ pb) {
+                                                   uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
+                                                   Counters* __restrict__ cnt) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        const uint32_t P = off[n];
+        cnt->p_eff = P < cap ? P : cap;
+        if (P > cap) atomicOr(&cnt->sticky, STICKY_PAIRS);
+    }
     if (t >= n) return;
     uint32_t o = off[t];
     const uint32_t i = sval[t];
@@ -217,12 +229,11 @@ __device__ __forceinline__ void store_pair(uint32_t p, const Man& m, uint32_t* _
 }
 
 __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
-                                                const uint32_t* __restrict__ pb, uint32_t P,
-                                                uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
-                                                float4* __restrict__ p_nf, uint32_t* __restrict__ epa_list,
-                                                Counters* __restrict__ cnt) {
+                                                const uint32_t* __restrict__ pb, uint32_t* __restrict__ p_np,
+                                                float4* __restrict__ p_pt, float4* __restrict__ p_nf,
+                                                uint32_t* __restrict__ epa_list, Counters* __restrict__ cnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    if (p >= cnt->p_eff) return;
     const uint32_t a = pa[p], b = pb[p];
     Man m;
     m.np = 0;
@@ -269,21 +280,26 @@ __global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, cons
     store_pair(p, m, p_np, p_pt, p_nf);
 }
 
-__global__ void __launch_bounds__(256) k_flag_nonzero(const uint32_t* __restrict__ np, uint32_t P,
-                                                      uint32_t* __restrict__ flag) {
+__global__ void __launch_bounds__(256) k_flag_nonzero(const uint32_t* __restrict__ np, uint32_t cap_pairs,
+                                                      uint32_t* __restrict__ flag, const Counters* __restrict__ cnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < P) flag[p] = np[p] > 0 ? 1u : 0u;
+    if (p < cap_pairs) flag[p] = (p < cnt->p_eff && np[p] > 0) ? 1u : 0u;
 }
 
 __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
-                                                 uint32_t P, const uint32_t* __restrict__ p_np,
+                                                 const uint32_t* __restrict__ p_np,
                                                  const float4* __restrict__ p_pt, const float4* __restrict__ p_nf,
                                                  const uint32_t* __restrict__ moff, uint32_t cap,
                                                  uint2* __restrict__ m_ab, uint32_t* __restrict__ m_np,
                                                  float4* __restrict__ c_pt, float4* __restrict__ c_nf,
                                                  Counters* __restrict__ cnt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    if (p == 0) {
+        const uint32_t M = cnt->manifolds;
+        cnt->m_eff = M < cap ? M : cap;
+        if (M > cap) atomicOr(&cnt->sticky, STICKY_MANIFOLDS);
+    }
+    if (p >= cnt->p_eff) return;
     const uint32_t np = p_np[p];
     if (np == 0) return;
     const uint32_t m = moff[p];
@@ -305,49 +321,130 @@ __device__ __forceinline__ uint32_t local_id(const Bodies& B, uint32_t b) {
     return (b & PLANE_BIT) ? b : b - B.world_first[B.world[b]];
 }
 
-__global__ void __launch_bounds__(256) k_col_init(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
-                                                  uint32_t* __restrict__ col, uint64_t* __restrict__ prio) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
-    const uint2 ab = m_ab[m];
-    col[m] = UNSET;
-    prio[m] = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
-}
-
-__global__ void __launch_bounds__(256) k_col_max(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
-                                                 const uint32_t* __restrict__ col, const uint64_t* __restrict__ prio,
-                                                 uint64_t* __restrict__ bmax, uint32_t round) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M || col[m] != UNSET) return;
-    const uint2 ab = m_ab[m];
-    const uint64_t t = ((uint64_t)(round + 1) << 48) | prio[m];
-    atomicMax((unsigned long long*)&bmax[ab.x], (unsigned long long)t);
-    if (dyn_body(B, ab.y)) atomicMax((unsigned long long*)&bmax[ab.y], (unsigned long long)t);
-}
-
-__global__ void __launch_bounds__(256) k_col_win(Bodies B, const uint2* __restrict__ m_ab, uint32_t M,
-                                                 uint32_t* __restrict__ col, const uint64_t* __restrict__ prio,
-                                                 const uint64_t* __restrict__ bmax, uint64_t* __restrict__ bmask,
-                                                 uint32_t round, Counters* __restrict__ cnt) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M || col[m] != UNSET) return;
-    const uint2 ab = m_ab[m];
-    const uint64_t t = ((uint64_t)(round + 1) << 48) | prio[m];
-    const bool bd = dyn_body(B, ab.y);
-    if (bmax[ab.x] != t || (bd && bmax[ab.y] != t)) return;
-    const uint64_t used = bmask[ab.x] | (bd ? bmask[ab.y] : 0ull);
-    uint32_t c;
-    if (~used == 0ull) {
-        c = 64u + round;
-    } else {
-        c = (uint32_t)__ffsll((unsigned long long)~used) - 1u;
-        bmask[ab.x] = bmask[ab.x] | (1ull << c);
-        if (bd) bmask[ab.y] = bmask[ab.y] | (1ull << c);
+// Grid-wide barrier for a kernel whose workgroups are all resident (grid sized
+// from the occupancy query).  Every wave fences at agent scope before arriving
+// (its own stores complete and leave the XCD's L2: __syncthreads does not wait
+// for other waves' stores) and again after leaving (stale L1/L2 lines are
+// invalidated).  A bounded wait sets STICKY_TIMEOUT instead of hanging.
+__device__ __forceinline__ void grid_sync(GridBar* bar, uint32_t nblocks, Counters* cnt) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = atomicAdd(&bar->count, 1u);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            atomicAdd(&bar->gen, 1u);
+        } else {
+            const uint64_t t0 = wall_clock64();  // 100 MHz constant clock
+            while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (wall_clock64() - t0 > 200000000ull) {  // 2 s
+                    atomicOr(&cnt->sticky, STICKY_TIMEOUT);
+                    break;
+                }
+            }
+        }
     }
-    col[m] = c;
-    atomicSub(&cnt->remaining, 1u);
-    atomicMax(&cnt->max_colour, c + 1u);
-    atomicMax(&cnt->last_round, round + 1u);
+    __syncthreads();
+    __threadfence();
+}
+
+// Jones-Plassmann colouring in one persistent launch.  Round r: every
+// uncoloured manifold offers tag(r, m) = (r+1) << 48 | hash24(a,b) << 24 | m at
+// its dynamic bodies (atomic max into the round's buffer); a manifold holding
+// the max at all its dynamic bodies takes the first colour free in both bodies'
+// 64-bit masks (64 + r once they are full).  Winners of one round share no
+// dynamic body, so their mask updates cannot collide.  Losers offer their next
+// tag into the other buffer in the same pass, so each round costs one barrier;
+// tags grow with the round, so the buffers never need clearing.
+__global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restrict__ m_ab, uint32_t n,
+                                                uint32_t* __restrict__ col, uint64_t* __restrict__ prio,
+                                                uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmax1,
+                                                uint64_t* __restrict__ bmask, uint32_t* __restrict__ bdeg,
+                                                GridBar* bar, Counters* cnt, uint32_t max_rounds) {
+    const uint32_t M = cnt->m_eff;
+    if (M == 0) return;  // uniform across the grid: no barrier is entered
+    const uint32_t nb = gridDim.x;
+    const uint32_t gs = nb * blockDim.x;
+    const uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = g0; i < n; i += gs) {
+        bmax0[i] = 0ull;
+        bmax1[i] = 0ull;
+        bmask[i] = 0ull;
+        bdeg[i] = 0u;
+    }
+    for (uint32_t m = g0; m < M; m += gs) {
+        const uint2 ab = m_ab[m];
+        col[m] = UNSET;
+        prio[m] = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
+    }
+    if (g0 == 0) cnt->remaining = M;
+    grid_sync(bar, nb, cnt);
+    for (uint32_t m = g0; m < M; m += gs) {
+        const uint2 ab = m_ab[m];
+        const uint64_t t = (1ull << 48) | prio[m];
+        atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
+        atomicAdd(&bdeg[ab.x], 1u);
+        if (dyn_body(B, ab.y)) {
+            atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
+            atomicAdd(&bdeg[ab.y], 1u);
+        }
+    }
+    grid_sync(bar, nb, cnt);
+    for (uint32_t r = 0;; ++r) {
+        const uint64_t* cur = (r & 1u) ? bmax1 : bmax0;
+        uint64_t* nxt = (r & 1u) ? bmax0 : bmax1;
+        uint32_t won = 0, cmax = 0;
+        for (uint32_t m = g0; m < M; m += gs) {
+            if (col[m] != UNSET) continue;
+            const uint2 ab = m_ab[m];
+            const bool bd = dyn_body(B, ab.y);
+            const uint64_t t = ((uint64_t)(r + 1) << 48) | prio[m];
+            if (cur[ab.x] == t && (!bd || cur[ab.y] == t)) {
+                const uint64_t used = bmask[ab.x] | (bd ? bmask[ab.y] : 0ull);
+                uint32_t c;
+                if (~used == 0ull) {
+                    c = 64u + r;
+                } else {
+                    c = (uint32_t)__ffsll((unsigned long long)~used) - 1u;
+                    bmask[ab.x] = bmask[ab.x] | (1ull << c);
+                    if (bd) bmask[ab.y] = bmask[ab.y] | (1ull << c);
+                }
+                col[m] = c;
+                ++won;
+                cmax = c + 1u > cmax ? c + 1u : cmax;
+            } else {
+                const uint64_t t2 = ((uint64_t)(r + 2) << 48) | prio[m];
+                atomicMax((unsigned long long*)&nxt[ab.x], (unsigned long long)t2);
+                if (bd) atomicMax((unsigned long long*)&nxt[ab.y], (unsigned long long)t2);
+            }
+        }
+        if (won) {
+            atomicSub(&cnt->remaining, won);
+            atomicMax(&cnt->max_colour, cmax);
+            atomicMax(&cnt->last_round, r + 1u);
+        }
+        grid_sync(bar, nb, cnt);
+        const uint32_t left = __hip_atomic_load(&cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t bad = __hip_atomic_load(&cnt->sticky, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (left == 0 || (bad & STICKY_TIMEOUT)) break;
+        if (r + 1 >= max_rounds) {
+            if (g0 == 0) atomicOr(&cnt->sticky, STICKY_COLOURS);
+            break;
+        }
+    }
+}
+
+// Colour sort input: keys = colour, values = manifold index.
+__global__ void __launch_bounds__(256) k_col_sort_input(const uint32_t* __restrict__ col, uint32_t cap,
+                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ val,
+                                                        const Counters* __restrict__ cnt) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= cap || m >= cnt->m_eff) return;
+    key[m] = col[m];
+    val[m] = m;
 }
 
 __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
@@ -356,9 +453,12 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
 }
 
 // colour offsets: off[c] = first sorted position with colour >= c (c = 0..K)
-__global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t M, uint32_t K, uint32_t* __restrict__ off) {
+__global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t max_k, uint32_t* __restrict__ off,
+                              Counters* __restrict__ cnt) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > K) return;
+    const uint32_t K = cnt->max_colour, M = cnt->m_eff;
+    if (c == 0 && K > max_k) atomicOr(&cnt->sticky, STICKY_COLOURS);
+    if (c > K || c > max_k) return;
     uint32_t lo = 0, hi = M;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -418,7 +518,7 @@ struct PrevCache {
 
 __device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32_t& s) {
     uint32_t h = (uint32_t)mix64(k) & c.mask;
-    for (;;) {
+    for (uint32_t probe = 0; probe <= c.mask; ++probe) {
         const uint64_t kk = c.key[h];
         if (kk == k) {
             s = c.slot[h];
@@ -427,16 +527,19 @@ __device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32
         if (kk == EMPTY_KEY) return false;
         h = (h + 1) & c.mask;
     }
+    return false;
 }
 
-__global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order, uint32_t M,
+__global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
                                               uint32_t cap, const uint2* __restrict__ m_ab,
                                               const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
-                                              const uint64_t* __restrict__ bmask, Counters* __restrict__ cnt) {
+                                              const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
+                                              Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
+    const uint32_t M = cnt->m_eff;
+    if (s >= M || cnt->sticky != 0u) return;  // a failed step must not touch the impulse buffers
     const uint32_t m = order[s];
     const uint2 ab = m_ab[m];
     const uint32_t np = m_np[m];
@@ -450,13 +553,17 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         const uint32_t c = col[m];
         uint2 dp = make_uint2(UNSET, UNSET);
         if (c < 64u) {
+            // a body's colour mask must hold one bit per manifold of the body, or the
+            // persistent sweep's visit counts could never be reached
             const unsigned long long below = (1ull << c) - 1ull;
             if (A.dyn) {
                 const unsigned long long mk = bmask[ab.x];
+                if ((uint32_t)__popcll(mk) != bdeg[ab.x]) atomicOr(&cnt->sticky, STICKY_INTERNAL);
                 dp.x = ((uint32_t)__popcll(mk) << 8) | (uint32_t)__popcll(mk & below);
             }
             if (Bd.dyn) {
                 const unsigned long long mk = bmask[ab.y];
+                if ((uint32_t)__popcll(mk) != bdeg[ab.y]) atomicOr(&cnt->sticky, STICKY_INTERNAL);
                 dp.y = ((uint32_t)__popcll(mk) << 8) | (uint32_t)__popcll(mk & below);
             }
         }
@@ -673,11 +780,19 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 // grid-wide barrier.  Chunks are taken in order by running waves, so the
 // smallest pending item always has its dependencies complete (no deadlock);
 // a bounded spin sets `err` instead of hanging.
-__global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, uint32_t M, uint32_t cap,
+__global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
                                                   uint32_t sweep0, uint32_t nsweeps, uint32_t* __restrict__ bdone,
-                                                  uint32_t* __restrict__ queue, uint32_t* __restrict__ err,
                                                   uint32_t spin_limit) {
     const uint32_t lane = __lane_id();
+    const uint32_t M = cnt->m_eff;
+    if (cnt->max_colour > 64u) {
+        // dependency ranks come from 64-bit colour masks: hand the step back for per-colour launches
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->sticky, STICKY_FLOW_COLOURS);
+        return;
+    }
+    if (cnt->sticky != 0u) return;
+    uint32_t* queue = &cnt->queue;
+    uint32_t* err = &cnt->sticky;
     const uint64_t total = (uint64_t)nsweeps * M;
     const uint64_t nchunks = (total + 63) / 64;
     for (;;) {
@@ -699,6 +814,7 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
         const uint32_t tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
         const uint32_t tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
         uint32_t spins = 0;
+        const uint64_t t_start = wall_clock64();
         while (__ballot(pending)) {
             if (pending) {
                 bool ready = true;
@@ -712,12 +828,13 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
                     if (dp.x != UNSET) __hip_atomic_fetch_add(bdone + h.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (dp.y != UNSET) __hip_atomic_fetch_add(bdone + h.y, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     pending = false;
-                } else if (++spins > spin_limit) {
-                    atomicOr(err, 1u);
-                    pending = false;
-                } else if ((spins & 1023u) == 0u &&
-                           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                    pending = false;  // another wave already gave up: drain
+                } else if ((++spins & 255u) == 0u) {
+                    if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
+                        atomicOr(err, STICKY_TIMEOUT);  // wait longer than spin_limit microseconds
+                        pending = false;
+                    } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                        pending = false;  // another wave already gave up: drain
+                    }
                 }
             }
             if (__ballot(pending)) __builtin_amdgcn_s_sleep(1);
@@ -725,9 +842,10 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
     }
 }
 
-__global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, SolverRows R, uint32_t M, uint32_t cap,
-                                                  uint32_t* __restrict__ out_bits) {
+__global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, SolverRows R, const Counters* cnt,
+                                                  uint32_t cap, uint32_t* __restrict__ out_bits) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t M = cnt->m_eff;
     float res = 0.0f;
     if (s < M) {
         const uint4 h = R.hdr[s];
@@ -770,14 +888,16 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
 }
 
 // Previous-step lookup table for the warm start: (a, b) -> slot.
-__global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, uint32_t M, uint64_t* __restrict__ key,
-                                                     uint32_t* __restrict__ slot, uint32_t mask) {
+__global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counters* cnt,
+                                                     uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                     uint32_t mask, uint32_t* __restrict__ np_out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
+    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    np_out[s] = R.hdr[s].z;
     const uint4 h = R.hdr[s];
     const uint64_t k = ((uint64_t)h.x << 32) | h.y;
     uint32_t q = (uint32_t)mix64(k) & mask;
-    for (;;) {
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
         const unsigned long long prev =
             atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
         if (prev == EMPTY_KEY || prev == k) {
@@ -786,15 +906,50 @@ __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, uint32_
         }
         q = (q + 1) & mask;
     }
+    atomicOr(&cnt->sticky, STICKY_INTERNAL);  // table full: never expected (sized >= 2x the slots)
 }
 
-__global__ void __launch_bounds__(256) k_copy_np(const SolverRows R, uint32_t M, uint32_t* __restrict__ np_out) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < M) np_out[s] = R.hdr[s].z;
-}
-
-__global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint32_t n) {
+// Moves the entries of an (a, b) -> slot table into a larger empty one.
+__global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict__ okey,
+                                                      const uint32_t* __restrict__ oslot, uint32_t on,
+                                                      uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                      uint32_t mask, Counters* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= on) return;
+    const uint64_t k = okey[i];
+    if (k == EMPTY_KEY) return;
+    const uint32_t v = oslot[i];
+    uint32_t q = (uint32_t)mix64(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+        if (prev == EMPTY_KEY || prev == k) {
+            slot[q] = v;
+            return;
+        }
+        q = (q + 1) & mask;
+    }
+    atomicOr(&cnt->sticky, STICKY_INTERNAL);
+}
+
+
+
+// Clears the per-step part of the control block.
+__global__ void k_counters_begin(Counters* cnt) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
+    if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
+}
+
+// Integration is skipped while any sticky error is set, so an overflowing step
+// leaves the state exactly as it was.
+__global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint32_t n, Counters* cnt,
+                                                   uint32_t sweeps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt->sticky != 0u) return;
+    if (i == 0) {
+        cnt->steps_done += 1u;
+        cnt->pgs_bytes_total += cnt->pgs_bytes * (unsigned long long)sweeps;
+    }
     if (i >= n) return;
     const float4 v4 = B.sv[i];
     if (!(v4.w > 0.0f)) return;
@@ -818,6 +973,15 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 // ---------------------------------------------------------------- host launchers
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 
+__global__ void k_flow_reset(uint32_t* __restrict__ bdone, uint32_t n, Counters* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) cnt->queue = 0u;
+    if (i < n) bdone[i] = 0u;
+}
+
+void launch_counters_begin(Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt);
+}
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
 }
@@ -830,8 +994,8 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
 }
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t);
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, hipStream_t s) {
@@ -839,44 +1003,47 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
-                      hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb);
+                      Counters* cnt, hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb, cnt);
 }
-void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t P,
-                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t epa_cap, Counters* cnt,
-                   hipStream_t s) {
-    if (!P) return;
-    hipLaunchKernelGGL(k_narrow, grid_for(P), dim3(256), 0, s, B, sp, pa, pb, P, p_np, p_pt, p_nf, epa_list, cnt);
-    hipLaunchKernelGGL(k_narrow_epa, grid_for(epa_cap, 64), dim3(64), 0, s, B, sp, pa, pb, epa_list, epa_cap, p_np,
+void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_narrow, grid_for(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list, cnt);
+    hipLaunchKernelGGL(k_narrow_epa, grid_for(cap_pairs, 64), dim3(64), 0, s, B, sp, pa, pb, epa_list, cap_pairs, p_np,
                        p_pt, p_nf, cnt);
 }
-void launch_flag_nonzero(const uint32_t* np, uint32_t P, uint32_t* flag, hipStream_t s) {
-    if (P) hipLaunchKernelGGL(k_flag_nonzero, grid_for(P), dim3(256), 0, s, np, P, flag);
+void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_flag_nonzero, grid_for(cap_pairs), dim3(256), 0, s, np, cap_pairs, flag, cnt);
 }
-void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t P, const uint32_t* p_np, const float4* p_pt,
-                    const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab, uint32_t* m_np, float4* c_pt,
-                    float4* c_nf, Counters* cnt, hipStream_t s) {
-    if (P)
-        hipLaunchKernelGGL(k_compact, grid_for(P), dim3(256), 0, s, pa, pb, P, p_np, p_pt, p_nf, moff, cap, m_ab, m_np,
-                           c_pt, c_nf, cnt);
+void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, const uint32_t* p_np,
+                    const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,
+                    uint32_t* m_np, float4* c_pt, float4* c_nf, Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact, grid_for(cap_pairs), dim3(256), 0, s, pa, pb, p_np, p_pt, p_nf, moff, cap, m_ab,
+                       m_np, c_pt, c_nf, cnt);
 }
-void launch_col_init(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, uint64_t* prio, hipStream_t s) {
-    if (M) hipLaunchKernelGGL(k_col_init, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio);
+int colour_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_colour, 256, 0) != hipSuccess) return 1;
+    return nb < 1 ? 1 : nb;
 }
-void launch_col_round(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, const uint64_t* prio,
-                      uint64_t* bmax, uint64_t* bmask, uint32_t round, Counters* cnt, hipStream_t s) {
-    if (!M) return;
-    hipLaunchKernelGGL(k_col_max, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio, bmax, round);
-    hipLaunchKernelGGL(k_col_win, grid_for(M), dim3(256), 0, s, B, m_ab, M, col, prio, bmax, bmask, round, cnt);
+void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
+                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, n, col, prio, bmax0, bmax1, bmask, bdeg, bar,
+                       cnt, 65000u);
 }
-void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* off, hipStream_t s) {
-    hipLaunchKernelGGL(k_col_offsets, grid_for(K + 1), dim3(256), 0, s, scol, M, K, off);
+void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_col_sort_input, grid_for(cap), dim3(256), 0, s, col, cap, key, val, cnt);
 }
-void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
-                 const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
+void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_col_offsets, grid_for(max_k + 1), dim3(256), 0, s, scol, max_k, off, cnt);
+}
+void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
+                 const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 Counters* cnt, hipStream_t s) {
-    if (!M) return;
+                 const uint32_t* bdeg, Counters* cnt, hipStream_t s) {
     PrevCache pc;
     pc.key = pca.key;
     pc.slot = pca.slot;
@@ -886,35 +1053,46 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
     pc.np = pca.np;
     pc.cap = pca.cap;
     pc.valid = pca.valid;
-    hipLaunchKernelGGL(k_prep, grid_for(M), dim3(256), 0, s, B, sp, order, M, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, col, bmask, cnt);
+    hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
+                       warm_in, col, bmask, bdeg, cnt);
 }
-void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
-                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, uint32_t* queue, uint32_t* err,
-                     uint32_t spin_limit, uint32_t max_groups, hipStream_t s) {
-    if (!M || !nsweeps) return;
-    const uint64_t chunks = ((uint64_t)nsweeps * M + 63) / 64;
+void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
+                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
+                     uint32_t max_groups, hipStream_t s) {
+    if (!nsweeps) return;
+    hipLaunchKernelGGL(k_flow_reset, grid_for(n_bodies), dim3(256), 0, s, bdone, n_bodies, cnt);
+    const uint64_t chunks = ((uint64_t)nsweeps * cap + 63) / 64;
     uint64_t groups = (chunks + 3) / 4;
     if (groups > max_groups) groups = max_groups;
-    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, M, cap, sweep0, nsweeps, bdone,
-                       queue, err, spin_limit);
+    if (groups < 1) groups = 1;
+    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, cnt, cap, sweep0, nsweeps, bdone,
+                       spin_limit);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {
     if (s1 > s0) hipLaunchKernelGGL(k_pgs, grid_for(s1 - s0), dim3(256), 0, s, B, sp, R, s0, s1, cap, first);
 }
-void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,
                      uint32_t* out_bits, hipStream_t s) {
-    if (M) hipLaunchKernelGGL(k_residual, grid_for(M), dim3(256), 0, s, B, sp, R, M, cap, out_bits);
+    hipLaunchKernelGGL(k_residual, grid_for(cap), dim3(256), 0, s, B, sp, R, cnt, cap, out_bits);
 }
-void launch_cache_build(const SolverRows& R, uint32_t M, uint64_t* key, uint32_t* slot, uint32_t mask,
-                        uint32_t* np_out, hipStream_t s) {
-    if (!M) return;
-    hipLaunchKernelGGL(k_cache_build, grid_for(M), dim3(256), 0, s, R, M, key, slot, mask);
-    hipLaunchKernelGGL(k_copy_np, grid_for(M), dim3(256), 0, s, R, M, np_out);
+__global__ void k_cache_clear(uint64_t* __restrict__ key, uint32_t n, const Counters* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cnt->sticky == 0u) key[i] = EMPTY_KEY;
 }
-void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n);
+
+void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
+                        uint32_t mask, uint32_t* np_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_cache_clear, grid_for(mask + 1), dim3(256), 0, s, key, mask + 1, cnt);
+    hipLaunchKernelGGL(k_cache_build, grid_for(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out);
+}
+void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
+                         uint32_t mask, Counters* cnt, hipStream_t s) {
+    if (on) hipLaunchKernelGGL(k_cache_rehash, grid_for(on), dim3(256), 0, s, okey, oslot, on, key, slot, mask, cnt);
+}
+void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n, cnt, sweeps);
 }
 
 }  // namespace axm

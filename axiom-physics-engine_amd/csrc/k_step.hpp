@@ -30,40 +30,46 @@ struct PrevCacheArgs {
     uint32_t valid;
 };
 
+void launch_counters_begin(Counters* cnt, hipStream_t s);
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, hipStream_t s);
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s);
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, hipStream_t s);
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
-                      hipStream_t s);
-void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t P,
-                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t epa_cap, Counters* cnt,
+                      Counters* cnt, hipStream_t s);
+void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, Counters* cnt, hipStream_t s);
+void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s);
+void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, const uint32_t* p_np,
+                    const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,
+                    uint32_t* m_np, float4* c_pt, float4* c_nf, Counters* cnt, hipStream_t s);
+int colour_blocks_per_cu();
+void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
+                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
                    hipStream_t s);
-void launch_flag_nonzero(const uint32_t* np, uint32_t P, uint32_t* flag, hipStream_t s);
-void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t P, const uint32_t* p_np, const float4* p_pt,
-                    const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab, uint32_t* m_np, float4* c_pt,
-                    float4* c_nf, Counters* cnt, hipStream_t s);
-void launch_col_init(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, uint64_t* prio, hipStream_t s);
-void launch_col_round(const Bodies& B, const uint2* m_ab, uint32_t M, uint32_t* col, const uint64_t* prio,
-                      uint64_t* bmax, uint64_t* bmask, uint32_t round, Counters* cnt, hipStream_t s);
-void launch_col_offsets(const uint32_t* scol, uint32_t M, uint32_t K, uint32_t* off, hipStream_t s);
-void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t M, uint32_t cap,
-                 const uint2* m_ab, const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
-                 const PrevCacheArgs& pc, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 Counters* cnt, hipStream_t s);
-void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
-                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, uint32_t* queue, uint32_t* err,
-                     uint32_t spin_limit, uint32_t max_groups, hipStream_t s);
+void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
+                           hipStream_t s);
+void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);
+void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
+                 const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
+                 const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
+                 const uint32_t* bdeg, Counters* cnt, hipStream_t s);
+void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
+                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
+                     uint32_t max_groups, hipStream_t s);
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s);
-void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t M, uint32_t cap,
+void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,
                      uint32_t* out_bits, hipStream_t s);
-void launch_cache_build(const SolverRows& R, uint32_t M, uint64_t* key, uint32_t* slot, uint32_t mask,
-                        uint32_t* np_out, hipStream_t s);
-void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, hipStream_t s);
+void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
+                        uint32_t mask, uint32_t* np_out, hipStream_t s);
+// Rehashes the `on`-slot table (okey, oslot) into the empty table (key, slot, mask).
+void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
+                         uint32_t mask, Counters* cnt, hipStream_t s);
+void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps, hipStream_t s);
 
 }  // namespace axm

@@ -58,9 +58,12 @@ struct CellTable {
     uint32_t mask;
 };
 
+// Device-resident step control block.  Every count a later kernel needs is read
+// here on the device, so a step launches without host synchronisation (and can
+// be captured into a HIP graph).  The first block is cleared at every step start.
 struct Counters {
-    uint32_t pairs;        // total candidate pairs
-    uint32_t manifolds;
+    uint32_t pairs;        // candidate pairs found (may exceed the capacity)
+    uint32_t manifolds;    // manifolds found (may exceed the capacity)
     uint32_t contacts;
     uint32_t gjk_fail;
     uint32_t epa_fail;
@@ -70,19 +73,45 @@ struct Counters {
     uint32_t last_round;   // colouring: last round that coloured something + 1
     uint32_t epa_count;    // deep box-capsule pairs queued for EPA
     uint32_t residual_bits;
-    uint32_t pad[1];
-    unsigned long long pgs_bytes;  // algorithmic bytes of one PGS sweep (all colours)
-    unsigned long long pad2;
+    uint32_t p_eff;        // min(pairs, pair capacity)
+    uint32_t m_eff;        // min(manifolds, manifold capacity)
+    uint32_t queue;        // persistent sweep: next chunk
+    uint32_t pad0[2];
+    unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
+    // ---- kept across steps
+    uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
+    uint32_t steps_done;   // steps whose integration ran
+    unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
+};
+constexpr uint32_t COUNTERS_RESET_WORDS = 18;  // words cleared per step (through pgs_bytes)
+enum : uint32_t {
+    STICKY_PAIRS = 1u,        // pair capacity exceeded
+    STICKY_MANIFOLDS = 2u,    // manifold capacity exceeded
+    STICKY_COLOURS = 4u,      // more colours than the colour sort supports (256)
+    STICKY_TIMEOUT = 8u,      // a bounded wait expired (grid barrier or sweep dependency)
+    STICKY_FLOW_COLOURS = 16u, // persistent sweep needs <= 64 colours: rerun with one launch per colour
+    STICKY_INTERNAL = 32u      // an internal consistency check failed (colour masks vs manifold counts)
+};
+
+// Grid barrier state (generation counter; `count` returns to 0 after each barrier).
+struct GridBar {
+    uint32_t count;
+    uint32_t gen;
+    uint32_t pad[2];
 };
 
 // k_sort.hip
 size_t scan_tmp_words(uint32_t n);
-void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s);
+// out[0..n] = exclusive prefix sums, out[n] = total (also written to *total_out when given).
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s,
+                    uint32_t* total_out = nullptr);
 size_t radix_tmp_words(uint32_t n);
-// Returns 1 when the sorted result lives in the *_alt buffers.
+// Returns 1 when the sorted result lives in the *_alt buffers.  `n` is the
+// capacity the grid is sized for; with n_dev the element count is read on the
+// device (min(*n_dev, n)), so the launch needs no host synchronisation.
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr);
 
 }  // namespace axm

@@ -1,13 +1,21 @@
 // Host side of the C ABI (include/axiom/axm.h): world lifecycle, device buffer
 // management and the orchestration of one step on the world's HIP stream.
 //
+// A step is enqueued without host synchronisation: every count a kernel needs
+// lives in the device control block (Counters), grids are sized by capacity,
+// and the stage kernels read the live counts.  That makes the step capturable
+// into a HIP graph (axm_world_run with graph = 1) and leaves exactly one
+// synchronisation per axm_world_step (to read the statistics and the sticky
+// error word).  A step that overflows a capacity, needs more colours than the
+// persistent sweep supports, or times out in a bounded wait sets a sticky bit;
+// every kernel that writes persistent state then skips, so the body state and
+// the warm-start cache stay exactly as before the step and the host can grow
+// the buffers (or switch the sweep to one launch per colour) and run it again.
+//
 // Error handling follows the reference's no-exception Result/ErrorCode
 // convention (include/axiom/core/result.hpp:15-283,
-// include/axiom/core/error_code.hpp:21-57): every entry point returns an
-// axm_status; a failed step leaves the body state untouched (the canonical
-// state is written only by the final integration kernel).
+// include/axiom/core/error_code.hpp:21-57).
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,18 +29,18 @@ using namespace axm;
 
 namespace {
 
-#define AXM_HIP(expr)                                   \
-    do {                                                \
-        hipError_t e_ = (expr);                         \
-        if (e_ != hipSuccess) {                         \
-            std::fprintf(stderr, "axm: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
-            return AXM_GPU_OPERATION_FAILED;            \
-        }                                               \
+#define AXM_HIP(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            std::fprintf(stderr, "axm: %s failed: %s\n", #expr, hipGetErrorString(e_));    \
+            return AXM_GPU_OPERATION_FAILED;                                                 \
+        }                                                                                    \
     } while (0)
 
-#define AXM_TRY(expr)                    \
-    do {                                 \
-        axm_status s_ = (expr);          \
+#define AXM_TRY(expr)                     \
+    do {                                  \
+        axm_status s_ = (expr);           \
         if (s_ != AXM_SUCCESS) return s_; \
     } while (0)
 
@@ -46,6 +54,8 @@ int bits_for(uint32_t v) {
     while (b < 32 && (1ull << b) <= v) ++b;
     return b;
 }
+
+constexpr uint32_t MAX_SORT_COLOURS = 255;  // the colour sort is one 8-bit radix pass
 
 struct Alloc {
     std::vector<void*> ptrs;
@@ -62,11 +72,11 @@ struct Alloc {
         if (!p) return;
         auto it = std::find(ptrs.begin(), ptrs.end(), (void*)p);
         if (it != ptrs.end()) ptrs.erase(it);
-        hipFree(p);
+        (void)hipFree(p);
         p = nullptr;
     }
     void release_all() {
-        for (void* p : ptrs) hipFree(p);
+        for (void* p : ptrs) (void)hipFree(p);
         ptrs.clear();
     }
 };
@@ -79,6 +89,8 @@ struct axm_world_s {
     Alloc mem;
     uint32_t n = 0, cap_n = 0, cap_pairs = 0, cap_man = 0, n_worlds = 1;
     float cell = 1.0f;
+    int cu_count = 256;
+    uint32_t colour_blocks = 256;
 
     Bodies B{};
     float4 *pre_v = nullptr, *pre_w = nullptr;  // solver input velocities of the last step (kept on request)
@@ -89,23 +101,19 @@ struct axm_world_s {
 
     uint32_t *pair_cnt = nullptr, *pair_off = nullptr, *scan_tmp = nullptr;
     uint32_t *pa = nullptr, *pb = nullptr;
-    uint32_t *p_np = nullptr, *p_flag = nullptr, *man_off = nullptr, *epa_list = nullptr;
+    uint32_t *p_np = nullptr, *p_flag = nullptr, *man_off = nullptr, *epa_list = nullptr, *pscan_tmp = nullptr;
     float4 *p_pt = nullptr, *p_nf = nullptr;
-    uint32_t pscan_tmp_words = 0;
-    uint32_t* pscan_tmp = nullptr;
 
     uint2* m_ab = nullptr;
     uint32_t* m_np = nullptr;
     float4 *c_pt = nullptr, *c_nf = nullptr;
 
     uint32_t* col = nullptr;
-    uint64_t *prio = nullptr, *bmax = nullptr, *bmask = nullptr;
-    uint32_t* bdone = nullptr;      // persistent sweep: finished visits per body
-    uint32_t* flow_ctl = nullptr;   // persistent sweep: [0] work queue, [1] spin-limit error
+    uint64_t *prio = nullptr, *bmax0 = nullptr, *bmax1 = nullptr, *bmask = nullptr;
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
     uint32_t* col_off = nullptr;
-    uint32_t col_off_cap = 0;
-    uint32_t* order = nullptr;  // points into cval / cval_alt
+    uint32_t* bdone = nullptr;
+    GridBar* bar = nullptr;
 
     SolverRows R[2]{};
     uint32_t rcap[2] = {0, 0};
@@ -120,68 +128,49 @@ struct axm_world_s {
     uint32_t res_cap = 0;
     Counters* cnt = nullptr;
     Counters* h_cnt = nullptr;
-    float4* warm_dev = nullptr;
-    uint32_t warm_cap = 0;
 
     // kernel profiling
     bool prof_on = false;
     std::vector<hipEvent_t> prof_ev;
     axm_kernel_profile prof{};
 
+    // captured steps (one per warm-start buffer parity); with profiling on, the
+    // step is replayed as two graphs around an eagerly launched, event-timed sweep
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraphExec_t gpre[2] = {nullptr, nullptr};
+    hipGraphExec_t gpost[2] = {nullptr, nullptr};
+    float graph_dt = 0.0f;
+    uint32_t graph_iters = 0;
+    hipEvent_t gprof_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // sweep-kernel events per parity
+
     // last step
-    uint32_t P = 0, M = 0, K = 0, rounds = 0, iters_run = 0;
     bool have_step = false;
-    std::vector<uint32_t> h_col_off;
+    uint32_t last_iters = 0;
     std::vector<float> h_res;
 
+    void drop_graphs() {
+        for (hipGraphExec_t* set : {gexec, gpre, gpost})
+            for (int p = 0; p < 2; ++p)
+                if (set[p]) {
+                    (void)hipGraphExecDestroy(set[p]);
+                    set[p] = nullptr;
+                }
+    }
     ~axm_world_s() {
-        if (stream) hipStreamSynchronize(stream);
-        for (hipEvent_t e : prof_ev) hipEventDestroy(e);
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : gprof_ev)
+            if (e) (void)hipEventDestroy(e);
         mem.release_all();
-        if (h_cnt) hipHostFree(h_cnt);
-        if (stream) hipStreamDestroy(stream);
+        if (h_cnt) (void)hipHostFree(h_cnt);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 namespace {
 
-axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
-    Bodies& B = w->B;
-    AXM_TRY(w->mem.get(B.x, cap));
-    AXM_TRY(w->mem.get(B.q, cap));
-    AXM_TRY(w->mem.get(B.vel, cap));
-    AXM_TRY(w->mem.get(B.angv, cap));
-    AXM_TRY(w->mem.get(B.ii, cap));
-    AXM_TRY(w->mem.get(B.shape, cap));
-    AXM_TRY(w->mem.get(B.world, cap));
-    AXM_TRY(w->mem.get(B.world_first, cap + 1));
-    AXM_TRY(w->mem.get(B.sv, cap));
-    AXM_TRY(w->mem.get(B.sw, cap));
-    AXM_TRY(w->mem.get(B.I0, cap));
-    AXM_TRY(w->mem.get(B.I1, cap));
-    AXM_TRY(w->mem.get(B.amin, cap));
-    AXM_TRY(w->mem.get(B.amax, cap));
-    AXM_TRY(w->mem.get(w->pre_v, cap));
-    AXM_TRY(w->mem.get(w->pre_w, cap));
-    AXM_TRY(w->mem.get(w->key, cap));
-    AXM_TRY(w->mem.get(w->key_alt, cap));
-    AXM_TRY(w->mem.get(w->val, cap));
-    AXM_TRY(w->mem.get(w->val_alt, cap));
-    AXM_TRY(w->mem.get(w->sort_tmp, radix_tmp_words(cap)));
-    const uint32_t T = pow2_at_least(2ull * cap + 16);
-    AXM_TRY(w->mem.get(w->ct.key, T));
-    AXM_TRY(w->mem.get(w->ct.val, T));
-    w->ct.mask = T - 1;
-    AXM_TRY(w->mem.get(w->pair_cnt, cap));
-    AXM_TRY(w->mem.get(w->pair_off, cap + 1));
-    AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
-    AXM_TRY(w->mem.get(w->bmax, cap));
-    AXM_TRY(w->mem.get(w->bmask, cap));
-    AXM_TRY(w->mem.get(w->bdone, cap));
-    w->cap_n = cap;
-    return AXM_SUCCESS;
-}
-
+// ------------------------------------------------------------------ buffers
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
@@ -199,13 +188,42 @@ void free_bodies(axm_world_s* w) {
     w->mem.release(w->pair_cnt);
     w->mem.release(w->pair_off);
     w->mem.release(w->scan_tmp);
-    w->mem.release(w->bmax);
+    w->mem.release(w->bmax0);
+    w->mem.release(w->bmax1);
     w->mem.release(w->bmask);
     w->mem.release(w->bdone);
     w->cap_n = 0;
 }
 
-void free_pairs(axm_world_s* w) {
+axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
+    free_bodies(w);
+    Bodies& B = w->B;
+    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
+                      &w->pre_v, &w->pre_w};
+    for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
+    AXM_TRY(w->mem.get(B.world, cap));
+    AXM_TRY(w->mem.get(B.world_first, cap + 1));
+    AXM_TRY(w->mem.get(w->key, cap));
+    AXM_TRY(w->mem.get(w->key_alt, cap));
+    AXM_TRY(w->mem.get(w->val, cap));
+    AXM_TRY(w->mem.get(w->val_alt, cap));
+    AXM_TRY(w->mem.get(w->sort_tmp, radix_tmp_words(cap)));
+    const uint32_t T = pow2_at_least(2ull * cap + 16);
+    AXM_TRY(w->mem.get(w->ct.key, T));
+    AXM_TRY(w->mem.get(w->ct.val, T));
+    w->ct.mask = T - 1;
+    AXM_TRY(w->mem.get(w->pair_cnt, cap));
+    AXM_TRY(w->mem.get(w->pair_off, cap + 1));
+    AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
+    AXM_TRY(w->mem.get(w->bmax0, cap));
+    AXM_TRY(w->mem.get(w->bmax1, cap));
+    AXM_TRY(w->mem.get(w->bmask, cap));
+    AXM_TRY(w->mem.get(w->bdone, cap));
+    w->cap_n = cap;
+    return AXM_SUCCESS;
+}
+
+axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->pa);
     w->mem.release(w->pb);
     w->mem.release(w->p_np);
@@ -215,10 +233,6 @@ void free_pairs(axm_world_s* w) {
     w->mem.release(w->p_pt);
     w->mem.release(w->p_nf);
     w->mem.release(w->pscan_tmp);
-}
-
-axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
-    free_pairs(w);
     AXM_TRY(w->mem.get(w->pa, cap));
     AXM_TRY(w->mem.get(w->pb, cap));
     AXM_TRY(w->mem.get(w->p_np, cap));
@@ -259,11 +273,78 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
     AXM_TRY(w->mem.get(w->pc_slot[set], T));
+    // an empty table: no warm-start hit until a step has filled it
+    if (hipMemset(w->pc_key[set], 0xFF, 8ull * T) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    if (hipMemset(w->np_slot[set], 0, 4ull * cap) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     w->pc_mask[set] = T - 1;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
 
+// Grows the row set holding the previous step's impulses while keeping them:
+// impulses and feature ids are re-strided from [k * old + s] to [k * cap + s]
+// and the (a, b) -> slot table is rehashed into one sized for `cap` slots.
+axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
+    SolverRows& R = w->R[set];
+    const uint32_t old = w->rcap[set];
+    if (old >= cap) return AXM_SUCCESS;
+    float4 *lam = nullptr, *lam0 = nullptr, *r0 = nullptr, *r1 = nullptr, *r2 = nullptr, *r3 = nullptr;
+    uint32_t *fid = nullptr, *np = nullptr;
+    uint4* hdr = nullptr;
+    uint2* dep = nullptr;
+    AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(np, cap));
+    AXM_TRY(w->mem.get(lam0, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(r0, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(r1, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(r2, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(r3, (size_t)cap * 4));
+    AXM_TRY(w->mem.get(hdr, cap));
+    AXM_TRY(w->mem.get(dep, cap));
+    const uint32_t T = pow2_at_least(2ull * cap + 16);
+    uint64_t* tkey = nullptr;
+    uint32_t* tslot = nullptr;
+    AXM_TRY(w->mem.get(tkey, T));
+    AXM_TRY(w->mem.get(tslot, T));
+    launch_fill_u64(tkey, EMPTY_KEY, T, w->stream);
+    launch_cache_rehash(w->pc_key[set], w->pc_slot[set], w->pc_mask[set] + 1, tkey, tslot, T - 1, w->cnt, w->stream);
+    AXM_HIP(hipMemcpy2DAsync(lam, 16ull * cap, R.lam, 16ull * old, 16ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
+    AXM_HIP(hipMemcpy2DAsync(fid, 4ull * cap, R.fid, 4ull * old, 4ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
+    AXM_HIP(hipMemcpyAsync(np, w->np_slot[set], 4ull * old, hipMemcpyDeviceToDevice, w->stream));
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    w->mem.release(R.lam);
+    w->mem.release(R.fid);
+    w->mem.release(w->np_slot[set]);
+    w->mem.release(R.lam0);
+    w->mem.release(R.r0);
+    w->mem.release(R.r1);
+    w->mem.release(R.r2);
+    w->mem.release(R.r3);
+    w->mem.release(R.hdr);
+    w->mem.release(R.dep);
+    w->mem.release(w->pc_key[set]);
+    w->mem.release(w->pc_slot[set]);
+    w->pc_key[set] = tkey;
+    w->pc_slot[set] = tslot;
+    w->pc_mask[set] = T - 1;
+    R.lam = lam;
+    R.fid = fid;
+    w->np_slot[set] = np;
+    R.lam0 = lam0;
+    R.r0 = r0;
+    R.r1 = r1;
+    R.r2 = r2;
+    R.r3 = r3;
+    R.hdr = hdr;
+    R.dep = dep;
+    w->rcap[set] = cap;
+    return AXM_SUCCESS;
+}
+
+// Grows the manifold buffers.  The current row set is reallocated; the other
+// set (the previous step's impulses, i.e. the next warm start) is grown in place
+// so both sets always hold at least `cap` slots.
 axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->m_ab);
     w->mem.release(w->m_np);
@@ -287,8 +368,9 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->cval, cap));
     AXM_TRY(w->mem.get(w->cval_alt, cap));
     AXM_TRY(w->mem.get(w->csort_tmp, radix_tmp_words(cap)));
-    if (w->rcap[w->cur] < cap) AXM_TRY(alloc_rows(w, w->cur, cap));
     w->cap_man = cap;
+    if (w->rcap[w->cur] < cap) AXM_TRY(alloc_rows(w, w->cur, cap));
+    if (w->rcap[1 - w->cur] > 0 && w->rcap[1 - w->cur] < cap) AXM_TRY(grow_rows_keep(w, 1 - w->cur, cap));
     return AXM_SUCCESS;
 }
 
@@ -325,171 +407,91 @@ axm_status read_counters(axm_world_s* w) {
     return AXM_SUCCESS;
 }
 
-struct StageTimer {
-    bool on = false;
-    hipStream_t s = nullptr;
-    hipEvent_t ev[6] = {};
-    int n = 0;
-    void init(bool enable, hipStream_t st) {
-        on = enable;
-        s = st;
-        if (!on) return;
-        for (auto& e : ev) hipEventCreate(&e);
+axm_status clear_sticky(axm_world_s* w) {
+    AXM_HIP(hipMemsetAsync(&w->cnt->sticky, 0, sizeof(uint32_t), w->stream));
+    return AXM_SUCCESS;
+}
+
+axm_status prof_mark(axm_world_s* w, size_t& nev, bool capturing) {
+    if (!w->prof_on || capturing) return AXM_SUCCESS;
+    while (w->prof_ev.size() < nev + 1) {
+        hipEvent_t e;
+        AXM_HIP(hipEventCreate(&e));
+        w->prof_ev.push_back(e);
     }
-    void mark() {
-        if (on && n < 6) hipEventRecord(ev[n++], s);
-    }
-    float ms(int a, int b) {
-        float r = 0.0f;
-        if (on && b < n) hipEventElapsedTime(&r, ev[a], ev[b]);
-        return r;
-    }
-    ~StageTimer() {
-        if (on)
-            for (auto& e : ev) hipEventDestroy(e);
-    }
+    AXM_HIP(hipEventRecord(w->prof_ev[nev++], w->stream));
+    return AXM_SUCCESS;
+}
+
+// A step is enqueued in three phases so that a profiled graph replay can launch
+// the sweep kernel eagerly between two captured graphs, bracketed by events.
+enum : uint32_t { PH_PRE = 1u, PH_SWEEP = 2u, PH_POST = 4u, PH_ALL = 7u };
+constexpr uint32_t SWEEP_SPIN_LIMIT_US = 2000000u;  // bounded dependency wait of the sweep kernel
+constexpr uint32_t SWEEP_MAX_GROUPS = 4096u;
+
+struct StepOpts {
+    uint32_t iters = 10;
+    uint32_t flags = 0;
+    const float4* warm_in = nullptr;
+    bool colour_launches = false;  // one sweep launch per colour (the host reads the colour ranges)
+    bool capture = false;          // being captured into a graph: no host interaction
+    hipEvent_t* ev = nullptr;      // optional stage events: start, broad, narrow, solve, end
+    uint32_t phases = PH_ALL;      // which part of the step to enqueue
 };
 
-// ---- phase 1: velocity integration into the solver copies, AABBs, keys,
-// sort, cell table, candidate pairs.  The canonical body state is not touched.
-axm_status phase_broad(axm_world_s* w, const StepParams& sp, uint32_t& P, axm_step_stats& S) {
+// ---- broadphase: solver velocities, AABBs, keys, sort, cell table, pairs
+void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
     launch_begin(w->B, sp, n, w->key, w->val, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s);
-    uint64_t* skey = alt ? w->key_alt : w->key;
-    uint32_t* sval = alt ? w->val_alt : w->val;
+    const uint64_t* skey = alt ? w->key_alt : w->key;
+    const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_fill_u64(w->ct.key, EMPTY_KEY, w->ct.mask + 1, s);
-    launch_cell_table(skey, n, w->ct, s);
+    launch_cell_table(skey, n, w->ct, w->cnt, s);
     launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, s);
-    scan_exclusive(w->pair_cnt, w->pair_off, n, w->scan_tmp, s);
-    AXM_HIP(hipMemcpyAsync(&w->h_cnt->pairs, w->pair_off + n, 4, hipMemcpyDeviceToHost, s));
-    AXM_HIP(hipStreamSynchronize(s));
-    P = w->h_cnt->pairs;
-    if (P > w->cap_pairs) {
-        if (!w->desc.auto_grow) {
-            S.pairs = P;
-            S.overflow = 1;
-            return AXM_OUT_OF_RANGE;
-        }
-        AXM_TRY(alloc_pairs(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, (uint64_t)P + P / 2 + 1024)));
-    }
-    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, s);
-    return AXM_SUCCESS;
+    scan_exclusive(w->pair_cnt, w->pair_off, n, w->scan_tmp, s, &w->cnt->pairs);
+    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, s);
 }
 
-// ---- phase 2: narrowphase into per-pair staging, compaction into manifolds.
-axm_status phase_narrow(axm_world_s* w, const StepParams& sp, uint32_t P, uint32_t& M, axm_step_stats& S) {
+// ---- narrowphase into per-pair staging, compaction into manifolds (pair order)
+void enqueue_narrow(axm_world_s* w, const StepParams& sp) {
     hipStream_t s = w->stream;
-    M = 0;
-    if (!P) return AXM_SUCCESS;
-    launch_narrow(w->B, sp, w->pa, w->pb, P, w->p_np, w->p_pt, w->p_nf, w->epa_list, P, w->cnt, s);
-    launch_flag_nonzero(w->p_np, P, w->p_flag, s);
-    scan_exclusive(w->p_flag, w->man_off, P, w->pscan_tmp, s);
-    AXM_HIP(hipMemcpyAsync(&w->h_cnt->manifolds, w->man_off + P, 4, hipMemcpyDeviceToHost, s));
-    AXM_HIP(hipStreamSynchronize(s));
-    M = w->h_cnt->manifolds;
-    if (M > w->cap_man || w->rcap[w->cur] < M) {
-        if (!w->desc.auto_grow && M > w->cap_man) {
-            S.pairs = P;
-            S.manifolds = M;
-            S.overflow = 2;
-            return AXM_OUT_OF_RANGE;
-        }
-        AXM_TRY(alloc_manifolds(w, std::max(w->cap_man, M + M / 2 + 1024)));
-    }
-    launch_compact(w->pa, w->pb, P, w->p_np, w->p_pt, w->p_nf, w->man_off, w->cap_man, w->m_ab, w->m_np, w->c_pt,
-                   w->c_nf, w->cnt, s);
-    return AXM_SUCCESS;
+    launch_narrow(w->B, sp, w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->epa_list, w->cnt, s);
+    launch_flag_nonzero(w->p_np, w->cap_pairs, w->p_flag, w->cnt, s);
+    scan_exclusive(w->p_flag, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds);
+    launch_compact(w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->man_off, w->cap_man, w->m_ab, w->m_np,
+                   w->c_pt, w->c_nf, w->cnt, s);
 }
 
-// ---- phase 3: Jones-Plassmann colouring, stable order by colour, colour offsets.
-axm_status phase_colour(axm_world_s* w, uint32_t M, uint32_t& K, uint32_t& rounds) {
-    hipStream_t s = w->stream;
-    K = rounds = 0;
-    if (!M) {
-        w->h_col_off.assign(1, 0);
-        return AXM_SUCCESS;
-    }
-    launch_col_init(w->B, w->m_ab, M, w->col, w->prio, s);
-    AXM_HIP(hipMemsetAsync(w->bmax, 0, sizeof(uint64_t) * w->n, s));
-    AXM_HIP(hipMemsetAsync(w->bmask, 0, sizeof(uint64_t) * w->n, s));
-    w->h_cnt->remaining = M;
-    AXM_HIP(hipMemcpyAsync(&w->cnt->remaining, &w->h_cnt->remaining, 4, hipMemcpyHostToDevice, s));
-    uint32_t r = 0;
-    const uint32_t batch = 4;
-    for (;;) {
-        for (uint32_t k = 0; k < batch; ++k, ++r)
-            launch_col_round(w->B, w->m_ab, M, w->col, w->prio, w->bmax, w->bmask, r, w->cnt, s);
-        AXM_TRY(read_counters(w));
-        if (w->h_cnt->remaining == 0) break;
-        if (r >= 65000u) return AXM_CONSTRAINT_SOLVER_FAILURE;
-    }
-    K = w->h_cnt->max_colour;
-    rounds = w->h_cnt->last_round;
-    AXM_HIP(hipMemcpyAsync(w->ckey, w->col, sizeof(uint32_t) * M, hipMemcpyDeviceToDevice, s));
-    launch_iota(w->cval, M, s);
-    const int calt =
-        radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, M, std::max(8, bits_for(K)), w->csort_tmp, s);
-    uint32_t* scol = calt ? w->ckey_alt : w->ckey;
-    w->order = calt ? w->cval_alt : w->cval;
-    if (w->col_off_cap < K + 1) {
-        w->mem.release(w->col_off);
-        AXM_TRY(w->mem.get(w->col_off, K + 1 + 256));
-        w->col_off_cap = K + 1 + 256;
-    }
-    launch_col_offsets(scol, M, K, w->col_off, s);
-    w->h_col_off.resize(K + 1);
-    AXM_HIP(hipMemcpyAsync(w->h_col_off.data(), w->col_off, sizeof(uint32_t) * (K + 1), hipMemcpyDeviceToHost, s));
-    AXM_HIP(hipStreamSynchronize(s));
-    return AXM_SUCCESS;
-}
-
-// One full step.  warm_in (optional, device, manifold order x 4 points) overrides the warm start.
-axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, axm_step_stats* st,
-                     const float4* warm_in) {
-    if (!(dt > 0.0f) || !std::isfinite(dt)) return AXM_INVALID_PARAMETER;
-    axm_step_stats S{};
+// Broadphase, narrowphase, colouring and row preparation of one step.
+axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
-    const StepParams sp = make_params(w, dt);
-    StageTimer tm;
-    tm.init((flags & AXM_STEP_TIMINGS) != 0, s);
-    tm.mark();
-    AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
-    if (n == 0) {
-        w->have_step = true;
-        w->P = w->M = w->K = w->rounds = 0;
-        w->h_res.clear();
-        if (st) *st = S;
-        return AXM_SUCCESS;
-    }
-    uint32_t P = 0, M = 0, K = 0, rounds = 0;
-    axm_status e = phase_broad(w, sp, P, S);
-    if (e != AXM_SUCCESS) {
-        if (st) *st = S;
-        return e;
-    }
-    if (flags & AXM_STEP_KEEP_STAGES) {
-        AXM_HIP(hipMemcpyAsync(w->pre_v, w->B.sv, 16ull * n, hipMemcpyDeviceToDevice, s));
-        AXM_HIP(hipMemcpyAsync(w->pre_w, w->B.sw, 16ull * n, hipMemcpyDeviceToDevice, s));
-        w->have_pre = true;
-    } else {
-        w->have_pre = false;
-    }
-    tm.mark();
-    e = phase_narrow(w, sp, P, M, S);
-    if (e != AXM_SUCCESS) {
-        if (st) *st = S;
-        return e;
-    }
-    tm.mark();
-    AXM_TRY(phase_colour(w, M, K, rounds));
-    // ---- solver
     const int c = w->cur, pv = 1 - c;
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
+    if (o.ev) AXM_HIP(hipEventRecord(o.ev[0], s));
+    launch_counters_begin(w->cnt, s);
+    enqueue_broad(w, sp);
+    if (o.ev) AXM_HIP(hipEventRecord(o.ev[1], s));
+    if (o.flags & AXM_STEP_KEEP_STAGES) {
+        AXM_HIP(hipMemcpyAsync(w->pre_v, w->B.sv, 16ull * n, hipMemcpyDeviceToDevice, s));
+        AXM_HIP(hipMemcpyAsync(w->pre_w, w->B.sw, 16ull * n, hipMemcpyDeviceToDevice, s));
+    }
+    enqueue_narrow(w, sp);
+    if (o.ev) AXM_HIP(hipEventRecord(o.ev[2], s));
+    // ---- colouring and the stable order by colour
+    // bdone doubles as the per-body manifold count of the colouring (the sweep clears it before use)
+    launch_colour(w->B, w->m_ab, n, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->bar, w->cnt,
+                  w->colour_blocks, s);
+    launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
+    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff);
+    const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers
+    const uint32_t* order = w->cval_alt;
+    launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
+    // ---- solver
     PrevCacheArgs pca{};
     pca.valid = (w->prev_valid && w->desc.warm_start != AXM_WARM_NONE) ? 1u : 0u;
     pca.key = w->pc_key[pv];
@@ -500,86 +502,76 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
     pca.np = w->np_slot[pv];
     pca.cap = w->rcap[pv];
     if (!pca.key) pca.valid = 0;
-    launch_prep(w->B, sp, w->order, M, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, warm_in, w->col, w->bmask,
-                w->cnt, s);
-    const bool tol_mode = w->desc.residual_tol > 0.0f;
-    const uint32_t max_it = tol_mode ? std::max<uint32_t>(w->desc.max_iterations, 1u) : iters;
-    if (w->res_cap < max_it + 1) {
-        w->mem.release(w->res_bits);
-        AXM_TRY(w->mem.get(w->res_bits, max_it + 1));
-        w->res_cap = max_it + 1;
-    }
-    AXM_HIP(hipMemsetAsync(w->res_bits, 0, sizeof(uint32_t) * (max_it + 1), s));
-    const bool want_res = (flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
-    const uint32_t every = std::max<uint32_t>(w->desc.residual_every, 1u);
-    uint32_t it = 0;
+    launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
+                w->bdone, w->cnt, s);
+    launch_fill_u32(w->res_bits, 0u, w->res_cap, s);
+    return AXM_SUCCESS;
+}
+
+// Enqueues one step.  Synchronises only for per-colour sweep launches and for
+// residual-tolerance checks (neither is used under graph capture).
+axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
+    // every kernel below sizes its grid and row strides by the current set's capacity
+    if (w->rcap[w->cur] < w->cap_man || w->rcap[1 - w->cur] < w->cap_man) return AXM_GPU_INVALID_OPERATION;
+    const uint32_t n = w->n;
+    hipStream_t s = w->stream;
+    const StepParams sp = make_params(w, dt);
     size_t nev = 0;
-    const bool flow = w->desc.solver_mode == 0 && K <= 64 && M > 0;
-    auto prof_mark = [&]() -> axm_status {
-        if (!w->prof_on) return AXM_SUCCESS;
-        while (w->prof_ev.size() < nev + 1) {
-            hipEvent_t e;
-            AXM_HIP(hipEventCreate(&e));
-            w->prof_ev.push_back(e);
-        }
-        AXM_HIP(hipEventRecord(w->prof_ev[nev++], s));
-        return AXM_SUCCESS;
-    };
-    if (flow) {
-        AXM_HIP(hipMemsetAsync(w->flow_ctl, 0, 16, s));
-        while (it < max_it) {
-            uint32_t seg = max_it - it;
-            if (w->desc.record_residual) seg = 1;
-            else if (tol_mode) seg = std::min(seg, every);
-            AXM_HIP(hipMemsetAsync(w->bdone, 0, sizeof(uint32_t) * n, s));
-            AXM_HIP(hipMemsetAsync(w->flow_ctl, 0, 4, s));
-            AXM_TRY(prof_mark());
-            launch_pgs_flow(w->B, sp, R, M, rc, it, seg, w->bdone, w->flow_ctl, w->flow_ctl + 1, 1u << 20, 4096u, s);
-            AXM_TRY(prof_mark());
-            it += seg;
-            const bool last = it == max_it;
-            const bool check = tol_mode && (it % every == 0);
-            if (w->desc.record_residual || check || (last && want_res)) {
-                launch_residual(w->B, sp, R, M, rc, w->res_bits + it - 1, s);
-                if (check) {
-                    AXM_HIP(hipMemcpyAsync(&w->h_cnt->residual_bits, w->res_bits + it - 1, 4, hipMemcpyDeviceToHost, s));
-                    AXM_HIP(hipStreamSynchronize(s));
-                    float res;
-                    std::memcpy(&res, &w->h_cnt->residual_bits, 4);
-                    if (res <= w->desc.residual_tol) break;
-                }
-            }
-        }
+    const bool pre = (o.phases & PH_PRE) != 0, sweep = (o.phases & PH_SWEEP) != 0, post = (o.phases & PH_POST) != 0;
+    const int c = w->cur, pv = 1 - c;
+    const SolverRows& R = w->R[c];
+    const uint32_t rc = w->rcap[c];
+    const bool tol_mode = w->desc.residual_tol > 0.0f && !o.capture;
+    const uint32_t max_it = tol_mode ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
+    std::vector<uint32_t> off;
+    if (pre) AXM_TRY(enqueue_pre(w, sp, o));
+    const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
+    const uint32_t every = std::max<uint32_t>(w->desc.residual_every, 1u);
+    uint32_t it = sweep ? 0u : max_it;
+    if (sweep && o.colour_launches) {
+        AXM_TRY(read_counters(w));
+        const uint32_t K = std::min(w->h_cnt->max_colour, MAX_SORT_COLOURS);
+        off.resize(K + 1);
+        AXM_HIP(hipMemcpyAsync(off.data(), w->col_off, 4ull * (K + 1), hipMemcpyDeviceToHost, s));
+        AXM_HIP(hipStreamSynchronize(s));
     }
-    for (; !flow && it < max_it; ++it) {
-        for (uint32_t col = 0; col < K; ++col) {
-            const uint32_t s0 = w->h_col_off[col], s1 = w->h_col_off[col + 1];
-            if (s1 <= s0) continue;
-            AXM_TRY(prof_mark());
-            launch_pgs(w->B, sp, R, s0, s1, rc, it == 0 ? 1u : 0u, s);
-            AXM_TRY(prof_mark());
+    while (it < max_it) {
+        uint32_t seg = max_it - it;
+        if (w->desc.record_residual) seg = 1;
+        else if (tol_mode) seg = std::min(seg, every - (it % every));
+        AXM_TRY(prof_mark(w, nev, o.capture));
+        if (o.colour_launches) {
+            for (uint32_t k = 0; k < seg; ++k)
+                for (size_t col = 0; col + 1 < off.size(); ++col)
+                    launch_pgs(w->B, sp, R, off[col], off[col + 1], rc, it + k == 0 ? 1u : 0u, s);
+        } else {
+            launch_pgs_flow(w->B, sp, R, rc, n, it, seg, w->bdone, w->cnt, SWEEP_SPIN_LIMIT_US, SWEEP_MAX_GROUPS, s);
         }
-        const bool last = it + 1 == max_it;
-        const bool check = tol_mode && ((it + 1) % every == 0);
+        AXM_TRY(prof_mark(w, nev, o.capture));
+        it += seg;
+        const bool last = it == max_it;
+        const bool check = tol_mode && (it % every == 0);
         if (w->desc.record_residual || check || (last && want_res)) {
-            launch_residual(w->B, sp, R, M, rc, w->res_bits + it, s);
+            launch_residual(w->B, sp, R, w->cnt, rc, w->res_bits + it - 1, s);
             if (check) {
-                AXM_HIP(hipMemcpyAsync(&w->h_cnt->residual_bits, w->res_bits + it, 4, hipMemcpyDeviceToHost, s));
+                AXM_HIP(hipMemcpyAsync(&w->h_cnt->residual_bits, w->res_bits + it - 1, 4, hipMemcpyDeviceToHost, s));
                 AXM_HIP(hipStreamSynchronize(s));
                 float res;
                 std::memcpy(&res, &w->h_cnt->residual_bits, 4);
-                if (res <= w->desc.residual_tol) {
-                    ++it;
-                    break;
-                }
+                if (res <= w->desc.residual_tol) break;
             }
         }
     }
-    const uint32_t iters_run = it;
-    if (w->prof_on) {
+    w->last_iters = it;
+    if (!post) return AXM_SUCCESS;
+    launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
+    if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
+    launch_integrate(w->B, sp, n, w->cnt, it, s);
+    if (o.ev) AXM_HIP(hipEventRecord(o.ev[4], s));
+    if (w->prof_on && !o.capture) {
         AXM_HIP(hipStreamSynchronize(s));
         double ms = 0.0;
-        for (size_t k = 0; k < nev; k += 2) {
+        for (size_t k = 0; k + 1 < nev; k += 2) {
             float e = 0.0f;
             AXM_HIP(hipEventElapsedTime(&e, w->prof_ev[k], w->prof_ev[k + 1]));
             ms += e;
@@ -588,63 +580,163 @@ axm_status step_impl(axm_world_s* w, float dt, uint32_t iters, uint32_t flags, a
         w->prof.steps += 1;
         w->prof.pgs_launches += nev / 2;
         w->prof.pgs_ms += ms;
-        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)iters_run;
-        w->prof.sweeps += iters_run;
+        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)it;
+        w->prof.sweeps += it;
     }
-    tm.mark();
-    // warm-start table for the next step
-    launch_fill_u64(w->pc_key[c], EMPTY_KEY, w->pc_mask[c] + 1, s);
-    launch_cache_build(R, M, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
-    launch_integrate(w->B, sp, n, s);
-    tm.mark();
-    // ---- bookkeeping
-    w->P = P;
-    w->M = M;
-    w->K = K;
-    w->rounds = rounds;
-    w->iters_run = iters_run;
+    w->cur = pv;
     w->have_step = true;
     w->prev_valid = true;
-    w->cur = pv;
-    if (w->rcap[w->cur] < w->cap_man) AXM_TRY(alloc_rows(w, w->cur, w->cap_man));
-    const bool need_sync = st || (flags & AXM_STEP_SYNC) || want_res || tm.on;
-    w->h_res.clear();
-    if (flow) {
-        uint32_t ctl[2] = {0, 0};
-        AXM_HIP(hipMemcpyAsync(ctl, w->flow_ctl, 8, hipMemcpyDeviceToHost, s));
-        AXM_HIP(hipStreamSynchronize(s));
-        if (ctl[1]) return AXM_GPU_TIMEOUT;
-    }
-    if (need_sync) {
-        AXM_TRY(read_counters(w));
-        if (want_res && iters_run) {
-            std::vector<uint32_t> bits(iters_run);
-            AXM_HIP(hipMemcpy(bits.data(), w->res_bits, 4ull * iters_run, hipMemcpyDeviceToHost));
-            w->h_res.assign(iters_run, 0.0f);
-            for (uint32_t k = 0; k < iters_run; ++k) std::memcpy(&w->h_res[k], &bits[k], 4);
+    w->have_pre = (o.flags & AXM_STEP_KEEP_STAGES) != 0;
+    return AXM_SUCCESS;
+}
+
+void fill_stats(axm_world_s* w, axm_step_stats& S) {
+    const Counters& h = *w->h_cnt;
+    S.pairs = h.pairs;
+    S.manifolds = h.manifolds;
+    S.contacts = h.contacts;
+    S.colours = h.max_colour;
+    S.colour_rounds = h.last_round;
+    S.iterations = w->last_iters;
+    S.gjk_fail = h.gjk_fail;
+    S.epa_fail = h.epa_fail;
+    S.warm_hits = h.warm_hits;
+    S.overflow = (h.sticky & STICKY_PAIRS ? 1u : 0u) | (h.sticky & STICKY_MANIFOLDS ? 2u : 0u);
+}
+
+// Handles a sticky failure of the step just run: grows capacities or switches
+// to per-colour sweep launches.  Returns SUCCESS when the step should be run
+// again, an error otherwise.  The body state and warm-start buffers are
+// unchanged (every persistent write of the failed step was skipped).
+axm_status recover(axm_world_s* w, StepOpts& o) {
+    const uint32_t st = w->h_cnt->sticky;
+    w->cur = 1 - w->cur;  // the failed step's buffer flip is undone
+    if (st & STICKY_TIMEOUT) return AXM_GPU_TIMEOUT;
+    if (st & STICKY_INTERNAL) return AXM_GPU_OPERATION_FAILED;
+    if (st & STICKY_COLOURS) return AXM_CONSTRAINT_SOLVER_FAILURE;
+    if (st & (STICKY_PAIRS | STICKY_MANIFOLDS)) {
+        if (!w->desc.auto_grow) return AXM_OUT_OF_RANGE;
+        if (st & STICKY_PAIRS) {
+            const uint64_t P = w->h_cnt->pairs;
+            AXM_TRY(alloc_pairs(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, P + P / 2 + 1024)));
         }
-        S.pairs = P;
-        S.manifolds = M;
-        S.contacts = w->h_cnt->contacts;
-        S.colours = K;
-        S.colour_rounds = rounds;
-        S.iterations = iters_run;
-        S.gjk_fail = w->h_cnt->gjk_fail;
-        S.epa_fail = w->h_cnt->epa_fail;
-        S.warm_hits = w->h_cnt->warm_hits;
+        if (st & STICKY_MANIFOLDS) {
+            const uint64_t M = w->h_cnt->manifolds;
+            AXM_TRY(alloc_manifolds(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, M + M / 2 + 1024)));
+        }
+        w->drop_graphs();
+    }
+    if (st & STICKY_FLOW_COLOURS) o.colour_launches = true;
+    AXM_TRY(clear_sticky(w));
+    return AXM_SUCCESS;
+}
+
+axm_status ensure_res(axm_world_s* w, uint32_t iters) {
+    const uint32_t need = std::max<uint32_t>(iters, w->desc.residual_tol > 0.0f ? w->desc.max_iterations : 0u) + 1;
+    if (w->res_cap >= need) return AXM_SUCCESS;
+    w->mem.release(w->res_bits);
+    AXM_TRY(w->mem.get(w->res_bits, need));
+    w->res_cap = need;
+    w->drop_graphs();
+    return AXM_SUCCESS;
+}
+
+// One step with recovery; synchronises once.
+axm_status step_sync(axm_world_s* w, float dt, StepOpts o, axm_step_stats* st) {
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return AXM_INVALID_PARAMETER;
+    if (w->desc.solver_mode == 1) o.colour_launches = true;
+    AXM_TRY(ensure_res(w, o.iters));
+    axm_step_stats S{};
+    if (w->n == 0) {
+        w->have_step = true;
+        if (st) *st = S;
+        return AXM_SUCCESS;
+    }
+    const bool timing = (o.flags & AXM_STEP_TIMINGS) != 0;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (timing) {
+        for (auto& e : ev) AXM_HIP(hipEventCreate(&e));
+        o.ev = ev;
+    }
+    auto free_ev = [&]() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (int attempt = 0;; ++attempt) {
+        const axm_status es = enqueue_step(w, dt, o);
+        if (es != AXM_SUCCESS) {
+            free_ev();
+            return es;
+        }
+        AXM_TRY(read_counters(w));
+        if (w->h_cnt->sticky == 0u) break;
+        fill_stats(w, S);
+        const axm_status r = recover(w, o);
+        if (r != AXM_SUCCESS || attempt >= 3) {
+            if (st) *st = S;
+            free_ev();
+            return r != AXM_SUCCESS ? r : AXM_GPU_OPERATION_FAILED;
+        }
+    }
+    fill_stats(w, S);
+    const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || w->desc.residual_tol > 0.0f;
+    w->h_res.clear();
+    if (want_res && w->last_iters) {
+        std::vector<uint32_t> bits(w->last_iters);
+        AXM_HIP(hipMemcpy(bits.data(), w->res_bits, 4ull * w->last_iters, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < w->last_iters; ++k) {
+            float f;
+            std::memcpy(&f, &bits[k], 4);
+            if (bits[k] != 0u || w->desc.record_residual || k + 1 == w->last_iters) w->h_res.push_back(f);
+        }
         if (!w->h_res.empty()) {
             S.residual = w->h_res.back();
             S.residual_first = w->h_res.front();
         }
-        if (tm.on) {
-            S.broadphase_ms = tm.ms(0, 1);
-            S.narrowphase_ms = tm.ms(1, 2);
-            S.solver_ms = tm.ms(2, 3);
-            S.integration_ms = tm.ms(3, 4);
-            S.total_ms = tm.ms(0, 4);
-        }
-        if (st) *st = S;
     }
+    if (timing) {
+        float t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < 4; ++k) (void)hipEventElapsedTime(&t[k], ev[k], ev[k + 1]);
+        S.broadphase_ms = t[0];
+        S.narrowphase_ms = t[1];
+        S.solver_ms = t[2];
+        S.integration_ms = t[3];
+        (void)hipEventElapsedTime(&S.total_ms, ev[0], ev[4]);
+        free_ev();
+    }
+    if (st) *st = S;
+    return AXM_SUCCESS;
+}
+
+// Captures one step with the given warm-start buffer parity into a graph.
+axm_status capture(axm_world_s* w, float dt, uint32_t iters, int parity, uint32_t phases, hipGraphExec_t* out) {
+    const int saved_cur = w->cur;
+    const bool saved_prev = w->prev_valid, saved_step = w->have_step, saved_pre = w->have_pre;
+    const uint32_t saved_iters = w->last_iters;
+    w->cur = parity;
+    w->prev_valid = true;
+    StepOpts o;
+    o.iters = iters;
+    o.capture = true;
+    o.phases = phases;
+    AXM_HIP(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    const axm_status st = enqueue_step(w, dt, o);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(w->stream, &g);
+    w->cur = saved_cur;
+    w->prev_valid = saved_prev;
+    w->have_step = saved_step;
+    w->have_pre = saved_pre;
+    w->last_iters = saved_iters;
+    if (st != AXM_SUCCESS || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != AXM_SUCCESS ? st : AXM_GPU_OPERATION_FAILED;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    *out = ex;
     return AXM_SUCCESS;
 }
 
@@ -672,6 +764,12 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     axm_world_s* w = new (std::nothrow) axm_world_s;
     if (!w) return AXM_OUT_OF_MEMORY;
     w->desc = *desc;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, desc->device) == hipSuccess && prop.multiProcessorCount > 0)
+        w->cu_count = prop.multiProcessorCount;
+    // colouring kernel: one workgroup per CU, all resident (its grid barrier needs that);
+    // the occupancy query confirms at least one 256-thread workgroup fits per CU
+    w->colour_blocks = (uint32_t)w->cu_count * (uint32_t)std::min(colour_blocks_per_cu(), 1);
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
         delete w;
         return AXM_GPU_OPERATION_FAILED;
@@ -681,11 +779,16 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
         return AXM_BUFFER_ALLOCATION_FAILED;
     }
     axm_status st = w->mem.get(w->cnt, 1);
-    if (st == AXM_SUCCESS) st = w->mem.get(w->flow_ctl, 4);
+    if (st == AXM_SUCCESS) st = w->mem.get(w->bar, 1);
+    if (st == AXM_SUCCESS) st = w->mem.get(w->col_off, MAX_SORT_COLOURS + 2);
     if (st == AXM_SUCCESS) st = alloc_bodies(w, std::max<uint32_t>(desc->max_bodies, 16u));
-    if (st == AXM_SUCCESS) st = alloc_pairs(w, std::max<uint32_t>(desc->max_pairs, 1024u));
-    if (st == AXM_SUCCESS) st = alloc_manifolds(w, std::max<uint32_t>(desc->max_manifolds, 1024u));
-    if (st == AXM_SUCCESS) st = alloc_rows(w, 1 - w->cur, w->cap_man);
+    // explicit capacities are honoured as given; 0 means "sized from the body count"
+    if (st == AXM_SUCCESS) st = alloc_pairs(w, desc->max_pairs ? desc->max_pairs : 1024u);
+    if (st == AXM_SUCCESS) st = alloc_manifolds(w, desc->max_manifolds ? desc->max_manifolds : 1024u);
+    if (st == AXM_SUCCESS && w->rcap[1 - w->cur] < w->cap_man) st = alloc_rows(w, 1 - w->cur, w->cap_man);
+    if (st == AXM_SUCCESS && (hipMemset(w->cnt, 0, sizeof(Counters)) != hipSuccess ||
+                              hipMemset(w->bar, 0, sizeof(GridBar)) != hipSuccess))
+        st = AXM_GPU_OPERATION_FAILED;
     if (st != AXM_SUCCESS) {
         delete w;
         return st;
@@ -696,7 +799,7 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
 
 axm_status axm_world_destroy(axm_world w) {
     if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
+    (void)hipSetDevice(w->desc.device);
     delete w;
     return AXM_SUCCESS;
 }
@@ -713,10 +816,9 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
               !b->shape_params))
         return AXM_NULL_POINTER;
     if (n >= (1u << 29)) return AXM_OUT_OF_RANGE;
-    hipSetDevice(w->desc.device);
+    (void)hipSetDevice(w->desc.device);
     std::vector<float4> x(n), q(n), v(n), av(n), ii(n), sh(n);
     std::vector<uint32_t> wd(n);
-    std::vector<uint32_t> first;
     float rmax = 0.0f;
     uint32_t nw = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -762,19 +864,19 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         std::memcpy(&tf, &tb, 4);
         sh[i] = make_float4(pr[0], pr[1], pr[2], tf);
     }
-    if (nw > 4096u * 1024u) return AXM_OUT_OF_RANGE;
-    first.assign(nw + 1, 0);
+    if (nw > (1u << 22)) return AXM_OUT_OF_RANGE;
+    std::vector<uint32_t> first(nw + 1, 0);
     for (uint32_t i = n; i-- > 0;) first[wd[i]] = i;
     const float need = 2.0f * (rmax + w->desc.margin);
     float cell = w->desc.cell_size;
     if (cell == 0.0f) cell = need > 0.0f ? need : 1.0f;
     if (cell < need) return AXM_INVALID_PARAMETER;
-    if (n > w->cap_n) {
-        free_bodies(w);
-        AXM_TRY(alloc_bodies(w, n));
-    }
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    w->drop_graphs();
+    if (n > w->cap_n) AXM_TRY(alloc_bodies(w, n));
     if (w->desc.max_pairs == 0 && w->cap_pairs < 8ull * n) AXM_TRY(alloc_pairs(w, 8 * n));
     if (w->desc.max_manifolds == 0 && w->cap_man < 4ull * n) AXM_TRY(alloc_manifolds(w, 4 * n));
+    if (w->rcap[1 - w->cur] < w->cap_man) AXM_TRY(alloc_rows(w, 1 - w->cur, w->cap_man));
     hipStream_t s = w->stream;
     if (n) {
         AXM_HIP(hipMemcpyAsync(w->B.x, x.data(), 16ull * n, hipMemcpyHostToDevice, s));
@@ -786,18 +888,20 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         AXM_HIP(hipMemcpyAsync(w->B.world, wd.data(), 4ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * (nw + 1), hipMemcpyHostToDevice, s));
     }
+    AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
     AXM_HIP(hipStreamSynchronize(s));
     w->n = n;
     w->n_worlds = std::max(nw, 1u);
     w->cell = cell;
     w->prev_valid = false;
     w->have_step = false;
+    w->have_pre = false;
     return AXM_SUCCESS;
 }
 
 axm_status axm_world_get_bodies(axm_world w, axm_body_soa* out) {
     if (!w || !out) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
+    (void)hipSetDevice(w->desc.device);
     const uint32_t n = w->n;
     std::vector<float4> buf(n);
     auto pull = [&](float4* src, float* dst, int comps) -> axm_status {
@@ -822,7 +926,7 @@ axm_status axm_world_get_bodies(axm_world w, axm_body_soa* out) {
 axm_status axm_world_get_pre_solve_velocities(axm_world w, float* lin_vel, float* ang_vel) {
     if (!w) return AXM_NULL_POINTER;
     if (!w->have_pre) return AXM_GPU_INVALID_OPERATION;
-    hipSetDevice(w->desc.device);
+    (void)hipSetDevice(w->desc.device);
     const uint32_t n = w->n;
     std::vector<float4> v(n), a(n);
     AXM_HIP(hipMemcpy(v.data(), w->pre_v, 16ull * n, hipMemcpyDeviceToHost));
@@ -834,55 +938,118 @@ axm_status axm_world_get_pre_solve_velocities(axm_world w, float* lin_vel, float
     return AXM_SUCCESS;
 }
 
-axm_status axm_broadphase(axm_world w, uint32_t* count) {
-    if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
-    axm_step_stats S{};
-    const StepParams sp = make_params(w, 1.0f / 120.0f);
-    AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), w->stream));
-    uint32_t P = 0;
-    if (w->n) AXM_TRY(phase_broad(w, sp, P, S));
-    AXM_HIP(hipStreamSynchronize(w->stream));
-    w->P = P;
-    w->M = 0;
-    w->K = 0;
-    w->have_step = true;
-    if (count) *count = P;
-    return AXM_SUCCESS;
-}
-
-axm_status axm_collide(axm_world w, uint32_t* manifold_count) {
-    if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
-    axm_step_stats S{};
-    const StepParams sp = make_params(w, 1.0f / 120.0f);
-    AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), w->stream));
-    uint32_t P = 0, M = 0;
-    if (w->n) {
-        AXM_TRY(phase_broad(w, sp, P, S));
-        AXM_TRY(phase_narrow(w, sp, P, M, S));
-    }
-    AXM_HIP(hipStreamSynchronize(w->stream));
-    w->P = P;
-    w->M = M;
-    w->K = 0;
-    w->have_step = true;
-    if (manifold_count) *manifold_count = M;
-    return AXM_SUCCESS;
-}
-
 axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags, axm_step_stats* stats) {
     if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
-    return step_impl(w, dt, iters, flags, stats, nullptr);
+    (void)hipSetDevice(w->desc.device);
+    StepOpts o;
+    o.iters = iters;
+    o.flags = flags;
+    return step_sync(w, dt, o, stats);
 }
 
 axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last) {
     if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
-    (void)graph;
-    for (uint32_t k = 0; k < steps; ++k)
-        AXM_TRY(step_impl(w, dt, iters, k + 1 == steps ? AXM_STEP_SYNC : 0u, k + 1 == steps ? last : nullptr, nullptr));
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return AXM_INVALID_PARAMETER;
+    (void)hipSetDevice(w->desc.device);
+    if (steps == 0) return AXM_SUCCESS;
+    const bool use_graph = graph && w->desc.residual_tol == 0.0f && !w->desc.record_residual &&
+                           w->desc.solver_mode == 0 && w->n > 0;
+    if (!use_graph) {
+        StepOpts o;
+        o.iters = iters;
+        for (uint32_t k = 0; k + 1 < steps; ++k) AXM_TRY(step_sync(w, dt, o, nullptr));
+        return step_sync(w, dt, o, last);
+    }
+    AXM_TRY(ensure_res(w, iters));
+    uint32_t done = 0;
+    if (!w->prev_valid) {
+        // the first step after new bodies has no warm-start cache: run it eagerly
+        StepOpts o;
+        o.iters = iters;
+        AXM_TRY(step_sync(w, dt, o, steps == 1 ? last : nullptr));
+        done = 1;
+    }
+    while (done < steps) {
+        if (w->graph_dt != dt || w->graph_iters != iters) {
+            w->drop_graphs();
+            w->graph_dt = dt;
+            w->graph_iters = iters;
+        }
+        for (int p = 0; p < 2; ++p) {
+            if (w->prof_on) {
+                if (!w->gpre[p]) AXM_TRY(capture(w, dt, iters, p, PH_PRE, &w->gpre[p]));
+                if (!w->gpost[p]) AXM_TRY(capture(w, dt, iters, p, PH_POST, &w->gpost[p]));
+            } else if (!w->gexec[p]) {
+                AXM_TRY(capture(w, dt, iters, p, PH_ALL, &w->gexec[p]));
+            }
+        }
+        if (w->prof_on)
+            for (auto& e : w->gprof_ev)
+                if (!e) AXM_HIP(hipEventCreate(&e));
+        const StepParams sp = make_params(w, dt);
+        AXM_TRY(read_counters(w));
+        const uint32_t base = w->h_cnt->steps_done;
+        const unsigned long long base_bytes = w->h_cnt->pgs_bytes_total;
+        const int cur0 = w->cur;
+        const uint32_t todo = steps - done;
+        // With profiling, the sweep kernel of step k-1 is timed while step k
+        // already runs: the host waits on the previous step only, so the GPU
+        // queue never drains.
+        double prof_ms = 0.0;
+        int prev = -1;
+        for (uint32_t k = 0; k < todo; ++k) {
+            const int c = w->cur;
+            if (w->prof_on) {
+                AXM_HIP(hipGraphLaunch(w->gpre[c], w->stream));
+                AXM_HIP(hipEventRecord(w->gprof_ev[2 * c], w->stream));
+                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->n, 0, iters, w->bdone, w->cnt, SWEEP_SPIN_LIMIT_US,
+                                SWEEP_MAX_GROUPS, w->stream);
+                AXM_HIP(hipEventRecord(w->gprof_ev[2 * c + 1], w->stream));
+                AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
+            } else {
+                AXM_HIP(hipGraphLaunch(w->gexec[c], w->stream));
+            }
+            if (w->prof_on && prev >= 0) {
+                float e = 0.0f;
+                AXM_HIP(hipEventSynchronize(w->gprof_ev[2 * prev + 1]));
+                AXM_HIP(hipEventElapsedTime(&e, w->gprof_ev[2 * prev], w->gprof_ev[2 * prev + 1]));
+                prof_ms += e;
+            }
+            prev = w->cur;
+            w->cur = 1 - w->cur;
+        }
+        AXM_TRY(read_counters(w));
+        if (w->prof_on && prev >= 0) {
+            float e = 0.0f;
+            AXM_HIP(hipEventElapsedTime(&e, w->gprof_ev[2 * prev], w->gprof_ev[2 * prev + 1]));
+            prof_ms += e;
+        }
+        const uint32_t ok = w->h_cnt->steps_done - base;
+        if (w->prof_on) {
+            w->prof.steps += todo;
+            w->prof.pgs_launches += todo;
+            w->prof.pgs_ms += prof_ms;
+            w->prof.pgs_bytes += (double)(w->h_cnt->pgs_bytes_total - base_bytes);
+            w->prof.sweeps += (uint64_t)ok * iters;
+        }
+        done += ok;
+        w->have_step = true;
+        w->last_iters = iters;
+        if (ok >= todo) break;
+        // step `ok` of this batch failed and every later one skipped: repair and replay it eagerly
+        w->cur = (ok & 1u) ? 1 - cur0 : cur0;
+        w->cur = 1 - w->cur;  // recover() undoes one flip
+        StepOpts o;
+        o.iters = iters;
+        AXM_TRY(recover(w, o));
+        AXM_TRY(step_sync(w, dt, o, nullptr));
+        done += 1;
+    }
+    if (last) {
+        axm_step_stats S{};
+        fill_stats(w, S);
+        *last = S;
+    }
     return AXM_SUCCESS;
 }
 
@@ -896,20 +1063,74 @@ axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out) {
 
 axm_status axm_world_sync(axm_world w) {
     if (!w) return AXM_NULL_POINTER;
-    hipSetDevice(w->desc.device);
+    (void)hipSetDevice(w->desc.device);
     AXM_HIP(hipStreamSynchronize(w->stream));
+    return AXM_SUCCESS;
+}
+
+axm_status axm_broadphase(axm_world w, uint32_t* count) {
+    if (!w) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    const StepParams sp = make_params(w, 1.0f / 120.0f);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        launch_counters_begin(w->cnt, w->stream);
+        if (w->n) enqueue_broad(w, sp);
+        AXM_TRY(read_counters(w));
+        if (!(w->h_cnt->sticky & STICKY_PAIRS)) break;
+        if (!w->desc.auto_grow) return AXM_OUT_OF_RANGE;
+        const uint64_t P = w->h_cnt->pairs;
+        AXM_TRY(alloc_pairs(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, P + P / 2 + 1024)));
+        w->drop_graphs();
+        AXM_TRY(clear_sticky(w));
+    }
+    AXM_TRY(clear_sticky(w));
+    w->have_step = true;
+    if (count) *count = w->h_cnt->pairs;
+    return AXM_SUCCESS;
+}
+
+axm_status axm_collide(axm_world w, uint32_t* manifold_count) {
+    if (!w) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    const StepParams sp = make_params(w, 1.0f / 120.0f);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        launch_counters_begin(w->cnt, w->stream);
+        if (w->n) {
+            enqueue_broad(w, sp);
+            enqueue_narrow(w, sp);
+        }
+        AXM_TRY(read_counters(w));
+        const uint32_t st = w->h_cnt->sticky;
+        if (!(st & (STICKY_PAIRS | STICKY_MANIFOLDS))) break;
+        if (!w->desc.auto_grow) return AXM_OUT_OF_RANGE;
+        if (st & STICKY_PAIRS) {
+            const uint64_t P = w->h_cnt->pairs;
+            AXM_TRY(alloc_pairs(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, P + P / 2 + 1024)));
+        }
+        if (st & STICKY_MANIFOLDS) {
+            const uint64_t M = w->h_cnt->manifolds;
+            AXM_TRY(alloc_manifolds(w, (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, M + M / 2 + 1024)));
+        }
+        w->drop_graphs();
+        AXM_TRY(clear_sticky(w));
+    }
+    AXM_TRY(clear_sticky(w));
+    w->have_step = true;
+    if (manifold_count) *manifold_count = w->h_cnt->manifolds;
     return AXM_SUCCESS;
 }
 
 axm_status axm_world_get_pairs(axm_world w, uint32_t* a, uint32_t* b, uint32_t cap, uint32_t* count) {
     if (!w || !count) return AXM_NULL_POINTER;
     if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
-    hipSetDevice(w->desc.device);
-    *count = w->P;
-    if (cap < w->P) return AXM_OUT_OF_RANGE;
-    if (w->P) {
-        if (a) AXM_HIP(hipMemcpy(a, w->pa, 4ull * w->P, hipMemcpyDeviceToHost));
-        if (b) AXM_HIP(hipMemcpy(b, w->pb, 4ull * w->P, hipMemcpyDeviceToHost));
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t P = w->h_cnt->p_eff;
+    *count = P;
+    if (cap < P) return AXM_OUT_OF_RANGE;
+    if (P) {
+        if (a) AXM_HIP(hipMemcpy(a, w->pa, 4ull * P, hipMemcpyDeviceToHost));
+        if (b) AXM_HIP(hipMemcpy(b, w->pb, 4ull * P, hipMemcpyDeviceToHost));
     }
     return AXM_SUCCESS;
 }
@@ -918,8 +1139,9 @@ axm_status axm_world_get_manifolds(axm_world w, uint32_t* a, uint32_t* b, uint32
                                    float* depth, uint32_t* feature, uint32_t cap, uint32_t* count) {
     if (!w || !count) return AXM_NULL_POINTER;
     if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
-    hipSetDevice(w->desc.device);
-    const uint32_t M = w->M;
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t M = w->h_cnt->m_eff;
     *count = M;
     if (cap < M) return AXM_OUT_OF_RANGE;
     if (!M) return AXM_SUCCESS;
@@ -954,13 +1176,15 @@ axm_status axm_world_get_colours(axm_world w, uint32_t* colour, uint32_t* solve_
                                  uint32_t* num_colours) {
     if (!w || !count) return AXM_NULL_POINTER;
     if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
-    hipSetDevice(w->desc.device);
-    *count = w->M;
-    if (num_colours) *num_colours = w->K;
-    if (cap < w->M) return AXM_OUT_OF_RANGE;
-    if (w->M) {
-        if (colour) AXM_HIP(hipMemcpy(colour, w->col, 4ull * w->M, hipMemcpyDeviceToHost));
-        if (solve_order) AXM_HIP(hipMemcpy(solve_order, w->order, 4ull * w->M, hipMemcpyDeviceToHost));
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t M = w->h_cnt->m_eff;
+    *count = M;
+    if (num_colours) *num_colours = w->h_cnt->max_colour;
+    if (cap < M) return AXM_OUT_OF_RANGE;
+    if (M) {
+        if (colour) AXM_HIP(hipMemcpy(colour, w->col, 4ull * M, hipMemcpyDeviceToHost));
+        if (solve_order) AXM_HIP(hipMemcpy(solve_order, w->cval_alt, 4ull * M, hipMemcpyDeviceToHost));
     }
     return AXM_SUCCESS;
 }
@@ -968,8 +1192,9 @@ axm_status axm_world_get_colours(axm_world w, uint32_t* colour, uint32_t* solve_
 axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, uint32_t cap, uint32_t* count) {
     if (!w || !count) return AXM_NULL_POINTER;
     if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
-    hipSetDevice(w->desc.device);
-    const uint32_t M = w->M;
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t M = w->h_cnt->m_eff;
     *count = M;
     if (cap < M) return AXM_OUT_OF_RANGE;
     if (!M) return AXM_SUCCESS;

@@ -56,6 +56,23 @@ def cpu_baseline(desc, bodies, iters, dt, budget_s):
                        "%d bodies x %d steps (%.1f s)" % (bodies.n, total_steps, t))
 
 
+def copy_calibration(mib, reps):
+    """Plain device-to-device copies of a known size (read `mib` MiB, write `mib` MiB per
+    dispatch), for scaling the HBM counters FETCH_SIZE / WRITE_SIZE of a counter run.
+    Uses PyTorch only (the physics library is not loaded in this mode)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": n * 4, "bytes_written": n * 4, "dispatches": reps}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +86,13 @@ def main():
     ap.add_argument("--max-iters", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run plain copies of MIB MiB (counter scaling), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build is single-GPU (every box has one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -99,7 +122,7 @@ def main():
     w.profile(True)
     w.sync()
     t0 = time.perf_counter()
-    st = w.run(dt, iters, args.steps)
+    st = w.run(dt, iters, args.steps, graph=not args.eager)
     w.sync()
     elapsed = time.perf_counter() - t0
     prof = w.profile(False)
@@ -132,6 +155,7 @@ def main():
         "config": {
             "workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "sweeps": iters,
             "residual_tol": args.tol, "dt": dt, "mode": "plain GPU, previous-step warm start",
+            "launch": "eager" if args.eager else "hipGraph replay",
         },
         "residual_last_step": diag.residual,
         "last_step": {k: diag.as_dict()[k] for k in ("pairs", "manifolds", "contacts", "colours", "colour_rounds",

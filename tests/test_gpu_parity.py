@@ -129,3 +129,87 @@ def test_colour_launch_mode_matches_persistent(axm, orc, gpu):
         out.append((g.pos.copy(), g.lin_vel.copy(), g.ang_vel.copy()))
     for x, y in zip(out[0], out[1]):
         assert np.array_equal(x, y)
+
+
+def test_graph_replay_matches_eager(axm, orc, gpu):
+    """Steps replayed from a captured HIP graph equal eagerly launched steps bit for bit."""
+    b, d = axm.scene(1, seed(1), 4000)
+    outs = []
+    # eager; whole-step graphs; profiled replay (two graphs around an event-timed sweep launch)
+    for graph, prof in ((False, False), (True, False), (True, True)):
+        w = axm.World(d)
+        w.set_bodies(b)
+        if prof:
+            w.profile(True)
+        w.run(DT, 20, 12, graph=graph)
+        if prof:
+            p = w.profile(False)
+            assert p.pgs_launches == 12 and p.pgs_ms > 0 and p.pgs_bytes > 0
+        g = w.get_bodies()
+        outs.append((g.pos.copy(), g.quat.copy(), g.lin_vel.copy(), g.ang_vel.copy()))
+        w.close()
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert np.array_equal(x, y)
+
+
+def test_capacity_overflow_recovers(axm, orc, gpu):
+    """Tiny pair / manifold capacities: the step grows its buffers and reruns, same result as a roomy world."""
+    b, d = axm.scene(0, seed(0), 1000)
+    ref = axm.World(d)
+    ref.set_bodies(b)
+    for _ in range(5):
+        ref.step(DT, 10)
+    r = ref.get_bodies()
+    d2 = axm.default_desc()
+    for f, _ in axm.WorldDesc._fields_:
+        setattr(d2, f, getattr(d, f))
+    d2.max_pairs, d2.max_manifolds = 64, 64
+    small = axm.World(d2)
+    small.set_bodies(b)
+    small.run(DT, 10, 5, graph=True)
+    g = small.get_bodies()
+    assert np.array_equal(g.pos, r.pos) and np.array_equal(g.lin_vel, r.lin_vel)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_capacity_growth_keeps_warm_start_table(axm, orc, gpu, graph):
+    """A dense pile from tiny capacities: growing the previous step's row set must also grow its
+    (a, b) -> slot table (it ends up holding ~1k manifolds, far beyond the 64-slot sizing)."""
+    b, d = axm.scene(1, seed(1), 3000)
+    ref = axm.World(d)
+    ref.set_bodies(b)
+    for _ in range(6):
+        ref.step(DT, 20)
+    r = ref.get_bodies()
+    d2 = axm.default_desc()
+    for f, _ in axm.WorldDesc._fields_:
+        setattr(d2, f, getattr(d, f))
+    d2.max_pairs, d2.max_manifolds = 64, 64
+    small = axm.World(d2)
+    small.set_bodies(b)
+    if graph:
+        small.run(DT, 20, 6, graph=True)
+    else:
+        for _ in range(6):
+            small.step(DT, 20)
+    st = small.step(DT, 20)
+    assert st.manifolds > 256
+    g = small.get_bodies()
+    ref.step(DT, 20)
+    r = ref.get_bodies()
+    assert np.array_equal(g.pos, r.pos) and np.array_equal(g.lin_vel, r.lin_vel)
+
+
+def test_capacity_overflow_reports_out_of_range(axm, orc, gpu):
+    b, d = axm.scene(0, seed(0), 1000)
+    d.max_pairs, d.max_manifolds, d.auto_grow = 64, 64, 0
+    w = axm.World(d)
+    w.set_bodies(b)
+    before = w.get_bodies()
+    with pytest.raises(axm.AxmError) as e:
+        w.step(DT, 10)
+    assert e.value.code == 601  # OutOfRange
+    after = w.get_bodies()
+    # the failed step left the state untouched
+    assert np.array_equal(after.pos, before.pos) and np.array_equal(after.lin_vel, before.lin_vel)
