@@ -87,7 +87,7 @@ EXPORTS = [
     "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
-    "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_rng_stream",
+    "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
 ]
 
 _lib = None
@@ -130,6 +130,7 @@ def lib():
     L.axm_gjk_epa_pairs.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, f32p, i32p]
     L.axm_scene_generate.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(BodySoA), u32p,
                                      C.POINTER(WorldDesc)]
+    L.axm_mass_properties.argtypes = [C.c_uint32, f32p, C.c_float, f32p, f32p]
     L.axm_rng_stream.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_float, C.c_float, f32p]
     for name in EXPORTS:
         f = getattr(L, name)
@@ -211,6 +212,16 @@ def scene(config, seed, n=0, n_worlds=0):
     s = b.soa()
     _check(L.axm_scene_generate(config, seed, n, n_worlds, C.byref(s), C.byref(cnt), C.byref(d)), "scene")
     return b, d
+
+
+def mass_properties(shape_type, params, density=1000.0):
+    """(inverse mass, inverse principal inertia[3]) of a solid shape."""
+    prm = np.zeros(3, np.float32)
+    prm[:len(params)] = params
+    im = C.c_float()
+    ii = np.zeros(3, np.float32)
+    _check(lib().axm_mass_properties(shape_type, _p(prm), density, C.byref(im), _p(ii)), "axm_mass_properties")
+    return im.value, ii
 
 
 def rng_stream(seed, n, lo=0.0, hi=1.0):

@@ -55,28 +55,43 @@ struct Body {
     uint32_t type, world;
 };
 
-// Mass properties for density 1000 kg/m^3 (closed forms: solid sphere, box on
-// half extents, capsule = cylinder + two hemispheres along Y).
-void mass_props(Body& b) {
-    double m = 0, I[3] = {0, 0, 0};
-    if (b.type == AXM_SHAPE_SPHERE) {
-        const double r = b.prm[0];
-        m = kDensity * 4.0 / 3.0 * M_PI * r * r * r;
+// Mass and body-frame principal inertia (closed forms: solid sphere; box on
+// half extents; capsule along Y = cylinder of height 2h + two hemispheres, each
+// hemisphere moved from its centroid (3r/8 off the flat face) by the parallel
+// axis theorem).  Returns false for an unknown type or non-positive size.
+bool mass_inertia(uint32_t type, const float* prm, double density, double& m, double I[3]) {
+    m = 0.0;
+    I[0] = I[1] = I[2] = 0.0;
+    if (!(density > 0.0)) return false;
+    if (type == AXM_SHAPE_SPHERE) {
+        const double r = prm[0];
+        if (!(r > 0.0)) return false;
+        m = density * 4.0 / 3.0 * M_PI * r * r * r;
         I[0] = I[1] = I[2] = 0.4 * m * r * r;
-    } else if (b.type == AXM_SHAPE_BOX) {
-        const double x = b.prm[0], y = b.prm[1], z = b.prm[2];
-        m = kDensity * 8.0 * x * y * z;
+    } else if (type == AXM_SHAPE_BOX) {
+        const double x = prm[0], y = prm[1], z = prm[2];
+        if (!(x > 0.0 && y > 0.0 && z > 0.0)) return false;
+        m = density * 8.0 * x * y * z;
         I[0] = m / 3.0 * (y * y + z * z);
         I[1] = m / 3.0 * (x * x + z * z);
         I[2] = m / 3.0 * (x * x + y * y);
-    } else {
-        const double r = b.prm[0], h = b.prm[1];
-        const double mc = kDensity * M_PI * r * r * (2.0 * h);
-        const double ms = kDensity * 4.0 / 3.0 * M_PI * r * r * r;
+    } else if (type == AXM_SHAPE_CAPSULE) {
+        const double r = prm[0], h = prm[1];
+        if (!(r > 0.0 && h >= 0.0)) return false;
+        const double mc = density * M_PI * r * r * (2.0 * h);
+        const double ms = density * 4.0 / 3.0 * M_PI * r * r * r;
         m = mc + ms;
         I[1] = mc * r * r / 2.0 + ms * 0.4 * r * r;
         I[0] = I[2] = mc * (r * r / 4.0 + (2.0 * h) * (2.0 * h) / 12.0) + ms * (0.4 * r * r + h * h + 0.75 * h * r);
+    } else {
+        return false;
     }
+    return true;
+}
+
+void mass_props(Body& b) {
+    double m, I[3];
+    mass_inertia(b.type, b.prm, kDensity, m, I);
     b.im = (float)(1.0 / m);
     for (int k = 0; k < 3; ++k) b.ii[k] = (float)(1.0 / I[k]);
 }
@@ -145,18 +160,21 @@ void gen_pile(uint64_t seed, uint32_t n, std::vector<Body>& out) {
 // 400 m cube above the ground (about 2% of bodies start overlapping).
 void gen_sparse(uint64_t seed, uint32_t n, std::vector<Body>& out) {
     Pcg r(seed);
+    // 4 m lattice (1M bodies fill the 400 m cube); smaller n keep the density
+    // and shrink the cube, so every sample has the same ~2 % initial overlap
     uint32_t side = (uint32_t)std::ceil(std::cbrt((double)n));
     if (side < 1) side = 1;
-    const float s = 400.0f / (float)side;
+    const float s = 4.0f;
     const float J = 0.45f * s;
+    const float half = 0.5f * s * (float)side;
     for (uint32_t i = 0; i < n; ++i) {
         Body b{};
         b.world = 0;
         draw_shape(r, b, (i & 1u) ? AXM_SHAPE_BOX : AXM_SHAPE_SPHERE);
         const uint32_t ix = i % side, iy = (i / side) % side, iz = i / (side * side);
-        b.pos[0] = -200.0f + s * ((float)ix + 0.5f) + r.nextFloat(-J, J);
+        b.pos[0] = -half + s * ((float)ix + 0.5f) + r.nextFloat(-J, J);
         b.pos[1] = 1.5f + s * ((float)iy + 0.5f) + r.nextFloat(-J, J);
-        b.pos[2] = -200.0f + s * ((float)iz + 0.5f) + r.nextFloat(-J, J);
+        b.pos[2] = -half + s * ((float)iz + 0.5f) + r.nextFloat(-J, J);
         r.quat(b.quat);
         for (int k = 0; k < 3; ++k) b.v[k] = r.normal(1.0f);
         for (int k = 0; k < 3; ++k) b.w[k] = r.normal(0.5f);
@@ -217,6 +235,16 @@ void axm_world_desc_default(axm_world_desc* d) {
     d->device = 0;
 }
 
+axm_status axm_mass_properties(uint32_t type, const float* params, float density, float* inv_mass,
+                               float* inv_inertia) {
+    if (!params || !inv_mass || !inv_inertia) return AXM_NULL_POINTER;
+    double m, I[3];
+    if (!mass_inertia(type, params, (double)density, m, I)) return AXM_INVALID_SHAPE;
+    *inv_mass = (float)(1.0 / m);
+    for (int k = 0; k < 3; ++k) inv_inertia[k] = (float)(1.0 / I[k]);
+    return AXM_SUCCESS;
+}
+
 void axm_rng_stream(uint64_t seed, uint32_t n, uint32_t* u32_out, float lo, float hi, float* f_out) {
     if (u32_out) {
         Pcg r(seed);
@@ -260,7 +288,9 @@ axm_status axm_scene_generate(uint32_t config, uint64_t seed, uint32_t n, uint32
             uint32_t nw = (uint32_t)std::floor(std::exp(lo + (double)sizes.nextFloat() * (hi - lo)));
             if (nw < 16) nw = 16;
             if (nw > 4096) nw = 4096;
-            gen_mixed(seed + 1 + w, nw, w, bodies);
+            // DeterministicRNG starts from seed | 1, so seeds that differ only in
+            // bit 0 give one stream: world seeds step by 2
+            gen_mixed(seed + 2ull * (w + 1), nw, w, bodies);
         }
         d.friction = 0.5f;
         break;

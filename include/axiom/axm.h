@@ -228,6 +228,13 @@ axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a,
  * the body count and the recommended world description. */
 axm_status axm_scene_generate(uint32_t config, uint64_t seed, uint32_t n, uint32_t n_worlds, axm_body_soa* out,
                               uint32_t* count, axm_world_desc* desc);
+/* Inverse mass and inverse body-frame principal inertia of a solid shape of the
+ * given density (kg/m^3): sphere (r), box (half extents), capsule (r, half-height,
+ * axis +Y).  Host only.  Returns AXM_INVALID_SHAPE for an unknown type or a
+ * non-positive size.  Replaces the mass/inertia fields a caller fills in
+ * gui::RigidBodyData (include/axiom/gui/body_inspector.hpp:58-90). */
+axm_status axm_mass_properties(uint32_t type, const float* params, float density, float* inv_mass,
+                               float* inv_inertia);
 /* Raw DeterministicRNG stream (u32) and nextFloat(min,max) stream for a seed. */
 void axm_rng_stream(uint64_t seed, uint32_t n, uint32_t* u32_out, float lo, float hi, float* f_out);
 

@@ -101,6 +101,15 @@ def lib():
     L.orc_mlp_loss.restype = C.c_double
     L.orc_mlp_loss.argtypes = [f32p, f32p, f32p, C.c_int]
     L.orc_adam.argtypes = [f32p, f32p, f32p, f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_bf16_round.argtypes = [f32p, f32p, C.c_int]
+    L.orc_quat_mul.argtypes = [f32p, f32p, f32p]
+    L.orc_quat_rotate.argtypes = [f32p, f32p, f32p]
+    L.orc_world_inv_inertia.argtypes = [f32p, f32p, f32p]
+    L.orc_tangents.argtypes = [f32p, f32p, f32p]
+    L.orc_morton3.restype = C.c_uint32
+    L.orc_morton3.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_pair_hash24.restype = C.c_uint32
+    L.orc_pair_hash24.argtypes = [C.c_uint32, C.c_uint32]
     _lib = L
     return L
 
@@ -301,3 +310,47 @@ def mlp_loss(prm, X, Y):
 
 def adam(prm, m, v, g, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     lib().orc_adam(_p(prm), _p(m), _p(v), _p(np.ascontiguousarray(g, np.float32)), prm.size, t, lr, b1, b2, eps)
+
+
+def _f32(x):
+    return np.ascontiguousarray(x, np.float32)
+
+
+def quat_mul(a, b):
+    out = np.zeros(4, np.float32)
+    lib().orc_quat_mul(_p(_f32(a)), _p(_f32(b)), _p(out))
+    return out
+
+
+def quat_rotate(q, v):
+    out = np.zeros(3, np.float32)
+    lib().orc_quat_rotate(_p(_f32(q)), _p(_f32(v)), _p(out))
+    return out
+
+
+def world_inv_inertia(q, d):
+    out = np.zeros(6, np.float32)
+    lib().orc_world_inv_inertia(_p(_f32(q)), _p(_f32(d)), _p(out))
+    return out
+
+
+def tangents(n):
+    t1 = np.zeros(3, np.float32)
+    t2 = np.zeros(3, np.float32)
+    lib().orc_tangents(_p(_f32(n)), _p(t1), _p(t2))
+    return t1, t2
+
+
+def morton3(x, y, z):
+    return lib().orc_morton3(x, y, z)
+
+
+def pair_hash24(a, b):
+    return lib().orc_pair_hash24(a, b)
+
+
+def bf16_round(x):
+    x = _f32(x)
+    y = np.zeros_like(x)
+    lib().orc_bf16_round(_p(x), _p(y), x.size)
+    return y

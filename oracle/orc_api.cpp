@@ -144,6 +144,31 @@ extern "C" {
 uint32_t orc_params_size(void) { return (uint32_t)sizeof(Params); }
 uint32_t orc_stats_size(void) { return (uint32_t)sizeof(Stats); }
 
+// ---------------------------------------------------------------- math primitives (for known-answer tests)
+// Hamilton product a (x) b of (x, y, z, w) quaternions (include/axiom/math/quat.hpp:29-227 convention).
+void orc_quat_mul(const float* a, const float* b, float* out) {
+    const Q4<float> r = qmul(Q4<float>{a[0], a[1], a[2], a[3]}, Q4<float>{b[0], b[1], b[2], b[3]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+// Rotates v by the unit quaternion q (through its rotation matrix, as every stage does).
+void orc_quat_rotate(const float* q, const float* v, float* out) {
+    const V3<float> r = mul(quat_to_mat(Q4<float>{q[0], q[1], q[2], q[3]}), V3<float>{v[0], v[1], v[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+// World inverse inertia R diag(d) R^T as (xx, yy, zz, xy, xz, yz).
+void orc_world_inv_inertia(const float* q, const float* d, float* out) {
+    const S3<float> s = world_inv_inertia(quat_to_mat(Q4<float>{q[0], q[1], q[2], q[3]}), V3<float>{d[0], d[1], d[2]});
+    out[0] = s.xx; out[1] = s.yy; out[2] = s.zz; out[3] = s.xy; out[4] = s.xz; out[5] = s.yz;
+}
+void orc_tangents(const float* n, float* t1, float* t2) {
+    V3<float> a, b;
+    tangents(V3<float>{n[0], n[1], n[2]}, a, b);
+    t1[0] = a.x; t1[1] = a.y; t1[2] = a.z;
+    t2[0] = b.x; t2[1] = b.y; t2[2] = b.z;
+}
+uint32_t orc_morton3(uint32_t x, uint32_t y, uint32_t z) { return morton3(x, y, z); }
+uint32_t orc_pair_hash24(uint32_t a, uint32_t b) { return pair_hash24(a, b); }
+
 void* orc_create(const Params* p, int fp64) {
     Handle* h = new Handle;
     h->fp64 = fp64;
@@ -403,6 +428,10 @@ int orc_gjk_epa(int fp64, uint32_t ta, const float* pa, const float* xa, const f
 
 // ---------------------------------------------------------------- warm-start MLP
 uint32_t orc_mlp_nparam(void) { return (uint32_t)MLP_NPARAM; }
+
+void orc_bf16_round(const float* x, float* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = bf16r(x[i]);
+}
 
 void orc_mlp_forward(const float* prm, const float* X, int B, int emu, float* Y) {
     MlpCache<float> c;

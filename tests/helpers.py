@@ -5,7 +5,9 @@ SEED_BASE = 0x4158494F4D000000
 
 
 def seed(config, world=0):
-    return SEED_BASE + 1000 * config + world
+    """Scene seed of a configuration (world > 0: the seed of batched world world-1).
+    Seeds step by 2 because DeterministicRNG starts from seed | 1."""
+    return SEED_BASE + 1000 * config + 2 * world
 
 
 def pair_set(a, b):

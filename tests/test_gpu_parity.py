@@ -213,3 +213,95 @@ def test_capacity_overflow_reports_out_of_range(axm, orc, gpu):
     after = w.get_bodies()
     # the failed step left the state untouched
     assert np.array_equal(after.pos, before.pos) and np.array_equal(after.lin_vel, before.lin_vel)
+
+
+def test_golden_fixture_config0(axm, orc, gpu):
+    """The HIP path against the committed oracle fixture (steps 1 and 10 of the 1k-body scene)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0_oracle.npz"))
+    b, d = axm.scene(0, seed(0), 1000)
+    w = axm.World(d)
+    w.set_bodies(b)
+    for step in range(1, 11):
+        w.step(DT, 10, axm.STEP_KEEP_STAGES)
+        if step not in (1, 10):
+            continue
+        s = "s%d_" % step
+        pa, pb = w.pairs()
+        assert np.array_equal(pa, g[s + "pair_a"]) and np.array_equal(pb, g[s + "pair_b"])
+        m = w.manifolds()
+        assert np.array_equal(m["a"], g[s + "man_a"]) and np.array_equal(m["b"], g[s + "man_b"])
+        col, order, K = w.colours()
+        assert K == int(g[s + "colours"][0]) and np.array_equal(col, g[s + "colour"])
+        assert np.array_equal(order, g[s + "order"])
+        st = w.get_bodies()
+        assert rel_err(st.lin_vel, g[s + "lin_vel"]) <= 1e-4
+        assert rel_err(st.ang_vel, g[s + "ang_vel"]) <= 1e-4
+        assert rel_err(st.pos, g[s + "pos"]) <= 1e-5
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (0, 2), (2, 0), (1, 1), (1, 2), (2, 1), (2, 2), (0, 3),
+                                   (1, 3), (2, 3)])
+def test_collide_pairs_match_oracle(axm, orc, gpu, ta, tb):
+    """Batched pairwise narrowphase on the GPU vs the oracle, per shape-type combination:
+    points, normals and depths within 1e-5 relative; point counts and feature ids exact."""
+    rng = np.random.default_rng(100 + 10 * ta + tb)
+    n = 512
+    prm = {0: lambda: [rng.uniform(0.25, 0.75), 0, 0], 1: lambda: list(rng.uniform(0.25, 0.75, 3)),
+           2: lambda: [rng.uniform(0.2, 0.5), rng.uniform(0.2, 1.0), 0]}
+
+    def quats(k):
+        q = rng.normal(size=(k, 4))
+        return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+
+    A = dict(t=np.full(n, ta, np.uint32), p=np.array([prm[ta]() for _ in range(n)], np.float32),
+             x=rng.uniform(-1, 1, (n, 3)).astype(np.float32), q=quats(n))
+    if tb == 3:
+        nrm = rng.normal(size=(n, 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        off = (np.einsum("ij,ij->i", nrm, A["x"]) + rng.uniform(-1.5, 0.3, n))
+        B = dict(t=np.full(n, 3, np.uint32), p=nrm.astype(np.float32),
+                 x=np.stack([off, np.zeros(n), np.zeros(n)], 1).astype(np.float32), q=quats(n))
+    else:
+        B = dict(t=np.full(n, tb, np.uint32), p=np.array([prm[tb]() for _ in range(n)], np.float32),
+                 x=(A["x"] + rng.uniform(-1.6, 1.6, (n, 3))).astype(np.float32), q=quats(n))
+    got = axm.collide_pairs(A["t"], A["p"], A["x"], A["q"], B["t"], B["p"], B["x"], B["q"], margin=0.02)
+    assert got["status"] == 0
+    hits = 0
+    for i in range(n):
+        o = orc.collide(ta, A["p"][i], A["x"][i], A["q"][i], tb, B["p"][i], B["x"][i], B["q"][i], margin=0.02)
+        k = o["npts"]
+        assert got["npts"][i] == k, i
+        if k == 0:
+            continue
+        hits += 1
+        assert np.array_equal(got["feature"][i, :k], o["feature"]), i
+        assert rel_err(got["pos"][i, :k], o["pos"]) <= 1e-5
+        assert rel_err(got["normal"][i, :k], o["normal"]) <= 1e-5
+        assert rel_err(got["depth"][i, :k], o["depth"]) <= 1e-5
+    assert hits > 30
+
+
+@pytest.mark.parametrize("ta,tb", [(1, 1), (1, 2), (2, 1), (1, 0), (2, 2)])
+def test_gjk_epa_pairs_match_oracle(axm, orc, gpu, ta, tb):
+    """GJK distance (separated cores) and EPA depth (intersecting cores) on the GPU vs the oracle."""
+    rng = np.random.default_rng(300 + 10 * ta + tb)
+    n = 256
+    prm = {0: lambda: [rng.uniform(0.25, 0.75), 0, 0], 1: lambda: list(rng.uniform(0.25, 0.75, 3)),
+           2: lambda: [rng.uniform(0.2, 0.5), rng.uniform(0.2, 1.0), 0]}
+    q = rng.normal(size=(2, n, 4))
+    q = (q / np.linalg.norm(q, axis=2, keepdims=True)).astype(np.float32)
+    pa = np.array([prm[ta]() for _ in range(n)], np.float32)
+    pb = np.array([prm[tb]() for _ in range(n)], np.float32)
+    xa = np.zeros((n, 3), np.float32)
+    xb = rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+    out, st = axm.gjk_epa_pairs(np.full(n, ta, np.uint32), pa, xa, q[0], np.full(n, tb, np.uint32), pb, xb, q[1])
+    kinds = set()
+    for i in range(n):
+        o, s = orc.gjk_epa(ta, pa[i], xa[i], q[0, i], tb, pb[i], xb[i], q[1, i])
+        assert st[i] == s, i
+        kinds.add(s)
+        assert abs(out[i, 0] - o[0]) <= 1e-5 * max(1.0, abs(o[0])), (i, out[i, 0], o[0])
+        if s == 1:
+            assert rel_err(out[i, 7:10], o[7:10]) <= 1e-4
+    assert 0 in kinds and 1 in kinds
