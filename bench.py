@@ -56,6 +56,22 @@ def cpu_baseline(desc, bodies, iters, dt, budget_s):
                        "%d bodies x %d steps (%.1f s)" % (bodies.n, total_steps, t))
 
 
+def measured_traffic(cfg, n, iters, args):
+    """HBM bytes per sweep-kernel launch from the committed counter runs (rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE, scaled by a plain copy of known size), when they cover this exact workload; else None."""
+    path = os.path.join(ROOT, "profiles", "k_pgs_flow_pile100k_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    w = t["workload"]
+    if (w["scene_config"], w["bodies"], w["sweeps"], w["warmup"], w["steps"]) != (cfg, n, iters, args.warmup,
+                                                                                   args.steps):
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
 def copy_calibration(mib, reps):
     """Plain device-to-device copies of a known size (read `mib` MiB, write `mib` MiB per
     dispatch), for scaling the HBM counters FETCH_SIZE / WRITE_SIZE of a counter run.
@@ -133,8 +149,8 @@ def main():
     achieved = (prof.pgs_bytes / (prof.pgs_ms * 1e-3) / 1e9) if prof.pgs_ms > 0 else 0.0
     roof = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-        "kernel": "k_pgs",
+        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(cfg, bodies.n, iters, args),
+        "kernel": "k_pgs_flow",
         "launches": int(prof.pgs_launches),
         "avg_launch_us": round(prof.pgs_ms * 1e3 / max(prof.pgs_launches, 1), 3),
         "bytes_per_launch": round(prof.pgs_bytes / max(prof.pgs_launches, 1), 1),

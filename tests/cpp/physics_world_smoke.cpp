@@ -1,0 +1,78 @@
+// Compile-and-run check of the C++20 API (include/axiom/dynamics/physics_world.hpp).
+// Without a GPU it checks the host-side pieces and that world creation fails
+// with a status (never a CPU fallback); with a GPU it steps a sphere resting on
+// the ground and checks the normal impulse carries the weight.
+#include <cmath>
+#include <cstdio>
+
+#include "axiom/dynamics/physics_world.hpp"
+
+namespace {
+// Field names of gui::RigidBodyData (include/axiom/gui/body_inspector.hpp:58-90).
+struct V3 { float x, y, z; };
+struct Q4 { float x, y, z, w; };
+struct RigidBodyLike {
+    V3 position{0, 0, 0};
+    Q4 rotation{0, 0, 0, 1};
+    V3 linearVelocity{0, 0, 0};
+    V3 angularVelocity{0, 0, 0};
+    float mass = 1.0f;
+    V3 inertiaTensor{1, 1, 1};
+    int shapeType = 0;
+    V3 shapeExtents{1, 1, 1};
+};
+struct StatsLike {
+    uint32_t totalBodies, activeBodies, sleepingBodies, staticBodies, dynamicBodies, kinematicBodies;
+    uint32_t contactPointCount, constraintCount, islandCount;
+    float totalStepTime, broadphaseTime, narrowphaseTime, solverTime, integrationTime;
+};
+int fails = 0;
+void check(bool c, const char* what) {
+    if (!c) { std::printf("FAIL %s\n", what); ++fails; }
+}
+}  // namespace
+
+int main() {
+    using namespace axiom::dynamics;
+    BodySet bodies;
+    const float r[3] = {0.5f, 0, 0}, at[3] = {0, 0.5f, 0}, q[4] = {0, 0, 0, 1}, zero[3] = {0, 0, 0};
+    check(bodies.add(ShapeType::Sphere, r, at, q, zero, zero).ok(), "add sphere");
+    const float bad[3] = {-1, 0, 0};
+    check(bodies.add(ShapeType::Sphere, bad, at, q, zero, zero).code() == AXM_INVALID_SHAPE, "invalid shape");
+    RigidBodyLike rb;
+    rb.position = {3, 1, 0};
+    rb.mass = 0.0f;  // static
+    rb.shapeType = 1;
+    rb.shapeExtents = {1, 1, 1};
+    check(bodies.addRigidBody(rb).ok(), "addRigidBody");
+    check(bodies.size() == 2 && bodies.inverseMass(1) == 0.0f, "static body");
+    const float m = 1000.0f * 4.0f / 3.0f * 3.14159265f * 0.125f;
+    check(std::fabs(1.0f / bodies.inverseMass(0) - m) < 1e-3f * m, "sphere mass");
+
+    Status s;
+    axm_world_desc d = PhysicsWorld::defaultDesc();
+    auto world = PhysicsWorld::create(d, &s);
+    if (axm_device_count() == 0) {
+        check(!world && (s.code() == AXM_GPU_OPERATION_FAILED || s.code() == AXM_GPU_INVALID_OPERATION),
+              "no device: creation fails with a GPU status");
+        std::printf("%s (no device: %s)\n", fails ? "FAILED" : "OK", s.message());
+        return fails ? 1 : 0;
+    }
+    check(world != nullptr, "create");
+    check(world->setBodies(bodies).ok(), "setBodies");
+    for (int k = 0; k < 60 && world; ++k) check(world->step(1.0f / 120.0f, 10).ok(), "step");
+    check(world->run(1.0f / 120.0f, 10, 30, true).ok(), "run graph");
+    BodySet out;
+    check(world->getBodies(out).ok() && out.size() == 2, "getBodies");
+    check(std::fabs(out.linearVelocity(0)[1]) < 1e-3f, "sphere at rest");
+    check(std::fabs(out.position(0)[1] - 0.5f) < 0.01f, "sphere on the ground");
+    PhysicsWorld::Manifolds mf;
+    check(world->contacts(mf).ok() && mf.a.size() == 1 && mf.npts[0] == 1, "one contact");
+    StatsLike st{};
+    world->fillStats(st, 2, 1);
+    check(st.contactPointCount == 1 && st.dynamicBodies == 1, "stats");
+    PhysicsWorld moved(std::move(*world));
+    check(moved.handle() != nullptr && world->handle() == nullptr, "move");
+    std::printf("%s\n", fails ? "FAILED" : "OK");
+    return fails ? 1 : 0;
+}

@@ -285,3 +285,90 @@ def test_epa_sphere_centre_inside_box(orc):
         out, st = orc.gjk_epa(BOX, h, [0, 0, 0], q, SPHERE, [0.3, 0, 0], x, I4)
         assert st == 1
         assert abs(out[0] - np.min(h - np.abs(local))) <= 1e-5
+
+
+def test_sphere_capsule_point_segment_distance(orc):
+    rng = np.random.default_rng(31)
+    n = 0
+    for _ in range(300):
+        r = float(rng.uniform(0.25, 0.75))
+        rc, hc = float(rng.uniform(0.2, 0.5)), float(rng.uniform(0.2, 1.0))
+        qc = rand_quat(rng)
+        xs = rng.uniform(-1.5, 1.5, 3).astype(np.float32)
+        p0, p1 = capsule_ends([0, 0, 0], qc, hc)
+        t = np.clip((xs - p0) @ (p1 - p0) / ((p1 - p0) @ (p1 - p0)), 0, 1)
+        dist = np.linalg.norm(xs - (p0 + t * (p1 - p0)))
+        c = orc.collide(SPHERE, [r, 0, 0], xs, I4, CAPSULE, [rc, hc, 0], [0, 0, 0], qc)
+        if r + rc - dist < -0.02 - 1e-5:
+            assert c["npts"] == 0
+            continue
+        if dist < 1e-3:
+            continue
+        n += 1
+        assert c["npts"] == 1
+        assert abs(c["depth"][0] - (r + rc - dist)) <= 1e-5 * max(1.0, r + rc)
+    assert n > 50
+
+
+def test_capsule_box_depth_brute_force(orc):
+    """Shallow capsule-box contacts.  One point: depth = r - distance(segment, box), minimised over
+    the segment by bounded search (point-box distance is a clamp in the box frame).  A capsule lying
+    along a face (|axis . n| < 0.2) gets one point per segment end instead, each with depth
+    r - distance(end, box)."""
+    rng = np.random.default_rng(37)
+    n = 0
+    for _ in range(300):
+        rc, hc = float(rng.uniform(0.2, 0.5)), float(rng.uniform(0.2, 1.0))
+        h = rng.uniform(0.25, 0.75, 3)
+        qc, qb = rand_quat(rng), rand_quat(rng)
+        xc = rng.uniform(-1.8, 1.8, 3).astype(np.float32)
+        Rb = rot(qb)
+        p0, p1 = capsule_ends(xc, qc, hc)
+
+        def d(t):
+            p = Rb.T @ (p0 + t * (p1 - p0))
+            return np.linalg.norm(p - np.clip(p, -h, h))
+
+        ts = np.linspace(0, 1, 201)
+        t0 = ts[np.argmin([d(t) for t in ts])]
+        res = optimize.minimize_scalar(d, bounds=(max(0, t0 - 0.01), min(1, t0 + 0.01)), method="bounded",
+                                       options={"xatol": 1e-10})
+        dist = min(res.fun, d(t0))
+        if dist < 0.05:  # deep or touching cores: EPA territory, checked elsewhere
+            continue
+        c = orc.collide(CAPSULE, [rc, hc, 0], xc, qc, BOX, h, [0, 0, 0], qb)
+        if rc - dist < -0.02 - 1e-4:
+            assert c["npts"] == 0
+            continue
+        if c["npts"] == 0:
+            assert rc - dist < -0.02 + 1e-4
+            continue
+        n += 1
+        if c["npts"] == 1:
+            assert abs(c["depth"][0] - (rc - dist)) <= 2e-5, (c["depth"], rc - dist)
+        else:
+            assert c["npts"] == 2
+            ends = sorted(rc - d(t) for t in (0.0, 1.0))
+            assert np.allclose(sorted(c["depth"]), ends, atol=2e-5)
+            assert c["depth"].max() <= rc - dist + 2e-5
+    assert n > 30
+
+
+def test_epa_box_segment_core_matches_sat(orc):
+    """A capsule's core is its segment, a box with half extents (0, h, 0) along the capsule axis:
+    the EPA depth of box vs segment equals the SAT overlap of the two boxes."""
+    rng = np.random.default_rng(41)
+    n = 0
+    for _ in range(200):
+        h = rng.uniform(0.3, 0.75, 3)
+        hc = float(rng.uniform(0.2, 1.0))
+        qa, qc = rand_quat(rng), rand_quat(rng)
+        xc = rng.uniform(-0.6, 0.6, 3)
+        depth, axis = sat_depth(h, np.zeros(3), qa, np.array([0.0, hc, 0.0]), xc, qc)
+        if depth < 0.01:
+            continue
+        out, st = orc.gjk_epa(BOX, h, [0, 0, 0], qa, CAPSULE, [0.3, hc, 0], xc, qc)
+        assert st == 1
+        n += 1
+        assert abs(out[0] - depth) <= 1e-5 * max(1.0, depth), (out[0], depth)
+    assert n > 50

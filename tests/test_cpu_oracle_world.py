@@ -272,3 +272,53 @@ def test_colouring_is_valid(orc, axm, config, n, steps):
         ow2.step(DT, 10)
     col2, order2, K2 = ow2.colours()
     assert K2 == K and np.array_equal(col, col2) and np.array_equal(order, order2)
+
+
+def test_single_contact_residual_vanishes_after_one_sweep(orc, axm):
+    """One sphere on the ground is one normal row (plus friction at rest): one Gauss-Seidel update
+    solves it exactly, so the fixed-point residual after the first sweep is ~0; a pile's is not."""
+    b = make(axm, [(0, [0.5], [0, 0.49, 0], [0.0, -1.0, 0], [0, 0, 0], False)])
+    ow, d = world(orc, axm, b, record_residual=1)
+    ow.step(DT, 3)
+    r = ow.residuals()
+    assert len(r) == 3 and r[0] < 1e-5
+    b2, d2 = axm.scene(1, seed(1), 1000)
+    d2.record_residual = 1
+    ow2 = orc.OracleWorld(orc.params_from_desc(d2))
+    ow2.set_bodies(b2)
+    for _ in range(30):
+        ow2.step(DT, 5)
+    assert ow2.residuals()[0] > 1e-3
+
+
+def test_warm_start_matches_features_across_frames(orc, axm):
+    """A box resting on the ground keeps its four corner features: every point of the second frame
+    finds its previous impulse (warm hits = 4) and starts from it."""
+    b = make(axm, [(1, [0.5, 0.25, 0.5], [0, 0.245, 0], [0, 0, 0], [0, 0, 0], False)])
+    ow, _ = world(orc, axm, b)
+    st1 = ow.step(DT, 10)
+    lam1 = ow.impulses()
+    st2 = ow.step(DT, 10)
+    assert st1.warm_hits == 0 and st2.warm_hits == 4
+    assert lam1[0, :4, 0].min() > 0
+
+
+def test_restitution_closed_form(orc, axm):
+    """A sphere hitting the ground at v > rest_threshold leaves at e * v (single frictionless point)."""
+    e, v = 0.4, -3.0
+    b = make(axm, [(0, [0.5], [0, 0.5, 0], [0, v, 0], [0, 0, 0], False)])
+    ow, d = world(orc, axm, b, restitution=e, friction=0.0, gravity=[0, 0, 0])
+    ow.step(DT, 10)
+    _, _, vel, _ = ow.get_state()
+    assert abs(vel[0, 1] - (-e * v)) < 1e-5
+
+
+def test_baumgarte_bias_closed_form(orc, axm):
+    """A sphere resting at depth d > slop is pushed out at beta / dt * (d - slop), capped at max_bias."""
+    for depth in (0.02, 0.5):
+        b = make(axm, [(0, [0.5], [0, 0.5 - depth, 0], [0, 0, 0], [0, 0, 0], False)])
+        ow, d = world(orc, axm, b, gravity=[0, 0, 0], friction=0.0)
+        ow.step(DT, 10)
+        _, _, vel, _ = ow.get_state()
+        want = min(d.baumgarte / np.float32(DT) * (depth - d.slop), d.max_bias)
+        assert abs(vel[0, 1] - want) <= 1e-4 * max(1.0, want), (depth, vel[0, 1], want)

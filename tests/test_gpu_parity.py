@@ -304,4 +304,6 @@ def test_gjk_epa_pairs_match_oracle(axm, orc, gpu, ta, tb):
         assert abs(out[i, 0] - o[0]) <= 1e-5 * max(1.0, abs(o[0])), (i, out[i, 0], o[0])
         if s == 1:
             assert rel_err(out[i, 7:10], o[7:10]) <= 1e-4
-    assert 0 in kinds and 1 in kinds
+    assert 0 in kinds
+    if (ta, tb) != (2, 2):  # two segment cores essentially never intersect
+        assert 1 in kinds
