@@ -88,7 +88,11 @@ EXPORTS = [
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
+    "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
+    "axm_mlp_forward", "axm_mlp_loss_grad", "axm_mlp_adam_step", "axm_mlp_train", "axm_world_set_mlp",
+    "axm_world_set_feature_logging", "axm_world_get_features",
 ]
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 24, 3, 20099
 
 _lib = None
 
@@ -131,6 +135,19 @@ def lib():
     L.axm_scene_generate.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(BodySoA), u32p,
                                      C.POINTER(WorldDesc)]
     L.axm_mass_properties.argtypes = [C.c_uint32, f32p, C.c_float, f32p, f32p]
+    f64p = C.POINTER(C.c_double)
+    L.axm_mlp_create.argtypes = [C.c_int32, C.POINTER(P)]
+    L.axm_mlp_destroy.argtypes = [P]
+    L.axm_mlp_init.argtypes = [P, C.c_uint64]
+    L.axm_mlp_set_params.argtypes = [P, f32p]
+    L.axm_mlp_get_params.argtypes = [P, f32p]
+    L.axm_mlp_forward.argtypes = [P, f32p, C.c_uint32, f32p]
+    L.axm_mlp_loss_grad.argtypes = [P, f32p, f32p, C.c_uint32, f32p, f64p]
+    L.axm_mlp_adam_step.argtypes = [P, f32p, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.axm_mlp_train.argtypes = [P, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, f32p]
+    L.axm_world_set_mlp.argtypes = [P, P]
+    L.axm_world_set_feature_logging.argtypes = [P, C.c_int]
+    L.axm_world_get_features.argtypes = [P, f32p, f32p, C.c_uint32, u32p]
     L.axm_rng_stream.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_float, C.c_float, f32p]
     for name in EXPORTS:
         f = getattr(L, name)
@@ -224,6 +241,70 @@ def mass_properties(shape_type, params, density=1000.0):
     return im.value, ii
 
 
+class Mlp:
+    """Learned warm-start network on one GPU (axm_mlp_*)."""
+
+    def __init__(self, device=0, seed=None):
+        self._L = lib()
+        h = C.c_void_p()
+        _check(self._L.axm_mlp_create(device, C.byref(h)), "axm_mlp_create")
+        self._h = h
+        if seed is not None:
+            self.init(seed)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.axm_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed):
+        _check(self._L.axm_mlp_init(self._h, seed), "axm_mlp_init")
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        _check(self._L.axm_mlp_get_params(self._h, _p(p)), "axm_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        _check(self._L.axm_mlp_set_params(self._h, _p(p)), "axm_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], MLP_OUTPUTS), np.float32)
+        _check(self._L.axm_mlp_forward(self._h, _p(X), X.shape[0], _p(Y)), "axm_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(self._L.axm_mlp_loss_grad(self._h, _p(X), _p(Y), X.shape[0], _p(g), C.byref(loss)), "axm_mlp_loss_grad")
+        return loss.value, g
+
+    def adam_step(self, grad=None, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        g = None if grad is None else np.ascontiguousarray(grad, np.float32)
+        _check(self._L.axm_mlp_adam_step(self._h, _p(g), lr, beta1, beta2, eps), "axm_mlp_adam_step")
+
+    def train(self, X, Y, epochs=1, batch=4096, lr=1e-3):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        _check(self._L.axm_mlp_train(self._h, _p(X), _p(Y), X.shape[0], epochs, batch, lr, _p(losses)),
+               "axm_mlp_train")
+        return losses
+
+
 def rng_stream(seed, n, lo=0.0, hi=1.0):
     u = np.zeros(n, np.uint32)
     f = np.zeros(n, np.float32)
@@ -291,6 +372,27 @@ class World:
         p = KernelProfile()
         _check(self._L.axm_world_profile(self._h, 1 if enable else 0, C.byref(p)), "axm_world_profile")
         return p
+
+    def set_mlp(self, mlp):
+        """Attaches a warm-start network (used when desc.warm_start == WARM_MLP); None detaches."""
+        self._mlp = mlp
+        _check(self._L.axm_world_set_mlp(self._h, mlp._h if mlp is not None else None), "axm_world_set_mlp")
+
+    def set_feature_logging(self, enable=True):
+        _check(self._L.axm_world_set_feature_logging(self._h, 1 if enable else 0), "axm_world_set_feature_logging")
+
+    def features(self):
+        """(X [M,4,24], Y [M,4,3]) network inputs and targets ((final - previous impulse) / u) of the
+        last step, manifold order."""
+        cnt = C.c_uint32()
+        code = self._L.axm_world_get_features(self._h, None, None, 0, C.byref(cnt))
+        if code not in (0, 601):
+            _check(code, "axm_world_get_features")
+        M = cnt.value
+        X = np.zeros((M, 4, MLP_INPUTS), np.float32)
+        Y = np.zeros((M, 4, MLP_OUTPUTS), np.float32)
+        _check(self._L.axm_world_get_features(self._h, _p(X), _p(Y), M, C.byref(cnt)), "axm_world_get_features")
+        return X, Y
 
     def broadphase(self):
         c = C.c_uint32()

@@ -19,6 +19,7 @@
 // solve-order arrays, so a wave of 64 manifolds of one colour reads 64
 // consecutive 16-byte rows.
 #include "dev_geom.hpp"
+#include "k_mlp.hpp"
 #include "k_step.hpp"
 
 namespace axm {
@@ -536,7 +537,7 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
                                               const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
-                                              Counters* __restrict__ cnt) {
+                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
     if (s >= M || cnt->sticky != 0u) return;  // a failed step must not touch the impulse buffers
@@ -592,7 +593,8 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         const float mN = kk[0] > 0.0f ? 1.0f / kk[0] : 0.0f;
         const float mT1 = kk[1] > 0.0f ? 1.0f / kk[1] : 0.0f;
         const float mT2 = kk[2] > 0.0f ? 1.0f / kk[2] : 0.0f;
-        const float vn0 = dot(vrel(A, Bd, rA, rB), n);
+        const f3 vr = vrel(A, Bd, rA, rB);
+        const float vn0 = dot(vr, n);
         float bias = 0.0f;
         if (depth > sp.slop) {
             bias = sp.baumgarte * (depth - sp.slop) * sp.inv_dt;
@@ -610,23 +612,39 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         R.r2[o] = f4(n, mT1);
         R.r3[o] = make_float4(mT2, kk[0], kk[1], kk[2]);
         R.fid[o] = fid;
-        float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (warm_in) {
-            L = warm_in[(size_t)m * 4 + k];
-        } else if (have_prev) {
+        // previous step's impulse of the same feature (the cached warm start)
+        float4 Lp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool hit = false;
+        if (have_prev) {
             const uint32_t pnp = pc.np[ps];
             for (uint32_t j = 0; j < pnp; ++j) {
                 const size_t po = (size_t)j * pc.cap + ps;
                 if (pc.fid[po] == fid) {
-                    L = pc.lam[po];
+                    Lp = pc.lam[po];
+                    hit = true;
                     ++hits;
                     break;
                 }
             }
         }
+        float4 L = warm_in ? warm_in[(size_t)m * 4 + k] : Lp;
         L.w = 0.0f;
         R.lam[o] = L;
         R.lam0[o] = L;
+        if (feat) {
+            // warm-start network inputs, contact frame, impulses in units of u = mN g dt
+            const float gdt = MLP_G_REF * sp.dt;
+            const float inv_gdt = 1.0f / gdt;
+            const float u = mN * gdt;
+            const float inv_u = u > 0.0f ? 1.0f / u : 0.0f;
+            float4* fr = (float4*)(feat + ((size_t)4 * s + k) * MLP_IN);
+            fr[0] = make_float4(n.x, n.y, n.z, depth * 20.0f);
+            fr[1] = make_float4(vn0 * inv_gdt, dot(vr, t1) * inv_gdt, dot(vr, t2) * inv_gdt, dot(rA, n));
+            fr[2] = make_float4(dot(rA, t1), dot(rA, t2), dot(rB, n), dot(rB, t1));
+            fr[3] = make_float4(dot(rB, t2), A.im * mN, Bd.im * mN, mT1 * kk[0]);
+            fr[4] = make_float4(mT2 * kk[0], bias * inv_gdt, Lp.x * inv_u, Lp.y * inv_u);
+            fr[5] = make_float4(Lp.z * inv_u, hit ? 1.0f : 0.0f, (float)np * 0.25f, plane ? 1.0f : 0.0f);
+        }
     }
     if (hits) atomicAdd(&cnt->warm_hits, hits);
     // algorithmic bytes one k_pgs visit of this manifold moves: header 16,
@@ -1043,7 +1061,7 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, hipStream_t s) {
+                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
     PrevCache pc;
     pc.key = pca.key;
     pc.slot = pca.slot;
@@ -1054,7 +1072,38 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
     pc.cap = pca.cap;
     pc.valid = pca.valid;
     hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, col, bmask, bdeg, cnt);
+                       warm_in, col, bmask, bdeg, cnt, feat);
+}
+
+// Learned warm start: the network predicts the correction to the previous
+// step's impulse of the same feature (already in the rows, zero for a new
+// point) in units of u = mN g dt; lambda = previous + output * u, projected onto
+// lambda_n >= 0 and the friction pyramid |lambda_t| <= mu lambda_n.
+__global__ void __launch_bounds__(256) k_mlp_apply(SolverRows R, const Counters* __restrict__ cnt, uint32_t cap,
+                                                   const float* __restrict__ Y, StepParams sp) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    const uint32_t np = R.hdr[s].z;
+    const float gdt = MLP_G_REF * sp.dt;
+    for (uint32_t k = 0; k < np; ++k) {
+        const size_t o = (size_t)k * cap + s;
+        const float u = R.r0[o].w * gdt;
+        const float* y = Y + ((size_t)4 * s + k) * MLP_OUT;
+        const float4 Lp = R.lam[o];
+        float ln = Lp.x + y[0] * u;
+        ln = ln > 0.0f ? ln : 0.0f;
+        const float maxf = sp.friction * ln;
+        float t1 = Lp.y + y[1] * u, t2 = Lp.z + y[2] * u;
+        t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
+        t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
+        const float4 L = make_float4(ln, t1, t2, 0.0f);
+        R.lam[o] = L;
+        R.lam0[o] = L;
+    }
+}
+void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, const float* Y, const StepParams& sp,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_apply, grid_for(cap), dim3(256), 0, s, R, cnt, cap, Y, sp);
 }
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
                      uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,

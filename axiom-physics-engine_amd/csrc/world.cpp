@@ -24,6 +24,7 @@
 
 #include "axiom/axm.h"
 #include "k_step.hpp"
+#include "mlp_impl.hpp"
 
 using namespace axm;
 
@@ -107,6 +108,14 @@ struct axm_world_s {
     uint2* m_ab = nullptr;
     uint32_t* m_np = nullptr;
     float4 *c_pt = nullptr, *c_nf = nullptr;
+
+    // learned warm start: per-point network inputs (rows 4 * slot + point, 24 floats)
+    // and outputs (3 floats), kept while logging or while a network is attached
+    bool want_feat = false;
+    float* feat = nullptr;
+    float* ymlp = nullptr;
+    axm_mlp_s* mlp = nullptr;
+    float last_dt = 0.0f;
 
     uint32_t* col = nullptr;
     uint64_t *prio = nullptr, *bmax0 = nullptr, *bmax1 = nullptr, *bmask = nullptr;
@@ -342,6 +351,16 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     return AXM_SUCCESS;
 }
 
+// Per-point network input / output buffers sized for the manifold capacity.
+axm_status alloc_feat(axm_world_s* w) {
+    w->mem.release(w->feat);
+    w->mem.release(w->ymlp);
+    if (!w->want_feat || !w->cap_man) return AXM_SUCCESS;
+    AXM_TRY(w->mem.get(w->feat, (size_t)w->cap_man * 4 * MLP_IN));
+    AXM_TRY(w->mem.get(w->ymlp, (size_t)w->cap_man * 4 * MLP_OUT));
+    return AXM_SUCCESS;
+}
+
 // Grows the manifold buffers.  The current row set is reallocated; the other
 // set (the previous step's impulses, i.e. the next warm start) is grown in place
 // so both sets always hold at least `cap` slots.
@@ -371,7 +390,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     w->cap_man = cap;
     if (w->rcap[w->cur] < cap) AXM_TRY(alloc_rows(w, w->cur, cap));
     if (w->rcap[1 - w->cur] > 0 && w->rcap[1 - w->cur] < cap) AXM_TRY(grow_rows_keep(w, 1 - w->cur, cap));
-    return AXM_SUCCESS;
+    return alloc_feat(w);
 }
 
 StepParams make_params(const axm_world_s* w, float dt) {
@@ -503,7 +522,12 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     pca.cap = w->rcap[pv];
     if (!pca.key) pca.valid = 0;
     launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
-                w->bdone, w->cnt, s);
+                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, s);
+    if (w->mlp && w->desc.warm_start == AXM_WARM_MLP && !o.warm_in && w->feat) {
+        launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
+                           2u * (uint32_t)w->cu_count, s);
+        launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
+    }
     launch_fill_u32(w->res_bits, 0u, w->res_cap, s);
     return AXM_SUCCESS;
 }
@@ -564,6 +588,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     }
     w->last_iters = it;
     if (!post) return AXM_SUCCESS;
+    w->last_dt = dt;
     launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
     launch_integrate(w->B, sp, n, w->cnt, it, s);
@@ -1213,6 +1238,76 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
             const size_t o = ((size_t)m * 4 + k) * 3;
             if (final_) { final_[o] = on ? l[s].x : 0.f; final_[o + 1] = on ? l[s].y : 0.f; final_[o + 2] = on ? l[s].z : 0.f; }
             if (initial) { initial[o] = on ? l0[s].x : 0.f; initial[o + 1] = on ? l0[s].y : 0.f; initial[o + 2] = on ? l0[s].z : 0.f; }
+        }
+    }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_set_mlp(axm_world w, axm_mlp m) {
+    if (!w) return AXM_NULL_POINTER;
+    if (m && m->device != w->desc.device) return AXM_INVALID_PARAMETER;
+    (void)hipSetDevice(w->desc.device);
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    w->mlp = m;
+    const bool want = w->want_feat || m != nullptr;
+    if (want != w->want_feat || (want && !w->feat)) {
+        w->want_feat = want;
+        AXM_TRY(alloc_feat(w));
+    }
+    w->drop_graphs();
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_set_feature_logging(axm_world w, int enable) {
+    if (!w) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    const bool want = enable != 0 || w->mlp != nullptr;
+    if (want != w->want_feat || (want && !w->feat)) {
+        w->want_feat = want;
+        AXM_TRY(alloc_feat(w));
+        w->drop_graphs();
+    }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_get_features(axm_world w, float* X, float* Y, uint32_t cap, uint32_t* count) {
+    if (!w || !count) return AXM_NULL_POINTER;
+    if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
+    if (!w->want_feat || !w->feat) return AXM_GPU_INVALID_OPERATION;
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t M = w->h_cnt->m_eff;
+    *count = M;
+    if (cap < M) return AXM_OUT_OF_RANGE;
+    if (!M) return AXM_SUCCESS;
+    const int set = 1 - w->cur;  // rows of the last step
+    const SolverRows& R = w->R[set];
+    const uint32_t rc = w->rcap[set];
+    std::vector<uint4> hdr(M);
+    AXM_HIP(hipMemcpy(hdr.data(), R.hdr, 16ull * M, hipMemcpyDeviceToHost));
+    std::vector<float> f((size_t)M * 4 * MLP_IN);
+    AXM_HIP(hipMemcpy(f.data(), w->feat, sizeof(float) * f.size(), hipMemcpyDeviceToHost));
+    std::vector<float4> l(M), r0(M);
+    const float gdt = MLP_G_REF * w->last_dt;
+    for (uint32_t k = 0; k < 4; ++k) {
+        AXM_HIP(hipMemcpy(l.data(), R.lam + (size_t)k * rc, 16ull * M, hipMemcpyDeviceToHost));
+        AXM_HIP(hipMemcpy(r0.data(), R.r0 + (size_t)k * rc, 16ull * M, hipMemcpyDeviceToHost));
+        for (uint32_t s = 0; s < M; ++s) {
+            const uint32_t m = hdr[s].w;
+            const bool on = k < hdr[s].z;
+            const size_t row = (size_t)m * 4 + k;
+            if (X)
+                for (int c = 0; c < MLP_IN; ++c) X[row * MLP_IN + c] = on ? f[((size_t)4 * s + k) * MLP_IN + c] : 0.0f;
+            if (Y) {
+                // target: (final - previous impulse) / u; the previous impulse / u is input 18..20
+                const float u = r0[s].w * gdt;
+                const float iu = (on && u > 0.0f) ? 1.0f / u : 0.0f;
+                const float* fp = &f[((size_t)4 * s + k) * MLP_IN + 18];
+                Y[row * 3] = on ? l[s].x * iu - fp[0] : 0.0f;
+                Y[row * 3 + 1] = on ? l[s].y * iu - fp[1] : 0.0f;
+                Y[row * 3 + 2] = on ? l[s].z * iu - fp[2] : 0.0f;
+            }
         }
     }
     return AXM_SUCCESS;

@@ -1,0 +1,100 @@
+"""Learned warm start: training-data logging from the plain solver, training of the
+network on the GPU (axm_mlp_train) and the iterations-to-tolerance comparison.
+
+The network predicts each contact point's change of impulse from the previous
+step's (in units of mN * 9.81 * dt) from 24 per-point inputs computed by the
+step's preparation kernel; see include/axiom/axm.h (axm_mlp_*) for the inputs
+and the packing of the weights.
+"""
+import numpy as np
+
+import axm
+
+
+def collect(bodies, desc, steps, sweeps, dt=1.0 / 120.0, every=1, device_graph=False):
+    """Runs the plain solver (previous-step warm start) with `sweeps` sweeps per step and returns
+    (X [n, 24], Y [n, 3]): the inputs of every contact point and its change of impulse / u."""
+    d = axm.default_desc()
+    for f, _ in axm.WorldDesc._fields_:
+        setattr(d, f, getattr(desc, f))
+    d.warm_start = axm.WARM_CACHE
+    d.residual_tol = 0.0
+    w = axm.World(d)
+    w.set_bodies(bodies)
+    w.set_feature_logging(True)
+    xs, ys = [], []
+    for k in range(steps):
+        w.step(dt, sweeps)
+        if k % every:
+            continue
+        X, Y = w.features()
+        m = w.manifolds()
+        valid = np.arange(4)[None, :] < m["npts"][:, None]
+        xs.append(X[valid])
+        ys.append(Y[valid])
+    w.close()
+    if not xs:
+        return np.zeros((0, axm.MLP_INPUTS), np.float32), np.zeros((0, axm.MLP_OUTPUTS), np.float32)
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def fold_normalisation(params, mu, sigma, yscale):
+    """Network trained on (X - mu) / sigma with targets Y / yscale -> the same network on raw X and Y:
+    W1 <- W1 / sigma, b1 <- b1 - W1 (mu / sigma), W3 <- yscale W3, b3 <- yscale b3."""
+    p = np.array(params, np.float64)
+    W1 = p[0:3072].reshape(128, 24)
+    b1 = p[3072:3200]
+    W3 = p[19712:20096].reshape(3, 128)
+    b3 = p[20096:20099]
+    b1[:] = b1 - W1 @ (mu / sigma)
+    W1[:] = W1 / sigma[None, :]
+    W3[:] = W3 * yscale[:, None]
+    b3[:] = b3 * yscale
+    return p.astype(np.float32)
+
+
+def train(X, Y, epochs=5, batch=4096, lr=1e-3, seed=1, device=0, shuffle_seed=7, clip=None):
+    """Trains a fresh network on (X, Y) with minibatch Adam on the GPU and returns (Mlp, per-epoch loss).
+    Inputs are standardised and targets scaled per output for training; the scalings are folded into
+    the first and last layers afterwards, so the returned network takes raw inputs.  `clip` bounds
+    |Y| (in units of u) to keep rare impact impulses from dominating the squared loss."""
+    X = np.asarray(X, np.float64)
+    Y = np.asarray(Y, np.float64)
+    if clip is not None:
+        Y = np.clip(Y, -clip, clip)
+    mu = X.mean(0)
+    sigma = np.maximum(X.std(0), 1e-3)
+    ys = np.maximum(Y.std(0), 1e-3)
+    Xn = ((X - mu) / sigma).astype(np.float32)
+    Yn = (Y / ys).astype(np.float32)
+    rng = np.random.default_rng(shuffle_seed)
+    perm = rng.permutation(len(Xn))
+    net = axm.Mlp(device, seed=seed)
+    losses = net.train(Xn[perm], Yn[perm], epochs=epochs, batch=batch, lr=lr)
+    net.params = fold_normalisation(net.params, mu, sigma, ys)
+    return net, losses
+
+
+def iterations_to_tolerance(bodies, desc, steps, tol, max_iters, warm, net=None, warmup=0, dt=1.0 / 120.0):
+    """Steps a world with the residual-stopped solver (residual checked after every sweep) and returns
+    the sweeps each of the `steps` counted steps needed (capped at max_iters) and whether it reached
+    `tol`; `warmup` earlier steps run the same way uncounted.  `warm` is axm.WARM_CACHE or axm.WARM_MLP."""
+    d = axm.default_desc()
+    for f, _ in axm.WorldDesc._fields_:
+        setattr(d, f, getattr(desc, f))
+    d.warm_start = warm
+    d.residual_tol = tol
+    d.residual_every = 1
+    d.max_iterations = max_iters
+    w = axm.World(d)
+    w.set_bodies(bodies)
+    if net is not None:
+        w.set_mlp(net)
+    iters, reached = [], []
+    for k in range(warmup + steps):
+        st = w.step(dt, max_iters, axm.STEP_RESIDUAL)
+        if k >= warmup:
+            iters.append(st.iterations)
+            reached.append(st.residual <= tol)
+    w.close()
+    return np.array(iters), np.array(reached)

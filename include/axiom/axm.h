@@ -222,6 +222,47 @@ axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a,
                              const float* quat_a, const uint32_t* type_b, const float* prm_b, const float* pos_b,
                              const float* quat_b, float* out, int32_t* status);
 
+/* ---------------------------------------------------------------- learned warm start
+ * A 24 -> 128 -> 128 -> 3 ReLU network (bf16 weights / activations, fp32
+ * accumulation, MFMA kernels) predicting, for each contact point, the change of
+ * its impulses (normal, tangent 1, tangent 2) from the previous step's impulse
+ * of the same feature, in units of u = mN * 9.81 m/s^2 * dt, from 24 per-point
+ * inputs (normal, depth, contact-frame relative velocity and lever arms, mass
+ * ratios, bias, the previous impulse).  With desc.warm_start == AXM_WARM_MLP and
+ * a network attached, the solver starts from previous + prediction (projected
+ * onto lambda_n >= 0 and the friction pyramid) and iterates as usual, so the
+ * converged answer does not depend on it; a zero network is the cached warm start.
+ * The reference has no learned component (no counterpart); these entry points
+ * serve the warm-start path of axm_world_step. */
+typedef struct axm_mlp_s* axm_mlp;
+#define AXM_MLP_INPUTS 24
+#define AXM_MLP_OUTPUTS 3
+#define AXM_MLP_PARAMS 20099 /* W1[128][24], b1[128], W2[128][128], b2[128], W3[3][128], b3[3] */
+axm_status axm_mlp_create(int32_t device, axm_mlp* out);
+axm_status axm_mlp_destroy(axm_mlp m);
+/* He-uniform weights from a seed, zero biases, Adam state cleared. */
+axm_status axm_mlp_init(axm_mlp m, uint64_t seed);
+axm_status axm_mlp_set_params(axm_mlp m, const float* params);
+axm_status axm_mlp_get_params(axm_mlp m, float* params);
+/* Y[n][3] = network(X[n][24]) (host arrays). */
+axm_status axm_mlp_forward(axm_mlp m, const float* X, uint32_t n, float* Y);
+/* Mean-squared-error loss over n rows and its gradient (host arrays; grad may be NULL). */
+axm_status axm_mlp_loss_grad(axm_mlp m, const float* X, const float* Y, uint32_t n, float* grad, double* loss);
+/* One bias-corrected Adam step (Kingma & Ba 2015) with the given gradient (NULL: the last
+ * axm_mlp_loss_grad gradient). */
+axm_status axm_mlp_adam_step(axm_mlp m, const float* grad, float lr, float beta1, float beta2, float eps);
+/* Minibatch Adam (beta 0.9 / 0.999, eps 1e-8) over n rows in the given order, `epochs` passes;
+ * epoch_loss (optional) receives each pass's mean batch loss. */
+axm_status axm_mlp_train(axm_mlp m, const float* X, const float* Y, uint32_t n, uint32_t epochs, uint32_t batch,
+                         float lr, float* epoch_loss);
+/* Attaches a network to a world (same device; NULL detaches).  The caller keeps it alive. */
+axm_status axm_world_set_mlp(axm_world w, axm_mlp m);
+/* Keeps every step's per-point network inputs for axm_world_get_features. */
+axm_status axm_world_set_feature_logging(axm_world w, int enable);
+/* Network inputs X[cap][4][24] and targets Y[cap][4][3] ((final - previous impulse) / u) of
+ * the last step, manifold order (pair order) x point; rows past a manifold's point count are zero. */
+axm_status axm_world_get_features(axm_world w, float* X, float* Y, uint32_t cap, uint32_t* count);
+
 /* Synthetic scenes (host only, seeded DeterministicRNG, include/axiom/math/random.hpp:19-68).
  * config: 0 mixed 1k, 1 dense pile, 2 sparse 1M, 3 batched (n_worlds worlds).
  * n = body count (0 = the config's default).  Call with out->pos == NULL to query

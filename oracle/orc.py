@@ -102,6 +102,8 @@ def lib():
     L.orc_mlp_loss.argtypes = [f32p, f32p, f32p, C.c_int]
     L.orc_adam.argtypes = [f32p, f32p, f32p, f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
     L.orc_bf16_round.argtypes = [f32p, f32p, C.c_int]
+    L.orc_get_features.restype = C.c_uint32
+    L.orc_get_features.argtypes = [P, f32p, C.c_uint32]
     L.orc_quat_mul.argtypes = [f32p, f32p, f32p]
     L.orc_quat_rotate.argtypes = [f32p, f32p, f32p]
     L.orc_world_inv_inertia.argtypes = [f32p, f32p, f32p]
@@ -236,6 +238,13 @@ class OracleWorld:
         lam = np.zeros((M, 4, 3), np.float32)
         self.L.orc_get_lambda(self.h, _p(lam), M)
         return lam
+
+    def features(self):
+        """Warm-start network inputs of the last step, [M, 4, 24] in manifold order."""
+        M = self.L.orc_get_features(self.h, None, 0)
+        X = np.zeros((M, 4, 24), np.float32)
+        self.L.orc_get_features(self.h, _p(X), M)
+        return X
 
     def residuals(self):
         n = self.L.orc_get_residuals(self.h, None, 0)

@@ -335,6 +335,17 @@ void orc_get_lambda(void* h, float* out, uint32_t cap_manifolds) {
     });
 }
 
+// Warm-start network inputs of the last step: manifold order x 4 points x 24.
+uint32_t orc_get_features(void* h, float* out, uint32_t cap_manifolds) {
+    uint32_t M = 0;
+    DISPATCH(h, {
+        M = (uint32_t)W.man.size();
+        if (out && cap_manifolds >= M)
+            for (size_t i = 0; i < (size_t)M * 96 && i < W.feat.size(); ++i) out[i] = (float)W.feat[i];
+    });
+    return M;
+}
+
 uint32_t orc_get_residuals(void* h, float* out, uint32_t cap) {
     uint32_t r = 0;
     DISPATCH(h, {

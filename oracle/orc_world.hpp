@@ -88,6 +88,7 @@ template <class T> struct World {
     };
     std::map<uint64_t, CacheEnt> cache;
     std::vector<T> lambda;  // per manifold x 4 points x 3 (solve output)
+    std::vector<T> feat;    // per manifold x 4 points x 24 warm-start network inputs
     Stats st{};
 
     bool dynamic(uint32_t i) const { return im[i] > T(0); }
@@ -325,6 +326,7 @@ template <class T> struct World {
         ground.dyn = false;
         std::vector<Row> rows((size_t)M * 4);
         lambda.assign((size_t)M * 12, T(0));
+        feat.assign((size_t)M * 4 * 24, T(0));
         const T inv_dt = T(1) / dt;
         const T mu = T(P.friction);
         st.warm_hits = 0;
@@ -334,7 +336,7 @@ template <class T> struct World {
             const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
             const uint64_t ck = ((uint64_t)a << 32) | b;
             const CacheEnt* ce = nullptr;
-            if (P.warm_start == 1) {
+            if (P.warm_start != 0) {
                 auto it = cache.find(ck);
                 if (it != cache.end()) ce = &it->second;
             }
@@ -371,18 +373,37 @@ template <class T> struct World {
                     if (rb > bias) bias = rb;
                 }
                 r.target = bias;
-                T* L = &lambda[((size_t)m * 4 + k) * 3];
-                if (warm_in) {
-                    for (int d = 0; d < 3; ++d) L[d] = warm_in[((size_t)m * 4 + k) * 3 + d];
-                } else if (ce) {
+                // previous step's impulse of the same feature (the cached warm start)
+                T Lp[3] = {T(0), T(0), T(0)};
+                bool hit = false;
+                if (ce) {
                     for (uint32_t j = 0; j < ce->np; ++j)
                         if (ce->fid[j] == c.fid) {
-                            L[0] = ce->lam[j][0];
-                            L[1] = ce->lam[j][1];
-                            L[2] = ce->lam[j][2];
+                            Lp[0] = ce->lam[j][0];
+                            Lp[1] = ce->lam[j][1];
+                            Lp[2] = ce->lam[j][2];
+                            hit = true;
                             ++st.warm_hits;
                             break;
                         }
+                }
+                T* L = &lambda[((size_t)m * 4 + k) * 3];
+                for (int d = 0; d < 3; ++d) L[d] = warm_in ? warm_in[((size_t)m * 4 + k) * 3 + d] : Lp[d];
+                {
+                    // warm-start network inputs: contact frame, impulses in units of u = mN g dt
+                    const T gdt = T(9.81f) * dt;
+                    const T inv_gdt = T(1) / gdt;
+                    const T u = r.mN * gdt;
+                    const T inv_u = u > T(0) ? T(1) / u : T(0);
+                    const V3<T> vr = vrel(A, B, r);
+                    T* f = &feat[((size_t)m * 4 + k) * 24];
+                    f[0] = r.n.x; f[1] = r.n.y; f[2] = r.n.z; f[3] = c.depth * T(20);
+                    f[4] = vn0 * inv_gdt; f[5] = dot(vr, r.t1) * inv_gdt; f[6] = dot(vr, r.t2) * inv_gdt;
+                    f[7] = dot(r.rA, r.n); f[8] = dot(r.rA, r.t1); f[9] = dot(r.rA, r.t2);
+                    f[10] = dot(r.rB, r.n); f[11] = dot(r.rB, r.t1); f[12] = dot(r.rB, r.t2);
+                    f[13] = A.im * r.mN; f[14] = B.im * r.mN; f[15] = r.mT1 * kk[0]; f[16] = r.mT2 * kk[0];
+                    f[17] = bias * inv_gdt; f[18] = Lp[0] * inv_u; f[19] = Lp[1] * inv_u; f[20] = Lp[2] * inv_u;
+                    f[21] = hit ? T(1) : T(0); f[22] = T(man[m].np) * T(0.25f); f[23] = (b & PLANE_BIT) ? T(1) : T(0);
                 }
             }
         }
