@@ -75,26 +75,51 @@ def train(X, Y, epochs=5, batch=4096, lr=1e-3, seed=1, device=0, shuffle_seed=7,
     return net, losses
 
 
-def iterations_to_tolerance(bodies, desc, steps, tol, max_iters, warm, net=None, warmup=0, dt=1.0 / 120.0):
-    """Steps a world with the residual-stopped solver (residual checked after every sweep) and returns
-    the sweeps each of the `steps` counted steps needed (capped at max_iters) and whether it reached
-    `tol`; `warmup` earlier steps run the same way uncounted.  `warm` is axm.WARM_CACHE or axm.WARM_MLP."""
+def iterations_to_tolerance(bodies, desc, steps, tol, max_iters, warm, net=None, warmup=0, dt=1.0 / 120.0,
+                            every=1, timing=None):
+    """Steps a world with the residual-stopped solver (residual checked every `every` sweeps) and
+    returns the sweeps each of the `steps` counted steps needed (capped at max_iters) and whether it
+    reached `tol`; `warmup` earlier steps run the same way uncounted.  `warm` is axm.WARM_CACHE or
+    axm.WARM_MLP.  With a dict as `timing`, the wall time of the counted steps goes to timing["s"]."""
+    import time
     d = axm.default_desc()
     for f, _ in axm.WorldDesc._fields_:
         setattr(d, f, getattr(desc, f))
     d.warm_start = warm
     d.residual_tol = tol
-    d.residual_every = 1
+    d.residual_every = every
     d.max_iterations = max_iters
     w = axm.World(d)
     w.set_bodies(bodies)
     if net is not None:
         w.set_mlp(net)
     iters, reached = [], []
+    t0 = None
     for k in range(warmup + steps):
+        if k == warmup:
+            w.sync()
+            t0 = time.perf_counter()
         st = w.step(dt, max_iters, axm.STEP_RESIDUAL)
         if k >= warmup:
             iters.append(st.iterations)
             reached.append(st.residual <= tol)
+    w.sync()
+    if timing is not None and t0 is not None:
+        timing["s"] = time.perf_counter() - t0
     w.close()
     return np.array(iters), np.array(reached)
+
+
+def train_on_config0(seeds=16, steps=240, sweeps=60, epochs=10, batch=2048, lr=1e-3):
+    """The reference training run: contacts logged from the plain solver on `seeds` seeded 1k-body
+    mixed scenes (world seeds 11..10+seeds of configuration 0), then minibatch Adam on the GPU."""
+    base = 0x4158494F4D000000
+    xs, ys = [], []
+    for sd in range(1, seeds + 1):
+        b, d = axm.scene(0, base + 2 * (10 + sd), 1000)
+        X, Y = collect(b, d, steps, sweeps)
+        xs.append(X)
+        ys.append(Y)
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    net, losses = train(X, Y, epochs=epochs, batch=batch, lr=lr)
+    return net, losses, len(X)

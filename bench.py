@@ -56,6 +56,39 @@ def cpu_baseline(desc, bodies, iters, dt, budget_s):
                        "%d bodies x %d steps (%.1f s)" % (bodies.n, total_steps, t))
 
 
+def tolerance_comparison(tol, max_sweeps, worlds):
+    """Body-steps/s and sweeps per step when every step iterates to residual <= tol (capped at
+    max_sweeps): the plain previous-step warm start against the learned one (network trained here on
+    the GPU on contacts logged from the plain solver), on one held-out 1k-body scene and on a batch of
+    `worlds` independent worlds stepped together."""
+    import warmstart
+
+    t0 = time.time()
+    net, losses, rows = warmstart.train_on_config0()
+    out = {"residual_tol": tol, "max_sweeps": max_sweeps,
+           "training": {"rows": rows, "epochs": len(losses), "loss_first": round(float(losses[0]), 5),
+                        "loss_last": round(float(losses[-1]), 5), "seconds": round(time.time() - t0, 2),
+                        "data": "16 seeded 1k-body scenes x 240 steps, plain solver at 60 sweeps"}}
+    log("warm-start network trained on %d contacts in %.1fs" % (rows, time.time() - t0))
+    b, d = axm.scene(0, SEED_BASE + 2, 1000)
+    b3, d3 = axm.scene(3, SEED_BASE + 3000, 0, worlds)
+    cases = (("single_1k_plain", b, d, axm.WARM_CACHE, None, 60, 120, 1),
+             ("single_1k_learned", b, d, axm.WARM_MLP, net, 60, 120, 1),
+             ("batched_plain", b3, d3, axm.WARM_CACHE, None, 60, 30, 5),
+             ("batched_learned", b3, d3, axm.WARM_MLP, net, 60, 30, 5))
+    for name, bb, dd, warm, nn, warmup, steps, every in cases:
+        tm = {}
+        it, reached = warmstart.iterations_to_tolerance(bb, dd, steps, tol, max_sweeps, warm, net=nn, warmup=warmup,
+                                                        every=every, timing=tm)
+        out[name] = {"bodies": bb.n, "steps": steps, "body_steps_per_s": round(bb.n * steps / tm["s"], 1),
+                     "mean_sweeps": round(float(it.mean()), 2), "median_sweeps": float(np.median(it)),
+                     "fraction_reached": round(float(reached.mean()), 3), "residual_check_every": every}
+        log("%s: %.0f body-steps/s, mean sweeps %.1f, reached %.2f" % (name, out[name]["body_steps_per_s"],
+                                                                       it.mean(), reached.mean()))
+    net.close()
+    return out
+
+
 def measured_traffic(cfg, n, iters, args):
     """HBM bytes per sweep-kernel launch from the committed counter runs (rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE, scaled by a plain copy of known size), when they cover this exact workload; else None."""
@@ -103,6 +136,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
+    ap.add_argument("--no-tolerance", action="store_true", help="skip the plain vs learned run at residual <= tol")
+    ap.add_argument("--tolerance", type=float, default=1e-4)
+    ap.add_argument("--tolerance-worlds", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain copies of MIB MiB (counter scaling), then exit")
     args = ap.parse_args()
@@ -179,13 +215,16 @@ def main():
                                                     "integration_ms", "total_ms")},
         "roofline": roof,
     }
+    w_state = w.get_bodies() if not args.no_cpu else None
+    w.close()
+    if not args.no_tolerance:
+        out["at_tolerance"] = tolerance_comparison(args.tolerance, args.max_iters * 3 // 2, args.tolerance_worlds)
     if not args.no_cpu:
-        g = w.get_bodies()
+        g = w_state
         state = bodies.copy()
         state.pos, state.quat, state.lin_vel, state.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
         out["cpu_baseline"] = cpu_baseline(desc, state, iters, dt, args.cpu_seconds)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 2)
-    w.close()
     print(json.dumps(out), flush=True)
 
 
