@@ -90,7 +90,7 @@ EXPORTS = [
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
     "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
     "axm_mlp_forward", "axm_mlp_loss_grad", "axm_mlp_adam_step", "axm_mlp_train", "axm_world_set_mlp",
-    "axm_world_set_feature_logging", "axm_world_get_features",
+    "axm_world_set_feature_logging", "axm_world_get_features", "axm_world_set_sweep_config",
 ]
 MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 24, 3, 20099
 
@@ -146,6 +146,7 @@ def lib():
     L.axm_mlp_adam_step.argtypes = [P, f32p, C.c_float, C.c_float, C.c_float, C.c_float]
     L.axm_mlp_train.argtypes = [P, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, f32p]
     L.axm_world_set_mlp.argtypes = [P, P]
+    L.axm_world_set_sweep_config.argtypes = [P, C.c_uint32, C.c_uint32]
     L.axm_world_set_feature_logging.argtypes = [P, C.c_int]
     L.axm_world_get_features.argtypes = [P, f32p, f32p, C.c_uint32, u32p]
     L.axm_rng_stream.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_float, C.c_float, f32p]
@@ -377,6 +378,9 @@ class World:
         """Attaches a warm-start network (used when desc.warm_start == WARM_MLP); None detaches."""
         self._mlp = mlp
         _check(self._L.axm_world_set_mlp(self._h, mlp._h if mlp is not None else None), "axm_world_set_mlp")
+
+    def set_sweep_config(self, groups=64, policy=0x10):
+        _check(self._L.axm_world_set_sweep_config(self._h, groups, policy), "axm_world_set_sweep_config")
 
     def set_feature_logging(self, enable=True):
         _check(self._L.axm_world_set_feature_logging(self._h, 1 if enable else 0), "axm_world_set_feature_logging")

@@ -584,11 +584,14 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         tangents(n, t1, t2);
         const f3 dirs[3] = {n, t1, t2};
         float kk[3];
+        f3 ja[3], jb[3], jia[3], jib[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const f3 ra = cross(rA, dirs[d]);
-            const f3 rb = cross(rB, dirs[d]);
-            kk[d] = A.im + Bd.im + dot(ra, smul(A.I, ra)) + dot(rb, smul(Bd.I, rb));
+            ja[d] = cross(rA, dirs[d]);
+            jb[d] = cross(rB, dirs[d]);
+            jia[d] = smul(A.I, ja[d]);
+            jib[d] = smul(Bd.I, jb[d]);
+            kk[d] = A.im + Bd.im + dot(ja[d], jia[d]) + dot(jb[d], jib[d]);
         }
         const float mN = kk[0] > 0.0f ? 1.0f / kk[0] : 0.0f;
         const float mT1 = kk[1] > 0.0f ? 1.0f / kk[1] : 0.0f;
@@ -607,10 +610,22 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
             if (rb > bias) bias = rb;
         }
         const size_t o = (size_t)k * cap + s;
-        R.r0[o] = f4(rA, mN);
-        R.r1[o] = f4(rB, bias);
-        R.r2[o] = f4(n, mT1);
-        R.r3[o] = make_float4(mT2, kk[0], kk[1], kk[2]);
+        float4* J = R.J + (size_t)k * JROWS * cap + s;
+        J[0 * (size_t)cap] = f4(n, mN);
+        J[1 * (size_t)cap] = f4(ja[0], kk[0]);
+        J[2 * (size_t)cap] = f4(jb[0], 0.0f);
+        J[3 * (size_t)cap] = f4(jia[0], bias);
+        J[4 * (size_t)cap] = f4(jib[0], 0.0f);
+        J[5 * (size_t)cap] = f4(t1, mT1);
+        J[6 * (size_t)cap] = f4(ja[1], kk[1]);
+        J[7 * (size_t)cap] = f4(jb[1], 0.0f);
+        J[8 * (size_t)cap] = f4(jia[1], 0.0f);
+        J[9 * (size_t)cap] = f4(jib[1], 0.0f);
+        J[10 * (size_t)cap] = f4(t2, mT2);
+        J[11 * (size_t)cap] = f4(ja[2], kk[2]);
+        J[12 * (size_t)cap] = f4(jb[2], 0.0f);
+        J[13 * (size_t)cap] = f4(jia[2], 0.0f);
+        J[14 * (size_t)cap] = f4(jib[2], 0.0f);
         R.fid[o] = fid;
         // previous step's impulse of the same feature (the cached warm start)
         float4 Lp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -647,24 +662,14 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         }
     }
     if (hits) atomicAdd(&cnt->warm_hits, hits);
-    // algorithmic bytes one k_pgs visit of this manifold moves: header 16,
-    // per point rows r0..r2 + lambda read/write (80), body A read 64 (+32 written
-    // when dynamic), body B likewise unless it is a plane
-    unsigned long long bytes = 16ull + 80ull * np + 64ull + (A.dyn ? 32ull : 0ull);
-    if (!plane) bytes += 64ull + (Bd.dyn ? 32ull : 0ull);
+    // algorithmic bytes one sweep visit of this manifold moves: header 16; per
+    // point 15 Jacobian rows (9 against a plane, whose b / ib rows are not read)
+    // and lambda read + write; each body's solver velocities read (32) and, when
+    // dynamic, written back (32); a plane has no body
+    unsigned long long bytes = 16ull + (plane ? 9ull * 16ull + 32ull : 15ull * 16ull + 32ull) * np + 32ull +
+                               (A.dyn ? 32ull : 0ull);
+    if (!plane) bytes += 32ull + (Bd.dyn ? 32ull : 0ull);
     atomicAdd(&cnt->pgs_bytes, bytes);
-}
-
-__device__ __forceinline__ void load_row(const SolverRows& R, size_t o, f3& rA, f3& rB, f3& n, float& mN, float& mT1,
-                                         float& mT2, float& target) {
-    const float4 a = R.r0[o], b = R.r1[o], c = R.r2[o], d = R.r3[o];
-    rA = xyz(a);
-    mN = a.w;
-    rB = xyz(b);
-    target = b.w;
-    n = xyz(c);
-    mT1 = c.w;
-    mT2 = d.x;
 }
 
 // Memory policy of the sweep: plain accesses when every colour is its own
@@ -698,13 +703,63 @@ template <bool COH> struct Mem {
     }
 };
 
-template <bool COH> __device__ __forceinline__ void load_sb_m(const Bodies& B, uint32_t i, SB& s) {
+// Solver velocities of a body (the sweep needs no inertia: I a and I b are in the rows).
+template <bool COH> __device__ __forceinline__ void load_vel(const Bodies& B, uint32_t i, SB& s) {
     const float4 v = Mem<COH>::ld(B.sv + i), w = Mem<COH>::ld(B.sw + i);
     s.v = xyz(v);
     s.im = v.w;
     s.w = xyz(w);
-    s.I = load_s3(B.I0[i], B.I1[i]);
     s.dyn = v.w > 0.0f;
+}
+
+// One constraint direction of a contact point: d, a = rA x d, b = rB x d, ia = IA a, ib = IB b.
+struct Dir {
+    f3 d, a, b, ia, ib;
+};
+// dot product with fused multiply-adds: x.z y.z + (x.y y.y + x.x y.x)
+__device__ __forceinline__ float fdot(f3 x, f3 y) { return fmaf(x.z, y.z, fmaf(x.y, y.y, x.x * y.x)); }
+// relative velocity along the direction: d.(vB - vA) + b.wB - a.wA
+__device__ __forceinline__ float jvel(const Dir& J, const SB& A, const SB& Bd) {
+    const float r = fdot(J.d, Bd.v - A.v) + fdot(J.b, Bd.w);
+    return r - fdot(J.a, A.w);
+}
+// impulse `lam` along the direction: A gets -lam, B +lam
+__device__ __forceinline__ void japply(const Dir& J, float lam, SB& A, SB& Bd) {
+    if (A.dyn) {
+        const float sa = -(lam * A.im);
+        A.v = f3{fmaf(sa, J.d.x, A.v.x), fmaf(sa, J.d.y, A.v.y), fmaf(sa, J.d.z, A.v.z)};
+        A.w = f3{fmaf(-lam, J.ia.x, A.w.x), fmaf(-lam, J.ia.y, A.w.y), fmaf(-lam, J.ia.z, A.w.z)};
+    }
+    if (Bd.dyn) {
+        const float sb = lam * Bd.im;
+        Bd.v = f3{fmaf(sb, J.d.x, Bd.v.x), fmaf(sb, J.d.y, Bd.v.y), fmaf(sb, J.d.z, Bd.v.z)};
+        Bd.w = f3{fmaf(lam, J.ib.x, Bd.w.x), fmaf(lam, J.ib.y, Bd.w.y), fmaf(lam, J.ib.z, Bd.w.z)};
+    }
+}
+// Direction `dir` (0 n, 1 t1, 2 t2) of the point whose rows start at J (stride cap); the
+// .w of the first row is returned in w0.  Against a plane b and ib are zero and not read.
+__device__ __forceinline__ Dir load_dir(const float4* J, uint32_t cap, int dir, bool plane, float& w0, float& w1,
+                                        float& w2, float& w3, float& w4) {
+    const float4* q = J + (size_t)(5 * dir) * cap;
+    const float4 r0 = q[0], r1 = q[(size_t)cap], r3 = q[3 * (size_t)cap];
+    Dir D;
+    D.d = xyz(r0);
+    D.a = xyz(r1);
+    D.ia = xyz(r3);
+    w0 = r0.w;
+    w1 = r1.w;
+    w3 = r3.w;
+    if (plane) {
+        D.b = D.ib = f3{0.0f, 0.0f, 0.0f};
+        w2 = w4 = 0.0f;
+    } else {
+        const float4 r2 = q[2 * (size_t)cap], r4 = q[4 * (size_t)cap];
+        D.b = xyz(r2);
+        D.ib = xyz(r4);
+        w2 = r2.w;
+        w4 = r4.w;
+    }
+    return D;
 }
 
 // One visit of the manifold in solve slot s: warm start (first sweep only),
@@ -715,61 +770,58 @@ __device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp
     const uint32_t a = h.x, b = h.y, np = h.z;
     const bool plane = (b & PLANE_BIT) != 0;
     SB A, Bd;
-    load_sb_m<COH>(B, a, A);
-    if (plane) zero_sb(Bd); else load_sb_m<COH>(B, b, Bd);
+    load_vel<COH>(B, a, A);
+    if (plane) zero_sb(Bd); else load_vel<COH>(B, b, Bd);
     const float mu = sp.friction;
+    float w0, w1, w2, w3, w4;
     if (first) {
         for (uint32_t k = 0; k < np; ++k) {
-            const size_t o = (size_t)k * cap + s;
-            f3 rA, rB, n, t1, t2;
-            float mN, mT1, mT2, tg;
-            load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
-            tangents(n, t1, t2);
-            const float4 L = Mem<COH>::ld(R.lam + o);
-            const f3 Pw = n * L.x + t1 * L.y + t2 * L.z;
-            apply_imp(A, Bd, rA, rB, Pw);
+            const float4* J = R.J + (size_t)k * JROWS * cap + s;
+            const float4 L = Mem<COH>::ld(R.lam + (size_t)k * cap + s);
+            const Dir Dn = load_dir(J, cap, 0, plane, w0, w1, w2, w3, w4);
+            const Dir D1 = load_dir(J, cap, 1, plane, w0, w1, w2, w3, w4);
+            const Dir D2 = load_dir(J, cap, 2, plane, w0, w1, w2, w3, w4);
+            japply(Dn, L.x, A, Bd);
+            japply(D1, L.y, A, Bd);
+            japply(D2, L.z, A, Bd);
         }
     }
     float4 Ls[4];
     for (uint32_t k = 0; k < np; ++k) {
-        const size_t o = (size_t)k * cap + s;
-        f3 rA, rB, n, t1, t2;
-        float mN, mT1, mT2, tg;
-        load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
-        tangents(n, t1, t2);
-        float4 L = Mem<COH>::ld(R.lam + o);
+        const float4* J = R.J + (size_t)k * JROWS * cap + s;
+        float4 L = Mem<COH>::ld(R.lam + (size_t)k * cap + s);
         const float maxf = mu * L.x;
         {
-            const float vt = dot(vrel(A, Bd, rA, rB), t1);
-            float nl = L.y - mT1 * vt;
+            float mT1;
+            const Dir D1 = load_dir(J, cap, 1, plane, mT1, w1, w2, w3, w4);
+            float nl = fmaf(-mT1, jvel(D1, A, Bd), L.y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L.y;
             L.y = nl;
-            apply_imp(A, Bd, rA, rB, t1 * d);
+            japply(D1, d, A, Bd);
         }
         {
-            const float vt = dot(vrel(A, Bd, rA, rB), t2);
-            float nl = L.z - mT2 * vt;
+            float mT2;
+            const Dir D2 = load_dir(J, cap, 2, plane, mT2, w1, w2, w3, w4);
+            float nl = fmaf(-mT2, jvel(D2, A, Bd), L.z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L.z;
             L.z = nl;
-            apply_imp(A, Bd, rA, rB, t2 * d);
+            japply(D2, d, A, Bd);
         }
         Ls[k & 3] = L;
     }
     for (uint32_t k = 0; k < np; ++k) {
-        const size_t o = (size_t)k * cap + s;
-        f3 rA, rB, n;
-        float mN, mT1, mT2, tg;
-        load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
+        const float4* J = R.J + (size_t)k * JROWS * cap + s;
+        float mN, x1, x2, tg, x4;
+        const Dir Dn = load_dir(J, cap, 0, plane, mN, x1, x2, tg, x4);
         float4 L = Ls[k & 3];
-        const float vn = dot(vrel(A, Bd, rA, rB), n);
-        float nl = L.x + mN * (tg - vn);
+        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L.x;
         L.x = nl;
-        apply_imp(A, Bd, rA, rB, n * d);
-        Mem<COH>::st(R.lam + o, L);
+        japply(Dn, d, A, Bd);
+        Mem<COH>::st(R.lam + (size_t)k * cap + s, L);
     }
     if (A.dyn) {
         Mem<COH>::st(B.sv + a, f4(A.v, A.im));
@@ -800,7 +852,7 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 // a bounded spin sets `err` instead of hanging.
 __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
                                                   uint32_t sweep0, uint32_t nsweeps, uint32_t* __restrict__ bdone,
-                                                  uint32_t spin_limit) {
+                                                  uint32_t spin_limit, uint32_t tune) {
     const uint32_t lane = __lane_id();
     const uint32_t M = cnt->m_eff;
     if (cnt->max_colour > 64u) {
@@ -831,31 +883,64 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
         }
         const uint32_t tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
         const uint32_t tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
+        // tune bit 0: solve the ready lanes together once every pending lane is ready, or once the
+        // ready set has not grown for 4 polls (a chunk can straddle two colours, and a lane of the
+        // later colour may wait for a lane of the same chunk); otherwise each lane as soon as it is
+        // ready.  Bits 4-6: poll back-off level.
+        const bool gather = (tune & 1u) != 0u;
+        const uint32_t nap = (tune >> 4) & 7u;
+        bool ready = !pending;
+        uint64_t last_ready = 0;
+        uint32_t still = 0;
         uint32_t spins = 0;
         const uint64_t t_start = wall_clock64();
-        while (__ballot(pending)) {
-            if (pending) {
-                bool ready = true;
+        for (;;) {
+            if (pending && !ready) {
+                ready = true;
                 if (dp.x != UNSET)
                     ready = __hip_atomic_load(bdone + h.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tA;
                 if (ready && dp.y != UNSET)
                     ready = __hip_atomic_load(bdone + h.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tB;
-                if (ready) {
-                    solve_slot<true>(B, sp, R, h, s, cap, sweep0 + sw == 0);
-                    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): body / impulse stores complete first
-                    if (dp.x != UNSET) __hip_atomic_fetch_add(bdone + h.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (dp.y != UNSET) __hip_atomic_fetch_add(bdone + h.y, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    pending = false;
-                } else if ((++spins & 255u) == 0u) {
-                    if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
-                        atomicOr(err, STICKY_TIMEOUT);  // wait longer than spin_limit microseconds
-                        pending = false;
-                    } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                        pending = false;  // another wave already gave up: drain
-                    }
-                }
             }
-            if (__ballot(pending)) __builtin_amdgcn_s_sleep(1);
+            bool go = true;
+            if (gather) {
+                const uint64_t rset = __ballot(pending && ready);
+                if (rset != last_ready) {
+                    last_ready = rset;
+                    still = 0;
+                } else {
+                    ++still;
+                }
+                go = __ballot(pending && !ready) == 0ull || (rset != 0ull && still >= 4u);
+            }
+            if (pending && ready && go) {
+                solve_slot<true>(B, sp, R, h, s, cap, sweep0 + sw == 0);
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): body / impulse stores complete first
+                if (dp.x != UNSET) __hip_atomic_fetch_add(bdone + h.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (dp.y != UNSET) __hip_atomic_fetch_add(bdone + h.y, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pending = false;
+            }
+            if (!__ballot(pending)) break;
+            if ((++spins & 255u) == 0u) {
+                bool stop = false;
+                if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
+                    if (pending) atomicOr(err, STICKY_TIMEOUT);  // wait longer than spin_limit microseconds
+                    stop = true;
+                } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    stop = true;  // another wave already gave up: drain
+                }
+                if (stop) pending = false;
+                if (!__ballot(pending)) break;
+            }
+            switch (nap) {
+            case 0: break;
+            case 1: __builtin_amdgcn_s_sleep(1); break;
+            case 2: __builtin_amdgcn_s_sleep(2); break;
+            case 3: __builtin_amdgcn_s_sleep(4); break;
+            case 4: __builtin_amdgcn_s_sleep(8); break;
+            case 5: __builtin_amdgcn_s_sleep(16); break;
+            default: __builtin_amdgcn_s_sleep(32); break;
+            }
         }
     }
 }
@@ -869,28 +954,26 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
         const uint4 h = R.hdr[s];
         const bool plane = (h.y & PLANE_BIT) != 0;
         SB A, Bd;
-        load_sb(B, h.x, A, false);
-        if (plane) zero_sb(Bd); else load_sb(B, h.y, Bd, false);
+        load_vel<false>(B, h.x, A);
+        if (plane) zero_sb(Bd); else load_vel<false>(B, h.y, Bd);
         const float mu = sp.friction;
         for (uint32_t k = 0; k < h.z; ++k) {
-            const size_t o = (size_t)k * cap + s;
-            f3 rA, rB, n, t1, t2;
-            float mN, mT1, mT2, tg;
-            load_row(R, o, rA, rB, n, mN, mT1, mT2, tg);
-            const float4 kk = R.r3[o];
-            tangents(n, t1, t2);
-            const float4 L = R.lam[o];
-            const f3 vr = vrel(A, Bd, rA, rB);
-            float nl = L.x + mN * (tg - dot(vr, n));
+            const float4* J = R.J + (size_t)k * JROWS * cap + s;
+            float mN, kN, tg, mT1, kT1, mT2, kT2, x2, x3, x4;
+            const Dir Dn = load_dir(J, cap, 0, plane, mN, kN, x2, tg, x4);
+            const Dir D1 = load_dir(J, cap, 1, plane, mT1, kT1, x2, x3, x4);
+            const Dir D2 = load_dir(J, cap, 2, plane, mT2, kT2, x2, x3, x4);
+            const float4 L = R.lam[(size_t)k * cap + s];
+            float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
             nl = nl > 0.0f ? nl : 0.0f;
-            float e = fabsf(nl - L.x) * kk.y;
+            float e = fabsf(nl - L.x) * kN;
             const float maxf = mu * L.x;
-            float q1 = L.y - mT1 * dot(vr, t1);
+            float q1 = fmaf(-mT1, jvel(D1, A, Bd), L.y);
             q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
-            const float e1 = fabsf(q1 - L.y) * kk.z;
-            float q2 = L.z - mT2 * dot(vr, t2);
+            const float e1 = fabsf(q1 - L.y) * kT1;
+            float q2 = fmaf(-mT2, jvel(D2, A, Bd), L.z);
             q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
-            const float e2 = fabsf(q2 - L.z) * kk.w;
+            const float e2 = fabsf(q2 - L.z) * kT2;
             if (e1 > e) e = e1;
             if (e2 > e) e = e2;
             if (e > res) res = e;
@@ -1087,7 +1170,7 @@ __global__ void __launch_bounds__(256) k_mlp_apply(SolverRows R, const Counters*
     const float gdt = MLP_G_REF * sp.dt;
     for (uint32_t k = 0; k < np; ++k) {
         const size_t o = (size_t)k * cap + s;
-        const float u = R.r0[o].w * gdt;
+        const float u = R.J[(size_t)k * JROWS * cap + s].w * gdt;  // mN
         const float* y = Y + ((size_t)4 * s + k) * MLP_OUT;
         const float4 Lp = R.lam[o];
         float ln = Lp.x + y[0] * u;
@@ -1107,7 +1190,7 @@ void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, co
 }
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
                      uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
-                     uint32_t max_groups, hipStream_t s) {
+                     uint32_t max_groups, uint32_t tune, hipStream_t s) {
     if (!nsweeps) return;
     hipLaunchKernelGGL(k_flow_reset, grid_for(n_bodies), dim3(256), 0, s, bdone, n_bodies, cnt);
     const uint64_t chunks = ((uint64_t)nsweeps * cap + 63) / 64;
@@ -1115,7 +1198,7 @@ void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R,
     if (groups > max_groups) groups = max_groups;
     if (groups < 1) groups = 1;
     hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, cnt, cap, sweep0, nsweeps, bdone,
-                       spin_limit);
+                       spin_limit, tune);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {

@@ -4,15 +4,20 @@
 
 namespace axm {
 
-// Solver rows in solve order, slot-major: element (point k, slot s) at k * cap + s.
-//   r0 = (rA, mN)  r1 = (rB, target)  r2 = (n, mT1)  r3 = (mT2, kN, kT1, kT2)
-//   lam = (lambda_n, lambda_t1, lambda_t2, -), lam0 = warm-start value, fid = feature id
+// Solver rows in solve order, slot-major.  Per contact point k of slot s, JROWS
+// float4 at J[(k * JROWS + j) * cap + s] hold the three constraint directions
+// d = n, t1, t2 with their angular Jacobians a = rA x d, b = rB x d and the
+// inverse-inertia-weighted terms ia = IA a, ib = IB b (precomputed once per
+// step, so a sweep's row update is dot products and fused multiply-adds):
+//   j0 (n, mN)     j1 (aN, kN)   j2 (bN, -)  j3 (iaN, target)  j4 (ibN, -)
+//   j5 (t1, mT1)   j6 (a1, kT1)  j7 (b1, -)  j8 (ia1, -)       j9 (ib1, -)
+//   j10 (t2, mT2)  j11 (a2, kT2) j12 (b2, -) j13 (ia2, -)      j14 (ib2, -)
+// (m = effective mass, k = 1 / m).  lam = (lambda_n, lambda_t1, lambda_t2, -) at
+// k * cap + s, lam0 = its warm-start value, fid = feature id.
+constexpr uint32_t JROWS = 15;
 struct SolverRows {
     uint4* hdr;  // (a, b, npts, manifold) per slot
-    float4* r0;
-    float4* r1;
-    float4* r2;
-    float4* r3;
+    float4* J;
     float4* lam;
     float4* lam0;
     uint32_t* fid;
@@ -63,7 +68,7 @@ void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, co
                       hipStream_t s);
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
                      uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
-                     uint32_t max_groups, hipStream_t s);
+                     uint32_t max_groups, uint32_t tune, hipStream_t s);
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,

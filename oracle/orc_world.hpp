@@ -280,9 +280,15 @@ template <class T> struct World {
     }
 
     // ---------------------------------------------------------------- solver
+    // Per contact point: the three constraint directions (n, t1, t2) with their
+    // angular Jacobians a = rA x d, b = rB x d and inverse-inertia-weighted
+    // terms ia = IA a, ib = IB b, effective masses m = 1 / k, the velocity target.
     struct Row {
         V3<T> rA, rB, n, t1, t2;
+        V3<T> a[3], b[3], ia[3], ib[3];
         T mN, mT1, mT2, kN, kT1, kT2, target;
+        V3<T> dir(int d) const { return d == 0 ? n : (d == 1 ? t1 : t2); }
+        T mass(int d) const { return d == 0 ? mN : (d == 1 ? mT1 : mT2); }
     };
     struct SBody {
         V3<T> v, w;
@@ -292,14 +298,26 @@ template <class T> struct World {
         bool dyn;
     };
 
-    static void apply(SBody& A, SBody& B, const Row& r, V3<T> P) {
+    // Row update primitives of the sweep (explicit fused multiply-adds, the same
+    // sequence as the device kernels).
+    static T fdot(V3<T> x, V3<T> y) { return std::fma(x.z, y.z, std::fma(x.y, y.y, x.x * y.x)); }
+    // relative velocity along direction d: d.(vB - vA) + b.wB - a.wA
+    static T jvel(const Row& r, int d, const SBody& A, const SBody& B) {
+        const T x = fdot(r.dir(d), B.v - A.v) + fdot(r.b[d], B.w);
+        return x - fdot(r.a[d], A.w);
+    }
+    // impulse lam along direction d: A gets -lam, B +lam
+    static void japply(const Row& r, int d, T lam, SBody& A, SBody& B) {
+        const V3<T> dd = r.dir(d);
         if (A.dyn) {
-            A.v = A.v - P * A.im;
-            A.w = A.w - smul(A.I, cross(r.rA, P));
+            const T sa = -(lam * A.im);
+            A.v = {std::fma(sa, dd.x, A.v.x), std::fma(sa, dd.y, A.v.y), std::fma(sa, dd.z, A.v.z)};
+            A.w = {std::fma(-lam, r.ia[d].x, A.w.x), std::fma(-lam, r.ia[d].y, A.w.y), std::fma(-lam, r.ia[d].z, A.w.z)};
         }
         if (B.dyn) {
-            B.v = B.v + P * B.im;
-            B.w = B.w + smul(B.I, cross(r.rB, P));
+            const T sb = lam * B.im;
+            B.v = {std::fma(sb, dd.x, B.v.x), std::fma(sb, dd.y, B.v.y), std::fma(sb, dd.z, B.v.z)};
+            B.w = {std::fma(lam, r.ib[d].x, B.w.x), std::fma(lam, r.ib[d].y, B.w.y), std::fma(lam, r.ib[d].z, B.w.z)};
         }
     }
     static V3<T> vrel(const SBody& A, const SBody& B, const Row& r) {
@@ -350,9 +368,11 @@ template <class T> struct World {
                 const V3<T> dirs[3] = {r.n, r.t1, r.t2};
                 T kk[3];
                 for (int d = 0; d < 3; ++d) {
-                    const V3<T> ra = cross(r.rA, dirs[d]);
-                    const V3<T> rb = cross(r.rB, dirs[d]);
-                    kk[d] = A.im + B.im + dot(ra, smul(A.I, ra)) + dot(rb, smul(B.I, rb));
+                    r.a[d] = cross(r.rA, dirs[d]);
+                    r.b[d] = cross(r.rB, dirs[d]);
+                    r.ia[d] = smul(A.I, r.a[d]);
+                    r.ib[d] = smul(B.I, r.b[d]);
+                    kk[d] = A.im + B.im + dot(r.a[d], r.ia[d]) + dot(r.b[d], r.ib[d]);
                 }
                 r.kN = kk[0];
                 r.kT1 = kk[1];
@@ -419,40 +439,31 @@ template <class T> struct World {
                     for (uint32_t k = 0; k < np; ++k) {
                         const Row& r = rows[(size_t)m * 4 + k];
                         const T* L = &lambda[((size_t)m * 4 + k) * 3];
-                        const V3<T> Pw = r.n * L[0] + r.t1 * L[1] + r.t2 * L[2];
-                        apply(A, B, r, Pw);
+                        japply(r, 0, L[0], A, B);
+                        japply(r, 1, L[1], A, B);
+                        japply(r, 2, L[2], A, B);
                     }
                 }
                 for (uint32_t k = 0; k < np; ++k) {
                     const Row& r = rows[(size_t)m * 4 + k];
                     T* L = &lambda[((size_t)m * 4 + k) * 3];
                     const T maxf = mu * L[0];
-                    {
-                        const T vt = dot(vrel(A, B, r), r.t1);
-                        T nl = L[1] - r.mT1 * vt;
+                    for (int dd = 1; dd <= 2; ++dd) {
+                        T nl = std::fma(-r.mass(dd), jvel(r, dd, A, B), L[dd]);
                         nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
-                        const T d = nl - L[1];
-                        L[1] = nl;
-                        apply(A, B, r, r.t1 * d);
-                    }
-                    {
-                        const T vt = dot(vrel(A, B, r), r.t2);
-                        T nl = L[2] - r.mT2 * vt;
-                        nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
-                        const T d = nl - L[2];
-                        L[2] = nl;
-                        apply(A, B, r, r.t2 * d);
+                        const T d = nl - L[dd];
+                        L[dd] = nl;
+                        japply(r, dd, d, A, B);
                     }
                 }
                 for (uint32_t k = 0; k < np; ++k) {
                     const Row& r = rows[(size_t)m * 4 + k];
                     T* L = &lambda[((size_t)m * 4 + k) * 3];
-                    const T vn = dot(vrel(A, B, r), r.n);
-                    T nl = L[0] + r.mN * (r.target - vn);
+                    T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), L[0]);
                     nl = nl > T(0) ? nl : T(0);
                     const T d = nl - L[0];
                     L[0] = nl;
-                    apply(A, B, r, r.n * d);
+                    japply(r, 0, d, A, B);
                 }
             }
             if (P.record_residual || it + 1 == iters) {
@@ -464,15 +475,14 @@ template <class T> struct World {
                     for (uint32_t k = 0; k < man[m].np; ++k) {
                         const Row& r = rows[(size_t)m * 4 + k];
                         const T* L = &lambda[((size_t)m * 4 + k) * 3];
-                        const V3<T> vr = vrel(A, B, r);
-                        T nl = L[0] + r.mN * (r.target - dot(vr, r.n));
+                        T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), L[0]);
                         nl = nl > T(0) ? nl : T(0);
                         T e = std::fabs(nl - L[0]) * r.kN;
                         const T maxf = mu * L[0];
-                        T t1 = L[1] - r.mT1 * dot(vr, r.t1);
+                        T t1 = std::fma(-r.mT1, jvel(r, 1, A, B), L[1]);
                         t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
                         const T e1 = std::fabs(t1 - L[1]) * r.kT1;
-                        T t2 = L[2] - r.mT2 * dot(vr, r.t2);
+                        T t2 = std::fma(-r.mT2, jvel(r, 2, A, B), L[2]);
                         t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
                         const T e2 = std::fabs(t2 - L[2]) * r.kT2;
                         if (e1 > e) e = e1;
