@@ -360,11 +360,38 @@ __device__ __forceinline__ void grid_sync(GridBar* bar, uint32_t nblocks, Counte
 // dynamic body, so their mask updates cannot collide.  Losers offer their next
 // tag into the other buffer in the same pass, so each round costs one barrier;
 // tags grow with the round, so the buffers never need clearing.
+struct PrevCache {
+    const uint64_t* key;  // (a << 32 | b) -> previous slot
+    const uint32_t* slot;
+    uint32_t mask;
+    const float4* lam;    // [k * cap + s]
+    const uint32_t* fid;  // [k * cap + s]
+    const uint32_t* np;   // points per previous slot
+    const uint32_t* colour;  // colour per previous slot
+    uint32_t cap;
+    uint32_t valid;
+};
+
+__device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32_t& s) {
+    uint32_t h = (uint32_t)mix64(k) & c.mask;
+    for (uint32_t probe = 0; probe <= c.mask; ++probe) {
+        const uint64_t kk = c.key[h];
+        if (kk == k) {
+            s = c.slot[h];
+            return true;
+        }
+        if (kk == EMPTY_KEY) return false;
+        h = (h + 1) & c.mask;
+    }
+    return false;
+}
+
 __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restrict__ m_ab, uint32_t n,
                                                 uint32_t* __restrict__ col, uint64_t* __restrict__ prio,
                                                 uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmax1,
                                                 uint64_t* __restrict__ bmask, uint32_t* __restrict__ bdeg,
-                                                GridBar* bar, Counters* cnt, uint32_t max_rounds) {
+                                                GridBar* bar, Counters* cnt, uint32_t max_rounds, PrevCache pc,
+                                                uint32_t recolour_period) {
     const uint32_t M = cnt->m_eff;
     if (M == 0) return;  // uniform across the grid: no barrier is entered
     const uint32_t nb = gridDim.x;
@@ -383,17 +410,36 @@ __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restric
     }
     if (g0 == 0) cnt->remaining = M;
     grid_sync(bar, nb, cnt);
+    // Temporal coherence: between full recolourings (every recolour_period steps)
+    // a manifold whose body pair was in contact in the previous step keeps that
+    // step's colour (the kept ones stay a valid colouring: a subset of one);
+    // Jones-Plassmann then colours only the new manifolds around them.
+    const bool incremental = pc.valid != 0u && (cnt->steps_done % recolour_period) != 0u;
+    uint32_t kept = 0;
     for (uint32_t m = g0; m < M; m += gs) {
         const uint2 ab = m_ab[m];
+        const bool bd = dyn_body(B, ab.y);
+        atomicAdd(&bdeg[ab.x], 1u);
+        if (bd) atomicAdd(&bdeg[ab.y], 1u);
+        uint32_t ps;
+        if (incremental && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps)) {
+            const uint32_t c = pc.colour[ps];
+            if (c < 64u) {
+                col[m] = c;
+                atomicOr((unsigned long long*)&bmask[ab.x], 1ull << c);
+                if (bd) atomicOr((unsigned long long*)&bmask[ab.y], 1ull << c);
+                atomicMax(&cnt->max_colour, c + 1u);
+                ++kept;
+                continue;
+            }
+        }
         const uint64_t t = (1ull << 48) | prio[m];
         atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
-        atomicAdd(&bdeg[ab.x], 1u);
-        if (dyn_body(B, ab.y)) {
-            atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
-            atomicAdd(&bdeg[ab.y], 1u);
-        }
+        if (bd) atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
     }
+    if (kept) atomicSub(&cnt->remaining, kept);
     grid_sync(bar, nb, cnt);
+    if (__hip_atomic_load(&cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;  // uniform
     for (uint32_t r = 0;; ++r) {
         const uint64_t* cur = (r & 1u) ? bmax1 : bmax0;
         uint64_t* nxt = (r & 1u) ? bmax0 : bmax1;
@@ -506,31 +552,6 @@ __device__ __forceinline__ f3 vrel(const SB& A, const SB& Bd, f3 rA, f3 rB) {
     return (Bd.v + cross(Bd.w, rB)) - (A.v + cross(A.w, rA));
 }
 
-struct PrevCache {
-    const uint64_t* key;  // (a << 32 | b) -> previous slot
-    const uint32_t* slot;
-    uint32_t mask;
-    const float4* lam;    // [k * cap + s]
-    const uint32_t* fid;  // [k * cap + s]
-    const uint32_t* np;   // points per previous slot
-    uint32_t cap;
-    uint32_t valid;
-};
-
-__device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32_t& s) {
-    uint32_t h = (uint32_t)mix64(k) & c.mask;
-    for (uint32_t probe = 0; probe <= c.mask; ++probe) {
-        const uint64_t kk = c.key[h];
-        if (kk == k) {
-            s = c.slot[h];
-            return true;
-        }
-        if (kk == EMPTY_KEY) return false;
-        h = (h + 1) & c.mask;
-    }
-    return false;
-}
-
 __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
                                               uint32_t cap, const uint2* __restrict__ m_ab,
                                               const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
@@ -552,6 +573,7 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
     {
         // dependency words for the persistent sweep: deg << 8 | rank per dynamic body
         const uint32_t c = col[m];
+        R.colour[s] = c;
         uint2 dp = make_uint2(UNSET, UNSET);
         if (c < 64u) {
             // a body's colour mask must hold one bit per manifold of the body, or the
@@ -1128,11 +1150,24 @@ int colour_blocks_per_cu() {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_colour, 256, 0) != hipSuccess) return 1;
     return nb < 1 ? 1 : nb;
 }
+static PrevCache make_prev(const PrevCacheArgs& pca) {
+    PrevCache pc;
+    pc.key = pca.key;
+    pc.slot = pca.slot;
+    pc.mask = pca.mask;
+    pc.lam = pca.lam;
+    pc.fid = pca.fid;
+    pc.np = pca.np;
+    pc.colour = pca.colour;
+    pc.cap = pca.cap;
+    pc.valid = pca.valid;
+    return pc;
+}
 void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
                    uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
-                   hipStream_t s) {
+                   const PrevCacheArgs& prev, hipStream_t s) {
     hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, n, col, prio, bmax0, bmax1, bmask, bdeg, bar,
-                       cnt, 65000u);
+                       cnt, 65000u, make_prev(prev), RECOLOUR_PERIOD);
 }
 void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
                            hipStream_t s) {
@@ -1145,15 +1180,7 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
                  const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
-    PrevCache pc;
-    pc.key = pca.key;
-    pc.slot = pca.slot;
-    pc.mask = pca.mask;
-    pc.lam = pca.lam;
-    pc.fid = pca.fid;
-    pc.np = pca.np;
-    pc.cap = pca.cap;
-    pc.valid = pca.valid;
+    const PrevCache pc = make_prev(pca);
     hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
                        warm_in, col, bmask, bdeg, cnt, feat);
 }

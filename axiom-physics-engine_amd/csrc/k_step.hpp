@@ -22,7 +22,10 @@ struct SolverRows {
     float4* lam0;
     uint32_t* fid;
     uint2* dep;  // per slot: (deg << 8 | rank) of body a and body b in the colour order, UNSET if not dynamic
+    uint32_t* colour;  // per slot: the manifold's colour (kept by the next step's incremental colouring)
 };
+// Steps between full recolourings (in between, persisting contacts keep their colour).
+constexpr uint32_t RECOLOUR_PERIOD = 64;
 
 struct PrevCacheArgs {
     const uint64_t* key;
@@ -31,6 +34,7 @@ struct PrevCacheArgs {
     const float4* lam;
     const uint32_t* fid;
     const uint32_t* np;
+    const uint32_t* colour;
     uint32_t cap;
     uint32_t valid;
 };
@@ -53,9 +57,11 @@ void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, 
                     const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,
                     uint32_t* m_np, float4* c_pt, float4* c_nf, Counters* cnt, hipStream_t s);
 int colour_blocks_per_cu();
+// Jones-Plassmann colouring; `prev` (previous step's (a, b) -> slot table and colours,
+// valid = a previous step exists) enables the incremental form between full recolourings.
 void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
                    uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
-                   hipStream_t s);
+                   const PrevCacheArgs& prev, hipStream_t s);
 void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
                            hipStream_t s);
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);

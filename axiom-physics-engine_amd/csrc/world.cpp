@@ -271,6 +271,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.lam0);
     w->mem.release(R.fid);
     w->mem.release(R.dep);
+    w->mem.release(R.colour);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -280,6 +281,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.lam0, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.dep, cap));
+    AXM_TRY(w->mem.get(R.colour, cap));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -303,6 +305,8 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint32_t *fid = nullptr, *np = nullptr;
     uint4* hdr = nullptr;
     uint2* dep = nullptr;
+    uint32_t* colour = nullptr;
+    AXM_TRY(w->mem.get(colour, cap));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(np, cap));
@@ -320,6 +324,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     AXM_HIP(hipMemcpy2DAsync(lam, 16ull * cap, R.lam, 16ull * old, 16ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
     AXM_HIP(hipMemcpy2DAsync(fid, 4ull * cap, R.fid, 4ull * old, 4ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
     AXM_HIP(hipMemcpyAsync(np, w->np_slot[set], 4ull * old, hipMemcpyDeviceToDevice, w->stream));
+    AXM_HIP(hipMemcpyAsync(colour, R.colour, 4ull * old, hipMemcpyDeviceToDevice, w->stream));
     AXM_HIP(hipStreamSynchronize(w->stream));
     w->mem.release(R.lam);
     w->mem.release(R.fid);
@@ -328,6 +333,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.J);
     w->mem.release(R.hdr);
     w->mem.release(R.dep);
+    w->mem.release(R.colour);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
     w->pc_key[set] = tkey;
@@ -340,6 +346,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     R.J = jr;
     R.hdr = hdr;
     R.dep = dep;
+    R.colour = colour;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
@@ -493,26 +500,28 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     }
     enqueue_narrow(w, sp);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[2], s));
-    // ---- colouring and the stable order by colour
-    // bdone doubles as the per-body manifold count of the colouring (the sweep clears it before use)
-    launch_colour(w->B, w->m_ab, n, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->bar, w->cnt,
-                  w->colour_blocks, s);
-    launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
-    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff);
-    const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers
-    const uint32_t* order = w->cval_alt;
-    launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
-    // ---- solver
+    // previous step's (a, b) -> slot table: warm start and incremental colouring
     PrevCacheArgs pca{};
-    pca.valid = (w->prev_valid && w->desc.warm_start != AXM_WARM_NONE) ? 1u : 0u;
     pca.key = w->pc_key[pv];
     pca.slot = w->pc_slot[pv];
     pca.mask = w->pc_mask[pv];
     pca.lam = w->R[pv].lam;
     pca.fid = w->R[pv].fid;
     pca.np = w->np_slot[pv];
+    pca.colour = w->R[pv].colour;
     pca.cap = w->rcap[pv];
-    if (!pca.key) pca.valid = 0;
+    pca.valid = (w->prev_valid && pca.key) ? 1u : 0u;
+    // ---- colouring and the stable order by colour
+    // bdone doubles as the per-body manifold count of the colouring (the sweep clears it before use)
+    launch_colour(w->B, w->m_ab, n, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->bar, w->cnt,
+                  w->colour_blocks, pca, s);
+    launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
+    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff);
+    const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers
+    const uint32_t* order = w->cval_alt;
+    launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
+    // ---- solver
+    if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
     launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
                 w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, s);
     if (w->mlp && w->desc.warm_start == AXM_WARM_MLP && !o.warm_in && w->feat) {

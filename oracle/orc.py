@@ -86,7 +86,7 @@ def lib():
     L.orc_get_residuals.restype = C.c_uint32
     L.orc_get_residuals.argtypes = [P, f32p, C.c_uint32]
     L.orc_step_given.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, u32p, u32p, u32p, f32p, f32p, f32p, u32p, f32p,
-                                 C.POINTER(Stats)]
+                                 u32p, C.POINTER(Stats)]
     L.orc_collide.restype = C.c_int
     L.orc_collide.argtypes = [C.c_int, C.c_uint32, f32p, f32p, f32p, C.c_uint32, f32p, f32p, f32p, C.c_float, u32p,
                               f32p, f32p, f32p, u32p]
@@ -252,14 +252,16 @@ class OracleWorld:
         self.L.orc_get_residuals(self.h, _p(r), n)
         return r
 
-    def step_given(self, dt, iters, man, warm=None):
+    def step_given(self, dt, iters, man, warm=None, colours=None):
+        """One step on given manifolds (optionally with a given warm start and colouring)."""
         st = Stats()
         M = len(man["a"])
         arr = {k: np.ascontiguousarray(v) for k, v in man.items()}
         w = np.ascontiguousarray(warm, np.float32) if warm is not None else None
+        c = np.ascontiguousarray(colours, np.uint32) if colours is not None else None
         self.L.orc_step_given(self.h, dt, iters, M, _p(arr["a"], C.c_uint32), _p(arr["b"], C.c_uint32),
                               _p(arr["npts"], C.c_uint32), _p(arr["pos"]), _p(arr["normal"]), _p(arr["depth"]),
-                              _p(arr["feature"], C.c_uint32), _p(w), C.byref(st))
+                              _p(arr["feature"], C.c_uint32), _p(w), _p(c, C.c_uint32), C.byref(st))
         return st
 
 

@@ -48,6 +48,7 @@ template <class T> void set_bodies(World<T>& W, uint32_t n, const float* pos, co
         W.world[i] = world ? world[i] : 0u;
     }
     W.cache.clear();
+    W.nsteps = 0;
 }
 
 template <class T> void get_bodies(const World<T>& W, float* pos, float* quat, float* vel, float* angvel) {
@@ -361,14 +362,14 @@ uint32_t orc_get_residuals(void* h, float* out, uint32_t cap) {
 // position integration.
 void orc_step_given(void* h, float dt, uint32_t iters, uint32_t M, const uint32_t* a, const uint32_t* b,
                     const uint32_t* np, const float* p, const float* n, const float* depth, const uint32_t* fid,
-                    const float* warm, Stats* st) {
+                    const float* warm, const uint32_t* colour_in, Stats* st) {
     DISPATCH(h, {
         W.st = Stats{};
         load_manifolds(W, M, a, b, np, p, n, depth, fid);
         W.integrate_velocities(T(dt));
         W.v_pre = W.v;
         W.w_pre = W.w;
-        W.colour_manifolds();
+        W.colour_manifolds(colour_in);
         std::vector<T> wt;
         if (warm) {
             wt.resize((size_t)M * 12);
@@ -376,6 +377,7 @@ void orc_step_given(void* h, float dt, uint32_t iters, uint32_t M, const uint32_
         }
         W.solve(T(dt), iters, warm ? wt.data() : nullptr);
         W.integrate_positions(T(dt));
+        ++W.nsteps;
         W.st.manifolds = M;
         if (st) *st = W.st;
     });

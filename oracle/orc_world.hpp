@@ -85,7 +85,10 @@ template <class T> struct World {
         uint32_t np;
         uint32_t fid[4];
         T lam[4][3];
+        uint32_t colour;
     };
+    uint32_t nsteps = 0;  // steps since the bodies were set (full recolouring every RECOLOUR_PERIOD)
+    static constexpr uint32_t RECOLOUR_PERIOD = 64;
     std::map<uint64_t, CacheEnt> cache;
     std::vector<T> lambda;  // per manifold x 4 points x 3 (solve output)
     std::vector<T> feat;    // per manifold x 4 points x 24 warm-start network inputs
@@ -222,9 +225,24 @@ template <class T> struct World {
         return b - world_first[world[b]];
     }
 
-    void colour_manifolds() {
+    // Deterministic Jones-Plassmann colouring (hash priorities, first-fit colours).
+    // Between full recolourings (every RECOLOUR_PERIOD steps) a manifold whose body
+    // pair was in contact in the previous step keeps that step's colour, and only
+    // the new manifolds are coloured around the kept ones.
+    void colour_manifolds(const uint32_t* colour_in = nullptr) {
         const uint32_t M = (uint32_t)man.size();
         colour.assign(M, 0xFFFFFFFFu);
+        if (colour_in) {
+            uint32_t K = 0;
+            for (uint32_t m = 0; m < M; ++m) {
+                colour[m] = colour_in[m];
+                K = std::max(K, colour[m] + 1);
+            }
+            st.rounds = 0;
+            st.colours = K;
+            order_by_colour(K);
+            return;
+        }
         // first body index of every world (bodies of a world are contiguous)
         uint32_t nw = 0;
         for (uint32_t i = 0; i < n; ++i) nw = std::max(nw, world[i] + 1);
@@ -236,6 +254,18 @@ template <class T> struct World {
         std::vector<uint64_t> bmax(n, 0);
         std::vector<uint64_t> mask(n, 0);
         uint32_t left = M, round = 0, K = 0;
+        if (nsteps > 0 && nsteps % RECOLOUR_PERIOD != 0) {
+            for (uint32_t m = 0; m < M; ++m) {
+                auto it = cache.find(((uint64_t)man_a[m] << 32) | man_b[m]);
+                if (it == cache.end() || it->second.colour >= 64) continue;
+                const uint32_t c = it->second.colour;
+                colour[m] = c;
+                mask[man_a[m]] |= 1ull << c;
+                if (dyn_partner(man_b[m])) mask[man_b[m]] |= 1ull << c;
+                K = std::max(K, c + 1);
+                --left;
+            }
+        }
         while (left > 0) {
             const uint64_t tagbase = (uint64_t)(round + 1) << 48;
             for (uint32_t m = 0; m < M; ++m) {
@@ -271,6 +301,11 @@ template <class T> struct World {
         }
         st.rounds = round;
         st.colours = K;
+        order_by_colour(K);
+    }
+
+    void order_by_colour(uint32_t K) {
+        const uint32_t M = (uint32_t)man.size();
         colour_off.assign(K + 1, 0);
         for (uint32_t m = 0; m < M; ++m) colour_off[colour[m] + 1]++;
         for (uint32_t c = 0; c < K; ++c) colour_off[c + 1] += colour_off[c];
@@ -506,6 +541,7 @@ template <class T> struct World {
         for (uint32_t m = 0; m < M; ++m) {
             CacheEnt e;
             e.np = man[m].np;
+            e.colour = colour[m];
             for (uint32_t k = 0; k < e.np; ++k) {
                 e.fid[k] = man[m].pt[k].fid;
                 for (int d = 0; d < 3; ++d) e.lam[k][d] = lambda[((size_t)m * 4 + k) * 3 + d];
@@ -567,6 +603,7 @@ template <class T> struct World {
         colour_manifolds();
         solve(dt, iters, nullptr);
         integrate_positions(dt);
+        ++nsteps;
         st.pairs = (uint32_t)pair_a.size();
         st.manifolds = (uint32_t)man.size();
         uint32_t c = 0;

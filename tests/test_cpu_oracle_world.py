@@ -162,7 +162,7 @@ def test_momentum_conserved_without_static_bodies(orc, axm, first):
         assert np.abs(L1 - L0).max() <= 1e-3 * max(1.0, np.abs(L0).max())
 
 
-@pytest.mark.parametrize("config,n,warm,sweeps,tol", [(1, 2000, 40, 20, None), (0, 1000, 150, 300, 1e-3)])
+@pytest.mark.parametrize("config,n,warm,sweeps,tol", [(1, 2000, 40, 20, None), (0, 1000, 150, 1000, 1e-2)])
 def test_impulses_stay_in_the_cone(orc, axm, config, n, warm, sweeps, tol):
     """lambda_n >= 0 always.  Each friction row is clamped to mu * lambda_n as it stands when the row
     is solved (friction rows run before the normal row of a point), so the final cone holds once the

@@ -153,7 +153,7 @@ def test_step_from_network_prediction(axm, orc, gpu):
     p.warm_start = 1
     ow = orc.OracleWorld(p)
     ow.set_bodies(ob)
-    ow.step_given(DT, 10, m, warm=lam0)
+    ow.step_given(DT, 10, m, warm=lam0, colours=w.colours()[0])
     g1 = w.get_bodies()
     pos1, quat1, v1, w1 = ow.get_state()
     assert rel_err(g1.lin_vel, v1) <= 1e-4
