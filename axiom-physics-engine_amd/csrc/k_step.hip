@@ -129,13 +129,24 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
 // Visits candidate partners of the body at sorted position t in the canonical
 // order: planes, then the 27 neighbour cells (dz, dy, dx in -1..1), then
 // ascending sorted position.  f(partner) is called for every emitted pair.
+// AABBs in sorted order: smin = (min, body index bits), smax = (max, 1 if dynamic),
+// so the candidates of a cell are contiguous and carry what the test needs.
+__global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* __restrict__ sval, uint32_t n) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = sval[t];
+    const float4 mn = B.amin[i], mx = B.amax[i];
+    B.smin[t] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
+    B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.vel[i].w > 0.0f ? 1.0f : 0.0f);
+}
+
 template <class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t, F&& f) {
-    const uint32_t i = sval[t];
     const uint64_t key = skey[t];
-    const float4 mn = B.amin[i], mx = B.amax[i];
-    const bool dyn_i = B.vel[i].w > 0.0f;
+    const float4 mn = B.smin[t], mx = B.smax[t];
+    const uint32_t i = __float_as_uint(mn.w);
+    const bool dyn_i = mx.w > 0.0f;
     if (dyn_i) {
         const f3 c = xyz(B.x[i]);
         const f3 e = xyz(mx) - c;
@@ -162,11 +173,10 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
                 uint2 r;
                 if (!ht_find(ct, nk, r)) continue;
                 for (uint32_t s = r.x; s < r.x + r.y; ++s) {
-                    const uint32_t j = sval[s];
+                    const float4 mnj = B.smin[s], mxj = B.smax[s];
+                    const uint32_t j = __float_as_uint(mnj.w);
                     if (j <= i) continue;
-                    const bool dyn_j = B.vel[j].w > 0.0f;
-                    if (!dyn_i && !dyn_j) continue;
-                    const float4 mnj = B.amin[j], mxj = B.amax[j];
+                    if (!dyn_i && !(mxj.w > 0.0f)) continue;
                     if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
                         mx.z >= mnj.z)
                         f(j);
@@ -176,13 +186,19 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     }
 }
 
+// Counts each body's pairs and keeps the first PAIR_STASH partners, so that the
+// emit pass only re-visits the grid for the few bodies with more.
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
-                                                    uint32_t* __restrict__ cnt) {
+                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint32_t c = 0;
-    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t) { ++c; });
+    uint32_t* st = stash + (size_t)t * PAIR_STASH;
+    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t j) {
+        if (c < PAIR_STASH) st[c] = j;
+        ++c;
+    });
     cnt[t] = c;
 }
 
@@ -190,7 +206,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                    const uint32_t* __restrict__ off, uint32_t cap,
                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
-                                                   Counters* __restrict__ cnt) {
+                                                   Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         const uint32_t P = off[n];
@@ -199,7 +215,16 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
     }
     if (t >= n) return;
     uint32_t o = off[t];
+    const uint32_t c = off[t + 1] - o;
     const uint32_t i = sval[t];
+    if (c <= PAIR_STASH) {
+        const uint32_t* st = stash + (size_t)t * PAIR_STASH;
+        for (uint32_t k = 0; k < c && o + k < cap; ++k) {
+            pa[o + k] = i;
+            pb[o + k] = st[k];
+        }
+        return;
+    }
     visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t j) {
         if (o < cap) {
             pa[o] = i;
@@ -1121,14 +1146,17 @@ void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Cou
     if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt);
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
+    hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
-                      Counters* cnt, hipStream_t s) {
+                      Counters* cnt, const uint32_t* stash, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb, cnt);
+        hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb, cnt,
+                           stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, Counters* cnt, hipStream_t s) {

@@ -50,7 +50,10 @@ struct Bodies {
     float4* I1;
     float4* amin;
     float4* amax;
+    float4* smin;  // AABBs in sorted (cell) order, w = body index bits
+    float4* smax;  // w = 1 for a dynamic body
 };
+constexpr uint32_t PAIR_STASH = 24;  // partners kept per body by the counting pass
 
 struct CellTable {
     uint64_t* key;

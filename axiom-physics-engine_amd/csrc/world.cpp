@@ -100,7 +100,7 @@ struct axm_world_s {
     uint32_t *val = nullptr, *val_alt = nullptr, *sort_tmp = nullptr;
     CellTable ct{};
 
-    uint32_t *pair_cnt = nullptr, *pair_off = nullptr, *scan_tmp = nullptr;
+    uint32_t *pair_cnt = nullptr, *pair_off = nullptr, *scan_tmp = nullptr, *pair_stash = nullptr;
     uint32_t *pa = nullptr, *pb = nullptr;
     uint32_t *p_np = nullptr, *p_flag = nullptr, *man_off = nullptr, *epa_list = nullptr, *pscan_tmp = nullptr;
     float4 *p_pt = nullptr, *p_nf = nullptr;
@@ -191,8 +191,9 @@ namespace {
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
+    w->mem.release(w->pair_stash);
     w->mem.release(B.world);
     w->mem.release(B.world_first);
     w->mem.release(w->key);
@@ -216,8 +217,9 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     free_bodies(w);
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
+    AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
     AXM_TRY(w->mem.get(w->key, cap));
@@ -468,9 +470,10 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_fill_u64(w->ct.key, EMPTY_KEY, w->ct.mask + 1, s);
     launch_cell_table(skey, n, w->ct, w->cnt, s);
-    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, s);
+    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, s);
     scan_exclusive(w->pair_cnt, w->pair_off, n, w->scan_tmp, s, &w->cnt->pairs);
-    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, s);
+    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
+                     s);
 }
 
 // ---- narrowphase into per-pair staging, compaction into manifolds (pair order)
