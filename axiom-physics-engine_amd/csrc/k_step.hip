@@ -809,74 +809,76 @@ __device__ __forceinline__ Dir load_dir(const float4* J, uint32_t cap, int dir, 
     return D;
 }
 
-// One visit of the manifold in solve slot s: warm start (first sweep only),
-// friction rows of every point, then normal rows; writes the bodies back.
-template <bool COH>
-__device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp, const SolverRows& R, uint4 h,
-                                           uint32_t s, uint32_t cap, bool first) {
-    const uint32_t a = h.x, b = h.y, np = h.z;
-    const bool plane = (b & PLANE_BIT) != 0;
-    SB A, Bd;
-    load_vel<COH>(B, a, A);
-    if (plane) zero_sb(Bd); else load_vel<COH>(B, b, Bd);
-    const float mu = sp.friction;
+// One visit of a manifold on registers: warm start (first sweep only), friction
+// rows of every point, then normal rows.  L[k] holds the point impulses in and out.
+__device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
+                                           float mu, bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w1, w2, w3, w4;
     if (first) {
         for (uint32_t k = 0; k < np; ++k) {
             const float4* J = R.J + (size_t)k * JROWS * cap + s;
-            const float4 L = Mem<COH>::ld(R.lam + (size_t)k * cap + s);
             const Dir Dn = load_dir(J, cap, 0, plane, w0, w1, w2, w3, w4);
             const Dir D1 = load_dir(J, cap, 1, plane, w0, w1, w2, w3, w4);
             const Dir D2 = load_dir(J, cap, 2, plane, w0, w1, w2, w3, w4);
-            japply(Dn, L.x, A, Bd);
-            japply(D1, L.y, A, Bd);
-            japply(D2, L.z, A, Bd);
+            japply(Dn, L[k].x, A, Bd);
+            japply(D1, L[k].y, A, Bd);
+            japply(D2, L[k].z, A, Bd);
         }
     }
-    float4 Ls[4];
     for (uint32_t k = 0; k < np; ++k) {
         const float4* J = R.J + (size_t)k * JROWS * cap + s;
-        float4 L = Mem<COH>::ld(R.lam + (size_t)k * cap + s);
-        const float maxf = mu * L.x;
+        const float maxf = mu * L[k].x;
         {
             float mT1;
             const Dir D1 = load_dir(J, cap, 1, plane, mT1, w1, w2, w3, w4);
-            float nl = fmaf(-mT1, jvel(D1, A, Bd), L.y);
+            float nl = fmaf(-mT1, jvel(D1, A, Bd), L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
-            const float d = nl - L.y;
-            L.y = nl;
+            const float d = nl - L[k].y;
+            L[k].y = nl;
             japply(D1, d, A, Bd);
         }
         {
             float mT2;
             const Dir D2 = load_dir(J, cap, 2, plane, mT2, w1, w2, w3, w4);
-            float nl = fmaf(-mT2, jvel(D2, A, Bd), L.z);
+            float nl = fmaf(-mT2, jvel(D2, A, Bd), L[k].z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
-            const float d = nl - L.z;
-            L.z = nl;
+            const float d = nl - L[k].z;
+            L[k].z = nl;
             japply(D2, d, A, Bd);
         }
-        Ls[k & 3] = L;
     }
     for (uint32_t k = 0; k < np; ++k) {
         const float4* J = R.J + (size_t)k * JROWS * cap + s;
         float mN, x1, x2, tg, x4;
         const Dir Dn = load_dir(J, cap, 0, plane, mN, x1, x2, tg, x4);
-        float4 L = Ls[k & 3];
-        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
+        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L[k].x);
         nl = nl > 0.0f ? nl : 0.0f;
-        const float d = nl - L.x;
-        L.x = nl;
+        const float d = nl - L[k].x;
+        L[k].x = nl;
         japply(Dn, d, A, Bd);
-        Mem<COH>::st(R.lam + (size_t)k * cap + s, L);
     }
+}
+
+// One visit of the manifold in solve slot s with plain loads and stores (one
+// launch per colour: every visit of a launch touches distinct bodies).
+__device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp, const SolverRows& R, uint4 h,
+                                           uint32_t s, uint32_t cap, bool first) {
+    const uint32_t a = h.x, b = h.y, np = h.z;
+    const bool plane = (b & PLANE_BIT) != 0;
+    SB A, Bd;
+    load_vel<false>(B, a, A);
+    if (plane) zero_sb(Bd); else load_vel<false>(B, b, Bd);
+    float4 L[4];
+    for (uint32_t k = 0; k < np; ++k) L[k] = R.lam[(size_t)k * cap + s];
+    solve_core(R, s, cap, np, plane, sp.friction, first, A, Bd, L);
+    for (uint32_t k = 0; k < np; ++k) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
     if (A.dyn) {
-        Mem<COH>::st(B.sv + a, f4(A.v, A.im));
-        Mem<COH>::st(B.sw + a, f4(A.w, 0.0f));
+        B.sv[a] = f4(A.v, A.im);
+        B.sw[a] = f4(A.w, 0.0f);
     }
     if (Bd.dyn) {
-        Mem<COH>::st(B.sv + b, f4(Bd.v, Bd.im));
-        Mem<COH>::st(B.sw + b, f4(Bd.w, 0.0f));
+        B.sv[b] = f4(Bd.v, Bd.im);
+        B.sw[b] = f4(Bd.w, 0.0f);
     }
 }
 
@@ -884,21 +886,108 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
                                              uint32_t cap, uint32_t first) {
     const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= s1) return;
-    solve_slot<false>(B, sp, R, R.hdr[s], s, cap, first != 0);
+    solve_slot(B, sp, R, R.hdr[s], s, cap, first != 0);
 }
 
 // Persistent sweep kernel.  Work item t = sweep * M + slot, handed out in
 // increasing order in chunks of 64 (one wave).  An item may run once every
-// earlier visit of its dynamic bodies is complete: body x keeps a counter of
-// finished visits, and the visit of slot s in sweep k is number
-// k * deg(x) + rank(x, s) of body x, with rank = colours of x's manifolds below
-// s's colour (the per-body colour masks of the colouring).  Each body therefore
-// sees exactly the update sequence of the colour-by-colour sweep, without a
-// grid-wide barrier.  Chunks are taken in order by running waves, so the
-// smallest pending item always has its dependencies complete (no deadlock);
-// a bounded spin sets `err` instead of hanging.
+// earlier visit of its dynamic bodies is complete: the visit of slot s in sweep
+// k is visit number k * deg(x) + rank(x, s) of body x, with rank = colours of
+// x's manifolds below s's colour (the per-body colour masks of the colouring),
+// so each body sees exactly the update sequence of the colour-by-colour sweep
+// without a grid-wide barrier.
+//
+// Versioned words: during the launch each body's solver velocity lives in six
+// 64-bit words (value bits | visits completed << 32) and each point impulse in
+// three (value | sweeps completed << 32).  A 64-bit store is single-copy atomic,
+// so a reader that loads a body's six words and finds every version equal to
+// the visit number it needs holds exactly that visit's velocities: the poll is
+// the data load, and a writer publishes with plain stores (no store drain, no
+// counter update).  Chunks are taken in order by running waves, so the smallest
+// pending item always has its dependencies complete (no deadlock); a bounded
+// spin sets `err` instead of hanging.
+constexpr uint32_t VB_WORDS = 8;  // words per body (6 used)
+constexpr uint32_t VL_WORDS = 4;  // words per point impulse (3 used)
+
+__device__ __forceinline__ uint64_t vpack(float v, uint32_t ver) {
+    return ((uint64_t)ver << 32) | (uint64_t)__float_as_uint(v);
+}
+__device__ __forceinline__ float vval(uint64_t w) { return __uint_as_float((uint32_t)w); }
+__device__ __forceinline__ uint32_t vver(uint64_t w) { return (uint32_t)(w >> 32); }
+__device__ __forceinline__ uint64_t vload(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Loads body i's versioned velocities; true when all six carry version `ver`.
+__device__ __forceinline__ bool vbody_load(const uint64_t* vb, uint32_t i, uint32_t ver, SB& x) {
+    const uint64_t* p = vb + (size_t)i * VB_WORDS;
+    const uint64_t w0 = vload(p), w1 = vload(p + 1), w2 = vload(p + 2), w3 = vload(p + 3), w4 = vload(p + 4),
+                   w5 = vload(p + 5);
+    x.v = f3{vval(w0), vval(w1), vval(w2)};
+    x.w = f3{vval(w3), vval(w4), vval(w5)};
+    return vver(w0) == ver && vver(w1) == ver && vver(w2) == ver && vver(w3) == ver && vver(w4) == ver &&
+           vver(w5) == ver;
+}
+__device__ __forceinline__ void vbody_store(uint64_t* vb, uint32_t i, uint32_t ver, const SB& x) {
+    uint64_t* p = vb + (size_t)i * VB_WORDS;
+    vstore(p, vpack(x.v.x, ver));
+    vstore(p + 1, vpack(x.v.y, ver));
+    vstore(p + 2, vpack(x.v.z, ver));
+    vstore(p + 3, vpack(x.w.x, ver));
+    vstore(p + 4, vpack(x.w.y, ver));
+    vstore(p + 5, vpack(x.w.z, ver));
+}
+
+// Before a launch: versioned copies (version 0) of the solver velocities and of the
+// impulses in the rows; resets the chunk queue.
+__global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, uint64_t* __restrict__ vlam, uint32_t n,
+                                                   uint32_t cap, Counters* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) cnt->queue = 0u;
+    if (i < n) {
+        const float4 v = B.sv[i], w = B.sw[i];
+        SB x;
+        x.v = xyz(v);
+        x.w = xyz(w);
+        vbody_store(B.vb, i, 0u, x);
+    }
+    // the rows (and their point counts) exist only when this step's preparation ran
+    if (i < cnt->m_eff && cnt->sticky == 0u) {
+        const uint32_t np = min(R.hdr[i].z, 4u);
+        for (uint32_t k = 0; k < np; ++k) {
+            const float4 L = R.lam[(size_t)k * cap + i];
+            uint64_t* q = vlam + ((size_t)k * cap + i) * VL_WORDS;
+            q[0] = vpack(L.x, 0u);
+            q[1] = vpack(L.y, 0u);
+            q[2] = vpack(L.z, 0u);
+        }
+    }
+}
+// After a launch: versioned words back into the solver velocities and the rows.
+__global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, const uint64_t* __restrict__ vlam,
+                                                     uint32_t n, uint32_t cap, const Counters* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && cnt->sticky == 0u) {
+        const float4 v = B.sv[i];
+        if (v.w > 0.0f) {
+            const uint64_t* p = B.vb + (size_t)i * VB_WORDS;
+            B.sv[i] = make_float4(vval(p[0]), vval(p[1]), vval(p[2]), v.w);
+            B.sw[i] = make_float4(vval(p[3]), vval(p[4]), vval(p[5]), 0.0f);
+        }
+    }
+    if (i < cnt->m_eff && cnt->sticky == 0u) {
+        const uint32_t np = min(R.hdr[i].z, 4u);
+        for (uint32_t k = 0; k < np; ++k) {
+            const uint64_t* q = vlam + ((size_t)k * cap + i) * VL_WORDS;
+            R.lam[(size_t)k * cap + i] = make_float4(vval(q[0]), vval(q[1]), vval(q[2]), 0.0f);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
-                                                  uint32_t sweep0, uint32_t nsweeps, uint32_t* __restrict__ bdone,
+                                                  uint32_t sweep0, uint32_t nsweeps, uint64_t* __restrict__ vlam,
                                                   uint32_t spin_limit, uint32_t tune) {
     const uint32_t lane = __lane_id();
     const uint32_t M = cnt->m_eff;
@@ -912,6 +1001,7 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
     uint32_t* err = &cnt->sticky;
     const uint64_t total = (uint64_t)nsweeps * M;
     const uint64_t nchunks = (total + 63) / 64;
+    const float mu = sp.friction;
     for (;;) {
         uint32_t chunk = 0;
         if (lane == 0) chunk = atomicAdd(queue, 1u);
@@ -930,42 +1020,63 @@ __global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, Solve
         }
         const uint32_t tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
         const uint32_t tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
-        // tune bit 0: solve the ready lanes together once every pending lane is ready, or once the
-        // ready set has not grown for 4 polls (a chunk can straddle two colours, and a lane of the
-        // later colour may wait for a lane of the same chunk); otherwise each lane as soon as it is
-        // ready.  Bits 4-6: poll back-off level.
-        const bool gather = (tune & 1u) != 0u;
+        const bool plane = (h.y & PLANE_BIT) != 0u;
+        const uint32_t np = min(h.z, 4u);
+        // the parts of the visit that no other visit writes: inverse masses, a static partner
+        SB A, Bd;
+        if (pending) {
+            A.im = B.sv[h.x].w;
+            A.dyn = dp.x != UNSET;
+            if (plane) {
+                zero_sb(Bd);
+            } else {
+                const float4 v = B.sv[h.y];
+                Bd.im = v.w;
+                Bd.dyn = dp.y != UNSET;
+                if (!Bd.dyn) {
+                    Bd.v = xyz(v);
+                    Bd.w = xyz(B.sw[h.y]);
+                }
+            }
+            if (!A.dyn) {
+                A.v = xyz(B.sv[h.x]);
+                A.w = xyz(B.sw[h.x]);
+            }
+        }
+        // tune bits 4-6: poll back-off level (bit 0, gathering a chunk before solving, is a
+        // policy of the counter-based protocol and is ignored here)
         const uint32_t nap = (tune >> 4) & 7u;
-        bool ready = !pending;
-        uint64_t last_ready = 0;
-        uint32_t still = 0;
         uint32_t spins = 0;
         const uint64_t t_start = wall_clock64();
         for (;;) {
-            if (pending && !ready) {
-                ready = true;
-                if (dp.x != UNSET)
-                    ready = __hip_atomic_load(bdone + h.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tA;
-                if (ready && dp.y != UNSET)
-                    ready = __hip_atomic_load(bdone + h.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= tB;
-            }
-            bool go = true;
-            if (gather) {
-                const uint64_t rset = __ballot(pending && ready);
-                if (rset != last_ready) {
-                    last_ready = rset;
-                    still = 0;
-                } else {
-                    ++still;
+            // cheap poll first: the last word a writer stores (w.z) of each dependency; the
+            // full load below then verifies every word's version
+            bool armed = pending;
+            if (armed && A.dyn) armed = vver(vload(B.vb + (size_t)h.x * VB_WORDS + 5)) == tA;
+            if (armed && Bd.dyn) armed = vver(vload(B.vb + (size_t)h.y * VB_WORDS + 5)) == tB;
+            if (armed) {
+                float4 L[4];
+                bool ok = true;
+                if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A);
+                if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd) && ok;
+                for (uint32_t k = 0; k < np; ++k) {
+                    const uint64_t* q = vlam + ((size_t)k * cap + s) * VL_WORDS;
+                    const uint64_t l0 = vload(q), l1 = vload(q + 1), l2 = vload(q + 2);
+                    L[k] = make_float4(vval(l0), vval(l1), vval(l2), 0.0f);
+                    ok = ok && vver(l0) == sw && vver(l1) == sw && vver(l2) == sw;
                 }
-                go = __ballot(pending && !ready) == 0ull || (rset != 0ull && still >= 4u);
-            }
-            if (pending && ready && go) {
-                solve_slot<true>(B, sp, R, h, s, cap, sweep0 + sw == 0);
-                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): body / impulse stores complete first
-                if (dp.x != UNSET) __hip_atomic_fetch_add(bdone + h.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (dp.y != UNSET) __hip_atomic_fetch_add(bdone + h.y, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                pending = false;
+                if (ok) {
+                    solve_core(R, s, cap, np, plane, mu, sweep0 + sw == 0, A, Bd, L);
+                    for (uint32_t k = 0; k < np; ++k) {
+                        uint64_t* q = vlam + ((size_t)k * cap + s) * VL_WORDS;
+                        vstore(q, vpack(L[k].x, sw + 1u));
+                        vstore(q + 1, vpack(L[k].y, sw + 1u));
+                        vstore(q + 2, vpack(L[k].z, sw + 1u));
+                    }
+                    if (A.dyn) vbody_store(B.vb, h.x, tA + 1u, A);
+                    if (Bd.dyn) vbody_store(B.vb, h.y, tB + 1u, Bd);
+                    pending = false;
+                }
             }
             if (!__ballot(pending)) break;
             if ((++spins & 255u) == 0u) {
@@ -997,8 +1108,9 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
     float res = 0.0f;
-    if (s < M) {
-        const uint4 h = R.hdr[s];
+    if (s < M && cnt->sticky == 0u) {  // rows exist only when the preparation ran
+        uint4 h = R.hdr[s];
+        h.z = min(h.z, 4u);
         const bool plane = (h.y & PLANE_BIT) != 0;
         SB A, Bd;
         load_vel<false>(B, h.x, A);
@@ -1121,11 +1233,6 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 // ---------------------------------------------------------------- host launchers
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 
-__global__ void k_flow_reset(uint32_t* __restrict__ bdone, uint32_t n, Counters* cnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) cnt->queue = 0u;
-    if (i < n) bdone[i] = 0u;
-}
 
 void launch_counters_begin(Counters* cnt, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt);
@@ -1247,13 +1354,15 @@ void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R,
                      uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
                      uint32_t max_groups, uint32_t tune, hipStream_t s) {
     if (!nsweeps) return;
-    hipLaunchKernelGGL(k_flow_reset, grid_for(n_bodies), dim3(256), 0, s, bdone, n_bodies, cnt);
+    const uint32_t nt = n_bodies > cap ? n_bodies : cap;
+    hipLaunchKernelGGL(k_flow_init, grid_for(nt), dim3(256), 0, s, B, R, R.vlam, n_bodies, cap, cnt);
     const uint64_t chunks = ((uint64_t)nsweeps * cap + 63) / 64;
     uint64_t groups = (chunks + 3) / 4;
     if (groups > max_groups) groups = max_groups;
     if (groups < 1) groups = 1;
-    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, cnt, cap, sweep0, nsweeps, bdone,
-                       spin_limit, tune);
+    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, cnt, cap, sweep0, nsweeps,
+                       R.vlam, spin_limit, tune);
+    hipLaunchKernelGGL(k_flow_finish, grid_for(nt), dim3(256), 0, s, B, R, R.vlam, n_bodies, cap, cnt);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {

@@ -23,6 +23,7 @@ struct SolverRows {
     uint32_t* fid;
     uint2* dep;  // per slot: (deg << 8 | rank) of body a and body b in the colour order, UNSET if not dynamic
     uint32_t* colour;  // per slot: the manifold's colour (kept by the next step's incremental colouring)
+    uint64_t* vlam;    // persistent sweep: versioned impulse words [(k * cap + s) * 4 + c]
 };
 // Steps between full recolourings (in between, persisting contacts keep their colour).
 constexpr uint32_t RECOLOUR_PERIOD = 64;

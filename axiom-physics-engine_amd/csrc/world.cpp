@@ -194,6 +194,7 @@ void free_bodies(axm_world_s* w) {
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
+    w->mem.release(B.vb);
     w->mem.release(B.world);
     w->mem.release(B.world_first);
     w->mem.release(w->key);
@@ -220,6 +221,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
+    AXM_TRY(w->mem.get(B.vb, (size_t)cap * 8));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
     AXM_TRY(w->mem.get(w->key, cap));
@@ -274,6 +276,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.fid);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
+    w->mem.release(R.vlam);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -284,6 +287,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.dep, cap));
     AXM_TRY(w->mem.get(R.colour, cap));
+    AXM_TRY(w->mem.get(R.vlam, (size_t)cap * 16));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -308,7 +312,9 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint4* hdr = nullptr;
     uint2* dep = nullptr;
     uint32_t* colour = nullptr;
+    uint64_t* vlam = nullptr;
     AXM_TRY(w->mem.get(colour, cap));
+    AXM_TRY(w->mem.get(vlam, (size_t)cap * 16));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(np, cap));
@@ -336,6 +342,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.hdr);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
+    w->mem.release(R.vlam);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
     w->pc_key[set] = tkey;
@@ -349,6 +356,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     R.hdr = hdr;
     R.dep = dep;
     R.colour = colour;
+    R.vlam = vlam;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
@@ -558,10 +566,14 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     uint32_t it = sweep ? 0u : max_it;
     if (sweep && o.colour_launches) {
         AXM_TRY(read_counters(w));
-        const uint32_t K = std::min(w->h_cnt->max_colour, MAX_SORT_COLOURS);
-        off.resize(K + 1);
-        AXM_HIP(hipMemcpyAsync(off.data(), w->col_off, 4ull * (K + 1), hipMemcpyDeviceToHost, s));
-        AXM_HIP(hipStreamSynchronize(s));
+        if (w->h_cnt->sticky != 0u) {
+            off.assign(1, 0u);  // the step already failed: its rows were not prepared, launch nothing
+        } else {
+            const uint32_t K = std::min(w->h_cnt->max_colour, MAX_SORT_COLOURS);
+            off.resize(K + 1);
+            AXM_HIP(hipMemcpyAsync(off.data(), w->col_off, 4ull * (K + 1), hipMemcpyDeviceToHost, s));
+            AXM_HIP(hipStreamSynchronize(s));
+        }
     }
     while (it < max_it) {
         uint32_t seg = max_it - it;
