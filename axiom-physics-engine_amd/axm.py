@@ -146,7 +146,7 @@ def lib():
     L.axm_mlp_adam_step.argtypes = [P, f32p, C.c_float, C.c_float, C.c_float, C.c_float]
     L.axm_mlp_train.argtypes = [P, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, f32p]
     L.axm_world_set_mlp.argtypes = [P, P]
-    L.axm_world_set_sweep_config.argtypes = [P, C.c_uint32, C.c_uint32]
+    L.axm_world_set_sweep_config.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
     L.axm_world_set_feature_logging.argtypes = [P, C.c_int]
     L.axm_world_get_features.argtypes = [P, f32p, f32p, C.c_uint32, u32p]
     L.axm_rng_stream.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_float, C.c_float, f32p]
@@ -379,8 +379,8 @@ class World:
         self._mlp = mlp
         _check(self._L.axm_world_set_mlp(self._h, mlp._h if mlp is not None else None), "axm_world_set_mlp")
 
-    def set_sweep_config(self, groups=64, policy=0x10):
-        _check(self._L.axm_world_set_sweep_config(self._h, groups, policy), "axm_world_set_sweep_config")
+    def set_sweep_config(self, groups=0, lds_kib=0, policy=0x10):
+        _check(self._L.axm_world_set_sweep_config(self._h, groups, lds_kib, policy), "axm_world_set_sweep_config")
 
     def set_feature_logging(self, enable=True):
         _check(self._L.axm_world_set_feature_logging(self._h, 1 if enable else 0), "axm_world_set_feature_logging")

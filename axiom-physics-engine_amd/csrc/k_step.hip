@@ -783,12 +783,13 @@ __device__ __forceinline__ void japply(const Dir& J, float lam, SB& A, SB& Bd) {
         Bd.w = f3{fmaf(lam, J.ib.x, Bd.w.x), fmaf(lam, J.ib.y, Bd.w.y), fmaf(lam, J.ib.z, Bd.w.z)};
     }
 }
-// Direction `dir` (0 n, 1 t1, 2 t2) of the point whose rows start at J (stride cap); the
-// .w of the first row is returned in w0.  Against a plane b and ib are zero and not read.
-__device__ __forceinline__ Dir load_dir(const float4* J, uint32_t cap, int dir, bool plane, float& w0, float& w1,
+// Direction `dir` (0 n, 1 t1, 2 t2) of the point whose rows start at J, consecutive
+// rows `rs` float4 apart; the .w of the first row is returned in w0.  Against a plane
+// b and ib are zero and not read.
+__device__ __forceinline__ Dir load_dir(const float4* J, size_t rs, int dir, bool plane, float& w0, float& w1,
                                         float& w2, float& w3, float& w4) {
-    const float4* q = J + (size_t)(5 * dir) * cap;
-    const float4 r0 = q[0], r1 = q[(size_t)cap], r3 = q[3 * (size_t)cap];
+    const float4* q = J + (size_t)(5 * dir) * rs;
+    const float4 r0 = q[0], r1 = q[rs], r3 = q[3 * rs];
     Dir D;
     D.d = xyz(r0);
     D.a = xyz(r1);
@@ -800,7 +801,7 @@ __device__ __forceinline__ Dir load_dir(const float4* J, uint32_t cap, int dir, 
         D.b = D.ib = f3{0.0f, 0.0f, 0.0f};
         w2 = w4 = 0.0f;
     } else {
-        const float4 r2 = q[2 * (size_t)cap], r4 = q[4 * (size_t)cap];
+        const float4 r2 = q[2 * rs], r4 = q[4 * rs];
         D.b = xyz(r2);
         D.ib = xyz(r4);
         w2 = r2.w;
@@ -810,27 +811,32 @@ __device__ __forceinline__ Dir load_dir(const float4* J, uint32_t cap, int dir, 
 }
 
 // One visit of a manifold on registers: warm start (first sweep only), friction
-// rows of every point, then normal rows.  L[k] holds the point impulses in and out.
-__device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
-                                           float mu, bool first, SB& A, SB& Bd, float4 L[4]) {
+// rows of every point, then normal rows.  Point k's rows start at J + k * ps, rows
+// `rs` apart; L[k] holds the point impulses in and out.
+__device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps, uint32_t np, bool plane, float mu,
+                                           bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w1, w2, w3, w4;
     if (first) {
-        for (uint32_t k = 0; k < np; ++k) {
-            const float4* J = R.J + (size_t)k * JROWS * cap + s;
-            const Dir Dn = load_dir(J, cap, 0, plane, w0, w1, w2, w3, w4);
-            const Dir D1 = load_dir(J, cap, 1, plane, w0, w1, w2, w3, w4);
-            const Dir D2 = load_dir(J, cap, 2, plane, w0, w1, w2, w3, w4);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (k >= np) break;
+            const float4* P = J + (size_t)k * ps;
+            const Dir Dn = load_dir(P, rs, 0, plane, w0, w1, w2, w3, w4);
+            const Dir D1 = load_dir(P, rs, 1, plane, w0, w1, w2, w3, w4);
+            const Dir D2 = load_dir(P, rs, 2, plane, w0, w1, w2, w3, w4);
             japply(Dn, L[k].x, A, Bd);
             japply(D1, L[k].y, A, Bd);
             japply(D2, L[k].z, A, Bd);
         }
     }
-    for (uint32_t k = 0; k < np; ++k) {
-        const float4* J = R.J + (size_t)k * JROWS * cap + s;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= np) break;
+        const float4* P = J + (size_t)k * ps;
         const float maxf = mu * L[k].x;
         {
             float mT1;
-            const Dir D1 = load_dir(J, cap, 1, plane, mT1, w1, w2, w3, w4);
+            const Dir D1 = load_dir(P, rs, 1, plane, mT1, w1, w2, w3, w4);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
@@ -839,7 +845,7 @@ __device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint
         }
         {
             float mT2;
-            const Dir D2 = load_dir(J, cap, 2, plane, mT2, w1, w2, w3, w4);
+            const Dir D2 = load_dir(P, rs, 2, plane, mT2, w1, w2, w3, w4);
             float nl = fmaf(-mT2, jvel(D2, A, Bd), L[k].z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].z;
@@ -847,16 +853,24 @@ __device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint
             japply(D2, d, A, Bd);
         }
     }
-    for (uint32_t k = 0; k < np; ++k) {
-        const float4* J = R.J + (size_t)k * JROWS * cap + s;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= np) break;
+        const float4* P = J + (size_t)k * ps;
         float mN, x1, x2, tg, x4;
-        const Dir Dn = load_dir(J, cap, 0, plane, mN, x1, x2, tg, x4);
+        const Dir Dn = load_dir(P, rs, 0, plane, mN, x1, x2, tg, x4);
         float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L[k].x);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L[k].x;
         L[k].x = nl;
         japply(Dn, d, A, Bd);
     }
+}
+// The same with the rows of solve slot s in the SoA row arrays (row r of point k at
+// J[(k * JROWS + r) * cap + s]).
+__device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
+                                           float mu, bool first, SB& A, SB& Bd, float4 L[4]) {
+    solve_rows(R.J + s, cap, (size_t)JROWS * cap, np, plane, mu, first, A, Bd, L);
 }
 
 // One visit of the manifold in solve slot s with plain loads and stores (one
@@ -889,25 +903,33 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
     solve_slot(B, sp, R, R.hdr[s], s, cap, first != 0);
 }
 
-// Persistent sweep kernel.  Work item t = sweep * M + slot, handed out in
-// increasing order in chunks of 64 (one wave).  An item may run once every
-// earlier visit of its dynamic bodies is complete: the visit of slot s in sweep
-// k is visit number k * deg(x) + rank(x, s) of body x, with rank = colours of
-// x's manifolds below s's colour (the per-body colour masks of the colouring),
-// so each body sees exactly the update sequence of the colour-by-colour sweep
-// without a grid-wide barrier.
+// Persistent sweep kernel.  An item is the visit of one manifold slot in one
+// sweep.  It may run once every earlier visit of its dynamic bodies is complete:
+// the visit of slot s in sweep k is visit number k * deg(x) + rank(x, s) of body
+// x, with rank = colours of x's manifolds below s's colour (the per-body colour
+// masks of the colouring), so each body sees exactly the update sequence of the
+// colour-by-colour sweep without a grid-wide barrier.
+//
+// Ownership: the slots are cut into chunks of 64; workgroup g (one wave) owns
+// chunks g, g + G, g + 2G, ... and lane l of it slot 64 * chunk + l, in every
+// sweep.  A lane walks its own items in increasing (sweep, slot) order on its
+// own, so the smallest unfinished item of the launch always has its
+// dependencies complete and the walk cannot deadlock while all G waves are
+// resident (the host sizes G from the occupancy of the kernel).  Because a lane
+// is the only one to visit its slots, the Jacobian rows and the impulses of its
+// points are staged once per launch into LDS (up to `ppl` points per lane, the
+// rest is read from and written to the row arrays) and never go back to memory
+// between sweeps; only body velocities move between waves.
 //
 // Versioned words: during the launch each body's solver velocity lives in six
-// 64-bit words (value bits | visits completed << 32) and each point impulse in
-// three (value | sweeps completed << 32).  A 64-bit store is single-copy atomic,
-// so a reader that loads a body's six words and finds every version equal to
-// the visit number it needs holds exactly that visit's velocities: the poll is
-// the data load, and a writer publishes with plain stores (no store drain, no
-// counter update).  Chunks are taken in order by running waves, so the smallest
-// pending item always has its dependencies complete (no deadlock); a bounded
-// spin sets `err` instead of hanging.
-constexpr uint32_t VB_WORDS = 8;  // words per body (6 used)
-constexpr uint32_t VL_WORDS = 4;  // words per point impulse (3 used)
+// 64-bit words (value bits | visits completed << 32).  A 64-bit store is
+// single-copy atomic, so a reader that loads a body's six words and finds every
+// version equal to the visit number it needs holds exactly that visit's
+// velocities, and a writer publishes with plain stores (no store drain, no
+// counter).  A bounded wait sets STICKY_TIMEOUT instead of hanging.
+constexpr uint32_t VB_WORDS = 8;    // words per body (6 used)
+constexpr uint32_t OWN_REC = 16;    // float4 per staged point: JROWS rows + impulses
+constexpr uint32_t OWN_WAVE = 64;   // threads per workgroup of the sweep kernel
 
 __device__ __forceinline__ uint64_t vpack(float v, uint32_t ver) {
     return ((uint64_t)ver << 32) | (uint64_t)__float_as_uint(v);
@@ -940,12 +962,9 @@ __device__ __forceinline__ void vbody_store(uint64_t* vb, uint32_t i, uint32_t v
     vstore(p + 5, vpack(x.w.z, ver));
 }
 
-// Before a launch: versioned copies (version 0) of the solver velocities and of the
-// impulses in the rows; resets the chunk queue.
-__global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, uint64_t* __restrict__ vlam, uint32_t n,
-                                                   uint32_t cap, Counters* cnt) {
+// Before a launch: versioned copies (version 0) of the solver velocities.
+__global__ void __launch_bounds__(256) k_flow_init(Bodies B, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) cnt->queue = 0u;
     if (i < n) {
         const float4 v = B.sv[i], w = B.sw[i];
         SB x;
@@ -953,21 +972,9 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, uint6
         x.w = xyz(w);
         vbody_store(B.vb, i, 0u, x);
     }
-    // the rows (and their point counts) exist only when this step's preparation ran
-    if (i < cnt->m_eff && cnt->sticky == 0u) {
-        const uint32_t np = min(R.hdr[i].z, 4u);
-        for (uint32_t k = 0; k < np; ++k) {
-            const float4 L = R.lam[(size_t)k * cap + i];
-            uint64_t* q = vlam + ((size_t)k * cap + i) * VL_WORDS;
-            q[0] = vpack(L.x, 0u);
-            q[1] = vpack(L.y, 0u);
-            q[2] = vpack(L.z, 0u);
-        }
-    }
 }
-// After a launch: versioned words back into the solver velocities and the rows.
-__global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, const uint64_t* __restrict__ vlam,
-                                                     uint32_t n, uint32_t cap, const Counters* __restrict__ cnt) {
+// After a launch: versioned words back into the solver velocities.
+__global__ void __launch_bounds__(256) k_flow_finish(Bodies B, uint32_t n, const Counters* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && cnt->sticky == 0u) {
         const float4 v = B.sv[i];
@@ -977,129 +984,198 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
             B.sw[i] = make_float4(vval(p[3]), vval(p[4]), vval(p[5]), 0.0f);
         }
     }
-    if (i < cnt->m_eff && cnt->sticky == 0u) {
-        const uint32_t np = min(R.hdr[i].z, 4u);
-        for (uint32_t k = 0; k < np; ++k) {
-            const uint64_t* q = vlam + ((size_t)k * cap + i) * VL_WORDS;
-            R.lam[(size_t)k * cap + i] = make_float4(vval(q[0]), vval(q[1]), vval(q[2]), 0.0f);
-        }
-    }
 }
 
-__global__ void __launch_bounds__(256) k_pgs_flow(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
-                                                  uint32_t sweep0, uint32_t nsweeps, uint64_t* __restrict__ vlam,
-                                                  uint32_t spin_limit, uint32_t tune) {
-    const uint32_t lane = __lane_id();
+// Row r of point k of the manifold in slot s is R.J[(k * JROWS + r) * cap + s]; a
+// staged point q of a lane keeps row r at lds[(r * ppl + q) * 64 + lane] and its
+// impulses at row JROWS.  Rows 2 and 4 of each direction (the B side) are not
+// staged against a plane, where the solve does not read them.
+__global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, SolverRows R, Counters* cnt,
+                                                      uint32_t cap, uint32_t sweep0, uint32_t nsweeps, uint32_t ppl,
+                                                      uint32_t spin_limit, uint32_t tune) {
+    extern __shared__ float4 own_lds[];
+    const uint32_t lane = threadIdx.x;
     const uint32_t M = cnt->m_eff;
     if (cnt->max_colour > 64u) {
         // dependency ranks come from 64-bit colour masks: hand the step back for per-colour launches
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->sticky, STICKY_FLOW_COLOURS);
+        if (blockIdx.x == 0 && lane == 0) atomicOr(&cnt->sticky, STICKY_FLOW_COLOURS);
         return;
     }
-    if (cnt->sticky != 0u) return;
-    uint32_t* queue = &cnt->queue;
+    if (cnt->sticky != 0u || nsweeps == 0u) return;
     uint32_t* err = &cnt->sticky;
-    const uint64_t total = (uint64_t)nsweeps * M;
-    const uint64_t nchunks = (total + 63) / 64;
-    const float mu = sp.friction;
-    for (;;) {
-        uint32_t chunk = 0;
-        if (lane == 0) chunk = atomicAdd(queue, 1u);
-        chunk = __shfl(chunk, 0, 64);
-        if ((uint64_t)chunk >= nchunks) break;
-        const uint64_t t = (uint64_t)chunk * 64 + lane;
-        bool pending = t < total;
-        uint32_t sw = 0, s = 0;
-        uint4 h = make_uint4(0, 0, 0, 0);
-        uint2 dp = make_uint2(UNSET, UNSET);
-        if (pending) {
-            sw = (uint32_t)(t / M);
-            s = (uint32_t)(t - (uint64_t)sw * M);
-            h = R.hdr[s];
-            dp = R.dep[s];
+    const uint32_t G = gridDim.x, g = blockIdx.x;
+    const uint32_t nchunks = (M + OWN_WAVE - 1) / OWN_WAVE;
+    const uint32_t own = g < nchunks ? (nchunks - 1 - g) / G + 1 : 0u;
+    // owned slots of this lane: j < nv (only the last owned chunk can be partial)
+    const uint32_t nv = own == 0u ? 0u : own - (((g + (own - 1) * G) * OWN_WAVE + lane) >= M ? 1u : 0u);
+    const size_t rs = (size_t)ppl * OWN_WAVE;  // LDS row stride (float4)
+    float4* mine = own_lds + lane;
+
+    // stage rows and impulses of the owned points
+    {
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < nv; ++j) {
+            const uint32_t s = (g + j * G) * OWN_WAVE + lane;
+            const uint4 h = R.hdr[s];
+            const uint32_t np = min(h.z, 4u);
+            const bool plane = (h.y & PLANE_BIT) != 0u;
+            if (off + np <= ppl) {
+                for (uint32_t k = 0; k < np; ++k) {
+                    const float4* src = R.J + (size_t)k * JROWS * cap + s;
+                    float4* dst = mine + (size_t)(off + k) * OWN_WAVE;
+                    for (uint32_t r = 0; r < JROWS; ++r) {
+                        const uint32_t rr = r % 5u;
+                        if (plane && (rr == 2u || rr == 4u)) continue;
+                        dst[r * rs] = src[(size_t)r * cap];
+                    }
+                    dst[JROWS * rs] = R.lam[(size_t)k * cap + s];
+                }
+            }
+            off += np;
         }
-        const uint32_t tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
-        const uint32_t tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
-        const bool plane = (h.y & PLANE_BIT) != 0u;
-        const uint32_t np = min(h.z, 4u);
-        // the parts of the visit that no other visit writes: inverse masses, a static partner
-        SB A, Bd;
-        if (pending) {
-            A.im = B.sv[h.x].w;
+    }
+
+    const float mu = sp.friction;
+    const uint32_t total = nv * nsweeps;
+    uint32_t done = 0, sw = 0, j = 0, off = 0;
+    uint32_t s = g * OWN_WAVE + lane;
+    uint4 h = make_uint4(0, 0, 0, 0), nh = h;
+    uint2 dp = make_uint2(UNSET, UNSET), ndp = dp;
+    if (total > 0u) {
+        h = R.hdr[s];
+        dp = R.dep[s];
+        const uint32_t s1 = ((nv > 1u ? 1u : 0u) * G + g) * OWN_WAVE + lane;
+        if (total > 1u) {
+            nh = R.hdr[s1];
+            ndp = R.dep[s1];
+        }
+    }
+    bool fresh = true;
+    uint32_t tA = 0, tB = 0, np = 0;
+    bool plane = false;
+    SB A, Bd;
+    zero_sb(A);
+    zero_sb(Bd);
+    // tune bits 4-6: poll back-off level; bit 0: poll every word instead of one word first
+    const uint32_t nap = (tune >> 4) & 7u;
+    const bool full_poll = (tune & 1u) != 0u;
+    uint32_t spins = 0;
+    const uint64_t t_start = wall_clock64();
+    for (;;) {
+        const bool pending = done < total;
+        if (pending && fresh) {
+            // the parts of the visit that no other visit writes: inverse masses, a static partner
+            fresh = false;
+            tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
+            tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
+            plane = (h.y & PLANE_BIT) != 0u;
+            np = min(h.z, 4u);
+            const float4 va = B.sv[h.x];
+            A.im = va.w;
             A.dyn = dp.x != UNSET;
+            if (!A.dyn) {
+                A.v = xyz(va);
+                A.w = xyz(B.sw[h.x]);
+            }
             if (plane) {
                 zero_sb(Bd);
             } else {
-                const float4 v = B.sv[h.y];
-                Bd.im = v.w;
+                const float4 vb = B.sv[h.y];
+                Bd.im = vb.w;
                 Bd.dyn = dp.y != UNSET;
                 if (!Bd.dyn) {
-                    Bd.v = xyz(v);
+                    Bd.v = xyz(vb);
                     Bd.w = xyz(B.sw[h.y]);
                 }
             }
-            if (!A.dyn) {
-                A.v = xyz(B.sv[h.x]);
-                A.w = xyz(B.sw[h.x]);
-            }
         }
-        // tune bits 4-6: poll back-off level (bit 0, gathering a chunk before solving, is a
-        // policy of the counter-based protocol and is ignored here)
-        const uint32_t nap = (tune >> 4) & 7u;
-        uint32_t spins = 0;
-        const uint64_t t_start = wall_clock64();
-        for (;;) {
-            // cheap poll first: the last word a writer stores (w.z) of each dependency; the
-            // full load below then verifies every word's version
-            bool armed = pending;
+        bool armed = pending;
+        if (!full_poll) {
+            // cheap poll first: the last word a writer stores (w.z) of each dependency
             if (armed && A.dyn) armed = vver(vload(B.vb + (size_t)h.x * VB_WORDS + 5)) == tA;
             if (armed && Bd.dyn) armed = vver(vload(B.vb + (size_t)h.y * VB_WORDS + 5)) == tB;
-            if (armed) {
+        }
+        if (armed) {
+            bool ok = true;
+            if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A);
+            if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd) && ok;
+            if (ok) {
+                const bool first = sweep0 + sw == 0u;
                 float4 L[4];
-                bool ok = true;
-                if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A);
-                if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd) && ok;
-                for (uint32_t k = 0; k < np; ++k) {
-                    const uint64_t* q = vlam + ((size_t)k * cap + s) * VL_WORDS;
-                    const uint64_t l0 = vload(q), l1 = vload(q + 1), l2 = vload(q + 2);
-                    L[k] = make_float4(vval(l0), vval(l1), vval(l2), 0.0f);
-                    ok = ok && vver(l0) == sw && vver(l1) == sw && vver(l2) == sw;
+                // the point loops run over a fixed 4 so that L stays in registers
+                if (off + np <= ppl) {
+                    float4* P = mine + (size_t)off * OWN_WAVE;
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) L[k] = P[JROWS * rs + (size_t)k * OWN_WAVE];
+                    solve_rows(P, rs, OWN_WAVE, np, plane, mu, first, A, Bd, L);
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) P[JROWS * rs + (size_t)k * OWN_WAVE] = L[k];
+                } else {
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) L[k] = R.lam[(size_t)k * cap + s];
+                    solve_core(R, s, cap, np, plane, mu, first, A, Bd, L);
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
                 }
-                if (ok) {
-                    solve_core(R, s, cap, np, plane, mu, sweep0 + sw == 0, A, Bd, L);
-                    for (uint32_t k = 0; k < np; ++k) {
-                        uint64_t* q = vlam + ((size_t)k * cap + s) * VL_WORDS;
-                        vstore(q, vpack(L[k].x, sw + 1u));
-                        vstore(q + 1, vpack(L[k].y, sw + 1u));
-                        vstore(q + 2, vpack(L[k].z, sw + 1u));
-                    }
-                    if (A.dyn) vbody_store(B.vb, h.x, tA + 1u, A);
-                    if (Bd.dyn) vbody_store(B.vb, h.y, tB + 1u, Bd);
-                    pending = false;
+                if (A.dyn) vbody_store(B.vb, h.x, tA + 1u, A);
+                if (Bd.dyn) vbody_store(B.vb, h.y, tB + 1u, Bd);
+                // next item of this lane: the next owned slot, or the first one of the next sweep
+                ++done;
+                off += np;
+                if (++j == nv) {
+                    j = 0;
+                    ++sw;
+                    off = 0;
                 }
-            }
-            if (!__ballot(pending)) break;
-            if ((++spins & 255u) == 0u) {
-                bool stop = false;
-                if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
-                    if (pending) atomicOr(err, STICKY_TIMEOUT);  // wait longer than spin_limit microseconds
-                    stop = true;
-                } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                    stop = true;  // another wave already gave up: drain
+                s = (g + j * G) * OWN_WAVE + lane;
+                h = nh;
+                dp = ndp;
+                fresh = true;
+                if (done + 1u < total) {
+                    const uint32_t j1 = j + 1u == nv ? 0u : j + 1u;
+                    const uint32_t s1 = (g + j1 * G) * OWN_WAVE + lane;
+                    nh = R.hdr[s1];
+                    ndp = R.dep[s1];
                 }
-                if (stop) pending = false;
-                if (!__ballot(pending)) break;
-            }
-            switch (nap) {
-            case 0: break;
-            case 1: __builtin_amdgcn_s_sleep(1); break;
-            case 2: __builtin_amdgcn_s_sleep(2); break;
-            case 3: __builtin_amdgcn_s_sleep(4); break;
-            case 4: __builtin_amdgcn_s_sleep(8); break;
-            case 5: __builtin_amdgcn_s_sleep(16); break;
-            default: __builtin_amdgcn_s_sleep(32); break;
             }
         }
+        if (!__ballot(done < total)) break;
+        if ((++spins & 255u) == 0u) {
+            bool stop = false;
+            if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
+                if (done < total) atomicOr(err, STICKY_TIMEOUT);  // waited longer than spin_limit microseconds
+                stop = true;
+            } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                stop = true;  // another wave already gave up: drain
+            }
+            if (stop) return;
+        }
+        switch (nap) {
+        case 0: break;
+        case 1: __builtin_amdgcn_s_sleep(1); break;
+        case 2: __builtin_amdgcn_s_sleep(2); break;
+        case 3: __builtin_amdgcn_s_sleep(4); break;
+        case 4: __builtin_amdgcn_s_sleep(8); break;
+        case 5: __builtin_amdgcn_s_sleep(16); break;
+        default: __builtin_amdgcn_s_sleep(32); break;
+        }
+    }
+
+    // impulses of the staged points back to the rows
+    uint32_t woff = 0;
+    for (uint32_t jj = 0; jj < nv; ++jj) {
+        const uint32_t ss = (g + jj * G) * OWN_WAVE + lane;
+        const uint32_t n = min(R.hdr[ss].z, 4u);
+        if (woff + n <= ppl)
+            for (uint32_t k = 0; k < n; ++k) {
+                const float4 L = mine[(JROWS * rs) + (size_t)(woff + k) * OWN_WAVE];
+                R.lam[(size_t)k * cap + ss] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+        woff += n;
     }
 }
 
@@ -1350,19 +1426,22 @@ void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, co
                       hipStream_t s) {
     hipLaunchKernelGGL(k_mlp_apply, grid_for(cap), dim3(256), 0, s, R, cnt, cap, Y, sp);
 }
+int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes) {
+    if (hipFuncSetAttribute((const void*)k_pgs_own, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds_bytes) !=
+        hipSuccess)
+        return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pgs_own, OWN_WAVE, lds_bytes) != hipSuccess) return 0;
+    return nb;
+}
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
-                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
-                     uint32_t max_groups, uint32_t tune, hipStream_t s) {
+                     uint32_t sweep0, uint32_t nsweeps, Counters* cnt, uint32_t spin_limit, uint32_t groups,
+                     uint32_t lds_bytes, uint32_t tune, hipStream_t s) {
     if (!nsweeps) return;
-    const uint32_t nt = n_bodies > cap ? n_bodies : cap;
-    hipLaunchKernelGGL(k_flow_init, grid_for(nt), dim3(256), 0, s, B, R, R.vlam, n_bodies, cap, cnt);
-    const uint64_t chunks = ((uint64_t)nsweeps * cap + 63) / 64;
-    uint64_t groups = (chunks + 3) / 4;
-    if (groups > max_groups) groups = max_groups;
-    if (groups < 1) groups = 1;
-    hipLaunchKernelGGL(k_pgs_flow, dim3((uint32_t)groups), dim3(256), 0, s, B, sp, R, cnt, cap, sweep0, nsweeps,
-                       R.vlam, spin_limit, tune);
-    hipLaunchKernelGGL(k_flow_finish, grid_for(nt), dim3(256), 0, s, B, R, R.vlam, n_bodies, cap, cnt);
+    hipLaunchKernelGGL(k_flow_init, grid_for(n_bodies), dim3(256), 0, s, B, n_bodies);
+    hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
+                       nsweeps, lds_bytes / (OWN_REC * (uint32_t)sizeof(float4) * OWN_WAVE), spin_limit, tune);
+    hipLaunchKernelGGL(k_flow_finish, grid_for(n_bodies), dim3(256), 0, s, B, n_bodies, cnt);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {

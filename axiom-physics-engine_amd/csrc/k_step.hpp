@@ -23,7 +23,6 @@ struct SolverRows {
     uint32_t* fid;
     uint2* dep;  // per slot: (deg << 8 | rank) of body a and body b in the colour order, UNSET if not dynamic
     uint32_t* colour;  // per slot: the manifold's colour (kept by the next step's incremental colouring)
-    uint64_t* vlam;    // persistent sweep: versioned impulse words [(k * cap + s) * 4 + c]
 };
 // Steps between full recolourings (in between, persisting contacts keep their colour).
 constexpr uint32_t RECOLOUR_PERIOD = 64;
@@ -75,9 +74,14 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
 // Writes the learned warm start (network outputs Y, rows 4 * slot + point) into the impulse rows.
 void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, const float* Y, const StepParams& sp,
                       hipStream_t s);
+// Persistent sweep: `groups` workgroups of one wave, each with `lds_bytes` of LDS for
+// its staged rows; all of them must be resident at once (see pgs_own_setup).
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
-                     uint32_t sweep0, uint32_t nsweeps, uint32_t* bdone, Counters* cnt, uint32_t spin_limit,
-                     uint32_t max_groups, uint32_t tune, hipStream_t s);
+                     uint32_t sweep0, uint32_t nsweeps, Counters* cnt, uint32_t spin_limit, uint32_t groups,
+                     uint32_t lds_bytes, uint32_t tune, hipStream_t s);
+// Raises the sweep kernel's dynamic-LDS limit to max_lds_bytes and returns how many of
+// its workgroups with lds_bytes each fit on one CU (0 on error).
+int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes);
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,

@@ -117,13 +117,14 @@ struct axm_world_s {
     axm_mlp_s* mlp = nullptr;
     float last_dt = 0.0f;
 
-    // persistent sweep launch shape: workgroups in flight and the wait policy
-    // (bit 0 gather a chunk before solving, bits 4-6 poll back-off).  The sweep
-    // is bound by its dependency chain, not by parallelism: few waves in flight
-    // (64 workgroups) poll less and beat a full machine of spinning waves
-    // (100k-body pile: 1.9 ms per 20-sweep launch at 64 vs 3.1 ms at 4096).
-    uint32_t sweep_groups = 64;
+    // persistent sweep launch shape: workgroups (one wave each, all resident at
+    // once), LDS per workgroup for the staged rows of its slots, and the wait
+    // policy (bit 0 poll every word, bits 4-6 poll back-off).  Set at creation
+    // to one workgroup per CU with the largest LDS a workgroup may have.
+    uint32_t sweep_groups = 0;
+    uint32_t sweep_lds = 0;
     uint32_t sweep_tune = 0x10;
+    uint32_t max_lds = 0;       // device limit of LDS per workgroup
 
     uint32_t* col = nullptr;
     uint64_t *prio = nullptr, *bmax0 = nullptr, *bmax1 = nullptr, *bmask = nullptr;
@@ -276,7 +277,6 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.fid);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
-    w->mem.release(R.vlam);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -287,7 +287,6 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.dep, cap));
     AXM_TRY(w->mem.get(R.colour, cap));
-    AXM_TRY(w->mem.get(R.vlam, (size_t)cap * 16));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -312,9 +311,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint4* hdr = nullptr;
     uint2* dep = nullptr;
     uint32_t* colour = nullptr;
-    uint64_t* vlam = nullptr;
     AXM_TRY(w->mem.get(colour, cap));
-    AXM_TRY(w->mem.get(vlam, (size_t)cap * 16));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(np, cap));
@@ -342,7 +339,6 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.hdr);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
-    w->mem.release(R.vlam);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
     w->pc_key[set] = tkey;
@@ -356,7 +352,6 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     R.hdr = hdr;
     R.dep = dep;
     R.colour = colour;
-    R.vlam = vlam;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
@@ -585,8 +580,8 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
                 for (size_t col = 0; col + 1 < off.size(); ++col)
                     launch_pgs(w->B, sp, R, off[col], off[col + 1], rc, it + k == 0 ? 1u : 0u, s);
         } else {
-            launch_pgs_flow(w->B, sp, R, rc, n, it, seg, w->bdone, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_tune,
-                            s);
+            launch_pgs_flow(w->B, sp, R, rc, n, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds,
+                            w->sweep_tune, s);
         }
         AXM_TRY(prof_mark(w, nev, o.capture));
         it += seg;
@@ -812,6 +807,15 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     // colouring kernel: one workgroup per CU, all resident (its grid barrier needs that);
     // the occupancy query confirms at least one 256-thread workgroup fits per CU
     w->colour_blocks = (uint32_t)w->cu_count * (uint32_t)std::min(colour_blocks_per_cu(), 1);
+    // sweep kernel: the largest LDS a workgroup may have (at most 160 KiB), one workgroup per CU
+    w->max_lds = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 160u * 1024u);
+    w->sweep_lds = w->max_lds & ~16383u;
+    const int own_per_cu = pgs_own_setup(w->max_lds, w->sweep_lds);
+    if (own_per_cu < 1) {
+        delete w;
+        return AXM_GPU_OPERATION_FAILED;
+    }
+    w->sweep_groups = (uint32_t)w->cu_count;
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
         delete w;
         return AXM_GPU_OPERATION_FAILED;
@@ -1044,8 +1048,8 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             if (w->prof_on) {
                 AXM_HIP(hipGraphLaunch(w->gpre[c], w->stream));
                 AXM_HIP(hipEventRecord(w->gprof_ev[2 * c], w->stream));
-                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->n, 0, iters, w->bdone, w->cnt, SWEEP_SPIN_LIMIT_US,
-                                w->sweep_groups, w->sweep_tune, w->stream);
+                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->n, 0, iters, w->cnt, SWEEP_SPIN_LIMIT_US,
+                                w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream);
                 AXM_HIP(hipEventRecord(w->gprof_ev[2 * c + 1], w->stream));
                 AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
             } else {
@@ -1260,10 +1264,20 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
     return AXM_SUCCESS;
 }
 
-axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t policy) {
+axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy) {
     if (!w) return AXM_NULL_POINTER;
-    if (groups == 0 || groups > 65536u || (policy & ~0x71u) != 0u) return AXM_INVALID_PARAMETER;
+    if ((policy & ~0x71u) != 0u) return AXM_INVALID_PARAMETER;
+    const uint32_t lds = lds_kib ? lds_kib * 1024u : (w->max_lds & ~16383u);
+    if (lds > w->max_lds) return AXM_INVALID_PARAMETER;
+    (void)hipSetDevice(w->desc.device);
+    // every workgroup of the sweep must be resident at once (its items wait on each other)
+    const int per_cu = pgs_own_setup(w->max_lds, lds);
+    if (per_cu < 1) return AXM_INVALID_PARAMETER;
+    const uint32_t resident = (uint32_t)per_cu * (uint32_t)w->cu_count;
+    if (groups == 0) groups = std::min<uint32_t>((uint32_t)w->cu_count, resident);
+    if (groups > resident) return AXM_INVALID_PARAMETER;
     w->sweep_groups = groups;
+    w->sweep_lds = lds;
     w->sweep_tune = policy;
     w->drop_graphs();
     return AXM_SUCCESS;

@@ -165,8 +165,8 @@ def main():
     log("scene config %d: %d bodies" % (cfg, bodies.n))
     w = axm.World(desc)
     if os.environ.get("AXM_SWEEP"):
-        g, pol = (int(x, 0) for x in os.environ["AXM_SWEEP"].split(","))
-        w.set_sweep_config(g, pol)
+        g, lds, pol = (int(x, 0) for x in os.environ["AXM_SWEEP"].split(","))
+        w.set_sweep_config(g, lds, pol)
     w.set_bodies(bodies)
     t0 = time.time()
     for k in range(args.warmup):
