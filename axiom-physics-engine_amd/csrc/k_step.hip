@@ -942,13 +942,15 @@ __device__ __forceinline__ uint64_t vload(const uint64_t* p) {
 __device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
     __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Loads body i's versioned velocities; true when all six carry version `ver`.
-__device__ __forceinline__ bool vbody_load(const uint64_t* vb, uint32_t i, uint32_t ver, SB& x) {
+// Loads body i's versioned velocities; true when all six carry version `ver`.  `seen`
+// receives the version of the last word.
+__device__ __forceinline__ bool vbody_load(const uint64_t* vb, uint32_t i, uint32_t ver, SB& x, uint32_t& seen) {
     const uint64_t* p = vb + (size_t)i * VB_WORDS;
     const uint64_t w0 = vload(p), w1 = vload(p + 1), w2 = vload(p + 2), w3 = vload(p + 3), w4 = vload(p + 4),
                    w5 = vload(p + 5);
     x.v = f3{vval(w0), vval(w1), vval(w2)};
     x.w = f3{vval(w3), vval(w4), vval(w5)};
+    seen = vver(w5);
     return vver(w0) == ver && vver(w1) == ver && vver(w2) == ver && vver(w3) == ver && vver(w4) == ver &&
            vver(w5) == ver;
 }
@@ -1059,6 +1061,8 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // tune bits 4-6: poll back-off level; bit 0: poll every word instead of one word first
     const uint32_t nap = (tune >> 4) & 7u;
     const bool full_poll = (tune & 1u) != 0u;
+    const bool near_mode = (tune & 2u) != 0u;
+    uint32_t seenA = 0, seenB = 0;  // versions of the dependencies seen by the last poll
     uint32_t spins = 0;
     const uint64_t t_start = wall_clock64();
     for (;;) {
@@ -1068,6 +1072,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             fresh = false;
             tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
             tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
+            seenA = seenB = 0u;
             plane = (h.y & PLANE_BIT) != 0u;
             np = min(h.z, 4u);
             const float4 va = B.sv[h.x];
@@ -1089,16 +1094,26 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 }
             }
         }
+        // a lane whose dependencies are each at most one visit away loads the full words
+        // at once (near mode); otherwise it polls the last word a writer stores (w.z) of
+        // each dependency first, both in flight together
+        bool full = full_poll || (near_mode && tA - seenA <= 1u && tB - seenB <= 1u);
         bool armed = pending;
-        if (!full_poll) {
-            // cheap poll first: the last word a writer stores (w.z) of each dependency
-            if (armed && A.dyn) armed = vver(vload(B.vb + (size_t)h.x * VB_WORDS + 5)) == tA;
-            if (armed && Bd.dyn) armed = vver(vload(B.vb + (size_t)h.y * VB_WORDS + 5)) == tB;
+        if (pending && !full) {
+            uint32_t va = tA, vb = tB;
+            if (A.dyn) va = vver(vload(B.vb + (size_t)h.x * VB_WORDS + 5));
+            if (Bd.dyn) vb = vver(vload(B.vb + (size_t)h.y * VB_WORDS + 5));
+            seenA = va;
+            seenB = vb;
+            armed = va == tA && vb == tB;
         }
         if (armed) {
             bool ok = true;
-            if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A);
-            if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd) && ok;
+            uint32_t va = tA, vb = tB;
+            if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A, va);
+            if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd, vb) && ok;
+            seenA = va;
+            seenB = vb;
             if (ok) {
                 const bool first = sweep0 + sw == 0u;
                 float4 L[4];

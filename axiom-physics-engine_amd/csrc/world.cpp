@@ -119,8 +119,9 @@ struct axm_world_s {
 
     // persistent sweep launch shape: workgroups (one wave each, all resident at
     // once), LDS per workgroup for the staged rows of its slots, and the wait
-    // policy (bit 0 poll every word, bits 4-6 poll back-off).  Set at creation
-    // to one workgroup per CU with the largest LDS a workgroup may have.
+    // policy (bit 0 poll every word, bit 1 poll every word once one visit away,
+    // bits 4-6 poll back-off).  Set at creation to one workgroup per CU with the
+    // largest LDS a workgroup may have.
     uint32_t sweep_groups = 0;
     uint32_t sweep_lds = 0;
     uint32_t sweep_tune = 0x10;
@@ -1266,7 +1267,7 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
 
 axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy) {
     if (!w) return AXM_NULL_POINTER;
-    if ((policy & ~0x71u) != 0u) return AXM_INVALID_PARAMETER;
+    if ((policy & ~0x73u) != 0u) return AXM_INVALID_PARAMETER;
     const uint32_t lds = lds_kib ? lds_kib * 1024u : (w->max_lds & ~16383u);
     if (lds > w->max_lds) return AXM_INVALID_PARAMETER;
     (void)hipSetDevice(w->desc.device);
