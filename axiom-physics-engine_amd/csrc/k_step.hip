@@ -348,20 +348,22 @@ __device__ __forceinline__ uint32_t local_id(const Bodies& B, uint32_t b) {
 }
 
 // Grid-wide barrier for a kernel whose workgroups are all resident (grid sized
-// from the occupancy query).  Every wave fences at agent scope before arriving
-// (its own stores complete and leave the XCD's L2: __syncthreads does not wait
-// for other waves' stores) and again after leaving (stale L1/L2 lines are
-// invalidated).  A bounded wait sets STICKY_TIMEOUT instead of hanging.
+// from the occupancy query).  Every wave first waits until its own stores are
+// complete in the XCD's L2 (a workgroup barrier does not wait for them); then one
+// thread per workgroup writes the L2 back and arrives (agent-scope release), waits
+// for the last arrival and invalidates the stale L1/L2 lines for the whole
+// workgroup (agent-scope acquire).  One writeback per workgroup instead of one per
+// wave: the writeback of an XCD's L2 is the expensive part of a fence.  A bounded
+// wait sets STICKY_TIMEOUT instead of hanging.
 __device__ __forceinline__ void grid_sync(GridBar* bar, uint32_t nblocks, Counters* cnt) {
-    __threadfence();
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t arrived = atomicAdd(&bar->count, 1u);
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
             __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            atomicAdd(&bar->gen, 1u);
+            __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const uint64_t t0 = wall_clock64();  // 100 MHz constant clock
             while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
@@ -371,10 +373,10 @@ __device__ __forceinline__ void grid_sync(GridBar* bar, uint32_t nblocks, Counte
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
     }
     __syncthreads();
-    __threadfence();
 }
 
 // Jones-Plassmann colouring in one persistent launch.  Round r: every
