@@ -379,7 +379,7 @@ class World:
         self._mlp = mlp
         _check(self._L.axm_world_set_mlp(self._h, mlp._h if mlp is not None else None), "axm_world_set_mlp")
 
-    def set_sweep_config(self, groups=0, lds_kib=0, policy=0x10):
+    def set_sweep_config(self, groups=0, lds_kib=0, policy=0x12):
         _check(self._L.axm_world_set_sweep_config(self._h, groups, lds_kib, policy), "axm_world_set_sweep_config")
 
     def set_feature_logging(self, enable=True):

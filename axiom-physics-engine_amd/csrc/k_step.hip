@@ -579,13 +579,20 @@ __device__ __forceinline__ f3 vrel(const SB& A, const SB& Bd, f3 rA, f3 rB) {
     return (Bd.v + cross(Bd.w, rB)) - (A.v + cross(A.w, rA));
 }
 
+// Kind of a visit, for grouping the sweep's work (colour c < 64, four kinds per
+// colour: against a plane or not, one point or more).
+__device__ __forceinline__ uint32_t visit_class(uint32_t c, bool plane, uint32_t np) {
+    return c * 4u + (plane ? 0u : 2u) + (np > 1u ? 1u : 0u);
+}
+
 __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
                                               uint32_t cap, const uint2* __restrict__ m_ab,
                                               const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
                                               const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
-                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
+                                              Counters* __restrict__ cnt, float* __restrict__ feat,
+                                              uint32_t* __restrict__ cls) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
     if (s >= M || cnt->sticky != 0u) return;  // a failed step must not touch the impulse buffers
@@ -602,7 +609,8 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         const uint32_t c = col[m];
         R.colour[s] = c;
         uint2 dp = make_uint2(UNSET, UNSET);
-        if (c < 64u) {
+        // (more than 64 colours: no dependency words; the sweep hands the step to per-colour launches)
+        if (c < 64u && cnt->max_colour <= 64u) {
             // a body's colour mask must hold one bit per manifold of the body, or the
             // persistent sweep's visit counts could never be reached
             const unsigned long long below = (1ull << c) - 1ull;
@@ -616,6 +624,12 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                 if ((uint32_t)__popcll(mk) != bdeg[ab.y]) atomicOr(&cnt->sticky, STICKY_INTERNAL);
                 dp.y = ((uint32_t)__popcll(mk) << 8) | (uint32_t)__popcll(mk & below);
             }
+            // who visits which body in which colour: the sweep's visits pass velocities on by it
+            if (A.dyn) B.vslot[(size_t)ab.x * 64u + c] = s << 1;
+            if (Bd.dyn) B.vslot[(size_t)ab.y * 64u + c] = (s << 1) | 1u;
+            // slots per colour and kind (plane or not, one point or more): the sweep hands a
+            // wave slots of one kind where it can, so that its lanes take the same path
+            atomicAdd(&cls[visit_class(c, plane, np)], 1u);
         }
         R.dep[s] = dp;
     }
@@ -923,15 +937,23 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 // rest is read from and written to the row arrays) and never go back to memory
 // between sweeps; only body velocities move between waves.
 //
-// Versioned words: during the launch each body's solver velocity lives in six
-// 64-bit words (value bits | visits completed << 32).  A 64-bit store is
-// single-copy atomic, so a reader that loads a body's six words and finds every
-// version equal to the visit number it needs holds exactly that visit's
-// velocities, and a writer publishes with plain stores (no store drain, no
-// counter).  A bounded wait sets STICKY_TIMEOUT instead of hanging.
-constexpr uint32_t VB_WORDS = 8;    // words per body (6 used)
-constexpr uint32_t OWN_REC = 16;    // float4 per staged point: JROWS rows + impulses
+// Inboxes: during the launch a body's solver velocity travels from one visit of
+// the body to the next.  Each slot has an inbox of 16 64-bit words (one 128-byte
+// line) holding, for the slot's next visit, body a's velocity in words 0..4 and
+// 6 and body b's in words 8..12 and 7, each word the value bits | the body's
+// visits completed << 32.  A visit waits until its inbox words carry the visit
+// numbers it needs (a 64-bit word is single-copy atomic, so matching versions
+// mean matching values), solves, and writes each dynamic body's new velocity into
+// the inbox of that body's next visit (the manifold of the next colour in the
+// body's colour mask, or its first one for the next sweep) with plain stores.
+// The last word of each side (6 for a, 7 for b, the last one a writer stores)
+// sits in one 16-byte pair: a poll reads that pair in one access as a hint and
+// loads the rest once both are in; the full load checks every version itself.
+// A bounded wait sets STICKY_TIMEOUT instead of hanging.
+constexpr uint32_t IB_WORDS = 16;   // inbox words per slot
+constexpr uint32_t IB_HINT = 6;     // words 6 (side a) and 7 (side b): the 16-byte poll pair
 constexpr uint32_t OWN_WAVE = 64;   // threads per workgroup of the sweep kernel
+constexpr uint32_t NO_VISIT = 0xFFFFFFFFu;  // version no visit count reaches
 
 __device__ __forceinline__ uint64_t vpack(float v, uint32_t ver) {
     return ((uint64_t)ver << 32) | (uint64_t)__float_as_uint(v);
@@ -944,59 +966,157 @@ __device__ __forceinline__ uint64_t vload(const uint64_t* p) {
 __device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
     __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Loads body i's versioned velocities; true when all six carry version `ver`.  `seen`
-// receives the version of the last word.
-__device__ __forceinline__ bool vbody_load(const uint64_t* vb, uint32_t i, uint32_t ver, SB& x, uint32_t& seen) {
-    const uint64_t* p = vb + (size_t)i * VB_WORDS;
-    const uint64_t w0 = vload(p), w1 = vload(p + 1), w2 = vload(p + 2), w3 = vload(p + 3), w4 = vload(p + 4),
-                   w5 = vload(p + 5);
-    x.v = f3{vval(w0), vval(w1), vval(w2)};
-    x.w = f3{vval(w3), vval(w4), vval(w5)};
-    seen = vver(w5);
-    return vver(w0) == ver && vver(w1) == ver && vver(w2) == ver && vver(w3) == ver && vver(w4) == ver &&
-           vver(w5) == ver;
+// Word k (0..5) of side `side` of the inbox of slot s.
+__device__ __forceinline__ size_t vin_word(uint32_t s, uint32_t side, uint32_t k) {
+    return (size_t)s * IB_WORDS + (k < 5u ? side * 8u + k : IB_HINT + side);
 }
-__device__ __forceinline__ void vbody_store(uint64_t* vb, uint32_t i, uint32_t ver, const SB& x) {
-    uint64_t* p = vb + (size_t)i * VB_WORDS;
-    vstore(p, vpack(x.v.x, ver));
-    vstore(p + 1, vpack(x.v.y, ver));
-    vstore(p + 2, vpack(x.v.z, ver));
-    vstore(p + 3, vpack(x.w.x, ver));
-    vstore(p + 4, vpack(x.w.y, ver));
-    vstore(p + 5, vpack(x.w.z, ver));
+// Writes a velocity into side `side` of slot s's inbox, the hint word last.
+__device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t side, uint32_t ver, f3 v, f3 w) {
+    vstore(inbox + vin_word(s, side, 0), vpack(v.x, ver));
+    vstore(inbox + vin_word(s, side, 1), vpack(v.y, ver));
+    vstore(inbox + vin_word(s, side, 2), vpack(v.z, ver));
+    vstore(inbox + vin_word(s, side, 3), vpack(w.x, ver));
+    vstore(inbox + vin_word(s, side, 4), vpack(w.y, ver));
+    vstore(inbox + vin_word(s, side, 5), vpack(w.z, ver));
+}
+// The two hint words of slot s (versions of sides a and b) in one coherent 16-byte
+// access; a torn read only delays or repeats the full load, which checks every word.
+__device__ __forceinline__ uint2 vin_hint(const uint64_t* inbox, uint32_t s) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(inbox), 0, 0x7FFFFFFF, 0x00020000);
+    // aux: sc0 | sc1, the cache policy of a system-coherent load (bypasses the non-coherent caches)
+    const auto r = __builtin_amdgcn_raw_buffer_load_b128(rs, s * (IB_WORDS * 8u) + IB_HINT * 8u, 0, 1 | 16);
+    return make_uint2(r[1], r[3]);
+}
+// The visit after the one of colour c of a body with colour mask mk: the next
+// colour up, or the body's first colour (next sweep).
+__device__ __forceinline__ uint32_t next_colour(unsigned long long mk, uint32_t c) {
+    const unsigned long long above = c >= 63u ? 0ull : mk & ~((2ull << c) - 1ull);
+    return (uint32_t)__ffsll(above ? above : mk) - 1u;
 }
 
-// Before a launch: versioned copies (version 0) of the solver velocities.
-__global__ void __launch_bounds__(256) k_flow_init(Bodies B, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const float4 v = B.sv[i], w = B.sw[i];
-        SB x;
-        x.v = xyz(v);
-        x.w = xyz(w);
-        vbody_store(B.vb, i, 0u, x);
-    }
-}
-// After a launch: versioned words back into the solver velocities.
-__global__ void __launch_bounds__(256) k_flow_finish(Bodies B, uint32_t n, const Counters* __restrict__ cnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && cnt->sticky == 0u) {
-        const float4 v = B.sv[i];
-        if (v.w > 0.0f) {
-            const uint64_t* p = B.vb + (size_t)i * VB_WORDS;
-            B.sv[i] = make_float4(vval(p[0]), vval(p[1]), vval(p[2]), v.w);
-            B.sw[i] = make_float4(vval(p[3]), vval(p[4]), vval(p[5]), 0.0f);
+// Before a launch, per slot: the next visit of each dynamic body, and the inbox
+// words: a body's first visit (rank 0) gets the body's solver velocity with
+// version 0, every other inbox a version no visit waits for.
+__global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const uint64_t* __restrict__ bmask,
+                                                   Counters* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ col_off, uint32_t* __restrict__ cls,
+                                                   uint32_t sweep0, bool by_kind) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    // the rows exist only when this step's preparation ran
+    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    const uint4 h = R.hdr[s];
+    const uint2 dp = R.dep[s];
+    const uint32_t c = R.colour[s];
+    if (sweep0 == 0u) {
+        if (by_kind && c < 64u && cnt->max_colour <= 64u) {
+            // work order of the sweep: within each colour, the slots of one kind together
+            // (the order inside a kind is arbitrary: visits of one colour share no body)
+            const uint32_t k = visit_class(c, (h.y & PLANE_BIT) != 0u, h.z);
+            uint32_t base = col_off[c];
+            for (uint32_t q = c * 4u; q < k; ++q) base += cls[q];
+            const uint32_t q = base + atomicAdd(&cls[256u + k], 1u);
+            if (q < cnt->m_eff) R.perm[q] = s;
+            else atomicOr(&cnt->sticky, STICKY_INTERNAL);  // kind counts disagree with the colour offsets
+        } else {
+            R.perm[s] = s;  // the solve order
         }
     }
+    uint2 nx = make_uint2(UNSET, UNSET);
+    if (dp.x != UNSET) {
+        nx.x = B.vslot[(size_t)h.x * 64u + next_colour(bmask[h.x], c)];
+        const bool first = (dp.x & 0xFFu) == 0u;
+        vin_store(R.inbox, s, 0u, first ? 0u : NO_VISIT, xyz(B.sv[h.x]), xyz(B.sw[h.x]));
+    }
+    if (dp.y != UNSET) {
+        nx.y = B.vslot[(size_t)h.y * 64u + next_colour(bmask[h.y], c)];
+        const bool first = (dp.y & 0xFFu) == 0u;
+        vin_store(R.inbox, s, 1u, first ? 0u : NO_VISIT, xyz(B.sv[h.y]), xyz(B.sw[h.y]));
+    }
+    R.next[s] = nx;
+}
+// After a launch: each dynamic body's final velocity is in the inbox of its first
+// visit (the last visit of the last sweep passed it on there).
+__global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, const Counters* __restrict__ cnt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    const uint4 h = R.hdr[s];
+    const uint2 dp = R.dep[s];
+    const uint64_t* ib = R.inbox;
+    if (dp.x != UNSET && (dp.x & 0xFFu) == 0u) {
+        B.sv[h.x] = make_float4(vval(ib[vin_word(s, 0, 0)]), vval(ib[vin_word(s, 0, 1)]), vval(ib[vin_word(s, 0, 2)]),
+                                B.sv[h.x].w);
+        B.sw[h.x] = make_float4(vval(ib[vin_word(s, 0, 3)]), vval(ib[vin_word(s, 0, 4)]), vval(ib[vin_word(s, 0, 5)]),
+                                0.0f);
+    }
+    if (dp.y != UNSET && (dp.y & 0xFFu) == 0u) {
+        B.sv[h.y] = make_float4(vval(ib[vin_word(s, 1, 0)]), vval(ib[vin_word(s, 1, 1)]), vval(ib[vin_word(s, 1, 2)]),
+                                B.sv[h.y].w);
+        B.sw[h.y] = make_float4(vval(ib[vin_word(s, 1, 3)]), vval(ib[vin_word(s, 1, 4)]), vval(ib[vin_word(s, 1, 5)]),
+                                0.0f);
+    }
 }
 
-// Row r of point k of the manifold in slot s is R.J[(k * JROWS + r) * cap + s]; a
-// staged point q of a lane keeps row r at lds[(r * ppl + q) * 64 + lane] and its
-// impulses at row JROWS.  Rows 2 and 4 of each direction (the B side) are not
-// staged against a plane, where the solve does not read them.
+// What a visit needs of its slot besides the rows: the slot, body indices and point count,
+// dependency words, the next visits of its bodies, inverse masses and the
+// velocity of a static partner (constant during the sweep).
+struct Item {
+    uint32_t s;  // the slot
+    uint4 h;
+    uint2 dp, nx;
+    float imA, imB;
+    f3 sv, sw;  // static partner's velocity (at most one side of a manifold is static)
+};
+__device__ __forceinline__ Item item_global(const Bodies& B, const SolverRows& R, uint32_t s) {
+    Item it;
+    it.s = s;
+    it.h = R.hdr[s];
+    it.dp = R.dep[s];
+    it.nx = R.next[s];
+    const bool plane = (it.h.y & PLANE_BIT) != 0u;
+    const float4 va = B.sv[it.h.x];
+    const float4 vb = plane ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : B.sv[it.h.y];
+    it.imA = va.w;
+    it.imB = vb.w;
+    const uint32_t st = it.dp.x == UNSET ? it.h.x : it.h.y;  // the static side, if any
+    const bool has_static = it.dp.x == UNSET || (!plane && it.dp.y == UNSET);
+    it.sv = has_static ? xyz(it.dp.x == UNSET ? va : vb) : f3{0.0f, 0.0f, 0.0f};
+    it.sw = has_static ? xyz(B.sw[st]) : f3{0.0f, 0.0f, 0.0f};
+    return it;
+}
+// staged slot record: 4 float4 at rec[r * 64], r = 0..3
+__device__ __forceinline__ void item_put(float4* rec, const Item& it) {
+    rec[0] = make_float4(__uint_as_float(it.h.x), __uint_as_float(it.h.y), __uint_as_float(it.h.z),
+                         __uint_as_float(it.s));
+    rec[OWN_WAVE] = make_float4(__uint_as_float(it.dp.x), __uint_as_float(it.dp.y), __uint_as_float(it.nx.x),
+                                __uint_as_float(it.nx.y));
+    rec[2 * OWN_WAVE] = make_float4(it.imA, it.imB, it.sv.x, it.sv.y);
+    rec[3 * OWN_WAVE] = make_float4(it.sv.z, it.sw.x, it.sw.y, it.sw.z);
+}
+__device__ __forceinline__ Item item_get(const float4* rec) {
+    const float4 r0 = rec[0], r1 = rec[OWN_WAVE], r2 = rec[2 * OWN_WAVE], r3 = rec[3 * OWN_WAVE];
+    Item it;
+    it.h = make_uint4(__float_as_uint(r0.x), __float_as_uint(r0.y), __float_as_uint(r0.z), 0u);
+    it.s = __float_as_uint(r0.w);
+    it.dp = make_uint2(__float_as_uint(r1.x), __float_as_uint(r1.y));
+    it.nx = make_uint2(__float_as_uint(r1.z), __float_as_uint(r1.w));
+    it.imA = r2.x;
+    it.imB = r2.y;
+    it.sv = f3{r2.z, r2.w, r3.x};
+    it.sw = f3{r3.y, r3.z, r3.w};
+    return it;
+}
+
+constexpr uint32_t OWN_SLOTS = 16;  // slots per lane with a staged record
+
+// LDS of a wave: records of its first min(owned chunks, OWN_SLOTS) slots per lane
+// (record j of a lane at lds[(j * 4 + r) * 64 + lane]), then the staged points:
+// point q of a lane keeps row r at lds[P0 + (r * ppl + q) * 64 + lane] (rows
+// 0..JROWS-1 the Jacobian rows of R.J[(k * JROWS + r) * cap + s], row JROWS its
+// impulses).  Slots and points beyond what fits are read from the row arrays.
 __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, SolverRows R, Counters* cnt,
-                                                      uint32_t cap, uint32_t sweep0, uint32_t nsweeps, uint32_t ppl,
-                                                      uint32_t spin_limit, uint32_t tune) {
+                                                      uint32_t cap, uint32_t sweep0, uint32_t nsweeps,
+                                                      uint32_t lds_f4, uint32_t spin_limit, uint32_t tune) {
     extern __shared__ float4 own_lds[];
     const uint32_t lane = threadIdx.x;
     const uint32_t M = cnt->m_eff;
@@ -1012,115 +1132,122 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     const uint32_t own = g < nchunks ? (nchunks - 1 - g) / G + 1 : 0u;
     // owned slots of this lane: j < nv (only the last owned chunk can be partial)
     const uint32_t nv = own == 0u ? 0u : own - (((g + (own - 1) * G) * OWN_WAVE + lane) >= M ? 1u : 0u);
-    const size_t rs = (size_t)ppl * OWN_WAVE;  // LDS row stride (float4)
-    float4* mine = own_lds + lane;
+    const uint32_t nrec = min(own, OWN_SLOTS);
+    const uint32_t p0 = nrec * 4u * OWN_WAVE;
+    const uint32_t ppl = lds_f4 > p0 ? (lds_f4 - p0) / ((JROWS + 1u) * OWN_WAVE) : 0u;
+    const size_t rs = (size_t)ppl * OWN_WAVE;  // LDS row stride of the points (float4)
+    float4* recs = own_lds + lane;
+    float4* mine = own_lds + p0 + lane;
+    // work item (g + jj * G) * 64 + lane of the launch is slot R.perm[item]
+    auto slot_of = [&](uint32_t jj) { return R.perm[(g + jj * G) * OWN_WAVE + lane]; };
 
-    // stage rows and impulses of the owned points
+    // stage slot records, rows and impulses
+    bool all_staged = own <= OWN_SLOTS;
     {
         uint32_t off = 0;
         for (uint32_t j = 0; j < nv; ++j) {
-            const uint32_t s = (g + j * G) * OWN_WAVE + lane;
-            const uint4 h = R.hdr[s];
-            const uint32_t np = min(h.z, 4u);
-            const bool plane = (h.y & PLANE_BIT) != 0u;
+            const uint32_t s = slot_of(j);
+            const Item it = item_global(B, R, s);
+            if (j < nrec) item_put(recs + (size_t)j * 4u * OWN_WAVE, it);
+            const uint32_t np = min(it.h.z, 4u);
             if (off + np <= ppl) {
                 for (uint32_t k = 0; k < np; ++k) {
                     const float4* src = R.J + (size_t)k * JROWS * cap + s;
                     float4* dst = mine + (size_t)(off + k) * OWN_WAVE;
-                    for (uint32_t r = 0; r < JROWS; ++r) {
-                        const uint32_t rr = r % 5u;
-                        if (plane && (rr == 2u || rr == 4u)) continue;
-                        dst[r * rs] = src[(size_t)r * cap];
-                    }
+#pragma unroll
+                    for (uint32_t r = 0; r < JROWS; ++r) dst[r * rs] = src[(size_t)r * cap];
                     dst[JROWS * rs] = R.lam[(size_t)k * cap + s];
                 }
+            } else {
+                all_staged = false;
             }
             off += np;
         }
     }
-
+    // The visits of a wave whose slots and points are all in LDS read nothing but
+    // the inbox words from memory; the general loop reads what is not staged from
+    // the row arrays.  Two instances, so that the first never waits on a memory
+    // load other than its poll (a wait for a load also waits for the stores before it).
     const float mu = sp.friction;
     const uint32_t total = nv * nsweeps;
-    uint32_t done = 0, sw = 0, j = 0, off = 0;
-    uint32_t s = g * OWN_WAVE + lane;
-    uint4 h = make_uint4(0, 0, 0, 0), nh = h;
-    uint2 dp = make_uint2(UNSET, UNSET), ndp = dp;
-    if (total > 0u) {
-        h = R.hdr[s];
-        dp = R.dep[s];
-        const uint32_t s1 = ((nv > 1u ? 1u : 0u) * G + g) * OWN_WAVE + lane;
-        if (total > 1u) {
-            nh = R.hdr[s1];
-            ndp = R.dep[s1];
-        }
-    }
-    bool fresh = true;
-    uint32_t tA = 0, tB = 0, np = 0;
-    bool plane = false;
-    SB A, Bd;
-    zero_sb(A);
-    zero_sb(Bd);
-    // tune bits 4-6: poll back-off level; bit 0: poll every word instead of one word first
+    // tune bits 4-6: poll back-off level; bit 1: poll the hint pair before the full load
+    // (inboxes within 2 GiB); bit 2: adaptive back-off (bit 0, grouping the work by kind, is
+    // applied by k_flow_init)
     const uint32_t nap = (tune >> 4) & 7u;
-    const bool full_poll = (tune & 1u) != 0u;
-    const bool near_mode = (tune & 2u) != 0u;
-    uint32_t seenA = 0, seenB = 0;  // versions of the dependencies seen by the last poll
-    uint32_t spins = 0;
+    const bool hint = (tune & 2u) != 0u && (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull,
+               adaptive = (tune & 4u) != 0u;
     const uint64_t t_start = wall_clock64();
-    for (;;) {
-        const bool pending = done < total;
-        if (pending && fresh) {
-            // the parts of the visit that no other visit writes: inverse masses, a static partner
-            fresh = false;
-            tA = dp.x == UNSET ? 0u : sw * (dp.x >> 8) + (dp.x & 0xFFu);
-            tB = dp.y == UNSET ? 0u : sw * (dp.y >> 8) + (dp.y & 0xFFu);
-            seenA = seenB = 0u;
-            plane = (h.y & PLANE_BIT) != 0u;
-            np = min(h.z, 4u);
-            const float4 va = B.sv[h.x];
-            A.im = va.w;
-            A.dyn = dp.x != UNSET;
-            if (!A.dyn) {
-                A.v = xyz(va);
-                A.w = xyz(B.sw[h.x]);
-            }
-            if (plane) {
-                zero_sb(Bd);
-            } else {
-                const float4 vb = B.sv[h.y];
-                Bd.im = vb.w;
-                Bd.dyn = dp.y != UNSET;
-                if (!Bd.dyn) {
-                    Bd.v = xyz(vb);
-                    Bd.w = xyz(B.sw[h.y]);
+    auto sweep_loop = [&]<bool STAGED>() -> bool {
+        uint32_t done = 0, sw = 0, j = 0, off = 0;
+        Item it;
+        it.s = 0u;
+        if (total > 0u) it = (STAGED || nrec > 0u) ? item_get(recs) : item_global(B, R, slot_of(0));
+        uint32_t s = it.s;
+        bool fresh = true;
+        uint32_t tA = 0, tB = 0, np = 0;
+        bool plane = false;
+        SB A, Bd;
+        zero_sb(A);
+        zero_sb(Bd);
+        uint32_t spins = 0, idle = 0;
+        for (;;) {
+            const bool pending = done < total;
+            if (pending && fresh) {
+                fresh = false;
+                tA = it.dp.x == UNSET ? 0u : sw * (it.dp.x >> 8) + (it.dp.x & 0xFFu);
+                tB = it.dp.y == UNSET ? 0u : sw * (it.dp.y >> 8) + (it.dp.y & 0xFFu);
+                plane = (it.h.y & PLANE_BIT) != 0u;
+                np = min(it.h.z, 4u);
+                A.im = it.imA;
+                A.dyn = it.dp.x != UNSET;
+                Bd.im = plane ? 0.0f : it.imB;
+                Bd.dyn = it.dp.y != UNSET;
+                if (!A.dyn) {
+                    A.v = it.sv;
+                    A.w = it.sw;
+                }
+                if (plane) {
+                    Bd.v = Bd.w = f3{0.0f, 0.0f, 0.0f};
+                } else if (!Bd.dyn) {
+                    Bd.v = it.sv;
+                    Bd.w = it.sw;
                 }
             }
-        }
-        // a lane whose dependencies are each at most one visit away loads the full words
-        // at once (near mode); otherwise it polls the last word a writer stores (w.z) of
-        // each dependency first, both in flight together
-        bool full = full_poll || (near_mode && tA - seenA <= 1u && tB - seenB <= 1u);
-        bool armed = pending;
-        if (pending && !full) {
-            uint32_t va = tA, vb = tB;
-            if (A.dyn) va = vver(vload(B.vb + (size_t)h.x * VB_WORDS + 5));
-            if (Bd.dyn) vb = vver(vload(B.vb + (size_t)h.y * VB_WORDS + 5));
-            seenA = va;
-            seenB = vb;
-            armed = va == tA && vb == tB;
-        }
-        if (armed) {
-            bool ok = true;
-            uint32_t va = tA, vb = tB;
-            if (A.dyn) ok = vbody_load(B.vb, h.x, tA, A, va);
-            if (Bd.dyn) ok = vbody_load(B.vb, h.y, tB, Bd, vb) && ok;
-            seenA = va;
-            seenB = vb;
+            // poll = load: all words of both bodies in this slot's inbox (one line), issued together
+            bool ok = false;
+            bool ready = pending;
+            if (pending && hint) {
+                // the hint pair first (one access); the full load below runs only for lanes
+                // whose hint words are in
+                const uint2 hv = vin_hint(R.inbox, s);
+                ready = ((hv.x == tA) | !A.dyn) & ((hv.y == tB) | !Bd.dyn);
+            }
+            if (ready) {
+                const uint64_t* ib = R.inbox;
+                const uint64_t a0 = vload(ib + vin_word(s, 0, 0)), a1 = vload(ib + vin_word(s, 0, 1)),
+                               a2 = vload(ib + vin_word(s, 0, 2)), a3 = vload(ib + vin_word(s, 0, 3)),
+                               a4 = vload(ib + vin_word(s, 0, 4)), a5 = vload(ib + vin_word(s, 0, 5));
+                const uint64_t b0 = vload(ib + vin_word(s, 1, 0)), b1 = vload(ib + vin_word(s, 1, 1)),
+                               b2 = vload(ib + vin_word(s, 1, 2)), b3 = vload(ib + vin_word(s, 1, 3)),
+                               b4 = vload(ib + vin_word(s, 1, 4)), b5 = vload(ib + vin_word(s, 1, 5));
+                // branch-free: every loaded word is consumed here, so no later instruction
+                // waits on the memory counter (which also counts the stores before it)
+                const uint32_t da = (vver(a0) ^ tA) | (vver(a1) ^ tA) | (vver(a2) ^ tA) | (vver(a3) ^ tA) |
+                                    (vver(a4) ^ tA) | (vver(a5) ^ tA);
+                const uint32_t db = (vver(b0) ^ tB) | (vver(b1) ^ tB) | (vver(b2) ^ tB) | (vver(b3) ^ tB) |
+                                    (vver(b4) ^ tB) | (vver(b5) ^ tB);
+                ok = ((da == 0u) | !A.dyn) & ((db == 0u) | !Bd.dyn);
+                const bool ta = ok & A.dyn, tb = ok & Bd.dyn;
+                A.v = f3{ta ? vval(a0) : A.v.x, ta ? vval(a1) : A.v.y, ta ? vval(a2) : A.v.z};
+                A.w = f3{ta ? vval(a3) : A.w.x, ta ? vval(a4) : A.w.y, ta ? vval(a5) : A.w.z};
+                Bd.v = f3{tb ? vval(b0) : Bd.v.x, tb ? vval(b1) : Bd.v.y, tb ? vval(b2) : Bd.v.z};
+                Bd.w = f3{tb ? vval(b3) : Bd.w.x, tb ? vval(b4) : Bd.w.y, tb ? vval(b5) : Bd.w.z};
+            }
             if (ok) {
                 const bool first = sweep0 + sw == 0u;
                 float4 L[4];
                 // the point loops run over a fixed 4 so that L stays in registers
-                if (off + np <= ppl) {
+                if (STAGED || off + np <= ppl) {
                     float4* P = mine + (size_t)off * OWN_WAVE;
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
@@ -1138,8 +1265,9 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
                 }
-                if (A.dyn) vbody_store(B.vb, h.x, tA + 1u, A);
-                if (Bd.dyn) vbody_store(B.vb, h.y, tB + 1u, Bd);
+                // pass the velocities on to each body's next visit
+                if (A.dyn) vin_store(R.inbox, it.nx.x >> 1, it.nx.x & 1u, tA + 1u, A.v, A.w);
+                if (Bd.dyn) vin_store(R.inbox, it.nx.y >> 1, it.nx.y & 1u, tB + 1u, Bd.v, Bd.w);
                 // next item of this lane: the next owned slot, or the first one of the next sweep
                 ++done;
                 off += np;
@@ -1148,45 +1276,46 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                     ++sw;
                     off = 0;
                 }
-                s = (g + j * G) * OWN_WAVE + lane;
-                h = nh;
-                dp = ndp;
                 fresh = true;
-                if (done + 1u < total) {
-                    const uint32_t j1 = j + 1u == nv ? 0u : j + 1u;
-                    const uint32_t s1 = (g + j1 * G) * OWN_WAVE + lane;
-                    nh = R.hdr[s1];
-                    ndp = R.dep[s1];
+                if (done < total) {
+                    it = (STAGED || j < nrec) ? item_get(recs + (size_t)j * 4u * OWN_WAVE)
+                                              : item_global(B, R, slot_of(j));
+                    s = it.s;
                 }
             }
-        }
-        if (!__ballot(done < total)) break;
-        if ((++spins & 255u) == 0u) {
-            bool stop = false;
-            if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
-                if (done < total) atomicOr(err, STICKY_TIMEOUT);  // waited longer than spin_limit microseconds
-                stop = true;
-            } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                stop = true;  // another wave already gave up: drain
+            if (!__ballot(done < total)) return true;
+            if ((++spins & 255u) == 0u) {
+                bool stop = false;
+                if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
+                    if (done < total) atomicOr(err, STICKY_TIMEOUT);  // waited longer than spin_limit microseconds
+                    stop = true;
+                } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    stop = true;  // another wave already gave up: drain
+                }
+                if (stop) return false;
             }
-            if (stop) return;
+            // adaptive back-off: a wave none of whose lanes got through sleeps longer each time
+            if (adaptive) idle = __ballot(ok) ? 0u : min(idle + 1u, 4u);
+            switch (min(nap + idle, 6u)) {
+            case 0: break;
+            case 1: __builtin_amdgcn_s_sleep(1); break;
+            case 2: __builtin_amdgcn_s_sleep(2); break;
+            case 3: __builtin_amdgcn_s_sleep(4); break;
+            case 4: __builtin_amdgcn_s_sleep(8); break;
+            case 5: __builtin_amdgcn_s_sleep(16); break;
+            default: __builtin_amdgcn_s_sleep(32); break;
+            }
         }
-        switch (nap) {
-        case 0: break;
-        case 1: __builtin_amdgcn_s_sleep(1); break;
-        case 2: __builtin_amdgcn_s_sleep(2); break;
-        case 3: __builtin_amdgcn_s_sleep(4); break;
-        case 4: __builtin_amdgcn_s_sleep(8); break;
-        case 5: __builtin_amdgcn_s_sleep(16); break;
-        default: __builtin_amdgcn_s_sleep(32); break;
-        }
-    }
+    };
+    const bool finished = __ballot(!all_staged) == 0ull ? sweep_loop.template operator()<true>()
+                                                        : sweep_loop.template operator()<false>();
+    if (!finished) return;
 
     // impulses of the staged points back to the rows
     uint32_t woff = 0;
     for (uint32_t jj = 0; jj < nv; ++jj) {
-        const uint32_t ss = (g + jj * G) * OWN_WAVE + lane;
-        const uint32_t n = min(R.hdr[ss].z, 4u);
+        const uint32_t ss = slot_of(jj);
+        const uint32_t n = min(jj < nrec ? __float_as_uint(recs[(size_t)jj * 4u * OWN_WAVE].z) : R.hdr[ss].z, 4u);
         if (woff + n <= ppl)
             for (uint32_t k = 0; k < n; ++k) {
                 const float4 L = mine[(JROWS * rs) + (size_t)(woff + k) * OWN_WAVE];
@@ -1288,9 +1417,13 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
 
 
 // Clears the per-step part of the control block.
-__global__ void k_counters_begin(Counters* cnt) {
+__global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ cls) {
     uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
     if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
+    if (cls) {
+        cls[threadIdx.x] = 0u;
+        cls[threadIdx.x + 256u] = 0u;
+    }
 }
 
 // Integration is skipped while any sticky error is set, so an overflowing step
@@ -1327,8 +1460,8 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 
 
-void launch_counters_begin(Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt);
+void launch_counters_begin(Counters* cnt, uint32_t* cls, hipStream_t s) {
+    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(256), 0, s, cnt, cls);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
@@ -1407,10 +1540,10 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
+                 const uint32_t* bdeg, Counters* cnt, float* feat, uint32_t* cls, hipStream_t s) {
     const PrevCache pc = make_prev(pca);
     hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, col, bmask, bdeg, cnt, feat);
+                       warm_in, col, bmask, bdeg, cnt, feat, cls);
 }
 
 // Learned warm start: the network predicts the correction to the previous
@@ -1451,14 +1584,15 @@ int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pgs_own, OWN_WAVE, lds_bytes) != hipSuccess) return 0;
     return nb;
 }
-void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, uint32_t n_bodies,
-                     uint32_t sweep0, uint32_t nsweeps, Counters* cnt, uint32_t spin_limit, uint32_t groups,
-                     uint32_t lds_bytes, uint32_t tune, hipStream_t s) {
+void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
+                     const uint32_t* col_off, uint32_t* cls, uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
+                     uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s) {
     if (!nsweeps) return;
-    hipLaunchKernelGGL(k_flow_init, grid_for(n_bodies), dim3(256), 0, s, B, n_bodies);
+    hipLaunchKernelGGL(k_flow_init, grid_for(cap), dim3(256), 0, s, B, R, bmask, cnt, col_off, cls, sweep0,
+                       (tune & 1u) != 0u);
     hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
-                       nsweeps, lds_bytes / (OWN_REC * (uint32_t)sizeof(float4) * OWN_WAVE), spin_limit, tune);
-    hipLaunchKernelGGL(k_flow_finish, grid_for(n_bodies), dim3(256), 0, s, B, n_bodies, cnt);
+                       nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune);
+    hipLaunchKernelGGL(k_flow_finish, grid_for(cap), dim3(256), 0, s, B, R, cnt);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {

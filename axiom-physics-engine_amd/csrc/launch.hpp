@@ -52,7 +52,7 @@ struct Bodies {
     float4* amax;
     float4* smin;  // AABBs in sorted (cell) order, w = body index bits
     float4* smax;  // w = 1 for a dynamic body
-    uint64_t* vb;  // persistent sweep: versioned velocity words, 8 per body
+    uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)
 };
 constexpr uint32_t PAIR_STASH = 24;  // partners kept per body by the counting pass
 

@@ -118,13 +118,14 @@ struct axm_world_s {
     float last_dt = 0.0f;
 
     // persistent sweep launch shape: workgroups (one wave each, all resident at
-    // once), LDS per workgroup for the staged rows of its slots, and the wait
-    // policy (bit 0 poll every word, bit 1 poll every word once one visit away,
-    // bits 4-6 poll back-off).  Set at creation to one workgroup per CU with the
-    // largest LDS a workgroup may have.
+    // once), LDS per workgroup for the staged rows of its slots, and the policy
+    // (bit 0 group each colour's work by kind of visit, bit 1 poll a 16-byte hint
+    // before the full load, bit 2 adaptive back-off, bits 4-6 poll back-off).
+    // Set at creation to one workgroup per CU with the largest LDS a workgroup
+    // may have.
     uint32_t sweep_groups = 0;
     uint32_t sweep_lds = 0;
-    uint32_t sweep_tune = 0x10;
+    uint32_t sweep_tune = 0x12;
     uint32_t max_lds = 0;       // device limit of LDS per workgroup
 
     uint32_t* col = nullptr;
@@ -132,6 +133,7 @@ struct axm_world_s {
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
     uint32_t* col_off = nullptr;
     uint32_t* bdone = nullptr;
+    uint32_t* cls = nullptr;  // per colour < 64 and kind of visit: slot counts (0..255) and fill cursors (256..511)
     GridBar* bar = nullptr;
 
     SolverRows R[2]{};
@@ -196,7 +198,7 @@ void free_bodies(axm_world_s* w) {
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
-    w->mem.release(B.vb);
+    w->mem.release(B.vslot);
     w->mem.release(B.world);
     w->mem.release(B.world_first);
     w->mem.release(w->key);
@@ -223,7 +225,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
-    AXM_TRY(w->mem.get(B.vb, (size_t)cap * 8));
+    AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
     AXM_TRY(w->mem.get(w->key, cap));
@@ -278,6 +280,9 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.fid);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
+    w->mem.release(R.inbox);
+    w->mem.release(R.next);
+    w->mem.release(R.perm);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -288,6 +293,9 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.dep, cap));
     AXM_TRY(w->mem.get(R.colour, cap));
+    AXM_TRY(w->mem.get(R.inbox, (size_t)cap * 16));
+    AXM_TRY(w->mem.get(R.next, cap));
+    AXM_TRY(w->mem.get(R.perm, cap));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -312,7 +320,13 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint4* hdr = nullptr;
     uint2* dep = nullptr;
     uint32_t* colour = nullptr;
+    uint64_t* inbox = nullptr;
+    uint2* next = nullptr;
+    uint32_t* perm = nullptr;
     AXM_TRY(w->mem.get(colour, cap));
+    AXM_TRY(w->mem.get(perm, cap));
+    AXM_TRY(w->mem.get(inbox, (size_t)cap * 16));
+    AXM_TRY(w->mem.get(next, cap));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(np, cap));
@@ -340,6 +354,9 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.hdr);
     w->mem.release(R.dep);
     w->mem.release(R.colour);
+    w->mem.release(R.inbox);
+    w->mem.release(R.next);
+    w->mem.release(R.perm);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
     w->pc_key[set] = tkey;
@@ -353,6 +370,9 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     R.hdr = hdr;
     R.dep = dep;
     R.colour = colour;
+    R.inbox = inbox;
+    R.next = next;
+    R.perm = perm;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
@@ -498,7 +518,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[0], s));
-    launch_counters_begin(w->cnt, s);
+    launch_counters_begin(w->cnt, w->cls, s);
     enqueue_broad(w, sp);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[1], s));
     if (o.flags & AXM_STEP_KEEP_STAGES) {
@@ -530,7 +550,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
     launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
-                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, s);
+                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, w->cls, s);
     if (w->mlp && w->desc.warm_start == AXM_WARM_MLP && !o.warm_in && w->feat) {
         launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
                            2u * (uint32_t)w->cu_count, s);
@@ -581,8 +601,8 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
                 for (size_t col = 0; col + 1 < off.size(); ++col)
                     launch_pgs(w->B, sp, R, off[col], off[col + 1], rc, it + k == 0 ? 1u : 0u, s);
         } else {
-            launch_pgs_flow(w->B, sp, R, rc, n, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds,
-                            w->sweep_tune, s);
+            launch_pgs_flow(w->B, sp, R, rc, w->bmask, w->col_off, w->cls, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US,
+                            w->sweep_groups, w->sweep_lds, w->sweep_tune, s);
         }
         AXM_TRY(prof_mark(w, nev, o.capture));
         it += seg;
@@ -827,6 +847,7 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     }
     axm_status st = w->mem.get(w->cnt, 1);
     if (st == AXM_SUCCESS) st = w->mem.get(w->bar, 1);
+    if (st == AXM_SUCCESS) st = w->mem.get(w->cls, 512);
     if (st == AXM_SUCCESS) st = w->mem.get(w->col_off, MAX_SORT_COLOURS + 2);
     if (st == AXM_SUCCESS) st = alloc_bodies(w, std::max<uint32_t>(desc->max_bodies, 16u));
     // explicit capacities are honoured as given; 0 means "sized from the body count"
@@ -1049,8 +1070,8 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             if (w->prof_on) {
                 AXM_HIP(hipGraphLaunch(w->gpre[c], w->stream));
                 AXM_HIP(hipEventRecord(w->gprof_ev[2 * c], w->stream));
-                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->n, 0, iters, w->cnt, SWEEP_SPIN_LIMIT_US,
-                                w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream);
+                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, w->col_off, w->cls, 0, iters, w->cnt,
+                                SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream);
                 AXM_HIP(hipEventRecord(w->gprof_ev[2 * c + 1], w->stream));
                 AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
             } else {
@@ -1120,7 +1141,7 @@ axm_status axm_broadphase(axm_world w, uint32_t* count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 3; ++attempt) {
-        launch_counters_begin(w->cnt, w->stream);
+        launch_counters_begin(w->cnt, w->cls, w->stream);
         if (w->n) enqueue_broad(w, sp);
         AXM_TRY(read_counters(w));
         if (!(w->h_cnt->sticky & STICKY_PAIRS)) break;
@@ -1141,7 +1162,7 @@ axm_status axm_collide(axm_world w, uint32_t* manifold_count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 4; ++attempt) {
-        launch_counters_begin(w->cnt, w->stream);
+        launch_counters_begin(w->cnt, w->cls, w->stream);
         if (w->n) {
             enqueue_broad(w, sp);
             enqueue_narrow(w, sp);
@@ -1267,7 +1288,7 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
 
 axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy) {
     if (!w) return AXM_NULL_POINTER;
-    if ((policy & ~0x73u) != 0u) return AXM_INVALID_PARAMETER;
+    if ((policy & ~0x77u) != 0u) return AXM_INVALID_PARAMETER;
     const uint32_t lds = lds_kib ? lds_kib * 1024u : (w->max_lds & ~16383u);
     if (lds > w->max_lds) return AXM_INVALID_PARAMETER;
     (void)hipSetDevice(w->desc.device);
