@@ -579,20 +579,13 @@ __device__ __forceinline__ f3 vrel(const SB& A, const SB& Bd, f3 rA, f3 rB) {
     return (Bd.v + cross(Bd.w, rB)) - (A.v + cross(A.w, rA));
 }
 
-// Kind of a visit, for grouping the sweep's work (colour c < 64, four kinds per
-// colour: against a plane or not, one point or more).
-__device__ __forceinline__ uint32_t visit_class(uint32_t c, bool plane, uint32_t np) {
-    return c * 4u + (plane ? 0u : 2u) + (np > 1u ? 1u : 0u);
-}
-
 __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
                                               uint32_t cap, const uint2* __restrict__ m_ab,
                                               const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
                                               const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
-                                              Counters* __restrict__ cnt, float* __restrict__ feat,
-                                              uint32_t* __restrict__ cls) {
+                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
     if (s >= M || cnt->sticky != 0u) return;  // a failed step must not touch the impulse buffers
@@ -627,9 +620,6 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
             // who visits which body in which colour: the sweep's visits pass velocities on by it
             if (A.dyn) B.vslot[(size_t)ab.x * 64u + c] = s << 1;
             if (Bd.dyn) B.vslot[(size_t)ab.y * 64u + c] = (s << 1) | 1u;
-            // slots per colour and kind (plane or not, one point or more): the sweep hands a
-            // wave slots of one kind where it can, so that its lanes take the same path
-            atomicAdd(&cls[visit_class(c, plane, np)], 1u);
         }
         R.dep[s] = dp;
     }
@@ -999,29 +989,13 @@ __device__ __forceinline__ uint32_t next_colour(unsigned long long mk, uint32_t 
 // words: a body's first visit (rank 0) gets the body's solver velocity with
 // version 0, every other inbox a version no visit waits for.
 __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const uint64_t* __restrict__ bmask,
-                                                   Counters* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ col_off, uint32_t* __restrict__ cls,
-                                                   uint32_t sweep0, bool by_kind) {
+                                                   const Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     // the rows exist only when this step's preparation ran
     if (s >= cnt->m_eff || cnt->sticky != 0u) return;
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
     const uint32_t c = R.colour[s];
-    if (sweep0 == 0u) {
-        if (by_kind && c < 64u && cnt->max_colour <= 64u) {
-            // work order of the sweep: within each colour, the slots of one kind together
-            // (the order inside a kind is arbitrary: visits of one colour share no body)
-            const uint32_t k = visit_class(c, (h.y & PLANE_BIT) != 0u, h.z);
-            uint32_t base = col_off[c];
-            for (uint32_t q = c * 4u; q < k; ++q) base += cls[q];
-            const uint32_t q = base + atomicAdd(&cls[256u + k], 1u);
-            if (q < cnt->m_eff) R.perm[q] = s;
-            else atomicOr(&cnt->sticky, STICKY_INTERNAL);  // kind counts disagree with the colour offsets
-        } else {
-            R.perm[s] = s;  // the solve order
-        }
-    }
     uint2 nx = make_uint2(UNSET, UNSET);
     if (dp.x != UNSET) {
         nx.x = B.vslot[(size_t)h.x * 64u + next_colour(bmask[h.x], c)];
@@ -1138,8 +1112,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     const size_t rs = (size_t)ppl * OWN_WAVE;  // LDS row stride of the points (float4)
     float4* recs = own_lds + lane;
     float4* mine = own_lds + p0 + lane;
-    // work item (g + jj * G) * 64 + lane of the launch is slot R.perm[item]
-    auto slot_of = [&](uint32_t jj) { return R.perm[(g + jj * G) * OWN_WAVE + lane]; };
+    auto slot_of = [&](uint32_t jj) { return (g + jj * G) * OWN_WAVE + lane; };
 
     // stage slot records, rows and impulses
     bool all_staged = own <= OWN_SLOTS;
@@ -1171,8 +1144,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     const float mu = sp.friction;
     const uint32_t total = nv * nsweeps;
     // tune bits 4-6: poll back-off level; bit 1: poll the hint pair before the full load
-    // (inboxes within 2 GiB); bit 2: adaptive back-off (bit 0, grouping the work by kind, is
-    // applied by k_flow_init)
+    // (inboxes within 2 GiB); bit 2: adaptive back-off
     const uint32_t nap = (tune >> 4) & 7u;
     const bool hint = (tune & 2u) != 0u && (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull,
                adaptive = (tune & 4u) != 0u;
@@ -1417,13 +1389,9 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
 
 
 // Clears the per-step part of the control block.
-__global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ cls) {
+__global__ void k_counters_begin(Counters* cnt) {
     uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
     if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
-    if (cls) {
-        cls[threadIdx.x] = 0u;
-        cls[threadIdx.x + 256u] = 0u;
-    }
 }
 
 // Integration is skipped while any sticky error is set, so an overflowing step
@@ -1460,8 +1428,8 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 
 
-void launch_counters_begin(Counters* cnt, uint32_t* cls, hipStream_t s) {
-    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(256), 0, s, cnt, cls);
+void launch_counters_begin(Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
@@ -1540,10 +1508,10 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, float* feat, uint32_t* cls, hipStream_t s) {
+                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
     const PrevCache pc = make_prev(pca);
     hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, col, bmask, bdeg, cnt, feat, cls);
+                       warm_in, col, bmask, bdeg, cnt, feat);
 }
 
 // Learned warm start: the network predicts the correction to the previous
@@ -1585,13 +1553,15 @@ int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes) {
     return nb;
 }
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
-                     const uint32_t* col_off, uint32_t* cls, uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
-                     uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s) {
+                     uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
+                     uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
+                     hipEvent_t ev0, hipEvent_t ev1) {
     if (!nsweeps) return;
-    hipLaunchKernelGGL(k_flow_init, grid_for(cap), dim3(256), 0, s, B, R, bmask, cnt, col_off, cls, sweep0,
-                       (tune & 1u) != 0u);
+    hipLaunchKernelGGL(k_flow_init, grid_for(cap), dim3(256), 0, s, B, R, bmask, cnt);
+    if (ev0) (void)hipEventRecord(ev0, s);
     hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
                        nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune);
+    if (ev1) (void)hipEventRecord(ev1, s);
     hipLaunchKernelGGL(k_flow_finish, grid_for(cap), dim3(256), 0, s, B, R, cnt);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,

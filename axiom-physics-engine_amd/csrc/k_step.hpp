@@ -25,7 +25,6 @@ struct SolverRows {
     uint32_t* colour;  // per slot: the manifold's colour (kept by the next step's incremental colouring)
     uint64_t* inbox;   // persistent sweep: per slot 16 words, velocities of body a (0..5) and b (8..13)
     uint2* next;       // persistent sweep: per slot, the next visit of body a and of b (slot << 1 | side)
-    uint32_t* perm;    // persistent sweep: work item -> slot (colour order, one kind of visit together)
 };
 // Steps between full recolourings (in between, persisting contacts keep their colour).
 constexpr uint32_t RECOLOUR_PERIOD = 64;
@@ -42,8 +41,7 @@ struct PrevCacheArgs {
     uint32_t valid;
 };
 
-// Clears the per-step counters and the 512 per-colour-kind counters `cls` (may be null).
-void launch_counters_begin(Counters* cnt, uint32_t* cls, hipStream_t s);
+void launch_counters_begin(Counters* cnt, hipStream_t s);
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
@@ -74,15 +72,17 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, float* feat, uint32_t* cls, hipStream_t s);
+                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s);
 // Writes the learned warm start (network outputs Y, rows 4 * slot + point) into the impulse rows.
 void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, const float* Y, const StepParams& sp,
                       hipStream_t s);
 // Persistent sweep: `groups` workgroups of one wave, each with `lds_bytes` of LDS for
-// its staged rows; all of them must be resident at once (see pgs_own_setup).
+// its staged rows; all of them must be resident at once (see pgs_own_setup).  Events
+// ev0 / ev1, when given, are recorded around the sweep kernel itself.
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
-                     const uint32_t* col_off, uint32_t* cls, uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
-                     uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s);
+                     uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
+                     uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Raises the sweep kernel's dynamic-LDS limit to max_lds_bytes and returns how many of
 // its workgroups with lds_bytes each fit on one CU (0 on error).
 int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes);

@@ -119,8 +119,8 @@ struct axm_world_s {
 
     // persistent sweep launch shape: workgroups (one wave each, all resident at
     // once), LDS per workgroup for the staged rows of its slots, and the policy
-    // (bit 0 group each colour's work by kind of visit, bit 1 poll a 16-byte hint
-    // before the full load, bit 2 adaptive back-off, bits 4-6 poll back-off).
+    // (bit 1 poll a 16-byte hint before the full load, bit 2 adaptive back-off,
+    // bits 4-6 poll back-off).
     // Set at creation to one workgroup per CU with the largest LDS a workgroup
     // may have.
     uint32_t sweep_groups = 0;
@@ -133,7 +133,6 @@ struct axm_world_s {
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
     uint32_t* col_off = nullptr;
     uint32_t* bdone = nullptr;
-    uint32_t* cls = nullptr;  // per colour < 64 and kind of visit: slot counts (0..255) and fill cursors (256..511)
     GridBar* bar = nullptr;
 
     SolverRows R[2]{};
@@ -282,7 +281,6 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.colour);
     w->mem.release(R.inbox);
     w->mem.release(R.next);
-    w->mem.release(R.perm);
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
@@ -295,7 +293,6 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.colour, cap));
     AXM_TRY(w->mem.get(R.inbox, (size_t)cap * 16));
     AXM_TRY(w->mem.get(R.next, cap));
-    AXM_TRY(w->mem.get(R.perm, cap));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
@@ -322,9 +319,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint32_t* colour = nullptr;
     uint64_t* inbox = nullptr;
     uint2* next = nullptr;
-    uint32_t* perm = nullptr;
     AXM_TRY(w->mem.get(colour, cap));
-    AXM_TRY(w->mem.get(perm, cap));
     AXM_TRY(w->mem.get(inbox, (size_t)cap * 16));
     AXM_TRY(w->mem.get(next, cap));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
@@ -356,7 +351,6 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.colour);
     w->mem.release(R.inbox);
     w->mem.release(R.next);
-    w->mem.release(R.perm);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
     w->pc_key[set] = tkey;
@@ -372,7 +366,6 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     R.colour = colour;
     R.inbox = inbox;
     R.next = next;
-    R.perm = perm;
     w->rcap[set] = cap;
     return AXM_SUCCESS;
 }
@@ -457,14 +450,22 @@ axm_status clear_sticky(axm_world_s* w) {
     return AXM_SUCCESS;
 }
 
-axm_status prof_mark(axm_world_s* w, size_t& nev, bool capturing) {
+// The next profiling event (created on first use), or null when not profiling.
+axm_status prof_event(axm_world_s* w, size_t& nev, bool capturing, hipEvent_t& e) {
+    e = nullptr;
     if (!w->prof_on || capturing) return AXM_SUCCESS;
     while (w->prof_ev.size() < nev + 1) {
-        hipEvent_t e;
-        AXM_HIP(hipEventCreate(&e));
-        w->prof_ev.push_back(e);
+        hipEvent_t x;
+        AXM_HIP(hipEventCreate(&x));
+        w->prof_ev.push_back(x);
     }
-    AXM_HIP(hipEventRecord(w->prof_ev[nev++], w->stream));
+    e = w->prof_ev[nev++];
+    return AXM_SUCCESS;
+}
+axm_status prof_mark(axm_world_s* w, size_t& nev, bool capturing) {
+    hipEvent_t e;
+    AXM_TRY(prof_event(w, nev, capturing, e));
+    if (e) AXM_HIP(hipEventRecord(e, w->stream));
     return AXM_SUCCESS;
 }
 
@@ -518,7 +519,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[0], s));
-    launch_counters_begin(w->cnt, w->cls, s);
+    launch_counters_begin(w->cnt, s);
     enqueue_broad(w, sp);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[1], s));
     if (o.flags & AXM_STEP_KEEP_STAGES) {
@@ -550,7 +551,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
     launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
-                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, w->cls, s);
+                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, s);
     if (w->mlp && w->desc.warm_start == AXM_WARM_MLP && !o.warm_in && w->feat) {
         launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
                            2u * (uint32_t)w->cu_count, s);
@@ -595,16 +596,20 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
         uint32_t seg = max_it - it;
         if (w->desc.record_residual) seg = 1;
         else if (tol_mode) seg = std::min(seg, every - (it % every));
-        AXM_TRY(prof_mark(w, nev, o.capture));
         if (o.colour_launches) {
+            AXM_TRY(prof_mark(w, nev, o.capture));
             for (uint32_t k = 0; k < seg; ++k)
                 for (size_t col = 0; col + 1 < off.size(); ++col)
                     launch_pgs(w->B, sp, R, off[col], off[col + 1], rc, it + k == 0 ? 1u : 0u, s);
+            AXM_TRY(prof_mark(w, nev, o.capture));
         } else {
-            launch_pgs_flow(w->B, sp, R, rc, w->bmask, w->col_off, w->cls, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US,
-                            w->sweep_groups, w->sweep_lds, w->sweep_tune, s);
+            // the events bracket the sweep kernel itself
+            hipEvent_t e0, e1;
+            AXM_TRY(prof_event(w, nev, o.capture, e0));
+            AXM_TRY(prof_event(w, nev, o.capture, e1));
+            launch_pgs_flow(w->B, sp, R, rc, w->bmask, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US,
+                            w->sweep_groups, w->sweep_lds, w->sweep_tune, s, e0, e1);
         }
-        AXM_TRY(prof_mark(w, nev, o.capture));
         it += seg;
         const bool last = it == max_it;
         const bool check = tol_mode && (it % every == 0);
@@ -847,7 +852,6 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     }
     axm_status st = w->mem.get(w->cnt, 1);
     if (st == AXM_SUCCESS) st = w->mem.get(w->bar, 1);
-    if (st == AXM_SUCCESS) st = w->mem.get(w->cls, 512);
     if (st == AXM_SUCCESS) st = w->mem.get(w->col_off, MAX_SORT_COLOURS + 2);
     if (st == AXM_SUCCESS) st = alloc_bodies(w, std::max<uint32_t>(desc->max_bodies, 16u));
     // explicit capacities are honoured as given; 0 means "sized from the body count"
@@ -1069,10 +1073,9 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             const int c = w->cur;
             if (w->prof_on) {
                 AXM_HIP(hipGraphLaunch(w->gpre[c], w->stream));
-                AXM_HIP(hipEventRecord(w->gprof_ev[2 * c], w->stream));
-                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, w->col_off, w->cls, 0, iters, w->cnt,
-                                SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream);
-                AXM_HIP(hipEventRecord(w->gprof_ev[2 * c + 1], w->stream));
+                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, 0, iters, w->cnt,
+                                SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream,
+                                w->gprof_ev[2 * c], w->gprof_ev[2 * c + 1]);
                 AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
             } else {
                 AXM_HIP(hipGraphLaunch(w->gexec[c], w->stream));
@@ -1141,7 +1144,7 @@ axm_status axm_broadphase(axm_world w, uint32_t* count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 3; ++attempt) {
-        launch_counters_begin(w->cnt, w->cls, w->stream);
+        launch_counters_begin(w->cnt, w->stream);
         if (w->n) enqueue_broad(w, sp);
         AXM_TRY(read_counters(w));
         if (!(w->h_cnt->sticky & STICKY_PAIRS)) break;
@@ -1162,7 +1165,7 @@ axm_status axm_collide(axm_world w, uint32_t* manifold_count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 4; ++attempt) {
-        launch_counters_begin(w->cnt, w->cls, w->stream);
+        launch_counters_begin(w->cnt, w->stream);
         if (w->n) {
             enqueue_broad(w, sp);
             enqueue_narrow(w, sp);
@@ -1288,7 +1291,7 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
 
 axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy) {
     if (!w) return AXM_NULL_POINTER;
-    if ((policy & ~0x77u) != 0u) return AXM_INVALID_PARAMETER;
+    if ((policy & ~0x76u) != 0u) return AXM_INVALID_PARAMETER;
     const uint32_t lds = lds_kib ? lds_kib * 1024u : (w->max_lds & ~16383u);
     if (lds > w->max_lds) return AXM_INVALID_PARAMETER;
     (void)hipSetDevice(w->desc.device);

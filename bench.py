@@ -92,7 +92,7 @@ def tolerance_comparison(tol, max_sweeps, worlds):
 def measured_traffic(cfg, n, iters, args):
     """HBM bytes per sweep-kernel launch from the committed counter runs (rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE, scaled by a plain copy of known size), when they cover this exact workload; else None."""
-    path = os.path.join(ROOT, "profiles", "k_pgs_flow_pile100k_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_pgs_own_pile100k_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -189,7 +189,7 @@ def main():
     roof = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(cfg, bodies.n, iters, args),
-        "kernel": "k_pgs_flow",
+        "kernel": "k_pgs_own",
         "launches": int(prof.pgs_launches),
         "avg_launch_us": round(prof.pgs_ms * 1e3 / max(prof.pgs_launches, 1), 3),
         "bytes_per_launch": round(prof.pgs_bytes / max(prof.pgs_launches, 1), 1),

@@ -188,10 +188,9 @@ axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out);
 /* Launch shape of the persistent sweep kernel: workgroups of one wave (0: one per CU;
  * at most what can be resident at once), KiB of LDS per workgroup for the staged
  * Jacobian rows and impulses of its slots (0: the device maximum, up to 160), and
- * policy bits (bit 0: hand each wave visits of one kind (plane or not, one point
- * or more) where possible; bit 1 (default on): poll one 16-byte hint before loading
- * the velocities; bit 2: back off longer while a wave makes no progress; bits 4-6:
- * poll back-off level 0..7, default 1).  Results
+ * policy bits (bit 1, default on: poll one 16-byte hint before loading the
+ * velocities; bit 2: back off longer while a wave makes no progress; bits 4-6: poll
+ * back-off level 0..7, default 1).  Results
  * do not depend on it; throughput does (tuning hook, no reference counterpart). */
 axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy);
 
