@@ -6,8 +6,7 @@
 //   order: each wave finds the lanes sharing its digit with 8 ballots
 //   (a match-any over the digit bits), ranks itself by the population count of
 //   lower peer lanes, and per-wave digit counts are prefixed in LDS.
-// Scan: per-tile reduction, a single-workgroup scan of tile sums, then a
-//   down-sweep with wave-level shuffles.
+// Scan: one pass, tiles chained by decoupled look-back (below).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -54,62 +53,84 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
     return base + inc - v;
 }
 
-__global__ void __launch_bounds__(SC_THREADS) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
-                                                             uint32_t* __restrict__ sums) {
-    const uint32_t base = blockIdx.x * SC_TILE;
-    uint32_t s = 0;
-#pragma unroll
-    for (int it = 0; it < SC_ITEMS; ++it) {
-        const uint32_t i = base + it * SC_THREADS + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-    uint32_t tot;
-    block_excl_scan(s, tot);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
+// Single-pass exclusive scan with decoupled look-back.  A workgroup takes the next
+// tile from a counter (so every lower tile is already held by a running
+// workgroup), publishes its tile total, then walks back over the published
+// status words of lower tiles until it meets an inclusive prefix, and publishes
+// its own.  Status word: bits 62-63 state (0 none, 1 tile total, 2 inclusive
+// prefix), bits 0-31 the value.  The last workgroup to finish clears the status
+// words and counters, so every call starts from zeros (the scratch is zeroed once
+// when allocated).  Scratch layout (u32 words): [0] tile counter, [1] finished
+// counter, then nb 64-bit status words.
+constexpr uint64_t SC_AGG = 1ull << 62, SC_PRE = 2ull << 62;
 
-// Single workgroup: exclusive scan of nb tile sums in place; sums[nb] = total.
-__global__ void __launch_bounds__(SC_THREADS) k_scan_sums(uint32_t* __restrict__ sums, uint32_t nb,
-                                                           uint32_t* __restrict__ total_out) {
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += SC_THREADS) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? sums[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, tot);
-        if (i < nb) sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) {
-        sums[nb] = carry;
-        if (total_out) *total_out = carry;
-    }
-}
-
-__global__ void __launch_bounds__(SC_THREADS) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                           uint32_t n, const uint32_t* __restrict__ sums, uint32_t nb) {
-    // Each thread owns SC_ITEMS consecutive elements of the tile.
-    const uint32_t base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+__global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                         uint32_t n, uint32_t* __restrict__ tmp, uint32_t nb,
+                                                         uint32_t* __restrict__ total_out) {
+    __shared__ uint32_t s_tile, s_prefix;
+    uint64_t* st = reinterpret_cast<uint64_t*>(tmp + 2);
+    if (threadIdx.x == 0) s_tile = atomicAdd(&tmp[0], 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    // each thread owns SC_ITEMS consecutive elements of the tile
+    const uint32_t base = tile * SC_TILE + threadIdx.x * SC_ITEMS;
     uint32_t v[SC_ITEMS];
-    uint32_t s = 0;
+    uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; ++k) {
         const uint32_t i = base + k;
         v[k] = i < n ? in[i] : 0u;
-        s += v[k];
+        sum += v[k];
     }
     uint32_t tot;
-    uint32_t run = sums[blockIdx.x] + block_excl_scan(s, tot);
+    const uint32_t ex = block_excl_scan(sum, tot);
+    if (threadIdx.x == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&st[0], SC_PRE | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st[tile], SC_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // lower tiles are held by running workgroups, so each publishes; the wall-clock
+            // bound only keeps a broken invariant from hanging the device
+            const uint64_t t0 = wall_clock64();
+            uint32_t j = tile - 1;
+            for (;;) {
+                const uint64_t w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((w >> 62) == 0u) {
+                    if (wall_clock64() - t0 > 200000000ull) break;  // 2 s
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += (uint32_t)w;
+                if ((w >> 62) == 2u || j == 0u) break;
+                --j;
+            }
+            __hip_atomic_store(&st[tile], SC_PRE | (uint32_t)(prefix + tot), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + ex;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; ++k) {
         const uint32_t i = base + k;
         if (i < n) out[i] = run;
         run += v[k];
     }
-    if (blockIdx.x == nb - 1 && threadIdx.x == SC_THREADS - 1) out[n] = sums[nb];
+    if (tile == nb - 1 && threadIdx.x == SC_THREADS - 1) {
+        out[n] = s_prefix + tot;
+        if (total_out) *total_out = s_prefix + tot;
+    }
+    if (threadIdx.x == 0 && atomicAdd(&tmp[1], 1u) == nb - 1) {
+        // every workgroup has finished its look-back: reset the scratch for the next call
+        for (uint32_t k = 0; k < nb; ++k) __hip_atomic_store(&st[k], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tmp[0] = 0u;
+        tmp[1] = 0u;
+    }
 }
 
-size_t scan_tmp_words(uint32_t n) { return (size_t)((n + SC_TILE - 1) / SC_TILE) + 1; }
+size_t scan_tmp_words(uint32_t n) { return 2 + 2 * (size_t)((n + SC_TILE - 1) / SC_TILE); }
 
 __global__ void k_zero1(uint32_t* p, uint32_t* q) {
     *p = 0;
@@ -122,9 +143,7 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp
         return;
     }
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SC_THREADS), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, tmp, nb, total_out);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb);
+    hipLaunchKernelGGL(k_scan_1p, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -210,7 +229,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__
 
 size_t radix_tmp_words(uint32_t n) {
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)256 * nb + 1 + scan_tmp_words(256u * nb);
+    return (size_t)256 * nb + 2 + scan_tmp_words(256u * nb);
 }
 
 template <class K>
@@ -220,7 +239,7 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* hist = tmp;
     uint32_t* offs = tmp;  // scanned in place into the second half
-    uint32_t* scan_tmp = tmp + (size_t)256 * nb + 1;
+    uint32_t* scan_tmp = tmp + (size_t)256 * nb + 2;  // 8-byte aligned for the scan's status words
     K* ki = keys;
     K* ko = keys_alt;
     uint32_t* vi = vals;

@@ -140,6 +140,23 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.vel[i].w > 0.0f ? 1.0f : 0.0f);
 }
 
+// Per cell, the union of its members' AABBs, stored at the cell's first sorted
+// position: a neighbour cell whose box misses a body's AABB holds no partner of it.
+__global__ void __launch_bounds__(256) k_cell_boxes(Bodies B, const uint64_t* __restrict__ skey, uint32_t n) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint64_t k = skey[s];
+    if (s > 0 && skey[s - 1] == k) return;
+    float4 lo = B.smin[s], hi = B.smax[s];
+    for (uint32_t t = s + 1; t < n && skey[t] == k; ++t) {
+        const float4 a = B.smin[t], b = B.smax[t];
+        lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.0f);
+        hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), fmaxf(hi.w, b.w));
+    }
+    B.cmin[s] = lo;
+    B.cmax[s] = hi;
+}
+
 template <class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t, F&& f) {
@@ -172,6 +189,12 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
                 const uint64_t nk = wbits | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
                 uint2 r;
                 if (!ht_find(ct, nk, r)) continue;
+                // skip a cell whose members' AABB union misses this body (or that has only
+                // static members when this body is static): it holds no partner
+                const float4 cl = B.cmin[r.x], ch = B.cmax[r.x];
+                if (mn.x > ch.x || mx.x < cl.x || mn.y > ch.y || mx.y < cl.y || mn.z > ch.z || mx.z < cl.z ||
+                    (!dyn_i && !(ch.w > 0.0f)))
+                    continue;
                 for (uint32_t s = r.x; s < r.x + r.y; ++s) {
                     const float4 mnj = B.smin[s], mxj = B.smax[s];
                     const uint32_t j = __float_as_uint(mnj.w);
@@ -254,12 +277,66 @@ __device__ __forceinline__ void store_pair(uint32_t p, const Man& m, uint32_t* _
     }
 }
 
+// Processing order of the narrowphase: pairs grouped by the kind of test (the two
+// shape types, or the shape type against a plane), so that a wave runs one test
+// rather than every test its lanes need.  Results stay indexed by pair.
+constexpr uint32_t PAIR_KINDS = 16;
+__device__ __forceinline__ uint32_t pair_kind(const Bodies& B, uint32_t a, uint32_t b) {
+    const uint32_t ta = fbits(B.shape[a].w) & 3u;
+    if (b & PLANE_BIT) return 12u + ta;
+    const uint32_t tb = fbits(B.shape[b].w) & 3u;
+    const uint32_t lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+    return lo * 4u + hi;
+}
+// Per pair its kind; per kind the count (kinds[0..15]; kinds[16..31] are the fill
+// cursors of k_pair_order, both cleared at the step start).
+__global__ void __launch_bounds__(256) k_pair_kind(Bodies B, const uint32_t* __restrict__ pa,
+                                                   const uint32_t* __restrict__ pb, uint32_t* __restrict__ p_kind,
+                                                   uint32_t* __restrict__ kinds, const Counters* __restrict__ cnt) {
+    __shared__ uint32_t h[PAIR_KINDS];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < PAIR_KINDS) h[threadIdx.x] = 0u;
+    __syncthreads();
+    if (p < cnt->p_eff) {
+        const uint32_t k = pair_kind(B, pa[p], pb[p]);
+        p_kind[p] = k;
+        atomicAdd(&h[k], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < PAIR_KINDS && h[threadIdx.x]) atomicAdd(&kinds[threadIdx.x], h[threadIdx.x]);
+}
+__global__ void __launch_bounds__(256) k_pair_order(const uint32_t* __restrict__ p_kind, uint32_t* __restrict__ kinds,
+                                                    uint32_t* __restrict__ order, const Counters* __restrict__ cnt) {
+    __shared__ uint32_t off[PAIR_KINDS], h[PAIR_KINDS];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t P = cnt->p_eff;
+    if (threadIdx.x < PAIR_KINDS) {
+        uint32_t o = 0;
+        for (uint32_t k = 0; k < threadIdx.x; ++k) o += kinds[k];
+        off[threadIdx.x] = o;
+        h[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    const uint32_t k = p < P ? p_kind[p] : 0u;
+    const uint32_t r = p < P ? atomicAdd(&h[k], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < PAIR_KINDS && h[threadIdx.x])
+        off[threadIdx.x] += atomicAdd(&kinds[PAIR_KINDS + threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (p < P) {
+        const uint32_t q = off[k] + r;
+        if (q < P) order[q] = p;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
                                                 const uint32_t* __restrict__ pb, uint32_t* __restrict__ p_np,
                                                 float4* __restrict__ p_pt, float4* __restrict__ p_nf,
-                                                uint32_t* __restrict__ epa_list, Counters* __restrict__ cnt) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= cnt->p_eff) return;
+                                                uint32_t* __restrict__ epa_list, Counters* __restrict__ cnt,
+                                                const uint32_t* __restrict__ order) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->p_eff) return;
+    const uint32_t p = order[t];
     const uint32_t a = pa[p], b = pb[p];
     Man m;
     m.np = 0;
@@ -284,26 +361,27 @@ __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const u
     store_pair(p, m, p_np, p_pt, p_nf);
 }
 
-// Deep box-capsule pairs (cores intersect): EPA, one thread per queued pair.
+// Deep box-capsule pairs (cores intersect): EPA, one thread per queued pair (a
+// fixed grid strides over the queue).
 __global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
                                                    const uint32_t* __restrict__ pb,
                                                    const uint32_t* __restrict__ epa_list, uint32_t cap,
                                                    uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
                                                    float4* __restrict__ p_nf, Counters* __restrict__ cnt) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nq = cnt->epa_count;
-    if (q >= nq || q >= cap) return;
-    const uint32_t p = epa_list[q];
-    const uint32_t a = pa[p], b = pb[p];
-    Shp A = load_shape(B, a), Bs = load_shape(B, b);
-    const bool swap = A.type > Bs.type;
-    EpaWork W;
-    Man m;
-    m.np = 0;
-    const int st = swap ? box_capsule_epa(m, Bs, A, W) : box_capsule_epa(m, A, Bs, W);
-    if (swap) flip_manifold(m);
-    if (st == GEOM_EPA_FAIL) atomicAdd(&cnt->epa_fail, 1u);
-    store_pair(p, m, p_np, p_pt, p_nf);
+    const uint32_t nq = min(cnt->epa_count, cap);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const uint32_t p = epa_list[q];
+        const uint32_t a = pa[p], b = pb[p];
+        Shp A = load_shape(B, a), Bs = load_shape(B, b);
+        const bool swap = A.type > Bs.type;
+        EpaWork W;
+        Man m;
+        m.np = 0;
+        const int st = swap ? box_capsule_epa(m, Bs, A, W) : box_capsule_epa(m, A, Bs, W);
+        if (swap) flip_manifold(m);
+        if (st == GEOM_EPA_FAIL) atomicAdd(&cnt->epa_fail, 1u);
+        store_pair(p, m, p_np, p_pt, p_nf);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_flag_nonzero(const uint32_t* __restrict__ np, uint32_t cap_pairs,
@@ -1389,9 +1467,10 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
 
 
 // Clears the per-step part of the control block.
-__global__ void k_counters_begin(Counters* cnt) {
+__global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ kinds) {
     uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
     if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
+    if (kinds && threadIdx.x < 2u * PAIR_KINDS) kinds[threadIdx.x] = 0u;
 }
 
 // Integration is skipped while any sticky error is set, so an overflowing step
@@ -1428,8 +1507,8 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 
 
-void launch_counters_begin(Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt);
+void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
+    hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
@@ -1450,6 +1529,7 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
+    hipLaunchKernelGGL(k_cell_boxes, grid_for(n), dim3(256), 0, s, B, skey, n);
     hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
@@ -1460,10 +1540,15 @@ void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* ske
                            stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
-                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_narrow, grid_for(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list, cnt);
-    hipLaunchKernelGGL(k_narrow_epa, grid_for(cap_pairs, 64), dim3(64), 0, s, B, sp, pa, pb, epa_list, cap_pairs, p_np,
-                       p_pt, p_nf, cnt);
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
+                   uint32_t* order, uint32_t epa_groups, Counters* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_kind, grid_for(cap_pairs), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
+    hipLaunchKernelGGL(k_pair_order, grid_for(cap_pairs), dim3(256), 0, s, p_kind, kinds, order, cnt);
+    hipLaunchKernelGGL(k_narrow, grid_for(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list, cnt,
+                       order);
+    // the deep pairs are few: a fixed grid strides over the queue
+    hipLaunchKernelGGL(k_narrow_epa, dim3(epa_groups), dim3(64), 0, s, B, sp, pa, pb, epa_list, cap_pairs, p_np, p_pt,
+                       p_nf, cnt);
 }
 void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s) {
     hipLaunchKernelGGL(k_flag_nonzero, grid_for(cap_pairs), dim3(256), 0, s, np, cap_pairs, flag, cnt);

@@ -41,7 +41,8 @@ struct PrevCacheArgs {
     uint32_t valid;
 };
 
-void launch_counters_begin(Counters* cnt, hipStream_t s);
+// Clears the per-step counters and, when given, the 32 pair-kind counters.
+void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
@@ -54,8 +55,12 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
                       Counters* cnt, const uint32_t* stash, hipStream_t s);
+// Narrowphase of every pair: the pairs are first ordered by kind of test (p_kind,
+// the 32 counters `kinds` cleared at the step start, `order`), then tested; deep
+// box-capsule pairs queue for EPA, run by `epa_groups` workgroups of 64.
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
-                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, Counters* cnt, hipStream_t s);
+                   uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
+                   uint32_t* order, uint32_t epa_groups, Counters* cnt, hipStream_t s);
 void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s);
 void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, const uint32_t* p_np,
                     const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,

@@ -103,6 +103,8 @@ struct axm_world_s {
     uint32_t *pair_cnt = nullptr, *pair_off = nullptr, *scan_tmp = nullptr, *pair_stash = nullptr;
     uint32_t *pa = nullptr, *pb = nullptr;
     uint32_t *p_np = nullptr, *p_flag = nullptr, *man_off = nullptr, *epa_list = nullptr, *pscan_tmp = nullptr;
+    uint32_t *p_kind = nullptr, *p_order = nullptr;  // narrowphase: kind of test per pair, pairs ordered by kind
+    uint32_t* pkinds = nullptr;                       // 16 pair-kind counts + 16 fill cursors
     float4 *p_pt = nullptr, *p_nf = nullptr;
 
     uint2* m_ab = nullptr;
@@ -194,7 +196,7 @@ namespace {
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &B.cmin, &B.cmax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
     w->mem.release(B.vslot);
@@ -221,7 +223,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     free_bodies(w);
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &B.cmin, &B.cmax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
@@ -232,6 +234,8 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->val, cap));
     AXM_TRY(w->mem.get(w->val_alt, cap));
     AXM_TRY(w->mem.get(w->sort_tmp, radix_tmp_words(cap)));
+    // the scans' look-back scratch must start zeroed (each scan leaves it zeroed)
+    if (hipMemset(w->sort_tmp, 0, 4 * radix_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->ct.key, T));
     AXM_TRY(w->mem.get(w->ct.val, T));
@@ -239,6 +243,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->pair_cnt, cap));
     AXM_TRY(w->mem.get(w->pair_off, cap + 1));
     AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
+    if (hipMemset(w->scan_tmp, 0, 4 * scan_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     AXM_TRY(w->mem.get(w->bmax0, cap));
     AXM_TRY(w->mem.get(w->bmax1, cap));
     AXM_TRY(w->mem.get(w->bmask, cap));
@@ -254,6 +259,8 @@ axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->p_flag);
     w->mem.release(w->man_off);
     w->mem.release(w->epa_list);
+    w->mem.release(w->p_kind);
+    w->mem.release(w->p_order);
     w->mem.release(w->p_pt);
     w->mem.release(w->p_nf);
     w->mem.release(w->pscan_tmp);
@@ -263,9 +270,12 @@ axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->p_flag, cap));
     AXM_TRY(w->mem.get(w->man_off, (size_t)cap + 1));
     AXM_TRY(w->mem.get(w->epa_list, cap));
+    AXM_TRY(w->mem.get(w->p_kind, cap));
+    AXM_TRY(w->mem.get(w->p_order, cap));
     AXM_TRY(w->mem.get(w->p_pt, (size_t)cap * 4));
     AXM_TRY(w->mem.get(w->p_nf, (size_t)cap * 4));
     AXM_TRY(w->mem.get(w->pscan_tmp, scan_tmp_words(cap)));
+    if (hipMemset(w->pscan_tmp, 0, 4 * scan_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     w->cap_pairs = cap;
     return AXM_SUCCESS;
 }
@@ -406,6 +416,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->cval, cap));
     AXM_TRY(w->mem.get(w->cval_alt, cap));
     AXM_TRY(w->mem.get(w->csort_tmp, radix_tmp_words(cap)));
+    if (hipMemset(w->csort_tmp, 0, 4 * radix_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     w->cap_man = cap;
     if (w->rcap[w->cur] < cap) AXM_TRY(alloc_rows(w, w->cur, cap));
     if (w->rcap[1 - w->cur] > 0 && w->rcap[1 - w->cur] < cap) AXM_TRY(grow_rows_keep(w, 1 - w->cur, cap));
@@ -504,7 +515,8 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
 // ---- narrowphase into per-pair staging, compaction into manifolds (pair order)
 void enqueue_narrow(axm_world_s* w, const StepParams& sp) {
     hipStream_t s = w->stream;
-    launch_narrow(w->B, sp, w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->epa_list, w->cnt, s);
+    launch_narrow(w->B, sp, w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->epa_list, w->p_kind, w->pkinds,
+                  w->p_order, 4u * (uint32_t)w->cu_count, w->cnt, s);
     launch_flag_nonzero(w->p_np, w->cap_pairs, w->p_flag, w->cnt, s);
     scan_exclusive(w->p_flag, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds);
     launch_compact(w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->man_off, w->cap_man, w->m_ab, w->m_np,
@@ -519,7 +531,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[0], s));
-    launch_counters_begin(w->cnt, s);
+    launch_counters_begin(w->cnt, w->pkinds, s);
     enqueue_broad(w, sp);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[1], s));
     if (o.flags & AXM_STEP_KEEP_STAGES) {
@@ -852,6 +864,7 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     }
     axm_status st = w->mem.get(w->cnt, 1);
     if (st == AXM_SUCCESS) st = w->mem.get(w->bar, 1);
+    if (st == AXM_SUCCESS) st = w->mem.get(w->pkinds, 32);
     if (st == AXM_SUCCESS) st = w->mem.get(w->col_off, MAX_SORT_COLOURS + 2);
     if (st == AXM_SUCCESS) st = alloc_bodies(w, std::max<uint32_t>(desc->max_bodies, 16u));
     // explicit capacities are honoured as given; 0 means "sized from the body count"
@@ -1144,7 +1157,7 @@ axm_status axm_broadphase(axm_world w, uint32_t* count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 3; ++attempt) {
-        launch_counters_begin(w->cnt, w->stream);
+        launch_counters_begin(w->cnt, w->pkinds, w->stream);
         if (w->n) enqueue_broad(w, sp);
         AXM_TRY(read_counters(w));
         if (!(w->h_cnt->sticky & STICKY_PAIRS)) break;
@@ -1165,7 +1178,7 @@ axm_status axm_collide(axm_world w, uint32_t* manifold_count) {
     (void)hipSetDevice(w->desc.device);
     const StepParams sp = make_params(w, 1.0f / 120.0f);
     for (int attempt = 0; attempt < 4; ++attempt) {
-        launch_counters_begin(w->cnt, w->stream);
+        launch_counters_begin(w->cnt, w->pkinds, w->stream);
         if (w->n) {
             enqueue_broad(w, sp);
             enqueue_narrow(w, sp);
