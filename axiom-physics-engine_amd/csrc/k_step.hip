@@ -29,9 +29,16 @@ __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.
 
 // ---------------------------------------------------------------- k_begin
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
-                                               uint32_t* __restrict__ val) {
+                                               uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
+                                               uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
+                                               uint32_t* __restrict__ bdeg) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // the colouring's per-body words start from zero every step
+    bmax0[i] = 0ull;
+    bmax1[i] = 0ull;
+    bmask[i] = 0ull;
+    bdeg[i] = 0u;
     const float4 x4 = B.x[i], q4v = B.q[i], v4 = B.vel[i], w4 = B.angv[i], ii = B.ii[i], sh = B.shape[i];
     const uint32_t type = fbits(sh.w);
     const bool dyn = v4.w > 0.0f;
@@ -491,88 +498,171 @@ __device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32
     return false;
 }
 
-__global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restrict__ m_ab, uint32_t n,
-                                                uint32_t* __restrict__ col, uint64_t* __restrict__ prio,
-                                                uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmax1,
-                                                uint64_t* __restrict__ bmask, uint32_t* __restrict__ bdeg,
-                                                GridBar* bar, Counters* cnt, uint32_t max_rounds, PrevCache pc,
-                                                uint32_t recolour_period) {
+// Pass 1 (one thread per manifold, no barrier): degrees, priorities, the kept
+// colours of persisting contacts (with their mask bits) and, for every other
+// manifold, its round-0 tag at its dynamic bodies and an entry in the list of
+// manifolds to colour (cnt->remaining counts them).  The per-body arrays were
+// cleared by k_begin.  Temporal coherence: between full recolourings (every
+// recolour_period steps) a manifold whose body pair was in contact in the
+// previous step keeps that step's colour (the kept ones stay a valid colouring:
+// a subset of one); Jones-Plassmann then colours only the others around them.
+__global__ void __launch_bounds__(256) k_colour_keep(Bodies B, const uint2* __restrict__ m_ab,
+                                                     uint32_t* __restrict__ col, uint64_t* __restrict__ prio,
+                                                     uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmask,
+                                                     uint32_t* __restrict__ bdeg, uint32_t* __restrict__ list,
+                                                     Counters* cnt, PrevCache pc, uint32_t recolour_period) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
-    if (M == 0) return;  // uniform across the grid: no barrier is entered
-    const uint32_t nb = gridDim.x;
-    const uint32_t gs = nb * blockDim.x;
-    const uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t i = g0; i < n; i += gs) {
-        bmax0[i] = 0ull;
-        bmax1[i] = 0ull;
-        bmask[i] = 0ull;
-        bdeg[i] = 0u;
-    }
-    for (uint32_t m = g0; m < M; m += gs) {
-        const uint2 ab = m_ab[m];
-        col[m] = UNSET;
-        prio[m] = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
-    }
-    if (g0 == 0) cnt->remaining = M;
-    grid_sync(bar, nb, cnt);
-    // Temporal coherence: between full recolourings (every recolour_period steps)
-    // a manifold whose body pair was in contact in the previous step keeps that
-    // step's colour (the kept ones stay a valid colouring: a subset of one);
-    // Jones-Plassmann then colours only the new manifolds around them.
     const bool incremental = pc.valid != 0u && (cnt->steps_done % recolour_period) != 0u;
-    uint32_t kept = 0;
-    for (uint32_t m = g0; m < M; m += gs) {
+    bool fresh = false;
+    if (m < M) {
         const uint2 ab = m_ab[m];
         const bool bd = dyn_body(B, ab.y);
         atomicAdd(&bdeg[ab.x], 1u);
         if (bd) atomicAdd(&bdeg[ab.y], 1u);
-        uint32_t ps;
+        const uint64_t pr = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
+        prio[m] = pr;
+        uint32_t c = UNSET, ps;
         if (incremental && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps)) {
-            const uint32_t c = pc.colour[ps];
-            if (c < 64u) {
-                col[m] = c;
-                atomicOr((unsigned long long*)&bmask[ab.x], 1ull << c);
-                if (bd) atomicOr((unsigned long long*)&bmask[ab.y], 1ull << c);
-                atomicMax(&cnt->max_colour, c + 1u);
-                ++kept;
-                continue;
-            }
+            const uint32_t pcol = pc.colour[ps];
+            if (pcol < 64u) c = pcol;
         }
-        const uint64_t t = (1ull << 48) | prio[m];
-        atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
-        if (bd) atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
+        col[m] = c;
+        if (c != UNSET) {
+            atomicOr((unsigned long long*)&bmask[ab.x], 1ull << c);
+            if (bd) atomicOr((unsigned long long*)&bmask[ab.y], 1ull << c);
+            atomicMax(&cnt->max_colour, c + 1u);
+        } else {
+            const uint64_t t = (1ull << 48) | pr;
+            atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
+            if (bd) atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
+            fresh = true;
+        }
     }
-    if (kept) atomicSub(&cnt->remaining, kept);
-    grid_sync(bar, nb, cnt);
-    if (__hip_atomic_load(&cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;  // uniform
+    // list entry per manifold to colour: one atomic per wave
+    const unsigned long long fb = __ballot(fresh);
+    if (fb) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll(fb) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cnt->remaining, (uint32_t)__popcll(fb));
+        base = __shfl(base, leader, 64);
+        if (fresh) list[base + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull))] = m;
+    }
+}
+
+// One Jones-Plassmann visit of list entry m in round r: a manifold holding the
+// round's tag at all its dynamic bodies takes the first colour free in both
+// bodies' 64-bit masks (64 + r once they are full); a loser offers its next tag
+// into the other buffer.  Winners of one round share no dynamic body, so their
+// mask updates cannot collide; tags grow with the round, so the buffers never
+// need clearing.  AT: shared words through agent-scope atomics (the one-workgroup
+// form, which has no cache invalidation between rounds).
+template <bool AT>
+__device__ __forceinline__ void jp_visit(const Bodies& B, const uint2* __restrict__ m_ab, uint32_t* __restrict__ col,
+                                         const uint64_t* __restrict__ prio, const uint64_t* cur, uint64_t* nxt,
+                                         uint64_t* bmask, uint32_t m, uint32_t r, uint32_t& won, uint32_t& cmax) {
+    if (col[m] != UNSET) return;
+    auto ld = [](const uint64_t* p) -> uint64_t {
+        if constexpr (AT) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else return *p;
+    };
+    auto st = [](uint64_t* p, uint64_t v) {
+        if constexpr (AT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *p = v;
+    };
+    const uint2 ab = m_ab[m];
+    const bool bd = dyn_body(B, ab.y);
+    const uint64_t t = ((uint64_t)(r + 1) << 48) | prio[m];
+    if (ld(&cur[ab.x]) == t && (!bd || ld(&cur[ab.y]) == t)) {
+        const uint64_t ma = ld(&bmask[ab.x]), mb = bd ? ld(&bmask[ab.y]) : 0ull;
+        const uint64_t used = ma | mb;
+        uint32_t c;
+        if (~used == 0ull) {
+            c = 64u + r;
+        } else {
+            c = (uint32_t)__ffsll((unsigned long long)~used) - 1u;
+            st(&bmask[ab.x], ma | (1ull << c));
+            if (bd) st(&bmask[ab.y], mb | (1ull << c));
+        }
+        col[m] = c;
+        ++won;
+        cmax = c + 1u > cmax ? c + 1u : cmax;
+    } else {
+        const uint64_t t2 = ((uint64_t)(r + 2) << 48) | prio[m];
+        atomicMax((unsigned long long*)&nxt[ab.x], (unsigned long long)t2);
+        if (bd) atomicMax((unsigned long long*)&nxt[ab.y], (unsigned long long)t2);
+    }
+}
+
+// The rounds for a short list: one workgroup, a workgroup barrier per round
+// (every wave first waits for its own stores: they are agent-scope and must have
+// landed before another wave loads them).  Runs when 0 < list length <= small.
+__global__ void __launch_bounds__(1024) k_colour_small(Bodies B, const uint2* __restrict__ m_ab,
+                                                       uint32_t* __restrict__ col, const uint64_t* __restrict__ prio,
+                                                       uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask,
+                                                       const uint32_t* __restrict__ list, Counters* cnt,
+                                                       uint32_t max_rounds, uint32_t small) {
+    // per-round totals, double-buffered: round r's are read after its barrier while
+    // round r + 1's are cleared
+    __shared__ uint32_t s_won[2], s_cmax[2];
+    const uint32_t L = cnt->remaining;
+    if (L == 0u || L > small) return;  // uniform
+    if (threadIdx.x < 2u) {
+        s_won[threadIdx.x] = 0u;
+        s_cmax[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    uint32_t left = L;
+    for (uint32_t r = 0;; ++r) {
+        const uint32_t q = r & 1u;
+        const uint64_t* cur = q ? bmax1 : bmax0;
+        uint64_t* nxt = q ? bmax0 : bmax1;
+        uint32_t won = 0, cmax = 0;
+        for (uint32_t i = threadIdx.x; i < L; i += blockDim.x)
+            jp_visit<true>(B, m_ab, col, prio, cur, nxt, bmask, list[i], r, won, cmax);
+        if (won) {
+            atomicAdd(&s_won[q], won);
+            atomicMax(&s_cmax[q], cmax);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        const uint32_t w = s_won[q];
+        if (threadIdx.x == 0) {
+            if (w) {
+                atomicMax(&cnt->max_colour, s_cmax[q]);
+                atomicMax(&cnt->last_round, r + 1u);
+            }
+            s_won[q ^ 1u] = 0u;
+            s_cmax[q ^ 1u] = 0u;
+        }
+        left -= w;
+        if (left == 0u) break;
+        if (r + 1 >= max_rounds) {
+            if (threadIdx.x == 0) atomicOr(&cnt->sticky, STICKY_COLOURS);
+            break;
+        }
+    }
+    if (threadIdx.x == 0) cnt->remaining = 0u;
+}
+
+// The rounds for a long list (a full recolouring): persistent, one workgroup per
+// CU, a grid barrier per round.  Runs when the list is longer than `small`.
+__global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restrict__ m_ab, uint32_t* __restrict__ col,
+                                                const uint64_t* __restrict__ prio, uint64_t* __restrict__ bmax0,
+                                                uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
+                                                const uint32_t* __restrict__ list, GridBar* bar, Counters* cnt,
+                                                uint32_t max_rounds, uint32_t small) {
+    const uint32_t L = cnt->remaining;
+    if (L <= small) return;  // uniform across the grid: no barrier is entered
+    const uint32_t nb = gridDim.x;
+    const uint32_t gs = nb * blockDim.x;
+    const uint32_t g0 = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t r = 0;; ++r) {
         const uint64_t* cur = (r & 1u) ? bmax1 : bmax0;
         uint64_t* nxt = (r & 1u) ? bmax0 : bmax1;
         uint32_t won = 0, cmax = 0;
-        for (uint32_t m = g0; m < M; m += gs) {
-            if (col[m] != UNSET) continue;
-            const uint2 ab = m_ab[m];
-            const bool bd = dyn_body(B, ab.y);
-            const uint64_t t = ((uint64_t)(r + 1) << 48) | prio[m];
-            if (cur[ab.x] == t && (!bd || cur[ab.y] == t)) {
-                const uint64_t used = bmask[ab.x] | (bd ? bmask[ab.y] : 0ull);
-                uint32_t c;
-                if (~used == 0ull) {
-                    c = 64u + r;
-                } else {
-                    c = (uint32_t)__ffsll((unsigned long long)~used) - 1u;
-                    bmask[ab.x] = bmask[ab.x] | (1ull << c);
-                    if (bd) bmask[ab.y] = bmask[ab.y] | (1ull << c);
-                }
-                col[m] = c;
-                ++won;
-                cmax = c + 1u > cmax ? c + 1u : cmax;
-            } else {
-                const uint64_t t2 = ((uint64_t)(r + 2) << 48) | prio[m];
-                atomicMax((unsigned long long*)&nxt[ab.x], (unsigned long long)t2);
-                if (bd) atomicMax((unsigned long long*)&nxt[ab.y], (unsigned long long)t2);
-            }
-        }
+        for (uint32_t i = g0; i < L; i += gs) jp_visit<false>(B, m_ab, col, prio, cur, nxt, bmask, list[i], r, won, cmax);
         if (won) {
             atomicSub(&cnt->remaining, won);
             atomicMax(&cnt->max_colour, cmax);
@@ -1510,8 +1600,9 @@ static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b -
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
-void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val);
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg);
 }
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u64, grid_for(n), dim3(256), 0, s, p, v, n);
@@ -1577,11 +1668,16 @@ static PrevCache make_prev(const PrevCacheArgs& pca) {
     pc.valid = pca.valid;
     return pc;
 }
-void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
-                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
-                   const PrevCacheArgs& prev, hipStream_t s) {
-    hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, n, col, prio, bmax0, bmax1, bmask, bdeg, bar,
-                       cnt, 65000u, make_prev(prev), RECOLOUR_PERIOD);
+void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_t* col, uint64_t* prio,
+                   uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
+                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s) {
+    constexpr uint32_t SMALL = 16384;  // longest list coloured by one workgroup
+    hipLaunchKernelGGL(k_colour_keep, grid_for(cap_man), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmask, bdeg, list,
+                       cnt, make_prev(prev), RECOLOUR_PERIOD);
+    hipLaunchKernelGGL(k_colour_small, dim3(1), dim3(1024), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, cnt,
+                       65000u, SMALL);
+    hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, bar,
+                       cnt, 65000u, SMALL);
 }
 void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
                            hipStream_t s) {

@@ -43,7 +43,10 @@ struct PrevCacheArgs {
 
 // Clears the per-step counters and, when given, the 32 pair-kind counters.
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
-void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, hipStream_t s);
+// Start of a step: velocities, world inertia, AABBs, cell keys; clears the
+// colouring's per-body words.
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
@@ -68,9 +71,11 @@ void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, 
 int colour_blocks_per_cu();
 // Jones-Plassmann colouring; `prev` (previous step's (a, b) -> slot table and colours,
 // valid = a previous step exists) enables the incremental form between full recolourings.
-void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t n, uint32_t* col, uint64_t* prio, uint64_t* bmax0,
-                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, GridBar* bar, Counters* cnt, uint32_t blocks,
-                   const PrevCacheArgs& prev, hipStream_t s);
+// Colouring: the keep pass, then Jones-Plassmann rounds over the manifolds not
+// kept (`list`), in one workgroup for a short list or on the whole grid.
+void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_t* col, uint64_t* prio,
+                   uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
+                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s);
 void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
                            hipStream_t s);
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);

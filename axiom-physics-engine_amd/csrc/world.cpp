@@ -131,6 +131,7 @@ struct axm_world_s {
     uint32_t max_lds = 0;       // device limit of LDS per workgroup
 
     uint32_t* col = nullptr;
+    uint32_t* clist = nullptr;  // colouring: the manifolds not kept from the previous step
     uint64_t *prio = nullptr, *bmax0 = nullptr, *bmax1 = nullptr, *bmask = nullptr;
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
     uint32_t* col_off = nullptr;
@@ -400,6 +401,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->c_nf);
     w->mem.release(w->col);
     w->mem.release(w->prio);
+    w->mem.release(w->clist);
     w->mem.release(w->ckey);
     w->mem.release(w->ckey_alt);
     w->mem.release(w->cval);
@@ -411,6 +413,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->c_nf, (size_t)cap * 4));
     AXM_TRY(w->mem.get(w->col, cap));
     AXM_TRY(w->mem.get(w->prio, cap));
+    AXM_TRY(w->mem.get(w->clist, cap));
     AXM_TRY(w->mem.get(w->ckey, cap));
     AXM_TRY(w->mem.get(w->ckey_alt, cap));
     AXM_TRY(w->mem.get(w->cval, cap));
@@ -499,7 +502,7 @@ struct StepOpts {
 void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
-    launch_begin(w->B, sp, n, w->key, w->val, s);
+    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s);
     const uint64_t* skey = alt ? w->key_alt : w->key;
@@ -552,9 +555,9 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     pca.cap = w->rcap[pv];
     pca.valid = (w->prev_valid && pca.key) ? 1u : 0u;
     // ---- colouring and the stable order by colour
-    // bdone doubles as the per-body manifold count of the colouring (the sweep clears it before use)
-    launch_colour(w->B, w->m_ab, n, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->bar, w->cnt,
-                  w->colour_blocks, pca, s);
+    // bdone is the per-body manifold count of the colouring (cleared by k_begin)
+    launch_colour(w->B, w->m_ab, w->cap_man, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->clist,
+                  w->bar, w->cnt, w->colour_blocks, pca, s);
     launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
     radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff);
     const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers

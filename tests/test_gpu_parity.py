@@ -93,6 +93,26 @@ def test_pile_step_parity(axm, orc, gpu):
         _compare_step(axm, w, ow, 20)
 
 
+def test_grid_colouring_matches_oracle(axm, orc, gpu):
+    """A step whose colouring has more manifolds than the one-workgroup rounds take (a first
+    step, so every manifold is coloured, on a settled 50k-body pile): the grid-wide rounds
+    give the oracle's colouring, round count and state."""
+    b, d = axm.scene(1, seed(1), 50000)
+    w = axm.World(d)
+    w.set_bodies(b)
+    for _ in range(150):
+        w.step(DT, 20)
+    g = w.get_bodies()
+    nb = b.copy()
+    nb.pos, nb.quat, nb.lin_vel, nb.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
+    w2 = axm.World(d)
+    w2.set_bodies(nb)
+    ow = orc.OracleWorld(orc.params_from_desc(d))
+    ow.set_bodies(nb)
+    st, ost = _compare_step(axm, w2, ow, 20)
+    assert st.manifolds > 16384 and st.colour_rounds > 1
+
+
 def test_batched_worlds_match_solo(axm, orc, gpu):
     """Per-world results of a batch equal stepping each world alone."""
     b, d = axm.scene(3, seed(3), 0, 6)
