@@ -107,6 +107,19 @@ __device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v) {
     }
     return false;
 }
+// Continues a lookup of key k at slot s (linear probing).
+__device__ __forceinline__ bool ht_find_from(const CellTable& t, uint64_t k, uint32_t s, uint2& v) {
+    for (uint32_t probe = 0; probe <= t.mask; ++probe) {
+        const uint64_t kk = t.key[s];
+        if (kk == k) {
+            v = t.val[s];
+            return true;
+        }
+        if (kk == EMPTY_KEY) return false;
+        s = (s + 1) & t.mask;
+    }
+    return false;
+}
 __device__ __forceinline__ bool ht_find(const CellTable& t, uint64_t k, uint2& v) {
     uint32_t s = (uint32_t)mix64(k) & t.mask;
     for (uint32_t probe = 0; probe <= t.mask; ++probe) {
@@ -147,23 +160,6 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.vel[i].w > 0.0f ? 1.0f : 0.0f);
 }
 
-// Per cell, the union of its members' AABBs, stored at the cell's first sorted
-// position: a neighbour cell whose box misses a body's AABB holds no partner of it.
-__global__ void __launch_bounds__(256) k_cell_boxes(Bodies B, const uint64_t* __restrict__ skey, uint32_t n) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint64_t k = skey[s];
-    if (s > 0 && skey[s - 1] == k) return;
-    float4 lo = B.smin[s], hi = B.smax[s];
-    for (uint32_t t = s + 1; t < n && skey[t] == k; ++t) {
-        const float4 a = B.smin[t], b = B.smax[t];
-        lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.0f);
-        hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), fmaxf(hi.w, b.w));
-    }
-    B.cmin[s] = lo;
-    B.cmax[s] = hi;
-}
-
 template <class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t, F&& f) {
@@ -187,30 +183,33 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     for (int dz = -1; dz <= 1; ++dz) {
         const int Z = cz + dz;
         if (Z < 0 || Z > 1023) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int Y = cy + dy;
-            if (Y < 0 || Y > 1023) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int X = cx + dx;
-                if (X < 0 || X > 1023) continue;
-                const uint64_t nk = wbits | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
-                uint2 r;
-                if (!ht_find(ct, nk, r)) continue;
-                // skip a cell whose members' AABB union misses this body (or that has only
-                // static members when this body is static): it holds no partner
-                const float4 cl = B.cmin[r.x], ch = B.cmax[r.x];
-                if (mn.x > ch.x || mx.x < cl.x || mn.y > ch.y || mx.y < cl.y || mn.z > ch.z || mx.z < cl.z ||
-                    (!dyn_i && !(ch.w > 0.0f)))
-                    continue;
-                for (uint32_t s = r.x; s < r.x + r.y; ++s) {
-                    const float4 mnj = B.smin[s], mxj = B.smax[s];
-                    const uint32_t j = __float_as_uint(mnj.w);
-                    if (j <= i) continue;
-                    if (!dyn_i && !(mxj.w > 0.0f)) continue;
-                    if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
-                        mx.z >= mnj.z)
-                        f(j);
-                }
+        // the nine cells of this z slice: first probes of all nine in flight together
+        uint64_t nk[9], kk[9];
+        uint2 vv[9];
+        uint32_t hs[9];
+        bool ok[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int Y = cy + q / 3 - 1, X = cx + q % 3 - 1;
+            ok[q] = Y >= 0 && Y <= 1023 && X >= 0 && X <= 1023;
+            nk[q] = wbits | morton3((uint32_t)(ok[q] ? X : 0), (uint32_t)(ok[q] ? Y : 0), (uint32_t)Z);
+            hs[q] = (uint32_t)mix64(nk[q]) & ct.mask;
+            kk[q] = ct.key[hs[q]];
+            vv[q] = ct.val[hs[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {  // (dy, dx) in canonical order
+            if (!ok[q] || kk[q] == EMPTY_KEY) continue;
+            uint2 r = vv[q];
+            if (kk[q] != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
+            for (uint32_t s = r.x; s < r.x + r.y; ++s) {
+                const float4 mnj = B.smin[s], mxj = B.smax[s];
+                const uint32_t j = __float_as_uint(mnj.w);
+                if (j <= i) continue;
+                if (!dyn_i && !(mxj.w > 0.0f)) continue;
+                if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
+                    mx.z >= mnj.z)
+                    f(j);
             }
         }
     }
@@ -1620,7 +1619,6 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
-    hipLaunchKernelGGL(k_cell_boxes, grid_for(n), dim3(256), 0, s, B, skey, n);
     hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,

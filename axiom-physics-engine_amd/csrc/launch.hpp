@@ -52,8 +52,6 @@ struct Bodies {
     float4* amax;
     float4* smin;  // AABBs in sorted (cell) order, w = body index bits
     float4* smax;  // w = 1 for a dynamic body
-    float4* cmin;  // at a cell's first sorted position: union of its members' AABBs
-    float4* cmax;  // (w = 1 if the cell has a dynamic member)
     uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)
 };
 constexpr uint32_t PAIR_STASH = 24;  // partners kept per body by the counting pass

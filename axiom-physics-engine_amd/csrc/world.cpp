@@ -197,7 +197,7 @@ namespace {
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &B.smin, &B.smax, &B.cmin, &B.cmax, &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
     w->mem.release(B.vslot);
@@ -224,7 +224,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     free_bodies(w);
     Bodies& B = w->B;
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
-                      &B.smin, &B.smax, &B.cmin, &B.cmax, &w->pre_v, &w->pre_w};
+                      &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
