@@ -77,6 +77,10 @@ class KernelProfile(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("pgs_launches", C.c_uint64), ("pgs_ms", C.c_double),
                 ("pgs_bytes", C.c_double), ("sweeps", C.c_uint64)]
 
+
+class SweepConfig(C.Structure):
+    _fields_ = [("groups", C.c_uint32), ("lds_kib", C.c_uint32), ("policy", C.c_uint32), ("sweep_ms", C.c_double)]
+
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
@@ -90,7 +94,7 @@ EXPORTS = [
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
     "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
     "axm_mlp_forward", "axm_mlp_loss_grad", "axm_mlp_adam_step", "axm_mlp_train", "axm_world_set_mlp",
-    "axm_world_set_feature_logging", "axm_world_get_features", "axm_world_set_sweep_config",
+    "axm_world_set_feature_logging", "axm_world_get_features", "axm_world_set_sweep_config", "axm_world_autotune",
 ]
 MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 24, 3, 20099
 
@@ -147,6 +151,7 @@ def lib():
     L.axm_mlp_train.argtypes = [P, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, f32p]
     L.axm_world_set_mlp.argtypes = [P, P]
     L.axm_world_set_sweep_config.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+    L.axm_world_autotune.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(SweepConfig)]
     L.axm_world_set_feature_logging.argtypes = [P, C.c_int]
     L.axm_world_get_features.argtypes = [P, f32p, f32p, C.c_uint32, u32p]
     L.axm_rng_stream.argtypes = [C.c_uint64, C.c_uint32, u32p, C.c_float, C.c_float, f32p]
@@ -381,6 +386,13 @@ class World:
 
     def set_sweep_config(self, groups=0, lds_kib=0, policy=0x12):
         _check(self._L.axm_world_set_sweep_config(self._h, groups, lds_kib, policy), "axm_world_set_sweep_config")
+
+    def autotune(self, dt, iters, steps=5):
+        """Picks the fastest sweep launch shape on this world (advances it by a few steps per
+        candidate); returns (groups, lds_kib, policy, sweep_ms)."""
+        c = SweepConfig()
+        _check(self._L.axm_world_autotune(self._h, dt, iters, steps, C.byref(c)), "axm_world_autotune")
+        return c.groups, c.lds_kib, c.policy, c.sweep_ms
 
     def set_feature_logging(self, enable=True):
         _check(self._L.axm_world_set_feature_logging(self._h, 1 if enable else 0), "axm_world_set_feature_logging")

@@ -1324,6 +1324,53 @@ axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds
     return AXM_SUCCESS;
 }
 
+axm_status axm_world_autotune(axm_world w, float dt, uint32_t iters, uint32_t steps, axm_sweep_config* out) {
+    if (!w) return AXM_NULL_POINTER;
+    if (steps == 0 || iters == 0) return AXM_INVALID_PARAMETER;
+    (void)hipSetDevice(w->desc.device);
+    // candidates: 1, 2 and 4 workgroups per CU with the LDS split between them (more
+    // lanes, fewer slots each: better when a step has many manifolds per lane)
+    struct Cand {
+        uint32_t groups, lds_kib, policy;
+    };
+    std::vector<Cand> cands;
+    const uint32_t max_kib = (w->max_lds & ~16383u) / 1024u;
+    for (uint32_t per_cu : {1u, 2u, 4u}) {
+        const uint32_t kib = (max_kib / per_cu) & ~15u;
+        if (kib < 16u) continue;
+        const int fit = pgs_own_setup(w->max_lds, kib * 1024u);
+        if (fit < (int)per_cu) continue;
+        cands.push_back({per_cu * (uint32_t)w->cu_count, kib, 0x12u});
+    }
+    if (cands.empty()) return AXM_GPU_OPERATION_FAILED;
+    const bool was_on = w->prof_on;
+    double best = 1e300;
+    Cand pick = cands[0];
+    for (const Cand& c : cands) {
+        AXM_TRY(axm_world_set_sweep_config(w, c.groups, c.lds_kib, c.policy));
+        AXM_TRY(axm_world_run(w, dt, iters, 1, 1, nullptr));  // capture outside the timing
+        AXM_TRY(axm_world_profile(w, 1, nullptr));
+        AXM_TRY(axm_world_run(w, dt, iters, steps, 1, nullptr));
+        AXM_HIP(hipStreamSynchronize(w->stream));
+        axm_kernel_profile p{};
+        AXM_TRY(axm_world_profile(w, 0, &p));
+        const double ms = p.pgs_launches ? p.pgs_ms / (double)p.pgs_launches : 1e300;
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    AXM_TRY(axm_world_set_sweep_config(w, pick.groups, pick.lds_kib, pick.policy));
+    w->prof_on = was_on;
+    if (out) {
+        out->groups = pick.groups;
+        out->lds_kib = pick.lds_kib;
+        out->policy = pick.policy;
+        out->sweep_ms = best;
+    }
+    return AXM_SUCCESS;
+}
+
 axm_status axm_world_set_mlp(axm_world w, axm_mlp m) {
     if (!w) return AXM_NULL_POINTER;
     if (m && m->device != w->desc.device) return AXM_INVALID_PARAMETER;

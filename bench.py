@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default sweep launch shape")
     ap.add_argument("--no-tolerance", action="store_true", help="skip the plain vs learned run at residual <= tol")
     ap.add_argument("--tolerance", type=float, default=1e-4)
     ap.add_argument("--tolerance-worlds", type=int, default=4096)
@@ -174,6 +175,12 @@ def main():
         if (k + 1) % 50 == 0:
             log("warmup %d/%d (%.1fs)" % (k + 1, args.warmup, time.time() - t0))
     w.sync()
+    tuned = None
+    if not args.no_autotune:
+        g, lds, pol, ms = w.autotune(dt, iters, 5)
+        tuned = {"groups": g, "lds_kib": lds, "policy": pol, "sweep_ms": round(ms, 4)}
+        log("autotuned sweep: %d workgroups, %d KiB LDS, policy 0x%x (%.3f ms)" % (g, lds, pol, ms))
+    w.sync()
     w.profile(True)
     w.sync()
     t0 = time.perf_counter()
@@ -211,6 +218,7 @@ def main():
             "workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "sweeps": iters,
             "residual_tol": args.tol, "dt": dt, "mode": "plain GPU, previous-step warm start",
             "launch": "eager" if args.eager else "hipGraph replay",
+            "sweep_launch": tuned if tuned else "default (one workgroup per CU)",
         },
         "residual_last_step": diag.residual,
         "last_step": {k: diag.as_dict()[k] for k in ("pairs", "manifolds", "contacts", "colours", "colour_rounds",

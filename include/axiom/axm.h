@@ -194,6 +194,18 @@ axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out);
  * do not depend on it; throughput does (tuning hook, no reference counterpart). */
 axm_status axm_world_set_sweep_config(axm_world w, uint32_t groups, uint32_t lds_kib, uint32_t policy);
 
+/* The sweep launch shape chosen by axm_world_autotune. */
+typedef struct axm_sweep_config {
+    uint32_t groups, lds_kib, policy;
+    double sweep_ms; /* its average sweep-kernel time over the trial steps */
+} axm_sweep_config;
+/* Autotuner of the persistent sweep: times `steps` graph-replayed steps (after one
+ * untimed capture step) for each candidate launch shape (one, two and four
+ * workgroups per CU, LDS split between them), keeps the fastest and reports it.
+ * The world advances by (steps + 1) x candidates steps; results do not depend on the
+ * shape (no reference counterpart: the batched-world tuning hook). */
+axm_status axm_world_autotune(axm_world w, float dt, uint32_t iters, uint32_t steps, axm_sweep_config* out);
+
 /* Stage outputs of the last step (collision::ContactPoint / debug::DebugContactPoint,
  * include/axiom/debug/physics_debug_draw.hpp:128-132).  Undersized buffers return
  * OutOfRange with *count set to the required size. */

@@ -113,6 +113,25 @@ def test_grid_colouring_matches_oracle(axm, orc, gpu):
     assert st.manifolds > 16384 and st.colour_rounds > 1
 
 
+def test_autotune_keeps_results(axm, orc, gpu):
+    """The autotuner only picks a launch shape: a batch of worlds stepped with the shape it
+    picks ends bit for bit where a world with the default shape ends after as many steps."""
+    b, d = axm.scene(3, seed(3), 0, 64)
+    w = axm.World(d)
+    w.set_bodies(b)
+    w.step(DT, 10)
+    groups, lds, pol, ms = w.autotune(DT, 10, 2)
+    assert groups > 0 and lds > 0 and ms > 0
+    for _ in range(5):
+        w.step(DT, 10)
+    ref = axm.World(d)
+    ref.set_bodies(b)
+    for _ in range(1 + 3 * 3 + 5):  # the autotuner's candidates step (2 + 1) times each
+        ref.step(DT, 10)
+    g, r = w.get_bodies(), ref.get_bodies()
+    assert np.array_equal(g.pos, r.pos) and np.array_equal(g.lin_vel, r.lin_vel)
+
+
 def test_batched_worlds_match_solo(axm, orc, gpu):
     """Per-world results of a batch equal stepping each world alone."""
     b, d = axm.scene(3, seed(3), 0, 6)
