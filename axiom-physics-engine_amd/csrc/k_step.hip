@@ -94,13 +94,14 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
 // Linear-probing tables.  Every probe loop visits at most mask + 1 slots, so a
 // full table ends the loop (insert returns false, lookup misses) instead of
 // spinning; tables are sized >= 2x their maximum entry count.
+__device__ __forceinline__ uint64_t ent_key(uint4 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
 __device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v) {
     uint32_t s = (uint32_t)mix64(k) & t.mask;
     for (uint32_t probe = 0; probe <= t.mask; ++probe) {
         const unsigned long long prev =
-            atomicCAS((unsigned long long*)&t.key[s], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+            atomicCAS((unsigned long long*)&t.ent[s], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
         if (prev == EMPTY_KEY || prev == k) {
-            t.val[s] = v;
+            reinterpret_cast<uint2*>(&t.ent[s])[1] = v;
             return true;
         }
         s = (s + 1) & t.mask;
@@ -110,9 +111,10 @@ __device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v) {
 // Continues a lookup of key k at slot s (linear probing).
 __device__ __forceinline__ bool ht_find_from(const CellTable& t, uint64_t k, uint32_t s, uint2& v) {
     for (uint32_t probe = 0; probe <= t.mask; ++probe) {
-        const uint64_t kk = t.key[s];
+        const uint4 e = t.ent[s];
+        const uint64_t kk = ent_key(e);
         if (kk == k) {
-            v = t.val[s];
+            v = make_uint2(e.z, e.w);
             return true;
         }
         if (kk == EMPTY_KEY) return false;
@@ -120,18 +122,13 @@ __device__ __forceinline__ bool ht_find_from(const CellTable& t, uint64_t k, uin
     }
     return false;
 }
-__device__ __forceinline__ bool ht_find(const CellTable& t, uint64_t k, uint2& v) {
-    uint32_t s = (uint32_t)mix64(k) & t.mask;
-    for (uint32_t probe = 0; probe <= t.mask; ++probe) {
-        const uint64_t kk = t.key[s];
-        if (kk == k) {
-            v = t.val[s];
-            return true;
-        }
-        if (kk == EMPTY_KEY) return false;
-        s = (s + 1) & t.mask;
-    }
-    return false;
+// Occupancy bit of a cell hash (the high half of the hash; the slot takes the low bits).
+__device__ __forceinline__ uint32_t cell_bit(uint64_t h, uint32_t bmask) { return (uint32_t)(h >> 32) & bmask; }
+
+__global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
+    if (i <= (t.bmask >> 5)) t.bits[i] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
@@ -143,6 +140,9 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
     uint32_t c = 1;
     while (s + c < n && skey[s + c] == k) ++c;
     if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
+    const uint32_t bit = cell_bit(mix64(k), t.bmask);
+    atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
+    atomicAdd(&cnt->cells, 1u);
 }
 
 // ---------------------------------------------------------------- pairs
@@ -162,7 +162,8 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
 
 template <class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
-                                            const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t, F&& f) {
+                                            const uint32_t* __restrict__ sval, const CellTable& ct, bool sparse,
+                                            uint32_t t, F&& f) {
     const uint64_t key = skey[t];
     const float4 mn = B.smin[t], mx = B.smax[t];
     const uint32_t i = __float_as_uint(mn.w);
@@ -183,9 +184,11 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     for (int dz = -1; dz <= 1; ++dz) {
         const int Z = cz + dz;
         if (Z < 0 || Z > 1023) continue;
-        // the nine cells of this z slice: first probes of all nine in flight together
-        uint64_t nk[9], kk[9];
-        uint2 vv[9];
+        // the nine cells of this z slice: first probes of all nine in flight together; in a
+        // sparse scene (few bodies per occupied cell) the occupancy bits, which stay in
+        // the cache, are read first and only occupied cells are probed
+        uint64_t nk[9];
+        uint4 e[9];
         uint32_t hs[9];
         bool ok[9];
 #pragma unroll
@@ -193,15 +196,22 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
             const int Y = cy + q / 3 - 1, X = cx + q % 3 - 1;
             ok[q] = Y >= 0 && Y <= 1023 && X >= 0 && X <= 1023;
             nk[q] = wbits | morton3((uint32_t)(ok[q] ? X : 0), (uint32_t)(ok[q] ? Y : 0), (uint32_t)Z);
-            hs[q] = (uint32_t)mix64(nk[q]) & ct.mask;
-            kk[q] = ct.key[hs[q]];
-            vv[q] = ct.val[hs[q]];
+            const uint64_t h = mix64(nk[q]);
+            hs[q] = (uint32_t)h & ct.mask;
+            if (sparse) {
+                const uint32_t bit = cell_bit(h, ct.bmask);
+                ok[q] = ok[q] && ((ct.bits[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
+            }
         }
 #pragma unroll
+        for (int q = 0; q < 9; ++q)
+            e[q] = ok[q] ? ct.ent[hs[q]] : make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
+#pragma unroll
         for (int q = 0; q < 9; ++q) {  // (dy, dx) in canonical order
-            if (!ok[q] || kk[q] == EMPTY_KEY) continue;
-            uint2 r = vv[q];
-            if (kk[q] != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
+            const uint64_t kk = ent_key(e[q]);
+            if (!ok[q] || kk == EMPTY_KEY) continue;
+            uint2 r = make_uint2(e[q].z, e[q].w);
+            if (kk != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
             for (uint32_t s = r.x; s < r.x + r.y; ++s) {
                 const float4 mnj = B.smin[s], mxj = B.smax[s];
                 const uint32_t j = __float_as_uint(mnj.w);
@@ -219,12 +229,14 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
 // emit pass only re-visits the grid for the few bodies with more.
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
-                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash) {
+                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
+                                                    const Counters* __restrict__ ctr) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    const bool sparse = (uint64_t)ctr->cells * 4u > n;  // fewer than four bodies per occupied cell
     uint32_t c = 0;
     uint32_t* st = stash + (size_t)t * PAIR_STASH;
-    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t j) {
+    visit_pairs(B, sp, skey, sval, ct, sparse, t, [&](uint32_t j) {
         if (c < PAIR_STASH) st[c] = j;
         ++c;
     });
@@ -243,6 +255,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
         if (P > cap) atomicOr(&cnt->sticky, STICKY_PAIRS);
     }
     if (t >= n) return;
+    const bool sparse = (uint64_t)cnt->cells * 4u > n;
     uint32_t o = off[t];
     const uint32_t c = off[t + 1] - o;
     const uint32_t i = sval[t];
@@ -254,7 +267,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
         }
         return;
     }
-    visit_pairs(B, sp, skey, sval, ct, t, [&](uint32_t j) {
+    visit_pairs(B, sp, skey, sval, ct, sparse, t, [&](uint32_t j) {
         if (o < cap) {
             pa[o] = i;
             pb[o] = j;
@@ -1612,14 +1625,19 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
 }
+void launch_cell_clear(const CellTable& t, hipStream_t s) {
+    const uint32_t n = std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
+    hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
+}
 void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, hipStream_t s) {
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, const Counters* ctr,
+                       hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
-    hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash);
+    hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash, ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,

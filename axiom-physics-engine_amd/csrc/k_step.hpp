@@ -50,11 +50,14 @@ void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* k
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
+// Empties the cell table and its occupancy bits.
+void launch_cell_clear(const CellTable& t, hipStream_t s);
 void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s);
 // Pairs per body (stashing the first PAIR_STASH partners), then the emit pass in
 // the canonical order into pa / pb at the scanned offsets.
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, hipStream_t s);
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, const Counters* ctr,
+                       hipStream_t s);
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
                       Counters* cnt, const uint32_t* stash, hipStream_t s);

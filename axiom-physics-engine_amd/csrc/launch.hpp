@@ -56,10 +56,14 @@ struct Bodies {
 };
 constexpr uint32_t PAIR_STASH = 24;  // partners kept per body by the counting pass
 
+// Cell hash table (linear probing): entry = (key lo, key hi, first sorted index,
+// count), one 16-byte access per probe; plus an occupancy bitmap (one bit per
+// hashed cell) that sparse scenes test before probing.
 struct CellTable {
-    uint64_t* key;
-    uint2* val;  // (start, count) in sorted order
+    uint4* ent;
     uint32_t mask;
+    uint32_t* bits;
+    uint32_t bmask;  // bit index mask (power of two minus one)
 };
 
 // Device-resident step control block.  Every count a later kernel needs is read
@@ -79,7 +83,7 @@ struct Counters {
     uint32_t residual_bits;
     uint32_t p_eff;        // min(pairs, pair capacity)
     uint32_t m_eff;        // min(manifolds, manifold capacity)
-    uint32_t queue;        // persistent sweep: next chunk
+    uint32_t cells;        // broadphase: occupied cells
     uint32_t pad0[2];
     unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
     // ---- kept across steps

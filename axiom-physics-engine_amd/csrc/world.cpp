@@ -208,8 +208,8 @@ void free_bodies(axm_world_s* w) {
     w->mem.release(w->val);
     w->mem.release(w->val_alt);
     w->mem.release(w->sort_tmp);
-    w->mem.release(w->ct.key);
-    w->mem.release(w->ct.val);
+    w->mem.release(w->ct.ent);
+    w->mem.release(w->ct.bits);
     w->mem.release(w->pair_cnt);
     w->mem.release(w->pair_off);
     w->mem.release(w->scan_tmp);
@@ -238,9 +238,11 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     // the scans' look-back scratch must start zeroed (each scan leaves it zeroed)
     if (hipMemset(w->sort_tmp, 0, 4 * radix_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     const uint32_t T = pow2_at_least(2ull * cap + 16);
-    AXM_TRY(w->mem.get(w->ct.key, T));
-    AXM_TRY(w->mem.get(w->ct.val, T));
+    AXM_TRY(w->mem.get(w->ct.ent, T));
     w->ct.mask = T - 1;
+    const uint32_t TB = std::min<uint32_t>(T, 1u << 24) * 8u;  // occupancy bits: 8 per table entry, at most 128 Mi
+    AXM_TRY(w->mem.get(w->ct.bits, TB / 32u));
+    w->ct.bmask = TB - 1;
     AXM_TRY(w->mem.get(w->pair_cnt, cap));
     AXM_TRY(w->mem.get(w->pair_off, cap + 1));
     AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
@@ -507,9 +509,9 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s);
     const uint64_t* skey = alt ? w->key_alt : w->key;
     const uint32_t* sval = alt ? w->val_alt : w->val;
-    launch_fill_u64(w->ct.key, EMPTY_KEY, w->ct.mask + 1, s);
+    launch_cell_clear(w->ct, s);
     launch_cell_table(skey, n, w->ct, w->cnt, s);
-    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, s);
+    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, w->cnt, s);
     scan_exclusive(w->pair_cnt, w->pair_off, n, w->scan_tmp, s, &w->cnt->pairs);
     launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
                      s);
