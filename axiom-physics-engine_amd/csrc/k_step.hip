@@ -1211,6 +1211,31 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
     }
 }
 
+// Residual of one contact point: the largest velocity-space change (impulse change
+// times the row's effective mass inverse) that one more visit would make along the
+// normal or either friction direction, from the current velocities and impulses L.
+// J points at the point's first Jacobian row, rs is the row stride.
+__device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool plane, float mu, const SB& A,
+                                                const SB& Bd, const float4 L) {
+    float mN, kN, tg, mT1, kT1, mT2, kT2, x2, x3, x4;
+    const Dir Dn = load_dir(J, rs, 0, plane, mN, kN, x2, tg, x4);
+    const Dir D1 = load_dir(J, rs, 1, plane, mT1, kT1, x2, x3, x4);
+    const Dir D2 = load_dir(J, rs, 2, plane, mT2, kT2, x2, x3, x4);
+    float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
+    nl = nl > 0.0f ? nl : 0.0f;
+    float e = fabsf(nl - L.x) * kN;
+    const float maxf = mu * L.x;
+    float q1 = fmaf(-mT1, jvel(D1, A, Bd), L.y);
+    q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
+    const float e1 = fabsf(q1 - L.y) * kT1;
+    float q2 = fmaf(-mT2, jvel(D2, A, Bd), L.z);
+    q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
+    const float e2 = fabsf(q2 - L.z) * kT2;
+    if (e1 > e) e = e1;
+    if (e2 > e) e = e2;
+    return e;
+}
+
 // What a visit needs of its slot besides the rows: the slot, body indices and point count,
 // dependency words, the next visits of its bodies, inverse masses and the
 // velocity of a static partner (constant during the sweep).
@@ -1268,9 +1293,12 @@ constexpr uint32_t OWN_SLOTS = 16;  // slots per lane with a staged record
 // point q of a lane keeps row r at lds[P0 + (r * ppl + q) * 64 + lane] (rows
 // 0..JROWS-1 the Jacobian rows of R.J[(k * JROWS + r) * cap + s], row JROWS its
 // impulses).  Slots and points beyond what fits are read from the row arrays.
+// With stop.bits, the sweeps run in chunks of stop.every with a residual check
+// after each chunk (every wave of the grid meets at one grid barrier per check).
 __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, SolverRows R, Counters* cnt,
                                                       uint32_t cap, uint32_t sweep0, uint32_t nsweeps,
-                                                      uint32_t lds_f4, uint32_t spin_limit, uint32_t tune) {
+                                                      uint32_t lds_f4, uint32_t spin_limit, uint32_t tune,
+                                                      const uint64_t* __restrict__ bmask, SweepStop stop) {
     extern __shared__ float4 own_lds[];
     const uint32_t lane = threadIdx.x;
     const uint32_t M = cnt->m_eff;
@@ -1322,14 +1350,108 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // the row arrays.  Two instances, so that the first never waits on a memory
     // load other than its poll (a wait for a load also waits for the stores before it).
     const float mu = sp.friction;
-    const uint32_t total = nv * nsweeps;
     // tune bits 4-6: poll back-off level; bit 1: poll the hint pair before the full load
     // (inboxes within 2 GiB); bit 2: adaptive back-off
     const uint32_t nap = (tune >> 4) & 7u;
     const bool hint = (tune & 2u) != 0u && (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull,
                adaptive = (tune & 4u) != 0u;
     const uint64_t t_start = wall_clock64();
+    const uint64_t t_limit = (uint64_t)spin_limit * 100ull;  // wall clock: 100 MHz
+
+    // Final velocity of dynamic body `body` after `sweeps` sweeps: the inbox of its
+    // first visit (lowest colour) once it holds version sweeps * deg.  No visit of a
+    // later sweep runs before the check's grid barrier, so the words stay put.
+    auto final_vel = [&](uint32_t body, uint32_t dpw, uint32_t sweeps, SB& X) -> bool {
+        const uint32_t v0 = B.vslot[(size_t)body * 64u + (uint32_t)__ffsll((unsigned long long)bmask[body]) - 1u];
+        const uint32_t want = sweeps * (dpw >> 8), vs = v0 >> 1, side = v0 & 1u;
+        const uint64_t* ib = R.inbox;
+        for (;;) {
+            const uint64_t a0 = vload(ib + vin_word(vs, side, 0)), a1 = vload(ib + vin_word(vs, side, 1)),
+                           a2 = vload(ib + vin_word(vs, side, 2)), a3 = vload(ib + vin_word(vs, side, 3)),
+                           a4 = vload(ib + vin_word(vs, side, 4)), a5 = vload(ib + vin_word(vs, side, 5));
+            const uint32_t d = (vver(a0) ^ want) | (vver(a1) ^ want) | (vver(a2) ^ want) | (vver(a3) ^ want) |
+                               (vver(a4) ^ want) | (vver(a5) ^ want);
+            if (d == 0u) {
+                X.v = f3{vval(a0), vval(a1), vval(a2)};
+                X.w = f3{vval(a3), vval(a4), vval(a5)};
+                return true;
+            }
+            if (wall_clock64() - t_start > t_limit) {
+                atomicOr(err, STICKY_TIMEOUT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+    // Residual check after `sweeps` sweeps: the k_residual value over this lane's
+    // slots (rows and impulses where they are staged), wave max, one atomic max per
+    // wave into stop.bits[sweeps - 1], then an arrival count in stop.arrive; once
+    // every workgroup has arrived, every wave reads the total.  Everything the waves
+    // exchange here and in the sweep goes through device-scope atomics, so the
+    // barrier needs no cache writeback or invalidation.  Returns 1 to stop, 0 to go
+    // on, -1 on an error.
+    auto check = [&](uint32_t sweeps) -> int {
+        float res = 0.0f;
+        uint32_t off = 0;
+        for (uint32_t j = 0; j < nv; ++j) {
+            const Item it = j < nrec ? item_get(recs + (size_t)j * 4u * OWN_WAVE) : item_global(B, R, slot_of(j));
+            const bool plane = (it.h.y & PLANE_BIT) != 0u;
+            const uint32_t np = min(it.h.z, 4u);
+            SB A, Bd;
+            zero_sb(A);
+            zero_sb(Bd);
+            A.v = it.sv;
+            A.w = it.sw;
+            if (!plane) {
+                Bd.v = it.sv;
+                Bd.w = it.sw;
+            }
+            bool got = true;
+            if (it.dp.x != UNSET) got &= final_vel(it.h.x, it.dp.x, sweeps, A);
+            if (it.dp.y != UNSET) got &= final_vel(it.h.y, it.dp.y, sweeps, Bd);
+            if (!got) break;
+            const bool staged = off + np <= ppl;
+            for (uint32_t k = 0; k < np; ++k) {
+                const float4* J = staged ? mine + (size_t)(off + k) * OWN_WAVE : R.J + (size_t)k * JROWS * cap + it.s;
+                const size_t jr = staged ? rs : (size_t)cap;
+                const float4 L = staged ? J[JROWS * jr] : R.lam[(size_t)k * cap + it.s];
+                const float e = point_residual(J, jr, plane, mu, A, Bd, L);
+                if (e > res) res = e;
+            }
+            off += np;
+        }
+        uint32_t bits = __float_as_uint(res);
+        for (int o = 32; o >= 1; o >>= 1) {
+            const uint32_t ob = __shfl_xor(bits, o, 64);
+            bits = ob > bits ? ob : bits;
+        }
+        uint32_t* arr = stop.arrive + sweeps - 1u;
+        if (lane == 0u) {
+            atomicMax(stop.bits + sweeps - 1u, bits);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the maximum is in before the arrival
+            __hip_atomic_fetch_add(arr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        bool late = false;
+        while (__hip_atomic_load(arr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < G) {
+            if (wall_clock64() - t_start > t_limit) {
+                late = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (late) {
+            if (lane == 0u) atomicOr(err, STICKY_TIMEOUT);
+            return -1;
+        }
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return -1;
+        const float total_res =
+            __uint_as_float(__hip_atomic_load(stop.bits + sweeps - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        return total_res <= stop.tol ? 1 : 0;
+    };
+    const bool checks = stop.bits != nullptr;
     auto sweep_loop = [&]<bool STAGED>() -> bool {
+        uint32_t chunk_end = checks ? min(stop.every, nsweeps) : nsweeps;
+        uint32_t total = nv * chunk_end;
         uint32_t done = 0, sw = 0, j = 0, off = 0;
         Item it;
         it.s = 0u;
@@ -1435,16 +1557,34 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                     s = it.s;
                 }
             }
-            if (!__ballot(done < total)) return true;
-            if ((++spins & 255u) == 0u) {
-                bool stop = false;
-                if (wall_clock64() - t_start > (uint64_t)spin_limit * 100ull) {
-                    if (done < total) atomicOr(err, STICKY_TIMEOUT);  // waited longer than spin_limit microseconds
-                    stop = true;
-                } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-                    stop = true;  // another wave already gave up: drain
+            if (!__ballot(done < total)) {
+                if (!checks) return true;
+                // every slot of this wave is through sweep chunk_end - 1
+                const int r = check(chunk_end);
+                if (r < 0) return false;
+                if (r > 0 || chunk_end == nsweeps) {
+                    if (g == 0u && lane == 0u)
+                        __hip_atomic_store(&cnt->sweeps_run, chunk_end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return true;
                 }
-                if (stop) return false;
+                chunk_end = min(chunk_end + stop.every, nsweeps);
+                total = nv * chunk_end;
+                if (nv > 0u) {
+                    it = (STAGED || nrec > 0u) ? item_get(recs) : item_global(B, R, slot_of(0));
+                    s = it.s;
+                }
+                fresh = true;
+                continue;
+            }
+            if ((++spins & 255u) == 0u) {
+                bool quit = false;
+                if (wall_clock64() - t_start > t_limit) {
+                    if (done < total) atomicOr(err, STICKY_TIMEOUT);  // waited longer than spin_limit microseconds
+                    quit = true;
+                } else if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                    quit = true;  // another wave already gave up: drain
+                }
+                if (quit) return false;
             }
             // adaptive back-off: a wave none of whose lanes got through sleeps longer each time
             if (adaptive) idle = __ballot(ok) ? 0u : min(idle + 1u, 4u);
@@ -1491,24 +1631,8 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
         if (plane) zero_sb(Bd); else load_vel<false>(B, h.y, Bd);
         const float mu = sp.friction;
         for (uint32_t k = 0; k < h.z; ++k) {
-            const float4* J = R.J + (size_t)k * JROWS * cap + s;
-            float mN, kN, tg, mT1, kT1, mT2, kT2, x2, x3, x4;
-            const Dir Dn = load_dir(J, cap, 0, plane, mN, kN, x2, tg, x4);
-            const Dir D1 = load_dir(J, cap, 1, plane, mT1, kT1, x2, x3, x4);
-            const Dir D2 = load_dir(J, cap, 2, plane, mT2, kT2, x2, x3, x4);
-            const float4 L = R.lam[(size_t)k * cap + s];
-            float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
-            nl = nl > 0.0f ? nl : 0.0f;
-            float e = fabsf(nl - L.x) * kN;
-            const float maxf = mu * L.x;
-            float q1 = fmaf(-mT1, jvel(D1, A, Bd), L.y);
-            q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
-            const float e1 = fabsf(q1 - L.y) * kT1;
-            float q2 = fmaf(-mT2, jvel(D2, A, Bd), L.z);
-            q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
-            const float e2 = fabsf(q2 - L.z) * kT2;
-            if (e1 > e) e = e1;
-            if (e2 > e) e = e2;
+            const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, mu, A, Bd,
+                                           R.lam[(size_t)k * cap + s]);
             if (e > res) res = e;
         }
     }
@@ -1583,7 +1707,8 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
     if (cnt->sticky != 0u) return;
     if (i == 0) {
         cnt->steps_done += 1u;
-        cnt->pgs_bytes_total += cnt->pgs_bytes * (unsigned long long)sweeps;
+        const uint32_t sw = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
+        cnt->pgs_bytes_total += cnt->pgs_bytes * (unsigned long long)sw;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];
@@ -1752,12 +1877,12 @@ int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes) {
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
                      uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
                      uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
-                     hipEvent_t ev0, hipEvent_t ev1) {
+                     hipEvent_t ev0, hipEvent_t ev1, const SweepStop& stop) {
     if (!nsweeps) return;
     hipLaunchKernelGGL(k_flow_init, grid_for(cap), dim3(256), 0, s, B, R, bmask, cnt);
     if (ev0) (void)hipEventRecord(ev0, s);
     hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
-                       nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune);
+                       nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune, bmask, stop);
     if (ev1) (void)hipEventRecord(ev1, s);
     hipLaunchKernelGGL(k_flow_finish, grid_for(cap), dim3(256), 0, s, B, R, cnt);
 }

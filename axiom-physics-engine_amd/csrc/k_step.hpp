@@ -89,13 +89,25 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
 // Writes the learned warm start (network outputs Y, rows 4 * slot + point) into the impulse rows.
 void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, const float* Y, const StepParams& sp,
                       hipStream_t s);
+// Residual checks inside one persistent sweep launch: after every `every` sweeps and
+// after the last one, the residual (the k_residual value) is written to
+// bits[sweeps - 1]; the launch stops at the first check whose residual is <= tol
+// (never when tol < 0) and stores the sweeps it ran in Counters::sweeps_run.
+// arrive[sweeps - 1] counts the workgroups whose part of that check is in; both
+// arrays start the step at zero.  bits == nullptr means no checks.
+struct SweepStop {
+    uint32_t* bits = nullptr;
+    uint32_t* arrive = nullptr;
+    float tol = -1.0f;
+    uint32_t every = 1;
+};
 // Persistent sweep: `groups` workgroups of one wave, each with `lds_bytes` of LDS for
 // its staged rows; all of them must be resident at once (see pgs_own_setup).  Events
 // ev0 / ev1, when given, are recorded around the sweep kernel itself.
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
                      uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
                      uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
-                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const SweepStop& stop = SweepStop{});
 // Raises the sweep kernel's dynamic-LDS limit to max_lds_bytes and returns how many of
 // its workgroups with lds_bytes each fit on one CU (0 on error).
 int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes);
@@ -108,6 +120,8 @@ void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64
 // Rehashes the `on`-slot table (okey, oslot) into the empty table (key, slot, mask).
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
                          uint32_t mask, Counters* cnt, hipStream_t s);
+// sweeps == SWEEPS_ON_DEVICE: the count is read from Counters::sweeps_run.
+constexpr uint32_t SWEEPS_ON_DEVICE = 0xFFFFFFFFu;
 void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps, hipStream_t s);
 
 }  // namespace axm

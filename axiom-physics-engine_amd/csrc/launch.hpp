@@ -84,7 +84,8 @@ struct Counters {
     uint32_t p_eff;        // min(pairs, pair capacity)
     uint32_t m_eff;        // min(manifolds, manifold capacity)
     uint32_t cells;        // broadphase: occupied cells
-    uint32_t pad0[2];
+    uint32_t sweeps_run;   // sweeps of a launch with residual checks (set when it ends)
+    uint32_t pad0;
     unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
     // ---- kept across steps
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)

@@ -147,7 +147,7 @@ struct axm_world_s {
     int cur = 0;
     bool prev_valid = false;
 
-    uint32_t* res_bits = nullptr;
+    uint32_t* res_bits = nullptr;  // 2 x res_cap words: residual per sweep, then check arrivals
     uint32_t res_cap = 0;
     Counters* cnt = nullptr;
     Counters* h_cnt = nullptr;
@@ -169,6 +169,7 @@ struct axm_world_s {
     // last step
     bool have_step = false;
     uint32_t last_iters = 0;
+    bool last_iters_dev = false;  // the last step's sweep count is in Counters::sweeps_run
     std::vector<float> h_res;
 
     void drop_graphs() {
@@ -574,12 +575,13 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
                            2u * (uint32_t)w->cu_count, s);
         launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
     }
-    launch_fill_u32(w->res_bits, 0u, w->res_cap, s);
+    launch_fill_u32(w->res_bits, 0u, 2u * w->res_cap, s);  // residuals, then check arrivals
     return AXM_SUCCESS;
 }
 
-// Enqueues one step.  Synchronises only for per-colour sweep launches and for
-// residual-tolerance checks (neither is used under graph capture).
+// Enqueues one step.  Synchronises only for per-colour sweep launches (with a
+// residual tolerance they check it on the host); the persistent sweep checks the
+// residual itself, so a step with a tolerance enqueues without synchronising.
 axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     // every kernel below sizes its grid and row strides by the current set's capacity
     if (w->rcap[w->cur] < w->cap_man || w->rcap[1 - w->cur] < w->cap_man) return AXM_GPU_INVALID_OPERATION;
@@ -592,7 +594,12 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
     const bool tol_mode = w->desc.residual_tol > 0.0f && !o.capture;
-    const uint32_t max_it = tol_mode ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
+    const uint32_t max_it =
+        w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
+    // residual checks inside the sweep launch: with a tolerance, or every sweep's residual recorded
+    const bool dev_checks =
+        sweep && !o.colour_launches && (w->desc.residual_tol > 0.0f || w->desc.record_residual);
+    w->last_iters_dev = false;
     std::vector<uint32_t> off;
     if (pre) AXM_TRY(enqueue_pre(w, sp, o));
     const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
@@ -608,6 +615,20 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             AXM_HIP(hipMemcpyAsync(off.data(), w->col_off, 4ull * (K + 1), hipMemcpyDeviceToHost, s));
             AXM_HIP(hipStreamSynchronize(s));
         }
+    }
+    if (dev_checks) {
+        SweepStop st;
+        st.bits = w->res_bits;
+        st.arrive = w->res_bits + w->res_cap;
+        st.tol = w->desc.residual_tol > 0.0f ? w->desc.residual_tol : -1.0f;
+        st.every = w->desc.record_residual ? 1u : every;
+        hipEvent_t e0, e1;
+        AXM_TRY(prof_event(w, nev, o.capture, e0));
+        AXM_TRY(prof_event(w, nev, o.capture, e1));
+        launch_pgs_flow(w->B, sp, R, rc, w->bmask, 0, max_it, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups,
+                        w->sweep_lds, w->sweep_tune, s, e0, e1, st);
+        it = max_it;  // at most; the count the launch ran is read back with the counters
+        w->last_iters_dev = true;
     }
     while (it < max_it) {
         uint32_t seg = max_it - it;
@@ -646,7 +667,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_dt = dt;
     launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
-    launch_integrate(w->B, sp, n, w->cnt, it, s);
+    launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[4], s));
     if (w->prof_on && !o.capture) {
         AXM_HIP(hipStreamSynchronize(s));
@@ -657,11 +678,12 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             ms += e;
         }
         AXM_TRY(read_counters(w));
+        const uint32_t ran = w->last_iters_dev ? w->h_cnt->sweeps_run : it;
         w->prof.steps += 1;
         w->prof.pgs_launches += nev / 2;
         w->prof.pgs_ms += ms;
-        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)it;
-        w->prof.sweeps += it;
+        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)ran;
+        w->prof.sweeps += ran;
     }
     w->cur = pv;
     w->have_step = true;
@@ -715,7 +737,7 @@ axm_status ensure_res(axm_world_s* w, uint32_t iters) {
     const uint32_t need = std::max<uint32_t>(iters, w->desc.residual_tol > 0.0f ? w->desc.max_iterations : 0u) + 1;
     if (w->res_cap >= need) return AXM_SUCCESS;
     w->mem.release(w->res_bits);
-    AXM_TRY(w->mem.get(w->res_bits, need));
+    AXM_TRY(w->mem.get(w->res_bits, 2ull * need));
     w->res_cap = need;
     w->drop_graphs();
     return AXM_SUCCESS;
@@ -749,6 +771,7 @@ axm_status step_sync(axm_world_s* w, float dt, StepOpts o, axm_step_stats* st) {
             return es;
         }
         AXM_TRY(read_counters(w));
+        if (w->last_iters_dev) w->last_iters = w->h_cnt->sweeps_run;
         if (w->h_cnt->sticky == 0u) break;
         fill_stats(w, S);
         const axm_status r = recover(w, o);
@@ -793,6 +816,7 @@ axm_status capture(axm_world_s* w, float dt, uint32_t iters, int parity, uint32_
     const int saved_cur = w->cur;
     const bool saved_prev = w->prev_valid, saved_step = w->have_step, saved_pre = w->have_pre;
     const uint32_t saved_iters = w->last_iters;
+    const bool saved_dev = w->last_iters_dev;
     w->cur = parity;
     w->prev_valid = true;
     StepOpts o;
@@ -808,6 +832,7 @@ axm_status capture(axm_world_s* w, float dt, uint32_t iters, int parity, uint32_
     w->have_step = saved_step;
     w->have_pre = saved_pre;
     w->last_iters = saved_iters;
+    w->last_iters_dev = saved_dev;
     if (st != AXM_SUCCESS || ee != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return st != AXM_SUCCESS ? st : AXM_GPU_OPERATION_FAILED;

@@ -133,6 +133,7 @@ def main():
     ap.add_argument("--iters", type=int, default=0)
     ap.add_argument("--tol", type=float, default=0.0, help="iterate each step until the residual <= tol")
     ap.add_argument("--max-iters", type=int, default=200)
+    ap.add_argument("--tol-every", type=int, default=1, help="with --tol: sweeps between residual checks")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
@@ -162,7 +163,7 @@ def main():
     if args.tol > 0:
         desc.residual_tol = args.tol
         desc.max_iterations = args.max_iters
-        desc.residual_every = 1
+        desc.residual_every = max(1, args.tol_every)
     log("scene config %d: %d bodies" % (cfg, bodies.n))
     w = axm.World(desc)
     if os.environ.get("AXM_SWEEP"):
@@ -176,15 +177,33 @@ def main():
             log("warmup %d/%d (%.1fs)" % (k + 1, args.warmup, time.time() - t0))
     w.sync()
     tuned = None
-    if not args.no_autotune:
-        g, lds, pol, ms = w.autotune(dt, iters, 5)
+    if not args.no_autotune and args.tol <= 0:
+        # tuned on a copy of the warmed-up state, so the timed world's trajectory is untouched
+        gs = w.get_bodies()
+        tb = bodies.copy()
+        tb.pos, tb.quat, tb.lin_vel, tb.ang_vel = gs.pos, gs.quat, gs.lin_vel, gs.ang_vel
+        tw = axm.World(desc)
+        tw.set_bodies(tb)
+        tw.step(dt, iters, 0)
+        g, lds, pol, ms = tw.autotune(dt, iters, 5)
+        tw.close()
+        w.set_sweep_config(g, lds, pol)
         tuned = {"groups": g, "lds_kib": lds, "policy": pol, "sweep_ms": round(ms, 4)}
         log("autotuned sweep: %d workgroups, %d KiB LDS, policy 0x%x (%.3f ms)" % (g, lds, pol, ms))
     w.sync()
     w.profile(True)
     w.sync()
     t0 = time.perf_counter()
-    st = w.run(dt, iters, args.steps, graph=not args.eager)
+    sweeps, reached = [], []
+    if args.tol > 0:
+        # residual-stopped steps: the sweep launch checks the residual itself; the host reads
+        # each step's sweep count and residual once the step is done
+        for _ in range(args.steps):
+            st = w.step(dt, args.max_iters, axm.STEP_RESIDUAL)
+            sweeps.append(st.iterations)
+            reached.append(st.residual <= args.tol)
+    else:
+        st = w.run(dt, iters, args.steps, graph=not args.eager)
     w.sync()
     elapsed = time.perf_counter() - t0
     prof = w.profile(False)
@@ -216,7 +235,8 @@ def main():
         "data": "synthetic (seeded DeterministicRNG scene)",
         "config": {
             "workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "sweeps": iters,
-            "residual_tol": args.tol, "dt": dt, "mode": "plain GPU, previous-step warm start",
+            "residual_tol": args.tol, "dt": dt,
+            "mode": "plain GPU, previous-step warm start" + (", residual-stopped" if args.tol > 0 else ""),
             "launch": "eager" if args.eager else "hipGraph replay",
             "sweep_launch": tuned if tuned else "default (one workgroup per CU)",
         },
@@ -228,13 +248,22 @@ def main():
     }
     w_state = w.get_bodies() if not args.no_cpu else None
     w.close()
+    if args.tol > 0:
+        out["residual_stopped"] = {"residual_tol": args.tol, "max_sweeps": args.max_iters,
+                                   "check_every": max(1, args.tol_every),
+                                   "mean_sweeps": round(float(np.mean(sweeps)), 2),
+                                   "max_sweeps_used": int(np.max(sweeps)),
+                                   "fraction_reached": round(float(np.mean(reached)), 3)}
     if not args.no_tolerance:
         out["at_tolerance"] = tolerance_comparison(args.tolerance, args.max_iters * 3 // 2, args.tolerance_worlds)
     if not args.no_cpu:
         g = w_state
         state = bodies.copy()
         state.pos, state.quat, state.lin_vel, state.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
-        out["cpu_baseline"] = cpu_baseline(desc, state, iters, dt, args.cpu_seconds)
+        # at a residual tolerance the oracle (the same arithmetic, so the same residuals) runs the
+        # GPU's mean sweep count per step
+        cpu_iters = max(1, int(round(float(np.mean(sweeps))))) if args.tol > 0 else iters
+        out["cpu_baseline"] = cpu_baseline(desc, state, cpu_iters, dt, args.cpu_seconds)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 2)
     print(json.dumps(out), flush=True)
 
