@@ -8,9 +8,12 @@ physics step (broadphase, narrowphase, colouring, solve, integration) of the
 whole scene with its state resident in HBM.
 
 Prints one JSON line: body-steps/s, the roofline of the dominant kernel (the
-sweep kernel k_pgs: algorithmic bytes / summed event-timed duration of its
-launches during the timed region) and the CPU oracle timed on a bounded sample
-of the same workload on the host.
+persistent sweep kernel k_pgs_own: algorithmic bytes / summed event-timed
+duration of its launches during the timed region) and the CPU oracle timed on a
+bounded sample of the same workload on the host.  Two further sections: the
+1M-body scene iterated to contact residual <= 1e-4 every step (with the oracle
+at the same sweep count beside it), and the plain vs learned warm start at that
+residual on one 1k-body scene and on 4096 batched worlds.
 """
 import argparse
 import json
@@ -89,6 +92,47 @@ def tolerance_comparison(tol, max_sweeps, worlds):
     return out
 
 
+def scene_at_residual(cfg, tol, max_sweeps, steps, warmup, cpu_seconds, no_cpu):
+    """Scene `cfg` with every step iterated until its contact residual is <= tol (checked after
+    every sweep inside the sweep launch, capped at max_sweeps): body-steps/s, sweeps per step, and
+    the oracle at the GPU's mean sweep count on the state the GPU reached."""
+    dt = 1.0 / 120.0
+    bodies, desc = axm.scene(cfg, SEED_BASE + 1000 * cfg, 0, 0)
+    desc.residual_tol = tol
+    desc.max_iterations = max_sweeps
+    desc.residual_every = 1
+    w = axm.World(desc)
+    w.set_bodies(bodies)
+    for _ in range(warmup):
+        w.step(dt, DEFAULT_ITERS[cfg], 0)
+    w.sync()
+    sweeps, reached = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = w.step(dt, max_sweeps, axm.STEP_RESIDUAL)
+        sweeps.append(st.iterations)
+        reached.append(st.residual <= tol)
+    w.sync()
+    el = time.perf_counter() - t0
+    out = {"workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "residual_tol": tol,
+           "check_every": 1, "max_sweeps": max_sweeps, "warmup": warmup, "steps": steps,
+           "body_steps_per_s": round(bodies.n * steps / el, 1), "ms_per_step": round(el * 1e3 / steps, 4),
+           "mean_sweeps": round(float(np.mean(sweeps)), 2), "max_sweeps_used": int(np.max(sweeps)),
+           "fraction_reached": round(float(np.mean(reached)), 3)}
+    log("scene %d at residual %g: %.0f body-steps/s, mean sweeps %.1f" % (cfg, tol, out["body_steps_per_s"],
+                                                                         np.mean(sweeps)))
+    if not no_cpu:
+        g = w.get_bodies()
+        state = bodies.copy()
+        state.pos, state.quat, state.lin_vel, state.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
+        desc.residual_tol = 0.0
+        out["cpu_baseline"] = cpu_baseline(desc, state, max(1, int(round(float(np.mean(sweeps))))), dt,
+                                           cpu_seconds)
+        out["gpu_over_cpu"] = round(out["body_steps_per_s"] / out["cpu_baseline"]["value"], 2)
+    w.close()
+    return out
+
+
 def measured_traffic(cfg, n, iters, args):
     """HBM bytes per sweep-kernel launch from the committed counter runs (rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE, scaled by a plain copy of known size), when they cover this exact workload; else None."""
@@ -139,6 +183,7 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sweep launch shape")
     ap.add_argument("--no-tolerance", action="store_true", help="skip the plain vs learned run at residual <= tol")
+    ap.add_argument("--no-scene-1m", action="store_true", help="skip the 1M-body scene at residual <= tol")
     ap.add_argument("--tolerance", type=float, default=1e-4)
     ap.add_argument("--tolerance-worlds", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -254,6 +299,8 @@ def main():
                                    "mean_sweeps": round(float(np.mean(sweeps)), 2),
                                    "max_sweeps_used": int(np.max(sweeps)),
                                    "fraction_reached": round(float(np.mean(reached)), 3)}
+    if not args.no_scene_1m and not (cfg == 2 and args.tol > 0):
+        out["scene_1m_at_residual"] = scene_at_residual(2, args.tolerance, 300, 20, 20, 10.0, args.no_cpu)
     if not args.no_tolerance:
         out["at_tolerance"] = tolerance_comparison(args.tolerance, args.max_iters * 3 // 2, args.tolerance_worlds)
     if not args.no_cpu:
