@@ -1709,6 +1709,7 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
         cnt->steps_done += 1u;
         const uint32_t sw = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
         cnt->pgs_bytes_total += cnt->pgs_bytes * (unsigned long long)sw;
+        cnt->sweeps_total += sw;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];

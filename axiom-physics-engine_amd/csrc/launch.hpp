@@ -91,6 +91,7 @@ struct Counters {
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
     uint32_t steps_done;   // steps whose integration ran
     unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
+    unsigned long long sweeps_total;     // sweeps of all completed steps
 };
 constexpr uint32_t COUNTERS_RESET_WORDS = 18;  // words cleared per step (through pgs_bytes)
 enum : uint32_t {

@@ -579,6 +579,17 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     return AXM_SUCCESS;
 }
 
+// The persistent sweep's residual checks (a tolerance stop or a per-sweep residual
+// history), when the step has any; the per-colour launches check on the host.
+bool sweep_checks(const axm_world_s* w, bool colour_launches, SweepStop& st) {
+    if (colour_launches || !(w->desc.residual_tol > 0.0f || w->desc.record_residual)) return false;
+    st.bits = w->res_bits;
+    st.arrive = w->res_bits + w->res_cap;
+    st.tol = w->desc.residual_tol > 0.0f ? w->desc.residual_tol : -1.0f;
+    st.every = w->desc.record_residual ? 1u : std::max<uint32_t>(w->desc.residual_every, 1u);
+    return true;
+}
+
 // Enqueues one step.  Synchronises only for per-colour sweep launches (with a
 // residual tolerance they check it on the host); the persistent sweep checks the
 // residual itself, so a step with a tolerance enqueues without synchronising.
@@ -597,9 +608,9 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     const uint32_t max_it =
         w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
     // residual checks inside the sweep launch: with a tolerance, or every sweep's residual recorded
-    const bool dev_checks =
-        sweep && !o.colour_launches && (w->desc.residual_tol > 0.0f || w->desc.record_residual);
-    w->last_iters_dev = false;
+    SweepStop stop;
+    const bool dev_checks = sweep_checks(w, o.colour_launches, stop);
+    w->last_iters_dev = dev_checks;
     std::vector<uint32_t> off;
     if (pre) AXM_TRY(enqueue_pre(w, sp, o));
     const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
@@ -616,21 +627,15 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             AXM_HIP(hipStreamSynchronize(s));
         }
     }
-    if (dev_checks) {
-        SweepStop st;
-        st.bits = w->res_bits;
-        st.arrive = w->res_bits + w->res_cap;
-        st.tol = w->desc.residual_tol > 0.0f ? w->desc.residual_tol : -1.0f;
-        st.every = w->desc.record_residual ? 1u : every;
+    if (dev_checks && sweep) {
         hipEvent_t e0, e1;
         AXM_TRY(prof_event(w, nev, o.capture, e0));
         AXM_TRY(prof_event(w, nev, o.capture, e1));
         launch_pgs_flow(w->B, sp, R, rc, w->bmask, 0, max_it, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups,
-                        w->sweep_lds, w->sweep_tune, s, e0, e1, st);
+                        w->sweep_lds, w->sweep_tune, s, e0, e1, stop);
         it = max_it;  // at most; the count the launch ran is read back with the counters
-        w->last_iters_dev = true;
     }
-    while (it < max_it) {
+    while (it < max_it && !dev_checks) {
         uint32_t seg = max_it - it;
         if (w->desc.record_residual) seg = 1;
         else if (tol_mode) seg = std::min(seg, every - (it % every));
@@ -1067,8 +1072,7 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
     if (!(dt > 0.0f) || !std::isfinite(dt)) return AXM_INVALID_PARAMETER;
     (void)hipSetDevice(w->desc.device);
     if (steps == 0) return AXM_SUCCESS;
-    const bool use_graph = graph && w->desc.residual_tol == 0.0f && !w->desc.record_residual &&
-                           w->desc.solver_mode == 0 && w->n > 0;
+    const bool use_graph = graph && !w->desc.record_residual && w->desc.solver_mode == 0 && w->n > 0;
     if (!use_graph) {
         StepOpts o;
         o.iters = iters;
@@ -1105,6 +1109,7 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         AXM_TRY(read_counters(w));
         const uint32_t base = w->h_cnt->steps_done;
         const unsigned long long base_bytes = w->h_cnt->pgs_bytes_total;
+        const unsigned long long base_sweeps = w->h_cnt->sweeps_total;
         const int cur0 = w->cur;
         const uint32_t todo = steps - done;
         // With profiling, the sweep kernel of step k-1 is timed while step k
@@ -1116,9 +1121,13 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             const int c = w->cur;
             if (w->prof_on) {
                 AXM_HIP(hipGraphLaunch(w->gpre[c], w->stream));
-                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, 0, iters, w->cnt,
-                                SWEEP_SPIN_LIMIT_US, w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream,
-                                w->gprof_ev[2 * c], w->gprof_ev[2 * c + 1]);
+                SweepStop stop;
+                const bool checks = sweep_checks(w, false, stop);
+                const uint32_t nsw = w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u)
+                                                                 : iters;
+                launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, 0, nsw, w->cnt, SWEEP_SPIN_LIMIT_US,
+                                w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream, w->gprof_ev[2 * c],
+                                w->gprof_ev[2 * c + 1], checks ? stop : SweepStop{});
                 AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
             } else {
                 AXM_HIP(hipGraphLaunch(w->gexec[c], w->stream));
@@ -1144,11 +1153,12 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             w->prof.pgs_launches += todo;
             w->prof.pgs_ms += prof_ms;
             w->prof.pgs_bytes += (double)(w->h_cnt->pgs_bytes_total - base_bytes);
-            w->prof.sweeps += (uint64_t)ok * iters;
+            w->prof.sweeps += w->h_cnt->sweeps_total - base_sweeps;
         }
         done += ok;
         w->have_step = true;
-        w->last_iters = iters;
+        // with a residual tolerance each step ran the sweeps its own checks chose
+        w->last_iters = w->desc.residual_tol > 0.0f ? w->h_cnt->sweeps_run : iters;
         if (ok >= todo) break;
         // step `ok` of this batch failed and every later one skipped: repair and replay it eagerly
         w->cur = (ok & 1u) ? 1 - cur0 : cur0;

@@ -94,7 +94,8 @@ typedef struct axm_world_desc {
     uint32_t max_pairs;
     uint32_t max_manifolds;
     uint32_t auto_grow;    /* 1: grow pair/manifold buffers on overflow instead of failing */
-    float residual_tol;    /* > 0: stop sweeping once the residual <= tol (checked every residual_every sweeps) */
+    float residual_tol;    /* > 0: stop sweeping once the residual <= tol (checked every residual_every sweeps,
+                              inside the sweep launch; stats.iterations reports the sweeps run) */
     uint32_t residual_every;
     uint32_t max_iterations; /* cap when residual_tol > 0 */
     int32_t device;        /* HIP device ordinal */
@@ -178,7 +179,8 @@ axm_status axm_world_body_count(axm_world w, uint32_t* n);
  * broadphase -> narrowphase -> solver (iters sweeps) -> integration. */
 axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags, axm_step_stats* stats);
 /* `steps` steps; with graph = 1 the step is captured once into a hipGraph and
- * replayed (requires fixed capacities and residual_tol == 0). */
+ * replayed (not with record_residual; residual-stopped steps replay too: the
+ * sweep launch checks the residual itself). */
 axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last);
 axm_status axm_world_sync(axm_world w);
 /* Enables / disables per-launch event timing of the sweep kernel; `out`

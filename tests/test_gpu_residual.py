@@ -101,3 +101,32 @@ def test_tolerance_stop_matches_fixed_sweeps(axm, orc, gpu, every):
     for s in got:
         for x, y in zip(s, ref):
             assert np.array_equal(x, y)
+
+
+def test_tolerance_steps_replay_from_graphs(axm, orc, gpu):
+    """Residual-stopped steps need no host round trip, so they replay from captured graphs: the
+    replayed steps (plain and profiled) end bit for bit where eagerly launched ones end, and the
+    sweeps they ran add up to the eager steps' counts."""
+    nb, d = _settled(axm, 3000, 60)
+    d.residual_tol = 2e-3
+    d.residual_every = 2
+    d.max_iterations = 80
+    w = axm.World(d)
+    w.set_bodies(nb)
+    counts = [w.step(DT, 1).iterations for _ in range(8)]
+    ref = _state(w)
+    w.close()
+    assert 0 < min(counts) and max(counts) <= 80 and all(c % 2 == 0 for c in counts)
+    for prof in (False, True):
+        w = axm.World(d)
+        w.set_bodies(nb)
+        if prof:
+            w.profile(True)
+        st = w.run(DT, 1, 8, graph=True)
+        assert st.iterations == counts[-1]
+        if prof:
+            p = w.profile(False)
+            assert p.sweeps == sum(counts), (p.sweeps, counts)
+        for x, y in zip(_state(w), ref):
+            assert np.array_equal(x, y)
+        w.close()
