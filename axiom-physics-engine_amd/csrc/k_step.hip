@@ -160,15 +160,17 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.vel[i].w > 0.0f ? 1.0f : 0.0f);
 }
 
+// The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
+// in canonical order; the planes come with slice -1.
 template <class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, bool sparse,
-                                            uint32_t t, F&& f) {
+                                            uint32_t t, int dz, F&& f) {
     const uint64_t key = skey[t];
     const float4 mn = B.smin[t], mx = B.smax[t];
     const uint32_t i = __float_as_uint(mn.w);
     const bool dyn_i = mx.w > 0.0f;
-    if (dyn_i) {
+    if (dyn_i && dz < 0) {
         const f3 c = xyz(B.x[i]);
         const f3 e = xyz(mx) - c;
         for (uint32_t k = 0; k < sp.num_planes; ++k) {
@@ -181,9 +183,9 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     const uint32_t mort = (uint32_t)(key & 0x3FFFFFFFull);
     const uint64_t wbits = key & ~0x3FFFFFFFull;
     const int cx = (int)compact1by2(mort), cy = (int)compact1by2(mort >> 1), cz = (int)compact1by2(mort >> 2);
-    for (int dz = -1; dz <= 1; ++dz) {
+    {
         const int Z = cz + dz;
-        if (Z < 0 || Z > 1023) continue;
+        if (Z < 0 || Z > 1023) return;
         // the nine cells of this z slice: first probes of all nine in flight together; in a
         // sparse scene (few bodies per occupied cell) the occupancy bits, which stay in
         // the cache, are read first and only occupied cells are probed
@@ -225,22 +227,26 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     }
 }
 
-// Counts each body's pairs and keeps the first PAIR_STASH partners, so that the
-// emit pass only re-visits the grid for the few bodies with more.
+// Counts the pairs of each (body, z slice) part u = 3 t + dz + 1 and keeps the
+// first PAIR_STASH partners, so that the emit pass only re-visits the grid for the
+// few parts with more.  Three threads per body: the probes of a body's slices are
+// in flight together.
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
                                                     const Counters* __restrict__ ctr) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= BP_PARTS * n) return;
+    const uint32_t t = u / BP_PARTS;
+    const int dz = (int)(u - BP_PARTS * t) - 1;
     const bool sparse = (uint64_t)ctr->cells * 4u > n;  // fewer than four bodies per occupied cell
     uint32_t c = 0;
-    uint32_t* st = stash + (size_t)t * PAIR_STASH;
-    visit_pairs(B, sp, skey, sval, ct, sparse, t, [&](uint32_t j) {
+    uint32_t* st = stash + (size_t)u * PAIR_STASH;
+    visit_pairs(B, sp, skey, sval, ct, sparse, t, dz, [&](uint32_t j) {
         if (c < PAIR_STASH) st[c] = j;
         ++c;
     });
-    cnt[t] = c;
+    cnt[u] = c;
 }
 
 __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
@@ -248,26 +254,28 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
                                                    const uint32_t* __restrict__ off, uint32_t cap,
                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                    Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
-        const uint32_t P = off[n];
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0) {
+        const uint32_t P = off[BP_PARTS * n];
         cnt->p_eff = P < cap ? P : cap;
         if (P > cap) atomicOr(&cnt->sticky, STICKY_PAIRS);
     }
-    if (t >= n) return;
+    if (u >= BP_PARTS * n) return;
+    const uint32_t t = u / BP_PARTS;
+    const int dz = (int)(u - BP_PARTS * t) - 1;
     const bool sparse = (uint64_t)cnt->cells * 4u > n;
-    uint32_t o = off[t];
-    const uint32_t c = off[t + 1] - o;
+    uint32_t o = off[u];
+    const uint32_t c = off[u + 1] - o;
     const uint32_t i = sval[t];
     if (c <= PAIR_STASH) {
-        const uint32_t* st = stash + (size_t)t * PAIR_STASH;
+        const uint32_t* st = stash + (size_t)u * PAIR_STASH;
         for (uint32_t k = 0; k < c && o + k < cap; ++k) {
             pa[o + k] = i;
             pb[o + k] = st[k];
         }
         return;
     }
-    visit_pairs(B, sp, skey, sval, ct, sparse, t, [&](uint32_t j) {
+    visit_pairs(B, sp, skey, sval, ct, sparse, t, dz, [&](uint32_t j) {
         if (o < cap) {
             pa[o] = i;
             pb[o] = j;
@@ -1763,14 +1771,15 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
                        hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
-    hipLaunchKernelGGL(k_pair_count, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash, ctr);
+    hipLaunchKernelGGL(k_pair_count, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash,
+                       ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
                       Counters* cnt, const uint32_t* stash, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_pair_emit, grid_for(n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap, pa, pb, cnt,
-                           stash);
+        hipLaunchKernelGGL(k_pair_emit, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
+                           pa, pb, cnt, stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,

@@ -53,8 +53,9 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // Empties the cell table and its occupancy bits.
 void launch_cell_clear(const CellTable& t, hipStream_t s);
 void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s);
-// Pairs per body (stashing the first PAIR_STASH partners), then the emit pass in
-// the canonical order into pa / pb at the scanned offsets.
+// Pairs per (body, z slice) part, BP_PARTS * n counts in canonical order (stashing
+// the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
+// the scanned offsets.
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, const Counters* ctr,
                        hipStream_t s);

@@ -54,7 +54,10 @@ struct Bodies {
     float4* smax;  // w = 1 for a dynamic body
     uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)
 };
-constexpr uint32_t PAIR_STASH = 24;  // partners kept per body by the counting pass
+// The broadphase counts and emits each body's pairs in three parts, one per z
+// slice of its 27 neighbour cells (the planes go with the first), one thread each.
+constexpr uint32_t BP_PARTS = 3;
+constexpr uint32_t PAIR_STASH = 16;  // partners kept per part by the counting pass
 
 // Cell hash table (linear probing): entry = (key lo, key hi, first sorted index,
 // count), one 16-byte access per probe; plus an occupancy bitmap (one bit per

@@ -227,7 +227,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
-    AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * PAIR_STASH));
+    AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * BP_PARTS * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
@@ -244,10 +244,10 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     const uint32_t TB = std::min<uint32_t>(T, 1u << 24) * 8u;  // occupancy bits: 8 per table entry, at most 128 Mi
     AXM_TRY(w->mem.get(w->ct.bits, TB / 32u));
     w->ct.bmask = TB - 1;
-    AXM_TRY(w->mem.get(w->pair_cnt, cap));
-    AXM_TRY(w->mem.get(w->pair_off, cap + 1));
-    AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(cap)));
-    if (hipMemset(w->scan_tmp, 0, 4 * scan_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    AXM_TRY(w->mem.get(w->pair_cnt, (size_t)BP_PARTS * cap));
+    AXM_TRY(w->mem.get(w->pair_off, (size_t)BP_PARTS * cap + 1));
+    AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(BP_PARTS * cap)));
+    if (hipMemset(w->scan_tmp, 0, 4 * scan_tmp_words(BP_PARTS * cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     AXM_TRY(w->mem.get(w->bmax0, cap));
     AXM_TRY(w->mem.get(w->bmax1, cap));
     AXM_TRY(w->mem.get(w->bmask, cap));
@@ -513,7 +513,7 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     launch_cell_clear(w->ct, s);
     launch_cell_table(skey, n, w->ct, w->cnt, s);
     launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, w->cnt, s);
-    scan_exclusive(w->pair_cnt, w->pair_off, n, w->scan_tmp, s, &w->cnt->pairs);
+    scan_exclusive(w->pair_cnt, w->pair_off, BP_PARTS * n, w->scan_tmp, s, &w->cnt->pairs);
     launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
                      s);
 }
