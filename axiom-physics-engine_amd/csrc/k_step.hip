@@ -1159,12 +1159,15 @@ __device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t 
 }
 // The two hint words of slot s (versions of sides a and b) in one coherent 16-byte
 // access; a torn read only delays or repeats the full load, which checks every word.
-__device__ __forceinline__ uint2 vin_hint(const uint64_t* inbox, uint32_t s) {
+// Words k and k + 1 of slot s's inbox in one coherent 16-byte access, as (value,
+// version) dword pairs; every word is checked against its own version, so a load
+// that sees one word before the other only delays the visit.  Inboxes within 2 GiB.
+__device__ __forceinline__ uint4 vin_quad(const uint64_t* inbox, uint32_t s, uint32_t k) {
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint64_t*>(inbox), 0, 0x7FFFFFFF, 0x00020000);
     // aux: sc0 | sc1, the cache policy of a system-coherent load (bypasses the non-coherent caches)
-    const auto r = __builtin_amdgcn_raw_buffer_load_b128(rs, s * (IB_WORDS * 8u) + IB_HINT * 8u, 0, 1 | 16);
-    return make_uint2(r[1], r[3]);
+    const auto r = __builtin_amdgcn_raw_buffer_load_b128(rs, s * (IB_WORDS * 8u) + k * 8u, 0, 1 | 16);
+    return make_uint4(r[0], r[1], r[2], r[3]);
 }
 // The visit after the one of colour c of a body with colour mask mk: the next
 // colour up, or the body's first colour (next sweep).
@@ -1297,12 +1300,13 @@ __device__ __forceinline__ Item item_get(const float4* rec) {
 constexpr uint32_t OWN_SLOTS = 16;  // slots per lane with a staged record
 
 // LDS of a wave: records of its first min(owned chunks, OWN_SLOTS) slots per lane
-// (record j of a lane at lds[(j * 4 + r) * 64 + lane]), then the staged points:
-// point q of a lane keeps row r at lds[P0 + (r * ppl + q) * 64 + lane] (rows
-// 0..JROWS-1 the Jacobian rows of R.J[(k * JROWS + r) * cap + s], row JROWS its
-// impulses).  Slots and points beyond what fits are read from the row arrays.
-// With stop.bits, the sweeps run in chunks of stop.every with a residual check
-// after each chunk (every wave of the grid meets at one grid barrier per check).
+// (as many as fit; record j of a lane at lds[(j * 4 + r) * 64 + lane]), then the
+// staged points of the slots chosen for staging, packed in slot order: point q of a
+// lane keeps row r at lds[P0 + (r * ppl + q) * 64 + lane] (rows 0..JROWS-1 the
+// Jacobian rows of R.J[(k * JROWS + r) * cap + s], row JROWS its impulses).  Slots
+// and points not staged are read from the row arrays.  With stop.bits, the sweeps
+// run in chunks of stop.every with a residual check after each chunk (every wave of
+// the grid meets at one arrival count per check).
 __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, SolverRows R, Counters* cnt,
                                                       uint32_t cap, uint32_t sweep0, uint32_t nsweeps,
                                                       uint32_t lds_f4, uint32_t spin_limit, uint32_t tune,
@@ -1322,7 +1326,8 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     const uint32_t own = g < nchunks ? (nchunks - 1 - g) / G + 1 : 0u;
     // owned slots of this lane: j < nv (only the last owned chunk can be partial)
     const uint32_t nv = own == 0u ? 0u : own - (((g + (own - 1) * G) * OWN_WAVE + lane) >= M ? 1u : 0u);
-    const uint32_t nrec = min(own, OWN_SLOTS);
+    // slot records that fit the LDS (the rest are read from the row arrays)
+    const uint32_t nrec = min(min(own, OWN_SLOTS), lds_f4 / (4u * OWN_WAVE));
     const uint32_t p0 = nrec * 4u * OWN_WAVE;
     const uint32_t ppl = lds_f4 > p0 ? (lds_f4 - p0) / ((JROWS + 1u) * OWN_WAVE) : 0u;
     const size_t rs = (size_t)ppl * OWN_WAVE;  // LDS row stride of the points (float4)
@@ -1330,25 +1335,67 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     float4* mine = own_lds + p0 + lane;
     auto slot_of = [&](uint32_t jj) { return (g + jj * G) * OWN_WAVE + lane; };
 
-    // stage slot records, rows and impulses
-    bool all_staged = own <= OWN_SLOTS;
+    // stage slot records (only owned slots below M exist: j < nv)
+    const uint32_t nr = min(nrec, nv);
+    for (uint32_t j = 0; j < nr; ++j) item_put(recs + (size_t)j * 4u * OWN_WAVE, item_global(B, R, slot_of(j)));
+    // Which owned slots get their rows and impulses staged (bit j of smask): as many
+    // as fit, among the slots with a staged record those on the longest dependency
+    // chains first (the larger visit count per sweep of their two bodies), then the
+    // other slots in order.  Only a lane's first 64 slots can be staged.
+    uint64_t smask = 0;
+    {
+        auto rec_np_crit = [&](uint32_t j, uint32_t& np, uint32_t& crit) {
+            const float4 r0 = recs[(size_t)j * 4u * OWN_WAVE], r1 = recs[(size_t)(j * 4u + 1u) * OWN_WAVE];
+            const uint32_t dx = __float_as_uint(r1.x), dy = __float_as_uint(r1.y);
+            np = min(__float_as_uint(r0.z), 4u);
+            crit = max(dx == UNSET ? 0u : dx >> 8, dy == UNSET ? 0u : dy >> 8);
+        };
+        uint32_t used = 0, level = 0xFFFFFFFFu;
+        for (;;) {  // criticality levels, descending
+            uint32_t next = 0;
+            bool any = false;
+            for (uint32_t j = 0; j < nr; ++j) {
+                uint32_t np, c;
+                rec_np_crit(j, np, c);
+                if (c < level && (!any || c > next)) {
+                    next = c;
+                    any = true;
+                }
+            }
+            if (!any) break;
+            for (uint32_t j = 0; j < nr; ++j) {
+                uint32_t np, c;
+                rec_np_crit(j, np, c);
+                if (c == next && used + np <= ppl) {
+                    smask |= 1ull << j;
+                    used += np;
+                }
+            }
+            level = next;
+        }
+        for (uint32_t j = nr; j < min(nv, 64u); ++j) {
+            const uint32_t np = min(R.hdr[slot_of(j)].z, 4u);
+            if (used + np <= ppl) {
+                smask |= 1ull << j;
+                used += np;
+            }
+        }
+    }
+    auto staged_slot = [&](uint32_t j) { return j < 64u && ((smask >> j) & 1ull) != 0ull; };
+    const bool all_staged = nv <= nrec && smask == (1ull << nv) - 1ull;  // nrec <= 16
+    // stage the selected slots' rows and impulses, packed in slot order
     {
         uint32_t off = 0;
         for (uint32_t j = 0; j < nv; ++j) {
+            if (!staged_slot(j)) continue;
             const uint32_t s = slot_of(j);
-            const Item it = item_global(B, R, s);
-            if (j < nrec) item_put(recs + (size_t)j * 4u * OWN_WAVE, it);
-            const uint32_t np = min(it.h.z, 4u);
-            if (off + np <= ppl) {
-                for (uint32_t k = 0; k < np; ++k) {
-                    const float4* src = R.J + (size_t)k * JROWS * cap + s;
-                    float4* dst = mine + (size_t)(off + k) * OWN_WAVE;
+            const uint32_t np = min(j < nrec ? __float_as_uint(recs[(size_t)j * 4u * OWN_WAVE].z) : R.hdr[s].z, 4u);
+            for (uint32_t k = 0; k < np; ++k) {
+                const float4* src = R.J + (size_t)k * JROWS * cap + s;
+                float4* dst = mine + (size_t)(off + k) * OWN_WAVE;
 #pragma unroll
-                    for (uint32_t r = 0; r < JROWS; ++r) dst[r * rs] = src[(size_t)r * cap];
-                    dst[JROWS * rs] = R.lam[(size_t)k * cap + s];
-                }
-            } else {
-                all_staged = false;
+                for (uint32_t r = 0; r < JROWS; ++r) dst[r * rs] = src[(size_t)r * cap];
+                dst[JROWS * rs] = R.lam[(size_t)k * cap + s];
             }
             off += np;
         }
@@ -1361,8 +1408,8 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // tune bits 4-6: poll back-off level; bit 1: poll the hint pair before the full load
     // (inboxes within 2 GiB); bit 2: adaptive back-off
     const uint32_t nap = (tune >> 4) & 7u;
-    const bool hint = (tune & 2u) != 0u && (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull,
-               adaptive = (tune & 4u) != 0u;
+    const bool wide = (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull;  // 16-byte inbox loads
+    const bool hint = (tune & 2u) != 0u && wide, adaptive = (tune & 4u) != 0u;
     const uint64_t t_start = wall_clock64();
     const uint64_t t_limit = (uint64_t)spin_limit * 100ull;  // wall clock: 100 MHz
 
@@ -1418,7 +1465,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             if (it.dp.x != UNSET) got &= final_vel(it.h.x, it.dp.x, sweeps, A);
             if (it.dp.y != UNSET) got &= final_vel(it.h.y, it.dp.y, sweeps, Bd);
             if (!got) break;
-            const bool staged = off + np <= ppl;
+            const bool staged = staged_slot(j);
             for (uint32_t k = 0; k < np; ++k) {
                 const float4* J = staged ? mine + (size_t)(off + k) * OWN_WAVE : R.J + (size_t)k * JROWS * cap + it.s;
                 const size_t jr = staged ? rs : (size_t)cap;
@@ -1426,7 +1473,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 const float e = point_residual(J, jr, plane, mu, A, Bd, L);
                 if (e > res) res = e;
             }
-            off += np;
+            off += staged ? np : 0u;
         }
         uint32_t bits = __float_as_uint(res);
         for (int o = 32; o >= 1; o >>= 1) {
@@ -1498,38 +1545,55 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             // poll = load: all words of both bodies in this slot's inbox (one line), issued together
             bool ok = false;
             bool ready = pending;
+            uint4 h67 = make_uint4(0u, 0u, 0u, 0u);
             if (pending && hint) {
                 // the hint pair first (one access); the full load below runs only for lanes
                 // whose hint words are in
-                const uint2 hv = vin_hint(R.inbox, s);
-                ready = ((hv.x == tA) | !A.dyn) & ((hv.y == tB) | !Bd.dyn);
+                h67 = vin_quad(R.inbox, s, IB_HINT);
+                ready = ((h67.y == tA) | !A.dyn) & ((h67.w == tB) | !Bd.dyn);
             }
             if (ready) {
-                const uint64_t* ib = R.inbox;
-                const uint64_t a0 = vload(ib + vin_word(s, 0, 0)), a1 = vload(ib + vin_word(s, 0, 1)),
-                               a2 = vload(ib + vin_word(s, 0, 2)), a3 = vload(ib + vin_word(s, 0, 3)),
-                               a4 = vload(ib + vin_word(s, 0, 4)), a5 = vload(ib + vin_word(s, 0, 5));
-                const uint64_t b0 = vload(ib + vin_word(s, 1, 0)), b1 = vload(ib + vin_word(s, 1, 1)),
-                               b2 = vload(ib + vin_word(s, 1, 2)), b3 = vload(ib + vin_word(s, 1, 3)),
-                               b4 = vload(ib + vin_word(s, 1, 4)), b5 = vload(ib + vin_word(s, 1, 5));
+                // (value, version) of the six words of each side
+                uint2 a[6], b[6];
+                if (wide) {
+                    // 16-byte loads; the hint pair, when polled, holds the sixth word of each side
+                    if (!hint) h67 = vin_quad(R.inbox, s, IB_HINT);
+                    const uint4 q0 = vin_quad(R.inbox, s, 0), q1 = vin_quad(R.inbox, s, 2),
+                                q2 = vin_quad(R.inbox, s, 4), q4 = vin_quad(R.inbox, s, 8),
+                                q5 = vin_quad(R.inbox, s, 10), q6 = vin_quad(R.inbox, s, 12);
+                    a[0] = make_uint2(q0.x, q0.y), a[1] = make_uint2(q0.z, q0.w), a[2] = make_uint2(q1.x, q1.y);
+                    a[3] = make_uint2(q1.z, q1.w), a[4] = make_uint2(q2.x, q2.y), a[5] = make_uint2(h67.x, h67.y);
+                    b[0] = make_uint2(q4.x, q4.y), b[1] = make_uint2(q4.z, q4.w), b[2] = make_uint2(q5.x, q5.y);
+                    b[3] = make_uint2(q5.z, q5.w), b[4] = make_uint2(q6.x, q6.y), b[5] = make_uint2(h67.z, h67.w);
+                } else {
+                    const uint64_t* ib = R.inbox;
+#pragma unroll
+                    for (uint32_t k = 0; k < 6; ++k) {
+                        const uint64_t wa = vload(ib + vin_word(s, 0, k)), wb = vload(ib + vin_word(s, 1, k));
+                        a[k] = make_uint2((uint32_t)wa, vver(wa));
+                        b[k] = make_uint2((uint32_t)wb, vver(wb));
+                    }
+                }
                 // branch-free: every loaded word is consumed here, so no later instruction
                 // waits on the memory counter (which also counts the stores before it)
-                const uint32_t da = (vver(a0) ^ tA) | (vver(a1) ^ tA) | (vver(a2) ^ tA) | (vver(a3) ^ tA) |
-                                    (vver(a4) ^ tA) | (vver(a5) ^ tA);
-                const uint32_t db = (vver(b0) ^ tB) | (vver(b1) ^ tB) | (vver(b2) ^ tB) | (vver(b3) ^ tB) |
-                                    (vver(b4) ^ tB) | (vver(b5) ^ tB);
+                const uint32_t da = (a[0].y ^ tA) | (a[1].y ^ tA) | (a[2].y ^ tA) | (a[3].y ^ tA) |
+                                    (a[4].y ^ tA) | (a[5].y ^ tA);
+                const uint32_t db = (b[0].y ^ tB) | (b[1].y ^ tB) | (b[2].y ^ tB) | (b[3].y ^ tB) |
+                                    (b[4].y ^ tB) | (b[5].y ^ tB);
                 ok = ((da == 0u) | !A.dyn) & ((db == 0u) | !Bd.dyn);
                 const bool ta = ok & A.dyn, tb = ok & Bd.dyn;
-                A.v = f3{ta ? vval(a0) : A.v.x, ta ? vval(a1) : A.v.y, ta ? vval(a2) : A.v.z};
-                A.w = f3{ta ? vval(a3) : A.w.x, ta ? vval(a4) : A.w.y, ta ? vval(a5) : A.w.z};
-                Bd.v = f3{tb ? vval(b0) : Bd.v.x, tb ? vval(b1) : Bd.v.y, tb ? vval(b2) : Bd.v.z};
-                Bd.w = f3{tb ? vval(b3) : Bd.w.x, tb ? vval(b4) : Bd.w.y, tb ? vval(b5) : Bd.w.z};
+                auto val = [](uint2 x) { return __uint_as_float(x.x); };
+                A.v = f3{ta ? val(a[0]) : A.v.x, ta ? val(a[1]) : A.v.y, ta ? val(a[2]) : A.v.z};
+                A.w = f3{ta ? val(a[3]) : A.w.x, ta ? val(a[4]) : A.w.y, ta ? val(a[5]) : A.w.z};
+                Bd.v = f3{tb ? val(b[0]) : Bd.v.x, tb ? val(b[1]) : Bd.v.y, tb ? val(b[2]) : Bd.v.z};
+                Bd.w = f3{tb ? val(b[3]) : Bd.w.x, tb ? val(b[4]) : Bd.w.y, tb ? val(b[5]) : Bd.w.z};
             }
             if (ok) {
                 const bool first = sweep0 + sw == 0u;
                 float4 L[4];
                 // the point loops run over a fixed 4 so that L stays in registers
-                if (STAGED || off + np <= ppl) {
+                const bool stg = STAGED || staged_slot(j);
+                if (stg) {
                     float4* P = mine + (size_t)off * OWN_WAVE;
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
@@ -1552,7 +1616,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 if (Bd.dyn) vin_store(R.inbox, it.nx.y >> 1, it.nx.y & 1u, tB + 1u, Bd.v, Bd.w);
                 // next item of this lane: the next owned slot, or the first one of the next sweep
                 ++done;
-                off += np;
+                off += stg ? np : 0u;
                 if (++j == nv) {
                     j = 0;
                     ++sw;
@@ -1615,12 +1679,12 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     uint32_t woff = 0;
     for (uint32_t jj = 0; jj < nv; ++jj) {
         const uint32_t ss = slot_of(jj);
+        if (!staged_slot(jj)) continue;
         const uint32_t n = min(jj < nrec ? __float_as_uint(recs[(size_t)jj * 4u * OWN_WAVE].z) : R.hdr[ss].z, 4u);
-        if (woff + n <= ppl)
-            for (uint32_t k = 0; k < n; ++k) {
-                const float4 L = mine[(JROWS * rs) + (size_t)(woff + k) * OWN_WAVE];
-                R.lam[(size_t)k * cap + ss] = make_float4(L.x, L.y, L.z, 0.0f);
-            }
+        for (uint32_t k = 0; k < n; ++k) {
+            const float4 L = mine[(JROWS * rs) + (size_t)(woff + k) * OWN_WAVE];
+            R.lam[(size_t)k * cap + ss] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
         woff += n;
     }
 }
