@@ -15,7 +15,7 @@
 namespace axm {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 4;  // keys per thread: small tiles, so that a 100k-key sort fills the GPU
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
@@ -55,10 +55,11 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 
 // Single-pass exclusive scan with decoupled look-back.  A workgroup takes the next
 // tile from a counter (so every lower tile is already held by a running
-// workgroup), publishes its tile total, then walks back over the published
-// status words of lower tiles until it meets an inclusive prefix, and publishes
-// its own.  Status word: bits 62-63 state (0 none, 1 tile total, 2 inclusive
-// prefix), bits 0-31 the value.  The last workgroup to finish clears the status
+// workgroup), publishes its tile total, then reads the status words of the 64
+// tiles below it at once (one per lane of its first wave), moving down 64 at a
+// time until it meets an inclusive prefix, and publishes its own.  Status word:
+// bits 62-63 state (0 none, 1 tile total, 2 inclusive prefix), bits 0-31 the
+// value.  The last workgroup to finish clears the status
 // words and counters, so every call starts from zeros (the scratch is zeroed once
 // when allocated).  Scratch layout (u32 words): [0] tile counter, [1] finished
 // counter, then nb 64-bit status words.
@@ -84,31 +85,44 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restri
     }
     uint32_t tot;
     const uint32_t ex = block_excl_scan(sum, tot);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64u) {
+        // the first wave publishes the tile and looks back over 64 lower tiles at a time
+        const uint32_t lane = threadIdx.x;
         uint32_t prefix = 0;
         if (tile == 0) {
-            __hip_atomic_store(&st[0], SC_PRE | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0u) __hip_atomic_store(&st[0], SC_PRE | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&st[tile], SC_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0u) __hip_atomic_store(&st[tile], SC_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // lower tiles are held by running workgroups, so each publishes; the wall-clock
             // bound only keeps a broken invariant from hanging the device
             const uint64_t t0 = wall_clock64();
-            uint32_t j = tile - 1;
+            int j0 = (int)tile - 1;  // window: tiles j0 - lane
             for (;;) {
-                const uint64_t w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((w >> 62) == 0u) {
+                const int j = j0 - (int)lane;
+                // below tile 0 reads as an inclusive prefix of 0 (tile 0 always publishes one first)
+                const uint64_t w =
+                    j >= 0 ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SC_PRE;
+                const uint32_t state = (uint32_t)(w >> 62);
+                const uint64_t pre = __ballot(state == 2u), none = __ballot(state == 0u);
+                // the nearest inclusive prefix, and every tile between it and this one published
+                const uint32_t first = pre ? (uint32_t)__ffsll((unsigned long long)pre) - 1u : 64u;
+                const uint64_t upto = first >= 63u ? ~0ull : (2ull << first) - 1ull;
+                if (none & upto) {
                     if (wall_clock64() - t0 > 200000000ull) break;  // 2 s
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += (uint32_t)w;
-                if ((w >> 62) == 2u || j == 0u) break;
-                --j;
+                uint32_t val = lane <= first ? (uint32_t)w : 0u;
+                val = __shfl(wave_incl_scan(val), 63, 64);
+                prefix += val;
+                if (first < 64u) break;
+                j0 -= 64;
             }
-            __hip_atomic_store(&st[tile], SC_PRE | (uint32_t)(prefix + tot), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0u)
+                __hip_atomic_store(&st[tile], SC_PRE | (uint32_t)(prefix + tot), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = prefix;
+        if (lane == 0u) s_prefix = prefix;
     }
     __syncthreads();
     uint32_t run = s_prefix + ex;
