@@ -27,6 +27,17 @@ namespace axm {
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+// Adds the sum of v over the wave to *dst with one atomic (every lane of the wave
+// must call it: the reduction reads all 64 lanes).
+__device__ __forceinline__ void wave_add(uint32_t* dst, uint32_t v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0u && v != 0u) atomicAdd(dst, v);
+}
+__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0u && v != 0ull) atomicAdd(dst, v);
+}
+
 // ---------------------------------------------------------------- k_begin
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
                                                uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
@@ -134,15 +145,16 @@ __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
 __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint64_t k = skey[s];
-    if (s > 0 && skey[s - 1] == k) return;
+    // the first body of each cell inserts it; one count per wave
+    const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
+    const bool head = s < n && (s == 0 || skey[s - 1] != k);
+    wave_add(&cnt->cells, head ? 1u : 0u);
+    if (!head) return;
     uint32_t c = 1;
     while (s + c < n && skey[s + c] == k) ++c;
     if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
     const uint32_t bit = cell_bit(mix64(k), t.bmask);
     atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
-    atomicAdd(&cnt->cells, 1u);
 }
 
 // ---------------------------------------------------------------- pairs
@@ -396,7 +408,9 @@ __global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, cons
                                                    uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
                                                    float4* __restrict__ p_nf, Counters* __restrict__ cnt) {
     const uint32_t nq = min(cnt->epa_count, cap);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    // pairs spread one per wave first (queue entry q goes to wave q mod waves): each
+    // pair's expansion runs its own iteration count without waiting on other lanes
+    for (uint32_t q = blockIdx.x + threadIdx.x * gridDim.x; q < nq; q += gridDim.x * blockDim.x) {
         const uint32_t p = epa_list[q];
         const uint32_t a = pa[p], b = pb[p];
         Shp A = load_shape(B, a), Bs = load_shape(B, b);
@@ -430,11 +444,10 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa
         cnt->m_eff = M < cap ? M : cap;
         if (M > cap) atomicOr(&cnt->sticky, STICKY_MANIFOLDS);
     }
-    if (p >= cnt->p_eff) return;
-    const uint32_t np = p_np[p];
+    const uint32_t np = p < cnt->p_eff ? p_np[p] : 0u;
+    wave_add(&cnt->contacts, np);
     if (np == 0) return;
     const uint32_t m = moff[p];
-    atomicAdd(&cnt->contacts, np);
     if (m >= cap) return;
     m_ab[m] = make_uint2(pa[p], pb[p]);
     m_np[m] = np;
@@ -767,16 +780,16 @@ __device__ __forceinline__ f3 vrel(const SB& A, const SB& Bd, f3 rA, f3 rB) {
     return (Bd.v + cross(Bd.w, rB)) - (A.v + cross(A.w, rA));
 }
 
-__global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
-                                              uint32_t cap, const uint2* __restrict__ m_ab,
-                                              const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
-                                              const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
-                                              const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
-                                              const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
-                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t M = cnt->m_eff;
-    if (s >= M || cnt->sticky != 0u) return;  // a failed step must not touch the impulse buffers
+// Prepares solve slot s; returns the warm-start hits and the algorithmic bytes of one
+// sweep visit of the slot (the kernel sums them per wave).
+__device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp, const uint32_t* __restrict__ order,
+                                          uint32_t cap, const uint2* __restrict__ m_ab,
+                                          const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
+                                          const float4* __restrict__ c_nf, const SolverRows& R, const PrevCache& pc,
+                                          const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
+                                          const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
+                                          Counters* __restrict__ cnt, float* __restrict__ feat, uint32_t s,
+                                          uint32_t& hits_out, unsigned long long& bytes_out) {
     const uint32_t m = order[s];
     const uint2 ab = m_ab[m];
     const uint32_t np = m_np[m];
@@ -902,7 +915,7 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
             fr[5] = make_float4(Lp.z * inv_u, hit ? 1.0f : 0.0f, (float)np * 0.25f, plane ? 1.0f : 0.0f);
         }
     }
-    if (hits) atomicAdd(&cnt->warm_hits, hits);
+    hits_out = hits;
     // algorithmic bytes one sweep visit of this manifold moves: header 16; per
     // point 15 Jacobian rows (9 against a plane, whose b / ib rows are not read)
     // and lambda read + write; each body's solver velocities read (32) and, when
@@ -910,7 +923,24 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
     unsigned long long bytes = 16ull + (plane ? 9ull * 16ull + 32ull : 15ull * 16ull + 32ull) * np + 32ull +
                                (A.dyn ? 32ull : 0ull);
     if (!plane) bytes += 32ull + (Bd.dyn ? 32ull : 0ull);
-    atomicAdd(&cnt->pgs_bytes, bytes);
+    bytes_out = bytes;
+}
+__global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uint32_t* __restrict__ order,
+                                              uint32_t cap, const uint2* __restrict__ m_ab,
+                                              const uint32_t* __restrict__ m_np, const float4* __restrict__ c_pt,
+                                              const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
+                                              const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
+                                              const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
+                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t hits = 0;
+    unsigned long long bytes = 0;
+    // a failed step must not touch the impulse buffers
+    if (s < cnt->m_eff && cnt->sticky == 0u)
+        prep_slot(B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc, warm_in, col, bmask, bdeg, cnt, feat, s, hits,
+                  bytes);
+    wave_add(&cnt->warm_hits, hits);
+    wave_add(&cnt->pgs_bytes, bytes);
 }
 
 // Memory policy of the sweep: plain accesses when every colour is its own
