@@ -65,82 +65,96 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 // counter, then nb 64-bit status words.
 constexpr uint64_t SC_AGG = 1ull << 62, SC_PRE = 2ull << 62;
 
+// With n_dev the element count is min(n, *n_dev), read on the device: the grid is
+// sized for n and the tiles above the count only take part in the scratch reset.
+// With flags, the input is read as (in[i] != 0).
 __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                         uint32_t n, uint32_t* __restrict__ tmp, uint32_t nb,
-                                                         uint32_t* __restrict__ total_out) {
+                                                         uint32_t n_cap, uint32_t* __restrict__ tmp, uint32_t nb,
+                                                         uint32_t* __restrict__ total_out,
+                                                         const uint32_t* __restrict__ n_dev, int flags) {
     __shared__ uint32_t s_tile, s_prefix;
     uint64_t* st = reinterpret_cast<uint64_t*>(tmp + 2);
     if (threadIdx.x == 0) s_tile = atomicAdd(&tmp[0], 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
-    // each thread owns SC_ITEMS consecutive elements of the tile
-    const uint32_t base = tile * SC_TILE + threadIdx.x * SC_ITEMS;
-    uint32_t v[SC_ITEMS];
-    uint32_t sum = 0;
+    const uint32_t n = n_dev ? min(n_cap, *n_dev) : n_cap;
+    const uint32_t nt = n == 0u ? 1u : (n + SC_TILE - 1u) / SC_TILE;  // tiles with elements (tile 0 always)
+    if (tile < nt) {
+        // each thread owns SC_ITEMS consecutive elements of the tile
+        const uint32_t base = tile * SC_TILE + threadIdx.x * SC_ITEMS;
+        uint32_t v[SC_ITEMS];
+        uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < SC_ITEMS; ++k) {
-        const uint32_t i = base + k;
-        v[k] = i < n ? in[i] : 0u;
-        sum += v[k];
-    }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(sum, tot);
-    if (threadIdx.x < 64u) {
-        // the first wave publishes the tile and looks back over 64 lower tiles at a time
-        const uint32_t lane = threadIdx.x;
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            if (lane == 0u) __hip_atomic_store(&st[0], SC_PRE | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (lane == 0u) __hip_atomic_store(&st[tile], SC_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // lower tiles are held by running workgroups, so each publishes; the wall-clock
-            // bound only keeps a broken invariant from hanging the device
-            const uint64_t t0 = wall_clock64();
-            int j0 = (int)tile - 1;  // window: tiles j0 - lane
-            for (;;) {
-                const int j = j0 - (int)lane;
-                // below tile 0 reads as an inclusive prefix of 0 (tile 0 always publishes one first)
-                const uint64_t w =
-                    j >= 0 ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SC_PRE;
-                const uint32_t state = (uint32_t)(w >> 62);
-                const uint64_t pre = __ballot(state == 2u), none = __ballot(state == 0u);
-                // the nearest inclusive prefix, and every tile between it and this one published
-                const uint32_t first = pre ? (uint32_t)__ffsll((unsigned long long)pre) - 1u : 64u;
-                const uint64_t upto = first >= 63u ? ~0ull : (2ull << first) - 1ull;
-                if (none & upto) {
-                    if (wall_clock64() - t0 > 200000000ull) break;  // 2 s
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                uint32_t val = lane <= first ? (uint32_t)w : 0u;
-                val = __shfl(wave_incl_scan(val), 63, 64);
-                prefix += val;
-                if (first < 64u) break;
-                j0 -= 64;
-            }
-            if (lane == 0u)
-                __hip_atomic_store(&st[tile], SC_PRE | (uint32_t)(prefix + tot), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < SC_ITEMS; ++k) {
+            const uint32_t i = base + k;
+            v[k] = i < n ? (flags ? (in[i] != 0u ? 1u : 0u) : in[i]) : 0u;
+            sum += v[k];
         }
-        if (lane == 0u) s_prefix = prefix;
-    }
-    __syncthreads();
-    uint32_t run = s_prefix + ex;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(sum, tot);
+        if (threadIdx.x < 64u) {
+            // the first wave publishes the tile and looks back over 64 lower tiles at a time
+            const uint32_t lane = threadIdx.x;
+            uint32_t prefix = 0;
+            if (tile == 0) {
+                if (lane == 0u) __hip_atomic_store(&st[0], SC_PRE | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (lane == 0u) __hip_atomic_store(&st[tile], SC_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // lower tiles are held by running workgroups, so each publishes; the wall-clock
+                // bound only keeps a broken invariant from hanging the device
+                const uint64_t t0 = wall_clock64();
+                int j0 = (int)tile - 1;  // window: tiles j0 - lane
+                for (;;) {
+                    const int j = j0 - (int)lane;
+                    // below tile 0 reads as an inclusive prefix of 0 (tile 0 always publishes one first)
+                    const uint64_t w =
+                        j >= 0 ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SC_PRE;
+                    const uint32_t state = (uint32_t)(w >> 62);
+                    const uint64_t pre = __ballot(state == 2u), none = __ballot(state == 0u);
+                    // the nearest inclusive prefix, and every tile between it and this one published
+                    const uint32_t first = pre ? (uint32_t)__ffsll((unsigned long long)pre) - 1u : 64u;
+                    const uint64_t upto = first >= 63u ? ~0ull : (2ull << first) - 1ull;
+                    if (none & upto) {
+                        if (wall_clock64() - t0 > 200000000ull) break;  // 2 s
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    uint32_t val = lane <= first ? (uint32_t)w : 0u;
+                    val = __shfl(wave_incl_scan(val), 63, 64);
+                    prefix += val;
+                    if (first < 64u) break;
+                    j0 -= 64;
+                }
+                if (lane == 0u)
+                    __hip_atomic_store(&st[tile], SC_PRE | (uint32_t)(prefix + tot), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (lane == 0u) s_prefix = prefix;
+        }
+        __syncthreads();
+        uint32_t run = s_prefix + ex;
 #pragma unroll
-    for (int k = 0; k < SC_ITEMS; ++k) {
-        const uint32_t i = base + k;
-        if (i < n) out[i] = run;
-        run += v[k];
+        for (int k = 0; k < SC_ITEMS; ++k) {
+            const uint32_t i = base + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        if (tile == nt - 1 && threadIdx.x == SC_THREADS - 1) {
+            out[n] = s_prefix + tot;
+            if (total_out) *total_out = s_prefix + tot;
+        }
     }
-    if (tile == nb - 1 && threadIdx.x == SC_THREADS - 1) {
-        out[n] = s_prefix + tot;
-        if (total_out) *total_out = s_prefix + tot;
-    }
-    if (threadIdx.x == 0 && atomicAdd(&tmp[1], 1u) == nb - 1) {
+    __shared__ bool s_last;
+    if (threadIdx.x == 0) s_last = atomicAdd(&tmp[1], 1u) == nb - 1;
+    __syncthreads();
+    if (s_last) {
         // every workgroup has finished its look-back: reset the scratch for the next call
-        for (uint32_t k = 0; k < nb; ++k) __hip_atomic_store(&st[k], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tmp[0] = 0u;
-        tmp[1] = 0u;
+        for (uint32_t k = threadIdx.x; k < nb; k += SC_THREADS)
+            __hip_atomic_store(&st[k], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) {
+            tmp[0] = 0u;
+            tmp[1] = 0u;
+        }
     }
 }
 
@@ -151,13 +165,15 @@ __global__ void k_zero1(uint32_t* p, uint32_t* q) {
     if (q) *q = 0;
 }
 
-void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out) {
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out,
+                    const uint32_t* n_dev, bool flags) {
     if (n == 0) {
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, out, total_out);
         return;
     }
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
-    hipLaunchKernelGGL(k_scan_1p, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out);
+    hipLaunchKernelGGL(k_scan_1p, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out, n_dev,
+                       flags ? 1 : 0);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -169,11 +185,17 @@ __device__ __forceinline__ uint32_t eff_count(uint32_t cap, const uint32_t* n_de
     return v < cap ? v : cap;
 }
 
+// Per-tile digit counts, digit-major: hist[d * nt + tile] for the nt tiles that hold
+// keys (nt from the count read on the device; the grid is sized for the capacity).
+// The first workgroup writes 256 nt, the length of the scan that follows, to *len.
 template <class K>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ keys, uint32_t cap,
                                                         const uint32_t* __restrict__ n_dev, int shift,
-                                                        uint32_t* __restrict__ hist, uint32_t nb) {
+                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ len) {
     const uint32_t n = eff_count(cap, n_dev);
+    const uint32_t nt = (n + RS_TILE - 1) / RS_TILE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *len = 256u * nt;
+    if (blockIdx.x >= nt) return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -184,20 +206,21 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const K* __restrict__ ke
         if (i < n) atomicAdd(&h[digit_of(keys[i], shift)], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    hist[(size_t)threadIdx.x * nt + blockIdx.x] = h[threadIdx.x];
 }
 
 template <class K>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            K* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t cap,
                                                            const uint32_t* __restrict__ n_dev, int shift,
-                                                           const uint32_t* __restrict__ offs, uint32_t nb) {
+                                                           const uint32_t* __restrict__ offs) {
     const uint32_t n = eff_count(cap, n_dev);
     if (blockIdx.x * RS_TILE >= n) return;
+    const uint32_t nt = (n + RS_TILE - 1) / RS_TILE;
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_wc[RS_THREADS / 64][256];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-    s_off[tid] = offs[(size_t)tid * nb + blockIdx.x];
+    s_off[tid] = offs[(size_t)tid * nt + blockIdx.x];
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t base = blockIdx.x * RS_TILE;
     for (int it = 0; it < RS_ITEMS; ++it) {
@@ -252,7 +275,10 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     if (n == 0) return 0;
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* hist = tmp;
-    uint32_t* offs = tmp;  // scanned in place into the second half
+    uint32_t* offs = tmp;  // scanned in place
+    // the histogram length in use (256 x tiles with keys); the word before it takes the
+    // scan's total (out[len])
+    uint32_t* len = tmp + (size_t)256 * nb + 1;
     uint32_t* scan_tmp = tmp + (size_t)256 * nb + 2;  // 8-byte aligned for the scan's status words
     K* ki = keys;
     K* ko = keys_alt;
@@ -260,10 +286,9 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     uint32_t* vo = vals_alt;
     int passes = 0;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, n_dev, shift, hist, nb);
-        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s, nullptr);
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, n_dev, shift, offs,
-                           nb);
+        hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, n_dev, shift, hist, len);
+        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s, nullptr, len);
+        hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, n_dev, shift, offs);
         K* tk = ki; ki = ko; ko = tk;
         uint32_t* tv = vi; vi = vo; vo = tv;
         ++passes;

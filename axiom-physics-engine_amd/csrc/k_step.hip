@@ -333,10 +333,10 @@ __global__ void __launch_bounds__(256) k_pair_kind(Bodies B, const uint32_t* __r
                                                    const uint32_t* __restrict__ pb, uint32_t* __restrict__ p_kind,
                                                    uint32_t* __restrict__ kinds, const Counters* __restrict__ cnt) {
     __shared__ uint32_t h[PAIR_KINDS];
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (threadIdx.x < PAIR_KINDS) h[threadIdx.x] = 0u;
     __syncthreads();
-    if (p < cnt->p_eff) {
+    const uint32_t P = cnt->p_eff;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
         const uint32_t k = pair_kind(B, pa[p], pb[p]);
         p_kind[p] = k;
         atomicAdd(&h[k], 1u);
@@ -346,25 +346,30 @@ __global__ void __launch_bounds__(256) k_pair_kind(Bodies B, const uint32_t* __r
 }
 __global__ void __launch_bounds__(256) k_pair_order(const uint32_t* __restrict__ p_kind, uint32_t* __restrict__ kinds,
                                                     uint32_t* __restrict__ order, const Counters* __restrict__ cnt) {
-    __shared__ uint32_t off[PAIR_KINDS], h[PAIR_KINDS];
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t base[PAIR_KINDS], off[PAIR_KINDS], h[PAIR_KINDS];
     const uint32_t P = cnt->p_eff;
     if (threadIdx.x < PAIR_KINDS) {
         uint32_t o = 0;
         for (uint32_t k = 0; k < threadIdx.x; ++k) o += kinds[k];
-        off[threadIdx.x] = o;
-        h[threadIdx.x] = 0u;
+        base[threadIdx.x] = o;
     }
-    __syncthreads();
-    const uint32_t k = p < P ? p_kind[p] : 0u;
-    const uint32_t r = p < P ? atomicAdd(&h[k], 1u) : 0u;
-    __syncthreads();
-    if (threadIdx.x < PAIR_KINDS && h[threadIdx.x])
-        off[threadIdx.x] += atomicAdd(&kinds[PAIR_KINDS + threadIdx.x], h[threadIdx.x]);
-    __syncthreads();
-    if (p < P) {
-        const uint32_t q = off[k] + r;
-        if (q < P) order[q] = p;
+    // a bounded grid strides over the pairs (block-uniform trip count: the barriers below)
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < P; b0 += gridDim.x * blockDim.x) {
+        const uint32_t p = b0 + threadIdx.x;
+        if (threadIdx.x < PAIR_KINDS) h[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t k = p < P ? p_kind[p] : 0u;
+        const uint32_t r = p < P ? atomicAdd(&h[k], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x < PAIR_KINDS)
+            off[threadIdx.x] = base[threadIdx.x] +
+                               (h[threadIdx.x] ? atomicAdd(&kinds[PAIR_KINDS + threadIdx.x], h[threadIdx.x]) : 0u);
+        __syncthreads();
+        if (p < P) {
+            const uint32_t q = off[k] + r;
+            if (q < P) order[q] = p;
+        }
+        __syncthreads();
     }
 }
 
@@ -373,8 +378,8 @@ __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const u
                                                 float4* __restrict__ p_pt, float4* __restrict__ p_nf,
                                                 uint32_t* __restrict__ epa_list, Counters* __restrict__ cnt,
                                                 const uint32_t* __restrict__ order) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt->p_eff) return;
+    const uint32_t P = cnt->p_eff;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
     const uint32_t p = order[t];
     const uint32_t a = pa[p], b = pb[p];
     Man m;
@@ -398,6 +403,7 @@ __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const u
         if (st == GEOM_GJK_FAIL) atomicAdd(&cnt->gjk_fail, 1u);
     }
     store_pair(p, m, p_np, p_pt, p_nf);
+    }
 }
 
 // Deep box-capsule pairs (cores intersect): EPA, one thread per queued pair (a
@@ -425,12 +431,6 @@ __global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, cons
     }
 }
 
-__global__ void __launch_bounds__(256) k_flag_nonzero(const uint32_t* __restrict__ np, uint32_t cap_pairs,
-                                                      uint32_t* __restrict__ flag, const Counters* __restrict__ cnt) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < cap_pairs) flag[p] = (p < cnt->p_eff && np[p] > 0) ? 1u : 0u;
-}
-
 __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa, const uint32_t* __restrict__ pb,
                                                  const uint32_t* __restrict__ p_np,
                                                  const float4* __restrict__ p_pt, const float4* __restrict__ p_nf,
@@ -438,23 +438,27 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa
                                                  uint2* __restrict__ m_ab, uint32_t* __restrict__ m_np,
                                                  float4* __restrict__ c_pt, float4* __restrict__ c_nf,
                                                  Counters* __restrict__ cnt) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t M = cnt->manifolds;
         cnt->m_eff = M < cap ? M : cap;
         if (M > cap) atomicOr(&cnt->sticky, STICKY_MANIFOLDS);
     }
-    const uint32_t np = p < cnt->p_eff ? p_np[p] : 0u;
-    wave_add(&cnt->contacts, np);
-    if (np == 0) return;
-    const uint32_t m = moff[p];
-    if (m >= cap) return;
-    m_ab[m] = make_uint2(pa[p], pb[p]);
-    m_np[m] = np;
-    for (uint32_t k = 0; k < np; ++k) {
-        c_pt[(size_t)m * 4 + k] = p_pt[(size_t)p * 4 + k];
-        c_nf[(size_t)m * 4 + k] = p_nf[(size_t)p * 4 + k];
+    const uint32_t P = cnt->p_eff;
+    uint32_t contacts = 0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const uint32_t np = p_np[p];
+        contacts += np;
+        if (np == 0) continue;
+        const uint32_t m = moff[p];
+        if (m >= cap) continue;
+        m_ab[m] = make_uint2(pa[p], pb[p]);
+        m_np[m] = np;
+        for (uint32_t k = 0; k < np; ++k) {
+            c_pt[(size_t)m * 4 + k] = p_pt[(size_t)p * 4 + k];
+            c_nf[(size_t)m * 4 + k] = p_nf[(size_t)p * 4 + k];
+        }
     }
+    wave_add(&cnt->contacts, contacts);
 }
 
 // ---------------------------------------------------------------- colouring
@@ -544,9 +548,11 @@ __global__ void __launch_bounds__(256) k_colour_keep(Bodies B, const uint2* __re
                                                      uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmask,
                                                      uint32_t* __restrict__ bdeg, uint32_t* __restrict__ list,
                                                      Counters* cnt, PrevCache pc, uint32_t recolour_period) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t M = cnt->m_eff;
     const bool incremental = pc.valid != 0u && (cnt->steps_done % recolour_period) != 0u;
+    // a bounded grid strides over the manifolds (block-uniform trip count: the ballot below)
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < M; b0 += gridDim.x * blockDim.x) {
+    const uint32_t m = b0 + threadIdx.x;
     bool fresh = false;
     if (m < M) {
         const uint2 ab = m_ab[m];
@@ -581,6 +587,7 @@ __global__ void __launch_bounds__(256) k_colour_keep(Bodies B, const uint2* __re
         if (lane == leader) base = atomicAdd(&cnt->remaining, (uint32_t)__popcll(fb));
         base = __shfl(base, leader, 64);
         if (fresh) list[base + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull))] = m;
+    }
     }
 }
 
@@ -716,10 +723,11 @@ __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restric
 __global__ void __launch_bounds__(256) k_col_sort_input(const uint32_t* __restrict__ col, uint32_t cap,
                                                         uint32_t* __restrict__ key, uint32_t* __restrict__ val,
                                                         const Counters* __restrict__ cnt) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= cap || m >= cnt->m_eff) return;
-    key[m] = col[m];
-    val[m] = m;
+    const uint32_t M = min(cap, cnt->m_eff);
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        key[m] = col[m];
+        val[m] = m;
+    }
 }
 
 __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
@@ -932,13 +940,17 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
                                               const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
                                               Counters* __restrict__ cnt, float* __restrict__ feat) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t hits = 0;
     unsigned long long bytes = 0;
     // a failed step must not touch the impulse buffers
-    if (s < cnt->m_eff && cnt->sticky == 0u)
-        prep_slot(B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc, warm_in, col, bmask, bdeg, cnt, feat, s, hits,
-                  bytes);
+    const uint32_t M = cnt->sticky == 0u ? cnt->m_eff : 0u;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+        uint32_t h = 0;
+        unsigned long long b = 0;
+        prep_slot(B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc, warm_in, col, bmask, bdeg, cnt, feat, s, h, b);
+        hits += h;
+        bytes += b;
+    }
     wave_add(&cnt->warm_hits, hits);
     wave_add(&cnt->pgs_bytes, bytes);
 }
@@ -1293,9 +1305,10 @@ __device__ __forceinline__ uint32_t next_colour(unsigned long long mk, uint32_t 
 // version 0, every other inbox a version no visit waits for.
 __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const uint64_t* __restrict__ bmask,
                                                    const Counters* __restrict__ cnt) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     // the rows exist only when this step's preparation ran
-    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = cnt->m_eff;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
     const uint32_t c = R.colour[s];
@@ -1311,12 +1324,14 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
         vin_store(R.inbox, s, 1u, first ? 0u : NO_VISIT, xyz(B.sv[h.y]), xyz(B.sw[h.y]));
     }
     R.next[s] = nx;
+    }
 }
 // After a launch: each dynamic body's final velocity is in the inbox of its first
 // visit (the last visit of the last sweep passed it on there).
 __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, const Counters* __restrict__ cnt) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = cnt->m_eff;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
     const uint64_t* ib = R.inbox;
@@ -1331,6 +1346,7 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
                                 B.sv[h.y].w);
         B.sw[h.y] = make_float4(vval(ib[vin_word(s, 1, 3)]), vval(ib[vin_word(s, 1, 4)]), vval(ib[vin_word(s, 1, 5)]),
                                 0.0f);
+    }
     }
 }
 
@@ -1835,22 +1851,25 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
 __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counters* cnt,
                                                      uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
                                                      uint32_t mask, uint32_t* __restrict__ np_out) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= cnt->m_eff || cnt->sticky != 0u) return;
-    np_out[s] = R.hdr[s].z;
-    const uint4 h = R.hdr[s];
-    const uint64_t k = ((uint64_t)h.x << 32) | h.y;
-    uint32_t q = (uint32_t)mix64(k) & mask;
-    for (uint32_t probe = 0; probe <= mask; ++probe) {
-        const unsigned long long prev =
-            atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
-        if (prev == EMPTY_KEY || prev == k) {
-            slot[q] = s;
-            return;
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = cnt->m_eff;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+        np_out[s] = R.hdr[s].z;
+        const uint4 h = R.hdr[s];
+        const uint64_t k = ((uint64_t)h.x << 32) | h.y;
+        uint32_t q = (uint32_t)mix64(k) & mask;
+        bool placed = false;
+        for (uint32_t probe = 0; probe <= mask && !placed; ++probe) {
+            const unsigned long long prev =
+                atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
+            if (prev == EMPTY_KEY || prev == k) {
+                slot[q] = s;
+                placed = true;
+            }
+            q = (q + 1) & mask;
         }
-        q = (q + 1) & mask;
+        if (!placed) atomicOr(&cnt->sticky, STICKY_INTERNAL);  // table full: never expected (sized >= 2x the slots)
     }
-    atomicOr(&cnt->sticky, STICKY_INTERNAL);  // table full: never expected (sized >= 2x the slots)
 }
 
 // Moves the entries of an (a, b) -> slot table into a larger empty one.
@@ -1919,6 +1938,12 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 
 // ---------------------------------------------------------------- host launchers
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
+// Kernels whose element count is read on the device stride over it with at most
+// 1024 workgroups, so that a capacity far above the count launches few idle waves.
+static inline dim3 grid_bounded(uint32_t cap) {
+    const uint32_t b = (cap + 255u) / 256u;
+    return dim3(b == 0u ? 1u : (b > 1024u ? 1024u : b));
+}
 
 
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
@@ -1962,21 +1987,18 @@ void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* ske
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
                    uint32_t* order, uint32_t epa_groups, Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_pair_kind, grid_for(cap_pairs), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
-    hipLaunchKernelGGL(k_pair_order, grid_for(cap_pairs), dim3(256), 0, s, p_kind, kinds, order, cnt);
-    hipLaunchKernelGGL(k_narrow, grid_for(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list, cnt,
-                       order);
+    hipLaunchKernelGGL(k_pair_kind, grid_bounded(cap_pairs), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
+    hipLaunchKernelGGL(k_pair_order, grid_bounded(cap_pairs), dim3(256), 0, s, p_kind, kinds, order, cnt);
+    hipLaunchKernelGGL(k_narrow, grid_bounded(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list,
+                       cnt, order);
     // the deep pairs are few: a fixed grid strides over the queue
     hipLaunchKernelGGL(k_narrow_epa, dim3(epa_groups), dim3(64), 0, s, B, sp, pa, pb, epa_list, cap_pairs, p_np, p_pt,
                        p_nf, cnt);
 }
-void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_flag_nonzero, grid_for(cap_pairs), dim3(256), 0, s, np, cap_pairs, flag, cnt);
-}
 void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, const uint32_t* p_np,
                     const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,
                     uint32_t* m_np, float4* c_pt, float4* c_nf, Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact, grid_for(cap_pairs), dim3(256), 0, s, pa, pb, p_np, p_pt, p_nf, moff, cap, m_ab,
+    hipLaunchKernelGGL(k_compact, grid_bounded(cap_pairs), dim3(256), 0, s, pa, pb, p_np, p_pt, p_nf, moff, cap, m_ab,
                        m_np, c_pt, c_nf, cnt);
 }
 int colour_blocks_per_cu() {
@@ -2001,7 +2023,7 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
                    uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
                    Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s) {
     constexpr uint32_t SMALL = 16384;  // longest list coloured by one workgroup
-    hipLaunchKernelGGL(k_colour_keep, grid_for(cap_man), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmask, bdeg, list,
+    hipLaunchKernelGGL(k_colour_keep, grid_bounded(cap_man), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmask, bdeg, list,
                        cnt, make_prev(prev), RECOLOUR_PERIOD);
     hipLaunchKernelGGL(k_colour_small, dim3(1), dim3(1024), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, cnt,
                        65000u, SMALL);
@@ -2010,7 +2032,7 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
 }
 void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_col_sort_input, grid_for(cap), dim3(256), 0, s, col, cap, key, val, cnt);
+    hipLaunchKernelGGL(k_col_sort_input, grid_bounded(cap), dim3(256), 0, s, col, cap, key, val, cnt);
 }
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s) {
     hipLaunchKernelGGL(k_col_offsets, grid_for(max_k + 1), dim3(256), 0, s, scol, max_k, off, cnt);
@@ -2020,7 +2042,7 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
                  const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
     const PrevCache pc = make_prev(pca);
-    hipLaunchKernelGGL(k_prep, grid_for(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
+    hipLaunchKernelGGL(k_prep, grid_bounded(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
                        warm_in, col, bmask, bdeg, cnt, feat);
 }
 
@@ -2067,12 +2089,12 @@ void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R,
                      uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
                      hipEvent_t ev0, hipEvent_t ev1, const SweepStop& stop) {
     if (!nsweeps) return;
-    hipLaunchKernelGGL(k_flow_init, grid_for(cap), dim3(256), 0, s, B, R, bmask, cnt);
+    hipLaunchKernelGGL(k_flow_init, grid_bounded(cap), dim3(256), 0, s, B, R, bmask, cnt);
     if (ev0) (void)hipEventRecord(ev0, s);
     hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
                        nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune, bmask, stop);
     if (ev1) (void)hipEventRecord(ev1, s);
-    hipLaunchKernelGGL(k_flow_finish, grid_for(cap), dim3(256), 0, s, B, R, cnt);
+    hipLaunchKernelGGL(k_flow_finish, grid_bounded(cap), dim3(256), 0, s, B, R, cnt);
 }
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s) {
@@ -2090,7 +2112,7 @@ __global__ void k_cache_clear(uint64_t* __restrict__ key, uint32_t n, const Coun
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
                         uint32_t mask, uint32_t* np_out, hipStream_t s) {
     hipLaunchKernelGGL(k_cache_clear, grid_for(mask + 1), dim3(256), 0, s, key, mask + 1, cnt);
-    hipLaunchKernelGGL(k_cache_build, grid_for(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out);
+    hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out);
 }
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
                          uint32_t mask, Counters* cnt, hipStream_t s) {

@@ -68,7 +68,6 @@ void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* ske
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
                    uint32_t* order, uint32_t epa_groups, Counters* cnt, hipStream_t s);
-void launch_flag_nonzero(const uint32_t* np, uint32_t cap_pairs, uint32_t* flag, const Counters* cnt, hipStream_t s);
 void launch_compact(const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs, const uint32_t* p_np,
                     const float4* p_pt, const float4* p_nf, const uint32_t* moff, uint32_t cap, uint2* m_ab,
                     uint32_t* m_np, float4* c_pt, float4* c_nf, Counters* cnt, hipStream_t s);

@@ -116,8 +116,10 @@ struct GridBar {
 // k_sort.hip
 size_t scan_tmp_words(uint32_t n);
 // out[0..n] = exclusive prefix sums, out[n] = total (also written to *total_out when given).
+// With n_dev the count is min(n, *n_dev), read on the device (the grid is sized for n);
+// with flags the input is read as (in[i] != 0).
 void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s,
-                    uint32_t* total_out = nullptr);
+                    uint32_t* total_out = nullptr, const uint32_t* n_dev = nullptr, bool flags = false);
 size_t radix_tmp_words(uint32_t n);
 // Returns 1 when the sorted result lives in the *_alt buffers.  `n` is the
 // capacity the grid is sized for; with n_dev the element count is read on the

@@ -102,7 +102,7 @@ struct axm_world_s {
 
     uint32_t *pair_cnt = nullptr, *pair_off = nullptr, *scan_tmp = nullptr, *pair_stash = nullptr;
     uint32_t *pa = nullptr, *pb = nullptr;
-    uint32_t *p_np = nullptr, *p_flag = nullptr, *man_off = nullptr, *epa_list = nullptr, *pscan_tmp = nullptr;
+    uint32_t *p_np = nullptr, *man_off = nullptr, *epa_list = nullptr, *pscan_tmp = nullptr;
     uint32_t *p_kind = nullptr, *p_order = nullptr;  // narrowphase: kind of test per pair, pairs ordered by kind
     uint32_t* pkinds = nullptr;                       // 16 pair-kind counts + 16 fill cursors
     float4 *p_pt = nullptr, *p_nf = nullptr;
@@ -260,7 +260,6 @@ axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->pa);
     w->mem.release(w->pb);
     w->mem.release(w->p_np);
-    w->mem.release(w->p_flag);
     w->mem.release(w->man_off);
     w->mem.release(w->epa_list);
     w->mem.release(w->p_kind);
@@ -271,7 +270,6 @@ axm_status alloc_pairs(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->pa, cap));
     AXM_TRY(w->mem.get(w->pb, cap));
     AXM_TRY(w->mem.get(w->p_np, cap));
-    AXM_TRY(w->mem.get(w->p_flag, cap));
     AXM_TRY(w->mem.get(w->man_off, (size_t)cap + 1));
     AXM_TRY(w->mem.get(w->epa_list, cap));
     AXM_TRY(w->mem.get(w->p_kind, cap));
@@ -523,8 +521,8 @@ void enqueue_narrow(axm_world_s* w, const StepParams& sp) {
     hipStream_t s = w->stream;
     launch_narrow(w->B, sp, w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->epa_list, w->p_kind, w->pkinds,
                   w->p_order, 4u * (uint32_t)w->cu_count, w->cnt, s);
-    launch_flag_nonzero(w->p_np, w->cap_pairs, w->p_flag, w->cnt, s);
-    scan_exclusive(w->p_flag, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds);
+    // manifold offsets: scan of (points > 0) over the pairs found this step
+    scan_exclusive(w->p_np, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds, &w->cnt->p_eff, true);
     launch_compact(w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->man_off, w->cap_man, w->m_ab, w->m_np,
                    w->c_pt, w->c_nf, w->cnt, s);
 }
