@@ -20,6 +20,7 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+constexpr uint32_t SC_MAX_GROUPS = 128;  // workgroups of a scan launch (each takes tiles in turn)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -59,27 +60,31 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 // tiles below it at once (one per lane of its first wave), moving down 64 at a
 // time until it meets an inclusive prefix, and publishes its own.  Status word:
 // bits 62-63 state (0 none, 1 tile total, 2 inclusive prefix), bits 0-31 the
-// value.  The last workgroup to finish clears the status
-// words and counters, so every call starts from zeros (the scratch is zeroed once
-// when allocated).  Scratch layout (u32 words): [0] tile counter, [1] finished
-// counter, then nb 64-bit status words.
+// value.  A workgroup takes tiles until the counter passes the last tile, so a
+// grid of at most SC_MAX_GROUPS workgroups covers any length with few atomics on
+// the two counters (atomics on one address queue one after another).  The last
+// workgroup to finish clears the status words and counters, so every call starts
+// from zeros (the scratch is zeroed once when allocated).  Scratch layout (u32
+// words): [0] tile counter, [1] finished counter, then nb 64-bit status words.
 constexpr uint64_t SC_AGG = 1ull << 62, SC_PRE = 2ull << 62;
 
-// With n_dev the element count is min(n, *n_dev), read on the device: the grid is
-// sized for n and the tiles above the count only take part in the scratch reset.
-// With flags, the input is read as (in[i] != 0).
+// With n_dev the element count is min(n, *n_dev), read on the device (nb, the
+// tiles the scratch holds, is sized for n).  With flags, the input is read as
+// (in[i] != 0).
 __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                          uint32_t n_cap, uint32_t* __restrict__ tmp, uint32_t nb,
                                                          uint32_t* __restrict__ total_out,
                                                          const uint32_t* __restrict__ n_dev, int flags) {
     __shared__ uint32_t s_tile, s_prefix;
     uint64_t* st = reinterpret_cast<uint64_t*>(tmp + 2);
-    if (threadIdx.x == 0) s_tile = atomicAdd(&tmp[0], 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
     const uint32_t n = n_dev ? min(n_cap, *n_dev) : n_cap;
     const uint32_t nt = n == 0u ? 1u : (n + SC_TILE - 1u) / SC_TILE;  // tiles with elements (tile 0 always)
-    if (tile < nt) {
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&tmp[0], 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        __syncthreads();  // every thread has the ticket before thread 0 takes the next one
+        if (tile >= nt) break;
         // each thread owns SC_ITEMS consecutive elements of the tile
         const uint32_t base = tile * SC_TILE + threadIdx.x * SC_ITEMS;
         uint32_t v[SC_ITEMS];
@@ -143,9 +148,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restri
             out[n] = s_prefix + tot;
             if (total_out) *total_out = s_prefix + tot;
         }
+        __syncthreads();  // s_prefix is rewritten by the next tile
     }
     __shared__ bool s_last;
-    if (threadIdx.x == 0) s_last = atomicAdd(&tmp[1], 1u) == nb - 1;
+    if (threadIdx.x == 0) s_last = atomicAdd(&tmp[1], 1u) == gridDim.x - 1;
     __syncthreads();
     if (s_last) {
         // every workgroup has finished its look-back: reset the scratch for the next call
@@ -172,8 +178,8 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp
         return;
     }
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
-    hipLaunchKernelGGL(k_scan_1p, dim3(nb), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out, n_dev,
-                       flags ? 1 : 0);
+    hipLaunchKernelGGL(k_scan_1p, dim3(std::min<uint32_t>(nb, SC_MAX_GROUPS)), dim3(SC_THREADS), 0, s, in, out, n, tmp,
+                       nb, total_out, n_dev, flags ? 1 : 0);
 }
 
 // ---------------------------------------------------------------- radix sort

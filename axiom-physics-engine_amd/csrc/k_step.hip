@@ -27,15 +27,17 @@ namespace axm {
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-// Adds the sum of v over the wave to *dst with one atomic (every lane of the wave
-// must call it: the reduction reads all 64 lanes).
-__device__ __forceinline__ void wave_add(uint32_t* dst, uint32_t v) {
+// Adds the sum of v over the wave to field f of the workgroup's spread row with one
+// atomic (every lane of the wave must call it: the reduction reads all 64 lanes).
+template <class T> __device__ __forceinline__ void spread_add(Counters* cnt, T SpreadRow::*f, T v) {
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63u) == 0u && v != 0u) atomicAdd(dst, v);
+    if ((threadIdx.x & 63u) == 0u && v != T(0)) atomicAdd(&(cnt->sp[blockIdx.x % SPREAD].*f), v);
 }
-__device__ __forceinline__ void wave_add(unsigned long long* dst, unsigned long long v) {
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63u) == 0u && v != 0ull) atomicAdd(dst, v);
+template <class T> __device__ __forceinline__ T spread_sum(const Counters* cnt, T SpreadRow::*f) {
+    T s = 0;
+#pragma unroll 8
+    for (uint32_t k = 0; k < SPREAD; ++k) s += cnt->sp[k].*f;
+    return s;
 }
 
 // ---------------------------------------------------------------- k_begin
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
     // the first body of each cell inserts it; one count per wave
     const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
     const bool head = s < n && (s == 0 || skey[s - 1] != k);
-    wave_add(&cnt->cells, head ? 1u : 0u);
+    spread_add(cnt, &SpreadRow::cells, head ? 1u : 0u);
     if (!head) return;
     uint32_t c = 1;
     while (s + c < n && skey[s + c] == k) ++c;
@@ -251,7 +253,8 @@ __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, con
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
-    const bool sparse = (uint64_t)ctr->cells * 4u > n;  // fewer than four bodies per occupied cell
+    // fewer than four bodies per occupied cell
+    const bool sparse = (uint64_t)spread_sum(ctr, &SpreadRow::cells) * 4u > n;
     uint32_t c = 0;
     uint32_t* st = stash + (size_t)u * PAIR_STASH;
     visit_pairs(B, sp, skey, sval, ct, sparse, t, dz, [&](uint32_t j) {
@@ -275,7 +278,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
-    const bool sparse = (uint64_t)cnt->cells * 4u > n;
+    const bool sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n;
     uint32_t o = off[u];
     const uint32_t c = off[u + 1] - o;
     const uint32_t i = sval[t];
@@ -458,7 +461,7 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa
             c_nf[(size_t)m * 4 + k] = p_nf[(size_t)p * 4 + k];
         }
     }
-    wave_add(&cnt->contacts, contacts);
+    spread_add(cnt, &SpreadRow::contacts, contacts);
 }
 
 // ---------------------------------------------------------------- colouring
@@ -951,8 +954,8 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
         hits += h;
         bytes += b;
     }
-    wave_add(&cnt->warm_hits, hits);
-    wave_add(&cnt->pgs_bytes, bytes);
+    spread_add(cnt, &SpreadRow::warm_hits, hits);
+    spread_add(cnt, &SpreadRow::pgs_bytes, bytes);
 }
 
 // Memory policy of the sweep: plain accesses when every colour is its own
@@ -1901,6 +1904,12 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
 __global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ kinds) {
     uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
     if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
+    for (uint32_t k = threadIdx.x; k < SPREAD; k += blockDim.x) {
+        cnt->sp[k].contacts = 0u;
+        cnt->sp[k].warm_hits = 0u;
+        cnt->sp[k].cells = 0u;
+        cnt->sp[k].pgs_bytes = 0ull;
+    }
     if (kinds && threadIdx.x < 2u * PAIR_KINDS) kinds[threadIdx.x] = 0u;
 }
 
@@ -1913,7 +1922,7 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
     if (i == 0) {
         cnt->steps_done += 1u;
         const uint32_t sw = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
-        cnt->pgs_bytes_total += cnt->pgs_bytes * (unsigned long long)sw;
+        cnt->pgs_bytes_total += spread_sum(cnt, &SpreadRow::pgs_bytes) * (unsigned long long)sw;
         cnt->sweeps_total += sw;
     }
     if (i >= n) return;

@@ -69,16 +69,31 @@ struct CellTable {
     uint32_t bmask;  // bit index mask (power of two minus one)
 };
 
+// Sums that thousands of workgroups add to are spread over SPREAD rows of one
+// 128-byte line each (workgroup b adds to row b mod SPREAD) and summed where they
+// are read: device-scope atomics on one line queue one after another.
+constexpr uint32_t SPREAD = 64;
+struct SpreadRow {
+    uint32_t contacts;   // contact points
+    uint32_t warm_hits;  // points that found last step's impulse
+    uint32_t cells;      // broadphase: occupied cells
+    uint32_t pad;
+    unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
+    uint32_t fill[26];
+};
+static_assert(sizeof(SpreadRow) == 128, "one line per row");
+
 // Device-resident step control block.  Every count a later kernel needs is read
 // here on the device, so a step launches without host synchronisation (and can
-// be captured into a HIP graph).  The first block is cleared at every step start.
+// be captured into a HIP graph).  The first block and the spread sums are
+// cleared at every step start.
 struct Counters {
     uint32_t pairs;        // candidate pairs found (may exceed the capacity)
     uint32_t manifolds;    // manifolds found (may exceed the capacity)
-    uint32_t contacts;
+    uint32_t pad1;
     uint32_t gjk_fail;
     uint32_t epa_fail;
-    uint32_t warm_hits;
+    uint32_t pad2;
     uint32_t remaining;    // colouring: uncoloured manifolds
     uint32_t max_colour;   // colouring: max colour + 1
     uint32_t last_round;   // colouring: last round that coloured something + 1
@@ -86,17 +101,24 @@ struct Counters {
     uint32_t residual_bits;
     uint32_t p_eff;        // min(pairs, pair capacity)
     uint32_t m_eff;        // min(manifolds, manifold capacity)
-    uint32_t cells;        // broadphase: occupied cells
+    uint32_t pad3;
     uint32_t sweeps_run;   // sweeps of a launch with residual checks (set when it ends)
     uint32_t pad0;
-    unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
+    unsigned long long pad4;
     // ---- kept across steps
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
     uint32_t steps_done;   // steps whose integration ran
     unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
     unsigned long long sweeps_total;     // sweeps of all completed steps
+    // ---- spread sums, cleared per step
+    alignas(128) SpreadRow sp[SPREAD];
 };
-constexpr uint32_t COUNTERS_RESET_WORDS = 18;  // words cleared per step (through pgs_bytes)
+template <class T> inline T spread_total(const Counters& c, T SpreadRow::*f) {
+    T s = 0;
+    for (uint32_t k = 0; k < SPREAD; ++k) s += c.sp[k].*f;
+    return s;
+}
+constexpr uint32_t COUNTERS_RESET_WORDS = 18;  // words of the first block (cleared per step)
 enum : uint32_t {
     STICKY_PAIRS = 1u,        // pair capacity exceeded
     STICKY_MANIFOLDS = 2u,    // manifold capacity exceeded

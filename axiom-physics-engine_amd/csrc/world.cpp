@@ -685,7 +685,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
         w->prof.steps += 1;
         w->prof.pgs_launches += nev / 2;
         w->prof.pgs_ms += ms;
-        w->prof.pgs_bytes += (double)w->h_cnt->pgs_bytes * (double)ran;
+        w->prof.pgs_bytes += (double)spread_total(*w->h_cnt, &SpreadRow::pgs_bytes) * (double)ran;
         w->prof.sweeps += ran;
     }
     w->cur = pv;
@@ -699,13 +699,13 @@ void fill_stats(axm_world_s* w, axm_step_stats& S) {
     const Counters& h = *w->h_cnt;
     S.pairs = h.pairs;
     S.manifolds = h.manifolds;
-    S.contacts = h.contacts;
+    S.contacts = spread_total(h, &SpreadRow::contacts);
     S.colours = h.max_colour;
     S.colour_rounds = h.last_round;
     S.iterations = w->last_iters;
     S.gjk_fail = h.gjk_fail;
     S.epa_fail = h.epa_fail;
-    S.warm_hits = h.warm_hits;
+    S.warm_hits = spread_total(h, &SpreadRow::warm_hits);
     S.overflow = (h.sticky & STICKY_PAIRS ? 1u : 0u) | (h.sticky & STICKY_MANIFOLDS ? 2u : 0u);
 }
 
