@@ -553,44 +553,57 @@ __global__ void __launch_bounds__(256) k_colour_keep(Bodies B, const uint2* __re
                                                      Counters* cnt, PrevCache pc, uint32_t recolour_period) {
     const uint32_t M = cnt->m_eff;
     const bool incremental = pc.valid != 0u && (cnt->steps_done % recolour_period) != 0u;
-    // a bounded grid strides over the manifolds (block-uniform trip count: the ballot below)
+    // per workgroup, the largest kept colour and the list entries are gathered in LDS
+    // and go out with one atomic each (atomics on one address queue one after another)
+    __shared__ uint32_t s_cmax, s_n, s_base;
+    // a bounded grid strides over the manifolds (block-uniform trip count: the barriers below)
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < M; b0 += gridDim.x * blockDim.x) {
-    const uint32_t m = b0 + threadIdx.x;
-    bool fresh = false;
-    if (m < M) {
-        const uint2 ab = m_ab[m];
-        const bool bd = dyn_body(B, ab.y);
-        atomicAdd(&bdeg[ab.x], 1u);
-        if (bd) atomicAdd(&bdeg[ab.y], 1u);
-        const uint64_t pr = ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
-        prio[m] = pr;
-        uint32_t c = UNSET, ps;
-        if (incremental && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps)) {
-            const uint32_t pcol = pc.colour[ps];
-            if (pcol < 64u) c = pcol;
+        const uint32_t m = b0 + threadIdx.x;
+        if (threadIdx.x == 0) {
+            s_cmax = 0u;
+            s_n = 0u;
         }
-        col[m] = c;
-        if (c != UNSET) {
-            atomicOr((unsigned long long*)&bmask[ab.x], 1ull << c);
-            if (bd) atomicOr((unsigned long long*)&bmask[ab.y], 1ull << c);
-            atomicMax(&cnt->max_colour, c + 1u);
-        } else {
-            const uint64_t t = (1ull << 48) | pr;
-            atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
-            if (bd) atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
-            fresh = true;
+        __syncthreads();
+        bool fresh = false;
+        if (m < M) {
+            const uint2 ab = m_ab[m];
+            const bool bd = dyn_body(B, ab.y);
+            atomicAdd(&bdeg[ab.x], 1u);
+            if (bd) atomicAdd(&bdeg[ab.y], 1u);
+            const uint64_t pr =
+                ((uint64_t)pair_hash24(local_id(B, ab.x), local_id(B, ab.y)) << 24) | (uint64_t)(m & 0xFFFFFFu);
+            prio[m] = pr;
+            uint32_t c = UNSET, ps;
+            if (incremental && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps)) {
+                const uint32_t pcol = pc.colour[ps];
+                if (pcol < 64u) c = pcol;
+            }
+            col[m] = c;
+            if (c != UNSET) {
+                atomicOr((unsigned long long*)&bmask[ab.x], 1ull << c);
+                if (bd) atomicOr((unsigned long long*)&bmask[ab.y], 1ull << c);
+                atomicMax(&s_cmax, c + 1u);
+            } else {
+                const uint64_t t = (1ull << 48) | pr;
+                atomicMax((unsigned long long*)&bmax0[ab.x], (unsigned long long)t);
+                if (bd) atomicMax((unsigned long long*)&bmax0[ab.y], (unsigned long long)t);
+                fresh = true;
+            }
         }
-    }
-    // list entry per manifold to colour: one atomic per wave
-    const unsigned long long fb = __ballot(fresh);
-    if (fb) {
+        // list entries of the manifolds to colour: a slot range per wave within the workgroup's
+        const unsigned long long fb = __ballot(fresh);
         const uint32_t lane = __lane_id();
-        const uint32_t leader = (uint32_t)__ffsll(fb) - 1u;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cnt->remaining, (uint32_t)__popcll(fb));
-        base = __shfl(base, leader, 64);
-        if (fresh) list[base + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull))] = m;
-    }
+        uint32_t wbase = 0;
+        if (fb && lane == 0u) wbase = atomicAdd(&s_n, (uint32_t)__popcll(fb));
+        wbase = __shfl(wbase, 0, 64);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_cmax) atomicMax(&cnt->max_colour, s_cmax);
+            s_base = s_n ? atomicAdd(&cnt->remaining, s_n) : 0u;
+        }
+        __syncthreads();
+        if (fresh) list[s_base + wbase + (uint32_t)__popcll(fb & ((1ull << lane) - 1ull))] = m;
+        __syncthreads();  // s_base and s_n are reset by the next iteration
     }
 }
 
@@ -1316,17 +1329,32 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
     const uint2 dp = R.dep[s];
     const uint32_t c = R.colour[s];
     uint2 nx = make_uint2(UNSET, UNSET);
+    f3 va = {0.0f, 0.0f, 0.0f}, wa = va, vb = va, wb = va;
+    uint32_t ra = NO_VISIT, rb = NO_VISIT;
     if (dp.x != UNSET) {
         nx.x = B.vslot[(size_t)h.x * 64u + next_colour(bmask[h.x], c)];
-        const bool first = (dp.x & 0xFFu) == 0u;
-        vin_store(R.inbox, s, 0u, first ? 0u : NO_VISIT, xyz(B.sv[h.x]), xyz(B.sw[h.x]));
+        if ((dp.x & 0xFFu) == 0u) ra = 0u;
+        va = xyz(B.sv[h.x]);
+        wa = xyz(B.sw[h.x]);
     }
     if (dp.y != UNSET) {
         nx.y = B.vslot[(size_t)h.y * 64u + next_colour(bmask[h.y], c)];
-        const bool first = (dp.y & 0xFFu) == 0u;
-        vin_store(R.inbox, s, 1u, first ? 0u : NO_VISIT, xyz(B.sv[h.y]), xyz(B.sw[h.y]));
+        if ((dp.y & 0xFFu) == 0u) rb = 0u;
+        vb = xyz(B.sv[h.y]);
+        wb = xyz(B.sw[h.y]);
     }
     R.next[s] = nx;
+    // the whole inbox (words 0..13, (value, version) pairs as laid out by vin_word) in
+    // seven 16-byte stores; the sweep kernel, a later launch, sees them after the
+    // launch boundary
+    uint4* q = reinterpret_cast<uint4*>(R.inbox + (size_t)s * IB_WORDS);
+    q[0] = make_uint4(fbits(va.x), ra, fbits(va.y), ra);
+    q[1] = make_uint4(fbits(va.z), ra, fbits(wa.x), ra);
+    q[2] = make_uint4(fbits(wa.y), ra, 0u, NO_VISIT);
+    q[3] = make_uint4(fbits(wa.z), ra, fbits(wb.z), rb);
+    q[4] = make_uint4(fbits(vb.x), rb, fbits(vb.y), rb);
+    q[5] = make_uint4(fbits(vb.z), rb, fbits(wb.x), rb);
+    q[6] = make_uint4(fbits(wb.y), rb, 0u, NO_VISIT);
     }
 }
 // After a launch: each dynamic body's final velocity is in the inbox of its first
@@ -1949,9 +1977,9 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 // Kernels whose element count is read on the device stride over it with at most
 // 1024 workgroups, so that a capacity far above the count launches few idle waves.
-static inline dim3 grid_bounded(uint32_t cap) {
+static inline dim3 grid_bounded(uint32_t cap, uint32_t max_groups = 1024u) {
     const uint32_t b = (cap + 255u) / 256u;
-    return dim3(b == 0u ? 1u : (b > 1024u ? 1024u : b));
+    return dim3(b == 0u ? 1u : (b > max_groups ? max_groups : b));
 }
 
 
@@ -1996,8 +2024,9 @@ void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* ske
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
                    uint32_t* order, uint32_t epa_groups, Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_pair_kind, grid_bounded(cap_pairs), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
-    hipLaunchKernelGGL(k_pair_order, grid_bounded(cap_pairs), dim3(256), 0, s, p_kind, kinds, order, cnt);
+    // 256 workgroups: each adds its kind counts to the 16 shared counters once per pass
+    hipLaunchKernelGGL(k_pair_kind, grid_bounded(cap_pairs, 256u), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
+    hipLaunchKernelGGL(k_pair_order, grid_bounded(cap_pairs, 256u), dim3(256), 0, s, p_kind, kinds, order, cnt);
     hipLaunchKernelGGL(k_narrow, grid_bounded(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list,
                        cnt, order);
     // the deep pairs are few: a fixed grid strides over the queue
