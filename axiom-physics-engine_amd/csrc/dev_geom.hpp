@@ -38,11 +38,17 @@ struct Man {
 };
 
 AXM_D void man_add(Man& m, f3 p, f3 n, float depth, uint32_t fid) {
+    // unrolled over the four slots with constant indices, so that the manifold stays
+    // in registers (an indexed write would place it in scratch memory)
     if (m.np < 4) {
-        m.pt[m.np].p = p;
-        m.pt[m.np].n = n;
-        m.pt[m.np].depth = depth;
-        m.pt[m.np].fid = fid;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (k == m.np) {
+                m.pt[k].p = p;
+                m.pt[k].n = n;
+                m.pt[k].depth = depth;
+                m.pt[k].fid = fid;
+            }
         ++m.np;
     }
 }
@@ -296,6 +302,7 @@ AXM_D bool simplex_closest(SV* s, int& n, float* lam, f3& v) {
     SV bs[3];
     float bl[3] = {0.0f, 0.0f, 0.0f};
     f3 bv3 = {0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         const f3 a = s[F[f][0]].w, b = s[F[f][1]].w, c = s[F[f][2]].w, d = s[F[f][3]].w;
         const f3 nrm = cross(b - a, c - a);
@@ -304,14 +311,17 @@ AXM_D bool simplex_closest(SV* s, int& n, float* lam, f3& v) {
             inside = false;
             SV ts[3] = {s[F[f][0]], s[F[f][1]], s[F[f][2]]};
             int tn = 3;
-            float tl[3];
+            float tl[3] = {0.0f, 0.0f, 0.0f};
             f3 tv;
             tri_closest(ts, tn, tl, tv);
             const float dd = dot(tv, tv);
             if (bestn == 0 || dd < best) {
                 best = dd;
                 bestn = tn;
-                for (int i = 0; i < tn; ++i) { bs[i] = ts[i]; bl[i] = tl[i]; }
+                // all three copied (constant indices keep the simplex in registers);
+                // entries from tn on are never read
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { bs[i] = ts[i]; bl[i] = tl[i]; }
                 bv3 = tv;
             }
         }
@@ -321,7 +331,8 @@ AXM_D bool simplex_closest(SV* s, int& n, float* lam, f3& v) {
         return false;
     }
     n = bestn;
-    for (int i = 0; i < n; ++i) { s[i] = bs[i]; lam[i] = bl[i]; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { s[i] = bs[i]; lam[i] = bl[i]; }
     v = bv3;
     return true;
 }
@@ -350,27 +361,37 @@ AXM_D GjkOut gjk(const Shp& Aw, const Shp& Bw, int max_iter = 32) {
         if (vv <= 1e-12f) { o.status = 1; break; }
         const SV w = mink_support(A, B, -v);
         if (vv - dot(v, w.w) <= 1e-5f * vv + 1e-9f) { o.status = 0; break; }
+        // loops over the four slots with constant indices keep the simplex in registers;
+        // slots from o.n on are never read
         bool dup = false;
-        for (int i = 0; i < o.n; ++i)
-            if (o.s[i].w.x == w.w.x && o.s[i].w.y == w.w.y && o.s[i].w.z == w.w.z) dup = true;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i < o.n && o.s[i].w.x == w.w.x && o.s[i].w.y == w.w.y && o.s[i].w.z == w.w.z) dup = true;
         if (dup) { o.status = 0; break; }
         SV ps[4];
         float pl[4];
         const int pn = o.n;
-        for (int i = 0; i < pn; ++i) { ps[i] = o.s[i]; pl[i] = lam[i]; }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { ps[i] = o.s[i]; pl[i] = lam[i]; }
         const f3 pv = v;
-        o.s[o.n++] = w;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i == o.n) o.s[i] = w;
+        ++o.n;
         if (!simplex_closest(o.s, o.n, lam, v)) { o.status = 1; break; }
         if (!(dot(v, v) < vv)) {
             o.n = pn;
-            for (int i = 0; i < pn; ++i) { o.s[i] = ps[i]; lam[i] = pl[i]; }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { o.s[i] = ps[i]; lam[i] = pl[i]; }
             v = pv;
             o.status = 0;
             break;
         }
     }
     f3 pa = {0.0f, 0.0f, 0.0f}, pb = {0.0f, 0.0f, 0.0f};
-    for (int i = 0; i < o.n; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= o.n) break;
         pa = pa + o.s[i].a * lam[i];
         pb = pb + o.s[i].b * lam[i];
     }
@@ -771,7 +792,9 @@ AXM_D int collide_canon(Man& m, const Shp& A, const Shp& B, float margin, bool& 
 
 // Converts a manifold computed as (B, A) back to (A, B).
 AXM_D void flip_manifold(Man& m) {
-    for (uint32_t k = 0; k < m.np; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= m.np) break;
         m.pt[k].p = m.pt[k].p - m.pt[k].n * m.pt[k].depth;
         m.pt[k].n = -m.pt[k].n;
     }

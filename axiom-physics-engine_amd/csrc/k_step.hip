@@ -313,7 +313,9 @@ __device__ __forceinline__ Shp load_shape(const Bodies& B, uint32_t i) {
 __device__ __forceinline__ void store_pair(uint32_t p, const Man& m, uint32_t* __restrict__ p_np,
                                            float4* __restrict__ p_pt, float4* __restrict__ p_nf) {
     p_np[p] = m.np;
-    for (uint32_t k = 0; k < m.np; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= m.np) break;
         p_pt[(size_t)p * 4 + k] = f4(m.pt[k].p, m.pt[k].depth);
         p_nf[(size_t)p * 4 + k] = f4(m.pt[k].n, __uint_as_float(m.pt[k].fid));
     }
