@@ -411,22 +411,24 @@ __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const u
     }
 }
 
-// Deep box-capsule pairs (cores intersect): EPA, one thread per queued pair (a
-// fixed grid strides over the queue).
+// Deep box-capsule pairs (cores intersect): EPA, one queued pair per workgroup of
+// one wave at a time (a fixed grid strides over the queue).
 __global__ void __launch_bounds__(64) k_narrow_epa(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
                                                    const uint32_t* __restrict__ pb,
                                                    const uint32_t* __restrict__ epa_list, uint32_t cap,
                                                    uint32_t* __restrict__ p_np, float4* __restrict__ p_pt,
                                                    float4* __restrict__ p_nf, Counters* __restrict__ cnt) {
     const uint32_t nq = min(cnt->epa_count, cap);
-    // pairs spread one per wave first (queue entry q goes to wave q mod waves): each
-    // pair's expansion runs its own iteration count without waiting on other lanes
-    for (uint32_t q = blockIdx.x + threadIdx.x * gridDim.x; q < nq; q += gridDim.x * blockDim.x) {
+    // one pair per workgroup (one wave) at a time, on its first lane: each pair's
+    // expansion runs its own iteration count, and its polytope lives in the
+    // workgroup's LDS instead of scratch memory
+    __shared__ EpaWork W;
+    if (threadIdx.x != 0u) return;
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
         const uint32_t p = epa_list[q];
         const uint32_t a = pa[p], b = pb[p];
         Shp A = load_shape(B, a), Bs = load_shape(B, b);
         const bool swap = A.type > Bs.type;
-        EpaWork W;
         Man m;
         m.np = 0;
         const int st = swap ? box_capsule_epa(m, Bs, A, W) : box_capsule_epa(m, A, Bs, W);
