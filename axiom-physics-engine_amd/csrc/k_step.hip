@@ -228,15 +228,29 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
             if (!ok[q] || kk == EMPTY_KEY) continue;
             uint2 r = make_uint2(e[q].z, e[q].w);
             if (kk != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
-            for (uint32_t s = r.x; s < r.x + r.y; ++s) {
-                const float4 mnj = B.smin[s], mxj = B.smax[s];
+            // the cell's bodies in sorted order, four loaded at a time (a dense cell holds
+            // dozens, and each load of the loop would otherwise wait on its own)
+            auto test = [&](const float4 mnj, const float4 mxj) {
                 const uint32_t j = __float_as_uint(mnj.w);
-                if (j <= i) continue;
-                if (!dyn_i && !(mxj.w > 0.0f)) continue;
+                if (j <= i) return;
+                if (!dyn_i && !(mxj.w > 0.0f)) return;
                 if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
                     mx.z >= mnj.z)
                     f(j);
+            };
+            const uint32_t se = r.x + r.y;
+            uint32_t s = r.x;
+            for (; s + 4u <= se; s += 4u) {
+                float4 mn4[4], mx4[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    mn4[u] = B.smin[s + u];
+                    mx4[u] = B.smax[s + u];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) test(mn4[u], mx4[u]);
             }
+            for (; s < se; ++s) test(B.smin[s], B.smax[s]);
         }
     }
 }
