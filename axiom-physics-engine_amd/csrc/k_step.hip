@@ -141,7 +141,15 @@ __device__ __forceinline__ uint32_t cell_bit(uint64_t h, uint32_t bmask) { retur
 __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
-    if (i <= (t.bmask >> 5)) t.bits[i] = 0u;
+    if (t.mbits) {
+        // the words the last step set (several bodies may name one word: all write zero)
+        if (i < t.mlist_n) {
+            const uint32_t w = t.mlist[i];
+            if (w != UNSET) t.mbits[w] = 0u;
+        }
+    } else if (i <= (t.bmask >> 5)) {
+        t.bits[i] = 0u;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
@@ -151,12 +159,18 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
     const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
     const bool head = s < n && (s == 0 || skey[s - 1] != k);
     spread_add(cnt, &SpreadRow::cells, head ? 1u : 0u);
+    if (t.mbits && s < n) t.mlist[s] = head ? (uint32_t)(k & 0x3FFFFFFFull) >> 5 : UNSET;
     if (!head) return;
     uint32_t c = 1;
     while (s + c < n && skey[s + c] == k) ++c;
     if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
-    const uint32_t bit = cell_bit(mix64(k), t.bmask);
-    atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
+    if (t.mbits) {
+        const uint32_t code = (uint32_t)(k & 0x3FFFFFFFull);
+        atomicOr(&t.mbits[code >> 5], 1u << (code & 31u));
+    } else {
+        const uint32_t bit = cell_bit(mix64(k), t.bmask);
+        atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
+    }
 }
 
 // ---------------------------------------------------------------- pairs
@@ -215,8 +229,9 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
             const uint64_t h = mix64(nk[q]);
             hs[q] = (uint32_t)h & ct.mask;
             if (sparse) {
-                const uint32_t bit = cell_bit(h, ct.bmask);
-                ok[q] = ok[q] && ((ct.bits[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
+                const uint32_t bit = ct.mbits ? (uint32_t)(nk[q] & 0x3FFFFFFFull) : cell_bit(h, ct.bmask);
+                const uint32_t* words = ct.mbits ? ct.mbits : ct.bits;
+                ok[q] = ok[q] && ((words[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
             }
         }
 #pragma unroll
@@ -2018,7 +2033,7 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
 }
 void launch_cell_clear(const CellTable& t, hipStream_t s) {
-    const uint32_t n = std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
+    const uint32_t n = std::max(t.mask + 1u, t.mbits ? t.mlist_n : (t.bmask >> 5) + 1u);
     hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
 }
 void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s) {

@@ -60,14 +60,22 @@ constexpr uint32_t BP_PARTS = 3;
 constexpr uint32_t PAIR_STASH = 16;  // partners kept per part by the counting pass
 
 // Cell hash table (linear probing): entry = (key lo, key hi, first sorted index,
-// count), one 16-byte access per probe; plus an occupancy bitmap (one bit per
-// hashed cell) that sparse scenes test before probing.
+// count), one 16-byte access per probe; plus an occupancy bitmap that sparse
+// scenes test before probing: one bit per hashed cell, or, for one large world,
+// one bit per cell at its 30-bit Morton code (neighbouring cells share words, and
+// bodies in sorted order probe the same lines).  The Morton bits a step sets are
+// listed per sorted body (the word, or UNSET) so the next step clears only those.
 struct CellTable {
     uint4* ent;
     uint32_t mask;
     uint32_t* bits;
     uint32_t bmask;  // bit index mask (power of two minus one)
+    uint32_t* mbits;  // Morton bitmap (2^25 words), or null
+    uint32_t* mlist;  // per sorted body of the last step: the Morton word it set, or UNSET
+    uint32_t mlist_n; // bodies of the last step (entries of mlist to clear)
 };
+constexpr uint32_t MORTON_WORDS = 1u << 25;        // 2^30 cells / 32
+constexpr uint32_t MORTON_MIN_BODIES = 1u << 16;   // one world at least this large uses the Morton bitmap
 
 // Sums that thousands of workgroups add to are spread over SPREAD rows of one
 // 128-byte line each (workgroup b adds to row b mod SPREAD) and summed where they

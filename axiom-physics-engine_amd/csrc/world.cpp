@@ -211,6 +211,7 @@ void free_bodies(axm_world_s* w) {
     w->mem.release(w->sort_tmp);
     w->mem.release(w->ct.ent);
     w->mem.release(w->ct.bits);
+    w->mem.release(w->ct.mlist);
     w->mem.release(w->pair_cnt);
     w->mem.release(w->pair_off);
     w->mem.release(w->scan_tmp);
@@ -244,6 +245,7 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     const uint32_t TB = std::min<uint32_t>(T, 1u << 24) * 8u;  // occupancy bits: 8 per table entry, at most 128 Mi
     AXM_TRY(w->mem.get(w->ct.bits, TB / 32u));
     w->ct.bmask = TB - 1;
+    AXM_TRY(w->mem.get(w->ct.mlist, cap));
     AXM_TRY(w->mem.get(w->pair_cnt, (size_t)BP_PARTS * cap));
     AXM_TRY(w->mem.get(w->pair_off, (size_t)BP_PARTS * cap + 1));
     AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(BP_PARTS * cap)));
@@ -1007,6 +1009,16 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * (nw + 1), hipMemcpyHostToDevice, s));
     }
     AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
+    // one large world: occupancy bits at the cells' Morton codes (cleared here in full,
+    // then per step only the words the previous step set)
+    const bool morton = nw <= 1u && n >= MORTON_MIN_BODIES;
+    if (morton && !w->ct.mbits) AXM_TRY(w->mem.get(w->ct.mbits, MORTON_WORDS));
+    if (!morton && w->ct.mbits) w->mem.release(w->ct.mbits);
+    w->ct.mlist_n = morton ? n : 0u;
+    if (morton) {
+        AXM_HIP(hipMemsetAsync(w->ct.mbits, 0, 4ull * MORTON_WORDS, s));
+        AXM_HIP(hipMemsetAsync(w->ct.mlist, 0xFF, 4ull * n, s));
+    }
     AXM_HIP(hipStreamSynchronize(s));
     w->n = n;
     w->n_worlds = std::max(nw, 1u);

@@ -87,6 +87,16 @@ def test_broadphase_bruteforce_config2_sample(axm, orc, gpu):
     assert len(ga) == len(set(zip(ga.tolist(), gb.tolist())))
 
 
+def test_morton_occupancy_bits_keep_pairs(axm, orc, gpu):
+    """One sparse world of 200k bodies (above the size where the occupancy bitmap is indexed by
+    the cells' Morton codes): pairs, manifolds, colouring and state equal the oracle's over three
+    steps (each step clears only the bitmap words the previous one set)."""
+    b, d, w, ow = _pair_up(axm, orc, 2, 200000)
+    for _ in range(3):
+        st, ost = _compare_step(axm, w, ow, 10)
+    assert st.pairs > 1000
+
+
 def test_pile_step_parity(axm, orc, gpu):
     b, d, w, ow = _pair_up(axm, orc, 1, 3000)
     for _ in range(3):
