@@ -211,62 +211,92 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     const uint32_t mort = (uint32_t)(key & 0x3FFFFFFFull);
     const uint64_t wbits = key & ~0x3FFFFFFFull;
     const int cx = (int)compact1by2(mort), cy = (int)compact1by2(mort >> 1), cz = (int)compact1by2(mort >> 2);
-    {
-        const int Z = cz + dz;
-        if (Z < 0 || Z > 1023) return;
-        // the nine cells of this z slice: first probes of all nine in flight together; in a
-        // sparse scene (few bodies per occupied cell) the occupancy bits, which stay in
-        // the cache, are read first and only occupied cells are probed
-        uint64_t nk[9];
-        uint4 e[9];
-        uint32_t hs[9];
-        bool ok[9];
+    const int Z = cz + dz;
+    if (Z < 0 || Z > 1023) return;
+    // the cell's bodies in sorted order, four loaded at a time (a dense cell holds
+    // dozens, and each load of the loop would otherwise wait on its own)
+    auto test = [&](const float4 mnj, const float4 mxj) {
+        const uint32_t j = __float_as_uint(mnj.w);
+        if (j <= i) return;
+        if (!dyn_i && !(mxj.w > 0.0f)) return;
+        if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z && mx.z >= mnj.z)
+            f(j);
+    };
+    auto visit_range = [&](uint2 r) {
+        const uint32_t se = r.x + r.y;
+        uint32_t s = r.x;
+        for (; s + 4u <= se; s += 4u) {
+            float4 mn4[4], mx4[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                mn4[u] = B.smin[s + u];
+                mx4[u] = B.smax[s + u];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) test(mn4[u], mx4[u]);
+        }
+        for (; s < se; ++s) test(B.smin[s], B.smax[s]);
+    };
+    // the nine cells of this z slice in canonical (dy, dx) order; Morton codes from the
+    // dilated coordinates (three per axis instead of nine full interleavings)
+    uint32_t px[3], py[3];
+    bool okx[3], oky[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int X = cx + a - 1, Y = cy + a - 1;
+        okx[a] = X >= 0 && X <= 1023;
+        oky[a] = Y >= 0 && Y <= 1023;
+        px[a] = part1by2((uint32_t)(okx[a] ? X : 0));
+        py[a] = part1by2((uint32_t)(oky[a] ? Y : 0)) << 1;
+    }
+    const uint32_t pz = part1by2((uint32_t)Z) << 2;
+    if (sparse) {
+        // few bodies per occupied cell: the occupancy bits (which stay in the cache) are
+        // read first, then only the occupied cells are hashed and probed, in order
+        uint32_t occ = 0;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            const int Y = cy + q / 3 - 1, X = cx + q % 3 - 1;
-            ok[q] = Y >= 0 && Y <= 1023 && X >= 0 && X <= 1023;
-            nk[q] = wbits | morton3((uint32_t)(ok[q] ? X : 0), (uint32_t)(ok[q] ? Y : 0), (uint32_t)Z);
-            const uint64_t h = mix64(nk[q]);
-            hs[q] = (uint32_t)h & ct.mask;
-            if (sparse) {
-                const uint32_t bit = ct.mbits ? (uint32_t)(nk[q] & 0x3FFFFFFFull) : cell_bit(h, ct.bmask);
-                const uint32_t* words = ct.mbits ? ct.mbits : ct.bits;
-                ok[q] = ok[q] && ((words[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
-            }
+            const uint32_t code = px[q % 3] | py[q / 3] | pz;
+            const uint32_t bit = ct.mbits ? code : cell_bit(mix64(wbits | code), ct.bmask);
+            const uint32_t* words = ct.mbits ? ct.mbits : ct.bits;
+            const bool on = okx[q % 3] && oky[q / 3] && ((words[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
+            occ |= on ? 1u << q : 0u;
         }
-#pragma unroll
-        for (int q = 0; q < 9; ++q)
-            e[q] = ok[q] ? ct.ent[hs[q]] : make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {  // (dy, dx) in canonical order
-            const uint64_t kk = ent_key(e[q]);
-            if (!ok[q] || kk == EMPTY_KEY) continue;
-            uint2 r = make_uint2(e[q].z, e[q].w);
-            if (kk != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
-            // the cell's bodies in sorted order, four loaded at a time (a dense cell holds
-            // dozens, and each load of the loop would otherwise wait on its own)
-            auto test = [&](const float4 mnj, const float4 mxj) {
-                const uint32_t j = __float_as_uint(mnj.w);
-                if (j <= i) return;
-                if (!dyn_i && !(mxj.w > 0.0f)) return;
-                if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z &&
-                    mx.z >= mnj.z)
-                    f(j);
-            };
-            const uint32_t se = r.x + r.y;
-            uint32_t s = r.x;
-            for (; s + 4u <= se; s += 4u) {
-                float4 mn4[4], mx4[4];
-#pragma unroll
-                for (uint32_t u = 0; u < 4u; ++u) {
-                    mn4[u] = B.smin[s + u];
-                    mx4[u] = B.smax[s + u];
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < 4u; ++u) test(mn4[u], mx4[u]);
-            }
-            for (; s < se; ++s) test(B.smin[s], B.smax[s]);
+        while (occ) {
+            const int q = __ffs(occ) - 1;
+            occ &= occ - 1u;
+            const uint64_t nk = wbits | (px[q % 3] | py[q / 3] | pz);
+            const uint32_t hs = (uint32_t)mix64(nk) & ct.mask;
+            const uint4 e = ct.ent[hs];
+            const uint64_t kk = ent_key(e);
+            if (kk == EMPTY_KEY) continue;
+            uint2 r = make_uint2(e.z, e.w);
+            if (kk != nk && !ht_find_from(ct, nk, (hs + 1u) & ct.mask, r)) continue;
+            visit_range(r);
         }
+        return;
+    }
+    // dense: the first probes of all nine cells in flight together
+    uint64_t nk[9];
+    uint4 e[9];
+    uint32_t hs[9];
+    bool ok[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        ok[q] = okx[q % 3] && oky[q / 3];
+        nk[q] = wbits | (px[q % 3] | py[q / 3] | pz);
+        hs[q] = (uint32_t)mix64(nk[q]) & ct.mask;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        e[q] = ok[q] ? ct.ent[hs[q]] : make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const uint64_t kk = ent_key(e[q]);
+        if (!ok[q] || kk == EMPTY_KEY) continue;
+        uint2 r = make_uint2(e[q].z, e[q].w);
+        if (kk != nk[q] && !ht_find_from(ct, nk[q], (hs[q] + 1u) & ct.mask, r)) continue;
+        visit_range(r);
     }
 }
 
