@@ -20,7 +20,9 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
-constexpr uint32_t SC_MAX_GROUPS = 128;  // workgroups of a scan launch (each takes tiles in turn)
+// workgroups of a scan launch (each takes tiles in turn): enough to keep many tiles in
+// flight; fewer when the count is read on the device and usually far below the capacity
+constexpr uint32_t SC_MAX_GROUPS = 512, SC_MAX_GROUPS_DEV = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -61,8 +63,8 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
 // time until it meets an inclusive prefix, and publishes its own.  Status word:
 // bits 62-63 state (0 none, 1 tile total, 2 inclusive prefix), bits 0-31 the
 // value.  A workgroup takes tiles until the counter passes the last tile, so a
-// grid of at most SC_MAX_GROUPS workgroups covers any length with few atomics on
-// the two counters (atomics on one address queue one after another).  The last
+// bounded grid covers any length with few atomics on the two counters (atomics on
+// one address queue one after another).  The last
 // workgroup to finish clears the status words and counters, so every call starts
 // from zeros (the scratch is zeroed once when allocated).  Scratch layout (u32
 // words): [0] tile counter, [1] finished counter, then nb 64-bit status words.
@@ -178,8 +180,9 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp
         return;
     }
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
-    hipLaunchKernelGGL(k_scan_1p, dim3(std::min<uint32_t>(nb, SC_MAX_GROUPS)), dim3(SC_THREADS), 0, s, in, out, n, tmp,
-                       nb, total_out, n_dev, flags ? 1 : 0);
+    const uint32_t groups = std::min<uint32_t>(nb, n_dev ? SC_MAX_GROUPS_DEV : SC_MAX_GROUPS);
+    hipLaunchKernelGGL(k_scan_1p, dim3(groups), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out, n_dev,
+                       flags ? 1 : 0);
 }
 
 // ---------------------------------------------------------------- radix sort
