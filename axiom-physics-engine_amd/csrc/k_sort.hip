@@ -56,45 +56,62 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total)
     return base + inc - v;
 }
 
-// Single-pass exclusive scan with decoupled look-back.  A workgroup takes the next
-// tile from a counter (so every lower tile is already held by a running
-// workgroup), publishes its tile total, then reads the status words of the 64
-// tiles below it at once (one per lane of its first wave), moving down 64 at a
-// time until it meets an inclusive prefix, and publishes its own.  Status word:
-// bits 62-63 state (0 none, 1 tile total, 2 inclusive prefix), bits 0-31 the
-// value.  A workgroup takes tiles until the counter passes the last tile, so a
-// bounded grid covers any length with few atomics on the two counters (atomics on
-// one address queue one after another).  The last
-// workgroup to finish clears the status words and counters, so every call starts
-// from zeros (the scratch is zeroed once when allocated).  Scratch layout (u32
-// words): [0] tile counter, [1] finished counter, then nb 64-bit status words.
+// Single-pass exclusive scan with decoupled look-back.  Workgroup b starts with
+// tile b; when there are more tiles than workgroups, the rest are taken in turn
+// from a counter (so every lower tile is already held by a running workgroup).  A
+// workgroup publishes its tile total, then reads the status words of the 64 tiles
+// below it at once (one per lane of its first wave), moving down 64 at a time until
+// it meets an inclusive prefix, and publishes its own.  Status word: bits 62-63
+// state (0 none, 1 tile total, 2 inclusive prefix), bits 0-31 the value.  The
+// launch is at most a few hundred workgroups, all resident together.  Atomics on
+// one address (or line) queue one after another, so the finish count that finds the
+// last workgroup is spread over SC_ROWS counters one line apart, each row's last
+// arrival adding to one top counter; the last workgroup to finish clears the status
+// words it used and the counters, so every call starts from zeros (the scratch is
+// zeroed once when allocated).  Scratch layout (u32 words): [0] tile counter,
+// [SC_ROW0 + 32 r] row r's finish count, [SC_TOP] rows finished, then from
+// SC_STATUS the nb 64-bit status words.
 constexpr uint64_t SC_AGG = 1ull << 62, SC_PRE = 2ull << 62;
+constexpr uint32_t SC_ROWS = 16, SC_ROW0 = 32, SC_TOP = SC_ROW0 + 32 * SC_ROWS, SC_STATUS = SC_TOP + 32;
 
 // With n_dev the element count is min(n, *n_dev), read on the device (nb, the
 // tiles the scratch holds, is sized for n).  With flags, the input is read as
-// (in[i] != 0).
-__global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                         uint32_t n_cap, uint32_t* __restrict__ tmp, uint32_t nb,
+// (in[i] != 0).  `in` may equal `out` (each thread reads its elements before it
+// writes them).
+__global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* in, uint32_t* out, uint32_t n_cap,
+                                                         uint32_t* __restrict__ tmp, uint32_t nb,
                                                          uint32_t* __restrict__ total_out,
                                                          const uint32_t* __restrict__ n_dev, int flags) {
     __shared__ uint32_t s_tile, s_prefix;
-    uint64_t* st = reinterpret_cast<uint64_t*>(tmp + 2);
+    uint64_t* st = reinterpret_cast<uint64_t*>(tmp + SC_STATUS);
     const uint32_t n = n_dev ? min(n_cap, *n_dev) : n_cap;
     const uint32_t nt = n == 0u ? 1u : (n + SC_TILE - 1u) / SC_TILE;  // tiles with elements (tile 0 always)
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&tmp[0], 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        __syncthreads();  // every thread has the ticket before thread 0 takes the next one
-        if (tile >= nt) break;
+    // 16-byte loads and stores when both arrays allow them (whole items of a full tile)
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0u;
+    uint32_t tile = blockIdx.x;
+    while (tile < nt) {
         // each thread owns SC_ITEMS consecutive elements of the tile
         const uint32_t base = tile * SC_TILE + threadIdx.x * SC_ITEMS;
+        const bool whole = vec && base + SC_ITEMS <= n;
         uint32_t v[SC_ITEMS];
+        if (whole) {
+            const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+            for (int q = 0; q < SC_ITEMS / 4; ++q) {
+                const uint4 x = p[q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+        }
         uint32_t sum = 0;
 #pragma unroll
         for (int k = 0; k < SC_ITEMS; ++k) {
-            const uint32_t i = base + k;
-            v[k] = i < n ? (flags ? (in[i] != 0u ? 1u : 0u) : in[i]) : 0u;
+            if (flags) v[k] = v[k] != 0u ? 1u : 0u;
             sum += v[k];
         }
         uint32_t tot;
@@ -140,33 +157,64 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* __restri
         }
         __syncthreads();
         uint32_t run = s_prefix + ex;
+        if (whole) {
+            uint4* p = reinterpret_cast<uint4*>(out + base);
 #pragma unroll
-        for (int k = 0; k < SC_ITEMS; ++k) {
-            const uint32_t i = base + k;
-            if (i < n) out[i] = run;
-            run += v[k];
+            for (int q = 0; q < SC_ITEMS / 4; ++q) {
+                uint4 x;
+                x.x = run;
+                run += v[4 * q];
+                x.y = run;
+                run += v[4 * q + 1];
+                x.z = run;
+                run += v[4 * q + 2];
+                x.w = run;
+                run += v[4 * q + 3];
+                p[q] = x;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < SC_ITEMS; ++k) {
+                if (base + k < n) out[base + k] = run;
+                run += v[k];
+            }
         }
         if (tile == nt - 1 && threadIdx.x == SC_THREADS - 1) {
             out[n] = s_prefix + tot;
             if (total_out) *total_out = s_prefix + tot;
         }
-        __syncthreads();  // s_prefix is rewritten by the next tile
+        if (nt <= gridDim.x) break;  // every tile was a first tile
+        if (threadIdx.x == 0) s_tile = gridDim.x + atomicAdd(&tmp[0], 1u);
+        __syncthreads();  // s_prefix is rewritten by the next tile; the ticket is in
+        tile = s_tile;
+        __syncthreads();  // every thread has the ticket before thread 0 takes the next one
     }
     __shared__ bool s_last;
-    if (threadIdx.x == 0) s_last = atomicAdd(&tmp[1], 1u) == gridDim.x - 1;
+    if (threadIdx.x == 0) {
+        // finish count: this workgroup's row, then the top counter once the row is complete
+        const uint32_t G = gridDim.x, r = blockIdx.x % SC_ROWS;
+        const uint32_t in_row = (G - r + SC_ROWS - 1u) / SC_ROWS, rows = G < SC_ROWS ? G : SC_ROWS;
+        bool last = false;
+        if (atomicAdd(&tmp[SC_ROW0 + 32u * r], 1u) == in_row - 1u) {
+            // every member of the row has added
+            __hip_atomic_store(&tmp[SC_ROW0 + 32u * r], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = atomicAdd(&tmp[SC_TOP], 1u) == rows - 1u;
+        }
+        s_last = last;
+    }
     __syncthreads();
     if (s_last) {
         // every workgroup has finished its look-back: reset the scratch for the next call
-        for (uint32_t k = threadIdx.x; k < nb; k += SC_THREADS)
+        for (uint32_t k = threadIdx.x; k < nt; k += SC_THREADS)
             __hip_atomic_store(&st[k], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (threadIdx.x == 0) {
-            tmp[0] = 0u;
-            tmp[1] = 0u;
+            __hip_atomic_store(&tmp[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&tmp[SC_TOP], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
-size_t scan_tmp_words(uint32_t n) { return 2 + 2 * (size_t)((n + SC_TILE - 1) / SC_TILE); }
+size_t scan_tmp_words(uint32_t n) { return SC_STATUS + 2 * (size_t)((n + SC_TILE - 1) / SC_TILE); }
 
 __global__ void k_zero1(uint32_t* p, uint32_t* q) {
     *p = 0;

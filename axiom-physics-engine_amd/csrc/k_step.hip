@@ -155,19 +155,42 @@ __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
 __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
     // the first body of each cell inserts it; one count per wave
     const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
     const bool head = s < n && (s == 0 || skey[s - 1] != k);
     spread_add(cnt, &SpreadRow::cells, head ? 1u : 0u);
-    if (t.mbits && s < n) t.mlist[s] = head ? (uint32_t)(k & 0x3FFFFFFFull) >> 5 : UNSET;
-    if (!head) return;
-    uint32_t c = 1;
-    while (s + c < n && skey[s + c] == k) ++c;
-    if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
+    const uint64_t heads = __ballot(head);
+    const uint32_t code = (uint32_t)(k & 0x3FFFFFFFull);
     if (t.mbits) {
-        const uint32_t code = (uint32_t)(k & 0x3FFFFFFFull);
-        atomicOr(&t.mbits[code >> 5], 1u << (code & 31u));
+        if (s < n) t.mlist[s] = head ? code >> 5 : UNSET;
+        // the wave's cells are consecutive in Morton order and share bitmap words: one
+        // atomic per distinct word, with the bits of every head in the wave that names it
+        uint64_t todo = heads;
+        while (todo) {
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+            const uint32_t word = __shfl(code >> 5, (int)lead, 64);
+            const bool mine = head && (code >> 5) == word;
+            uint32_t b = mine ? 1u << (code & 31u) : 0u;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) b |= __shfl_xor(b, o, 64);
+            if (lane == lead) atomicOr(&t.mbits[word], b);
+            todo &= ~__ballot(mine);
+        }
+    }
+    if (!head) return;
+    // the cell's count: up to the next head in the wave, else read on past the wave
+    const uint64_t above = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
+    uint32_t c;
+    if (above) {
+        c = (uint32_t)__ffsll((unsigned long long)above) - 1u - lane;
     } else {
+        c = 64u - lane;
+        while (s + c < n && skey[s + c] == k) ++c;
+        if (s + c > n) c = n - s;
+    }
+    if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
+    if (!t.mbits) {
         const uint32_t bit = cell_bit(mix64(k), t.bmask);
         atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
     }
@@ -1334,6 +1357,8 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 constexpr uint32_t IB_WORDS = 16;   // inbox words per slot
 constexpr uint32_t IB_HINT = 6;     // words 6 (side a) and 7 (side b): the 16-byte poll pair
 constexpr uint32_t OWN_WAVE = 64;   // threads per workgroup of the sweep kernel
+// residual check word: workgroups arrived in the low 16 bits, workgroups above the tolerance above them
+constexpr uint32_t CHECK_ABOVE = 1u << 16;
 constexpr uint32_t NO_VISIT = 0xFFFFFFFFu;  // version no visit count reaches
 
 __device__ __forceinline__ uint64_t vpack(float v, uint32_t ver) {
@@ -1662,11 +1687,14 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     };
     // Residual check after `sweeps` sweeps: the k_residual value over this lane's
     // slots (rows and impulses where they are staged), wave max, one atomic max per
-    // wave into stop.bits[sweeps - 1], then an arrival count in stop.arrive; once
-    // every workgroup has arrived, every wave reads the total.  Everything the waves
-    // exchange here and in the sweep goes through device-scope atomics, so the
-    // barrier needs no cache writeback or invalidation.  Returns 1 to stop, 0 to go
-    // on, -1 on an error.
+    // wave into stop.bits[sweeps - 1] (the history the host reads), and one atomic
+    // add per wave into stop.arrive[sweeps - 1] that carries both the arrival (low
+    // 16 bits) and whether this wave's maximum is above the tolerance (bit 16 up):
+    // the decision needs no second round trip for the maximum.  Once every
+    // workgroup has arrived, the launch stops when no wave was above the tolerance
+    // (the same test as maximum <= tol).  Everything the waves exchange here and in
+    // the sweep goes through device-scope atomics, so the barrier needs no cache
+    // writeback or invalidation.  Returns 1 to stop, 0 to go on, -1 on an error.
     auto check = [&](uint32_t sweeps) -> int {
         float res = 0.0f;
         uint32_t off = 0;
@@ -1705,11 +1733,12 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
         uint32_t* arr = stop.arrive + sweeps - 1u;
         if (lane == 0u) {
             atomicMax(stop.bits + sweeps - 1u, bits);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the maximum is in before the arrival
-            __hip_atomic_fetch_add(arr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const bool above = __uint_as_float(bits) > stop.tol;
+            __hip_atomic_fetch_add(arr, 1u + (above ? CHECK_ABOVE : 0u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         bool late = false;
-        while (__hip_atomic_load(arr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < G) {
+        uint32_t seen;
+        while (((seen = __hip_atomic_load(arr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & (CHECK_ABOVE - 1u)) < G) {
             if (wall_clock64() - t_start > t_limit) {
                 late = true;
                 break;
@@ -1721,9 +1750,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             return -1;
         }
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return -1;
-        const float total_res =
-            __uint_as_float(__hip_atomic_load(stop.bits + sweeps - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        return total_res <= stop.tol ? 1 : 0;
+        return seen < CHECK_ABOVE ? 1 : 0;
     };
     const bool checks = stop.bits != nullptr;
     auto sweep_loop = [&]<bool STAGED>() -> bool {
