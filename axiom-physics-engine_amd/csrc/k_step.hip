@@ -202,8 +202,13 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
 // ascending sorted position.  f(partner) is called for every emitted pair.
 // AABBs in sorted order: smin = (min, body index bits), smax = (max, 1 if dynamic),
 // so the candidates of a cell are contiguous and carry what the test needs.
-__global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* __restrict__ sval, uint32_t n) {
+// Also decides the broadphase mode of the step: sparse when there are fewer than four
+// bodies per occupied cell (the pair kernels come in a sparse and a dense build, each
+// with only the registers its mode needs, and the one not chosen returns at once).
+__global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* __restrict__ sval, uint32_t n,
+                                                     Counters* __restrict__ cnt) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) cnt->bp_sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n ? 1u : 0u;
     if (t >= n) return;
     const uint32_t i = sval[t];
     const float4 mn = B.amin[i], mx = B.amax[i];
@@ -213,10 +218,10 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
 
 // The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
 // in canonical order; the planes come with slice -1.
-template <class F>
+template <bool SPARSE, class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
-                                            const uint32_t* __restrict__ sval, const CellTable& ct, bool sparse,
-                                            uint32_t t, int dz, F&& f) {
+                                            const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t,
+                                            int dz, F&& f) {
     const uint64_t key = skey[t];
     const float4 mn = B.smin[t], mx = B.smax[t];
     const uint32_t i = __float_as_uint(mn.w);
@@ -273,7 +278,7 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
         py[a] = part1by2((uint32_t)(oky[a] ? Y : 0)) << 1;
     }
     const uint32_t pz = part1by2((uint32_t)Z) << 2;
-    if (sparse) {
+    if constexpr (SPARSE) {
         // few bodies per occupied cell: the occupancy bits (which stay in the cache) are
         // read first, then only the occupied cells are hashed and probed, in order
         uint32_t occ = 0;
@@ -327,30 +332,32 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
 // first PAIR_STASH partners, so that the emit pass only re-visits the grid for the
 // few parts with more.  Three threads per body: the probes of a body's slices are
 // in flight together.
+template <bool SPARSE>
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
                                                     const Counters* __restrict__ ctr) {
+    if ((ctr->bp_sparse != 0u) != SPARSE) return;
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
-    // fewer than four bodies per occupied cell
-    const bool sparse = (uint64_t)spread_sum(ctr, &SpreadRow::cells) * 4u > n;
     uint32_t c = 0;
     uint32_t* st = stash + (size_t)u * PAIR_STASH;
-    visit_pairs(B, sp, skey, sval, ct, sparse, t, dz, [&](uint32_t j) {
+    visit_pairs<SPARSE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
         if (c < PAIR_STASH) st[c] = j;
         ++c;
     });
     cnt[u] = c;
 }
 
+template <bool SPARSE>
 __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                    const uint32_t* __restrict__ off, uint32_t cap,
                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                    Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
+    if ((cnt->bp_sparse != 0u) != SPARSE) return;
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u == 0) {
         const uint32_t P = off[BP_PARTS * n];
@@ -360,7 +367,6 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
-    const bool sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n;
     uint32_t o = off[u];
     const uint32_t c = off[u + 1] - o;
     const uint32_t i = sval[t];
@@ -372,7 +378,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
         }
         return;
     }
-    visit_pairs(B, sp, skey, sval, ct, sparse, t, dz, [&](uint32_t j) {
+    visit_pairs<SPARSE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
         if (o < cap) {
             pa[o] = i;
             pb[o] = j;
@@ -2097,19 +2103,23 @@ void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Cou
     if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, const Counters* ctr,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, Counters* ctr,
                        hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n);
-    hipLaunchKernelGGL(k_pair_count, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt, stash,
-                       ctr);
+    hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n, ctr);
+    hipLaunchKernelGGL(k_pair_count<true>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
+                       stash, ctr);
+    hipLaunchKernelGGL(k_pair_count<false>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
+                       stash, ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
                       Counters* cnt, const uint32_t* stash, hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_pair_emit, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
-                           pa, pb, cnt, stash);
+    if (!n) return;
+    hipLaunchKernelGGL(k_pair_emit<true>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
+                       pa, pb, cnt, stash);
+    hipLaunchKernelGGL(k_pair_emit<false>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
+                       pa, pb, cnt, stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,

@@ -57,7 +57,7 @@ void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Cou
 // the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
 // the scanned offsets.
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, const Counters* ctr,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, Counters* ctr,
                        hipStream_t s);
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,

@@ -98,7 +98,7 @@ static_assert(sizeof(SpreadRow) == 128, "one line per row");
 struct Counters {
     uint32_t pairs;        // candidate pairs found (may exceed the capacity)
     uint32_t manifolds;    // manifolds found (may exceed the capacity)
-    uint32_t pad1;
+    uint32_t bp_sparse;    // broadphase mode of the step (1: fewer than four bodies per occupied cell)
     uint32_t gjk_fail;
     uint32_t epa_fail;
     uint32_t pad2;
