@@ -108,8 +108,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
 // full table ends the loop (insert returns false, lookup misses) instead of
 // spinning; tables are sized >= 2x their maximum entry count.
 __device__ __forceinline__ uint64_t ent_key(uint4 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
-__device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v) {
-    uint32_t s = (uint32_t)mix64(k) & t.mask;
+__device__ __forceinline__ bool ht_insert(CellTable t, uint64_t k, uint2 v, uint32_t s) {
     for (uint32_t probe = 0; probe <= t.mask; ++probe) {
         const unsigned long long prev =
             atomicCAS((unsigned long long*)&t.ent[s], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
@@ -152,6 +151,13 @@ __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
     }
 }
 
+// The cell table is built in two passes, so that most cells need no atomic.  Pass 1
+// (k_cell_table): the first body of each cell stores the cell's key into its home
+// slot (an untorn 8-byte store; of heads whose homes collide, one key stays), sets
+// its occupancy bit and counts it.  Pass 2 (k_cell_place): a head whose key is in
+// its home slot writes its range there; the others insert by compare-and-swap from
+// the next slot on.  Slots claimed in pass 1 never change, so every key still lies
+// after its home with no empty slot between (the linear-probing invariant).
 __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,6 +185,21 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
         }
     }
     if (!head) return;
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(&t.ent[(uint32_t)mix64(k) & t.mask]), k, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    if (!t.mbits) {
+        const uint32_t bit = cell_bit(mix64(k), t.bmask);
+        atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
+    }
+}
+__global__ void __launch_bounds__(256) k_cell_place(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+                                                    Counters* __restrict__ cnt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
+    const bool head = s < n && (s == 0 || skey[s - 1] != k);
+    const uint64_t heads = __ballot(head);
+    if (!head) return;
     // the cell's count: up to the next head in the wave, else read on past the wave
     const uint64_t above = lane == 63u ? 0ull : heads & (~0ull << (lane + 1u));
     uint32_t c;
@@ -189,10 +210,11 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
         while (s + c < n && skey[s + c] == k) ++c;
         if (s + c > n) c = n - s;
     }
-    if (!ht_insert(t, k, make_uint2(s, c))) atomicOr(&cnt->sticky, STICKY_INTERNAL);
-    if (!t.mbits) {
-        const uint32_t bit = cell_bit(mix64(k), t.bmask);
-        atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
+    const uint32_t h = (uint32_t)mix64(k) & t.mask;
+    if (*reinterpret_cast<const uint64_t*>(&t.ent[h]) == k) {
+        reinterpret_cast<uint2*>(&t.ent[h])[1] = make_uint2(s, c);
+    } else if (!ht_insert(t, k, make_uint2(s, c), (h + 1u) & t.mask)) {
+        atomicOr(&cnt->sticky, STICKY_INTERNAL);
     }
 }
 
@@ -1694,10 +1716,10 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // Residual check after `sweeps` sweeps: the k_residual value over this lane's
     // slots (rows and impulses where they are staged), wave max, one atomic max per
     // wave into stop.bits[sweeps - 1] (the history the host reads), and one atomic
-    // add per wave into stop.arrive[sweeps - 1] that carries both the arrival (low
-    // 16 bits) and whether this wave's maximum is above the tolerance (bit 16 up):
-    // the decision needs no second round trip for the maximum.  Once every
-    // workgroup has arrived, the launch stops when no wave was above the tolerance
+    // add per wave into its arrival row of the check that carries both the arrival
+    // (low 16 bits) and whether this wave's maximum is above the tolerance (bit 16
+    // up): the decision needs no second round trip for the maximum.  Once the rows
+    // sum to every workgroup, the launch stops when no wave was above the tolerance
     // (the same test as maximum <= tol).  Everything the waves exchange here and in
     // the sweep goes through device-scope atomics, so the barrier needs no cache
     // writeback or invalidation.  Returns 1 to stop, 0 to go on, -1 on an error.
@@ -1736,15 +1758,23 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             const uint32_t ob = __shfl_xor(bits, o, 64);
             bits = ob > bits ? ob : bits;
         }
-        uint32_t* arr = stop.arrive + sweeps - 1u;
+        uint32_t* arr = stop.arrive + (size_t)(sweeps - 1u) * CHECK_ROWS * CHECK_ROW_WORDS;
         if (lane == 0u) {
             atomicMax(stop.bits + sweeps - 1u, bits);
             const bool above = __uint_as_float(bits) > stop.tol;
-            __hip_atomic_fetch_add(arr, 1u + (above ? CHECK_ABOVE : 0u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(arr + (g % CHECK_ROWS) * CHECK_ROW_WORDS, 1u + (above ? CHECK_ABOVE : 0u),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         bool late = false;
         uint32_t seen;
-        while (((seen = __hip_atomic_load(arr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & (CHECK_ABOVE - 1u)) < G) {
+        for (;;) {
+            // the rows summed (arrivals stay below 2^16, so the halves add apart); uniform
+            // loads, no cross-lane steps (the sweep loop's register allocation stays as it is)
+            seen = 0u;
+#pragma unroll
+            for (uint32_t r = 0; r < CHECK_ROWS; ++r)
+                seen += __hip_atomic_load(arr + r * CHECK_ROW_WORDS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((seen & (CHECK_ABOVE - 1u)) >= G) break;
             if (wall_clock64() - t_start > t_limit) {
                 late = true;
                 break;
@@ -2100,7 +2130,9 @@ void launch_cell_clear(const CellTable& t, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
 }
 void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
+    if (!n) return;
+    hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
+    hipLaunchKernelGGL(k_cell_place, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, Counters* ctr,

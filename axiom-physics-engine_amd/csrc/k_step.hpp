@@ -93,8 +93,14 @@ void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, co
 // after the last one, the residual (the k_residual value) is written to
 // bits[sweeps - 1]; the launch stops at the first check whose residual is <= tol
 // (never when tol < 0) and stores the sweeps it ran in Counters::sweeps_run.
-// arrive[sweeps - 1] counts the workgroups whose part of that check is in; both
+// The arrivals of the check after `sweeps` sweeps are counted in CHECK_ROWS words one
+// line apart, arrive[((sweeps - 1) * CHECK_ROWS + r) * CHECK_ROW_WORDS] (workgroup g
+// adds to row g mod CHECK_ROWS: atomics on one line queue one after another); both
 // arrays start the step at zero.  bits == nullptr means no checks.
+constexpr uint32_t CHECK_ROWS = 8, CHECK_ROW_WORDS = 32;
+// Words of a residual buffer for `cap` sweeps: the residual of each sweep, then the
+// arrival rows of a check after each sweep.
+inline uint32_t res_words(uint32_t cap) { return cap * (1u + CHECK_ROWS * CHECK_ROW_WORDS); }
 struct SweepStop {
     uint32_t* bits = nullptr;
     uint32_t* arrive = nullptr;

@@ -147,7 +147,7 @@ struct axm_world_s {
     int cur = 0;
     bool prev_valid = false;
 
-    uint32_t* res_bits = nullptr;  // 2 x res_cap words: residual per sweep, then check arrivals
+    uint32_t* res_bits = nullptr;  // residual per sweep (res_cap words), then check arrival rows
     uint32_t res_cap = 0;
     Counters* cnt = nullptr;
     Counters* h_cnt = nullptr;
@@ -575,7 +575,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
                            2u * (uint32_t)w->cu_count, s);
         launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
     }
-    launch_fill_u32(w->res_bits, 0u, 2u * w->res_cap, s);  // residuals, then check arrivals
+    launch_fill_u32(w->res_bits, 0u, res_words(w->res_cap), s);  // residuals, then check arrivals
     return AXM_SUCCESS;
 }
 
@@ -742,7 +742,7 @@ axm_status ensure_res(axm_world_s* w, uint32_t iters) {
     const uint32_t need = std::max<uint32_t>(iters, w->desc.residual_tol > 0.0f ? w->desc.max_iterations : 0u) + 1;
     if (w->res_cap >= need) return AXM_SUCCESS;
     w->mem.release(w->res_bits);
-    AXM_TRY(w->mem.get(w->res_bits, 2ull * need));
+    AXM_TRY(w->mem.get(w->res_bits, res_words(need)));
     w->res_cap = need;
     w->drop_graphs();
     return AXM_SUCCESS;
