@@ -1404,8 +1404,26 @@ __device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
 __device__ __forceinline__ size_t vin_word(uint32_t s, uint32_t side, uint32_t k) {
     return (size_t)s * IB_WORDS + (k < 5u ? side * 8u + k : IB_HINT + side);
 }
-// Writes a velocity into side `side` of slot s's inbox, the hint word last.
-__device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t side, uint32_t ver, f3 v, f3 w) {
+// Writes a velocity into side `side` of slot s's inbox, the hint word last.  With
+// `wide` (inboxes within 2 GiB) words 0-1 and 2-3 go out as one 16-byte store each,
+// with the cache policy of a device-scope store: four writes instead of six.  Every
+// word carries its own version, so a reader that sees one half of a pair before the
+// other only waits.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t side, uint32_t ver, f3 v, f3 w,
+                                          bool wide) {
+    if (wide) {
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(inbox, 0, 0x7FFFFFFF, 0x00020000);
+        const uint32_t base = s * (IB_WORDS * 8u) + side * 64u;  // byte offset of word 0 of this side
+        const u32x4_t q0 = {__float_as_uint(v.x), ver, __float_as_uint(v.y), ver};
+        const u32x4_t q1 = {__float_as_uint(v.z), ver, __float_as_uint(w.x), ver};
+        // aux: sc1, as the device-scope atomic store below
+        __builtin_amdgcn_raw_buffer_store_b128(q0, rs, (int)base, 0, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(q1, rs, (int)(base + 16u), 0, 16);
+        vstore(inbox + vin_word(s, side, 4), vpack(w.y, ver));
+        vstore(inbox + vin_word(s, side, 5), vpack(w.z, ver));
+        return;
+    }
     vstore(inbox + vin_word(s, side, 0), vpack(v.x, ver));
     vstore(inbox + vin_word(s, side, 1), vpack(v.y, ver));
     vstore(inbox + vin_word(s, side, 2), vpack(v.z, ver));
@@ -1899,8 +1917,8 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                         if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
                 }
                 // pass the velocities on to each body's next visit
-                if (A.dyn) vin_store(R.inbox, it.nx.x >> 1, it.nx.x & 1u, tA + 1u, A.v, A.w);
-                if (Bd.dyn) vin_store(R.inbox, it.nx.y >> 1, it.nx.y & 1u, tB + 1u, Bd.v, Bd.w);
+                if (A.dyn) vin_store(R.inbox, it.nx.x >> 1, it.nx.x & 1u, tA + 1u, A.v, A.w, wide);
+                if (Bd.dyn) vin_store(R.inbox, it.nx.y >> 1, it.nx.y & 1u, tB + 1u, Bd.v, Bd.w, wide);
                 // next item of this lane: the next owned slot, or the first one of the next sweep
                 ++done;
                 off += stg ? np : 0u;
