@@ -1371,19 +1371,20 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 //
 // Inboxes: during the launch a body's solver velocity travels from one visit of
 // the body to the next.  Each slot has an inbox of 16 64-bit words (one 128-byte
-// line) holding, for the slot's next visit, body a's velocity in words 0..4 and
-// 6 and body b's in words 8..12 and 7, each word the value bits | the body's
-// visits completed << 32.  A visit waits until its inbox words carry the visit
+// line) holding, for the slot's next visit, body a's velocity in words 0..5 and
+// body b's in words 8..13, each word the value bits | the body's visits
+// completed << 32.  A visit waits until its inbox words carry the visit
 // numbers it needs (a 64-bit word is single-copy atomic, so matching versions
 // mean matching values), solves, and writes each dynamic body's new velocity into
 // the inbox of that body's next visit (the manifold of the next colour in the
-// body's colour mask, or its first one for the next sweep) with plain stores.
-// The last word of each side (6 for a, 7 for b, the last one a writer stores)
-// sits in one 16-byte pair: a poll reads that pair in one access as a hint and
-// loads the rest once both are in; the full load checks every version itself.
+// body's colour mask, or its first one for the next sweep) with plain stores
+// (three 16-byte ones where the inboxes lie within 2 GiB).  The last word of each
+// side (5 for a, 13 for b, in the last store a writer makes) is a hint: a poll
+// reads the two hints and loads the rest once both are in; the full load checks
+// every version itself.
 // A bounded wait sets STICKY_TIMEOUT instead of hanging.
 constexpr uint32_t IB_WORDS = 16;   // inbox words per slot
-constexpr uint32_t IB_HINT = 6;     // words 6 (side a) and 7 (side b): the 16-byte poll pair
+constexpr uint32_t IB_HINT = 5;     // word 5 of each side (polled first), stored with word 4
 constexpr uint32_t OWN_WAVE = 64;   // threads per workgroup of the sweep kernel
 // residual check word: workgroups arrived in the low 16 bits, workgroups above the tolerance above them
 constexpr uint32_t CHECK_ABOVE = 1u << 16;
@@ -1402,11 +1403,11 @@ __device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
 }
 // Word k (0..5) of side `side` of the inbox of slot s.
 __device__ __forceinline__ size_t vin_word(uint32_t s, uint32_t side, uint32_t k) {
-    return (size_t)s * IB_WORDS + (k < 5u ? side * 8u + k : IB_HINT + side);
+    return (size_t)s * IB_WORDS + side * 8u + k;
 }
 // Writes a velocity into side `side` of slot s's inbox, the hint word last.  With
-// `wide` (inboxes within 2 GiB) words 0-1 and 2-3 go out as one 16-byte store each,
-// with the cache policy of a device-scope store: four writes instead of six.  Every
+// `wide` (inboxes within 2 GiB) words 0-1, 2-3 and 4-5 go out as one 16-byte store
+// each, with the cache policy of a device-scope store: three writes instead of six.  Every
 // word carries its own version, so a reader that sees one half of a pair before the
 // other only waits.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -1420,8 +1421,8 @@ __device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t 
         // aux: sc1, as the device-scope atomic store below
         __builtin_amdgcn_raw_buffer_store_b128(q0, rs, (int)base, 0, 16);
         __builtin_amdgcn_raw_buffer_store_b128(q1, rs, (int)(base + 16u), 0, 16);
-        vstore(inbox + vin_word(s, side, 4), vpack(w.y, ver));
-        vstore(inbox + vin_word(s, side, 5), vpack(w.z, ver));
+        const u32x4_t q2 = {__float_as_uint(w.y), ver, __float_as_uint(w.z), ver};
+        __builtin_amdgcn_raw_buffer_store_b128(q2, rs, (int)(base + 32u), 0, 16);
         return;
     }
     vstore(inbox + vin_word(s, side, 0), vpack(v.x, ver));
@@ -1431,8 +1432,6 @@ __device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t 
     vstore(inbox + vin_word(s, side, 4), vpack(w.y, ver));
     vstore(inbox + vin_word(s, side, 5), vpack(w.z, ver));
 }
-// The two hint words of slot s (versions of sides a and b) in one coherent 16-byte
-// access; a torn read only delays or repeats the full load, which checks every word.
 // Words k and k + 1 of slot s's inbox in one coherent 16-byte access, as (value,
 // version) dword pairs; every word is checked against its own version, so a load
 // that sees one word before the other only delays the visit.  Inboxes within 2 GiB.
@@ -1478,17 +1477,16 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
         wb = xyz(B.sw[h.y]);
     }
     R.next[s] = nx;
-    // the whole inbox (words 0..13, (value, version) pairs as laid out by vin_word) in
-    // seven 16-byte stores; the sweep kernel, a later launch, sees them after the
+    // the whole inbox (words 0-5 and 8-13, (value, version) pairs as laid out by
+    // vin_word) in six 16-byte stores; the sweep kernel, a later launch, sees them after the
     // launch boundary
     uint4* q = reinterpret_cast<uint4*>(R.inbox + (size_t)s * IB_WORDS);
     q[0] = make_uint4(fbits(va.x), ra, fbits(va.y), ra);
     q[1] = make_uint4(fbits(va.z), ra, fbits(wa.x), ra);
-    q[2] = make_uint4(fbits(wa.y), ra, 0u, NO_VISIT);
-    q[3] = make_uint4(fbits(wa.z), ra, fbits(wb.z), rb);
+    q[2] = make_uint4(fbits(wa.y), ra, fbits(wa.z), ra);
     q[4] = make_uint4(fbits(vb.x), rb, fbits(vb.y), rb);
     q[5] = make_uint4(fbits(vb.z), rb, fbits(wb.x), rb);
-    q[6] = make_uint4(fbits(wb.y), rb, 0u, NO_VISIT);
+    q[6] = make_uint4(fbits(wb.y), rb, fbits(wb.z), rb);
     }
 }
 // After a launch: each dynamic body's final velocity is in the inbox of its first
@@ -1698,7 +1696,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // the row arrays.  Two instances, so that the first never waits on a memory
     // load other than its poll (a wait for a load also waits for the stores before it).
     const float mu = sp.friction;
-    // tune bits 4-6: poll back-off level; bit 1: poll the hint pair before the full load
+    // tune bits 4-6: poll back-off level; bit 1: poll the two hint words before the full load
     // (inboxes within 2 GiB); bit 2: adaptive back-off
     const uint32_t nap = (tune >> 4) & 7u;
     const bool wide = (uint64_t)cap * IB_WORDS * 8u <= 0x7FFFFFFFull;  // 16-byte inbox loads
@@ -1848,26 +1846,24 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
             // poll = load: all words of both bodies in this slot's inbox (one line), issued together
             bool ok = false;
             bool ready = pending;
-            uint4 h67 = make_uint4(0u, 0u, 0u, 0u);
             if (pending && hint) {
-                // the hint pair first (one access); the full load below runs only for lanes
+                // the two hint words first; the full load below runs only for lanes
                 // whose hint words are in
-                h67 = vin_quad(R.inbox, s, IB_HINT);
-                ready = ((h67.y == tA) | !A.dyn) & ((h67.w == tB) | !Bd.dyn);
+                const uint64_t ha = vload(R.inbox + vin_word(s, 0, IB_HINT)), hb = vload(R.inbox + vin_word(s, 1, IB_HINT));
+                ready = ((vver(ha) == tA) | !A.dyn) & ((vver(hb) == tB) | !Bd.dyn);
             }
             if (ready) {
                 // (value, version) of the six words of each side
                 uint2 a[6], b[6];
                 if (wide) {
-                    // 16-byte loads; the hint pair, when polled, holds the sixth word of each side
-                    if (!hint) h67 = vin_quad(R.inbox, s, IB_HINT);
+                    // 16-byte loads, three per side
                     const uint4 q0 = vin_quad(R.inbox, s, 0), q1 = vin_quad(R.inbox, s, 2),
                                 q2 = vin_quad(R.inbox, s, 4), q4 = vin_quad(R.inbox, s, 8),
                                 q5 = vin_quad(R.inbox, s, 10), q6 = vin_quad(R.inbox, s, 12);
                     a[0] = make_uint2(q0.x, q0.y), a[1] = make_uint2(q0.z, q0.w), a[2] = make_uint2(q1.x, q1.y);
-                    a[3] = make_uint2(q1.z, q1.w), a[4] = make_uint2(q2.x, q2.y), a[5] = make_uint2(h67.x, h67.y);
+                    a[3] = make_uint2(q1.z, q1.w), a[4] = make_uint2(q2.x, q2.y), a[5] = make_uint2(q2.z, q2.w);
                     b[0] = make_uint2(q4.x, q4.y), b[1] = make_uint2(q4.z, q4.w), b[2] = make_uint2(q5.x, q5.y);
-                    b[3] = make_uint2(q5.z, q5.w), b[4] = make_uint2(q6.x, q6.y), b[5] = make_uint2(h67.z, h67.w);
+                    b[3] = make_uint2(q5.z, q5.w), b[4] = make_uint2(q6.x, q6.y), b[5] = make_uint2(q6.z, q6.w);
                 } else {
                     const uint64_t* ib = R.inbox;
 #pragma unroll
