@@ -43,7 +43,7 @@ class WorldDesc(C.Structure):
         ("rest_threshold", C.c_float), ("baumgarte", C.c_float), ("slop", C.c_float), ("max_bias", C.c_float),
         ("margin", C.c_float), ("lin_damp", C.c_float), ("ang_damp", C.c_float), ("cell_size", C.c_float),
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
-        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("max_bodies", C.c_uint32),
+        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float), ("max_bodies", C.c_uint32),
         ("max_pairs", C.c_uint32), ("max_manifolds", C.c_uint32), ("auto_grow", C.c_uint32),
         ("residual_tol", C.c_float), ("residual_every", C.c_uint32), ("max_iterations", C.c_uint32),
         ("device", C.c_int32), ("solver_mode", C.c_uint32),
@@ -78,6 +78,14 @@ class KernelProfile(C.Structure):
                 ("pgs_bytes", C.c_double), ("sweeps", C.c_uint64)]
 
 
+class WorldTotals(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("steps_at_tol", C.c_uint64), ("sweeps", C.c_uint64),
+                ("contact_sweeps", C.c_uint64), ("max_sweeps", C.c_uint32), ("pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
 class SweepConfig(C.Structure):
     _fields_ = [("groups", C.c_uint32), ("lds_kib", C.c_uint32), ("policy", C.c_uint32), ("sweep_ms", C.c_double)]
 
@@ -88,7 +96,7 @@ class SweepConfig(C.Structure):
 # Exported C-ABI symbols (the header's declarations).
 EXPORTS = [
     "axm_device_count", "axm_status_string", "axm_world_desc_default", "axm_world_create", "axm_world_destroy", "axm_world_set_bodies",
-    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile",
+    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
@@ -125,6 +133,7 @@ def lib():
     L.axm_world_run.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(StepStats)]
     L.axm_world_sync.argtypes = [P]
     L.axm_world_profile.argtypes = [P, C.c_int, C.POINTER(KernelProfile)]
+    L.axm_world_get_totals.argtypes = [P, C.POINTER(WorldTotals), C.c_int]
     L.axm_world_get_pairs.argtypes = [P, u32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_manifolds.argtypes = [P, u32p, u32p, u32p, f32p, f32p, f32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_colours.argtypes = [P, u32p, u32p, C.c_uint32, u32p, u32p]
@@ -378,6 +387,13 @@ class World:
         p = KernelProfile()
         _check(self._L.axm_world_profile(self._h, 1 if enable else 0, C.byref(p)), "axm_world_profile")
         return p
+
+    def totals(self, reset=False):
+        """Device-counted totals since the last reset (steps, steps at tolerance, sweeps, contact x sweeps,
+        max sweeps of one step)."""
+        t = WorldTotals()
+        _check(self._L.axm_world_get_totals(self._h, C.byref(t), 1 if reset else 0), "axm_world_get_totals")
+        return t
 
     def set_mlp(self, mlp):
         """Attaches a warm-start network (used when desc.warm_start == WARM_MLP); None detaches."""

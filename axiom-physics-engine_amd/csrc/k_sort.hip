@@ -81,7 +81,8 @@ constexpr uint32_t SC_ROWS = 16, SC_ROW0 = 32, SC_TOP = SC_ROW0 + 32 * SC_ROWS, 
 __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* in, uint32_t* out, uint32_t n_cap,
                                                          uint32_t* __restrict__ tmp, uint32_t nb,
                                                          uint32_t* __restrict__ total_out,
-                                                         const uint32_t* __restrict__ n_dev, int flags) {
+                                                         const uint32_t* __restrict__ n_dev, int flags,
+                                                         uint32_t* __restrict__ err) {
     __shared__ uint32_t s_tile, s_prefix;
     uint64_t* st = reinterpret_cast<uint64_t*>(tmp + SC_STATUS);
     const uint32_t n = n_dev ? min(n_cap, *n_dev) : n_cap;
@@ -139,7 +140,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_1p(const uint32_t* in, uint
                     const uint32_t first = pre ? (uint32_t)__ffsll((unsigned long long)pre) - 1u : 64u;
                     const uint64_t upto = first >= 63u ? ~0ull : (2ull << first) - 1ull;
                     if (none & upto) {
-                        if (wall_clock64() - t0 > 200000000ull) break;  // 2 s
+                        if (wall_clock64() - t0 > 200000000ull) {  // 2 s: report, do not hang
+                            if (lane == 0u && err) atomicOr(err, STICKY_TIMEOUT);
+                            break;
+                        }
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
@@ -222,7 +226,7 @@ __global__ void k_zero1(uint32_t* p, uint32_t* q) {
 }
 
 void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s, uint32_t* total_out,
-                    const uint32_t* n_dev, bool flags) {
+                    const uint32_t* n_dev, bool flags, uint32_t* err) {
     if (n == 0) {
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, out, total_out);
         return;
@@ -230,7 +234,7 @@ void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp
     const uint32_t nb = (n + SC_TILE - 1) / SC_TILE;
     const uint32_t groups = std::min<uint32_t>(nb, n_dev ? SC_MAX_GROUPS_DEV : SC_MAX_GROUPS);
     hipLaunchKernelGGL(k_scan_1p, dim3(groups), dim3(SC_THREADS), 0, s, in, out, n, tmp, nb, total_out, n_dev,
-                       flags ? 1 : 0);
+                       flags ? 1 : 0, err);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -328,7 +332,7 @@ size_t radix_tmp_words(uint32_t n) {
 
 template <class K>
 static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                           uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
+                           uint32_t* tmp, hipStream_t s, const uint32_t* n_dev, uint32_t* err) {
     if (n == 0) return 0;
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* hist = tmp;
@@ -344,7 +348,7 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     int passes = 0;
     for (int shift = 0; shift < end_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, n, n_dev, shift, hist, len);
-        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s, nullptr, len);
+        scan_exclusive(hist, offs, 256u * nb, scan_tmp, s, nullptr, len, false, err);
         hipLaunchKernelGGL(k_rs_scatter<K>, dim3(nb), dim3(RS_THREADS), 0, s, ki, vi, ko, vo, n, n_dev, shift, offs);
         K* tk = ki; ki = ko; ko = tk;
         uint32_t* tv = vi; vi = vo; vo = tv;
@@ -354,13 +358,13 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
 }
 
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
-    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev, uint32_t* err) {
+    return radix_sort_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev, err);
 }
 
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev) {
-    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev, uint32_t* err) {
+    return radix_sort_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, end_bit, tmp, s, n_dev, err);
 }
 
 }  // namespace axm

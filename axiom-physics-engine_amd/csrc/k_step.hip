@@ -594,28 +594,41 @@ __device__ __forceinline__ uint32_t local_id(const Bodies& B, uint32_t b) {
 // workgroup (agent-scope acquire).  One writeback per workgroup instead of one per
 // wave: the writeback of an XCD's L2 is the expensive part of a fence.  A bounded
 // wait sets STICKY_TIMEOUT instead of hanging.
-__device__ __forceinline__ void grid_sync(GridBar* bar, uint32_t nblocks, Counters* cnt) {
+// Grid barrier over `nblocks` resident workgroups.  The last workgroup to arrive
+// copies *snap (a counter the workgroups update before arriving) into the barrier
+// before releasing it, and every workgroup returns that copy: all of them see the
+// value as it stood when the whole grid had arrived, even if a fast workgroup
+// already updates the counter again in the next round.
+__device__ __forceinline__ uint32_t grid_sync(GridBar* bar, uint32_t nblocks, Counters* cnt, const uint32_t* snap) {
+    __shared__ uint32_t s_snap;
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
+            const uint32_t v = __hip_atomic_load(snap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar->pad[0], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            s_snap = v;
         } else {
             const uint64_t t0 = wall_clock64();  // 100 MHz constant clock
+            bool late = false;
             while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
                 __builtin_amdgcn_s_sleep(2);
                 if (wall_clock64() - t0 > 200000000ull) {  // 2 s
                     atomicOr(&cnt->sticky, STICKY_TIMEOUT);
+                    late = true;
                     break;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_snap = late ? 0u : __hip_atomic_load(&bar->pad[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
+    return s_snap;
 }
 
 // Jones-Plassmann colouring in one persistent launch.  Round r: every
@@ -773,21 +786,22 @@ __global__ void __launch_bounds__(1024) k_colour_small(Bodies B, const uint2* __
                                                        uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask,
                                                        const uint32_t* __restrict__ list, Counters* cnt,
                                                        uint32_t max_rounds, uint32_t small) {
-    // per-round totals, double-buffered: round r's are read after its barrier while
-    // round r + 1's are cleared
-    __shared__ uint32_t s_won[2], s_cmax[2];
+    // per-round totals in three slots: round r adds into slot r % 3 and reads it after
+    // barrier r; thread 0 then clears slot (r + 2) % 3, which nobody touches before
+    // round r + 2 (it starts after barrier r + 1, which thread 0 reaches after the clear)
+    __shared__ uint32_t s_won[3], s_cmax[3];
     const uint32_t L = cnt->remaining;
     if (L == 0u || L > small) return;  // uniform
-    if (threadIdx.x < 2u) {
+    if (threadIdx.x < 3u) {
         s_won[threadIdx.x] = 0u;
         s_cmax[threadIdx.x] = 0u;
     }
     __syncthreads();
     uint32_t left = L;
     for (uint32_t r = 0;; ++r) {
-        const uint32_t q = r & 1u;
-        const uint64_t* cur = q ? bmax1 : bmax0;
-        uint64_t* nxt = q ? bmax0 : bmax1;
+        const uint32_t q = r % 3u;
+        const uint64_t* cur = (r & 1u) ? bmax1 : bmax0;
+        uint64_t* nxt = (r & 1u) ? bmax0 : bmax1;
         uint32_t won = 0, cmax = 0;
         for (uint32_t i = threadIdx.x; i < L; i += blockDim.x)
             jp_visit<true>(B, m_ab, col, prio, cur, nxt, bmask, list[i], r, won, cmax);
@@ -803,8 +817,8 @@ __global__ void __launch_bounds__(1024) k_colour_small(Bodies B, const uint2* __
                 atomicMax(&cnt->max_colour, s_cmax[q]);
                 atomicMax(&cnt->last_round, r + 1u);
             }
-            s_won[q ^ 1u] = 0u;
-            s_cmax[q ^ 1u] = 0u;
+            s_won[(r + 2u) % 3u] = 0u;
+            s_cmax[(r + 2u) % 3u] = 0u;
         }
         left -= w;
         if (left == 0u) break;
@@ -838,8 +852,8 @@ __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restric
             atomicMax(&cnt->max_colour, cmax);
             atomicMax(&cnt->last_round, r + 1u);
         }
-        grid_sync(bar, nb, cnt);
-        const uint32_t left = __hip_atomic_load(&cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the count left after round r, as it stood when every workgroup had finished round r
+        const uint32_t left = grid_sync(bar, nb, cnt, &cnt->remaining);
         const uint32_t bad = __hip_atomic_load(&cnt->sticky, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (left == 0 || (bad & STICKY_TIMEOUT)) break;
         if (r + 1 >= max_rounds) {
@@ -983,7 +997,7 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
             jb[d] = cross(rB, dirs[d]);
             jia[d] = smul(A.I, ja[d]);
             jib[d] = smul(Bd.I, jb[d]);
-            kk[d] = A.im + Bd.im + dot(ja[d], jia[d]) + dot(jb[d], jib[d]);
+            kk[d] = (A.im + Bd.im + dot(ja[d], jia[d]) + dot(jb[d], jib[d])) * sp.soft1;
         }
         const float mN = kk[0] > 0.0f ? 1.0f / kk[0] : 0.0f;
         const float mT1 = kk[1] > 0.0f ? 1.0f / kk[1] : 0.0f;
@@ -1180,7 +1194,7 @@ __device__ __forceinline__ Dir load_dir(const float4* J, size_t rs, int dir, boo
 // rows of every point, then normal rows.  Point k's rows start at J + k * ps, rows
 // `rs` apart; L[k] holds the point impulses in and out.
 __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps, uint32_t np, bool plane, float mu,
-                                           bool first, SB& A, SB& Bd, float4 L[4]) {
+                                           float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w1, w2, w3, w4;
     if (first) {
 #pragma unroll
@@ -1203,7 +1217,7 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
         {
             float mT1;
             const Dir D1 = load_dir(P, rs, 1, plane, mT1, w1, w2, w3, w4);
-            float nl = fmaf(-mT1, jvel(D1, A, Bd), L[k].y);
+            float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
             L[k].y = nl;
@@ -1212,7 +1226,7 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
         {
             float mT2;
             const Dir D2 = load_dir(P, rs, 2, plane, mT2, w1, w2, w3, w4);
-            float nl = fmaf(-mT2, jvel(D2, A, Bd), L[k].z);
+            float nl = fmaf(-mT2, jvel(D2, A, Bd), keep * L[k].z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].z;
             L[k].z = nl;
@@ -1225,7 +1239,7 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
         const float4* P = J + (size_t)k * ps;
         float mN, x1, x2, tg, x4;
         const Dir Dn = load_dir(P, rs, 0, plane, mN, x1, x2, tg, x4);
-        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L[k].x);
+        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L[k].x);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L[k].x;
         L[k].x = nl;
@@ -1265,7 +1279,7 @@ __device__ __forceinline__ void load_rows(const SolverRows& R, uint32_t s, uint3
     }
 }
 __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_t np, bool plane, float mu,
-                                           bool first, SB& A, SB& Bd, float4 L[4]) {
+                                           float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w3;
     if (first) {
 #pragma unroll
@@ -1286,7 +1300,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
         {
             float mT1;
             const Dir D1 = dir_regs(Jr[k], 1, plane, mT1, w3);
-            float nl = fmaf(-mT1, jvel(D1, A, Bd), L[k].y);
+            float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
             L[k].y = nl;
@@ -1295,7 +1309,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
         {
             float mT2;
             const Dir D2 = dir_regs(Jr[k], 2, plane, mT2, w3);
-            float nl = fmaf(-mT2, jvel(D2, A, Bd), L[k].z);
+            float nl = fmaf(-mT2, jvel(D2, A, Bd), keep * L[k].z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].z;
             L[k].z = nl;
@@ -1307,7 +1321,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
         if (k >= np) break;
         float mN, tg;
         const Dir Dn = dir_regs(Jr[k], 0, plane, mN, tg);
-        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L[k].x);
+        float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L[k].x);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L[k].x;
         L[k].x = nl;
@@ -1317,8 +1331,8 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
 // The same with the rows of solve slot s in the SoA row arrays (row r of point k at
 // J[(k * JROWS + r) * cap + s]).
 __device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
-                                           float mu, bool first, SB& A, SB& Bd, float4 L[4]) {
-    solve_rows(R.J + s, cap, (size_t)JROWS * cap, np, plane, mu, first, A, Bd, L);
+                                           float mu, float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
+    solve_rows(R.J + s, cap, (size_t)JROWS * cap, np, plane, mu, keep, first, A, Bd, L);
 }
 
 // One visit of the manifold in solve slot s with plain loads and stores (one
@@ -1332,7 +1346,7 @@ __device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp
     if (plane) zero_sb(Bd); else load_vel<false>(B, b, Bd);
     float4 L[4];
     for (uint32_t k = 0; k < np; ++k) L[k] = R.lam[(size_t)k * cap + s];
-    solve_core(R, s, cap, np, plane, sp.friction, first, A, Bd, L);
+    solve_core(R, s, cap, np, plane, sp.friction, sp.keep, first, A, Bd, L);
     for (uint32_t k = 0; k < np; ++k) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
     if (A.dyn) {
         B.sv[a] = f4(A.v, A.im);
@@ -1517,20 +1531,20 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
 // times the row's effective mass inverse) that one more visit would make along the
 // normal or either friction direction, from the current velocities and impulses L.
 // J points at the point's first Jacobian row, rs is the row stride.
-__device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool plane, float mu, const SB& A,
-                                                const SB& Bd, const float4 L) {
+__device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool plane, float mu, float keep,
+                                                const SB& A, const SB& Bd, const float4 L) {
     float mN, kN, tg, mT1, kT1, mT2, kT2, x2, x3, x4;
     const Dir Dn = load_dir(J, rs, 0, plane, mN, kN, x2, tg, x4);
     const Dir D1 = load_dir(J, rs, 1, plane, mT1, kT1, x2, x3, x4);
     const Dir D2 = load_dir(J, rs, 2, plane, mT2, kT2, x2, x3, x4);
-    float nl = fmaf(mN, tg - jvel(Dn, A, Bd), L.x);
+    float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
     nl = nl > 0.0f ? nl : 0.0f;
     float e = fabsf(nl - L.x) * kN;
     const float maxf = mu * L.x;
-    float q1 = fmaf(-mT1, jvel(D1, A, Bd), L.y);
+    float q1 = fmaf(-mT1, jvel(D1, A, Bd), keep * L.y);
     q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
     const float e1 = fabsf(q1 - L.y) * kT1;
-    float q2 = fmaf(-mT2, jvel(D2, A, Bd), L.z);
+    float q2 = fmaf(-mT2, jvel(D2, A, Bd), keep * L.z);
     q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
     const float e2 = fabsf(q2 - L.z) * kT2;
     if (e1 > e) e = e1;
@@ -1696,6 +1710,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // the row arrays.  Two instances, so that the first never waits on a memory
     // load other than its poll (a wait for a load also waits for the stores before it).
     const float mu = sp.friction;
+    const float keep = sp.keep;
     // tune bits 4-6: poll back-off level; bit 1: poll the two hint words before the full load
     // (inboxes within 2 GiB); bit 2: adaptive back-off
     const uint32_t nap = (tune >> 4) & 7u;
@@ -1764,7 +1779,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 const float4* J = staged ? mine + (size_t)(off + k) * OWN_WAVE : R.J + (size_t)k * JROWS * cap + it.s;
                 const size_t jr = staged ? rs : (size_t)cap;
                 const float4 L = staged ? J[JROWS * jr] : R.lam[(size_t)k * cap + it.s];
-                const float e = point_residual(J, jr, plane, mu, A, Bd, L);
+                const float e = point_residual(J, jr, plane, mu, keep, A, Bd, L);
                 if (e > res) res = e;
             }
             off += staged ? np : 0u;
@@ -1897,7 +1912,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) L[k] = P[JROWS * rs + (size_t)k * OWN_WAVE];
-                    solve_rows(P, rs, OWN_WAVE, np, plane, mu, first, A, Bd, L);
+                    solve_rows(P, rs, OWN_WAVE, np, plane, mu, keep, first, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) P[JROWS * rs + (size_t)k * OWN_WAVE] = L[k];
@@ -1907,7 +1922,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) L[k] = R.lam[(size_t)k * cap + s];
-                    solve_regs(Jr, np, plane, mu, first, A, Bd, L);
+                    solve_regs(Jr, np, plane, mu, keep, first, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
@@ -1936,8 +1951,10 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 const int r = check(chunk_end);
                 if (r < 0) return false;
                 if (r > 0 || chunk_end == nsweeps) {
-                    if (g == 0u && lane == 0u)
+                    if (g == 0u && lane == 0u) {
                         __hip_atomic_store(&cnt->sweeps_run, chunk_end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(&cnt->tol_hit, r > 0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
                     return true;
                 }
                 chunk_end = min(chunk_end + stop.every, nsweeps);
@@ -2004,7 +2021,7 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
         if (plane) zero_sb(Bd); else load_vel<false>(B, h.y, Bd);
         const float mu = sp.friction;
         for (uint32_t k = 0; k < h.z; ++k) {
-            const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, mu, A, Bd,
+            const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, mu, sp.keep, A, Bd,
                                            R.lam[(size_t)k * cap + s]);
             if (e > res) res = e;
         }
@@ -2092,6 +2109,11 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
         const uint32_t sw = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
         cnt->pgs_bytes_total += spread_sum(cnt, &SpreadRow::pgs_bytes) * (unsigned long long)sw;
         cnt->sweeps_total += sw;
+        cnt->acc_steps += 1u;
+        cnt->acc_reached += cnt->tol_hit;
+        cnt->acc_max_sweeps = max(cnt->acc_max_sweeps, sw);
+        cnt->acc_sweeps += sw;
+        cnt->acc_contact_sweeps += (unsigned long long)spread_sum(cnt, &SpreadRow::contacts) * sw;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];

@@ -27,6 +27,7 @@ struct StepParams {
     float gx, gy, gz;
     float dt, inv_dt;
     float friction, restitution, rest_threshold;
+    float soft1, keep;  // 1 + softness and its inverse (regularised rows)
     float baumgarte, slop, max_bias, margin;
     float lin_keep, ang_keep;
     float inv_cell;
@@ -111,13 +112,20 @@ struct Counters {
     uint32_t m_eff;        // min(manifolds, manifold capacity)
     uint32_t pad3;
     uint32_t sweeps_run;   // sweeps of a launch with residual checks (set when it ends)
-    uint32_t pad0;
+    uint32_t tol_hit;      // 1: the step's sweeps stopped because the residual met the tolerance
     unsigned long long pad4;
     // ---- kept across steps
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
     uint32_t steps_done;   // steps whose integration ran
     unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
     unsigned long long sweeps_total;     // sweeps of all completed steps
+    // ---- run totals since the last axm_world_get_totals reset (steps whose integration ran)
+    uint32_t acc_steps;
+    uint32_t acc_reached;  // steps that stopped at the residual tolerance
+    uint32_t acc_max_sweeps;
+    uint32_t acc_pad;
+    unsigned long long acc_sweeps;
+    unsigned long long acc_contact_sweeps;  // sum of contact points x sweeps
     // ---- spread sums, cleared per step
     alignas(128) SpreadRow sp[SPREAD];
 };
@@ -147,16 +155,19 @@ struct GridBar {
 size_t scan_tmp_words(uint32_t n);
 // out[0..n] = exclusive prefix sums, out[n] = total (also written to *total_out when given).
 // With n_dev the count is min(n, *n_dev), read on the device (the grid is sized for n);
-// with flags the input is read as (in[i] != 0).
+// with flags the input is read as (in[i] != 0).  With err, a look-back that exceeds its
+// bounded wait (only a broken invariant can cause it) ORs STICKY_TIMEOUT into *err, so
+// the step that used the partial prefix fails instead of running on it.
 void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s,
-                    uint32_t* total_out = nullptr, const uint32_t* n_dev = nullptr, bool flags = false);
+                    uint32_t* total_out = nullptr, const uint32_t* n_dev = nullptr, bool flags = false,
+                    uint32_t* err = nullptr);
 size_t radix_tmp_words(uint32_t n);
 // Returns 1 when the sorted result lives in the *_alt buffers.  `n` is the
 // capacity the grid is sized for; with n_dev the element count is read on the
 // device (min(*n_dev, n)), so the launch needs no host synchronisation.
 int radix_sort_u64(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr, uint32_t* err = nullptr);
 int radix_sort_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n, int end_bit,
-                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr);
+                   uint32_t* tmp, hipStream_t s, const uint32_t* n_dev = nullptr, uint32_t* err = nullptr);
 
 }  // namespace axm

@@ -228,6 +228,7 @@ void axm_world_desc_default(axm_world_desc* d) {
     d->planes[0][3] = 0.0f;
     d->warm_start = AXM_WARM_CACHE;
     d->record_residual = 0;
+    d->softness = 0.1f;
     d->auto_grow = 1;
     d->residual_tol = 0.0f;
     d->residual_every = 1;

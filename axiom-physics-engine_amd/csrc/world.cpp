@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -440,6 +441,8 @@ StepParams make_params(const axm_world_s* w, float dt) {
     sp.friction = d.friction;
     sp.restitution = d.restitution;
     sp.rest_threshold = d.rest_threshold;
+    sp.soft1 = 1.0f + d.softness;
+    sp.keep = 1.0f / sp.soft1;
     sp.baumgarte = d.baumgarte;
     sp.slop = d.slop;
     sp.max_bias = d.max_bias;
@@ -507,13 +510,15 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     hipStream_t s = w->stream;
     launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
-    const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s);
+    const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s, nullptr,
+                                   &w->cnt->sticky);
     const uint64_t* skey = alt ? w->key_alt : w->key;
     const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_cell_clear(w->ct, s);
     launch_cell_table(skey, n, w->ct, w->cnt, s);
     launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, w->cnt, s);
-    scan_exclusive(w->pair_cnt, w->pair_off, BP_PARTS * n, w->scan_tmp, s, &w->cnt->pairs);
+    scan_exclusive(w->pair_cnt, w->pair_off, BP_PARTS * n, w->scan_tmp, s, &w->cnt->pairs, nullptr, false,
+                   &w->cnt->sticky);
     launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
                      s);
 }
@@ -524,7 +529,8 @@ void enqueue_narrow(axm_world_s* w, const StepParams& sp) {
     launch_narrow(w->B, sp, w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->epa_list, w->p_kind, w->pkinds,
                   w->p_order, 4u * (uint32_t)w->cu_count, w->cnt, s);
     // manifold offsets: scan of (points > 0) over the pairs found this step
-    scan_exclusive(w->p_np, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds, &w->cnt->p_eff, true);
+    scan_exclusive(w->p_np, w->man_off, w->cap_pairs, w->pscan_tmp, s, &w->cnt->manifolds, &w->cnt->p_eff, true,
+                   &w->cnt->sticky);
     launch_compact(w->pa, w->pb, w->cap_pairs, w->p_np, w->p_pt, w->p_nf, w->man_off, w->cap_man, w->m_ab, w->m_np,
                    w->c_pt, w->c_nf, w->cnt, s);
 }
@@ -562,7 +568,8 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     launch_colour(w->B, w->m_ab, w->cap_man, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->clist,
                   w->bar, w->cnt, w->colour_blocks, pca, s);
     launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
-    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff);
+    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff,
+                   &w->cnt->sticky);
     const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers
     const uint32_t* order = w->cval_alt;
     launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
@@ -663,7 +670,10 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
                 AXM_HIP(hipStreamSynchronize(s));
                 float res;
                 std::memcpy(&res, &w->h_cnt->residual_bits, 4);
-                if (res <= w->desc.residual_tol) break;
+                if (res <= w->desc.residual_tol) {
+                    launch_fill_u32(&w->cnt->tol_hit, 1u, 1u, s);
+                    break;
+                }
             }
         }
     }
@@ -1183,6 +1193,28 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         axm_step_stats S{};
         fill_stats(w, S);
         *last = S;
+    }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_get_totals(axm_world w, axm_world_totals* out, int reset) {
+    if (!w || !out) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const Counters& h = *w->h_cnt;
+    out->steps = h.acc_steps;
+    out->steps_at_tol = h.acc_reached;
+    out->sweeps = h.acc_sweeps;
+    out->contact_sweeps = h.acc_contact_sweeps;
+    out->max_sweeps = h.acc_max_sweeps;
+    out->pad = 0;
+    if (reset) {
+        // the accumulators sit between steps_done's block and the spread rows
+        const size_t off = offsetof(Counters, acc_steps);
+        const size_t len = offsetof(Counters, acc_contact_sweeps) + sizeof(unsigned long long) - off;
+        launch_fill_u32(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w->cnt) + off), 0u,
+                        (uint32_t)(len / 4), w->stream);
+        AXM_HIP(hipStreamSynchronize(w->stream));
     }
     return AXM_SUCCESS;
 }

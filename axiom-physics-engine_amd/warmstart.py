@@ -123,3 +123,11 @@ def train_on_config0(seeds=16, steps=240, sweeps=60, epochs=10, batch=2048, lr=1
     X, Y = np.concatenate(xs), np.concatenate(ys)
     net, losses = train(X, Y, epochs=epochs, batch=batch, lr=lr)
     return net, losses, len(X)
+
+
+TRAIN_DATA = "16 seeded 1k-body scenes x 240 steps, plain solver at 60 sweeps"
+
+
+def train_default():
+    """The training run the benchmark uses."""
+    return train_on_config0()

@@ -1,24 +1,30 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the rigid-body step on one MI355X.
 
-Workload (default): the dense stacking pile — 100,000 bodies (thirds spheres /
-boxes / capsules) on a 1.5 m lattice over a 20 x 20 m footprint inside four wall
-planes, friction 0.6, dt = 1/120 s, 20 solver sweeps.  A "step" is one full
-physics step (broadphase, narrowphase, colouring, solve, integration) of the
-whole scene with its state resident in HBM.
+Headline workload (default): the sparse 1M-body scene — 1,000,000 spheres and
+boxes (50/50, sizes U(0.25, 0.75) m) on a jittered 4 m lattice filling a 400 m
+cube above the ground plane, about 2 % of bodies starting in contact, dt =
+1/120 s — with every step's sweeps run until the contact residual is <= 1e-4
+(checked inside the sweep launch after every sweep, at most 300 sweeps).  A
+"step" is one full physics step (broadphase, narrowphase, colouring, solve to
+the tolerance, integration) of the whole scene with its state resident in HBM.
+The scene runs its 120 steps: 10 warm-up steps, then 110 timed steps replayed
+from hipGraphs.
 
-Prints one JSON line: body-steps/s, the roofline of the dominant kernel (the
-persistent sweep kernel k_pgs_own: algorithmic bytes / summed event-timed
-duration of its launches during the timed region) and the CPU oracle timed on a
-bounded sample of the same workload on the host.  Two further sections: the
-1M-body scene iterated to contact residual <= 1e-4 every step, the residual
-checked after every sweep and after every fourth (with the oracle at the same
-sweep count beside it), and the plain vs learned warm start at that
-residual on one 1k-body scene and on 4096 batched worlds.
+Prints one JSON line: body-steps/s at the tolerance and the fraction of steps
+that reached it, the roofline of the dominant kernel (the persistent sweep
+kernel k_pgs_own: 240 B per contact point per sweep, divided by its event-timed
+launch durations) and of the whole step (304 N + 104 P + 304 C + 12 R C +
+240 I C bytes per step), the energy drift of one step against an FP64 CPU
+re-solve of the same manifolds, and the CPU oracle on the same workload (one
+thread and all host cores).  Further sections: the dense 100k-body pile at its
+fixed 20 sweeps over the configuration's 600 steps, and the plain vs learned
+warm start at residual <= 1e-4.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -31,6 +37,7 @@ import axm  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SEED_BASE = 0x4158494F4D000000
+DT = 1.0 / 120.0
 WORKLOADS = {
     0: "mixed spheres/boxes, 1k bodies, ground plane",
     1: "dense stacking pile: 100k spheres/boxes/capsules, 20x20 m footprint, 4 walls + ground",
@@ -38,48 +45,278 @@ WORKLOADS = {
     3: "4096 independent worlds, 16-4096 bodies each",
 }
 DEFAULT_ITERS = {0: 10, 1: 20, 2: 10, 3: 10}
+# algorithmic bytes (per body, pair, contact point, contact x colouring round, contact x sweep)
+B_BODY, B_PAIR, B_CONTACT, B_ROUND, B_SWEEP = 304, 104, 304, 12, 240
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(desc, bodies, iters, dt, budget_s):
-    """Times the CPU oracle (one thread) on the given state for a bounded number of steps."""
+def host_cpu():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return model, nproc
+
+
+def oracle_rate(desc, state, tol, max_sweeps, every, iters, threads, budget_s, fp64=False):
+    """Steps the CPU oracle from `state` with the same solver settings (residual-stopped when tol > 0)
+    for about budget_s seconds; returns (body-steps/s, steps, seconds, mean sweeps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
-    ow = orc.OracleWorld(orc.params_from_desc(desc))
-    ow.set_bodies(bodies)
-    t1 = ow.run(dt, iters, 1)
-    steps = max(1, min(200, int(budget_s / max(t1, 1e-6))))
-    t = ow.run(dt, iters, steps)
-    total_steps = steps
-    return dict(value=bodies.n * total_steps / t, unit="body-steps/s", cores=1, kind="port",
-                sample="oracle (C++, 1 thread) continuing from the GPU state after the timed region: "
-                       "%d bodies x %d steps (%.1f s)" % (bodies.n, total_steps, t))
+    ow = orc.OracleWorld(orc.params_from_desc(desc), fp64=fp64)
+    ow.set_bodies(state)
+    ow.set_threads(threads)
+    if tol > 0:
+        ow.set_tolerance(tol, max_sweeps, every)
+    t0 = time.perf_counter()
+    steps, sweeps = 0, 0
+    while True:
+        st = ow.step(DT, iters)
+        steps += 1
+        sweeps += st.iterations
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 400:
+            break
+    ow.close()
+    return state.n * steps / el, steps, el, sweeps / steps
+
+
+def cpu_baseline(desc, state, tol, max_sweeps, every, iters, budget_s):
+    """The CPU oracle (C++, the same algorithms and sweep rule) continuing from the GPU's state: one
+    thread and every host core of this process (at most 16, the box's CPU share)."""
+    model, nproc = host_cpu()
+    cores = max(1, min(nproc, 16))
+    r1, s1, t1, sw1 = oracle_rate(desc, state, tol, max_sweeps, every, iters, 1, budget_s)
+    rn, sn, tn, swn = oracle_rate(desc, state, tol, max_sweeps, every, iters, cores, budget_s)
+    what = "residual <= %g" % tol if tol > 0 else "%d sweeps" % iters
+    return {
+        "value": round(rn, 1), "unit": "body-steps/s", "cores": cores, "kind": "port",
+        "sample": "oracle (C++ restatement, %d threads) continuing from the GPU state after the timed region, "
+                  "%s: %d bodies x %d steps in %.1f s (mean %.1f sweeps)" % (cores, what, state.n, sn, tn, swn),
+        "one_thread": {"value": round(r1, 1), "steps": s1, "seconds": round(t1, 2), "mean_sweeps": round(sw1, 2)},
+        "cpu_model": model, "nproc": nproc,
+    }
+
+
+def state_of(w, bodies):
+    g = w.get_bodies()
+    s = bodies.copy()
+    s.pos, s.quat, s.lin_vel, s.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
+    return s
+
+
+def kinetic_energy(b, v, w):
+    """Translational + rotational kinetic energy (J, FP64) of the dynamic bodies."""
+    dyn = b.inv_mass > 0
+    m = 1.0 / b.inv_mass[dyn].astype(np.float64)
+    ke = 0.5 * (m * (v[dyn].astype(np.float64) ** 2).sum(1)).sum()
+    q = b.quat[dyn].astype(np.float64)
+    x, y, z, s = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - s * z), 2 * (x * z + s * y)], -1),
+                  np.stack([2 * (x * y + s * z), 1 - 2 * (x * x + z * z), 2 * (y * z - s * x)], -1),
+                  np.stack([2 * (x * z - s * y), 2 * (y * z + s * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+    wl = np.einsum("nji,nj->ni", R, w[dyn].astype(np.float64))
+    inertia = 1.0 / b.inv_inertia[dyn].astype(np.float64)
+    return ke + 0.5 * (inertia * wl * wl).sum()
+
+
+def fp64_resolve(w, bodies, desc):
+    """One more GPU step (FP32) and the FP64 oracle solving the same manifolds from the same state,
+    warm start and colours at the same sweep count: the relative difference of the kinetic energy
+    after the step (the energy drift of one step against the FP64 re-solve), the largest velocity
+    difference, and the FP64 solve's residual."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+
+    before = state_of(w, bodies)
+    st = w.step(DT, max(desc.max_iterations, 1), axm.STEP_RESIDUAL | axm.STEP_KEEP_STAGES)
+    man = w.manifolds()
+    col, _, _ = w.colours()
+    warm, _ = w.impulses()
+    after = w.get_bodies()
+    d64 = axm.WorldDesc.from_buffer_copy(desc)
+    d64.residual_tol = 0.0
+    ow = orc.OracleWorld(orc.params_from_desc(d64), fp64=True)
+    ow.set_bodies(before)
+    ost = ow.step_given(DT, st.iterations, man, warm=warm, colours=col)
+    _, _, v64, w64 = ow.get_state()
+    ow.close()
+    e32 = kinetic_energy(bodies, after.lin_vel, after.ang_vel)
+    e64 = kinetic_energy(bodies, v64, w64)
+    dv = np.abs(after.lin_vel.astype(np.float64) - v64).max()
+    return {"sweeps": int(st.iterations), "manifolds": int(st.manifolds), "ke_fp32": e32, "ke_fp64": e64,
+            "energy_drift": (e32 - e64) / e64 if e64 else 0.0, "max_dv": float(dv),
+            "residual_fp32": float(st.residual), "residual_fp64": float(ost.residual)}
+
+
+def headline(args):
+    cfg = args.config
+    bodies, desc = axm.scene(cfg, SEED_BASE + 1000 * cfg, args.bodies, args.worlds)
+    iters = args.iters or DEFAULT_ITERS[cfg]
+    if args.tol > 0:
+        desc.residual_tol = args.tol
+        desc.max_iterations = args.max_iters
+        desc.residual_every = max(1, args.tol_every)
+    log("scene config %d: %d bodies" % (cfg, bodies.n))
+    w = axm.World(desc)
+    w.set_bodies(bodies)
+    sweeps_arg = args.max_iters if args.tol > 0 else iters
+    t0 = time.time()
+    w.run(DT, sweeps_arg, args.warmup, graph=not args.eager)
+    w.sync()
+    log("warm-up %d steps (%.1fs)" % (args.warmup, time.time() - t0))
+    tuned = None
+    if args.autotune and args.tol <= 0:
+        # tuned on a copy of the warmed-up state, so the timed world's trajectory is untouched
+        tw = axm.World(desc)
+        tw.set_bodies(state_of(w, bodies))
+        tw.step(DT, iters, 0)
+        g, lds, pol, ms = tw.autotune(DT, iters, 5)
+        tw.close()
+        w.set_sweep_config(g, lds, pol)
+        tuned = {"groups": g, "lds_kib": lds, "policy": pol, "sweep_ms": round(ms, 4)}
+    w.totals(reset=True)
+    w.profile(True)
+    w.sync()
+    t0 = time.perf_counter()
+    w.run(DT, sweeps_arg, args.steps, graph=not args.eager)
+    w.sync()
+    elapsed = time.perf_counter() - t0
+    prof = w.profile(False)
+    tot = w.totals(reset=True)
+    log("timed %d steps in %.3fs" % (args.steps, elapsed))
+    diag = w.step(DT, sweeps_arg, axm.STEP_RESIDUAL | axm.STEP_TIMINGS)
+    value = bodies.n * args.steps / elapsed
+    steps_done = max(int(tot.steps), 1)
+    mean_sweeps = tot.sweeps / steps_done
+    # dominant kernel: the sweep kernel, 240 B per contact point per sweep
+    sweep_bytes = B_SWEEP * float(tot.contact_sweeps)
+    sweep_s = prof.pgs_ms * 1e-3
+    achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    launches = max(int(prof.pgs_launches), 1)
+    roof = {
+        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_pgs_own",
+        "launches": int(prof.pgs_launches), "avg_launch_us": round(prof.pgs_ms * 1e3 / launches, 3),
+        "bytes_per_launch": round(sweep_bytes / launches, 1),
+        "per_unit": "240 B per contact point per sweep", "share_of_step": round(sweep_s / elapsed, 4),
+    }
+    # whole step: 304 N + 104 P + 304 C + 12 R C + 240 I C per step (P, C, R of the diagnostic step)
+    dyn = int((bodies.inv_mass > 0).sum())
+    b_step = (B_BODY * dyn + B_PAIR * diag.pairs + B_CONTACT * diag.contacts + B_ROUND * diag.colour_rounds *
+              diag.contacts) + B_SWEEP * float(tot.contact_sweeps) / steps_done
+    step_gbs = b_step * args.steps / elapsed / 1e9
+    out = {
+        "metric": "body-steps/sec at contact residual <=1e-4: plain GPU vs batched+learned warm-start",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded DeterministicRNG scene)",
+        "config": {
+            "workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "dt": DT,
+            "residual_tol": args.tol, "max_sweeps": sweeps_arg if args.tol > 0 else None,
+            "residual_check_every": max(1, args.tol_every) if args.tol > 0 else None,
+            "sweeps": None if args.tol > 0 else iters, "softness": round(float(desc.softness), 4),
+            "mode": "plain GPU, previous-step warm start" + (", residual-stopped" if args.tol > 0 else ""),
+            "launch": "eager" if args.eager else "hipGraph replay",
+            "sweep_launch": tuned if tuned else "default (one workgroup per CU)",
+        },
+        "at_tolerance": {
+            "steps": int(tot.steps), "steps_at_tol": int(tot.steps_at_tol),
+            "fraction_reached": round(tot.steps_at_tol / steps_done, 4) if args.tol > 0 else None,
+            "mean_sweeps": round(mean_sweeps, 2), "max_sweeps_used": int(tot.max_sweeps),
+            "mean_contacts": round(tot.contact_sweeps / max(tot.sweeps, 1), 1),
+        },
+        "last_step": {k: diag.as_dict()[k] for k in ("pairs", "manifolds", "contacts", "colours", "colour_rounds",
+                                                    "iterations", "residual", "broadphase_ms", "narrowphase_ms",
+                                                    "solver_ms", "integration_ms", "total_ms")},
+        "roofline": roof,
+        "roofline_step": {"bytes_per_step": round(b_step, 1), "achieved": round(step_gbs, 2), "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 5),
+                          "formula": "304 N + 104 P + 304 C + 12 R C + 240 I C"},
+    }
+    if not args.no_fp64:
+        r = fp64_resolve(w, bodies, desc)
+        out["fp64_resolve"] = {k: (round(v, 9) if isinstance(v, float) else v) for k, v in r.items()}
+        log("fp64 re-solve: energy drift %.3g, max dv %.3g" % (r["energy_drift"], r["max_dv"]))
+    state = state_of(w, bodies) if not args.no_cpu else None
+    w.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, state, args.tol, args.max_iters, max(1, args.tol_every), iters,
+                                           args.cpu_seconds)
+        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 2)
+        out["gpu_over_cpu_1thread"] = round(value / out["cpu_baseline"]["one_thread"]["value"], 2)
+    return out
+
+
+def pile_section(steps, no_cpu, cpu_seconds):
+    """The dense pile (100k bodies, 20 sweeps) over the configuration's full run of `steps` steps from
+    its initial state: body-steps/s, contacts at the end, the sweep kernel's share."""
+    bodies, desc = axm.scene(1, SEED_BASE + 1000, 0, 0)
+    w = axm.World(desc)
+    w.set_bodies(bodies)
+    w.step(DT, 20, 0)
+    w.sync()
+    w.totals(reset=True)
+    t0 = time.perf_counter()
+    w.run(DT, 20, steps - 1, graph=True)
+    w.sync()
+    el = time.perf_counter() - t0
+    tot = w.totals(reset=True)
+    diag = w.step(DT, 20, axm.STEP_RESIDUAL)
+    out = {"workload": WORKLOADS[1], "bodies": bodies.n, "sweeps": 20, "steps": steps - 1,
+           "body_steps_per_s": round(bodies.n * (steps - 1) / el, 1), "ms_per_step": round(el * 1e3 / (steps - 1), 4),
+           "contacts_last_step": int(diag.contacts), "manifolds_last_step": int(diag.manifolds),
+           "contacts_per_body": round(diag.contacts / bodies.n, 3), "colours": int(diag.colours),
+           "mean_contacts": round(tot.contact_sweeps / max(tot.sweeps, 1), 1),
+           "residual_last_step": float(diag.residual)}
+    log("pile: %.0f body-steps/s, %d contacts at the end" % (out["body_steps_per_s"], diag.contacts))
+    if not no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, state_of(w, bodies), 0.0, 0, 1, 20, cpu_seconds)
+    w.close()
+    return out
 
 
 def tolerance_comparison(tol, max_sweeps, worlds):
-    """Body-steps/s and sweeps per step when every step iterates to residual <= tol (capped at
-    max_sweeps): the plain previous-step warm start against the learned one (network trained here on
-    the GPU on contacts logged from the plain solver), on one held-out 1k-body scene and on a batch of
-    `worlds` independent worlds stepped together."""
+    """Body-steps/s and sweeps per step when every step iterates to residual <= tol: the plain
+    previous-step warm start against the learned one (network trained here on the GPU on contacts
+    logged from the plain solver), on one held-out 1k-body scene and on a batch of `worlds` worlds."""
     import warmstart
 
     t0 = time.time()
-    net, losses, rows = warmstart.train_on_config0()
+    net, losses, rows = warmstart.train_default()
     out = {"residual_tol": tol, "max_sweeps": max_sweeps,
            "training": {"rows": rows, "epochs": len(losses), "loss_first": round(float(losses[0]), 5),
                         "loss_last": round(float(losses[-1]), 5), "seconds": round(time.time() - t0, 2),
-                        "data": "16 seeded 1k-body scenes x 240 steps, plain solver at 60 sweeps"}}
+                        "data": warmstart.TRAIN_DATA}}
     log("warm-start network trained on %d contacts in %.1fs" % (rows, time.time() - t0))
     b, d = axm.scene(0, SEED_BASE + 2, 1000)
     b3, d3 = axm.scene(3, SEED_BASE + 3000, 0, worlds)
     cases = (("single_1k_plain", b, d, axm.WARM_CACHE, None, 60, 120, 1),
              ("single_1k_learned", b, d, axm.WARM_MLP, net, 60, 120, 1),
-             ("batched_plain", b3, d3, axm.WARM_CACHE, None, 60, 30, 5),
-             ("batched_learned", b3, d3, axm.WARM_MLP, net, 60, 30, 5))
+             ("batched_plain", b3, d3, axm.WARM_CACHE, None, 60, 30, 1),
+             ("batched_learned", b3, d3, axm.WARM_MLP, net, 60, 30, 1))
     for name, bb, dd, warm, nn, warmup, steps, every in cases:
         tm = {}
         it, reached = warmstart.iterations_to_tolerance(bb, dd, steps, tol, max_sweeps, warm, net=nn, warmup=warmup,
@@ -89,65 +326,12 @@ def tolerance_comparison(tol, max_sweeps, worlds):
                      "fraction_reached": round(float(reached.mean()), 3), "residual_check_every": every}
         log("%s: %.0f body-steps/s, mean sweeps %.1f, reached %.2f" % (name, out[name]["body_steps_per_s"],
                                                                        it.mean(), reached.mean()))
+    for kind in ("single_1k", "batched"):
+        p, l_ = out[kind + "_plain"], out[kind + "_learned"]
+        out[kind + "_gain"] = {"sweeps_ratio": round(p["mean_sweeps"] / max(l_["mean_sweeps"], 1e-9), 3),
+                               "rate_ratio": round(l_["body_steps_per_s"] / max(p["body_steps_per_s"], 1e-9), 3)}
     net.close()
     return out
-
-
-def scene_at_residual(cfg, tol, max_sweeps, steps, warmup, cpu_seconds, no_cpu, every=1):
-    """Scene `cfg` with every step iterated until its contact residual is <= tol (checked inside
-    the sweep launch after every `every` sweeps, capped at max_sweeps): body-steps/s, sweeps per
-    step, and the oracle at the GPU's mean sweep count on the state the GPU reached."""
-    dt = 1.0 / 120.0
-    bodies, desc = axm.scene(cfg, SEED_BASE + 1000 * cfg, 0, 0)
-    desc.residual_tol = tol
-    desc.max_iterations = max_sweeps
-    desc.residual_every = every
-    w = axm.World(desc)
-    w.set_bodies(bodies)
-    for _ in range(warmup):
-        w.step(dt, DEFAULT_ITERS[cfg], 0)
-    w.sync()
-    sweeps, reached = [], []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st = w.step(dt, max_sweeps, axm.STEP_RESIDUAL)
-        sweeps.append(st.iterations)
-        reached.append(st.residual <= tol)
-    w.sync()
-    el = time.perf_counter() - t0
-    out = {"workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "residual_tol": tol,
-           "check_every": every, "max_sweeps": max_sweeps, "warmup": warmup, "steps": steps,
-           "body_steps_per_s": round(bodies.n * steps / el, 1), "ms_per_step": round(el * 1e3 / steps, 4),
-           "mean_sweeps": round(float(np.mean(sweeps)), 2), "max_sweeps_used": int(np.max(sweeps)),
-           "fraction_reached": round(float(np.mean(reached)), 3)}
-    log("scene %d at residual %g (checked every %d): %.0f body-steps/s, mean sweeps %.1f" % (
-        cfg, tol, every, out["body_steps_per_s"], np.mean(sweeps)))
-    if not no_cpu:
-        g = w.get_bodies()
-        state = bodies.copy()
-        state.pos, state.quat, state.lin_vel, state.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
-        desc.residual_tol = 0.0
-        out["cpu_baseline"] = cpu_baseline(desc, state, max(1, int(round(float(np.mean(sweeps))))), dt,
-                                           cpu_seconds)
-        out["gpu_over_cpu"] = round(out["body_steps_per_s"] / out["cpu_baseline"]["value"], 2)
-    w.close()
-    return out
-
-
-def measured_traffic(cfg, n, iters, args):
-    """HBM bytes per sweep-kernel launch from the committed counter runs (rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE, scaled by a plain copy of known size), when they cover this exact workload; else None."""
-    path = os.path.join(ROOT, "profiles", "k_pgs_own_pile100k_traffic.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except OSError:
-        return None
-    w = t["workload"]
-    if (w["scene_config"], w["bodies"], w["sweeps"], w["warmup"], w["steps"]) != (cfg, n, iters, args.warmup,
-                                                                                   args.steps):
-        return None
-    return t["traffic_bytes_per_launch"]
 
 
 def copy_calibration(mib, reps):
@@ -170,22 +354,23 @@ def copy_calibration(mib, reps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=150)
-    ap.add_argument("--warmup", type=int, default=150)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=110)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--bodies", type=int, default=0)
     ap.add_argument("--worlds", type=int, default=0)
-    ap.add_argument("--iters", type=int, default=0)
-    ap.add_argument("--tol", type=float, default=0.0, help="iterate each step until the residual <= tol")
-    ap.add_argument("--max-iters", type=int, default=200)
+    ap.add_argument("--iters", type=int, default=0, help="fixed sweeps per step (with --tol 0)")
+    ap.add_argument("--tol", type=float, default=1e-4, help="iterate each step until the residual <= tol (0: fixed)")
+    ap.add_argument("--max-iters", type=int, default=300)
     ap.add_argument("--tol-every", type=int, default=1, help="with --tol: sweeps between residual checks")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 re-solve of one step")
     ap.add_argument("--eager", action="store_true", help="launch every step's kernels instead of replaying a graph")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default sweep launch shape")
+    ap.add_argument("--autotune", action="store_true", help="autotune the sweep launch shape (fixed sweeps only)")
+    ap.add_argument("--no-pile", action="store_true", help="skip the dense pile section")
+    ap.add_argument("--pile-steps", type=int, default=600)
     ap.add_argument("--no-tolerance", action="store_true", help="skip the plain vs learned run at residual <= tol")
-    ap.add_argument("--no-scene-1m", action="store_true", help="skip the 1M-body scene at residual <= tol")
-    ap.add_argument("--tolerance", type=float, default=1e-4)
     ap.add_argument("--tolerance-worlds", type=int, default=4096)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain copies of MIB MiB (counter scaling), then exit")
@@ -200,128 +385,11 @@ def main():
     if axm.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    cfg = args.config
-    iters = args.iters or DEFAULT_ITERS[cfg]
-    dt = 1.0 / 120.0
-    bodies, desc = axm.scene(cfg, SEED_BASE + 1000 * cfg, args.bodies, args.worlds)
-    if os.environ.get("AXM_SOLVER_MODE"):
-        desc.solver_mode = int(os.environ["AXM_SOLVER_MODE"])
-    if args.tol > 0:
-        desc.residual_tol = args.tol
-        desc.max_iterations = args.max_iters
-        desc.residual_every = max(1, args.tol_every)
-    log("scene config %d: %d bodies" % (cfg, bodies.n))
-    w = axm.World(desc)
-    if os.environ.get("AXM_SWEEP"):
-        g, lds, pol = (int(x, 0) for x in os.environ["AXM_SWEEP"].split(","))
-        w.set_sweep_config(g, lds, pol)
-    w.set_bodies(bodies)
-    t0 = time.time()
-    for k in range(args.warmup):
-        w.step(dt, iters, 0)
-        if (k + 1) % 50 == 0:
-            log("warmup %d/%d (%.1fs)" % (k + 1, args.warmup, time.time() - t0))
-    w.sync()
-    tuned = None
-    if not args.no_autotune and args.tol <= 0:
-        # tuned on a copy of the warmed-up state, so the timed world's trajectory is untouched
-        gs = w.get_bodies()
-        tb = bodies.copy()
-        tb.pos, tb.quat, tb.lin_vel, tb.ang_vel = gs.pos, gs.quat, gs.lin_vel, gs.ang_vel
-        tw = axm.World(desc)
-        tw.set_bodies(tb)
-        tw.step(dt, iters, 0)
-        g, lds, pol, ms = tw.autotune(dt, iters, 5)
-        tw.close()
-        w.set_sweep_config(g, lds, pol)
-        tuned = {"groups": g, "lds_kib": lds, "policy": pol, "sweep_ms": round(ms, 4)}
-        log("autotuned sweep: %d workgroups, %d KiB LDS, policy 0x%x (%.3f ms)" % (g, lds, pol, ms))
-    w.sync()
-    w.profile(True)
-    w.sync()
-    t0 = time.perf_counter()
-    sweeps, reached = [], []
-    if args.tol > 0:
-        # residual-stopped steps: the sweep launch checks the residual itself; the host reads
-        # each step's sweep count and residual once the step is done
-        for _ in range(args.steps):
-            st = w.step(dt, args.max_iters, axm.STEP_RESIDUAL)
-            sweeps.append(st.iterations)
-            reached.append(st.residual <= args.tol)
-    else:
-        st = w.run(dt, iters, args.steps, graph=not args.eager)
-    w.sync()
-    elapsed = time.perf_counter() - t0
-    prof = w.profile(False)
-    log("timed %d steps in %.3fs" % (args.steps, elapsed))
-    # diagnostics on one extra (untimed) step
-    diag = w.step(dt, iters, axm.STEP_RESIDUAL | axm.STEP_TIMINGS)
-    value = bodies.n * args.steps / elapsed
-    achieved = (prof.pgs_bytes / (prof.pgs_ms * 1e-3) / 1e9) if prof.pgs_ms > 0 else 0.0
-    roof = {
-        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(cfg, bodies.n, iters, args),
-        "kernel": "k_pgs_own",
-        "launches": int(prof.pgs_launches),
-        "avg_launch_us": round(prof.pgs_ms * 1e3 / max(prof.pgs_launches, 1), 3),
-        "bytes_per_launch": round(prof.pgs_bytes / max(prof.pgs_launches, 1), 1),
-    }
-    out = {
-        "metric": "body-steps/sec at contact residual <=1e-4: plain GPU vs batched+learned warm-start",
-        "value": round(value, 1),
-        "unit": "body-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded DeterministicRNG scene)",
-        "config": {
-            "workload": WORKLOADS[cfg], "scene_config": cfg, "bodies": bodies.n, "sweeps": iters,
-            "residual_tol": args.tol, "dt": dt,
-            "mode": "plain GPU, previous-step warm start" + (", residual-stopped" if args.tol > 0 else ""),
-            "launch": "eager" if args.eager else "hipGraph replay",
-            "sweep_launch": tuned if tuned else "default (one workgroup per CU)",
-        },
-        "residual_last_step": diag.residual,
-        "last_step": {k: diag.as_dict()[k] for k in ("pairs", "manifolds", "contacts", "colours", "colour_rounds",
-                                                    "iterations", "broadphase_ms", "narrowphase_ms", "solver_ms",
-                                                    "integration_ms", "total_ms")},
-        "roofline": roof,
-    }
-    w_state = w.get_bodies() if not args.no_cpu else None
-    w.close()
-    if args.tol > 0:
-        out["residual_stopped"] = {"residual_tol": args.tol, "max_sweeps": args.max_iters,
-                                   "check_every": max(1, args.tol_every),
-                                   "mean_sweeps": round(float(np.mean(sweeps)), 2),
-                                   "max_sweeps_used": int(np.max(sweeps)),
-                                   "fraction_reached": round(float(np.mean(reached)), 3)}
-    if not args.no_scene_1m and not (cfg == 2 and args.tol > 0):
-        r1 = scene_at_residual(2, args.tolerance, 300, 20, 20, 10.0, args.no_cpu)
-        # the same with the residual checked after every fourth sweep (each check is a grid-wide
-        # meeting of the sweep's workgroups); the oracle's rate is the one measured above
-        r4 = scene_at_residual(2, args.tolerance, 300, 20, 20, 10.0, True, every=4)
-        keep = ("check_every", "body_steps_per_s", "ms_per_step", "mean_sweeps", "max_sweeps_used",
-                "fraction_reached")
-        r1["checked_every_4"] = {k: r4[k] for k in keep}
-        if "cpu_baseline" in r1:
-            r1["checked_every_4"]["gpu_over_cpu"] = round(r4["body_steps_per_s"] / r1["cpu_baseline"]["value"], 2)
-        out["scene_1m_at_residual"] = r1
+    out = headline(args)
+    if not args.no_pile:
+        out["pile_100k"] = pile_section(args.pile_steps, args.no_cpu, args.cpu_seconds)
     if not args.no_tolerance:
-        out["at_tolerance"] = tolerance_comparison(args.tolerance, args.max_iters * 3 // 2, args.tolerance_worlds)
-    if not args.no_cpu:
-        g = w_state
-        state = bodies.copy()
-        state.pos, state.quat, state.lin_vel, state.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
-        # at a residual tolerance the oracle (the same arithmetic, so the same residuals) runs the
-        # GPU's mean sweep count per step
-        cpu_iters = max(1, int(round(float(np.mean(sweeps))))) if args.tol > 0 else iters
-        out["cpu_baseline"] = cpu_baseline(desc, state, cpu_iters, dt, args.cpu_seconds)
-        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 2)
+        out["plain_vs_learned"] = tolerance_comparison(1e-4, args.max_iters, args.tolerance_worlds)
     print(json.dumps(out), flush=True)
 
 

@@ -70,7 +70,7 @@ enum {
 
 /* World description.  Stands in for gui::PhysicsWorldConfig
  * (include/axiom/gui/physics_panel.hpp:34-43) plus the solver constants the
- * reference leaves unspecified.  The leading fields up to `record_residual`
+ * reference leaves unspecified.  The leading fields up to `softness`
  * have the same layout as the CPU oracle's parameter block. */
 typedef struct axm_world_desc {
     float gravity[3];
@@ -89,6 +89,9 @@ typedef struct axm_world_desc {
     float planes[AXM_MAX_PLANES][4]; /* n.xyz, d: solid where dot(n,x) < d */
     uint32_t warm_start;   /* AXM_WARM_* */
     uint32_t record_residual; /* 1: residual after every sweep (diagnostic) */
+    float softness;        /* relative compliance eps of every contact row: a row solves v + eps*k*lambda = target
+                              (k: the row's inverse effective mass).  0 = rigid rows; > 0 makes each step's
+                              contact problem strictly convex, so the sweeps converge to any residual */
     /* capacities (0 = sized from the body count) */
     uint32_t max_bodies;
     uint32_t max_pairs;
@@ -183,6 +186,21 @@ axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags,
  * sweep launch checks the residual itself). */
 axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last);
 axm_status axm_world_sync(axm_world w);
+/* Totals over the steps run since the last reset (counted on the device, so graph-replayed
+ * steps need no host round trip): steps, steps whose sweeps stopped because the residual met
+ * residual_tol, sweeps, contact points x sweeps (the per-step contact count times that step's
+ * sweeps), and the most sweeps one step ran.  Synchronises the world's stream; reset = 1
+ * clears the totals after reading them.  Benchmark hook, no reference counterpart. */
+typedef struct axm_world_totals {
+    uint64_t steps;
+    uint64_t steps_at_tol;
+    uint64_t sweeps;
+    uint64_t contact_sweeps;
+    uint32_t max_sweeps;
+    uint32_t pad;
+} axm_world_totals;
+axm_status axm_world_get_totals(axm_world w, axm_world_totals* out, int reset);
+
 /* Enables / disables per-launch event timing of the sweep kernel; `out`
  * (optional) receives the accumulated profile, which is then reset. */
 axm_status axm_world_profile(axm_world w, int enable, axm_kernel_profile* out);

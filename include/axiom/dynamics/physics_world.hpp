@@ -76,19 +76,30 @@ public:
     }
 
     // Adds a body from any struct with gui::RigidBodyData's fields: position,
-    // rotation (x, y, z, w), linearVelocity, angularVelocity, mass (0 = static),
-    // inertiaTensor (diagonal), shapeType (Sphere=0, Box=1, Capsule=2),
-    // shapeExtents (sphere: x = r; box: half extents; capsule: x = r, y = half-height).
+    // rotation (x, y, z, w), linearVelocity, angularVelocity, mass, inertiaTensor
+    // (diagonal), shapeType (Sphere=0, Box=1, Capsule=2), shapeExtents (sphere: x = r;
+    // box: half extents; capsule: x = r, y = half-height) and, when present, type
+    // (gui::BodyType: Static = 0, Dynamic = 1, Kinematic = 2,
+    // include/axiom/gui/body_inspector.hpp:13-17).  A Static body gets zero inverse
+    // mass and inertia whatever its mass field holds (RigidBodyData defaults mass to
+    // 1); a Dynamic body needs mass > 0; Kinematic bodies are not supported and
+    // return InvalidRigidBody.  Without a type field, mass 0 means static.
     template <class RigidBodyDataT> Status addRigidBody(const RigidBodyDataT& rb, uint32_t world = 0) {
         const uint32_t t = static_cast<uint32_t>(rb.shapeType);
         if (t > AXM_SHAPE_CAPSULE) return Status(AXM_INVALID_SHAPE);
         if (!(rb.mass >= 0.0f)) return Status(AXM_INVALID_RIGID_BODY);
+        bool dyn = rb.mass > 0.0f;
+        if constexpr (requires { rb.type; }) {
+            const uint32_t bt = static_cast<uint32_t>(rb.type);
+            if (bt == 2u || bt > 2u) return Status(AXM_INVALID_RIGID_BODY);
+            if (bt == 1u && !(rb.mass > 0.0f)) return Status(AXM_INVALID_RIGID_BODY);
+            dyn = bt == 1u;
+        }
         const float prm[3] = {rb.shapeExtents.x, rb.shapeExtents.y, rb.shapeExtents.z};
         const float pos[3] = {rb.position.x, rb.position.y, rb.position.z};
         const float rot[4] = {rb.rotation.x, rb.rotation.y, rb.rotation.z, rb.rotation.w};
         const float v[3] = {rb.linearVelocity.x, rb.linearVelocity.y, rb.linearVelocity.z};
         const float w[3] = {rb.angularVelocity.x, rb.angularVelocity.y, rb.angularVelocity.z};
-        const bool dyn = rb.mass > 0.0f;
         const float ii[3] = {dyn && rb.inertiaTensor.x > 0.0f ? 1.0f / rb.inertiaTensor.x : 0.0f,
                              dyn && rb.inertiaTensor.y > 0.0f ? 1.0f / rb.inertiaTensor.y : 0.0f,
                              dyn && rb.inertiaTensor.z > 0.0f ? 1.0f / rb.inertiaTensor.z : 0.0f};

@@ -20,7 +20,7 @@ class Params(C.Structure):
         ("rest_threshold", C.c_float), ("baumgarte", C.c_float), ("slop", C.c_float), ("max_bias", C.c_float),
         ("margin", C.c_float), ("lin_damp", C.c_float), ("ang_damp", C.c_float), ("cell_size", C.c_float),
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
-        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32),
+        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float),
     ]
 
 
@@ -60,6 +60,8 @@ def lib():
     L.orc_create.restype = P
     L.orc_create.argtypes = [C.POINTER(Params), C.c_int]
     L.orc_destroy.argtypes = [P]
+    L.orc_set_threads.argtypes = [P, C.c_uint32]
+    L.orc_set_tolerance.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32]
     L.orc_set_params.argtypes = [P, C.POINTER(Params)]
     L.orc_set_bodies.argtypes = [P, C.c_uint32, f32p, f32p, f32p, f32p, f32p, f32p, u32p, f32p, u32p]
     L.orc_get_bodies.argtypes = [P, f32p, f32p, f32p, f32p]
@@ -156,6 +158,12 @@ class OracleWorld:
             self.close()
         except Exception:
             pass
+
+    def set_threads(self, threads):
+        self.L.orc_set_threads(self.h, int(threads))
+
+    def set_tolerance(self, tol, max_iters, every=1):
+        self.L.orc_set_tolerance(self.h, float(tol), int(max_iters), int(every))
 
     def set_params(self, params):
         self.params = params

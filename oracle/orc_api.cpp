@@ -185,6 +185,20 @@ void* orc_create(const Params* p, int fp64) {
 
 void orc_destroy(void* h) { delete (Handle*)h; }
 
+// Host threads of the oracle's parallel loops (results do not depend on the count).
+void orc_set_threads(void* h, uint32_t threads) { DISPATCH(h, W.set_threads(threads)); }
+
+// Residual-stopped sweeps (tol > 0): residual checked after every `every` sweeps, the
+// solve stops once it is <= tol, at most max_iters sweeps (the step's iters argument is
+// then ignored).  tol = 0 restores fixed sweep counts.
+void orc_set_tolerance(void* h, float tol, uint32_t max_iters, uint32_t every) {
+    DISPATCH(h, {
+        W.residual_tol = T(tol);
+        W.max_iters = max_iters;
+        W.res_every = every ? every : 1u;
+    });
+}
+
 void orc_set_params(void* h, const Params* p) { DISPATCH(h, W.P = *p); }
 
 void orc_set_bodies(void* h, uint32_t n, const float* pos, const float* quat, const float* vel, const float* angvel,

@@ -25,15 +25,26 @@
 //   6. stable order of manifolds by colour = solve order
 //   7. prep per point, warm start (cache keyed by (a, b, feature id))
 //   8. I PGS sweeps, colour by colour, manifold by manifold: friction rows of
-//      all points, then normal rows (iteration 0 first applies the warm start)
+//      all points, then normal rows (iteration 0 first applies the warm start).
+//      Every row is regularised with a compliance eps*k proportional to its own
+//      diagonal k (soft constraint, constraint-force mixing): the row solves
+//      v + eps*k*lam = target, so one visit is
+//          lam' = proj(keep*lam + m*(target - v)),  m = 1/(k(1+eps)), keep = 1/(1+eps).
+//      The regularised problem is strictly convex in the normal rows, so the
+//      sweeps converge even where a manifold's rows are redundant (four
+//      coplanar points, two friction rows per point) and their targets are not
+//      mutually attainable, which otherwise leaves PGS drifting along the
+//      null space with a constant residual.
 //   9. x += v dt; q += (dt/2) (w,0) q; q /= |q|
 #pragma once
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "orc_geom.hpp"
+#include "orc_par.hpp"
 
 namespace orc {
 
@@ -52,6 +63,7 @@ struct Params {
     float planes[MAX_PLANES][4];  // n.xyz, d
     uint32_t warm_start;          // 0 none, 1 cache
     uint32_t record_residual;     // compute residual after every sweep
+    float softness;               // relative compliance eps of every row (0: rigid rows)
 };
 
 struct Stats {
@@ -93,6 +105,29 @@ template <class T> struct World {
     std::vector<T> lambda;  // per manifold x 4 points x 3 (solve output)
     std::vector<T> feat;    // per manifold x 4 points x 24 warm-start network inputs
     Stats st{};
+    // residual-stopped sweeps: with tol > 0 the residual is checked after every sweep and
+    // the solve stops once it is <= tol (at most max_iters sweeps)
+    T residual_tol = T(0);
+    uint32_t max_iters = 0;
+    uint32_t res_every = 1;
+    // host threads: every parallel loop below splits its range by a fixed grain and
+    // combines partial results in range order, so results do not depend on the count
+    std::unique_ptr<Pool> pool;
+    void set_threads(unsigned t) {
+        if (t <= 1) pool.reset();
+        else pool.reset(new Pool(t));
+    }
+    // f(chunk, begin, end) over [0, n) in Pool::chunks_for(n, grain) ranges
+    template <class F> void par(uint64_t n, uint64_t grain, F&& f) {
+        if (n == 0) return;
+        const std::function<void(uint64_t, uint64_t, uint64_t)> fn = f;
+        if (pool) {
+            pool->parallel_for(n, grain, fn);
+        } else {
+            const uint64_t c = Pool::chunks_for(n, grain);
+            for (uint64_t k = 0; k < c; ++k) fn(k, n * k / c, n * (k + 1) / c);
+        }
+    }
 
     bool dynamic(uint32_t i) const { return im[i] > T(0); }
 
@@ -136,57 +171,93 @@ template <class T> struct World {
         const T inv = T(1) / T(P.cell_size);
         const V3<T> o = {T(P.origin[0]), T(P.origin[1]), T(P.origin[2])};
         std::vector<uint32_t> cx(n), cy(n), cz(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            const M3<T> R = quat_to_mat(q[i]);
-            V3<T> e;
-            aabb_ext(i, R, e);
-            mn[i] = x[i] - e;
-            mx[i] = x[i] + e;
-            cx[i] = cell_coord(x[i].x, o.x, inv);
-            cy[i] = cell_coord(x[i].y, o.y, inv);
-            cz[i] = cell_coord(x[i].z, o.z, inv);
-            keys[i] = ((uint64_t)world[i] << 30) | morton3(cx[i], cy[i], cz[i]);
+        par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
+            for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
+                const M3<T> R = quat_to_mat(q[i]);
+                V3<T> ext;
+                aabb_ext(i, R, ext);
+                mn[i] = x[i] - ext;
+                mx[i] = x[i] + ext;
+                cx[i] = cell_coord(x[i].x, o.x, inv);
+                cy[i] = cell_coord(x[i].y, o.y, inv);
+                cz[i] = cell_coord(x[i].z, o.z, inv);
+                keys[i] = ((uint64_t)world[i] << 30) | morton3(cx[i], cy[i], cz[i]);
+            }
+        });
+        // stable sort by key = sort by (key, index): sorted runs merged pairwise
+        {
+            std::vector<std::pair<uint64_t, uint32_t>> kv(n), tmp(n);
+            for (uint32_t i = 0; i < n; ++i) kv[i] = {keys[i], i};
+            const uint64_t RUN = 65536;
+            const uint64_t runs = (n + RUN - 1) / RUN;
+            par(runs, 1, [&](uint64_t, uint64_t b, uint64_t e) {
+                for (uint64_t r = b; r < e; ++r)
+                    std::sort(kv.begin() + r * RUN, kv.begin() + std::min<uint64_t>(n, (r + 1) * RUN));
+            });
+            for (uint64_t wd = RUN; wd < n; wd *= 2) {
+                const uint64_t pairs = (n + 2 * wd - 1) / (2 * wd);
+                par(pairs, 1, [&](uint64_t, uint64_t b, uint64_t e) {
+                    for (uint64_t r = b; r < e; ++r) {
+                        const uint64_t lo = r * 2 * wd, mid = std::min<uint64_t>(n, lo + wd),
+                                       hi = std::min<uint64_t>(n, lo + 2 * wd);
+                        std::merge(kv.begin() + lo, kv.begin() + mid, kv.begin() + mid, kv.begin() + hi,
+                                   tmp.begin() + lo);
+                    }
+                });
+                kv.swap(tmp);
+            }
+            order.resize(n);
+            for (uint32_t t = 0; t < n; ++t) order[t] = kv[t].second;
         }
-        order.resize(n);
-        for (uint32_t i = 0; i < n; ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
         std::vector<uint64_t> sk(n);
         for (uint32_t t = 0; t < n; ++t) sk[t] = keys[order[t]];
         pair_a.clear();
         pair_b.clear();
-        for (uint32_t t = 0; t < n; ++t) {
-            const uint32_t i = order[t];
-            if (dynamic(i)) {
-                for (uint32_t k = 0; k < P.num_planes; ++k) {
-                    const V3<T> pn = {T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])};
-                    const V3<T> c = x[i];
-                    const V3<T> e = mx[i] - c;
-                    const T s = dot(pn, c) - (std::fabs(pn.x) * e.x + std::fabs(pn.y) * e.y + std::fabs(pn.z) * e.z);
-                    if (s <= T(P.planes[k][3])) {
-                        pair_a.push_back(i);
-                        pair_b.push_back(PLANE_BIT | k);
-                    }
-                }
-            }
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int64_t X = (int64_t)cx[i] + dx, Y = (int64_t)cy[i] + dy, Z = (int64_t)cz[i] + dz;
-                        if (X < 0 || Y < 0 || Z < 0 || X > CELL_MAX || Y > CELL_MAX || Z > CELL_MAX) continue;
-                        const uint64_t key = ((uint64_t)world[i] << 30) | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
-                        auto lo = std::lower_bound(sk.begin(), sk.end(), key);
-                        auto hi = std::upper_bound(lo, sk.end(), key);
-                        for (auto it = lo; it != hi; ++it) {
-                            const uint32_t j = order[(size_t)(it - sk.begin())];
-                            if (j <= i) continue;
-                            if (!dynamic(i) && !dynamic(j)) continue;
-                            if (mn[i].x <= mx[j].x && mx[i].x >= mn[j].x && mn[i].y <= mx[j].y && mx[i].y >= mn[j].y &&
-                                mn[i].z <= mx[j].z && mx[i].z >= mn[j].z) {
-                                pair_a.push_back(i);
-                                pair_b.push_back(j);
-                            }
+        const uint64_t PGRAIN = 2048;
+        const uint64_t nchunk = Pool::chunks_for(n, PGRAIN);
+        std::vector<std::vector<uint32_t>> ca(nchunk), cb(nchunk);
+        par(n, PGRAIN, [&](uint64_t ch, uint64_t tb, uint64_t te) {
+            std::vector<uint32_t>& pa = ca[ch];
+            std::vector<uint32_t>& pb = cb[ch];
+            for (uint32_t t = (uint32_t)tb; t < (uint32_t)te; ++t) {
+                const uint32_t i = order[t];
+                if (dynamic(i)) {
+                    for (uint32_t k = 0; k < P.num_planes; ++k) {
+                        const V3<T> pn = {T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])};
+                        const V3<T> c = x[i];
+                        const V3<T> e = mx[i] - c;
+                        const T sd = dot(pn, c) - (std::fabs(pn.x) * e.x + std::fabs(pn.y) * e.y + std::fabs(pn.z) * e.z);
+                        if (sd <= T(P.planes[k][3])) {
+                            pa.push_back(i);
+                            pb.push_back(PLANE_BIT | k);
                         }
                     }
+                }
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int64_t X = (int64_t)cx[i] + dx, Y = (int64_t)cy[i] + dy, Z = (int64_t)cz[i] + dz;
+                            if (X < 0 || Y < 0 || Z < 0 || X > CELL_MAX || Y > CELL_MAX || Z > CELL_MAX) continue;
+                            const uint64_t key =
+                                ((uint64_t)world[i] << 30) | morton3((uint32_t)X, (uint32_t)Y, (uint32_t)Z);
+                            auto lo = std::lower_bound(sk.begin(), sk.end(), key);
+                            auto hi = std::upper_bound(lo, sk.end(), key);
+                            for (auto it = lo; it != hi; ++it) {
+                                const uint32_t j = order[(size_t)(it - sk.begin())];
+                                if (j <= i) continue;
+                                if (!dynamic(i) && !dynamic(j)) continue;
+                                if (mn[i].x <= mx[j].x && mx[i].x >= mn[j].x && mn[i].y <= mx[j].y &&
+                                    mx[i].y >= mn[j].y && mn[i].z <= mx[j].z && mx[i].z >= mn[j].z) {
+                                    pa.push_back(i);
+                                    pb.push_back(j);
+                                }
+                            }
+                        }
+            }
+        });
+        for (uint64_t c = 0; c < nchunk; ++c) {
+            pair_a.insert(pair_a.end(), ca[c].begin(), ca[c].end());
+            pair_b.insert(pair_b.end(), cb[c].begin(), cb[c].end());
         }
     }
 
@@ -196,23 +267,29 @@ template <class T> struct World {
         man_b.clear();
         man.clear();
         const T mg = T(P.margin);
-        for (size_t p = 0; p < pair_a.size(); ++p) {
-            const uint32_t a = pair_a[p], b = pair_b[p];
-            Man<T> m;
-            const Shape<T> A = shape_of(a);
-            if (b & PLANE_BIT) {
-                const uint32_t k = b & ~PLANE_BIT;
-                collide_plane(m, A, V3<T>{T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])}, T(P.planes[k][3]),
-                              mg);
-            } else {
-                const int s = collide(m, A, shape_of(b), mg);
-                if (s == GEOM_GJK_FAIL) ++st.gjk_fail;
-                if (s == GEOM_EPA_FAIL) ++st.epa_fail;
+        const size_t NP = pair_a.size();
+        std::vector<Man<T>> pm(NP);
+        std::vector<int> ps(NP, GEOM_OK);
+        par(NP, 1024, [&](uint64_t, uint64_t b, uint64_t e) {
+            for (size_t p = b; p < e; ++p) {
+                const uint32_t a = pair_a[p], bb = pair_b[p];
+                const Shape<T> A = shape_of(a);
+                if (bb & PLANE_BIT) {
+                    const uint32_t k = bb & ~PLANE_BIT;
+                    collide_plane(pm[p], A, V3<T>{T(P.planes[k][0]), T(P.planes[k][1]), T(P.planes[k][2])},
+                                  T(P.planes[k][3]), mg);
+                } else {
+                    ps[p] = collide(pm[p], A, shape_of(bb), mg);
+                }
             }
-            if (m.np > 0) {
-                man_a.push_back(a);
-                man_b.push_back(b);
-                man.push_back(m);
+        });
+        for (size_t p = 0; p < NP; ++p) {
+            if (ps[p] == GEOM_GJK_FAIL) ++st.gjk_fail;
+            if (ps[p] == GEOM_EPA_FAIL) ++st.epa_fail;
+            if (pm[p].np > 0) {
+                man_a.push_back(pair_a[p]);
+                man_b.push_back(pair_b[p]);
+                man.push_back(pm[p]);
             }
         }
     }
@@ -362,16 +439,18 @@ template <class T> struct World {
     void solve(T dt, uint32_t iters, const T* warm_in /* optional: M x 4 x 3 in manifold order */) {
         const uint32_t M = (uint32_t)man.size();
         std::vector<SBody> sb(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            SBody& b = sb[i];
-            b.v = v[i];
-            b.w = w[i];
-            b.im = im[i];
-            b.dyn = dynamic(i);
-            b.x = x[i];
-            const M3<T> R = quat_to_mat(q[i]);
-            b.I = world_inv_inertia(R, ii[i]);
-        }
+        par(n, 4096, [&](uint64_t, uint64_t b0, uint64_t b1) {
+            for (uint32_t i = (uint32_t)b0; i < (uint32_t)b1; ++i) {
+                SBody& b = sb[i];
+                b.v = v[i];
+                b.w = w[i];
+                b.im = im[i];
+                b.dyn = dynamic(i);
+                b.x = x[i];
+                const M3<T> R = quat_to_mat(q[i]);
+                b.I = world_inv_inertia(R, ii[i]);
+            }
+        });
         SBody ground{};
         ground.v = ground.w = ground.x = {T(0), T(0), T(0)};
         ground.im = T(0);
@@ -382,8 +461,12 @@ template <class T> struct World {
         feat.assign((size_t)M * 4 * 24, T(0));
         const T inv_dt = T(1) / dt;
         const T mu = T(P.friction);
+        const T soft1 = T(1.0f + P.softness);
+        const T keep = T(1) / soft1;
         st.warm_hits = 0;
-        for (uint32_t m = 0; m < M; ++m) {
+        std::vector<uint32_t> mhits(M, 0u);
+        par(M, 256, [&](uint64_t, uint64_t m0, uint64_t m1) {
+        for (uint32_t m = (uint32_t)m0; m < (uint32_t)m1; ++m) {
             const uint32_t a = man_a[m], b = man_b[m];
             const SBody& A = sb[a];
             const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
@@ -407,7 +490,7 @@ template <class T> struct World {
                     r.b[d] = cross(r.rB, dirs[d]);
                     r.ia[d] = smul(A.I, r.a[d]);
                     r.ib[d] = smul(B.I, r.b[d]);
-                    kk[d] = A.im + B.im + dot(r.a[d], r.ia[d]) + dot(r.b[d], r.ib[d]);
+                    kk[d] = (A.im + B.im + dot(r.a[d], r.ia[d]) + dot(r.b[d], r.ib[d])) * soft1;
                 }
                 r.kN = kk[0];
                 r.kT1 = kk[1];
@@ -438,7 +521,7 @@ template <class T> struct World {
                             Lp[1] = ce->lam[j][1];
                             Lp[2] = ce->lam[j][2];
                             hit = true;
-                            ++st.warm_hits;
+                            ++mhits[m];
                             break;
                         }
                 }
@@ -462,62 +545,68 @@ template <class T> struct World {
                 }
             }
         }
+        });
+        for (uint32_t m = 0; m < M; ++m) st.warm_hits += mhits[m];
         residual_hist.clear();
-        for (uint32_t it = 0; it < iters; ++it) {
-            for (uint32_t s = 0; s < M; ++s) {
-                const uint32_t m = solve_order[s];
-                const uint32_t a = man_a[m], b = man_b[m];
-                SBody& A = sb[a];
-                SBody& B = (b & PLANE_BIT) ? ground : sb[b];
-                const uint32_t np = man[m].np;
-                if (it == 0) {
-                    for (uint32_t k = 0; k < np; ++k) {
-                        const Row& r = rows[(size_t)m * 4 + k];
-                        const T* L = &lambda[((size_t)m * 4 + k) * 3];
-                        japply(r, 0, L[0], A, B);
-                        japply(r, 1, L[1], A, B);
-                        japply(r, 2, L[2], A, B);
-                    }
-                }
+        // one visit of solve slot s (manifold solve_order[s])
+        auto visit = [&](uint32_t s, bool first) {
+            const uint32_t m = solve_order[s];
+            const uint32_t a = man_a[m], b = man_b[m];
+            SBody& A = sb[a];
+            SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+            const uint32_t np = man[m].np;
+            if (first) {
                 for (uint32_t k = 0; k < np; ++k) {
                     const Row& r = rows[(size_t)m * 4 + k];
-                    T* L = &lambda[((size_t)m * 4 + k) * 3];
-                    const T maxf = mu * L[0];
-                    for (int dd = 1; dd <= 2; ++dd) {
-                        T nl = std::fma(-r.mass(dd), jvel(r, dd, A, B), L[dd]);
-                        nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
-                        const T d = nl - L[dd];
-                        L[dd] = nl;
-                        japply(r, dd, d, A, B);
-                    }
-                }
-                for (uint32_t k = 0; k < np; ++k) {
-                    const Row& r = rows[(size_t)m * 4 + k];
-                    T* L = &lambda[((size_t)m * 4 + k) * 3];
-                    T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), L[0]);
-                    nl = nl > T(0) ? nl : T(0);
-                    const T d = nl - L[0];
-                    L[0] = nl;
-                    japply(r, 0, d, A, B);
+                    const T* L = &lambda[((size_t)m * 4 + k) * 3];
+                    japply(r, 0, L[0], A, B);
+                    japply(r, 1, L[1], A, B);
+                    japply(r, 2, L[2], A, B);
                 }
             }
-            if (P.record_residual || it + 1 == iters) {
+            for (uint32_t k = 0; k < np; ++k) {
+                const Row& r = rows[(size_t)m * 4 + k];
+                T* L = &lambda[((size_t)m * 4 + k) * 3];
+                const T maxf = mu * L[0];
+                for (int dd = 1; dd <= 2; ++dd) {
+                    T nl = std::fma(-r.mass(dd), jvel(r, dd, A, B), keep * L[dd]);
+                    nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+                    const T d = nl - L[dd];
+                    L[dd] = nl;
+                    japply(r, dd, d, A, B);
+                }
+            }
+            for (uint32_t k = 0; k < np; ++k) {
+                const Row& r = rows[(size_t)m * 4 + k];
+                T* L = &lambda[((size_t)m * 4 + k) * 3];
+                T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), keep * L[0]);
+                nl = nl > T(0) ? nl : T(0);
+                const T d = nl - L[0];
+                L[0] = nl;
+                japply(r, 0, d, A, B);
+            }
+        };
+        // residual: the largest velocity-space change one more visit would make to any row
+        auto residual = [&]() {
+            const uint64_t RG = 1024;
+            std::vector<T> part(Pool::chunks_for(M, RG), T(0));
+            par(M, RG, [&](uint64_t ch, uint64_t m0, uint64_t m1) {
                 T res = T(0);
-                for (uint32_t m = 0; m < M; ++m) {
+                for (uint32_t m = (uint32_t)m0; m < (uint32_t)m1; ++m) {
                     const uint32_t a = man_a[m], b = man_b[m];
                     const SBody& A = sb[a];
                     const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
                     for (uint32_t k = 0; k < man[m].np; ++k) {
                         const Row& r = rows[(size_t)m * 4 + k];
                         const T* L = &lambda[((size_t)m * 4 + k) * 3];
-                        T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), L[0]);
+                        T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), keep * L[0]);
                         nl = nl > T(0) ? nl : T(0);
                         T e = std::fabs(nl - L[0]) * r.kN;
                         const T maxf = mu * L[0];
-                        T t1 = std::fma(-r.mT1, jvel(r, 1, A, B), L[1]);
+                        T t1 = std::fma(-r.mT1, jvel(r, 1, A, B), keep * L[1]);
                         t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
                         const T e1 = std::fabs(t1 - L[1]) * r.kT1;
-                        T t2 = std::fma(-r.mT2, jvel(r, 2, A, B), L[2]);
+                        T t2 = std::fma(-r.mT2, jvel(r, 2, A, B), keep * L[2]);
                         t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
                         const T e2 = std::fabs(t2 - L[2]) * r.kT2;
                         if (e1 > e) e = e1;
@@ -525,10 +614,57 @@ template <class T> struct World {
                         if (e > res) res = e;
                     }
                 }
+                part[ch] = res;
+            });
+            T res = T(0);
+            for (const T p : part)
+                if (p > res) res = p;
+            return res;
+        };
+        // colour batches: the manifolds of one colour share no dynamic body, so a batch's
+        // visits are independent and run in parallel without changing any result
+        const uint32_t K = colour_off.empty() ? 0u : (uint32_t)colour_off.size() - 1u;
+        const bool tol_mode = residual_tol > T(0);
+        const uint32_t cap = tol_mode ? std::max<uint32_t>(max_iters, 1u) : iters;
+        const uint32_t every = std::max<uint32_t>(res_every, 1u);
+        uint32_t ran = 0;
+        if (cap == 0) {
+            // no sweeps: apply the starting impulses (the warm start) and measure their residual
+            for (uint32_t s = 0; s < M; ++s) {
+                const uint32_t m = solve_order[s];
+                const uint32_t a = man_a[m], b = man_b[m];
+                SBody& A = sb[a];
+                SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+                for (uint32_t k = 0; k < man[m].np; ++k) {
+                    const Row& r = rows[(size_t)m * 4 + k];
+                    const T* L = &lambda[((size_t)m * 4 + k) * 3];
+                    japply(r, 0, L[0], A, B);
+                    japply(r, 1, L[1], A, B);
+                    japply(r, 2, L[2], A, B);
+                }
+            }
+            residual_hist.push_back(residual());
+        }
+        for (uint32_t it = 0; it < cap; ++it) {
+            for (uint32_t c = 0; c < K; ++c) {
+                const uint32_t s0 = colour_off[c], s1 = colour_off[c + 1];
+                if (s1 - s0 >= 512 && pool) {
+                    par(s1 - s0, 256, [&](uint64_t, uint64_t b0, uint64_t b1) {
+                        for (uint32_t s = s0 + (uint32_t)b0; s < s0 + (uint32_t)b1; ++s) visit(s, it == 0);
+                    });
+                } else {
+                    for (uint32_t s = s0; s < s1; ++s) visit(s, it == 0);
+                }
+            }
+            ran = it + 1;
+            const bool check = tol_mode && ran % every == 0;
+            if (P.record_residual || check || ran == cap) {
+                const T res = residual();
                 residual_hist.push_back(res);
+                if (check && res <= residual_tol) break;
             }
         }
-        st.iterations = iters;
+        st.iterations = ran;
         st.residual = residual_hist.empty() ? 0.0f : (float)residual_hist.back();
         st.residual_first = residual_hist.empty() ? 0.0f : (float)residual_hist.front();
         for (uint32_t i = 0; i < n; ++i) {
@@ -553,42 +689,62 @@ template <class T> struct World {
     void integrate_velocities(T dt) {
         const V3<T> g = {T(P.gravity[0]), T(P.gravity[1]), T(P.gravity[2])};
         const T kl = T(1) - T(P.lin_damp), ka = T(1) - T(P.ang_damp);
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!dynamic(i)) continue;
-            v[i] = (v[i] + g * dt) * kl;
-            w[i] = w[i] * ka;
-        }
+        par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
+            for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
+                if (!dynamic(i)) continue;
+                v[i] = (v[i] + g * dt) * kl;
+                w[i] = w[i] * ka;
+            }
+        });
     }
 
     void integrate_positions(T dt) {
         const T hdt = T(0.5) * dt;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!dynamic(i)) continue;
-            x[i] = x[i] + v[i] * dt;
-            const Q4<T> wq = {w[i].x, w[i].y, w[i].z, T(0)};
-            const Q4<T> dq = qmul(wq, q[i]);
-            Q4<T> nq = {q[i].x + dq.x * hdt, q[i].y + dq.y * hdt, q[i].z + dq.z * hdt, q[i].w + dq.w * hdt};
-            const T len = std::sqrt(nq.x * nq.x + nq.y * nq.y + nq.z * nq.z + nq.w * nq.w);
-            const T inv = T(1) / len;
-            q[i] = {nq.x * inv, nq.y * inv, nq.z * inv, nq.w * inv};
-        }
+        par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
+            for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
+                if (!dynamic(i)) continue;
+                x[i] = x[i] + v[i] * dt;
+                const Q4<T> wq = {w[i].x, w[i].y, w[i].z, T(0)};
+                const Q4<T> dq = qmul(wq, q[i]);
+                Q4<T> nq = {q[i].x + dq.x * hdt, q[i].y + dq.y * hdt, q[i].z + dq.z * hdt, q[i].w + dq.w * hdt};
+                const T len = std::sqrt(nq.x * nq.x + nq.y * nq.y + nq.z * nq.z + nq.w * nq.w);
+                const T inv = T(1) / len;
+                q[i] = {nq.x * inv, nq.y * inv, nq.z * inv, nq.w * inv};
+            }
+        });
     }
 
-    void energy(double& ke, double& pe) const {
+    // Kinetic energy (translational + rotational about the centre, body-frame
+    // principal inertia) and potential energy -m g.x of the dynamic bodies, in
+    // double; summed per 4096-body range, ranges in order.
+    void energy(double& ke, double& pe) {
+        const uint64_t G = 4096;
+        const uint64_t nch = Pool::chunks_for(n, G);
+        std::vector<double> pk(nch, 0.0), pp(nch, 0.0);
+        par(n, G, [&](uint64_t ch, uint64_t b, uint64_t e) {
+            double k = 0.0, u = 0.0;
+            for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
+                if (!dynamic(i)) continue;
+                const double m = 1.0 / (double)im[i];
+                const double vx = v[i].x, vy = v[i].y, vz = v[i].z;
+                k += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+                const M3<T> R = quat_to_mat(q[i]);
+                const V3<T> wl = mulT(R, w[i]);
+                const double I[3] = {ii[i].x > 0 ? 1.0 / (double)ii[i].x : 0.0,
+                                     ii[i].y > 0 ? 1.0 / (double)ii[i].y : 0.0,
+                                     ii[i].z > 0 ? 1.0 / (double)ii[i].z : 0.0};
+                k += 0.5 * (I[0] * (double)wl.x * (double)wl.x + I[1] * (double)wl.y * (double)wl.y +
+                            I[2] * (double)wl.z * (double)wl.z);
+                u -= m * ((double)P.gravity[0] * (double)x[i].x + (double)P.gravity[1] * (double)x[i].y +
+                          (double)P.gravity[2] * (double)x[i].z);
+            }
+            pk[ch] = k;
+            pp[ch] = u;
+        });
         ke = pe = 0.0;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!dynamic(i)) continue;
-            const double m = 1.0 / (double)im[i];
-            const double vx = v[i].x, vy = v[i].y, vz = v[i].z;
-            ke += 0.5 * m * (vx * vx + vy * vy + vz * vz);
-            const M3<T> R = quat_to_mat(q[i]);
-            const V3<T> wl = mulT(R, w[i]);
-            const double I[3] = {ii[i].x > 0 ? 1.0 / (double)ii[i].x : 0.0, ii[i].y > 0 ? 1.0 / (double)ii[i].y : 0.0,
-                                 ii[i].z > 0 ? 1.0 / (double)ii[i].z : 0.0};
-            ke += 0.5 * (I[0] * (double)wl.x * (double)wl.x + I[1] * (double)wl.y * (double)wl.y +
-                         I[2] * (double)wl.z * (double)wl.z);
-            pe -= m * ((double)P.gravity[0] * (double)x[i].x + (double)P.gravity[1] * (double)x[i].y +
-                       (double)P.gravity[2] * (double)x[i].z);
+        for (uint64_t c = 0; c < pk.size(); ++c) {
+            ke += pk[c];
+            pe += pp[c];
         }
     }
 

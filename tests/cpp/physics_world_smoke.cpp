@@ -21,6 +21,11 @@ struct RigidBodyLike {
     int shapeType = 0;
     V3 shapeExtents{1, 1, 1};
 };
+// The same with gui::RigidBodyData's body type (include/axiom/gui/body_inspector.hpp:13-17).
+enum class BodyTypeLike : uint32_t { Static = 0, Dynamic = 1, Kinematic = 2 };
+struct TypedBodyLike : RigidBodyLike {
+    BodyTypeLike type = BodyTypeLike::Dynamic;
+};
 struct StatsLike {
     uint32_t totalBodies, activeBodies, sleepingBodies, staticBodies, dynamicBodies, kinematicBodies;
     uint32_t contactPointCount, constraintCount, islandCount;
@@ -46,6 +51,21 @@ int main() {
     rb.shapeExtents = {1, 1, 1};
     check(bodies.addRigidBody(rb).ok(), "addRigidBody");
     check(bodies.size() == 2 && bodies.inverseMass(1) == 0.0f, "static body");
+    // a Static body keeps zero inverse mass although its mass field holds the default 1 kg
+    TypedBodyLike tb;
+    tb.position = {-3, 0.4f, 0};
+    tb.shapeType = 1;
+    tb.shapeExtents = {0.5f, 0.4f, 0.5f};
+    tb.type = BodyTypeLike::Static;
+    check(tb.mass == 1.0f && bodies.addRigidBody(tb).ok(), "addRigidBody static");
+    check(bodies.size() == 3 && bodies.inverseMass(2) == 0.0f, "static type overrides mass");
+    TypedBodyLike kin = tb;
+    kin.type = BodyTypeLike::Kinematic;
+    check(bodies.addRigidBody(kin).code() == AXM_INVALID_RIGID_BODY && bodies.size() == 3, "kinematic rejected");
+    TypedBodyLike dyn0 = tb;
+    dyn0.type = BodyTypeLike::Dynamic;
+    dyn0.mass = 0.0f;
+    check(bodies.addRigidBody(dyn0).code() == AXM_INVALID_RIGID_BODY, "dynamic needs mass");
     const float m = 1000.0f * 4.0f / 3.0f * 3.14159265f * 0.125f;
     check(std::fabs(1.0f / bodies.inverseMass(0) - m) < 1e-3f * m, "sphere mass");
 
@@ -61,15 +81,16 @@ int main() {
     check(world != nullptr, "create");
     check(world->setBodies(bodies).ok(), "setBodies");
     for (int k = 0; k < 60 && world; ++k) check(world->step(1.0f / 120.0f, 10).ok(), "step");
-    check(world->run(1.0f / 120.0f, 10, 30, true).ok(), "run graph");
+    check(world->run(1.0f / 120.0f, 10, 240, true).ok(), "run graph");
     BodySet out;
-    check(world->getBodies(out).ok() && out.size() == 2, "getBodies");
+    check(world->getBodies(out).ok() && out.size() == 3, "getBodies");
+    check(out.position(2)[1] == 0.4f && out.linearVelocity(2)[1] == 0.0f, "static box stays put");
     check(std::fabs(out.linearVelocity(0)[1]) < 1e-3f, "sphere at rest");
     check(std::fabs(out.position(0)[1] - 0.5f) < 0.01f, "sphere on the ground");
     PhysicsWorld::Manifolds mf;
     check(world->contacts(mf).ok() && mf.a.size() == 1 && mf.npts[0] == 1, "one contact");
     StatsLike st{};
-    world->fillStats(st, 2, 1);
+    world->fillStats(st, 3, 2);
     check(st.contactPointCount == 1 && st.dynamicBodies == 1, "stats");
     PhysicsWorld moved(std::move(*world));
     check(moved.handle() != nullptr && world->handle() == nullptr, "move");

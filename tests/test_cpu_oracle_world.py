@@ -82,12 +82,15 @@ def test_torque_free_rotation_closed_form(orc, axm):
     assert abs(np.linalg.norm(quat[0]) - 1) < 1e-6
 
 
-def test_resting_sphere_normal_impulse_is_weight(orc, axm):
-    """A sphere resting on the ground: once settled, lambda_n = m g dt and it stays put."""
+@pytest.mark.parametrize("soft", [0.0, 0.1])
+def test_resting_sphere_normal_impulse_is_weight(orc, axm, soft):
+    """A sphere resting on the ground: once settled, lambda_n = m g dt and it stays put.  With rigid
+    rows any depth inside the slop is a rest state; a soft row (compliance eps k) only holds still
+    where the Baumgarte target equals eps k lambda_n = eps g dt, i.e. at depth slop + eps g dt^2 / beta."""
     r = 0.5
     b = make(axm, [(0, [r], [0, r, 0], [0, 0, 0], [0, 0, 0], False)])
-    ow, _ = world(orc, axm, b)
-    for _ in range(60):
+    ow, d = world(orc, axm, b, softness=soft)
+    for _ in range(300):
         ow.step(DT, 10)
     lam = ow.impulses()
     m = 1.0 / b.inv_mass[0]
@@ -95,14 +98,22 @@ def test_resting_sphere_normal_impulse_is_weight(orc, axm):
     assert abs(lam[0, 0, 0] - m * 9.81 * DT) / (m * 9.81 * DT) < 1e-4
     pos, quat, v, w = ow.get_state()
     assert np.abs(v[0]).max() < 1e-4
-    assert abs(pos[0, 1] - r) < 0.006  # within the slop
+    depth = r - pos[0, 1]
+    if soft == 0.0:
+        assert -1e-6 <= depth <= d.slop + 1e-6
+    else:
+        want = d.slop + soft * 9.81 * DT * DT / d.baumgarte
+        assert abs(depth - want) < 2e-6, (depth, want)
 
 
-def test_sliding_box_decelerates_at_mu_g(orc, axm):
-    """Friction saturates while sliding: |lambda_t| = mu lambda_n, so v drops by mu g dt per step."""
+@pytest.mark.parametrize("soft", [0.0, 0.1])
+def test_sliding_box_decelerates_at_mu_g(orc, axm, soft):
+    """Friction saturates while sliding: |lambda_t| = mu lambda_n, so v drops by mu g dt per step.  The
+    box starts at its resting depth (slop + eps g dt^2 / beta for soft rows), where lambda_n = m g dt."""
     mu, v0 = 0.5, 4.0
-    b = make(axm, [(1, [0.5, 0.25, 0.5], [0, 0.25, 0], [v0, 0, 0], [0, 0, 0], False)])
-    ow, _ = world(orc, axm, b, friction=mu, restitution=0.0)
+    d0 = 0.005 + soft * 9.81 * DT * DT / 0.2
+    b = make(axm, [(1, [0.5, 0.25, 0.5], [0, 0.25 - d0, 0], [v0, 0, 0], [0, 0, 0], False)])
+    ow, _ = world(orc, axm, b, friction=mu, restitution=0.0, softness=soft)
     for _ in range(5):
         ow.step(DT, 20)
     _, _, v_a, _ = ow.get_state()
@@ -303,22 +314,27 @@ def test_warm_start_matches_features_across_frames(orc, axm):
     assert lam1[0, :4, 0].min() > 0
 
 
-def test_restitution_closed_form(orc, axm):
-    """A sphere hitting the ground at v > rest_threshold leaves at e * v (single frictionless point)."""
+@pytest.mark.parametrize("soft", [0.0, 0.1])
+def test_restitution_closed_form(orc, axm, soft):
+    """A sphere hitting the ground at v > rest_threshold leaves at e * v (single frictionless point).
+    A soft row converges to v' = v + (target - v) / (1 + eps) with target = -e v."""
     e, v = 0.4, -3.0
     b = make(axm, [(0, [0.5], [0, 0.5, 0], [0, v, 0], [0, 0, 0], False)])
-    ow, d = world(orc, axm, b, restitution=e, friction=0.0, gravity=[0, 0, 0])
+    ow, d = world(orc, axm, b, restitution=e, friction=0.0, gravity=[0, 0, 0], softness=soft)
     ow.step(DT, 10)
     _, _, vel, _ = ow.get_state()
-    assert abs(vel[0, 1] - (-e * v)) < 1e-5
+    want = v + (-e * v - v) / (1.0 + soft)
+    assert abs(vel[0, 1] - want) < 1e-5, (vel[0, 1], want)
 
 
-def test_baumgarte_bias_closed_form(orc, axm):
-    """A sphere resting at depth d > slop is pushed out at beta / dt * (d - slop), capped at max_bias."""
+@pytest.mark.parametrize("soft", [0.0, 0.1])
+def test_baumgarte_bias_closed_form(orc, axm, soft):
+    """A sphere resting at depth d > slop is pushed out at beta / dt * (d - slop), capped at max_bias
+    (divided by 1 + eps for a soft row starting at rest)."""
     for depth in (0.02, 0.5):
         b = make(axm, [(0, [0.5], [0, 0.5 - depth, 0], [0, 0, 0], [0, 0, 0], False)])
-        ow, d = world(orc, axm, b, gravity=[0, 0, 0], friction=0.0)
+        ow, d = world(orc, axm, b, gravity=[0, 0, 0], friction=0.0, softness=soft)
         ow.step(DT, 10)
         _, _, vel, _ = ow.get_state()
-        want = min(d.baumgarte / np.float32(DT) * (depth - d.slop), d.max_bias)
+        want = min(d.baumgarte / np.float32(DT) * (depth - d.slop), d.max_bias) / (1.0 + soft)
         assert abs(vel[0, 1] - want) <= 1e-4 * max(1.0, want), (depth, vel[0, 1], want)
