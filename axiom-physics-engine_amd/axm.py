@@ -43,7 +43,9 @@ class WorldDesc(C.Structure):
         ("rest_threshold", C.c_float), ("baumgarte", C.c_float), ("slop", C.c_float), ("max_bias", C.c_float),
         ("margin", C.c_float), ("lin_damp", C.c_float), ("ang_damp", C.c_float), ("cell_size", C.c_float),
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
-        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float), ("max_bodies", C.c_uint32),
+        ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float),
+        ("allow_sleep", C.c_uint32), ("sleep_linear", C.c_float), ("sleep_angular", C.c_float),
+        ("sleep_time", C.c_float), ("max_bodies", C.c_uint32),
         ("max_pairs", C.c_uint32), ("max_manifolds", C.c_uint32), ("auto_grow", C.c_uint32),
         ("residual_tol", C.c_float), ("residual_every", C.c_uint32), ("max_iterations", C.c_uint32),
         ("device", C.c_int32), ("solver_mode", C.c_uint32),
@@ -66,7 +68,7 @@ class StepStats(C.Structure):
         ("residual_first", C.c_float), ("gjk_fail", C.c_uint32), ("epa_fail", C.c_uint32),
         ("warm_hits", C.c_uint32), ("overflow", C.c_uint32), ("broadphase_ms", C.c_float),
         ("narrowphase_ms", C.c_float), ("solver_ms", C.c_float), ("integration_ms", C.c_float),
-        ("total_ms", C.c_float),
+        ("total_ms", C.c_float), ("islands", C.c_uint32), ("sleeping", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -96,7 +98,7 @@ class SweepConfig(C.Structure):
 # Exported C-ABI symbols (the header's declarations).
 EXPORTS = [
     "axm_device_count", "axm_status_string", "axm_world_desc_default", "axm_world_create", "axm_world_destroy", "axm_world_set_bodies",
-    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals",
+    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
@@ -134,6 +136,7 @@ def lib():
     L.axm_world_sync.argtypes = [P]
     L.axm_world_profile.argtypes = [P, C.c_int, C.POINTER(KernelProfile)]
     L.axm_world_get_totals.argtypes = [P, C.POINTER(WorldTotals), C.c_int]
+    L.axm_world_get_islands.argtypes = [P, u32p, u32p, f32p]
     L.axm_world_get_pairs.argtypes = [P, u32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_manifolds.argtypes = [P, u32p, u32p, u32p, f32p, f32p, f32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_colours.argtypes = [P, u32p, u32p, C.c_uint32, u32p, u32p]
@@ -387,6 +390,16 @@ class World:
         p = KernelProfile()
         _check(self._L.axm_world_profile(self._h, 1 if enable else 0, C.byref(p)), "axm_world_profile")
         return p
+
+    def islands(self):
+        """(label, asleep, rest time) of every body after the last step."""
+        n = self.n
+        lab = np.zeros(n, np.uint32)
+        sl = np.zeros(n, np.uint32)
+        rest = np.zeros(n, np.float32)
+        _check(self._L.axm_world_get_islands(self._h, _p(lab, C.c_uint32), _p(sl, C.c_uint32), _p(rest)),
+               "axm_world_get_islands")
+        return lab, sl, rest
 
     def totals(self, reset=False):
         """Device-counted totals since the last reset (steps, steps at tolerance, sweeps, contact x sweeps,

@@ -44,7 +44,7 @@ template <class T> __device__ __forceinline__ T spread_sum(const Counters* cnt, 
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
                                                uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
                                                uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
-                                               uint32_t* __restrict__ bdeg) {
+                                               uint32_t* __restrict__ bdeg, const Counters* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // the colouring's per-body words start from zero every step
@@ -52,19 +52,33 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
     bmax1[i] = 0ull;
     bmask[i] = 0ull;
     bdeg[i] = 0u;
+    // islands start as singletons; a sleeping body whose island an awake body touched in
+    // the previous step wakes up (the request carries the number of this step)
+    B.upar[i] = i;
+    B.imin[i] = 0x7F800000u;  // +inf
+    bool asleep = false;
+    {
+        const uint2 sl = B.slp[i];
+        if (sl.y & ASLEEP_BIT) {
+            const uint32_t lab = sl.y & LABEL_MASK;
+            if (lab < n && B.wake[lab] == cnt->steps_done) B.slp[i] = make_uint2(0u, lab);
+            else asleep = true;
+        }
+    }
     const float4 x4 = B.x[i], q4v = B.q[i], v4 = B.vel[i], w4 = B.angv[i], ii = B.ii[i], sh = B.shape[i];
     const uint32_t type = fbits(sh.w);
-    const bool dyn = v4.w > 0.0f;
+    // a sleeping body acts as a static one for the whole step
+    const bool dyn = v4.w > 0.0f && !asleep;
     f3 v = xyz(v4), w = xyz(w4);
     if (dyn) {
         const f3 g = {sp.gx, sp.gy, sp.gz};
         v = (v + g * sp.dt) * sp.lin_keep;
         w = w * sp.ang_keep;
     }
-    B.sv[i] = f4(v, v4.w);
+    B.sv[i] = f4(v, dyn ? v4.w : 0.0f);
     B.sw[i] = f4(w, 0.0f);
     const m3 R = quat_to_mat(q4v);
-    const s3 I = world_inv_inertia(R, xyz(ii));
+    const s3 I = dyn ? world_inv_inertia(R, xyz(ii)) : s3{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     B.I0[i] = make_float4(I.xx, I.yy, I.zz, I.xy);
     B.I1[i] = make_float4(I.xz, I.yz, 0.0f, 0.0f);
     const float mg = sp.margin;
@@ -235,7 +249,7 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     const uint32_t i = sval[t];
     const float4 mn = B.amin[i], mx = B.amax[i];
     B.smin[t] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
-    B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.vel[i].w > 0.0f ? 1.0f : 0.0f);
+    B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.sv[i].w > 0.0f ? 1.0f : 0.0f);
 }
 
 // The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
@@ -580,7 +594,7 @@ __global__ void __launch_bounds__(256) k_compact(const uint32_t* __restrict__ pa
 
 // ---------------------------------------------------------------- colouring
 __device__ __forceinline__ bool dyn_body(const Bodies& B, uint32_t b) {
-    return !(b & PLANE_BIT) && B.vel[b].w > 0.0f;
+    return !(b & PLANE_BIT) && B.sv[b].w > 0.0f;
 }
 __device__ __forceinline__ uint32_t local_id(const Bodies& B, uint32_t b) {
     return (b & PLANE_BIT) ? b : b - B.world_first[B.world[b]];
@@ -2090,6 +2104,8 @@ __global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ kinds) {
     uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
     if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
     for (uint32_t k = threadIdx.x; k < SPREAD; k += blockDim.x) {
+        cnt->sp[k].islands = 0u;
+        cnt->sp[k].sleeping = 0u;
         cnt->sp[k].contacts = 0u;
         cnt->sp[k].warm_hits = 0u;
         cnt->sp[k].cells = 0u;
@@ -2135,6 +2151,104 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
     B.angv[i] = make_float4(w.x, w.y, w.z, 0.0f);
 }
 
+// ---------------------------------------------------------------- islands and sleeping
+// Union-find over the step's manifolds between two awake dynamic bodies.  Every link
+// points from a larger root to a smaller one (compare-and-swap on the root), so the
+// root of a finished component is its smallest body index whatever the order of the
+// links (lock-free union-find with path halving: Jaiganesh & Burtscher 2018, ECL-CC).
+// The parents are shared between workgroups, so they move through agent-scope atomics.
+__device__ __forceinline__ uint32_t uf_load(uint32_t* p, uint32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
+    uint32_t cur = uf_load(par, x);
+    if (cur != x) {
+        uint32_t prev = x, next;
+        while (cur > (next = uf_load(par, cur))) {
+            __hip_atomic_store(par + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            prev = cur;
+            cur = next;
+        }
+    }
+    return cur;
+}
+__global__ void __launch_bounds__(256) k_isl_union(Bodies B, const uint2* __restrict__ m_ab, uint32_t cap,
+                                                   uint32_t n, const Counters* __restrict__ cnt) {
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = min(cap, cnt->m_eff);
+    const uint32_t stamp = cnt->steps_done;  // the integration already counted this step
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint2 ab = m_ab[m];
+        if (ab.y & PLANE_BIT) continue;
+        const bool da = B.sv[ab.x].w > 0.0f, db = B.sv[ab.y].w > 0.0f;
+        if (da && db) {
+            uint32_t ra = uf_find(B.upar, ab.x), rb = uf_find(B.upar, ab.y);
+            while (ra != rb) {
+                if (ra < rb) {
+                    const uint32_t old = atomicCAS(B.upar + rb, rb, ra);
+                    if (old == rb) break;
+                    rb = uf_find(B.upar, old);
+                } else {
+                    const uint32_t old = atomicCAS(B.upar + ra, ra, rb);
+                    if (old == ra) break;
+                    ra = uf_find(B.upar, old);
+                }
+            }
+        } else if (da || db) {
+            // an awake dynamic body touches a sleeping one: its island wakes at the next step
+            const uint32_t o = da ? ab.y : ab.x;
+            const uint32_t y = B.slp[o].y;
+            if ((y & ASLEEP_BIT) && (y & LABEL_MASK) < n) B.wake[y & LABEL_MASK] = stamp;
+        }
+    }
+}
+// Labels, rest times and each island's smallest rest time; counts islands and sleeping bodies.
+__global__ void __launch_bounds__(256) k_isl_label(Bodies B, StepParams sp, uint32_t n, Counters* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt->sticky != 0u || i >= n) return;
+    uint32_t isl = 0, slp = 0;
+    if (B.sv[i].w > 0.0f) {
+        uint32_t r = i, p;
+        while ((p = B.upar[r]) != r) r = p;  // after the union kernel: a finished forest
+        const float4 v = B.vel[i], w = B.angv[i];
+        const bool still = v.x * v.x + v.y * v.y + v.z * v.z < sp.sleep_lin2 &&
+                           w.x * w.x + w.y * w.y + w.z * w.z < sp.sleep_ang2;
+        const float t = still ? __uint_as_float(B.slp[i].x) + sp.dt : 0.0f;
+        B.slp[i] = make_uint2(__float_as_uint(t), r);
+        atomicMin(B.imin + r, __float_as_uint(t));  // rest times are >= 0: bit order = value order
+        isl = r == i ? 1u : 0u;
+    } else if (B.vel[i].w > 0.0f) {
+        // asleep (a static body has no inverse mass)
+        const uint32_t y = B.slp[i].y;
+        isl = (y & LABEL_MASK) == i ? 1u : 0u;
+        slp = 1u;
+    }
+    const uint32_t row = (blockIdx.x * 4u + (threadIdx.x >> 6)) % SPREAD;
+    const uint32_t wi = __popcll(__ballot(isl != 0u)), ws = __popcll(__ballot(slp != 0u));
+    if ((threadIdx.x & 63u) == 0u) {
+        if (wi) atomicAdd(&cnt->sp[row].islands, wi);
+        if (ws) atomicAdd(&cnt->sp[row].sleeping, ws);
+    }
+}
+// Islands whose bodies have all rested sleep_time seconds fall asleep.
+__global__ void __launch_bounds__(256) k_isl_sleep(Bodies B, StepParams sp, uint32_t n, Counters* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cnt->sticky != 0u || i >= n) return;
+    bool now = false;
+    if (sp.allow_sleep && B.sv[i].w > 0.0f) {
+        const uint2 sl = B.slp[i];
+        if (B.imin[sl.y] >= __float_as_uint(sp.sleep_time)) {
+            B.slp[i] = make_uint2(sl.x, sl.y | ASLEEP_BIT);
+            B.vel[i] = make_float4(0.0f, 0.0f, 0.0f, B.vel[i].w);
+            B.angv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            now = true;
+        }
+    }
+    const uint32_t row = (blockIdx.x * 4u + (threadIdx.x >> 6)) % SPREAD;
+    const uint32_t ws = __popcll(__ballot(now));
+    if ((threadIdx.x & 63u) == 0u && ws) atomicAdd(&cnt->sp[row].sleeping, ws);
+}
+
 // ---------------------------------------------------------------- host launchers
 static inline dim3 grid_for(uint32_t n, uint32_t b = 256) { return dim3((n + b - 1) / b ? (n + b - 1) / b : 1); }
 // Kernels whose element count is read on the device stride over it with at most
@@ -2149,8 +2263,8 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg);
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg, cnt);
 }
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u64, grid_for(n), dim3(256), 0, s, p, v, n);
@@ -2329,4 +2443,11 @@ void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counter
     hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n, cnt, sweeps);
 }
 
+void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, const uint2* m_ab, uint32_t cap_man,
+                    Counters* cnt, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_isl_union, grid_bounded(cap_man), dim3(256), 0, s, B, m_ab, cap_man, n, cnt);
+    hipLaunchKernelGGL(k_isl_label, grid_for(n), dim3(256), 0, s, B, sp, n, cnt);
+    hipLaunchKernelGGL(k_isl_sleep, grid_for(n), dim3(256), 0, s, B, sp, n, cnt);
+}
 }  // namespace axm

@@ -46,7 +46,12 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 // Start of a step: velocities, world inertia, AABBs, cell keys; clears the
 // colouring's per-body words.
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, hipStream_t s);
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, hipStream_t s);
+// End of a step (after the integration): island labels by union-find over the step's
+// manifolds between awake dynamic bodies (label = smallest body index), wake requests
+// for sleeping islands an awake body touched, rest times, and islands falling asleep.
+void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, const uint2* m_ab, uint32_t cap_man,
+                    Counters* cnt, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);

@@ -28,6 +28,8 @@ struct StepParams {
     float dt, inv_dt;
     float friction, restitution, rest_threshold;
     float soft1, keep;  // 1 + softness and its inverse (regularised rows)
+    uint32_t allow_sleep;
+    float sleep_lin2, sleep_ang2, sleep_time;  // squared thresholds (m/s, rad/s), s
     float baumgarte, slop, max_bias, margin;
     float lin_keep, ang_keep;
     float inv_cell;
@@ -54,7 +56,14 @@ struct Bodies {
     float4* smin;  // AABBs in sorted (cell) order, w = body index bits
     float4* smax;  // w = 1 for a dynamic body
     uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)
+    // islands and sleeping: per body (rest time bits, asleep bit 31 | island label); per step the
+    // union-find parents and each island's smallest rest time; per label the step it wakes at
+    uint2* slp;
+    uint32_t* upar;
+    uint32_t* imin;
+    uint32_t* wake;
 };
+constexpr uint32_t ASLEEP_BIT = 0x80000000u, LABEL_MASK = 0x7FFFFFFFu, LABEL_NONE = 0x7FFFFFFFu;
 // The broadphase counts and emits each body's pairs in three parts, one per z
 // slice of its 27 neighbour cells (the planes go with the first), one thread each.
 constexpr uint32_t BP_PARTS = 3;
@@ -88,7 +97,9 @@ struct SpreadRow {
     uint32_t cells;      // broadphase: occupied cells
     uint32_t pad;
     unsigned long long pgs_bytes;  // algorithmic bytes of one sweep (all colours)
-    uint32_t fill[26];
+    uint32_t islands;    // islands of dynamic bodies after the step
+    uint32_t sleeping;   // bodies asleep after the step
+    uint32_t fill[24];
 };
 static_assert(sizeof(SpreadRow) == 128, "one line per row");
 

@@ -229,6 +229,10 @@ void axm_world_desc_default(axm_world_desc* d) {
     d->warm_start = AXM_WARM_CACHE;
     d->record_residual = 0;
     d->softness = 0.1f;
+    d->allow_sleep = 0;
+    d->sleep_linear = 0.01f;
+    d->sleep_angular = 0.01f;
+    d->sleep_time = 0.5f;
     d->auto_grow = 1;
     d->residual_tol = 0.0f;
     d->residual_every = 1;

@@ -203,6 +203,10 @@ void free_bodies(axm_world_s* w) {
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
     w->mem.release(B.vslot);
+    w->mem.release(B.slp);
+    w->mem.release(B.upar);
+    w->mem.release(B.imin);
+    w->mem.release(B.wake);
     w->mem.release(B.world);
     w->mem.release(B.world_first);
     w->mem.release(w->key);
@@ -231,6 +235,10 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * BP_PARTS * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
+    AXM_TRY(w->mem.get(B.slp, cap));
+    AXM_TRY(w->mem.get(B.upar, cap));
+    AXM_TRY(w->mem.get(B.imin, cap));
+    AXM_TRY(w->mem.get(B.wake, cap));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
     AXM_TRY(w->mem.get(w->key, cap));
@@ -443,6 +451,10 @@ StepParams make_params(const axm_world_s* w, float dt) {
     sp.rest_threshold = d.rest_threshold;
     sp.soft1 = 1.0f + d.softness;
     sp.keep = 1.0f / sp.soft1;
+    sp.allow_sleep = d.allow_sleep;
+    sp.sleep_lin2 = d.sleep_linear * d.sleep_linear;
+    sp.sleep_ang2 = d.sleep_angular * d.sleep_angular;
+    sp.sleep_time = d.sleep_time;
     sp.baumgarte = d.baumgarte;
     sp.slop = d.slop;
     sp.max_bias = d.max_bias;
@@ -508,7 +520,7 @@ struct StepOpts {
 void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
-    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, s);
+    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, w->cnt, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s, nullptr,
                                    &w->cnt->sticky);
@@ -683,6 +695,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
     launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
+    launch_islands(w->B, sp, n, w->m_ab, w->cap_man, w->cnt, s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[4], s));
     if (w->prof_on && !o.capture) {
         AXM_HIP(hipStreamSynchronize(s));
@@ -719,6 +732,8 @@ void fill_stats(axm_world_s* w, axm_step_stats& S) {
     S.epa_fail = h.epa_fail;
     S.warm_hits = spread_total(h, &SpreadRow::warm_hits);
     S.overflow = (h.sticky & STICKY_PAIRS ? 1u : 0u) | (h.sticky & STICKY_MANIFOLDS ? 2u : 0u);
+    S.islands = spread_total(h, &SpreadRow::islands);
+    S.sleeping = spread_total(h, &SpreadRow::sleeping);
 }
 
 // Handles a sticky failure of the step just run: grows capacities or switches
@@ -1017,6 +1032,10 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         AXM_HIP(hipMemcpyAsync(w->B.shape, sh.data(), 16ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.world, wd.data(), 4ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * (nw + 1), hipMemcpyHostToDevice, s));
+        // every body starts awake with no rest time and no island; no island has a wake request
+        const std::vector<uint2> slp(n, make_uint2(0u, LABEL_NONE));
+        AXM_HIP(hipMemcpyAsync(w->B.slp, slp.data(), 8ull * n, hipMemcpyHostToDevice, s));
+        AXM_HIP(hipMemsetAsync(w->B.wake, 0xFF, 4ull * n, s));
     }
     AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
     // one large world: occupancy bits at the cells' Morton codes (cleared here in full,
@@ -1193,6 +1212,22 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         axm_step_stats S{};
         fill_stats(w, S);
         *last = S;
+    }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_get_islands(axm_world w, uint32_t* label, uint32_t* asleep, float* rest_time) {
+    if (!w) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    const uint32_t n = w->n;
+    if (!n) return AXM_SUCCESS;
+    std::vector<uint2> h(n);
+    AXM_HIP(hipMemcpyAsync(h.data(), w->B.slp, 8ull * n, hipMemcpyDeviceToHost, w->stream));
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (label) label[i] = h[i].y & LABEL_MASK;
+        if (asleep) asleep[i] = (h[i].y & ASLEEP_BIT) ? 1u : 0u;
+        if (rest_time) std::memcpy(rest_time + i, &h[i].x, 4);
     }
     return AXM_SUCCESS;
 }

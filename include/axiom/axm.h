@@ -70,7 +70,7 @@ enum {
 
 /* World description.  Stands in for gui::PhysicsWorldConfig
  * (include/axiom/gui/physics_panel.hpp:34-43) plus the solver constants the
- * reference leaves unspecified.  The leading fields up to `softness`
+ * reference leaves unspecified.  The leading fields up to `sleep_time`
  * have the same layout as the CPU oracle's parameter block. */
 typedef struct axm_world_desc {
     float gravity[3];
@@ -92,6 +92,12 @@ typedef struct axm_world_desc {
     float softness;        /* relative compliance eps of every contact row: a row solves v + eps*k*lambda = target
                               (k: the row's inverse effective mass).  0 = rigid rows; > 0 makes each step's
                               contact problem strictly convex, so the sweeps converge to any residual */
+    uint32_t allow_sleep;  /* PhysicsWorldConfig::allowSleep: an island whose bodies all rest (|v| <
+                              sleep_linear, |w| < sleep_angular) for sleep_time seconds falls asleep and
+                              acts as static until an awake body touches it (physics_panel.hpp:39-42) */
+    float sleep_linear;    /* PhysicsWorldConfig::sleepLinearThreshold (m/s) */
+    float sleep_angular;   /* PhysicsWorldConfig::sleepAngularThreshold (rad/s) */
+    float sleep_time;      /* PhysicsWorldConfig::sleepTimeThreshold (s) */
     /* capacities (0 = sized from the body count) */
     uint32_t max_bodies;
     uint32_t max_pairs;
@@ -142,6 +148,8 @@ typedef struct axm_step_stats {
     float solver_ms;        /* PhysicsWorldStats::solverTime */
     float integration_ms;   /* PhysicsWorldStats::integrationTime */
     float total_ms;         /* PhysicsWorldStats::totalStepTime */
+    uint32_t islands;       /* PhysicsWorldStats::islandCount: islands of dynamic bodies, awake and asleep */
+    uint32_t sleeping;      /* PhysicsWorldStats::sleepingBodies */
 } axm_step_stats;
 
 /* Kernel-level timing of the solver sweep kernel, accumulated over steps run
@@ -186,6 +194,13 @@ axm_status axm_world_step(axm_world w, float dt, uint32_t iters, uint32_t flags,
  * sweep launch checks the residual itself). */
 axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, int graph, axm_step_stats* last);
 axm_status axm_world_sync(axm_world w);
+/* Islands and sleep state after the last step (the reference's SleepInfo and island
+ * statistics: include/axiom/gui/body_inspector.hpp:45-51, physics_panel.hpp:23): per body
+ * the island label (the smallest body index of its island; 0x7FFFFFFF for static bodies and
+ * bodies that have not been in a step yet), whether it is asleep, and its rest time in
+ * seconds (SleepInfo::sleepTime).  NULL outputs are skipped. */
+axm_status axm_world_get_islands(axm_world w, uint32_t* label, uint32_t* asleep, float* rest_time);
+
 /* Totals over the steps run since the last reset (counted on the device, so graph-replayed
  * steps need no host round trip): steps, steps whose sweeps stopped because the residual met
  * residual_tol, sweeps, contact points x sweeps (the per-step contact count times that step's

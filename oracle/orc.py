@@ -21,6 +21,8 @@ class Params(C.Structure):
         ("margin", C.c_float), ("lin_damp", C.c_float), ("ang_damp", C.c_float), ("cell_size", C.c_float),
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
         ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float),
+        ("allow_sleep", C.c_uint32), ("sleep_linear", C.c_float), ("sleep_angular", C.c_float),
+        ("sleep_time", C.c_float),
     ]
 
 
@@ -29,7 +31,7 @@ class Stats(C.Structure):
         ("pairs", C.c_uint32), ("manifolds", C.c_uint32), ("contacts", C.c_uint32), ("colours", C.c_uint32),
         ("rounds", C.c_uint32), ("iterations", C.c_uint32), ("residual", C.c_float), ("residual_first", C.c_float),
         ("kinetic", C.c_double), ("potential", C.c_double), ("gjk_fail", C.c_uint32), ("epa_fail", C.c_uint32),
-        ("warm_hits", C.c_uint32),
+        ("warm_hits", C.c_uint32), ("islands", C.c_uint32), ("sleeping", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -61,6 +63,7 @@ def lib():
     L.orc_create.argtypes = [C.POINTER(Params), C.c_int]
     L.orc_destroy.argtypes = [P]
     L.orc_set_threads.argtypes = [P, C.c_uint32]
+    L.orc_get_islands.argtypes = [P, u32p, u32p, f32p]
     L.orc_set_tolerance.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32]
     L.orc_set_params.argtypes = [P, C.POINTER(Params)]
     L.orc_set_bodies.argtypes = [P, C.c_uint32, f32p, f32p, f32p, f32p, f32p, f32p, u32p, f32p, u32p]
@@ -158,6 +161,14 @@ class OracleWorld:
             self.close()
         except Exception:
             pass
+
+    def islands(self):
+        """(label, asleep, rest time) of every body after the last step."""
+        lab = np.zeros(self.n, np.uint32)
+        sl = np.zeros(self.n, np.uint32)
+        rest = np.zeros(self.n, np.float32)
+        self.L.orc_get_islands(self.h, _p(lab, C.c_uint32), _p(sl, C.c_uint32), _p(rest))
+        return lab, sl, rest
 
     def set_threads(self, threads):
         self.L.orc_set_threads(self.h, int(threads))

@@ -49,6 +49,7 @@ template <class T> void set_bodies(World<T>& W, uint32_t n, const float* pos, co
     }
     W.cache.clear();
     W.nsteps = 0;
+    W.reset_sleep();
 }
 
 template <class T> void get_bodies(const World<T>& W, float* pos, float* quat, float* vel, float* angvel) {
@@ -184,6 +185,17 @@ void* orc_create(const Params* p, int fp64) {
 }
 
 void orc_destroy(void* h) { delete (Handle*)h; }
+
+// Island label (0x7FFFFFFF: none), asleep flag and rest time of every body after the last step.
+void orc_get_islands(void* h, uint32_t* label, uint32_t* asleep, float* rest) {
+    DISPATCH(h, {
+        for (uint32_t i = 0; i < W.n; ++i) {
+            if (label) label[i] = W.label[i];
+            if (asleep) asleep[i] = W.asleep[i];
+            if (rest) rest[i] = (float)W.rest[i];
+        }
+    });
+}
 
 // Host threads of the oracle's parallel loops (results do not depend on the count).
 void orc_set_threads(void* h, uint32_t threads) { DISPATCH(h, W.set_threads(threads)); }
@@ -391,6 +403,7 @@ void orc_step_given(void* h, float dt, uint32_t iters, uint32_t M, const uint32_
         }
         W.solve(T(dt), iters, warm ? wt.data() : nullptr);
         W.integrate_positions(T(dt));
+        W.islands_and_sleep(T(dt));
         ++W.nsteps;
         W.st.manifolds = M;
         if (st) *st = W.st;

@@ -36,9 +36,18 @@
 //      mutually attainable, which otherwise leaves PGS drifting along the
 //      null space with a constant residual.
 //   9. x += v dt; q += (dt/2) (w,0) q; q /= |q|
+//  10. islands: connected components of the awake dynamic bodies over this step's
+//      manifolds (a manifold with a static, sleeping or plane partner does not
+//      connect); an island's label is its smallest body index.  Sleeping: an
+//      awake body whose |v| and |w| are below the thresholds adds dt to its rest
+//      time (else the time restarts at 0); an island whose bodies have all rested
+//      sleep_time seconds falls asleep (velocities zeroed).  A sleeping body acts
+//      as a static one in every stage; a manifold between an awake dynamic body and
+//      a sleeping one wakes the sleeping body's island at the start of the next step.
 #pragma once
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <vector>
@@ -64,6 +73,10 @@ struct Params {
     uint32_t warm_start;          // 0 none, 1 cache
     uint32_t record_residual;     // compute residual after every sweep
     float softness;               // relative compliance eps of every row (0: rigid rows)
+    uint32_t allow_sleep;         // islands whose bodies all rest for sleep_time seconds go to sleep
+    float sleep_linear;           // m/s: |v| below this counts as resting
+    float sleep_angular;          // rad/s: |w| below this counts as resting
+    float sleep_time;             // s
 };
 
 struct Stats {
@@ -72,6 +85,8 @@ struct Stats {
     float residual_first;
     double kinetic, potential;
     uint32_t gjk_fail, epa_fail, warm_hits;
+    uint32_t islands;   // islands of dynamic bodies (awake and asleep) after the step
+    uint32_t sleeping;  // bodies asleep after the step
 };
 
 template <class T> struct World {
@@ -104,6 +119,13 @@ template <class T> struct World {
     std::map<uint64_t, CacheEnt> cache;
     std::vector<T> lambda;  // per manifold x 4 points x 3 (solve output)
     std::vector<T> feat;    // per manifold x 4 points x 24 warm-start network inputs
+    // islands and sleeping (per body): island label (LABEL_NONE for static bodies and
+    // bodies that never joined an island), rest time, asleep flag; wake stamps per label
+    static constexpr uint32_t LABEL_NONE = 0x7FFFFFFFu;
+    std::vector<uint32_t> label;
+    std::vector<T> rest;
+    std::vector<uint8_t> asleep;
+    std::vector<uint32_t> wake;  // indexed by label: the step number at which that island wakes
     Stats st{};
     // residual-stopped sweeps: with tol > 0 the residual is checked after every sweep and
     // the solve stops once it is <= tol (at most max_iters sweeps)
@@ -129,7 +151,7 @@ template <class T> struct World {
         }
     }
 
-    bool dynamic(uint32_t i) const { return im[i] > T(0); }
+    bool dynamic(uint32_t i) const { return im[i] > T(0) && !asleep[i]; }
 
     Shape<T> shape_of(uint32_t i) const {
         Shape<T> s;
@@ -444,11 +466,11 @@ template <class T> struct World {
                 SBody& b = sb[i];
                 b.v = v[i];
                 b.w = w[i];
-                b.im = im[i];
                 b.dyn = dynamic(i);
+                b.im = b.dyn ? im[i] : T(0);
                 b.x = x[i];
                 const M3<T> R = quat_to_mat(q[i]);
-                b.I = world_inv_inertia(R, ii[i]);
+                b.I = b.dyn ? world_inv_inertia(R, ii[i]) : S3<T>{T(0), T(0), T(0), T(0), T(0), T(0)};
             }
         });
         SBody ground{};
@@ -724,7 +746,7 @@ template <class T> struct World {
         par(n, G, [&](uint64_t ch, uint64_t b, uint64_t e) {
             double k = 0.0, u = 0.0;
             for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
-                if (!dynamic(i)) continue;
+                if (!(im[i] > T(0))) continue;
                 const double m = 1.0 / (double)im[i];
                 const double vx = v[i].x, vy = v[i].y, vz = v[i].z;
                 k += 0.5 * m * (vx * vx + vy * vy + vz * vz);
@@ -748,8 +770,81 @@ template <class T> struct World {
         }
     }
 
+    void reset_sleep() {
+        label.assign(n, LABEL_NONE);
+        rest.assign(n, T(0));
+        asleep.assign(n, 0);
+        wake.assign(n, 0xFFFFFFFFu);
+    }
+
+    // Start of a step: sleeping bodies whose island was touched by an awake body in the
+    // previous step wake up.
+    void wake_up() {
+        par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
+            for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i)
+                if (asleep[i] && label[i] < n && wake[label[i]] == nsteps) {
+                    asleep[i] = 0;
+                    rest[i] = T(0);
+                }
+        });
+    }
+
+    static uint32_t uf_find(std::vector<uint32_t>& p, uint32_t x) {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+
+    // End of a step (after integration): wake requests, island labels, rest times, sleep.
+    void islands_and_sleep(T dt) {
+        const uint32_t M = (uint32_t)man_a.size();
+        std::vector<uint32_t> p(n);
+        for (uint32_t i = 0; i < n; ++i) p[i] = i;
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint32_t a = man_a[m], b = man_b[m];
+            if (b & PLANE_BIT) continue;
+            const bool da = dynamic(a), db = dynamic(b);
+            if (da && db) {
+                uint32_t ra = uf_find(p, a), rb = uf_find(p, b);
+                if (ra != rb) {
+                    if (ra < rb) p[rb] = ra;
+                    else p[ra] = rb;
+                }
+            } else if (da && asleep[b] && label[b] < n) {
+                wake[label[b]] = nsteps + 1;
+            } else if (db && asleep[a] && label[a] < n) {
+                wake[label[a]] = nsteps + 1;
+            }
+        }
+        const T lin2 = T(P.sleep_linear) * T(P.sleep_linear), ang2 = T(P.sleep_angular) * T(P.sleep_angular);
+        std::vector<T> imin(n, std::numeric_limits<T>::infinity());
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!dynamic(i)) continue;
+            const uint32_t r = uf_find(p, i);
+            label[i] = r;
+            const bool still = dot(v[i], v[i]) < lin2 && dot(w[i], w[i]) < ang2;
+            rest[i] = still ? rest[i] + dt : T(0);
+            if (rest[i] < imin[r]) imin[r] = rest[i];
+        }
+        uint32_t islands = 0, sleeping = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!(im[i] > T(0))) continue;
+            if (!asleep[i] && P.allow_sleep && imin[label[i]] >= T(P.sleep_time)) {
+                asleep[i] = 1;
+                v[i] = w[i] = V3<T>{T(0), T(0), T(0)};
+            }
+            if (label[i] == i) ++islands;
+            if (asleep[i]) ++sleeping;
+        }
+        st.islands = islands;
+        st.sleeping = sleeping;
+    }
+
     void step(T dt, uint32_t iters) {
         st = Stats{};
+        wake_up();
         std::vector<V3<T>> mn, mx;
         broadphase(mn, mx);
         narrowphase();
@@ -759,6 +854,7 @@ template <class T> struct World {
         colour_manifolds();
         solve(dt, iters, nullptr);
         integrate_positions(dt);
+        islands_and_sleep(dt);
         ++nsteps;
         st.pairs = (uint32_t)pair_a.size();
         st.manifolds = (uint32_t)man.size();
