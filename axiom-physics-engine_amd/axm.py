@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaxm.so")
 
 SUCCESS = 0
+OUT_OF_RANGE = 601
 ERROR_NAMES = {
     0: "Success", 200: "OutOfMemory", 202: "NullPointer", 300: "InvalidShape", 301: "GJKFailedToConverge",
     302: "EPAFailedToConverge", 400: "InvalidRigidBody", 401: "ConstraintSolverFailure",
@@ -45,7 +46,7 @@ class WorldDesc(C.Structure):
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
         ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float),
         ("allow_sleep", C.c_uint32), ("sleep_linear", C.c_float), ("sleep_angular", C.c_float),
-        ("sleep_time", C.c_float), ("max_bodies", C.c_uint32),
+        ("sleep_time", C.c_float), ("island_solve", C.c_uint32), ("max_bodies", C.c_uint32),
         ("max_pairs", C.c_uint32), ("max_manifolds", C.c_uint32), ("auto_grow", C.c_uint32),
         ("residual_tol", C.c_float), ("residual_every", C.c_uint32), ("max_iterations", C.c_uint32),
         ("device", C.c_int32), ("solver_mode", C.c_uint32),
@@ -58,6 +59,7 @@ class BodySoA(C.Structure):
         ("ang_vel", C.POINTER(C.c_float)), ("inv_mass", C.POINTER(C.c_float)),
         ("inv_inertia", C.POINTER(C.c_float)), ("shape_type", C.POINTER(C.c_uint32)),
         ("shape_params", C.POINTER(C.c_float)), ("world", C.POINTER(C.c_uint32)),
+        ("material", C.POINTER(C.c_float)), ("flags", C.POINTER(C.c_uint32)), ("filter", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -98,8 +100,8 @@ class SweepConfig(C.Structure):
 # Exported C-ABI symbols (the header's declarations).
 EXPORTS = [
     "axm_device_count", "axm_status_string", "axm_world_desc_default", "axm_world_create", "axm_world_destroy", "axm_world_set_bodies",
-    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands",
-    "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses",
+    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands", "axm_world_configure", "axm_world_get_desc",
+    "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses", "axm_world_get_friction_bounds",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
     "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
@@ -137,10 +139,13 @@ def lib():
     L.axm_world_profile.argtypes = [P, C.c_int, C.POINTER(KernelProfile)]
     L.axm_world_get_totals.argtypes = [P, C.POINTER(WorldTotals), C.c_int]
     L.axm_world_get_islands.argtypes = [P, u32p, u32p, f32p]
+    L.axm_world_configure.argtypes = [P, C.POINTER(WorldDesc)]
+    L.axm_world_get_desc.argtypes = [P, C.POINTER(WorldDesc)]
     L.axm_world_get_pairs.argtypes = [P, u32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_manifolds.argtypes = [P, u32p, u32p, u32p, f32p, f32p, f32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_colours.argtypes = [P, u32p, u32p, C.c_uint32, u32p, u32p]
     L.axm_world_get_impulses.argtypes = [P, f32p, f32p, C.c_uint32, u32p]
+    L.axm_world_get_friction_bounds.argtypes = [P, f32p, C.c_uint32, u32p]
     L.axm_world_get_residuals.argtypes = [P, f32p, C.c_uint32, u32p]
     L.axm_world_get_pre_solve_velocities.argtypes = [P, f32p, f32p]
     L.axm_broadphase.argtypes = [P, u32p]
@@ -215,11 +220,20 @@ class Bodies:
         self.shape_type = np.zeros(n, np.uint32)
         self.shape_params = np.zeros((n, 3), np.float32)
         self.world = np.zeros(n, np.uint32)
+        # optional extras (None: the world description's values): material [n, 4] (friction,
+        # restitution, linear damping, angular damping), flags [n] (bit 0 kinematic), filter [n, 3]
+        # (category, mask, group as int32 bits)
+        self.material = None
+        self.flags = None
+        self.filter = None
 
     def copy(self):
         b = Bodies(self.n)
         for f in self.FIELDS:
             setattr(b, f, getattr(self, f).copy())
+        for f in ("material", "flags", "filter"):
+            a = getattr(self, f)
+            setattr(b, f, None if a is None else a.copy())
         return b
 
     def soa(self):
@@ -233,6 +247,13 @@ class Bodies:
         s.shape_type = _p(self.shape_type, C.c_uint32)
         s.shape_params = _p(self.shape_params)
         s.world = _p(self.world, C.c_uint32)
+        for f, ct, dt in (("material", C.c_float, np.float32), ("flags", C.c_uint32, np.uint32),
+                          ("filter", C.c_uint32, np.uint32)):
+            a = getattr(self, f)
+            if a is not None:
+                a = np.ascontiguousarray(a, dt)
+                setattr(self, f, a)
+                setattr(s, f, _p(a, ct))
         return s
 
 
@@ -391,6 +412,11 @@ class World:
         _check(self._L.axm_world_profile(self._h, 1 if enable else 0, C.byref(p)), "axm_world_profile")
         return p
 
+    def configure(self, desc):
+        """Replaces the solver settings of the live world (capacities, grid and device stay)."""
+        _check(self._L.axm_world_configure(self._h, C.byref(desc)), "axm_world_configure")
+        self.desc = desc
+
     def islands(self):
         """(label, asleep, rest time) of every body after the last step."""
         n = self.n
@@ -480,6 +506,15 @@ class World:
         _check(self._L.axm_world_get_colours(self._h, _p(col, C.c_uint32), _p(order, C.c_uint32), c.value,
                                              C.byref(c), C.byref(k)), "axm_world_get_colours")
         return col, order, k.value
+
+    def friction_bounds(self):
+        """Friction bound mu * B of every contact point of the last step, [M, 4] in manifold order."""
+        c = C.c_uint32()
+        self._L.axm_world_get_friction_bounds(self._h, None, 0, C.byref(c))
+        out = np.zeros((c.value, 4), np.float32)
+        _check(self._L.axm_world_get_friction_bounds(self._h, _p(out), c.value, C.byref(c)),
+               "axm_world_get_friction_bounds")
+        return out
 
     def impulses(self):
         c = C.c_uint32()

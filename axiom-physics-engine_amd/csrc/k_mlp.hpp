@@ -23,6 +23,7 @@ constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_OUT * MLP_H;
 constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 20099
 constexpr int MLP_ROWS_PER_BLOCK = 64;             // 4 waves x 16 rows
 constexpr float MLP_G_REF = 9.81f;                 // impulse unit: effective mass * 9.81 m/s^2 * dt
+constexpr float MLP_Y_MAX = 1024.0f;               // largest prediction used (impulse units)
 
 struct MlpDev {
     const float* prm;      // fp32 master parameters (biases are read from here)

@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
     // the previous step wakes up (the request carries the number of this step)
     B.upar[i] = i;
     B.imin[i] = 0x7F800000u;  // +inf
+    B.icnt[i] = 0u;
     bool asleep = false;
     {
         const uint2 sl = B.slp[i];
@@ -72,8 +73,9 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
     f3 v = xyz(v4), w = xyz(w4);
     if (dyn) {
         const f3 g = {sp.gx, sp.gy, sp.gz};
-        v = (v + g * sp.dt) * sp.lin_keep;
-        w = w * sp.ang_keep;
+        const float4 mt = B.mat[i];
+        v = (v + g * sp.dt) * mt.z;
+        w = w * mt.w;
     }
     B.sv[i] = f4(v, dyn ? v4.w : 0.0f);
     B.sw[i] = f4(w, 0.0f);
@@ -252,6 +254,14 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.sv[i].w > 0.0f ? 1.0f : 0.0f);
 }
 
+// Collision filter of two bodies (flags, category, mask, group): a shared nonzero group
+// decides (positive: always, negative: never), else each category must meet the other's mask.
+__device__ __forceinline__ bool should_collide(uint4 a, uint4 b) {
+    const int ga = (int)a.w, gb = (int)b.w;
+    if (ga == gb && ga != 0) return ga > 0;
+    return (a.z & b.y) != 0u && (a.y & b.z) != 0u;
+}
+
 // The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
 // in canonical order; the planes come with slice -1.
 template <bool SPARSE, class F>
@@ -283,8 +293,10 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
         const uint32_t j = __float_as_uint(mnj.w);
         if (j <= i) return;
         if (!dyn_i && !(mxj.w > 0.0f)) return;
-        if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z && mx.z >= mnj.z)
+        if (mn.x <= mxj.x && mx.x >= mnj.x && mn.y <= mxj.y && mx.y >= mnj.y && mn.z <= mxj.z && mx.z >= mnj.z) {
+            if (sp.use_filter && !should_collide(B.bfl[i], B.bfl[j])) return;
             f(j);
+        }
     };
     auto visit_range = [&](uint2 r) {
         const uint32_t se = r.x + r.y;
@@ -877,14 +889,73 @@ __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restric
     }
 }
 
-// Colour sort input: keys = colour, values = manifold index.
-__global__ void __launch_bounds__(256) k_col_sort_input(const uint32_t* __restrict__ col, uint32_t cap,
-                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ val,
-                                                        const Counters* __restrict__ cnt) {
+// Union-find over body indices.  Every link points from a larger root to a smaller one
+// (compare-and-swap on the root), so the root of a finished component is its smallest
+// body index whatever the order of the links (lock-free union-find with path halving:
+// Jaiganesh & Burtscher 2018, ECL-CC).  The parents are shared between workgroups, so
+// they move through agent-scope atomics.
+__device__ __forceinline__ uint32_t uf_load(uint32_t* p, uint32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
+    uint32_t cur = uf_load(par, x);
+    if (cur != x) {
+        uint32_t prev = x, next;
+        while (cur > (next = uf_load(par, cur))) {
+            __hip_atomic_store(par + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            prev = cur;
+            cur = next;
+        }
+    }
+    return cur;
+}
+// Island (union-find root) of a manifold: that of its dynamic body, UNSET with none.
+__device__ __forceinline__ uint32_t man_root(const Bodies& B, uint2 ab) {
+    const bool plane = (ab.y & PLANE_BIT) != 0u;
+    const uint32_t d = B.sv[ab.x].w > 0.0f ? ab.x : (!plane && B.sv[ab.y].w > 0.0f ? ab.y : UNSET);
+    return d == UNSET ? UNSET : uf_find(B.upar, d);
+}
+// Manifolds per island (after the union kernel).
+__global__ void __launch_bounds__(256) k_isl_count(Bodies B, const uint2* __restrict__ m_ab, uint32_t cap,
+                                                   const Counters* __restrict__ cnt) {
+    if (cnt->sticky != 0u) return;
     const uint32_t M = min(cap, cnt->m_eff);
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        key[m] = col[m];
+        const uint32_t r = man_root(B, m_ab[m]);
+        if (r != UNSET) atomicAdd(B.icnt + r, 1u);
+    }
+}
+// Colour sort input: values = manifold index, keys = colour, or with the island split
+// LOCAL_KEY | root << 8 | colour for a manifold of an island of at most ISLAND_LOCAL_MAX
+// manifolds.  A stable sort then gives the solve order: the large islands' manifolds by
+// colour, then the small islands one after another, each by colour (manifold index last).
+__global__ void __launch_bounds__(256) k_col_sort_input(Bodies B, const uint2* __restrict__ m_ab,
+                                                        const uint32_t* __restrict__ col, uint32_t cap,
+                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ val,
+                                                        const Counters* __restrict__ cnt, uint32_t split) {
+    const uint32_t M = min(cap, cnt->m_eff);
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        uint32_t k = col[m];
+        if (split) {
+            const uint32_t r = man_root(B, m_ab[m]);
+            if (r != UNSET && B.icnt[r] <= ISLAND_LOCAL_MAX) k |= LOCAL_KEY | (r << 8);
+        }
+        key[m] = k;
         val[m] = m;
+    }
+}
+// The small islands' work lists, from the sorted keys: (first slot, manifolds) of each,
+// one-lane islands from the front of `list`, one-workgroup islands from its back.
+__global__ void __launch_bounds__(256) k_isl_list(const Bodies B, const uint32_t* __restrict__ skey,
+                                                  uint2* __restrict__ list, uint32_t cap, Counters* cnt) {
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob;
+    for (uint32_t s = M0 + blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+        const uint32_t k = skey[s];
+        if (s != M0 && ((skey[s - 1] ^ k) >> 8) == 0u) continue;
+        const uint32_t size = B.icnt[(k & ~LOCAL_KEY) >> 8];
+        if (size <= ISLAND_LANE_MAX) list[atomicAdd(&cnt->isl_lane_n, 1u)] = make_uint2(s, size);
+        else list[cap - 1u - atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
     }
 }
 
@@ -906,6 +977,7 @@ __global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t max_k,
         if (scol[mid] < c) lo = mid + 1; else hi = mid;
     }
     off[c] = lo;
+    if (c == K) cnt->m_glob = lo;  // the small islands' keys sort above every colour
 }
 
 // ---------------------------------------------------------------- solver
@@ -992,6 +1064,15 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
     }
     uint32_t ps = 0;
     const bool have_prev = pc.valid && prev_find(pc, ((uint64_t)ab.x << 32) | ab.y, ps);
+    // the pair's friction and restitution: mu = sqrt(mu_A mu_B) (mu_A when equal), e = max(e_A, e_B);
+    // a plane has the world's values
+    float mu, rest_e;
+    {
+        const float4 ma = B.mat[ab.x];
+        const float mub = plane ? sp.friction : B.mat[ab.y].x, eb = plane ? sp.restitution : B.mat[ab.y].y;
+        mu = ma.x == mub ? ma.x : sqrtf(ma.x * mub);
+        rest_e = ma.y > eb ? ma.y : eb;
+    }
     uint32_t hits = 0;
     for (uint32_t k = 0; k < np; ++k) {
         const float4 pt = c_pt[(size_t)m * 4 + k], nf = c_nf[(size_t)m * 4 + k];
@@ -1026,7 +1107,7 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
             bias = depth * sp.inv_dt;
         }
         if (vn0 < -sp.rest_threshold) {
-            const float rb = -sp.restitution * vn0;
+            const float rb = -rest_e * vn0;
             if (rb > bias) bias = rb;
         }
         const size_t o = (size_t)k * cap + s;
@@ -1039,7 +1120,6 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
         J[5 * (size_t)cap] = f4(t1, mT1);
         J[6 * (size_t)cap] = f4(ja[1], kk[1]);
         J[7 * (size_t)cap] = f4(jb[1], 0.0f);
-        J[8 * (size_t)cap] = f4(jia[1], 0.0f);
         J[9 * (size_t)cap] = f4(jib[1], 0.0f);
         J[10 * (size_t)cap] = f4(t2, mT2);
         J[11 * (size_t)cap] = f4(ja[2], kk[2]);
@@ -1061,6 +1141,13 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
                     break;
                 }
             }
+        }
+        {
+            // friction bound of this step: mu x the previous normal impulse of the same feature,
+            // or for a new point the impulse that stops its approach on its own
+            const float est = mN * (bias - vn0);
+            const float nb = hit ? Lp.x : est;
+            J[8 * (size_t)cap] = f4(jia[1], mu * (nb > 0.0f ? nb : 0.0f));
         }
         float4 L = warm_in ? warm_in[(size_t)m * 4 + k] : Lp;
         L.w = 0.0f;
@@ -1207,8 +1294,8 @@ __device__ __forceinline__ Dir load_dir(const float4* J, size_t rs, int dir, boo
 // One visit of a manifold on registers: warm start (first sweep only), friction
 // rows of every point, then normal rows.  Point k's rows start at J + k * ps, rows
 // `rs` apart; L[k] holds the point impulses in and out.
-__device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps, uint32_t np, bool plane, float mu,
-                                           float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
+__device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps, uint32_t np, bool plane, float keep,
+                                           bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w1, w2, w3, w4;
     if (first) {
 #pragma unroll
@@ -1227,10 +1314,10 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= np) break;
         const float4* P = J + (size_t)k * ps;
-        const float maxf = mu * L[k].x;
+        float maxf;  // the step's friction bound (row 8's .w)
         {
             float mT1;
-            const Dir D1 = load_dir(P, rs, 1, plane, mT1, w1, w2, w3, w4);
+            const Dir D1 = load_dir(P, rs, 1, plane, mT1, w1, w2, maxf, w4);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
@@ -1292,8 +1379,8 @@ __device__ __forceinline__ void load_rows(const SolverRows& R, uint32_t s, uint3
         }
     }
 }
-__device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_t np, bool plane, float mu,
-                                           float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
+__device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_t np, bool plane, float keep,
+                                           bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w3;
     if (first) {
 #pragma unroll
@@ -1310,10 +1397,10 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= np) break;
-        const float maxf = mu * L[k].x;
+        float maxf;  // the step's friction bound (row 8's .w)
         {
             float mT1;
-            const Dir D1 = dir_regs(Jr[k], 1, plane, mT1, w3);
+            const Dir D1 = dir_regs(Jr[k], 1, plane, mT1, maxf);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
@@ -1345,8 +1432,8 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
 // The same with the rows of solve slot s in the SoA row arrays (row r of point k at
 // J[(k * JROWS + r) * cap + s]).
 __device__ __forceinline__ void solve_core(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
-                                           float mu, float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
-    solve_rows(R.J + s, cap, (size_t)JROWS * cap, np, plane, mu, keep, first, A, Bd, L);
+                                           float keep, bool first, SB& A, SB& Bd, float4 L[4]) {
+    solve_rows(R.J + s, cap, (size_t)JROWS * cap, np, plane, keep, first, A, Bd, L);
 }
 
 // One visit of the manifold in solve slot s with plain loads and stores (one
@@ -1360,7 +1447,7 @@ __device__ __forceinline__ void solve_slot(const Bodies& B, const StepParams& sp
     if (plane) zero_sb(Bd); else load_vel<false>(B, b, Bd);
     float4 L[4];
     for (uint32_t k = 0; k < np; ++k) L[k] = R.lam[(size_t)k * cap + s];
-    solve_core(R, s, cap, np, plane, sp.friction, sp.keep, first, A, Bd, L);
+    solve_core(R, s, cap, np, plane, sp.keep, first, A, Bd, L);
     for (uint32_t k = 0; k < np; ++k) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
     if (A.dyn) {
         B.sv[a] = f4(A.v, A.im);
@@ -1484,7 +1571,7 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
                                                    const Counters* __restrict__ cnt) {
     // the rows exist only when this step's preparation ran
     if (cnt->sticky != 0u) return;
-    const uint32_t M = cnt->m_eff;
+    const uint32_t M = cnt->m_glob;  // the large islands' slots
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
@@ -1521,7 +1608,7 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
 // visit (the last visit of the last sweep passed it on there).
 __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, const Counters* __restrict__ cnt) {
     if (cnt->sticky != 0u) return;
-    const uint32_t M = cnt->m_eff;
+    const uint32_t M = cnt->m_glob;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
@@ -1545,16 +1632,16 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
 // times the row's effective mass inverse) that one more visit would make along the
 // normal or either friction direction, from the current velocities and impulses L.
 // J points at the point's first Jacobian row, rs is the row stride.
-__device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool plane, float mu, float keep,
-                                                const SB& A, const SB& Bd, const float4 L) {
+__device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool plane, float keep, const SB& A,
+                                                const SB& Bd, const float4 L) {
     float mN, kN, tg, mT1, kT1, mT2, kT2, x2, x3, x4;
     const Dir Dn = load_dir(J, rs, 0, plane, mN, kN, x2, tg, x4);
-    const Dir D1 = load_dir(J, rs, 1, plane, mT1, kT1, x2, x3, x4);
+    float maxf;  // the step's friction bound (row 8's .w)
+    const Dir D1 = load_dir(J, rs, 1, plane, mT1, kT1, x2, maxf, x4);
     const Dir D2 = load_dir(J, rs, 2, plane, mT2, kT2, x2, x3, x4);
     float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
     nl = nl > 0.0f ? nl : 0.0f;
     float e = fabsf(nl - L.x) * kN;
-    const float maxf = mu * L.x;
     float q1 = fmaf(-mT1, jvel(D1, A, Bd), keep * L.y);
     q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
     const float e1 = fabsf(q1 - L.y) * kT1;
@@ -1632,13 +1719,21 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                                                       const uint64_t* __restrict__ bmask, SweepStop stop) {
     extern __shared__ float4 own_lds[];
     const uint32_t lane = threadIdx.x;
-    const uint32_t M = cnt->m_eff;
+    const uint32_t M = cnt->m_glob;  // the large islands' slots (the small ones are solved on their own)
     if (cnt->max_colour > 64u) {
         // dependency ranks come from 64-bit colour masks: hand the step back for per-colour launches
         if (blockIdx.x == 0 && lane == 0) atomicOr(&cnt->sticky, STICKY_FLOW_COLOURS);
         return;
     }
     if (cnt->sticky != 0u || nsweeps == 0u) return;
+    if (M == 0u && cnt->m_eff > 0u) {
+        // every island was solved on its own: no sweep of the large ones runs
+        if (blockIdx.x == 0 && lane == 0 && stop.bits) {
+            __hip_atomic_store(&cnt->sweeps_run, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&cnt->tol_hit, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
     uint32_t* err = &cnt->sticky;
     const uint32_t G = gridDim.x, g = blockIdx.x;
     const uint32_t nchunks = (M + OWN_WAVE - 1) / OWN_WAVE;
@@ -1723,7 +1818,6 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // the inbox words from memory; the general loop reads what is not staged from
     // the row arrays.  Two instances, so that the first never waits on a memory
     // load other than its poll (a wait for a load also waits for the stores before it).
-    const float mu = sp.friction;
     const float keep = sp.keep;
     // tune bits 4-6: poll back-off level; bit 1: poll the two hint words before the full load
     // (inboxes within 2 GiB); bit 2: adaptive back-off
@@ -1793,7 +1887,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 const float4* J = staged ? mine + (size_t)(off + k) * OWN_WAVE : R.J + (size_t)k * JROWS * cap + it.s;
                 const size_t jr = staged ? rs : (size_t)cap;
                 const float4 L = staged ? J[JROWS * jr] : R.lam[(size_t)k * cap + it.s];
-                const float e = point_residual(J, jr, plane, mu, keep, A, Bd, L);
+                const float e = point_residual(J, jr, plane, keep, A, Bd, L);
                 if (e > res) res = e;
             }
             off += staged ? np : 0u;
@@ -1926,7 +2020,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) L[k] = P[JROWS * rs + (size_t)k * OWN_WAVE];
-                    solve_rows(P, rs, OWN_WAVE, np, plane, mu, keep, first, A, Bd, L);
+                    solve_rows(P, rs, OWN_WAVE, np, plane, keep, first, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) P[JROWS * rs + (size_t)k * OWN_WAVE] = L[k];
@@ -1936,7 +2030,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) L[k] = R.lam[(size_t)k * cap + s];
-                    solve_regs(Jr, np, plane, mu, keep, first, A, Bd, L);
+                    solve_regs(Jr, np, plane, keep, first, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
@@ -2024,7 +2118,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, SolverRows R, const Counters* cnt,
                                                   uint32_t cap, uint32_t* __restrict__ out_bits) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t M = cnt->m_eff;
+    const uint32_t M = cnt->m_glob;  // the large islands' slots (the small ones report their own)
     float res = 0.0f;
     if (s < M && cnt->sticky == 0u) {  // rows exist only when the preparation ran
         uint4 h = R.hdr[s];
@@ -2033,9 +2127,8 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
         SB A, Bd;
         load_vel<false>(B, h.x, A);
         if (plane) zero_sb(Bd); else load_vel<false>(B, h.y, Bd);
-        const float mu = sp.friction;
         for (uint32_t k = 0; k < h.z; ++k) {
-            const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, mu, sp.keep, A, Bd,
+            const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, sp.keep, A, Bd,
                                            R.lam[(size_t)k * cap + s]);
             if (e > res) res = e;
         }
@@ -2047,6 +2140,139 @@ __global__ void __launch_bounds__(256) k_residual(Bodies B, StepParams sp, Solve
         bits = ob > bits ? ob : bits;
     }
     if ((threadIdx.x & 63) == 0) atomicMax(out_bits, bits);
+}
+
+// ---------------------------------------------------------------- islands solved on their own
+// The manifolds of an island of at most ISLAND_LOCAL_MAX manifolds share no dynamic body
+// with any other island, so the island is solved by itself: sweeps over its slots in
+// solve order (colour, then manifold), and with a tolerance it stops after the first
+// sweep that leaves its own residual <= tol.  Without a tolerance it runs every sweep, and
+// the result is bit-identical to sweeping all islands together colour by colour.
+
+// Residual of solve slot s (the k_residual value of its points).
+__device__ __forceinline__ float slot_residual(const Bodies& B, float keep, const SolverRows& R, uint32_t s,
+                                               uint32_t cap) {
+    const uint4 h = R.hdr[s];
+    const uint32_t np = min(h.z, 4u);
+    const bool plane = (h.y & PLANE_BIT) != 0;
+    SB A, Bd;
+    load_vel<false>(B, h.x, A);
+    if (plane) zero_sb(Bd); else load_vel<false>(B, h.y, Bd);
+    float res = 0.0f;
+    for (uint32_t k = 0; k < np; ++k) {
+        const float e = point_residual(R.J + (size_t)k * JROWS * cap + s, cap, plane, keep, A, Bd,
+                                       R.lam[(size_t)k * cap + s]);
+        if (e > res) res = e;
+    }
+    return res;
+}
+// One island's result: sweeps run, final residual (bits), whether it missed the tolerance,
+// contact points; folded into the counters.
+__device__ __forceinline__ void local_done(Counters* cnt, uint32_t sweeps, float res, bool miss, uint32_t pts) {
+    atomicMax(&cnt->loc_sweeps, sweeps);
+    atomicMax(&cnt->loc_res_bits, __float_as_uint(res));
+    if (miss) atomicOr(&cnt->loc_miss, 1u);
+    atomicAdd(&cnt->loc_points, pts);
+    atomicAdd(&cnt->loc_point_sweeps, (unsigned long long)pts * sweeps);
+}
+// Islands of at most ISLAND_LANE_MAX manifolds: one lane each, visits in slot order.
+__global__ void __launch_bounds__(256) k_pgs_lane(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
+                                                  const uint2* __restrict__ list, float tol, uint32_t max_sweeps) {
+    if (cnt->sticky != 0u) return;
+    const uint32_t n = cnt->isl_lane_n;
+    const bool tol_mode = tol > 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint2 e = list[i];
+        const uint32_t s0 = e.x, s1 = e.x + e.y;
+        uint32_t it = 0;
+        float res = 0.0f;
+        for (; it < max_sweeps; ++it) {
+            for (uint32_t s = s0; s < s1; ++s) solve_slot(B, sp, R, R.hdr[s], s, cap, it == 0u);
+            if (tol_mode || it + 1u == max_sweeps) {
+                res = 0.0f;
+                for (uint32_t s = s0; s < s1; ++s) {
+                    const float r = slot_residual(B, sp.keep, R, s, cap);
+                    if (r > res) res = r;
+                }
+                if (tol_mode && res <= tol) {
+                    ++it;
+                    break;
+                }
+            }
+        }
+        uint32_t pts = 0;
+        for (uint32_t s = s0; s < s1; ++s) pts += min(R.hdr[s].z, 4u);
+        local_done(cnt, it, res, tol_mode && !(res <= tol), pts);
+    }
+}
+// Block-wide maximum and sum of 256 threads (red: 8 words of LDS); every thread gets both.
+__device__ __forceinline__ void block_max_sum(uint32_t& mx, uint32_t& sum, uint32_t* red) {
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t om = __shfl_xor(mx, o, 64), os = __shfl_xor(sum, o, 64);
+        mx = om > mx ? om : mx;
+        sum += os;
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        red[threadIdx.x >> 6] = mx;
+        red[4 + (threadIdx.x >> 6)] = sum;
+    }
+    __syncthreads();
+    mx = max(max(red[0], red[1]), max(red[2], red[3]));
+    sum = red[4] + red[5] + red[6] + red[7];
+    __syncthreads();
+}
+// Larger islands: one workgroup each.  A colour's slots share no dynamic body, so the
+// workgroup visits them together and meets at a barrier before the next colour (the
+// velocities move through global memory, visible within the workgroup after the barrier).
+__global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
+                                                   const uint2* __restrict__ list, const uint32_t* __restrict__ skey,
+                                                   float tol, uint32_t max_sweeps) {
+    __shared__ uint32_t red[8];
+    if (cnt->sticky != 0u) return;
+    const uint32_t n = cnt->isl_group_n, t = threadIdx.x;
+    const bool tol_mode = tol > 0.0f;
+    for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
+        const uint2 e = list[cap - 1u - g];
+        const uint32_t s0 = e.x, s1 = e.x + e.y;
+        const uint32_t k0 = skey[s0], k1 = skey[s1 - 1u];
+        uint32_t it = 0;
+        float res = 0.0f;
+        for (; it < max_sweeps; ++it) {
+            uint32_t a = s0;
+            for (uint32_t c = k0 & 0xFFu; c <= (k1 & 0xFFu); ++c) {
+                // [a, b): the island's slots of colour c (the keys ascend, equal above the colour)
+                const uint32_t kc = (k0 & ~0xFFu) | c;
+                uint32_t lo = a, hi = s1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (skey[mid] <= kc) lo = mid + 1u; else hi = mid;
+                }
+                if (lo > a) {
+                    for (uint32_t s = a + t; s < lo; s += blockDim.x) solve_slot(B, sp, R, R.hdr[s], s, cap, it == 0u);
+                    __syncthreads();
+                }
+                a = lo;
+            }
+            if (tol_mode || it + 1u == max_sweeps) {
+                float r = 0.0f;
+                for (uint32_t s = s0 + t; s < s1; s += blockDim.x) {
+                    const float q = slot_residual(B, sp.keep, R, s, cap);
+                    if (q > r) r = q;
+                }
+                uint32_t mx = __float_as_uint(r), unused = 0u;
+                block_max_sum(mx, unused, red);
+                res = __uint_as_float(mx);
+                if (tol_mode && res <= tol) {
+                    ++it;
+                    break;
+                }
+            }
+        }
+        uint32_t pts = 0, mx = 0u;
+        for (uint32_t s = s0 + t; s < s1; s += blockDim.x) pts += min(R.hdr[s].z, 4u);
+        block_max_sum(mx, pts, red);
+        if (t == 0u) local_done(cnt, it, res, tol_mode && !(res <= tol), pts);
+    }
 }
 
 // Previous-step lookup table for the warm start: (a, b) -> slot.
@@ -2122,18 +2348,21 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
     if (cnt->sticky != 0u) return;
     if (i == 0) {
         cnt->steps_done += 1u;
-        const uint32_t sw = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
+        // sweeps of the large islands, and of the step (the most any island ran)
+        const uint32_t swg = sweeps == SWEEPS_ON_DEVICE ? cnt->sweeps_run : sweeps;
+        const uint32_t sw = max(swg, cnt->loc_sweeps);
         cnt->pgs_bytes_total += spread_sum(cnt, &SpreadRow::pgs_bytes) * (unsigned long long)sw;
         cnt->sweeps_total += sw;
         cnt->acc_steps += 1u;
-        cnt->acc_reached += cnt->tol_hit;
+        cnt->acc_reached += cnt->tol_hit != 0u && cnt->loc_miss == 0u ? 1u : 0u;
         cnt->acc_max_sweeps = max(cnt->acc_max_sweeps, sw);
         cnt->acc_sweeps += sw;
-        cnt->acc_contact_sweeps += (unsigned long long)spread_sum(cnt, &SpreadRow::contacts) * sw;
+        cnt->acc_contact_sweeps +=
+            (unsigned long long)(spread_sum(cnt, &SpreadRow::contacts) - cnt->loc_points) * swg + cnt->loc_point_sweeps;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];
-    if (!(v4.w > 0.0f)) return;
+    if (!(v4.w > 0.0f) && !(B.bfl[i].x & BODY_KINEMATIC)) return;
     const float4 w4 = B.sw[i];
     const float4 x4 = B.x[i];
     const float4 qv = B.q[i];
@@ -2152,31 +2381,12 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
 }
 
 // ---------------------------------------------------------------- islands and sleeping
-// Union-find over the step's manifolds between two awake dynamic bodies.  Every link
-// points from a larger root to a smaller one (compare-and-swap on the root), so the
-// root of a finished component is its smallest body index whatever the order of the
-// links (lock-free union-find with path halving: Jaiganesh & Burtscher 2018, ECL-CC).
-// The parents are shared between workgroups, so they move through agent-scope atomics.
-__device__ __forceinline__ uint32_t uf_load(uint32_t* p, uint32_t x) {
-    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
-    uint32_t cur = uf_load(par, x);
-    if (cur != x) {
-        uint32_t prev = x, next;
-        while (cur > (next = uf_load(par, cur))) {
-            __hip_atomic_store(par + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            prev = cur;
-            cur = next;
-        }
-    }
-    return cur;
-}
+// Union-find over the step's manifolds between two awake dynamic bodies (uf_find).
 __global__ void __launch_bounds__(256) k_isl_union(Bodies B, const uint2* __restrict__ m_ab, uint32_t cap,
                                                    uint32_t n, const Counters* __restrict__ cnt) {
     if (cnt->sticky != 0u) return;
     const uint32_t M = min(cap, cnt->m_eff);
-    const uint32_t stamp = cnt->steps_done;  // the integration already counted this step
+    const uint32_t stamp = cnt->steps_done + 1u;  // runs before the integration counts this step
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         const uint2 ab = m_ab[m];
         if (ab.y & PLANE_BIT) continue;
@@ -2350,12 +2560,21 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
     hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, bar,
                        cnt, 65000u, SMALL);
 }
-void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_col_sort_input, grid_bounded(cap), dim3(256), 0, s, col, cap, key, val, cnt);
+void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
+                           uint32_t* val, const Counters* cnt, uint32_t split, hipStream_t s) {
+    if (split) hipLaunchKernelGGL(k_isl_count, grid_bounded(cap), dim3(256), 0, s, B, m_ab, cap, cnt);
+    hipLaunchKernelGGL(k_col_sort_input, grid_bounded(cap), dim3(256), 0, s, B, m_ab, col, cap, key, val, cnt, split);
 }
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s) {
     hipLaunchKernelGGL(k_col_offsets, grid_for(max_k + 1), dim3(256), 0, s, scol, max_k, off, cnt);
+}
+void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
+                      const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_isl_list, grid_bounded(cap), dim3(256), 0, s, B, skey, list, cap, cnt);
+    if (max_sweeps == 0u) return;
+    hipLaunchKernelGGL(k_pgs_group, dim3(groups), dim3(256), 0, s, B, sp, R, cnt, cap, list, skey, tol, max_sweeps);
+    hipLaunchKernelGGL(k_pgs_lane, grid_bounded(cap), dim3(256), 0, s, B, sp, R, cnt, cap, list, tol, max_sweeps);
 }
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
@@ -2369,7 +2588,7 @@ void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, u
 // Learned warm start: the network predicts the correction to the previous
 // step's impulse of the same feature (already in the rows, zero for a new
 // point) in units of u = mN g dt; lambda = previous + output * u, projected onto
-// lambda_n >= 0 and the friction pyramid |lambda_t| <= mu lambda_n.
+// lambda_n >= 0 and the step's friction box |lambda_t| <= mu B.
 __global__ void __launch_bounds__(256) k_mlp_apply(SolverRows R, const Counters* __restrict__ cnt, uint32_t cap,
                                                    const float* __restrict__ Y, StepParams sp) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2381,10 +2600,16 @@ __global__ void __launch_bounds__(256) k_mlp_apply(SolverRows R, const Counters*
         const float u = R.J[(size_t)k * JROWS * cap + s].w * gdt;  // mN
         const float* y = Y + ((size_t)4 * s + k) * MLP_OUT;
         const float4 Lp = R.lam[o];
-        float ln = Lp.x + y[0] * u;
+        // a non-finite output leaves the point exactly at its cached impulse (as the plain warm
+        // start has it); finite outputs are capped at MLP_Y_MAX units, so any network gives a
+        // finite start the sweeps then correct
+        if (!(isfinite(y[0]) && isfinite(y[1]) && isfinite(y[2]))) continue;
+        auto cap_y = [](float v) { return v < -MLP_Y_MAX ? -MLP_Y_MAX : (v > MLP_Y_MAX ? MLP_Y_MAX : v); };
+        const float y0 = cap_y(y[0]), y1 = cap_y(y[1]), y2 = cap_y(y[2]);
+        float ln = Lp.x + y0 * u;
         ln = ln > 0.0f ? ln : 0.0f;
-        const float maxf = sp.friction * ln;
-        float t1 = Lp.y + y[1] * u, t2 = Lp.z + y[2] * u;
+        const float maxf = R.J[((size_t)k * JROWS + 8) * cap + s].w;  // the step's friction bound
+        float t1 = Lp.y + y1 * u, t2 = Lp.z + y2 * u;
         t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
         t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
         const float4 L = make_float4(ln, t1, t2, 0.0f);
@@ -2443,10 +2668,12 @@ void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counter
     hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n, cnt, sweeps);
 }
 
-void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, const uint2* m_ab, uint32_t cap_man,
-                    Counters* cnt, hipStream_t s) {
+void launch_isl_union(const Bodies& B, uint32_t n, const uint2* m_ab, uint32_t cap_man, Counters* cnt,
+                      hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_isl_union, grid_bounded(cap_man), dim3(256), 0, s, B, m_ab, cap_man, n, cnt);
+}
+void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_isl_union, grid_bounded(cap_man), dim3(256), 0, s, B, m_ab, cap_man, n, cnt);
     hipLaunchKernelGGL(k_isl_label, grid_for(n), dim3(256), 0, s, B, sp, n, cnt);
     hipLaunchKernelGGL(k_isl_sleep, grid_for(n), dim3(256), 0, s, B, sp, n, cnt);
 }

@@ -47,11 +47,13 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 // colouring's per-body words.
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, hipStream_t s);
-// End of a step (after the integration): island labels by union-find over the step's
-// manifolds between awake dynamic bodies (label = smallest body index), wake requests
-// for sleeping islands an awake body touched, rest times, and islands falling asleep.
-void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, const uint2* m_ab, uint32_t cap_man,
-                    Counters* cnt, hipStream_t s);
+// Islands of the step, after the narrowphase: union-find over the manifolds between awake
+// dynamic bodies (root = smallest body index), and wake requests for sleeping islands an
+// awake body touched.
+void launch_isl_union(const Bodies& B, uint32_t n, const uint2* m_ab, uint32_t cap_man, Counters* cnt,
+                      hipStream_t s);
+// End of a step (after the integration): island labels, rest times, and islands falling asleep.
+void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
@@ -84,9 +86,15 @@ int colour_blocks_per_cu();
 void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_t* col, uint64_t* prio,
                    uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
                    Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s);
-void launch_col_sort_input(const uint32_t* col, uint32_t cap, uint32_t* key, uint32_t* val, const Counters* cnt,
+void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
+                           uint32_t* val, const Counters* cnt, uint32_t split,
                            hipStream_t s);
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);
+// The islands solved on their own (after the row preparation): work lists from the sorted
+// keys, then one lane or one workgroup per island; tol > 0 stops each at its own residual.
+void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
+                      const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
+                      hipStream_t s);
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,

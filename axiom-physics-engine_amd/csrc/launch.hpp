@@ -30,6 +30,8 @@ struct StepParams {
     float soft1, keep;  // 1 + softness and its inverse (regularised rows)
     uint32_t allow_sleep;
     float sleep_lin2, sleep_ang2, sleep_time;  // squared thresholds (m/s, rad/s), s
+    uint32_t use_filter;  // some body has a collision filter
+    uint32_t island_split;  // islands of at most ISLAND_LOCAL_MAX manifolds are solved on their own
     float baumgarte, slop, max_bias, margin;
     float lin_keep, ang_keep;
     float inv_cell;
@@ -58,12 +60,21 @@ struct Bodies {
     uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)
     // islands and sleeping: per body (rest time bits, asleep bit 31 | island label); per step the
     // union-find parents and each island's smallest rest time; per label the step it wakes at
+    float4* mat;   // friction, restitution, linear keep (1 - damping), angular keep
+    uint4* bfl;    // flags (bit 0 kinematic), filter category, mask, group
     uint2* slp;
     uint32_t* upar;
     uint32_t* imin;
     uint32_t* wake;
+    uint32_t* icnt;  // per step, per island root: the island's manifolds
 };
+// Islands of at most ISLAND_LOCAL_MAX manifolds are solved on their own (one lane each up
+// to ISLAND_LANE_MAX manifolds, else one workgroup), each to its own residual; their
+// solve slots follow those of the large islands, keyed LOCAL_KEY | root << 8 | colour.
+constexpr uint32_t ISLAND_LOCAL_MAX = 4096u, ISLAND_LANE_MAX = 8u;
+constexpr uint32_t LOCAL_KEY = 0x80000000u, ISLAND_ROOT_LIMIT = 1u << 23;
 constexpr uint32_t ASLEEP_BIT = 0x80000000u, LABEL_MASK = 0x7FFFFFFFu, LABEL_NONE = 0x7FFFFFFFu;
+constexpr uint32_t BODY_KINEMATIC = 1u;
 // The broadphase counts and emits each body's pairs in three parts, one per z
 // slice of its 27 neighbour cells (the planes go with the first), one thread each.
 constexpr uint32_t BP_PARTS = 3;
@@ -113,7 +124,8 @@ struct Counters {
     uint32_t bp_sparse;    // broadphase mode of the step (1: fewer than four bodies per occupied cell)
     uint32_t gjk_fail;
     uint32_t epa_fail;
-    uint32_t pad2;
+    uint32_t m_glob;       // solve slots of the large islands, [0, m_glob); the rest belong to islands
+                           // solved on their own
     uint32_t remaining;    // colouring: uncoloured manifolds
     uint32_t max_colour;   // colouring: max colour + 1
     uint32_t last_round;   // colouring: last round that coloured something + 1
@@ -121,10 +133,16 @@ struct Counters {
     uint32_t residual_bits;
     uint32_t p_eff;        // min(pairs, pair capacity)
     uint32_t m_eff;        // min(manifolds, manifold capacity)
-    uint32_t pad3;
+    uint32_t loc_sweeps;   // the most sweeps an island solved on its own ran
     uint32_t sweeps_run;   // sweeps of a launch with residual checks (set when it ends)
     uint32_t tol_hit;      // 1: the step's sweeps stopped because the residual met the tolerance
-    unsigned long long pad4;
+    uint32_t loc_res_bits; // the largest residual an island solved on its own ended with
+    uint32_t loc_miss;     // 1: an island solved on its own stopped at the sweep cap above the tolerance
+    uint32_t isl_lane_n;   // islands solved by one lane each
+    uint32_t isl_group_n;  // islands solved by one workgroup each
+    uint32_t loc_points;   // contact points of the islands solved on their own
+    uint32_t pad5;
+    unsigned long long loc_point_sweeps;  // their contact points x the sweeps each island ran
     // ---- kept across steps
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
     uint32_t steps_done;   // steps whose integration ran
@@ -145,7 +163,8 @@ template <class T> inline T spread_total(const Counters& c, T SpreadRow::*f) {
     for (uint32_t k = 0; k < SPREAD; ++k) s += c.sp[k].*f;
     return s;
 }
-constexpr uint32_t COUNTERS_RESET_WORDS = 18;  // words of the first block (cleared per step)
+constexpr uint32_t COUNTERS_RESET_WORDS = 24;  // words of the first block (cleared per step)
+static_assert(offsetof(Counters, sticky) == 4 * COUNTERS_RESET_WORDS, "per-step block ends at sticky");
 enum : uint32_t {
     STICKY_PAIRS = 1u,        // pair capacity exceeded
     STICKY_MANIFOLDS = 2u,    // manifold capacity exceeded

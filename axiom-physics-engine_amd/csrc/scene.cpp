@@ -233,6 +233,7 @@ void axm_world_desc_default(axm_world_desc* d) {
     d->sleep_linear = 0.01f;
     d->sleep_angular = 0.01f;
     d->sleep_time = 0.5f;
+    d->island_solve = 1;
     d->auto_grow = 1;
     d->residual_tol = 0.0f;
     d->residual_every = 1;

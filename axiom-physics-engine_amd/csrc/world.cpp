@@ -135,6 +135,8 @@ struct axm_world_s {
     uint32_t* clist = nullptr;  // colouring: the manifolds not kept from the previous step
     uint64_t *prio = nullptr, *bmax0 = nullptr, *bmax1 = nullptr, *bmask = nullptr;
     uint32_t *ckey = nullptr, *ckey_alt = nullptr, *cval = nullptr, *cval_alt = nullptr, *csort_tmp = nullptr;
+    uint2* isl_list = nullptr;  // the small islands' work lists (first slot, manifolds)
+    const uint32_t* solve_order = nullptr;  // the sort output holding the last step's solve order
     uint32_t* col_off = nullptr;
     uint32_t* bdone = nullptr;
     GridBar* bar = nullptr;
@@ -170,7 +172,8 @@ struct axm_world_s {
     // last step
     bool have_step = false;
     uint32_t last_iters = 0;
-    bool last_iters_dev = false;  // the last step's sweep count is in Counters::sweeps_run
+    bool last_iters_dev = false;
+    bool use_filter = false;  // some body of the current set has a collision filter  // the last step's sweep count is in Counters::sweeps_run
     std::vector<float> h_res;
 
     void drop_graphs() {
@@ -203,10 +206,13 @@ void free_bodies(axm_world_s* w) {
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
     w->mem.release(B.vslot);
+    w->mem.release(B.mat);
+    w->mem.release(B.bfl);
     w->mem.release(B.slp);
     w->mem.release(B.upar);
     w->mem.release(B.imin);
     w->mem.release(B.wake);
+    w->mem.release(B.icnt);
     w->mem.release(B.world);
     w->mem.release(B.world_first);
     w->mem.release(w->key);
@@ -235,10 +241,13 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * BP_PARTS * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
+    AXM_TRY(w->mem.get(B.mat, cap));
+    AXM_TRY(w->mem.get(B.bfl, cap));
     AXM_TRY(w->mem.get(B.slp, cap));
     AXM_TRY(w->mem.get(B.upar, cap));
     AXM_TRY(w->mem.get(B.imin, cap));
     AXM_TRY(w->mem.get(B.wake, cap));
+    AXM_TRY(w->mem.get(B.icnt, cap));
     AXM_TRY(w->mem.get(B.world, cap));
     AXM_TRY(w->mem.get(B.world_first, cap + 1));
     AXM_TRY(w->mem.get(w->key, cap));
@@ -419,6 +428,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     w->mem.release(w->cval);
     w->mem.release(w->cval_alt);
     w->mem.release(w->csort_tmp);
+    w->mem.release(w->isl_list);
     AXM_TRY(w->mem.get(w->m_ab, cap));
     AXM_TRY(w->mem.get(w->m_np, cap));
     AXM_TRY(w->mem.get(w->c_pt, (size_t)cap * 4));
@@ -431,6 +441,7 @@ axm_status alloc_manifolds(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->cval, cap));
     AXM_TRY(w->mem.get(w->cval_alt, cap));
     AXM_TRY(w->mem.get(w->csort_tmp, radix_tmp_words(cap)));
+    AXM_TRY(w->mem.get(w->isl_list, cap));
     if (hipMemset(w->csort_tmp, 0, 4 * radix_tmp_words(cap)) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     w->cap_man = cap;
     if (w->rcap[w->cur] < cap) AXM_TRY(alloc_rows(w, w->cur, cap));
@@ -452,6 +463,7 @@ StepParams make_params(const axm_world_s* w, float dt) {
     sp.soft1 = 1.0f + d.softness;
     sp.keep = 1.0f / sp.soft1;
     sp.allow_sleep = d.allow_sleep;
+    sp.use_filter = w->use_filter ? 1u : 0u;
     sp.sleep_lin2 = d.sleep_linear * d.sleep_linear;
     sp.sleep_ang2 = d.sleep_angular * d.sleep_angular;
     sp.sleep_time = d.sleep_time;
@@ -535,6 +547,12 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
                      s);
 }
 
+// Whether the step solves islands of at most ISLAND_LOCAL_MAX manifolds on their own
+// (not with a per-sweep residual history, which is one over all manifolds).
+uint32_t island_split(const axm_world_s* w) {
+    return w->desc.island_solve && !w->desc.record_residual && w->n < ISLAND_ROOT_LIMIT ? 1u : 0u;
+}
+
 // ---- narrowphase into per-pair staging, compaction into manifolds (pair order)
 void enqueue_narrow(axm_world_s* w, const StepParams& sp) {
     hipStream_t s = w->stream;
@@ -575,15 +593,20 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     pca.colour = w->R[pv].colour;
     pca.cap = w->rcap[pv];
     pca.valid = (w->prev_valid && pca.key) ? 1u : 0u;
-    // ---- colouring and the stable order by colour
+    // ---- islands, colouring and the solve order (by colour; the small islands after the
+    // large ones, island by island)
+    launch_isl_union(w->B, n, w->m_ab, w->cap_man, w->cnt, s);
     // bdone is the per-body manifold count of the colouring (cleared by k_begin)
     launch_colour(w->B, w->m_ab, w->cap_man, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->clist,
                   w->bar, w->cnt, w->colour_blocks, pca, s);
-    launch_col_sort_input(w->col, w->cap_man, w->ckey, w->cval, w->cnt, s);
-    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, 8, w->csort_tmp, s, &w->cnt->m_eff,
-                   &w->cnt->sticky);
-    const uint32_t* scol = w->ckey_alt;  // one 8-bit pass: the result is in the alternate buffers
-    const uint32_t* order = w->cval_alt;
+    const uint32_t split = island_split(w);
+    launch_col_sort_input(w->B, w->m_ab, w->col, w->cap_man, w->ckey, w->cval, w->cnt, split, s);
+    radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, split ? 32 : 8, w->csort_tmp, s,
+                   &w->cnt->m_eff, &w->cnt->sticky);
+    // an odd number of 8-bit passes leaves the result in the alternate buffers
+    const uint32_t* scol = split ? w->ckey : w->ckey_alt;
+    const uint32_t* order = split ? w->cval : w->cval_alt;
+    w->solve_order = order;
     launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
@@ -593,6 +616,12 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
         launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
                            2u * (uint32_t)w->cu_count, s);
         launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
+    }
+    if (split) {
+        const uint32_t max_it =
+            w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
+        launch_pgs_local(w->B, sp, R, w->cnt, rc, scol, w->isl_list, w->desc.residual_tol, max_it,
+                         2u * (uint32_t)w->cu_count, s);
     }
     launch_fill_u32(w->res_bits, 0u, res_words(w->res_cap), s);  // residuals, then check arrivals
     return AXM_SUCCESS;
@@ -654,6 +683,13 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
                         w->sweep_lds, w->sweep_tune, s, e0, e1, stop);
         it = max_it;  // at most; the count the launch ran is read back with the counters
     }
+    bool all_local = false;
+    if (sweep && o.colour_launches && !off.empty() && off.back() == 0u && w->h_cnt->m_eff > 0u) {
+        // every island was solved on its own: no sweep of the large ones runs
+        all_local = true;
+        it = max_it;
+        launch_fill_u32(&w->cnt->tol_hit, 1u, 1u, s);
+    }
     while (it < max_it && !dev_checks) {
         uint32_t seg = max_it - it;
         if (w->desc.record_residual) seg = 1;
@@ -689,13 +725,14 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             }
         }
     }
+    if (all_local) it = 0;
     w->last_iters = it;
     if (!post) return AXM_SUCCESS;
     w->last_dt = dt;
     launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
     launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
-    launch_islands(w->B, sp, n, w->m_ab, w->cap_man, w->cnt, s);
+    launch_islands(w->B, sp, n, w->cnt, s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[4], s));
     if (w->prof_on && !o.capture) {
         AXM_HIP(hipStreamSynchronize(s));
@@ -727,7 +764,7 @@ void fill_stats(axm_world_s* w, axm_step_stats& S) {
     S.contacts = spread_total(h, &SpreadRow::contacts);
     S.colours = h.max_colour;
     S.colour_rounds = h.last_round;
-    S.iterations = w->last_iters;
+    S.iterations = std::max(w->last_iters, h.loc_sweeps);  // the most sweeps any island ran
     S.gjk_fail = h.gjk_fail;
     S.epa_fail = h.epa_fail;
     S.warm_hits = spread_total(h, &SpreadRow::warm_hits);
@@ -814,18 +851,25 @@ axm_status step_sync(axm_world_s* w, float dt, StepOpts o, axm_step_stats* st) {
     fill_stats(w, S);
     const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || w->desc.residual_tol > 0.0f;
     w->h_res.clear();
+    // the islands solved on their own are finished before the large ones' sweeps: every
+    // entry of the history reports the larger of the two residuals
+    float loc_res;
+    std::memcpy(&loc_res, &w->h_cnt->loc_res_bits, 4);
     if (want_res && w->last_iters) {
         std::vector<uint32_t> bits(w->last_iters);
         AXM_HIP(hipMemcpy(bits.data(), w->res_bits, 4ull * w->last_iters, hipMemcpyDeviceToHost));
         for (uint32_t k = 0; k < w->last_iters; ++k) {
             float f;
             std::memcpy(&f, &bits[k], 4);
-            if (bits[k] != 0u || w->desc.record_residual || k + 1 == w->last_iters) w->h_res.push_back(f);
+            if (bits[k] != 0u || w->desc.record_residual || k + 1 == w->last_iters)
+                w->h_res.push_back(std::max(f, loc_res));
         }
-        if (!w->h_res.empty()) {
-            S.residual = w->h_res.back();
-            S.residual_first = w->h_res.front();
-        }
+    } else if (want_res && w->h_cnt->loc_sweeps) {
+        w->h_res.push_back(loc_res);  // only islands solved on their own
+    }
+    if (!w->h_res.empty()) {
+        S.residual = w->h_res.back();
+        S.residual_first = w->h_res.front();
     }
     if (timing) {
         float t[4] = {0, 0, 0, 0};
@@ -962,8 +1006,10 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         return AXM_NULL_POINTER;
     if (n >= (1u << 29)) return AXM_OUT_OF_RANGE;
     (void)hipSetDevice(w->desc.device);
-    std::vector<float4> x(n), q(n), v(n), av(n), ii(n), sh(n);
+    std::vector<float4> x(n), q(n), v(n), av(n), ii(n), sh(n), mat(n);
+    std::vector<uint4> bfl(n);
     std::vector<uint32_t> wd(n);
+    const axm_world_desc& D = w->desc;
     float rmax = 0.0f;
     uint32_t nw = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1008,6 +1054,22 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         float tf;
         std::memcpy(&tf, &tb, 4);
         sh[i] = make_float4(pr[0], pr[1], pr[2], tf);
+        float mu = D.friction, e = D.restitution, ld = D.lin_damp, ad = D.ang_damp;
+        if (b->material) {
+            const float* mt = b->material + 4 * i;
+            if (!std::isfinite(mt[0]) || mt[0] < 0.0f || !std::isfinite(mt[1]) || mt[1] < 0.0f ||
+                !std::isfinite(mt[2]) || !std::isfinite(mt[3]))
+                return AXM_INVALID_RIGID_BODY;
+            mu = mt[0];
+            e = mt[1];
+            ld = mt[2];
+            ad = mt[3];
+        }
+        mat[i] = make_float4(mu, e, 1.0f - ld, 1.0f - ad);
+        const uint32_t fl = b->flags ? b->flags[i] : 0u;
+        if ((fl & BODY_KINEMATIC) && im != 0.0f) return AXM_INVALID_RIGID_BODY;  // kinematic bodies have no mass
+        bfl[i] = b->filter ? make_uint4(fl, b->filter[3 * i], b->filter[3 * i + 1], b->filter[3 * i + 2])
+                           : make_uint4(fl, 1u, 0xFFFFFFFFu, 0u);
     }
     if (nw > (1u << 22)) return AXM_OUT_OF_RANGE;
     std::vector<uint32_t> first(nw + 1, 0);
@@ -1031,6 +1093,8 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         AXM_HIP(hipMemcpyAsync(w->B.ii, ii.data(), 16ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.shape, sh.data(), 16ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.world, wd.data(), 4ull * n, hipMemcpyHostToDevice, s));
+        AXM_HIP(hipMemcpyAsync(w->B.mat, mat.data(), 16ull * n, hipMemcpyHostToDevice, s));
+        AXM_HIP(hipMemcpyAsync(w->B.bfl, bfl.data(), 16ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * (nw + 1), hipMemcpyHostToDevice, s));
         // every body starts awake with no rest time and no island; no island has a wake request
         const std::vector<uint2> slp(n, make_uint2(0u, LABEL_NONE));
@@ -1050,6 +1114,7 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
     }
     AXM_HIP(hipStreamSynchronize(s));
     w->n = n;
+    w->use_filter = b->filter != nullptr;
     w->n_worlds = std::max(nw, 1u);
     w->cell = cell;
     w->prev_valid = false;
@@ -1213,6 +1278,31 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         fill_stats(w, S);
         *last = S;
     }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_get_desc(axm_world w, axm_world_desc* out) {
+    if (!w || !out) return AXM_NULL_POINTER;
+    *out = w->desc;
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_configure(axm_world w, const axm_world_desc* desc) {
+    if (!w || !desc) return AXM_NULL_POINTER;
+    if (desc->num_planes > AXM_MAX_PLANES || !(desc->softness >= 0.0f) || !(desc->friction >= 0.0f) ||
+        !std::isfinite(desc->gravity[0]) || !std::isfinite(desc->gravity[1]) || !std::isfinite(desc->gravity[2]))
+        return AXM_INVALID_PARAMETER;
+    axm_world_desc d = *desc;
+    // what sizes or places the buffers stays
+    d.cell_size = w->desc.cell_size;
+    for (int k = 0; k < 3; ++k) d.origin[k] = w->desc.origin[k];
+    d.max_bodies = w->desc.max_bodies;
+    d.max_pairs = w->desc.max_pairs;
+    d.max_manifolds = w->desc.max_manifolds;
+    d.auto_grow = w->desc.auto_grow;
+    d.device = w->desc.device;
+    w->desc = d;
+    w->drop_graphs();
     return AXM_SUCCESS;
 }
 
@@ -1385,7 +1475,8 @@ axm_status axm_world_get_colours(axm_world w, uint32_t* colour, uint32_t* solve_
     if (cap < M) return AXM_OUT_OF_RANGE;
     if (M) {
         if (colour) AXM_HIP(hipMemcpy(colour, w->col, 4ull * M, hipMemcpyDeviceToHost));
-        if (solve_order) AXM_HIP(hipMemcpy(solve_order, w->cval_alt, 4ull * M, hipMemcpyDeviceToHost));
+        if (solve_order && w->solve_order)
+            AXM_HIP(hipMemcpy(solve_order, w->solve_order, 4ull * M, hipMemcpyDeviceToHost));
     }
     return AXM_SUCCESS;
 }
@@ -1415,6 +1506,29 @@ axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, ui
             if (final_) { final_[o] = on ? l[s].x : 0.f; final_[o + 1] = on ? l[s].y : 0.f; final_[o + 2] = on ? l[s].z : 0.f; }
             if (initial) { initial[o] = on ? l0[s].x : 0.f; initial[o + 1] = on ? l0[s].y : 0.f; initial[o + 2] = on ? l0[s].z : 0.f; }
         }
+    }
+    return AXM_SUCCESS;
+}
+
+axm_status axm_world_get_friction_bounds(axm_world w, float* bound, uint32_t cap, uint32_t* count) {
+    if (!w || !count) return AXM_NULL_POINTER;
+    if (!w->have_step) return AXM_GPU_INVALID_OPERATION;
+    (void)hipSetDevice(w->desc.device);
+    AXM_TRY(read_counters(w));
+    const uint32_t M = w->h_cnt->m_eff;
+    *count = M;
+    if (cap < M) return AXM_OUT_OF_RANGE;
+    if (!M || !bound) return AXM_SUCCESS;
+    const int set = 1 - w->cur;  // rows of the last step
+    const SolverRows& R = w->R[set];
+    const uint32_t rc = w->rcap[set];
+    std::vector<uint4> hdr(M);
+    AXM_HIP(hipMemcpy(hdr.data(), R.hdr, 16ull * M, hipMemcpyDeviceToHost));
+    std::vector<float4> row(M);
+    for (uint32_t k = 0; k < 4; ++k) {
+        // row 8 of point k (t1's inverse-inertia row) carries the bound in .w
+        AXM_HIP(hipMemcpy(row.data(), R.J + ((size_t)k * JROWS + 8) * rc, 16ull * M, hipMemcpyDeviceToHost));
+        for (uint32_t s = 0; s < M; ++s) bound[(size_t)hdr[s].w * 4 + k] = k < hdr[s].z ? row[s].w : 0.0f;
     }
     return AXM_SUCCESS;
 }

@@ -136,7 +136,7 @@ def kinetic_energy(b, v, w):
 
 def fp64_resolve(w, bodies, desc):
     """One more GPU step (FP32) and the FP64 oracle solving the same manifolds from the same state,
-    warm start and colours at the same sweep count: the relative difference of the kinetic energy
+    warm start, friction bounds and colours at the same sweep count: the relative difference of the kinetic energy
     after the step (the energy drift of one step against the FP64 re-solve), the largest velocity
     difference, and the FP64 solve's residual."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -147,12 +147,13 @@ def fp64_resolve(w, bodies, desc):
     man = w.manifolds()
     col, _, _ = w.colours()
     warm, _ = w.impulses()
+    bounds = w.friction_bounds()
     after = w.get_bodies()
     d64 = axm.WorldDesc.from_buffer_copy(desc)
     d64.residual_tol = 0.0
     ow = orc.OracleWorld(orc.params_from_desc(d64), fp64=True)
     ow.set_bodies(before)
-    ost = ow.step_given(DT, st.iterations, man, warm=warm, colours=col)
+    ost = ow.step_given(DT, st.iterations, man, warm=warm, colours=col, bounds=bounds)
     _, _, v64, w64 = ow.get_state()
     ow.close()
     e32 = kinetic_energy(bodies, after.lin_vel, after.ang_vel)
@@ -298,10 +299,22 @@ def pile_section(steps, no_cpu, cpu_seconds):
     return out
 
 
+def settled(cfg, seed, n, worlds, steps, sweeps):
+    """The state of a scene after `steps` plain steps at a fixed sweep count (graph replay)."""
+    b, d = axm.scene(cfg, seed, n, worlds)
+    w = axm.World(d)
+    w.set_bodies(b)
+    w.run(DT, sweeps, steps, graph=True)
+    s = state_of(w, b)
+    w.close()
+    return s, d
+
+
 def tolerance_comparison(tol, max_sweeps, worlds):
     """Body-steps/s and sweeps per step when every step iterates to residual <= tol: the plain
     previous-step warm start against the learned one (network trained here on the GPU on contacts
-    logged from the plain solver), on one held-out 1k-body scene and on a batch of `worlds` worlds."""
+    logged from the plain solver on seeded dense piles), on a held-out dense pile settled for 300
+    steps and on a batch of `worlds` independent worlds after 60 steps."""
     import warmstart
 
     t0 = time.time()
@@ -311,22 +324,22 @@ def tolerance_comparison(tol, max_sweeps, worlds):
                         "loss_last": round(float(losses[-1]), 5), "seconds": round(time.time() - t0, 2),
                         "data": warmstart.TRAIN_DATA}}
     log("warm-start network trained on %d contacts in %.1fs" % (rows, time.time() - t0))
-    b, d = axm.scene(0, SEED_BASE + 2, 1000)
-    b3, d3 = axm.scene(3, SEED_BASE + 3000, 0, worlds)
-    cases = (("single_1k_plain", b, d, axm.WARM_CACHE, None, 60, 120, 1),
-             ("single_1k_learned", b, d, axm.WARM_MLP, net, 60, 120, 1),
-             ("batched_plain", b3, d3, axm.WARM_CACHE, None, 60, 30, 1),
-             ("batched_learned", b3, d3, axm.WARM_MLP, net, 60, 30, 1))
-    for name, bb, dd, warm, nn, warmup, steps, every in cases:
+    pile, dp = settled(1, SEED_BASE + 1000 + 2 * 200, 0, 0, 300, 20)
+    batch, db = settled(3, SEED_BASE + 3000, 0, worlds, 60, 10)
+    cases = (("pile_plain", pile, dp, axm.WARM_CACHE, None, 3, 20),
+             ("pile_learned", pile, dp, axm.WARM_MLP, net, 3, 20),
+             ("batched_plain", batch, db, axm.WARM_CACHE, None, 3, 30),
+             ("batched_learned", batch, db, axm.WARM_MLP, net, 3, 30))
+    for name, bb, dd, warm, nn, warmup, steps in cases:
         tm = {}
         it, reached = warmstart.iterations_to_tolerance(bb, dd, steps, tol, max_sweeps, warm, net=nn, warmup=warmup,
-                                                        every=every, timing=tm)
+                                                        every=1, timing=tm)
         out[name] = {"bodies": bb.n, "steps": steps, "body_steps_per_s": round(bb.n * steps / tm["s"], 1),
                      "mean_sweeps": round(float(it.mean()), 2), "median_sweeps": float(np.median(it)),
-                     "fraction_reached": round(float(reached.mean()), 3), "residual_check_every": every}
+                     "fraction_reached": round(float(reached.mean()), 3)}
         log("%s: %.0f body-steps/s, mean sweeps %.1f, reached %.2f" % (name, out[name]["body_steps_per_s"],
                                                                        it.mean(), reached.mean()))
-    for kind in ("single_1k", "batched"):
+    for kind in ("pile", "batched"):
         p, l_ = out[kind + "_plain"], out[kind + "_learned"]
         out[kind + "_gain"] = {"sweeps_ratio": round(p["mean_sweeps"] / max(l_["mean_sweeps"], 1e-9), 3),
                                "rate_ratio": round(l_["body_steps_per_s"] / max(p["body_steps_per_s"], 1e-9), 3)}

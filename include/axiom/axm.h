@@ -98,6 +98,11 @@ typedef struct axm_world_desc {
     float sleep_linear;    /* PhysicsWorldConfig::sleepLinearThreshold (m/s) */
     float sleep_angular;   /* PhysicsWorldConfig::sleepAngularThreshold (rad/s) */
     float sleep_time;      /* PhysicsWorldConfig::sleepTimeThreshold (s) */
+    uint32_t island_solve; /* 1: every island of at most 4096 manifolds (bodies joined by contacts; static
+                              bodies and planes join nothing) is solved on its own and, with residual_tol,
+                              stops at its own residual; stats.iterations reports the most sweeps any island
+                              ran.  Without a tolerance the results are identical either way.  Not applied
+                              with record_residual or with 2^23 bodies or more */
     /* capacities (0 = sized from the body count) */
     uint32_t max_bodies;
     uint32_t max_pairs;
@@ -126,6 +131,17 @@ typedef struct axm_body_soa {
     uint32_t* shape_type;/* n */
     float* shape_params; /* 3n */
     uint32_t* world;     /* n or NULL; batched worlds: world index per body (bodies of one world contiguous) */
+    /* Optional per-body extras (NULL: the world description's values), following
+     * gui::RigidBodyData's material, damping, type and filter fields
+     * (include/axiom/gui/body_inspector.hpp:29-42, 76-87): */
+    float* material;     /* 4n or NULL: friction, restitution, linear damping, angular damping.  A contact uses
+                            mu = sqrt(mu_A mu_B) (mu_A when equal) and e = max(e_A, e_B); a plane has the
+                            world's friction and restitution */
+    uint32_t* flags;     /* n or NULL: bit 0 kinematic (BodyType::Kinematic: inverse mass 0, moved by its own
+                            velocity, pushes dynamic bodies and ignores static ones) */
+    uint32_t* filter;    /* 3n or NULL: FilterInfo categoryBits, maskBits, groupIndex (as int32).  Two bodies
+                            collide iff they share a nonzero group that is positive, or, otherwise, each one's
+                            category meets the other's mask; planes collide with every dynamic body */
 } axm_body_soa;
 
 /* Per-step statistics.  Mirrors gui::PhysicsWorldStats
@@ -178,6 +194,13 @@ void axm_world_desc_default(axm_world_desc* desc);
  * BufferAllocationFailed / GPU_OPERATION_FAILED on failure. */
 axm_status axm_world_create(const axm_world_desc* desc, axm_world* out);
 axm_status axm_world_destroy(axm_world w);
+/* Replaces the solver settings of a live world (gravity, friction, restitution and the other
+ * contact constants, softness, sleep settings, damping, planes, warm start, residual tolerance;
+ * as gui::PhysicsWorldConfig changes between frames, physics_panel.hpp:34-43).  Capacities, the
+ * grid, auto_grow and the device keep their values; captured step graphs are rebuilt. */
+axm_status axm_world_configure(axm_world w, const axm_world_desc* desc);
+/* The world's current description. */
+axm_status axm_world_get_desc(axm_world w, axm_world_desc* out);
 
 /* Uploads n bodies (copies in).  Invalid bodies (NaN state, negative mass,
  * unknown shape) return InvalidRigidBody / InvalidShape and leave the world unchanged. */
@@ -253,6 +276,11 @@ axm_status axm_world_get_colours(axm_world w, uint32_t* colour, uint32_t* solve_
                                  uint32_t* num_colours);
 /* impulses per manifold (pair order) x 4 points x (normal, t1, t2): initial (warm start) and final. */
 axm_status axm_world_get_impulses(axm_world w, float* initial, float* final_, uint32_t cap, uint32_t* count);
+/* friction bound mu * B of every contact point of the last step (manifold order x 4 points):
+ * each friction row is clamped to |lambda_t| <= mu * B, with B fixed when the step starts: the
+ * point's normal impulse of the previous step when the warm-start cache holds its feature,
+ * else m_N (target - v_n)+, the impulse that stops its approach on its own. */
+axm_status axm_world_get_friction_bounds(axm_world w, float* bound, uint32_t cap, uint32_t* count);
 /* residual after each sweep of the last step (needs record_residual). */
 axm_status axm_world_get_residuals(axm_world w, float* out, uint32_t cap, uint32_t* count);
 /* velocities entering the solver of the last step (after gravity). */

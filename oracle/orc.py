@@ -22,7 +22,7 @@ class Params(C.Structure):
         ("origin", C.c_float * 3), ("num_planes", C.c_uint32), ("planes", (C.c_float * 4) * MAX_PLANES),
         ("warm_start", C.c_uint32), ("record_residual", C.c_uint32), ("softness", C.c_float),
         ("allow_sleep", C.c_uint32), ("sleep_linear", C.c_float), ("sleep_angular", C.c_float),
-        ("sleep_time", C.c_float),
+        ("sleep_time", C.c_float), ("island_solve", C.c_uint32),
     ]
 
 
@@ -63,6 +63,7 @@ def lib():
     L.orc_create.argtypes = [C.POINTER(Params), C.c_int]
     L.orc_destroy.argtypes = [P]
     L.orc_set_threads.argtypes = [P, C.c_uint32]
+    L.orc_set_body_extras.argtypes = [P, f32p, u32p, u32p]
     L.orc_get_islands.argtypes = [P, u32p, u32p, f32p]
     L.orc_set_tolerance.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32]
     L.orc_set_params.argtypes = [P, C.POINTER(Params)]
@@ -91,7 +92,9 @@ def lib():
     L.orc_get_residuals.restype = C.c_uint32
     L.orc_get_residuals.argtypes = [P, f32p, C.c_uint32]
     L.orc_step_given.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, u32p, u32p, u32p, f32p, f32p, f32p, u32p, f32p,
-                                 u32p, C.POINTER(Stats)]
+                                 u32p, f32p, C.POINTER(Stats)]
+    L.orc_get_friction_bounds.restype = C.c_uint32
+    L.orc_get_friction_bounds.argtypes = [P, f32p, C.c_uint32]
     L.orc_collide.restype = C.c_int
     L.orc_collide.argtypes = [C.c_int, C.c_uint32, f32p, f32p, f32p, C.c_uint32, f32p, f32p, f32p, C.c_float, u32p,
                               f32p, f32p, f32p, u32p]
@@ -186,6 +189,11 @@ class OracleWorld:
         self.L.orc_set_bodies(self.h, b.n, _p(b.pos), _p(b.quat), _p(b.lin_vel), _p(b.ang_vel), _p(b.inv_mass),
                               _p(b.inv_inertia), _p(b.shape_type, C.c_uint32), _p(b.shape_params),
                               _p(b.world, C.c_uint32))
+        mat, fl, fi = getattr(b, "material", None), getattr(b, "flags", None), getattr(b, "filter", None)
+        if mat is not None or fl is not None or fi is not None:
+            self._extras = [None if a is None else np.ascontiguousarray(a) for a in (mat, fl, fi)]
+            self.L.orc_set_body_extras(self.h, _p(self._extras[0]), _p(self._extras[1], C.c_uint32),
+                                       _p(self._extras[2], C.c_uint32))
 
     def get_state(self):
         pos = np.zeros((self.n, 3), np.float32)
@@ -271,17 +279,26 @@ class OracleWorld:
         self.L.orc_get_residuals(self.h, _p(r), n)
         return r
 
-    def step_given(self, dt, iters, man, warm=None, colours=None):
-        """One step on given manifolds (optionally with a given warm start and colouring)."""
+    def step_given(self, dt, iters, man, warm=None, colours=None, bounds=None):
+        """One step on given manifolds (optionally with a given warm start, colouring and friction
+        bounds [M, 4])."""
         st = Stats()
         M = len(man["a"])
         arr = {k: np.ascontiguousarray(v) for k, v in man.items()}
         w = np.ascontiguousarray(warm, np.float32) if warm is not None else None
         c = np.ascontiguousarray(colours, np.uint32) if colours is not None else None
+        fb = np.ascontiguousarray(bounds, np.float32) if bounds is not None else None
         self.L.orc_step_given(self.h, dt, iters, M, _p(arr["a"], C.c_uint32), _p(arr["b"], C.c_uint32),
                               _p(arr["npts"], C.c_uint32), _p(arr["pos"]), _p(arr["normal"]), _p(arr["depth"]),
-                              _p(arr["feature"], C.c_uint32), _p(w), _p(c, C.c_uint32), C.byref(st))
+                              _p(arr["feature"], C.c_uint32), _p(w), _p(c, C.c_uint32), _p(fb), C.byref(st))
         return st
+
+    def friction_bounds(self):
+        """Friction bound of every contact point of the last step, [M, 4] in manifold order."""
+        M = self.L.orc_get_friction_bounds(self.h, None, 0)
+        out = np.zeros((M, 4), np.float32)
+        self.L.orc_get_friction_bounds(self.h, _p(out), M)
+        return out
 
 
 def collide(ta, pa, xa, qa, tb, pb, xb, qb, margin=0.02, fp64=False):

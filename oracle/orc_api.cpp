@@ -50,6 +50,7 @@ template <class T> void set_bodies(World<T>& W, uint32_t n, const float* pos, co
     W.cache.clear();
     W.nsteps = 0;
     W.reset_sleep();
+    W.reset_extras();
 }
 
 template <class T> void get_bodies(const World<T>& W, float* pos, float* quat, float* vel, float* angvel) {
@@ -185,6 +186,41 @@ void* orc_create(const Params* p, int fp64) {
 }
 
 void orc_destroy(void* h) { delete (Handle*)h; }
+
+// Per-body extras after orc_set_bodies (NULL arrays keep the defaults): material (4n: friction,
+// restitution, linear damping, angular damping), flags (n: bit 0 kinematic), filter (3n: category,
+// mask, group as int32).
+void orc_set_body_extras(void* h, const float* material, const uint32_t* flags, const uint32_t* filter) {
+    DISPATCH(h, {
+        W.reset_extras();
+        for (uint32_t i = 0; i < W.n; ++i) {
+            if (material) {
+                W.mu_b[i] = T(material[4 * i]);
+                W.e_b[i] = T(material[4 * i + 1]);
+                W.lkeep[i] = T(1) - T(material[4 * i + 2]);
+                W.akeep[i] = T(1) - T(material[4 * i + 3]);
+            }
+            if (flags) W.kin[i] = (flags[i] & 1u) ? 1 : 0;
+            if (filter) {
+                W.fcat[i] = filter[3 * i];
+                W.fmask[i] = filter[3 * i + 1];
+                W.fgroup[i] = (int32_t)filter[3 * i + 2];
+            }
+        }
+        W.use_filter = filter != nullptr;
+    });
+}
+
+// Friction bound of every contact point of the last step (M x 4, manifold order).
+uint32_t orc_get_friction_bounds(void* h, float* out, uint32_t cap) {
+    uint32_t M = 0;
+    DISPATCH(h, {
+        M = (uint32_t)W.man.size();
+        if (out)
+            for (size_t s = 0; s < (size_t)std::min(M, cap) * 4 && s < W.fbound.size(); ++s) out[s] = (float)W.fbound[s];
+    });
+    return M;
+}
 
 // Island label (0x7FFFFFFF: none), asleep flag and rest time of every body after the last step.
 void orc_get_islands(void* h, uint32_t* label, uint32_t* asleep, float* rest) {
@@ -388,7 +424,7 @@ uint32_t orc_get_residuals(void* h, float* out, uint32_t cap) {
 // position integration.
 void orc_step_given(void* h, float dt, uint32_t iters, uint32_t M, const uint32_t* a, const uint32_t* b,
                     const uint32_t* np, const float* p, const float* n, const float* depth, const uint32_t* fid,
-                    const float* warm, const uint32_t* colour_in, Stats* st) {
+                    const float* warm, const uint32_t* colour_in, const float* bounds, Stats* st) {
     DISPATCH(h, {
         W.st = Stats{};
         load_manifolds(W, M, a, b, np, p, n, depth, fid);
@@ -401,7 +437,14 @@ void orc_step_given(void* h, float dt, uint32_t iters, uint32_t M, const uint32_
             wt.resize((size_t)M * 12);
             for (size_t s = 0; s < wt.size(); ++s) wt[s] = T(warm[s]);
         }
+        std::vector<T> bt;
+        if (bounds) {
+            bt.resize((size_t)M * 4);
+            for (size_t s = 0; s < bt.size(); ++s) bt[s] = T(bounds[s]);
+            W.fb_in = bt.data();
+        }
         W.solve(T(dt), iters, warm ? wt.data() : nullptr);
+        W.fb_in = nullptr;
         W.integrate_positions(T(dt));
         W.islands_and_sleep(T(dt));
         ++W.nsteps;

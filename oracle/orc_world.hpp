@@ -26,6 +26,13 @@
 //   7. prep per point, warm start (cache keyed by (a, b, feature id))
 //   8. I PGS sweeps, colour by colour, manifold by manifold: friction rows of
 //      all points, then normal rows (iteration 0 first applies the warm start).
+//      Each friction row is clamped to |lam_t| <= mu * B (pyramid), with the bound B
+//      fixed when the step starts: the point's normal impulse of the previous step
+//      when the warm-start cache holds its feature, else the impulse m_N (target -
+//      v_n)+ that stops the point's approach on its own.  With fixed bounds the step's problem is a
+//      convex quadratic program over a box, so the sweeps cannot cycle (a bound that
+//      follows lam_n during the sweeps can leave friction and normal rows trading
+//      impulse in a period-2 cycle that never settles).
 //      Every row is regularised with a compliance eps*k proportional to its own
 //      diagonal k (soft constraint, constraint-force mixing): the row solves
 //      v + eps*k*lam = target, so one visit is
@@ -36,6 +43,13 @@
 //      mutually attainable, which otherwise leaves PGS drifting along the
 //      null space with a constant residual.
 //   9. x += v dt; q += (dt/2) (w,0) q; q /= |q|
+//  Per-body extras (defaults from the world parameters): material {mu, e} combined
+//  per contact as mu = sqrt(mu_A mu_B) (mu_A when equal) and e = max(e_A, e_B), a
+//  plane taking the world's mu and e; per-body damping factors; kinematic bodies
+//  (no mass, moved by their own velocity, pushing dynamic bodies as static ones
+//  do); collision filters (category, mask, group: a pair with one nonzero group
+//  collides iff that group is positive, else iff each category meets the other's
+//  mask; planes collide with every dynamic body).
 //  10. islands: connected components of the awake dynamic bodies over this step's
 //      manifolds (a manifold with a static, sleeping or plane partner does not
 //      connect); an island's label is its smallest body index.  Sleeping: an
@@ -77,6 +91,7 @@ struct Params {
     float sleep_linear;           // m/s: |v| below this counts as resting
     float sleep_angular;          // rad/s: |w| below this counts as resting
     float sleep_time;             // s
+    uint32_t island_solve;        // islands of at most LOCAL_MANIFOLDS manifolds solved on their own
 };
 
 struct Stats {
@@ -126,7 +141,38 @@ template <class T> struct World {
     std::vector<T> rest;
     std::vector<uint8_t> asleep;
     std::vector<uint32_t> wake;  // indexed by label: the step number at which that island wakes
+    // per-body extras
+    std::vector<T> mu_b, e_b, lkeep, akeep;
+    std::vector<uint8_t> kin;
+    std::vector<uint32_t> fcat, fmask;
+    std::vector<int32_t> fgroup;
+    bool use_filter = false;
     Stats st{};
+
+    void reset_extras() {
+        mu_b.assign(n, T(P.friction));
+        e_b.assign(n, T(P.restitution));
+        lkeep.assign(n, T(1) - T(P.lin_damp));
+        akeep.assign(n, T(1) - T(P.ang_damp));
+        kin.assign(n, 0);
+        fcat.assign(n, 1u);
+        fmask.assign(n, 0xFFFFFFFFu);
+        fgroup.assign(n, 0);
+        use_filter = false;
+    }
+    bool should_collide(uint32_t i, uint32_t j) const {
+        if (!use_filter) return true;
+        if (fgroup[i] == fgroup[j] && fgroup[i] != 0) return fgroup[i] > 0;
+        return (fmask[i] & fcat[j]) != 0u && (fcat[i] & fmask[j]) != 0u;
+    }
+    T mix_mu(uint32_t a, uint32_t b) const {
+        if (b & 0x80000000u) return T(P.friction) == mu_b[a] ? mu_b[a] : std::sqrt(mu_b[a] * T(P.friction));
+        return mu_b[a] == mu_b[b] ? mu_b[a] : std::sqrt(mu_b[a] * mu_b[b]);
+    }
+    T mix_e(uint32_t a, uint32_t b) const {
+        const T eb = (b & 0x80000000u) ? T(P.restitution) : e_b[b];
+        return e_b[a] > eb ? e_b[a] : eb;
+    }
     // residual-stopped sweeps: with tol > 0 the residual is checked after every sweep and
     // the solve stops once it is <= tol (at most max_iters sweeps)
     T residual_tol = T(0);
@@ -268,6 +314,7 @@ template <class T> struct World {
                                 const uint32_t j = order[(size_t)(it - sk.begin())];
                                 if (j <= i) continue;
                                 if (!dynamic(i) && !dynamic(j)) continue;
+                                if (!should_collide(i, j)) continue;
                                 if (mn[i].x <= mx[j].x && mx[i].x >= mn[j].x && mn[i].y <= mx[j].y &&
                                     mx[i].y >= mn[j].y && mn[i].z <= mx[j].z && mx[i].z >= mn[j].z) {
                                     pa.push_back(i);
@@ -403,14 +450,75 @@ template <class T> struct World {
         order_by_colour(K);
     }
 
+    // Solve order (P.island_solve, without record_residual, below 2^23 bodies): the manifolds
+    // of the large islands first, by colour (then manifold index); then the small islands (at
+    // most LOCAL_MANIFOLDS manifolds; an island is the bodies joined by manifolds between two
+    // dynamic bodies), one after another in the order of their roots (smallest body index),
+    // each by colour (then manifold index).  A manifold with no dynamic body belongs to the
+    // large part.  colour_off covers the large islands' part [0, n_glob); local_off[i] is
+    // where small island i starts (local_off.back() = M).  Without the split every manifold
+    // is in the large part.
+    static constexpr uint32_t LOCAL_MANIFOLDS = 4096;
+    uint32_t n_glob = 0;
+    std::vector<uint32_t> local_off;
+    std::vector<uint32_t> step_root;  // island (union-find root) of every body in this step
+
+    void step_islands() {
+        const uint32_t M = (uint32_t)man.size();
+        step_root.resize(n);
+        for (uint32_t i = 0; i < n; ++i) step_root[i] = i;
+        for (uint32_t m = 0; m < M; ++m) {
+            const uint32_t a = man_a[m], b = man_b[m];
+            if ((b & PLANE_BIT) || !dynamic(a) || !dynamic(b)) continue;
+            uint32_t ra = uf_find(step_root, a), rb = uf_find(step_root, b);
+            if (ra < rb) step_root[rb] = ra;
+            else if (rb < ra) step_root[ra] = rb;
+        }
+        for (uint32_t i = 0; i < n; ++i) step_root[i] = uf_find(step_root, i);
+    }
+    // island of manifold m: that of its dynamic body (UINT32_MAX with none)
+    uint32_t man_root(uint32_t m) const {
+        const uint32_t a = man_a[m], b = man_b[m];
+        if (dynamic(a)) return step_root[a];
+        if (!(b & PLANE_BIT) && dynamic(b)) return step_root[b];
+        return UINT32_MAX;
+    }
+
     void order_by_colour(uint32_t K) {
         const uint32_t M = (uint32_t)man.size();
+        std::vector<uint8_t> loc(M, 0);
+        if (P.island_solve && !P.record_residual && n < (1u << 23)) {
+            step_islands();
+            std::vector<uint32_t> size(n, 0u);
+            for (uint32_t m = 0; m < M; ++m)
+                if (man_root(m) != UINT32_MAX) ++size[man_root(m)];
+            for (uint32_t m = 0; m < M; ++m) {
+                const uint32_t r = man_root(m);
+                loc[m] = r != UINT32_MAX && size[r] <= LOCAL_MANIFOLDS ? 1 : 0;
+            }
+        }
         colour_off.assign(K + 1, 0);
-        for (uint32_t m = 0; m < M; ++m) colour_off[colour[m] + 1]++;
+        for (uint32_t m = 0; m < M; ++m)
+            if (!loc[m]) colour_off[colour[m] + 1]++;
         for (uint32_t c = 0; c < K; ++c) colour_off[c + 1] += colour_off[c];
+        n_glob = colour_off[K];
         solve_order.resize(M);
         std::vector<uint32_t> fill(colour_off.begin(), colour_off.end() - 1);
-        for (uint32_t m = 0; m < M; ++m) solve_order[fill[colour[m]]++] = m;
+        std::vector<uint32_t> lm;
+        for (uint32_t m = 0; m < M; ++m) {
+            if (!loc[m]) solve_order[fill[colour[m]]++] = m;
+            else lm.push_back(m);
+        }
+        std::stable_sort(lm.begin(), lm.end(), [&](uint32_t x, uint32_t y) {
+            const uint32_t rx = man_root(x), ry = man_root(y);
+            return rx != ry ? rx < ry : colour[x] < colour[y];
+        });
+        local_off.clear();
+        for (size_t j = 0; j < lm.size(); ++j) {
+            solve_order[n_glob + j] = lm[j];
+            if (j == 0 || man_root(lm[j]) != man_root(lm[j - 1])) local_off.push_back(n_glob + (uint32_t)j);
+        }
+        local_off.push_back(M);
     }
 
     // ---------------------------------------------------------------- solver
@@ -421,6 +529,7 @@ template <class T> struct World {
         V3<T> rA, rB, n, t1, t2;
         V3<T> a[3], b[3], ia[3], ib[3];
         T mN, mT1, mT2, kN, kT1, kT2, target;
+        T fb;  // friction bound mu * B of this step
         V3<T> dir(int d) const { return d == 0 ? n : (d == 1 ? t1 : t2); }
         T mass(int d) const { return d == 0 ? mN : (d == 1 ? mT1 : mT2); }
     };
@@ -458,6 +567,9 @@ template <class T> struct World {
         return (B.v + cross(B.w, r.rB)) - (A.v + cross(A.w, r.rA));
     }
 
+    std::vector<T> fbound;  // friction bound of every point of the last step (M x 4)
+    const T* fb_in = nullptr;  // optional: bounds to use instead of computing them (M x 4)
+
     void solve(T dt, uint32_t iters, const T* warm_in /* optional: M x 4 x 3 in manifold order */) {
         const uint32_t M = (uint32_t)man.size();
         std::vector<SBody> sb(n);
@@ -481,8 +593,8 @@ template <class T> struct World {
         std::vector<Row> rows((size_t)M * 4);
         lambda.assign((size_t)M * 12, T(0));
         feat.assign((size_t)M * 4 * 24, T(0));
+        fbound.assign((size_t)M * 4, T(0));
         const T inv_dt = T(1) / dt;
-        const T mu = T(P.friction);
         const T soft1 = T(1.0f + P.softness);
         const T keep = T(1) / soft1;
         st.warm_hits = 0;
@@ -498,6 +610,7 @@ template <class T> struct World {
                 auto it = cache.find(ck);
                 if (it != cache.end()) ce = &it->second;
             }
+            const T mu = mix_mu(a, b), rest_e = mix_e(a, b);
             for (uint32_t k = 0; k < man[m].np; ++k) {
                 const CPt<T>& c = man[m].pt[k];
                 Row& r = rows[(size_t)m * 4 + k];
@@ -529,7 +642,7 @@ template <class T> struct World {
                     bias = c.depth * inv_dt;
                 }
                 if (vn0 < -T(P.rest_threshold)) {
-                    const T rb = -T(P.restitution) * vn0;
+                    const T rb = -rest_e * vn0;
                     if (rb > bias) bias = rb;
                 }
                 r.target = bias;
@@ -546,6 +659,12 @@ template <class T> struct World {
                             ++mhits[m];
                             break;
                         }
+                }
+                {
+                    const T est = r.mN * (bias - vn0);
+                    const T nb = hit ? Lp[0] : est;
+                    r.fb = fb_in ? fb_in[(size_t)m * 4 + k] : mu * (nb > T(0) ? nb : T(0));
+                    fbound[(size_t)m * 4 + k] = r.fb;
                 }
                 T* L = &lambda[((size_t)m * 4 + k) * 3];
                 for (int d = 0; d < 3; ++d) L[d] = warm_in ? warm_in[((size_t)m * 4 + k) * 3 + d] : Lp[d];
@@ -589,7 +708,7 @@ template <class T> struct World {
             for (uint32_t k = 0; k < np; ++k) {
                 const Row& r = rows[(size_t)m * 4 + k];
                 T* L = &lambda[((size_t)m * 4 + k) * 3];
-                const T maxf = mu * L[0];
+                const T maxf = r.fb;
                 for (int dd = 1; dd <= 2; ++dd) {
                     T nl = std::fma(-r.mass(dd), jvel(r, dd, A, B), keep * L[dd]);
                     nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
@@ -624,7 +743,7 @@ template <class T> struct World {
                         T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), keep * L[0]);
                         nl = nl > T(0) ? nl : T(0);
                         T e = std::fabs(nl - L[0]) * r.kN;
-                        const T maxf = mu * L[0];
+                        const T maxf = r.fb;
                         T t1 = std::fma(-r.mT1, jvel(r, 1, A, B), keep * L[1]);
                         t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
                         const T e1 = std::fabs(t1 - L[1]) * r.kT1;
@@ -667,7 +786,69 @@ template <class T> struct World {
             }
             residual_hist.push_back(residual());
         }
-        for (uint32_t it = 0; it < cap; ++it) {
+        // residual of the manifolds in solve slots [s0, s1)
+        auto residual_slots = [&](uint32_t s0, uint32_t s1) {
+            T res = T(0);
+            for (uint32_t s = s0; s < s1; ++s) {
+                const uint32_t m = solve_order[s];
+                const uint32_t a = man_a[m], b = man_b[m];
+                const SBody& A = sb[a];
+                const SBody& B = (b & PLANE_BIT) ? ground : sb[b];
+                for (uint32_t k = 0; k < man[m].np; ++k) {
+                    const Row& r = rows[(size_t)m * 4 + k];
+                    const T* L = &lambda[((size_t)m * 4 + k) * 3];
+                    T nl = std::fma(r.mN, r.target - jvel(r, 0, A, B), keep * L[0]);
+                    nl = nl > T(0) ? nl : T(0);
+                    T e = std::fabs(nl - L[0]) * r.kN;
+                    const T maxf = r.fb;
+                    T t1 = std::fma(-r.mT1, jvel(r, 1, A, B), keep * L[1]);
+                    t1 = t1 < -maxf ? -maxf : (t1 > maxf ? maxf : t1);
+                    const T e1 = std::fabs(t1 - L[1]) * r.kT1;
+                    T t2 = std::fma(-r.mT2, jvel(r, 2, A, B), keep * L[2]);
+                    t2 = t2 < -maxf ? -maxf : (t2 > maxf ? maxf : t2);
+                    const T e2 = std::fabs(t2 - L[2]) * r.kT2;
+                    if (e1 > e) e = e1;
+                    if (e2 > e) e = e2;
+                    if (e > res) res = e;
+                }
+            }
+            return res;
+        };
+        // small islands: each runs its own sweeps, visiting its slots in order, and (with a
+        // tolerance) stops after the first sweep that leaves its own residual <= tol
+        const uint32_t nloc = local_off.empty() ? 0u : (uint32_t)local_off.size() - 1u;
+        std::vector<uint32_t> loc_ran(nloc, 0u);
+        std::vector<T> loc_res(nloc, T(0));
+        if (cap > 0)
+            par(nloc, 64, [&](uint64_t, uint64_t i0, uint64_t i1) {
+                for (uint32_t i = (uint32_t)i0; i < (uint32_t)i1; ++i) {
+                    const uint32_t s0 = local_off[i], s1 = local_off[i + 1];
+                    uint32_t it = 0;
+                    T res = T(0);
+                    for (; it < cap; ++it) {
+                        for (uint32_t s = s0; s < s1; ++s) visit(s, it == 0);
+                        if (tol_mode || it + 1 == cap) {
+                            res = residual_slots(s0, s1);
+                            if (tol_mode && res <= residual_tol) {
+                                ++it;
+                                break;
+                            }
+                        }
+                    }
+                    loc_ran[i] = it > cap ? cap : it;
+                    loc_res[i] = res;
+                }
+            });
+        uint32_t loc_max = 0;
+        T loc_res_max = T(0);
+        for (uint32_t i = 0; i < nloc; ++i) {
+            loc_max = std::max(loc_max, loc_ran[i]);
+            if (loc_res[i] > loc_res_max) loc_res_max = loc_res[i];
+        }
+        // large islands: together, colour batch by colour batch, over solve slots [0, n_glob)
+        // (also with no manifold at all, so an empty step counts its sweeps as before)
+        const bool glob = n_glob > 0 || nloc == 0;
+        for (uint32_t it = 0; it < cap && glob; ++it) {
             for (uint32_t c = 0; c < K; ++c) {
                 const uint32_t s0 = colour_off[c], s1 = colour_off[c + 1];
                 if (s1 - s0 >= 512 && pool) {
@@ -681,11 +862,14 @@ template <class T> struct World {
             ran = it + 1;
             const bool check = tol_mode && ran % every == 0;
             if (P.record_residual || check || ran == cap) {
-                const T res = residual();
-                residual_hist.push_back(res);
-                if (check && res <= residual_tol) break;
+                // the large islands stop on their own residual; the step reports the largest
+                const T gres = nloc ? residual_slots(0, n_glob) : residual();
+                residual_hist.push_back(gres > loc_res_max ? gres : loc_res_max);
+                if (check && gres <= residual_tol) break;
             }
         }
+        if (!glob && cap > 0) residual_hist.push_back(loc_res_max);
+        ran = std::max(ran, loc_max);
         st.iterations = ran;
         st.residual = residual_hist.empty() ? 0.0f : (float)residual_hist.back();
         st.residual_first = residual_hist.empty() ? 0.0f : (float)residual_hist.front();
@@ -710,12 +894,11 @@ template <class T> struct World {
 
     void integrate_velocities(T dt) {
         const V3<T> g = {T(P.gravity[0]), T(P.gravity[1]), T(P.gravity[2])};
-        const T kl = T(1) - T(P.lin_damp), ka = T(1) - T(P.ang_damp);
         par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
             for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
                 if (!dynamic(i)) continue;
-                v[i] = (v[i] + g * dt) * kl;
-                w[i] = w[i] * ka;
+                v[i] = (v[i] + g * dt) * lkeep[i];
+                w[i] = w[i] * akeep[i];
             }
         });
     }
@@ -724,7 +907,7 @@ template <class T> struct World {
         const T hdt = T(0.5) * dt;
         par(n, 4096, [&](uint64_t, uint64_t b, uint64_t e) {
             for (uint32_t i = (uint32_t)b; i < (uint32_t)e; ++i) {
-                if (!dynamic(i)) continue;
+                if (!dynamic(i) && !kin[i]) continue;
                 x[i] = x[i] + v[i] * dt;
                 const Q4<T> wq = {w[i].x, w[i].y, w[i].z, T(0)};
                 const Q4<T> dq = qmul(wq, q[i]);

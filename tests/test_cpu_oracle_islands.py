@@ -4,6 +4,7 @@ an island falling asleep after sleep_time seconds, and a sleeping island woken t
 awake body touches it."""
 import numpy as np
 
+from helpers import seed
 from test_cpu_oracle_world import DT, make, world
 
 NONE = 0x7FFFFFFF
@@ -72,3 +73,52 @@ def test_resting_island_falls_asleep_and_wakes_on_contact(orc, axm):
             assert not asleep[0] and lab[0] == lab[1] == 0
             break
     assert slept_at is not None and slept_at < 60 and contact_at is not None
+
+
+def _settled_oracle(orc, axm, config, n, steps):
+    b, d = axm.scene(config, seed(config), n)
+    ow = orc.OracleWorld(orc.params_from_desc(d))
+    ow.set_bodies(b)
+    ow.set_threads(4)
+    for _ in range(steps):
+        ow.step(DT, 20)
+    pos, quat, v, w = ow.get_state()
+    s = b.copy()
+    s.pos, s.quat, s.lin_vel, s.ang_vel = pos, quat, v, w
+    return s, d
+
+
+def test_fixed_sweeps_unchanged_by_island_split(orc, axm):
+    """Without a tolerance every island runs every sweep, and islands share no dynamic body, so each
+    body sees the same visits in the same order: bit-identical steps with and without the split."""
+    s, d = _settled_oracle(orc, axm, 0, 1000, 100)
+    out = []
+    for split in (0, 1):
+        d.island_solve = split
+        ow = orc.OracleWorld(orc.params_from_desc(d))
+        ow.set_bodies(s)
+        for _ in range(3):
+            st = ow.step(DT, 15)
+        out.append((ow.get_state(), st.residual, st.iterations))
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2] == 15
+    for x, y in zip(out[0][0], out[1][0]):
+        assert np.array_equal(x, y)
+
+
+def test_island_stops_never_need_more_sweeps(orc, axm):
+    """With a tolerance, an island solved on its own stops at the first sweep that leaves its own
+    residual <= tol; solved together, the step stops at the first sweep where every island is below
+    it, which no island reaches before its own first.  So from the same state the split step runs
+    at most as many sweeps, and both reach the tolerance."""
+    s, d = _settled_oracle(orc, axm, 0, 1000, 100)
+    res = {}
+    for split in (0, 1):
+        d.island_solve = split
+        ow = orc.OracleWorld(orc.params_from_desc(d))
+        ow.set_bodies(s)
+        ow.set_tolerance(1e-4, 300, 1)
+        st = ow.step(DT, 300)
+        res[split] = st
+    assert res[0].residual <= 1e-4 and res[1].residual <= 1e-4
+    assert res[1].iterations <= res[0].iterations
+    assert res[0].manifolds == res[1].manifolds > 50

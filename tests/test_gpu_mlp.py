@@ -148,7 +148,8 @@ def test_step_from_network_prediction(axm, orc, gpu):
     assert np.all(u > 0)
     ln = np.maximum((prev_u[have_u] + y[have_u, 0]) * u, 0)
     assert np.allclose(l0[have_u, 0], ln, rtol=2e-2, atol=2e-2 * u)
-    lim = d.friction * l0[have_u, 0] * (1 + 1e-6) + 1e-7
+    fb = w.friction_bounds()
+    lim = fb[valid][have_u] * (1 + 1e-6) + 1e-7
     assert np.all(np.abs(l0[have_u, 1]) <= lim) and np.all(np.abs(l0[have_u, 2]) <= lim)
     # the same step on the oracle from the GPU's warm start
     ob = b.copy()
@@ -157,9 +158,99 @@ def test_step_from_network_prediction(axm, orc, gpu):
     p.warm_start = 1
     ow = orc.OracleWorld(p)
     ow.set_bodies(ob)
-    ow.step_given(DT, 10, m, warm=lam0, colours=w.colours()[0])
+    ow.step_given(DT, 10, m, warm=lam0, colours=w.colours()[0], bounds=fb)
     g1 = w.get_bodies()
     pos1, quat1, v1, w1 = ow.get_state()
     assert rel_err(g1.lin_vel, v1) <= 1e-4
     assert rel_err(g1.ang_vel, w1) <= 1e-4
     assert st.manifolds == len(m["a"])
+
+
+def _pile_state(axm, steps=60):
+    b, d = axm.scene(0, seed(0), 1000)
+    w = axm.World(d)
+    w.set_bodies(b)
+    for _ in range(steps):
+        w.step(DT, 10)
+    g = w.get_bodies()
+    s = b.copy()
+    s.pos, s.quat, s.lin_vel, s.ang_vel = g.pos, g.quat, g.lin_vel, g.ang_vel
+    w.close()
+    return s, d
+
+
+def _run(axm, state, d, warm, net, steps, tol, with_first=False):
+    dd = axm.WorldDesc.from_buffer_copy(d)
+    dd.warm_start = warm
+    dd.residual_tol = tol
+    dd.max_iterations = 1000
+    w = axm.World(dd)
+    w.set_bodies(state)
+    if net is not None:
+        w.set_mlp(net)
+    its, first = [], []
+    for _ in range(steps):
+        st = w.step(DT, 1000, axm.STEP_RESIDUAL)
+        its.append(st.iterations)
+        first.append(st.residual_first)
+    g = w.get_bodies()
+    w.close()
+    if with_first:
+        return g, its, first
+    return g, its
+
+
+def test_nan_network_falls_back_to_the_cached_start(axm, gpu):
+    """A network whose outputs are NaN everywhere leaves every point at its cached impulse: the steps
+    are bit-identical to the plain previous-step warm start.  (Parameters must be finite; these
+    overflow: hidden layer 1 at 1e38, layer 2 at +inf, and output weights of alternating sign
+    sum +inf and -inf.)"""
+    state, d = _pile_state(axm)
+    net = axm.Mlp(0)
+    p = np.zeros(axm.MLP_PARAMS, np.float32)
+    p[3072:3200] = 1e38                    # b1
+    p[3200:19584] = 1e10                   # W2
+    w3 = np.ones((3, 128), np.float32)
+    w3[:, 1::2] = -1.0
+    p[19712:20096] = w3.ravel()            # W3
+    net.params = p
+    y = net.forward(np.zeros((4, axm.MLP_INPUTS), np.float32))
+    assert np.all(np.isnan(y))
+    g_nan, it_nan = _run(axm, state, d, axm.WARM_MLP, net, 20, 1e-4)
+    g_ref, it_ref = _run(axm, state, d, axm.WARM_CACHE, None, 20, 1e-4)
+    assert it_nan == it_ref
+    assert np.array_equal(g_nan.lin_vel, g_ref.lin_vel) and np.array_equal(g_nan.pos, g_ref.pos)
+
+
+def test_adversarial_network_ends_at_the_same_answer(axm, gpu):
+    """A network whose outputs are huge (capped at the largest prediction used) starts the sweeps far
+    from the solution; iterated to a tight residual, the step ends at the plain start's answer."""
+    state, d = _pile_state(axm)
+    net = axm.Mlp(0)
+    rng = np.random.default_rng(3)
+    net.params = (rng.normal(size=axm.MLP_PARAMS) * 50.0).astype(np.float32)
+    tol = 2e-5
+    dd = axm.WorldDesc.from_buffer_copy(d)
+    dd.warm_start = axm.WARM_MLP
+    dd.residual_tol = tol
+    dd.max_iterations = 1000
+    w = axm.World(dd)
+    w.set_bodies(state)
+    w.set_mlp(net)
+    w.set_feature_logging(True)
+    st_adv = w.step(DT, 1000, axm.STEP_RESIDUAL)
+    X, _ = w.features()
+    m = w.manifolds()
+    g_adv = w.get_bodies()
+    w.close()
+    valid = np.arange(4)[None, :] < m["npts"][:, None]
+    y = net.forward(np.ascontiguousarray(X[valid]))
+    print("adversarial outputs: finite %.3f, max |y| %.3g" % (np.isfinite(y).mean(), np.abs(y[np.isfinite(y)]).max()))
+    assert np.isfinite(y).mean() > 0.5 and np.abs(y[np.isfinite(y)]).max() > 10.0  # the premise
+    g_ref, it_ref, r_ref = _run(axm, state, d, axm.WARM_CACHE, None, 1, tol, with_first=True)
+    print("sweeps: adversarial %d plain %d; first residual %.3g vs %.3g"
+          % (st_adv.iterations, it_ref[0], st_adv.residual_first, r_ref[0]))
+    assert st_adv.residual_first > r_ref[0]  # it started further from the solution
+    assert st_adv.residual <= tol
+    assert rel_err(g_adv.lin_vel, g_ref.lin_vel) <= 1e-4
+    assert rel_err(g_adv.ang_vel, g_ref.ang_vel) <= 1e-4
