@@ -103,7 +103,7 @@ EXPORTS = [
     "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands", "axm_world_configure", "axm_world_get_desc",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses", "axm_world_get_friction_bounds",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
-    "axm_collide_pairs", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
+    "axm_collide_pairs", "axm_collide_pairs_noalloc", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
     "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
     "axm_mlp_forward", "axm_mlp_loss_grad", "axm_mlp_adam_step", "axm_mlp_train", "axm_world_set_mlp",
     "axm_world_set_feature_logging", "axm_world_get_features", "axm_world_set_sweep_config", "axm_world_autotune",
@@ -151,6 +151,8 @@ def lib():
     L.axm_broadphase.argtypes = [P, u32p]
     L.axm_collide.argtypes = [P, u32p]
     L.axm_collide_pairs.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, C.c_float,
+                                    u32p, f32p, f32p, f32p, u32p]
+    L.axm_collide_pairs_noalloc.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, C.c_float,
                                     u32p, f32p, f32p, f32p, u32p]
     L.axm_gjk_epa_pairs.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, f32p, i32p]
     L.axm_scene_generate.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(BodySoA), u32p,
@@ -538,15 +540,17 @@ class World:
         return v, w
 
 
-def collide_pairs(ta, pa, xa, qa, tb, pb, xb, qb, margin=0.02, device=0):
-    """Batched narrowphase on the GPU; returns dict of per-pair manifolds."""
+def collide_pairs(ta, pa, xa, qa, tb, pb, xb, qb, margin=0.02, device=0, noalloc=False):
+    """Batched narrowphase on the GPU; returns dict of per-pair manifolds.  noalloc: the entry point
+    that stages through the library's fixed device buffer (at most 16384 pairs, no allocation)."""
     n = len(ta)
     arrs = [np.ascontiguousarray(x, dt) for x, dt in
             ((ta, np.uint32), (pa, np.float32), (xa, np.float32), (qa, np.float32),
              (tb, np.uint32), (pb, np.float32), (xb, np.float32), (qb, np.float32))]
     out = dict(npts=np.zeros(n, np.uint32), pos=np.zeros((n, 4, 3), np.float32), normal=np.zeros((n, 4, 3), np.float32),
                depth=np.zeros((n, 4), np.float32), feature=np.zeros((n, 4), np.uint32))
-    code = lib().axm_collide_pairs(
+    fn = lib().axm_collide_pairs_noalloc if noalloc else lib().axm_collide_pairs
+    code = fn(
         device, n, _p(arrs[0], C.c_uint32), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]), _p(arrs[4], C.c_uint32),
         _p(arrs[5]), _p(arrs[6]), _p(arrs[7]), margin, _p(out["npts"], C.c_uint32), _p(out["pos"]),
         _p(out["normal"]), _p(out["depth"]), _p(out["feature"], C.c_uint32))

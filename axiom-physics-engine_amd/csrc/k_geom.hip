@@ -2,13 +2,27 @@
 // and GJK / EPA on shape cores, one thread per pair.  These expose the
 // collision routines the world step uses (dev_geom.hpp) on caller-chosen
 // configurations.
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "axiom/axm.h"
 #include "dev_geom.hpp"
 
 using namespace axm;
+
+// ---------------------------------------------------------------- allocation-free staging
+// Staging of axm_collide_pairs_noalloc: the inputs, outputs and per-pair status of at
+// most NOALLOC_MAX_PAIRS pairs, one region of 32-bit words per field (each region holds
+// that field for every pair slot: NOALLOC_MAX_PAIRS x width words).  One copy per
+// device, held until the process ends; every call on a device shares it.
+constexpr uint32_t NOALLOC_MAX_PAIRS = 16384;
+// region offsets, in words per pair slot: ta 1, pa 3, xa 3, qa 4, tb 1, pb 3, xb 3, qb 4 (inputs),
+// npts 1, pos 12, normal 12, depth 4, feature 4, status 1 (outputs)
+constexpr uint32_t NA_TA = 0, NA_PA = 1, NA_XA = 4, NA_QA = 7, NA_TB = 11, NA_PB = 12, NA_XB = 15, NA_QB = 18,
+                   NA_NP = 22, NA_POS = 23, NA_NRM = 35, NA_DEP = 47, NA_FID = 51, NA_ST = 55, NA_WORDS = 56;
+static __device__ __align__(256) uint32_t scratch[(size_t)NOALLOC_MAX_PAIRS * NA_WORDS];
 
 namespace {
 
@@ -124,6 +138,25 @@ bool upload_pairs(Arena& ar, uint32_t n, const uint32_t* ta, const float* pa, co
     return in.ta && in.pa && in.xa && in.qa && in.tb && in.pb && in.xb && in.qb;
 }
 
+// per device: the address of its copy of `scratch`, set on first use
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with that device
+// current (each device holds its own copy at its own address; the host-side name of the
+// array is not a device address and is never passed to the device).
+uint32_t* scratch_on(int32_t device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
 bool valid_types(uint32_t n, const uint32_t* ta, const uint32_t* tb, bool allow_plane_b) {
     for (uint32_t i = 0; i < n; ++i) {
         if (ta[i] > AXM_SHAPE_CAPSULE) return false;
@@ -170,6 +203,65 @@ axm_status axm_collide_pairs(int32_t device, uint32_t n, const uint32_t* type_a,
         return AXM_GPU_OPERATION_FAILED;
     for (uint32_t i = 0; i < n; ++i)
         if (st[i] != GEOM_OK) return st[i];
+    return AXM_SUCCESS;
+}
+
+axm_status axm_collide_pairs_noalloc(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a,
+                                     const float* pos_a, const float* quat_a, const uint32_t* type_b,
+                                     const float* prm_b, const float* pos_b, const float* quat_b, float margin,
+                                     uint32_t* npts, float* pos, float* normal, float* depth, uint32_t* feature) {
+    if (!n) return AXM_SUCCESS;
+    if (!type_a || !prm_a || !pos_a || !quat_a || !type_b || !prm_b || !pos_b || !quat_b || !npts || !pos || !normal ||
+        !depth || !feature)
+        return AXM_NULL_POINTER;
+    if (n > NOALLOC_MAX_PAIRS) return AXM_OUT_OF_RANGE;
+    if (!valid_types(n, type_a, type_b, true)) return AXM_INVALID_SHAPE;
+    if (hipSetDevice(device) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    uint32_t* base = scratch_on(device);
+    if (!base) return AXM_GPU_OPERATION_FAILED;
+    // region r starts at base + r * NOALLOC_MAX_PAIRS
+    auto reg = [&](uint32_t r) { return base + (size_t)r * NOALLOC_MAX_PAIRS; };
+    struct Up {
+        uint32_t r;
+        const void* h;
+        size_t bytes;
+    };
+    const Up ups[] = {{NA_TA, type_a, 4ull * n}, {NA_PA, prm_a, 12ull * n}, {NA_XA, pos_a, 12ull * n},
+                      {NA_QA, quat_a, 16ull * n}, {NA_TB, type_b, 4ull * n}, {NA_PB, prm_b, 12ull * n},
+                      {NA_XB, pos_b, 12ull * n}, {NA_QB, quat_b, 16ull * n}};
+    for (const Up& u : ups)
+        if (hipMemcpy(reg(u.r), u.h, u.bytes, hipMemcpyHostToDevice) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    PairIn in;
+    in.ta = reg(NA_TA);
+    in.pa = reinterpret_cast<const float*>(reg(NA_PA));
+    in.xa = reinterpret_cast<const float*>(reg(NA_XA));
+    in.qa = reinterpret_cast<const float*>(reg(NA_QA));
+    in.tb = reg(NA_TB);
+    in.pb = reinterpret_cast<const float*>(reg(NA_PB));
+    in.xb = reinterpret_cast<const float*>(reg(NA_XB));
+    in.qb = reinterpret_cast<const float*>(reg(NA_QB));
+    // the kernel writes every output word of the n pairs (zeros past a pair's point count),
+    // so nothing left by an earlier call is read back
+    hipLaunchKernelGGL(k_collide_pairs, dim3((n + 63) / 64), dim3(64), 0, 0, in, n, margin, reg(NA_NP),
+                       reinterpret_cast<float*>(reg(NA_POS)), reinterpret_cast<float*>(reg(NA_NRM)),
+                       reinterpret_cast<float*>(reg(NA_DEP)), reg(NA_FID), reinterpret_cast<int32_t*>(reg(NA_ST)));
+    if (hipDeviceSynchronize() != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    if (hipMemcpy(npts, reg(NA_NP), 4ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pos, reg(NA_POS), 48ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(normal, reg(NA_NRM), 48ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(depth, reg(NA_DEP), 16ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(feature, reg(NA_FID), 16ull * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return AXM_GPU_OPERATION_FAILED;
+    // the first failing pair's status, as axm_collide_pairs reports it; read 4096 at a time
+    // into a fixed host buffer
+    int32_t st[4096];
+    for (uint32_t i0 = 0; i0 < n; i0 += 4096) {
+        const uint32_t c = std::min<uint32_t>(4096u, n - i0);
+        if (hipMemcpy(st, reg(NA_ST) + i0, 4ull * c, hipMemcpyDeviceToHost) != hipSuccess)
+            return AXM_GPU_OPERATION_FAILED;
+        for (uint32_t i = 0; i < c; ++i)
+            if (st[i] != GEOM_OK) return st[i];
+    }
     return AXM_SUCCESS;
 }
 

@@ -298,6 +298,17 @@ axm_status axm_collide_pairs(int32_t device, uint32_t n, const uint32_t* type_a,
                              const float* quat_a, const uint32_t* type_b, const float* prm_b, const float* pos_b,
                              const float* quat_b, float margin, uint32_t* npts, float* pos, float* normal, float* depth,
                              uint32_t* feature);
+/* axm_collide_pairs without allocating: the same kernel and results, with the device
+ * copies of the inputs, the outputs and the per-pair status staged in a fixed buffer the
+ * library holds on each device until the process ends (3.5 MiB: at most 16384 pairs per
+ * call; more returns OutOfRange).  It calls no hipMalloc / hipFree, for callers that
+ * cannot allocate (tight stepping loops, harnesses that forbid device allocation).  Calls
+ * that overlap on one device share the buffer and race: serialise them.  No counterpart in
+ * the reference. */
+axm_status axm_collide_pairs_noalloc(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a,
+                                     const float* pos_a, const float* quat_a, const uint32_t* type_b,
+                                     const float* prm_b, const float* pos_b, const float* quat_b, float margin,
+                                     uint32_t* npts, float* pos, float* normal, float* depth, uint32_t* feature);
 /* Batched GJK (+ EPA when the cores overlap) on shape cores.  out: 10 floats per
  * pair (distance or depth, pA[3], pB[3], n[3]); status per pair: 0 separated,
  * 1 penetrating, 301 / 302 on failure (GJKFailedToConverge / EPAFailedToConverge,
