@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaxm.so")
 
 SUCCESS = 0
+INVALID_PARAMETER = 600
 OUT_OF_RANGE = 601
 ERROR_NAMES = {
     0: "Success", 200: "OutOfMemory", 202: "NullPointer", 300: "InvalidShape", 301: "GJKFailedToConverge",
@@ -79,12 +80,14 @@ class StepStats(C.Structure):
 
 class KernelProfile(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("pgs_launches", C.c_uint64), ("pgs_ms", C.c_double),
-                ("pgs_bytes", C.c_double), ("sweeps", C.c_uint64)]
+                ("pgs_bytes", C.c_double), ("sweeps", C.c_uint64), ("bundle_launches", C.c_uint64),
+                ("bundle_ms", C.c_double), ("bundle_point_sweeps", C.c_uint64)]
 
 
 class WorldTotals(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("steps_at_tol", C.c_uint64), ("sweeps", C.c_uint64),
-                ("contact_sweeps", C.c_uint64), ("max_sweeps", C.c_uint32), ("pad", C.c_uint32)]
+                ("contact_sweeps", C.c_uint64), ("max_sweeps", C.c_uint32), ("pad", C.c_uint32),
+                ("local_contact_sweeps", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
@@ -100,7 +103,7 @@ class SweepConfig(C.Structure):
 # Exported C-ABI symbols (the header's declarations).
 EXPORTS = [
     "axm_device_count", "axm_status_string", "axm_world_desc_default", "axm_world_create", "axm_world_destroy", "axm_world_set_bodies",
-    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands", "axm_world_configure", "axm_world_get_desc",
+    "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands", "axm_world_configure", "axm_world_get_desc", "axm_world_save", "axm_world_load", "axm_mlp_save", "axm_mlp_load",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses", "axm_world_get_friction_bounds",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
     "axm_collide_pairs", "axm_collide_pairs_noalloc", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
@@ -141,6 +144,10 @@ def lib():
     L.axm_world_get_islands.argtypes = [P, u32p, u32p, f32p]
     L.axm_world_configure.argtypes = [P, C.POINTER(WorldDesc)]
     L.axm_world_get_desc.argtypes = [P, C.POINTER(WorldDesc)]
+    L.axm_world_save.argtypes = [P, C.c_char_p]
+    L.axm_world_load.argtypes = [P, C.c_char_p]
+    L.axm_mlp_save.argtypes = [P, C.c_char_p]
+    L.axm_mlp_load.argtypes = [P, C.c_char_p]
     L.axm_world_get_pairs.argtypes = [P, u32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_manifolds.argtypes = [P, u32p, u32p, u32p, f32p, f32p, f32p, u32p, C.c_uint32, u32p]
     L.axm_world_get_colours.argtypes = [P, u32p, u32p, C.c_uint32, u32p, u32p]
@@ -345,6 +352,14 @@ class Mlp:
                "axm_mlp_train")
         return losses
 
+    def save(self, path):
+        """Writes the parameters and the Adam state to `path`."""
+        _check(self._L.axm_mlp_save(self._h, os.fsencode(str(path))), "axm_mlp_save")
+
+    def load(self, path):
+        """Restores parameters and Adam state from a file written by save()."""
+        _check(self._L.axm_mlp_load(self._h, os.fsencode(str(path))), "axm_mlp_load")
+
 
 def rng_stream(seed, n, lo=0.0, hi=1.0):
     u = np.zeros(n, np.uint32)
@@ -418,6 +433,24 @@ class World:
         """Replaces the solver settings of the live world (capacities, grid and device stay)."""
         _check(self._L.axm_world_configure(self._h, C.byref(desc)), "axm_world_configure")
         self.desc = desc
+
+    def get_desc(self):
+        """The world's description as the library holds it."""
+        d = WorldDesc()
+        _check(self._L.axm_world_get_desc(self._h, C.byref(d)), "axm_world_get_desc")
+        return d
+
+    def save(self, path):
+        """Writes a snapshot (description, bodies, last step's warm-start cache) to `path`."""
+        _check(self._L.axm_world_save(self._h, os.fsencode(str(path))), "axm_world_save")
+
+    def load(self, path):
+        """Replaces this world's description (not its device), bodies and cache with a snapshot's."""
+        _check(self._L.axm_world_load(self._h, os.fsencode(str(path))), "axm_world_load")
+        self.desc = self.get_desc()
+        cnt = C.c_uint32(0)
+        _check(self._L.axm_world_body_count(self._h, C.byref(cnt)), "axm_world_body_count")
+        self.n = int(cnt.value)
 
     def islands(self):
         """(label, asleep, rest time) of every body after the last step."""

@@ -944,8 +944,9 @@ __global__ void __launch_bounds__(256) k_col_sort_input(Bodies B, const uint2* _
         val[m] = m;
     }
 }
-// The small islands' work lists, from the sorted keys: (first slot, manifolds) of each,
-// one-lane islands from the front of `list`, one-workgroup islands from its back.
+// The work list of the islands solved by one workgroup each, from the sorted keys: (first
+// slot, manifolds) of every small island of more than ISLAND_LANE_MAX manifolds; the smaller
+// ones are only counted (the bundle kernel finds them by their slots).
 __global__ void __launch_bounds__(256) k_isl_list(const Bodies B, const uint32_t* __restrict__ skey,
                                                   uint2* __restrict__ list, uint32_t cap, Counters* cnt) {
     if (cnt->sticky != 0u) return;
@@ -954,8 +955,8 @@ __global__ void __launch_bounds__(256) k_isl_list(const Bodies B, const uint32_t
         const uint32_t k = skey[s];
         if (s != M0 && ((skey[s - 1] ^ k) >> 8) == 0u) continue;
         const uint32_t size = B.icnt[(k & ~LOCAL_KEY) >> 8];
-        if (size <= ISLAND_LANE_MAX) list[atomicAdd(&cnt->isl_lane_n, 1u)] = make_uint2(s, size);
-        else list[cap - 1u - atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
+        if (size <= ISLAND_LANE_MAX) atomicAdd(&cnt->isl_lane_n, 1u);
+        else list[atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
     }
 }
 
@@ -2168,42 +2169,198 @@ __device__ __forceinline__ float slot_residual(const Bodies& B, float keep, cons
 }
 // One island's result: sweeps run, final residual (bits), whether it missed the tolerance,
 // contact points; folded into the counters.
-__device__ __forceinline__ void local_done(Counters* cnt, uint32_t sweeps, float res, bool miss, uint32_t pts) {
+__device__ __forceinline__ void local_done(Counters* cnt, uint32_t sweeps, float res, bool miss, uint32_t pts,
+                                           bool bundle) {
     atomicMax(&cnt->loc_sweeps, sweeps);
     atomicMax(&cnt->loc_res_bits, __float_as_uint(res));
     if (miss) atomicOr(&cnt->loc_miss, 1u);
     atomicAdd(&cnt->loc_points, pts);
     atomicAdd(&cnt->loc_point_sweeps, (unsigned long long)pts * sweeps);
+    if (bundle) atomicAdd(&cnt->bundle_point_sweeps, (unsigned long long)pts * sweeps);
 }
-// Islands of at most ISLAND_LANE_MAX manifolds: one lane each, visits in slot order.
-__global__ void __launch_bounds__(256) k_pgs_lane(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
-                                                  const uint2* __restrict__ list, float tol, uint32_t max_sweeps) {
+// Islands of at most ISLAND_LANE_MAX manifolds, bundled: the islands whose first slot lies
+// in window w (BUNDLE_SLOTS slots from the first small-island slot on) are solved by
+// workgroup w, one wave, one lane per slot (at most BUNDLE_SLOTS + ISLAND_LANE_MAX - 1 lanes).
+// Each sweep visits the bundle's colours in turn, every lane of that colour at once (within
+// an island a colour's slots share no dynamic body, and islands share none at all), with a
+// barrier between colours; then each island's residual is the maximum over its lanes, and an
+// island that is done stops while the others go on.
+//
+// Everything a sweep touches lives in LDS for the whole solve: each lane's Jacobian rows and
+// impulses (row r of point k at BUNDLE_LDS_ROW(k, r) * 64 + lane, row JROWS the impulses),
+// and each body's solver velocity in one home slot, the first (lane, side) of its island that
+// names it (dynamic bodies appear in one island only; a static partner keeps its own slot).
+// The visits are the arithmetic of solve_slot on those copies; velocities and impulses go
+// back to memory once, at the end.
+constexpr uint32_t BUNDLE_SLOTS = 56;
+static_assert(BUNDLE_SLOTS + ISLAND_LANE_MAX - 1u <= 64u, "a bundle fits one wave");
+constexpr uint32_t BUNDLE_ROWS = JROWS + 1u;  // per point: the Jacobian rows and the impulses
+constexpr uint32_t BUNDLE_BODY0 = 4u * BUNDLE_ROWS * 64u;  // first body slot (float4 units)
+constexpr uint32_t BUNDLE_LDS_BYTES = (BUNDLE_BODY0 + 4u * 64u) * 16u;
+__device__ __forceinline__ void home_load(const float4* lds, uint32_t h, SB& X) {
+    const float4 v = lds[BUNDLE_BODY0 + h], w = lds[BUNDLE_BODY0 + 128u + h];
+    X.v = xyz(v);
+    X.im = v.w;
+    X.w = xyz(w);
+    X.dyn = v.w > 0.0f;
+}
+__device__ __forceinline__ void home_store(float4* lds, uint32_t h, const SB& X) {
+    lds[BUNDLE_BODY0 + h] = f4(X.v, X.im);
+    lds[BUNDLE_BODY0 + 128u + h] = f4(X.w, 0.0f);
+}
+__global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
+                                                   const uint32_t* __restrict__ skey, float tol,
+                                                   uint32_t max_sweeps) {
+    extern __shared__ float4 bl[];
+    __shared__ uint32_t isl_res[2][64];  // per island (first slot - window start): residual bits, two sweeps
+    __shared__ uint32_t isl_pts[64];
+    __shared__ uint2 ab[64];
+    __shared__ uint32_t kmax_w;
     if (cnt->sticky != 0u) return;
-    const uint32_t n = cnt->isl_lane_n;
+    const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob, lane = threadIdx.x;
+    const uint32_t nwin = M > M0 ? (M - M0 + BUNDLE_SLOTS - 1u) / BUNDLE_SLOTS : 0u;
     const bool tol_mode = tol > 0.0f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint2 e = list[i];
-        const uint32_t s0 = e.x, s1 = e.x + e.y;
-        uint32_t it = 0;
-        float res = 0.0f;
-        for (; it < max_sweeps; ++it) {
-            for (uint32_t s = s0; s < s1; ++s) solve_slot(B, sp, R, R.hdr[s], s, cap, it == 0u);
-            if (tol_mode || it + 1u == max_sweeps) {
-                res = 0.0f;
-                for (uint32_t s = s0; s < s1; ++s) {
-                    const float r = slot_residual(B, sp.keep, R, s, cap);
-                    if (r > res) res = r;
-                }
-                if (tol_mode && res <= tol) {
-                    ++it;
-                    break;
-                }
+    const float keep = sp.keep;
+    float4* rows = bl + lane;
+    for (uint32_t win = blockIdx.x; win < nwin; win += gridDim.x) {
+        const uint32_t w0 = M0 + win * BUNDLE_SLOTS, s = w0 + lane;
+        // this lane's slot belongs to the bundle when its island is a one-lane island whose
+        // first slot lies in the window
+        bool mine = false;
+        uint32_t li = 0, colour = 0;
+        if (s < M) {
+            const uint32_t k = skey[s];
+            if (B.icnt[(k & ~LOCAL_KEY) >> 8] <= ISLAND_LANE_MAX) {
+                uint32_t st = s;
+                while (st > M0 && ((skey[st - 1u] ^ k) >> 8) == 0u) --st;  // at most ISLAND_LANE_MAX - 1 steps
+                mine = st >= w0 && st < w0 + BUNDLE_SLOTS;
+                li = mine ? st - w0 : 0u;
+                colour = k & 0xFFu;
             }
         }
-        uint32_t pts = 0;
-        for (uint32_t s = s0; s < s1; ++s) pts += min(R.hdr[s].z, 4u);
-        local_done(cnt, it, res, tol_mode && !(res <= tol), pts);
+        const uint4 h = mine ? R.hdr[s] : make_uint4(UNSET, UNSET, 0u, 0u);
+        const uint32_t np = min(h.z, 4u);
+        const bool plane = (h.y & PLANE_BIT) != 0u;
+        ab[lane] = make_uint2(h.x, h.y);
+        isl_res[0][lane] = 0u;
+        isl_res[1][lane] = 0u;
+        isl_pts[lane] = 0u;
+        if (lane == 0u) kmax_w = 0u;
+        if (mine) {
+            // own copies of both bodies' velocities, the rows and the impulses
+            const float4 va = B.sv[h.x], wa = B.sw[h.x];
+            bl[BUNDLE_BODY0 + lane] = va;
+            bl[BUNDLE_BODY0 + 128u + lane] = make_float4(wa.x, wa.y, wa.z, 0.0f);
+            if (!plane) {
+                const float4 vb = B.sv[h.y], wb = B.sw[h.y];
+                bl[BUNDLE_BODY0 + 64u + lane] = vb;
+                bl[BUNDLE_BODY0 + 192u + lane] = make_float4(wb.x, wb.y, wb.z, 0.0f);
+            }
+            for (uint32_t k = 0; k < np; ++k) {
+                const float4* src = R.J + (size_t)k * JROWS * cap + s;
+#pragma unroll
+                for (uint32_t r = 0; r < JROWS; ++r) rows[(k * BUNDLE_ROWS + r) * 64u] = src[(size_t)r * cap];
+                rows[(k * BUNDLE_ROWS + JROWS) * 64u] = R.lam[(size_t)k * cap + s];
+            }
+        }
+        __syncthreads();
+        // home slot of each side: the first (lane, side) of the island naming the body
+        uint32_t home_a = lane, home_b = 64u + lane;
+        if (mine) {
+            atomicMax(&kmax_w, colour);
+            atomicAdd(&isl_pts[li], np);
+            auto home_of = [&](uint32_t body, uint32_t own) {
+                for (uint32_t j = li; j <= lane; ++j) {
+                    const uint2 o = ab[j];
+                    if (o.x == body) return j;
+                    if (o.y == body) return 64u + j;
+                }
+                return own;
+            };
+            home_a = home_of(h.x, lane);
+            if (!plane) home_b = home_of(h.y, 64u + lane);
+        }
+        __syncthreads();
+        const uint32_t kmax = kmax_w;
+        bool active = mine;
+        uint32_t sweeps = 0;
+        float res = 0.0f;
+        for (uint32_t it = 0; it < max_sweeps; ++it) {
+            if (!__syncthreads_or(active)) break;
+            for (uint32_t c = 0; c <= kmax; ++c) {
+                if (active && colour == c) {
+                    SB A, Bd;
+                    home_load(bl, home_a, A);
+                    if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
+                    float4 L[4];
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) L[k] = rows[(k * BUNDLE_ROWS + JROWS) * 64u];
+                    solve_rows(rows, 64u, (size_t)BUNDLE_ROWS * 64u, np, plane, keep, it == 0u, A, Bd, L);
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k)
+                        if (k < np) rows[(k * BUNDLE_ROWS + JROWS) * 64u] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
+                    if (A.dyn) home_store(bl, home_a, A);
+                    if (Bd.dyn) home_store(bl, home_b, Bd);
+                }
+                __syncthreads();
+            }
+            if (tol_mode || it + 1u == max_sweeps) {
+                uint32_t* cur = isl_res[it & 1u];
+                isl_res[(it + 1u) & 1u][lane] = 0u;  // the next sweep's words (not read in this one)
+                if (active) {
+                    SB A, Bd;
+                    home_load(bl, home_a, A);
+                    if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
+                    float r = 0.0f;
+                    for (uint32_t k = 0; k < np; ++k) {
+                        const float e = point_residual(rows + (size_t)k * BUNDLE_ROWS * 64u, 64u, plane, keep, A, Bd,
+                                                       rows[(k * BUNDLE_ROWS + JROWS) * 64u]);
+                        if (e > r) r = e;
+                    }
+                    atomicMax(&cur[li], __float_as_uint(r));
+                }
+                __syncthreads();
+                if (active) {
+                    res = __uint_as_float(cur[li]);
+                    if (tol_mode && res <= tol) {
+                        sweeps = it + 1u;
+                        active = false;
+                    }
+                }
+            }
+            if (active && it + 1u == max_sweeps) {
+                sweeps = max_sweeps;
+                active = false;
+            }
+        }
+        __syncthreads();
+        if (mine) {
+            // impulses and the velocities of the bodies whose home this lane holds back to memory
+            for (uint32_t k = 0; k < np; ++k) R.lam[(size_t)k * cap + s] = rows[(k * BUNDLE_ROWS + JROWS) * 64u];
+            SB A;
+            home_load(bl, lane, A);
+            if (home_a == lane && A.dyn) {
+                B.sv[h.x] = f4(A.v, A.im);
+                B.sw[h.x] = f4(A.w, 0.0f);
+            }
+            if (!plane && home_b == 64u + lane) {
+                SB Bd;
+                home_load(bl, 64u + lane, Bd);
+                if (Bd.dyn) {
+                    B.sv[h.y] = f4(Bd.v, Bd.im);
+                    B.sw[h.y] = f4(Bd.w, 0.0f);
+                }
+            }
+            // one lane per island (its first slot) reports it
+            if (s == w0 + li) local_done(cnt, sweeps, res, tol_mode && !(res <= tol), isl_pts[li], true);
+        }
+        __syncthreads();
     }
+}
+int pgs_bundle_setup() {
+    return hipFuncSetAttribute((const void*)k_pgs_bundle, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)BUNDLE_LDS_BYTES) == hipSuccess ? 1 : 0;
 }
 // Block-wide maximum and sum of 256 threads (red: 8 words of LDS); every thread gets both.
 __device__ __forceinline__ void block_max_sum(uint32_t& mx, uint32_t& sum, uint32_t* red) {
@@ -2232,7 +2389,7 @@ __global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, Solv
     const uint32_t n = cnt->isl_group_n, t = threadIdx.x;
     const bool tol_mode = tol > 0.0f;
     for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
-        const uint2 e = list[cap - 1u - g];
+        const uint2 e = list[g];
         const uint32_t s0 = e.x, s1 = e.x + e.y;
         const uint32_t k0 = skey[s0], k1 = skey[s1 - 1u];
         uint32_t it = 0;
@@ -2271,7 +2428,7 @@ __global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, Solv
         uint32_t pts = 0, mx = 0u;
         for (uint32_t s = s0 + t; s < s1; s += blockDim.x) pts += min(R.hdr[s].z, 4u);
         block_max_sum(mx, pts, red);
-        if (t == 0u) local_done(cnt, it, res, tol_mode && !(res <= tol), pts);
+        if (t == 0u) local_done(cnt, it, res, tol_mode && !(res <= tol), pts, false);
     }
 }
 
@@ -2353,12 +2510,14 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
         const uint32_t sw = max(swg, cnt->loc_sweeps);
         cnt->pgs_bytes_total += spread_sum(cnt, &SpreadRow::pgs_bytes) * (unsigned long long)sw;
         cnt->sweeps_total += sw;
+        cnt->bundle_point_sweeps_total += cnt->bundle_point_sweeps;
         cnt->acc_steps += 1u;
         cnt->acc_reached += cnt->tol_hit != 0u && cnt->loc_miss == 0u ? 1u : 0u;
         cnt->acc_max_sweeps = max(cnt->acc_max_sweeps, sw);
         cnt->acc_sweeps += sw;
         cnt->acc_contact_sweeps +=
             (unsigned long long)(spread_sum(cnt, &SpreadRow::contacts) - cnt->loc_points) * swg + cnt->loc_point_sweeps;
+        cnt->acc_local_contact_sweeps += cnt->loc_point_sweeps;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];
@@ -2570,11 +2729,15 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 }
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
     hipLaunchKernelGGL(k_isl_list, grid_bounded(cap), dim3(256), 0, s, B, skey, list, cap, cnt);
     if (max_sweeps == 0u) return;
     hipLaunchKernelGGL(k_pgs_group, dim3(groups), dim3(256), 0, s, B, sp, R, cnt, cap, list, skey, tol, max_sweeps);
-    hipLaunchKernelGGL(k_pgs_lane, grid_bounded(cap), dim3(256), 0, s, B, sp, R, cnt, cap, list, tol, max_sweeps);
+    const uint32_t windows = (cap + BUNDLE_SLOTS - 1u) / BUNDLE_SLOTS;
+    if (ev0) (void)hipEventRecord(ev0, s);
+    hipLaunchKernelGGL(k_pgs_bundle, dim3(std::max(1u, std::min(windows, 16384u))), dim3(64), BUNDLE_LDS_BYTES, s, B,
+                       sp, R, cnt, cap, skey, tol, max_sweeps);
+    if (ev1) (void)hipEventRecord(ev1, s);
 }
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,

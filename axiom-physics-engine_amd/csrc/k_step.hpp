@@ -94,7 +94,7 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 // keys, then one lane or one workgroup per island; tol > 0 stops each at its own residual.
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
@@ -130,6 +130,8 @@ void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R,
 // Raises the sweep kernel's dynamic-LDS limit to max_lds_bytes and returns how many of
 // its workgroups with lds_bytes each fit on one CU (0 on error).
 int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes);
+// Allows the bundle kernel of the small islands its LDS (once per process); 0 on failure.
+int pgs_bundle_setup();
 void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t s0, uint32_t s1, uint32_t cap,
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,

@@ -142,12 +142,14 @@ struct Counters {
     uint32_t isl_group_n;  // islands solved by one workgroup each
     uint32_t loc_points;   // contact points of the islands solved on their own
     uint32_t pad5;
-    unsigned long long loc_point_sweeps;  // their contact points x the sweeps each island ran
+    unsigned long long loc_point_sweeps;     // their contact points x the sweeps each island ran
+    unsigned long long bundle_point_sweeps;  // the same for the islands of the bundle kernel
     // ---- kept across steps
     uint32_t sticky;       // STICKY_* bits; while set the integration is skipped (state untouched)
     uint32_t steps_done;   // steps whose integration ran
     unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
     unsigned long long sweeps_total;     // sweeps of all completed steps
+    unsigned long long bundle_point_sweeps_total;  // bundle_point_sweeps of all completed steps
     // ---- run totals since the last axm_world_get_totals reset (steps whose integration ran)
     uint32_t acc_steps;
     uint32_t acc_reached;  // steps that stopped at the residual tolerance
@@ -155,6 +157,7 @@ struct Counters {
     uint32_t acc_pad;
     unsigned long long acc_sweeps;
     unsigned long long acc_contact_sweeps;  // sum of contact points x sweeps
+    unsigned long long acc_local_contact_sweeps;  // the part in islands solved on their own
     // ---- spread sums, cleared per step
     alignas(128) SpreadRow sp[SPREAD];
 };
@@ -163,7 +166,7 @@ template <class T> inline T spread_total(const Counters& c, T SpreadRow::*f) {
     for (uint32_t k = 0; k < SPREAD; ++k) s += c.sp[k].*f;
     return s;
 }
-constexpr uint32_t COUNTERS_RESET_WORDS = 24;  // words of the first block (cleared per step)
+constexpr uint32_t COUNTERS_RESET_WORDS = 26;  // words of the first block (cleared per step)
 static_assert(offsetof(Counters, sticky) == 4 * COUNTERS_RESET_WORDS, "per-step block ends at sticky");
 enum : uint32_t {
     STICKY_PAIRS = 1u,        // pair capacity exceeded

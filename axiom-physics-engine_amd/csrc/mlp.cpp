@@ -8,6 +8,7 @@
 #include <new>
 #include <vector>
 
+#include "file_io.hpp"
 #include "mlp_impl.hpp"
 
 using namespace axm;
@@ -251,6 +252,55 @@ axm_status axm_mlp_train(axm_mlp m, const float* X, const float* Y, uint32_t n, 
         }
     }
     dfree(losses);
+    return st;
+}
+
+
+// Network file (version 1): magic "AXMMLP01", version, parameter count, Adam step count,
+// then the fp32 parameters, Adam's first and second moments (packing of axm_mlp_set_params)
+// and the FNV-1a hash.  Loading restores the network and its optimiser state exactly, so
+// training resumed from a file continues bit for bit.
+axm_status axm_mlp_save(axm_mlp m, const char* path) {
+    if (!m || !path) return AXM_NULL_POINTER;
+    (void)hipSetDevice(m->device);
+    std::vector<float> p(MLP_NPARAM), mo(MLP_NPARAM), vo(MLP_NPARAM);
+    MLP_HIP(hipMemcpyAsync(p.data(), m->prm, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipMemcpyAsync(mo.data(), m->m, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipMemcpyAsync(vo.data(), m->v, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipStreamSynchronize(m->stream));
+    axm::BinWriter f(path);
+    if (!f.ok()) return AXM_INVALID_PARAMETER;
+    const uint32_t head[3] = {1u, (uint32_t)MLP_NPARAM, m->adam_t};
+    f.put("AXMMLP01", 8);
+    f.put(head, sizeof(head));
+    f.vec(p);
+    f.vec(mo);
+    f.vec(vo);
+    return f.finish() ? AXM_SUCCESS : AXM_INVALID_PARAMETER;
+}
+
+axm_status axm_mlp_load(axm_mlp m, const char* path) {
+    if (!m || !path) return AXM_NULL_POINTER;
+    axm::BinReader f(path);
+    if (!f.ok()) return AXM_INVALID_PARAMETER;
+    char magic[8];
+    uint32_t head[3];
+    f.get(magic, 8);
+    f.get(head, sizeof(head));
+    if (!f.ok() || std::memcmp(magic, "AXMMLP01", 8) != 0 || head[0] != 1u || head[1] != (uint32_t)MLP_NPARAM)
+        return AXM_INVALID_PARAMETER;
+    std::vector<float> p, mo, vo;
+    f.vec(p, MLP_NPARAM);
+    f.vec(mo, MLP_NPARAM);
+    f.vec(vo, MLP_NPARAM);
+    if (!f.finish()) return AXM_INVALID_PARAMETER;
+    for (int i = 0; i < MLP_NPARAM; ++i)
+        if (!std::isfinite(p[i]) || !std::isfinite(mo[i]) || !std::isfinite(vo[i])) return AXM_INVALID_PARAMETER;
+    (void)hipSetDevice(m->device);
+    MLP_HIP(hipMemcpyAsync(m->m, mo.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, m->stream));
+    MLP_HIP(hipMemcpyAsync(m->v, vo.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, m->stream));
+    const axm_status st = upload_params(m, p.data());
+    if (st == AXM_SUCCESS) m->adam_t = head[2];
     return st;
 }
 

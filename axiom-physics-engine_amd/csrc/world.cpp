@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "axiom/axm.h"
+#include "file_io.hpp"
 #include "k_step.hpp"
 #include "mlp_impl.hpp"
 
@@ -168,6 +169,8 @@ struct axm_world_s {
     float graph_dt = 0.0f;
     uint32_t graph_iters = 0;
     hipEvent_t gprof_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // sweep-kernel events per parity
+    hipEvent_t bprof_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // bundle-kernel events (eager / per parity)
+    const uint32_t* solve_keys = nullptr;  // the sorted keys of the last step's solve order
 
     // last step
     bool have_step = false;
@@ -189,6 +192,8 @@ struct axm_world_s {
         drop_graphs();
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (hipEvent_t e : gprof_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : bprof_ev)
             if (e) (void)hipEventDestroy(e);
         mem.release_all();
         if (h_cnt) (void)hipHostFree(h_cnt);
@@ -494,6 +499,13 @@ axm_status clear_sticky(axm_world_s* w) {
     return AXM_SUCCESS;
 }
 
+// The bundle kernel's profiling events of slot pair p (created on first use).
+axm_status bundle_events(axm_world_s* w, int p) {
+    for (int k = 2 * p; k < 2 * p + 2; ++k)
+        if (!w->bprof_ev[k]) AXM_HIP(hipEventCreate(&w->bprof_ev[k]));
+    return AXM_SUCCESS;
+}
+
 // The next profiling event (created on first use), or null when not profiling.
 axm_status prof_event(axm_world_s* w, size_t& nev, bool capturing, hipEvent_t& e) {
     e = nullptr;
@@ -607,6 +619,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const uint32_t* scol = split ? w->ckey : w->ckey_alt;
     const uint32_t* order = split ? w->cval : w->cval_alt;
     w->solve_order = order;
+    w->solve_keys = scol;
     launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
@@ -616,12 +629,6 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
         launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
                            2u * (uint32_t)w->cu_count, s);
         launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
-    }
-    if (split) {
-        const uint32_t max_it =
-            w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u) : o.iters;
-        launch_pgs_local(w->B, sp, R, w->cnt, rc, scol, w->isl_list, w->desc.residual_tol, max_it,
-                         2u * (uint32_t)w->cu_count, s);
     }
     launch_fill_u32(w->res_bits, 0u, res_words(w->res_cap), s);  // residuals, then check arrivals
     return AXM_SUCCESS;
@@ -661,6 +668,17 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_iters_dev = dev_checks;
     std::vector<uint32_t> off;
     if (pre) AXM_TRY(enqueue_pre(w, sp, o));
+    if (sweep && island_split(w)) {
+        // the islands solved on their own, before the large ones' sweeps
+        hipEvent_t b0 = nullptr, b1 = nullptr;
+        if (w->prof_on && !o.capture) {
+            AXM_TRY(bundle_events(w, 0));
+            b0 = w->bprof_ev[0];
+            b1 = w->bprof_ev[1];
+        }
+        launch_pgs_local(w->B, sp, R, w->cnt, rc, w->solve_keys, w->isl_list, w->desc.residual_tol, max_it,
+                         2u * (uint32_t)w->cu_count, s, b0, b1);
+    }
     const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
     const uint32_t every = std::max<uint32_t>(w->desc.residual_every, 1u);
     uint32_t it = sweep ? 0u : max_it;
@@ -749,6 +767,13 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
         w->prof.pgs_ms += ms;
         w->prof.pgs_bytes += (double)spread_total(*w->h_cnt, &SpreadRow::pgs_bytes) * (double)ran;
         w->prof.sweeps += ran;
+        if (sweep && island_split(w)) {
+            float e = 0.0f;
+            AXM_HIP(hipEventElapsedTime(&e, w->bprof_ev[0], w->bprof_ev[1]));
+            w->prof.bundle_launches += 1;
+            w->prof.bundle_ms += e;
+            w->prof.bundle_point_sweeps += w->h_cnt->bundle_point_sweeps;
+        }
     }
     w->cur = pv;
     w->have_step = true;
@@ -953,7 +978,7 @@ axm_status axm_world_create(const axm_world_desc* desc, axm_world* out) {
     w->max_lds = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock, 160u * 1024u);
     w->sweep_lds = w->max_lds & ~16383u;
     const int own_per_cu = pgs_own_setup(w->max_lds, w->sweep_lds);
-    if (own_per_cu < 1) {
+    if (own_per_cu < 1 || !pgs_bundle_setup()) {
         delete w;
         return AXM_GPU_OPERATION_FAILED;
     }
@@ -998,6 +1023,18 @@ axm_status axm_world_body_count(axm_world w, uint32_t* n) {
     *n = w->n;
     return AXM_SUCCESS;
 }
+
+// Per-body arrays in device layout (x, q, vel, angv, ii, shape, mat float4; bfl uint4;
+// world u32; world_first per world + 1; sleep words and wake stamps, empty for a fresh set).
+struct HostBodies {
+    std::vector<float4> x, q, v, av, ii, sh, mat;
+    std::vector<uint4> bfl;
+    std::vector<uint32_t> wd, first;
+    std::vector<uint2> slp;
+    std::vector<uint32_t> wake;
+};
+axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, bool use_filter,
+                          const HostBodies& hb);
 
 axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) {
     if (!w || !b) return AXM_NULL_POINTER;
@@ -1078,6 +1115,19 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
     float cell = w->desc.cell_size;
     if (cell == 0.0f) cell = need > 0.0f ? need : 1.0f;
     if (cell < need) return AXM_INVALID_PARAMETER;
+    HostBodies hb{std::move(x), std::move(q), std::move(v), std::move(av), std::move(ii), std::move(sh),
+                  std::move(mat), std::move(bfl), std::move(wd), std::move(first), {}, {}};
+    return install_bodies(w, n, std::max(nw, 1u), cell, b->filter != nullptr, hb);
+}
+
+
+// Uploads a body set and resets the per-world step state (counters, warm-start cache,
+// occupancy bitmap).
+axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, bool use_filter,
+                          const HostBodies& hb) {
+    const std::vector<float4>&x = hb.x, &q = hb.q, &v = hb.v, &av = hb.av, &ii = hb.ii, &sh = hb.sh, &mat = hb.mat;
+    const std::vector<uint4>& bfl = hb.bfl;
+    const std::vector<uint32_t>&wd = hb.wd, &first = hb.first;
     AXM_HIP(hipStreamSynchronize(w->stream));
     w->drop_graphs();
     if (n > w->cap_n) AXM_TRY(alloc_bodies(w, n));
@@ -1095,11 +1145,16 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
         AXM_HIP(hipMemcpyAsync(w->B.world, wd.data(), 4ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.mat, mat.data(), 16ull * n, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(w->B.bfl, bfl.data(), 16ull * n, hipMemcpyHostToDevice, s));
-        AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * (nw + 1), hipMemcpyHostToDevice, s));
-        // every body starts awake with no rest time and no island; no island has a wake request
-        const std::vector<uint2> slp(n, make_uint2(0u, LABEL_NONE));
-        AXM_HIP(hipMemcpyAsync(w->B.slp, slp.data(), 8ull * n, hipMemcpyHostToDevice, s));
-        AXM_HIP(hipMemsetAsync(w->B.wake, 0xFF, 4ull * n, s));
+        AXM_HIP(hipMemcpyAsync(w->B.world_first, first.data(), 4ull * first.size(), hipMemcpyHostToDevice, s));
+        if (hb.slp.size() == n && hb.wake.size() == n) {
+            AXM_HIP(hipMemcpyAsync(w->B.slp, hb.slp.data(), 8ull * n, hipMemcpyHostToDevice, s));
+            AXM_HIP(hipMemcpyAsync(w->B.wake, hb.wake.data(), 4ull * n, hipMemcpyHostToDevice, s));
+        } else {
+            // every body starts awake with no rest time and no island; no island has a wake request
+            const std::vector<uint2> slp(n, make_uint2(0u, LABEL_NONE));
+            AXM_HIP(hipMemcpyAsync(w->B.slp, slp.data(), 8ull * n, hipMemcpyHostToDevice, s));
+            AXM_HIP(hipMemsetAsync(w->B.wake, 0xFF, 4ull * n, s));
+        }
     }
     AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
     // one large world: occupancy bits at the cells' Morton codes (cleared here in full,
@@ -1114,7 +1169,7 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
     }
     AXM_HIP(hipStreamSynchronize(s));
     w->n = n;
-    w->use_filter = b->filter != nullptr;
+    w->use_filter = use_filter;
     w->n_worlds = std::max(nw, 1u);
     w->cell = cell;
     w->prev_valid = false;
@@ -1206,20 +1261,25 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
                 AXM_TRY(capture(w, dt, iters, p, PH_ALL, &w->gexec[p]));
             }
         }
-        if (w->prof_on)
+        const bool split = island_split(w) != 0;
+        if (w->prof_on) {
             for (auto& e : w->gprof_ev)
                 if (!e) AXM_HIP(hipEventCreate(&e));
+            AXM_TRY(bundle_events(w, 0));
+            AXM_TRY(bundle_events(w, 1));
+        }
         const StepParams sp = make_params(w, dt);
         AXM_TRY(read_counters(w));
         const uint32_t base = w->h_cnt->steps_done;
         const unsigned long long base_bytes = w->h_cnt->pgs_bytes_total;
         const unsigned long long base_sweeps = w->h_cnt->sweeps_total;
+        const unsigned long long base_bundle = w->h_cnt->bundle_point_sweeps_total;
         const int cur0 = w->cur;
         const uint32_t todo = steps - done;
         // With profiling, the sweep kernel of step k-1 is timed while step k
         // already runs: the host waits on the previous step only, so the GPU
         // queue never drains.
-        double prof_ms = 0.0;
+        double prof_ms = 0.0, bundle_ms = 0.0;
         int prev = -1;
         for (uint32_t k = 0; k < todo; ++k) {
             const int c = w->cur;
@@ -1229,6 +1289,10 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
                 const bool checks = sweep_checks(w, false, stop);
                 const uint32_t nsw = w->desc.residual_tol > 0.0f ? std::max<uint32_t>(w->desc.max_iterations, 1u)
                                                                  : iters;
+                if (split)
+                    launch_pgs_local(w->B, sp, w->R[c], w->cnt, w->rcap[c], w->solve_keys, w->isl_list,
+                                     w->desc.residual_tol, nsw, 2u * (uint32_t)w->cu_count, w->stream,
+                                     w->bprof_ev[2 * c], w->bprof_ev[2 * c + 1]);
                 launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, 0, nsw, w->cnt, SWEEP_SPIN_LIMIT_US,
                                 w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream, w->gprof_ev[2 * c],
                                 w->gprof_ev[2 * c + 1], checks ? stop : SweepStop{});
@@ -1241,6 +1305,10 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
                 AXM_HIP(hipEventSynchronize(w->gprof_ev[2 * prev + 1]));
                 AXM_HIP(hipEventElapsedTime(&e, w->gprof_ev[2 * prev], w->gprof_ev[2 * prev + 1]));
                 prof_ms += e;
+                if (split) {
+                    AXM_HIP(hipEventElapsedTime(&e, w->bprof_ev[2 * prev], w->bprof_ev[2 * prev + 1]));
+                    bundle_ms += e;
+                }
             }
             prev = w->cur;
             w->cur = 1 - w->cur;
@@ -1250,6 +1318,10 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             float e = 0.0f;
             AXM_HIP(hipEventElapsedTime(&e, w->gprof_ev[2 * prev], w->gprof_ev[2 * prev + 1]));
             prof_ms += e;
+            if (split) {
+                AXM_HIP(hipEventElapsedTime(&e, w->bprof_ev[2 * prev], w->bprof_ev[2 * prev + 1]));
+                bundle_ms += e;
+            }
         }
         const uint32_t ok = w->h_cnt->steps_done - base;
         if (w->prof_on) {
@@ -1258,6 +1330,11 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
             w->prof.pgs_ms += prof_ms;
             w->prof.pgs_bytes += (double)(w->h_cnt->pgs_bytes_total - base_bytes);
             w->prof.sweeps += w->h_cnt->sweeps_total - base_sweeps;
+            if (split) {
+                w->prof.bundle_launches += todo;
+                w->prof.bundle_ms += bundle_ms;
+                w->prof.bundle_point_sweeps += w->h_cnt->bundle_point_sweeps_total - base_bundle;
+            }
         }
         done += ok;
         w->have_step = true;
@@ -1333,10 +1410,11 @@ axm_status axm_world_get_totals(axm_world w, axm_world_totals* out, int reset) {
     out->contact_sweeps = h.acc_contact_sweeps;
     out->max_sweeps = h.acc_max_sweeps;
     out->pad = 0;
+    out->local_contact_sweeps = h.acc_local_contact_sweeps;
     if (reset) {
         // the accumulators sit between steps_done's block and the spread rows
         const size_t off = offsetof(Counters, acc_steps);
-        const size_t len = offsetof(Counters, acc_contact_sweeps) + sizeof(unsigned long long) - off;
+        const size_t len = offsetof(Counters, acc_local_contact_sweeps) + sizeof(unsigned long long) - off;
         launch_fill_u32(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w->cnt) + off), 0u,
                         (uint32_t)(len / 4), w->stream);
         AXM_HIP(hipStreamSynchronize(w->stream));
@@ -1675,6 +1753,157 @@ axm_status axm_world_get_residuals(axm_world w, float* out, uint32_t cap, uint32
     if (cap < *count) return AXM_OUT_OF_RANGE;
     if (out)
         for (size_t k = 0; k < w->h_res.size(); ++k) out[k] = w->h_res[k];
+    return AXM_SUCCESS;
+}
+
+
+// ---------------------------------------------------------------- world snapshots
+// File layout (version 1): magic "AXMWORLD", SnapHeader, the world description, the
+// per-body arrays in device layout (x, q, vel, angv, ii, shape, world, mat, bfl, sleep
+// words, wake stamps, world_first), then, after a step, the warm-start cache of that
+// step (impulses and feature ids of point k = 0..3 over its m_prev slots, slot headers,
+// colours) and the FNV-1a hash.  Loading it and stepping on gives bit-identical results
+// to stepping on from where it was saved: every input of the next step is restored (the
+// per-step buffers are rebuilt by the step itself).
+}  // extern "C"
+namespace {
+constexpr char SNAP_MAGIC[8] = {'A', 'X', 'M', 'W', 'O', 'R', 'L', 'D'};
+constexpr uint32_t SNAP_VERSION = 1;
+struct SnapHeader {
+    uint32_t version, desc_bytes;
+    uint32_t n, n_worlds, use_filter, prev_valid;
+    float cell;
+    uint32_t steps_done, m_prev, reserved;
+};
+template <class T> axm_status pull_vec(std::vector<T>& v, const T* src, size_t n, hipStream_t s) {
+    v.resize(n);
+    if (n) AXM_HIP(hipMemcpyAsync(v.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    return AXM_SUCCESS;
+}
+}  // namespace
+extern "C" {
+
+axm_status axm_world_save(axm_world w, const char* path) {
+    if (!w || !path) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    AXM_TRY(read_counters(w));
+    const uint32_t n = w->n, nw = w->n_worlds;
+    const int pv = 1 - w->cur;
+    const uint32_t m_prev = w->prev_valid ? std::min(w->h_cnt->m_eff, w->rcap[pv]) : 0u;
+    hipStream_t s = w->stream;
+    HostBodies hb;
+    AXM_TRY(pull_vec(hb.x, w->B.x, n, s));
+    AXM_TRY(pull_vec(hb.q, w->B.q, n, s));
+    AXM_TRY(pull_vec(hb.v, w->B.vel, n, s));
+    AXM_TRY(pull_vec(hb.av, w->B.angv, n, s));
+    AXM_TRY(pull_vec(hb.ii, w->B.ii, n, s));
+    AXM_TRY(pull_vec(hb.sh, w->B.shape, n, s));
+    AXM_TRY(pull_vec(hb.wd, w->B.world, n, s));
+    AXM_TRY(pull_vec(hb.mat, w->B.mat, n, s));
+    AXM_TRY(pull_vec(hb.bfl, w->B.bfl, n, s));
+    AXM_TRY(pull_vec(hb.slp, w->B.slp, n, s));
+    AXM_TRY(pull_vec(hb.wake, w->B.wake, n, s));
+    AXM_TRY(pull_vec(hb.first, w->B.world_first, n ? (size_t)nw + 1 : 0, s));
+    const SolverRows& R = w->R[pv];
+    const size_t rc = w->rcap[pv];
+    std::vector<float4> lam[4];
+    std::vector<uint32_t> fid[4];
+    std::vector<uint4> hdr;
+    std::vector<uint32_t> col;
+    if (m_prev) {
+        for (int k = 0; k < 4; ++k) {
+            AXM_TRY(pull_vec(lam[k], R.lam + (size_t)k * rc, m_prev, s));
+            AXM_TRY(pull_vec(fid[k], R.fid + (size_t)k * rc, m_prev, s));
+        }
+        AXM_TRY(pull_vec(hdr, R.hdr, m_prev, s));
+        AXM_TRY(pull_vec(col, R.colour, m_prev, s));
+    }
+    AXM_HIP(hipStreamSynchronize(s));
+    BinWriter f(path);
+    if (!f.ok()) return AXM_INVALID_PARAMETER;
+    SnapHeader h{SNAP_VERSION, (uint32_t)sizeof(axm_world_desc), n, nw, w->use_filter ? 1u : 0u,
+                 m_prev ? 1u : 0u, w->cell, w->h_cnt->steps_done, m_prev, 0u};
+    f.put(SNAP_MAGIC, 8);
+    f.pod(h);
+    f.pod(w->desc);
+    for (const auto* v : {&hb.x, &hb.q, &hb.v, &hb.av, &hb.ii, &hb.sh}) f.vec(*v);
+    f.vec(hb.wd);
+    f.vec(hb.mat);
+    f.vec(hb.bfl);
+    f.vec(hb.slp);
+    f.vec(hb.wake);
+    f.vec(hb.first);
+    for (int k = 0; k < 4 && m_prev; ++k) {
+        f.vec(lam[k]);
+        f.vec(fid[k]);
+    }
+    f.vec(hdr);
+    f.vec(col);
+    return f.finish() ? AXM_SUCCESS : AXM_INVALID_PARAMETER;
+}
+
+axm_status axm_world_load(axm_world w, const char* path) {
+    if (!w || !path) return AXM_NULL_POINTER;
+    (void)hipSetDevice(w->desc.device);
+    BinReader f(path);
+    if (!f.ok()) return AXM_INVALID_PARAMETER;
+    char magic[8];
+    f.get(magic, 8);
+    SnapHeader h{};
+    f.pod(h);
+    if (!f.ok() || std::memcmp(magic, SNAP_MAGIC, 8) != 0 || h.version != SNAP_VERSION ||
+        h.desc_bytes != sizeof(axm_world_desc) || h.n >= (1u << 29) || h.n_worlds > (1u << 22) ||
+        (h.n && h.n_worlds == 0) || h.m_prev > 4u * (h.n + 16u))
+        return AXM_INVALID_PARAMETER;
+    axm_world_desc d{};
+    f.pod(d);
+    const uint32_t n = h.n;
+    HostBodies hb;
+    for (auto* v : {&hb.x, &hb.q, &hb.v, &hb.av, &hb.ii, &hb.sh}) f.vec(*v, n);
+    f.vec(hb.wd, n);
+    f.vec(hb.mat, n);
+    f.vec(hb.bfl, n);
+    f.vec(hb.slp, n);
+    f.vec(hb.wake, n);
+    f.vec(hb.first, n ? (size_t)h.n_worlds + 1 : 0);
+    const uint32_t mp = h.m_prev;
+    std::vector<float4> lam[4];
+    std::vector<uint32_t> fid[4];
+    std::vector<uint4> hdr;
+    std::vector<uint32_t> col;
+    for (int k = 0; k < 4 && mp; ++k) {
+        f.vec(lam[k], mp);
+        f.vec(fid[k], mp);
+    }
+    f.vec(hdr, mp);
+    f.vec(col, mp);
+    if (!f.finish()) return AXM_INVALID_PARAMETER;
+    // the world keeps its device; everything else comes from the file
+    d.device = w->desc.device;
+    AXM_HIP(hipStreamSynchronize(w->stream));
+    w->desc = d;
+    AXM_TRY(install_bodies(w, n, std::max(h.n_worlds, 1u), h.cell, h.use_filter != 0, hb));
+    hipStream_t s = w->stream;
+    AXM_HIP(hipMemcpy(&w->cnt->steps_done, &h.steps_done, 4, hipMemcpyHostToDevice));
+    if (mp) {
+        // the previous step's cache in the set the next step reads, and its lookup table
+        if (w->cap_man < mp) AXM_TRY(alloc_manifolds(w, mp));
+        const int pv = 1 - w->cur;
+        if (w->rcap[pv] < w->cap_man) AXM_TRY(alloc_rows(w, pv, w->cap_man));
+        const SolverRows& R = w->R[pv];
+        const size_t rc = w->rcap[pv];
+        for (int k = 0; k < 4; ++k) {
+            AXM_HIP(hipMemcpyAsync(R.lam + (size_t)k * rc, lam[k].data(), 16ull * mp, hipMemcpyHostToDevice, s));
+            AXM_HIP(hipMemcpyAsync(R.fid + (size_t)k * rc, fid[k].data(), 4ull * mp, hipMemcpyHostToDevice, s));
+        }
+        AXM_HIP(hipMemcpyAsync(R.hdr, hdr.data(), 16ull * mp, hipMemcpyHostToDevice, s));
+        AXM_HIP(hipMemcpyAsync(R.colour, col.data(), 4ull * mp, hipMemcpyHostToDevice, s));
+        AXM_HIP(hipMemcpyAsync(&w->cnt->m_eff, &mp, 4, hipMemcpyHostToDevice, s));
+        launch_cache_build(R, w->cnt, w->rcap[pv], w->pc_key[pv], w->pc_slot[pv], w->pc_mask[pv], w->np_slot[pv], s);
+        AXM_HIP(hipStreamSynchronize(s));
+        w->prev_valid = true;
+    }
     return AXM_SUCCESS;
 }
 

@@ -204,18 +204,28 @@ def headline(args):
     value = bodies.n * args.steps / elapsed
     steps_done = max(int(tot.steps), 1)
     mean_sweeps = tot.sweeps / steps_done
-    # dominant kernel: the sweep kernel, 240 B per contact point per sweep
-    sweep_bytes = B_SWEEP * float(tot.contact_sweeps)
-    sweep_s = prof.pgs_ms * 1e-3
-    achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
-    launches = max(int(prof.pgs_launches), 1)
-    roof = {
-        "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_pgs_own",
-        "launches": int(prof.pgs_launches), "avg_launch_us": round(prof.pgs_ms * 1e3 / launches, 3),
-        "bytes_per_launch": round(sweep_bytes / launches, 1),
-        "per_unit": "240 B per contact point per sweep", "share_of_step": round(sweep_s / elapsed, 4),
-    }
+    # the sweep kernels, 240 B per contact point per sweep each: the bundle kernel of the small
+    # islands solved on their own (their points x the sweeps each island ran) and the persistent
+    # sweep of the large islands (the rest); the roofline line is the one with more time
+    kern = {"k_pgs_own": (B_SWEEP * float(tot.contact_sweeps - tot.local_contact_sweeps), prof.pgs_ms * 1e-3,
+                          int(prof.pgs_launches))}
+    if prof.bundle_launches:
+        kern["k_pgs_bundle"] = (B_SWEEP * float(prof.bundle_point_sweeps), prof.bundle_ms * 1e-3,
+                                int(prof.bundle_launches))
+
+    def roof_of(name):
+        nbytes, secs, n = kern[name]
+        n = max(n, 1)
+        ach = nbytes / secs / 1e9 if secs > 0 else 0.0
+        return {
+            "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
+            "launches": int(kern[name][2]), "avg_launch_us": round(secs * 1e6 / n, 3),
+            "bytes_per_launch": round(nbytes / n, 1),
+            "per_unit": "240 B per contact point per sweep", "share_of_step": round(secs / elapsed, 4),
+        }
+    dominant = max(kern, key=lambda k: kern[k][1])
+    roof = roof_of(dominant)
     # whole step: 304 N + 104 P + 304 C + 12 R C + 240 I C per step (P, C, R of the diagnostic step)
     dyn = int((bodies.inv_mass > 0).sum())
     b_step = (B_BODY * dyn + B_PAIR * diag.pairs + B_CONTACT * diag.contacts + B_ROUND * diag.colour_rounds *
@@ -253,6 +263,7 @@ def headline(args):
                                                     "iterations", "residual", "broadphase_ms", "narrowphase_ms",
                                                     "solver_ms", "integration_ms", "total_ms")},
         "roofline": roof,
+        "roofline_sweep_kernels": {k: roof_of(k) for k in kern},
         "roofline_step": {"bytes_per_step": round(b_step, 1), "achieved": round(step_gbs, 2), "unit": "GB/s",
                           "frac": round(step_gbs / HBM_PEAK_GBS, 5),
                           "formula": "304 N + 104 P + 304 C + 12 R C + 240 I C"},

@@ -176,6 +176,10 @@ typedef struct axm_kernel_profile {
     double pgs_ms;        /* summed kernel durations */
     double pgs_bytes;     /* summed algorithmic bytes of those launches */
     uint64_t sweeps;
+    /* the bundle kernel of the islands solved on their own (island_solve) */
+    uint64_t bundle_launches;
+    double bundle_ms;     /* summed kernel durations */
+    uint64_t bundle_point_sweeps; /* contact points x sweeps of its islands (240 B each) */
 } axm_kernel_profile;
 
 typedef struct axm_world_s* axm_world;
@@ -236,8 +240,20 @@ typedef struct axm_world_totals {
     uint64_t contact_sweeps;
     uint32_t max_sweeps;
     uint32_t pad;
+    uint64_t local_contact_sweeps;  /* the part of contact_sweeps in islands solved on their own */
 } axm_world_totals;
 axm_status axm_world_get_totals(axm_world w, axm_world_totals* out, int reset);
+
+/* World snapshot: the description, every body's state (including sleep state, material,
+ * flags and filters) and the warm-start cache of the last step, in a checksummed binary
+ * file (magic "AXMWORLD", version 1).  axm_world_load replaces the world's description
+ * (keeping its device), bodies and cache with the file's: stepping on from a loaded
+ * snapshot gives bit-identical results to stepping on from where it was saved.  A file
+ * that does not match returns InvalidParameter and leaves the world as it was.  Stand-in
+ * for the reference's save/load of binary caches (PipelineCache::save/load,
+ * include/axiom/gpu/vk_compute_pipeline.hpp:263-270). */
+axm_status axm_world_save(axm_world w, const char* path);
+axm_status axm_world_load(axm_world w, const char* path);
 
 /* Enables / disables per-launch event timing of the sweep kernel; `out`
  * (optional) receives the accumulated profile, which is then reset. */
@@ -350,6 +366,13 @@ axm_status axm_mlp_adam_step(axm_mlp m, const float* grad, float lr, float beta1
  * epoch_loss (optional) receives each pass's mean batch loss. */
 axm_status axm_mlp_train(axm_mlp m, const float* X, const float* Y, uint32_t n, uint32_t epochs, uint32_t batch,
                          float lr, float* epoch_loss);
+/* Network file: parameters, Adam moments and step count in a checksummed binary file
+ * (magic "AXMMLP01", version 1).  Loading restores the network and its optimiser
+ * state exactly; a file that does not match returns InvalidParameter.  Stand-in for the
+ * reference's binary cache files (PipelineCache::save/load,
+ * include/axiom/gpu/vk_compute_pipeline.hpp:263-270). */
+axm_status axm_mlp_save(axm_mlp m, const char* path);
+axm_status axm_mlp_load(axm_mlp m, const char* path);
 /* Attaches a network to a world (same device; NULL detaches).  The caller keeps it alive. */
 axm_status axm_world_set_mlp(axm_world w, axm_mlp m);
 /* Keeps every step's per-point network inputs for axm_world_get_features. */
