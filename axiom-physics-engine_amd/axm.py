@@ -50,7 +50,7 @@ class WorldDesc(C.Structure):
         ("sleep_time", C.c_float), ("island_solve", C.c_uint32), ("max_bodies", C.c_uint32),
         ("max_pairs", C.c_uint32), ("max_manifolds", C.c_uint32), ("auto_grow", C.c_uint32),
         ("residual_tol", C.c_float), ("residual_every", C.c_uint32), ("max_iterations", C.c_uint32),
-        ("device", C.c_int32), ("solver_mode", C.c_uint32),
+        ("device", C.c_int32), ("solver_mode", C.c_uint32), ("strict", C.c_uint32),
     ]
 
 
@@ -106,7 +106,7 @@ EXPORTS = [
     "axm_world_get_bodies", "axm_world_body_count", "axm_world_step", "axm_world_run", "axm_world_sync", "axm_world_profile", "axm_world_get_totals", "axm_world_get_islands", "axm_world_configure", "axm_world_get_desc", "axm_world_save", "axm_world_load", "axm_mlp_save", "axm_mlp_load",
     "axm_world_get_pairs", "axm_world_get_manifolds", "axm_world_get_colours", "axm_world_get_impulses", "axm_world_get_friction_bounds",
     "axm_world_get_residuals", "axm_world_get_pre_solve_velocities", "axm_broadphase", "axm_collide",
-    "axm_collide_pairs", "axm_collide_pairs_noalloc", "axm_gjk_epa_pairs", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
+    "axm_collide_pairs", "axm_collide_pairs_noalloc", "axm_gjk_epa_pairs", "axm_gjk_epa_pairs_capped", "axm_scene_generate", "axm_mass_properties", "axm_rng_stream",
     "axm_mlp_create", "axm_mlp_destroy", "axm_mlp_init", "axm_mlp_set_params", "axm_mlp_get_params",
     "axm_mlp_forward", "axm_mlp_loss_grad", "axm_mlp_adam_step", "axm_mlp_train", "axm_world_set_mlp",
     "axm_world_set_feature_logging", "axm_world_get_features", "axm_world_set_sweep_config", "axm_world_autotune",
@@ -162,6 +162,8 @@ def lib():
     L.axm_collide_pairs_noalloc.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, C.c_float,
                                     u32p, f32p, f32p, f32p, u32p]
     L.axm_gjk_epa_pairs.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, f32p, i32p]
+    L.axm_gjk_epa_pairs_capped.argtypes = [C.c_int32, C.c_uint32, u32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p,
+                                           C.c_uint32, C.c_uint32, f32p, i32p]
     L.axm_scene_generate.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(BodySoA), u32p,
                                      C.POINTER(WorldDesc)]
     L.axm_mass_properties.argtypes = [C.c_uint32, f32p, C.c_float, f32p, f32p]
@@ -591,14 +593,17 @@ def collide_pairs(ta, pa, xa, qa, tb, pb, xb, qb, margin=0.02, device=0, noalloc
     return out
 
 
-def gjk_epa_pairs(ta, pa, xa, qa, tb, pb, xb, qb, device=0):
+def gjk_epa_pairs(ta, pa, xa, qa, tb, pb, xb, qb, device=0, gjk_iters=0, epa_iters=0):
+    """Batched GJK (+ EPA) on shape cores: (out [n, 10], status [n]); gjk_iters / epa_iters cap the
+    iterations (0: the defaults)."""
     n = len(ta)
     arrs = [np.ascontiguousarray(x, dt) for x, dt in
             ((ta, np.uint32), (pa, np.float32), (xa, np.float32), (qa, np.float32),
              (tb, np.uint32), (pb, np.float32), (xb, np.float32), (qb, np.float32))]
     out = np.zeros((n, 10), np.float32)
     st = np.zeros(n, np.int32)
-    _check(lib().axm_gjk_epa_pairs(device, n, _p(arrs[0], C.c_uint32), _p(arrs[1]), _p(arrs[2]), _p(arrs[3]),
-                                   _p(arrs[4], C.c_uint32), _p(arrs[5]), _p(arrs[6]), _p(arrs[7]), _p(out),
-                                   _p(st, C.c_int32)), "axm_gjk_epa_pairs")
+    _check(lib().axm_gjk_epa_pairs_capped(device, n, _p(arrs[0], C.c_uint32), _p(arrs[1]), _p(arrs[2]),
+                                          _p(arrs[3]), _p(arrs[4], C.c_uint32), _p(arrs[5]), _p(arrs[6]),
+                                          _p(arrs[7]), gjk_iters, epa_iters, _p(out), _p(st, C.c_int32)),
+           "axm_gjk_epa_pairs_capped")
     return out, st

@@ -86,16 +86,17 @@ __global__ void __launch_bounds__(64) k_collide_pairs(PairIn in, uint32_t n, flo
     }
 }
 
-__global__ void __launch_bounds__(64) k_gjk_epa_pairs(PairIn in, uint32_t n, float* out, int32_t* status) {
+__global__ void __launch_bounds__(64) k_gjk_epa_pairs(PairIn in, uint32_t n, float* out, int32_t* status,
+                                                       int gjk_iters, int epa_iters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Shp A = shape_at(in.ta, in.pa, in.xa, in.qa, i);
     const Shp B = shape_at(in.tb, in.pb, in.xb, in.qb, i);
-    const GjkOut g = gjk(A, B);
+    const GjkOut g = gjk(A, B, gjk_iters);
     float* o = out + (size_t)i * 10;
     if (g.status == 1) {
         EpaWork W;
-        const EpaOut e = epa(A, B, g, W);
+        const EpaOut e = epa(A, B, g, W, epa_iters);
         o[0] = e.depth;
         o[1] = e.pA.x; o[2] = e.pA.y; o[3] = e.pA.z;
         o[4] = e.pB.x; o[5] = e.pB.y; o[6] = e.pB.z;
@@ -268,6 +269,16 @@ axm_status axm_collide_pairs_noalloc(int32_t device, uint32_t n, const uint32_t*
 axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a,
                              const float* pos_a, const float* quat_a, const uint32_t* type_b, const float* prm_b,
                              const float* pos_b, const float* quat_b, float* out, int32_t* status) {
+    return axm_gjk_epa_pairs_capped(device, n, type_a, prm_a, pos_a, quat_a, type_b, prm_b, pos_b, quat_b, 0u, 0u, out,
+                                    status);
+}
+
+axm_status axm_gjk_epa_pairs_capped(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a,
+                                    const float* pos_a, const float* quat_a, const uint32_t* type_b,
+                                    const float* prm_b, const float* pos_b, const float* quat_b,
+                                    uint32_t gjk_max_iters, uint32_t epa_max_iters, float* out, int32_t* status) {
+    if (gjk_max_iters > 4096u || epa_max_iters > 4096u) return AXM_INVALID_PARAMETER;
+    const int gi = gjk_max_iters ? (int)gjk_max_iters : 32, ei = epa_max_iters ? (int)epa_max_iters : 48;
     if (!n) return AXM_SUCCESS;
     if (!type_a || !prm_a || !pos_a || !quat_a || !type_b || !prm_b || !pos_b || !quat_b || !out || !status)
         return AXM_NULL_POINTER;
@@ -280,7 +291,7 @@ axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a,
     float* d_o = ar.up<float>(nullptr, 10ull * n);
     int32_t* d_s = ar.up<int32_t>(nullptr, n);
     if (!d_o || !d_s) return AXM_BUFFER_ALLOCATION_FAILED;
-    hipLaunchKernelGGL(k_gjk_epa_pairs, dim3((n + 63) / 64), dim3(64), 0, 0, in, n, d_o, d_s);
+    hipLaunchKernelGGL(k_gjk_epa_pairs, dim3((n + 63) / 64), dim3(64), 0, 0, in, n, d_o, d_s, gi, ei);
     if (hipDeviceSynchronize() != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     if (hipMemcpy(out, d_o, 40ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(status, d_s, 4ull * n, hipMemcpyDeviceToHost) != hipSuccess)

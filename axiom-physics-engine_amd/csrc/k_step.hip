@@ -2518,6 +2518,8 @@ __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint
         cnt->acc_contact_sweeps +=
             (unsigned long long)(spread_sum(cnt, &SpreadRow::contacts) - cnt->loc_points) * swg + cnt->loc_point_sweeps;
         cnt->acc_local_contact_sweeps += cnt->loc_point_sweeps;
+        cnt->acc_gjk_fail += cnt->gjk_fail;
+        cnt->acc_epa_fail += cnt->epa_fail;
     }
     if (i >= n) return;
     const float4 v4 = B.sv[i];

@@ -158,6 +158,7 @@ struct Counters {
     unsigned long long acc_sweeps;
     unsigned long long acc_contact_sweeps;  // sum of contact points x sweeps
     unsigned long long acc_local_contact_sweeps;  // the part in islands solved on their own
+    uint32_t acc_gjk_fail, acc_epa_fail;          // GJK / EPA failures
     // ---- spread sums, cleared per step
     alignas(128) SpreadRow sp[SPREAD];
 };

@@ -836,6 +836,22 @@ axm_status ensure_res(axm_world_s* w, uint32_t iters) {
 }
 
 // One step with recovery; synchronises once.
+// Whether the last step met the residual tolerance (large islands and every island solved on
+// its own); true without a tolerance.
+bool step_reached(const axm_world_s* w) {
+    if (!(w->desc.residual_tol > 0.0f)) return true;
+    return w->h_cnt->tol_hit != 0u && w->h_cnt->loc_miss == 0u;
+}
+// With desc.strict, the status of steps that completed with GJK or EPA failures or with steps
+// that stopped at max_iterations above the tolerance (their results are kept).
+axm_status strict_status(const axm_world_s* w, uint32_t gjk_fail, uint32_t epa_fail, uint32_t missed) {
+    if (!w->desc.strict) return AXM_SUCCESS;
+    if (gjk_fail) return AXM_GJK_FAILED_TO_CONVERGE;
+    if (epa_fail) return AXM_EPA_FAILED_TO_CONVERGE;
+    if (missed) return AXM_CONSTRAINT_SOLVER_FAILURE;
+    return AXM_SUCCESS;
+}
+
 axm_status step_sync(axm_world_s* w, float dt, StepOpts o, axm_step_stats* st) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return AXM_INVALID_PARAMETER;
     if (w->desc.solver_mode == 1) o.colour_launches = true;
@@ -907,7 +923,7 @@ axm_status step_sync(axm_world_s* w, float dt, StepOpts o, axm_step_stats* st) {
         free_ev();
     }
     if (st) *st = S;
-    return AXM_SUCCESS;
+    return strict_status(w, S.gjk_fail, S.epa_fail, step_reached(w) ? 0u : 1u);
 }
 
 // Captures one step with the given warm-start buffer parity into a graph.
@@ -1034,7 +1050,7 @@ struct HostBodies {
     std::vector<uint32_t> wake;
 };
 axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, bool use_filter,
-                          const HostBodies& hb);
+                          const HostBodies& hb, bool keep_graphs = false);
 
 axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) {
     if (!w || !b) return AXM_NULL_POINTER;
@@ -1122,14 +1138,18 @@ axm_status axm_world_set_bodies(axm_world w, const axm_body_soa* b, uint32_t n) 
 
 
 // Uploads a body set and resets the per-world step state (counters, warm-start cache,
-// occupancy bitmap).
+// occupancy bitmap).  With keep_graphs the captured steps stay (the caller has checked that
+// nothing they hold changes: description, body count, worlds, cell, filters) unless a buffer
+// has to grow.
 axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, bool use_filter,
-                          const HostBodies& hb) {
+                          const HostBodies& hb, bool keep_graphs) {
     const std::vector<float4>&x = hb.x, &q = hb.q, &v = hb.v, &av = hb.av, &ii = hb.ii, &sh = hb.sh, &mat = hb.mat;
     const std::vector<uint4>& bfl = hb.bfl;
     const std::vector<uint32_t>&wd = hb.wd, &first = hb.first;
     AXM_HIP(hipStreamSynchronize(w->stream));
-    w->drop_graphs();
+    const bool grow = n > w->cap_n || (w->desc.max_pairs == 0 && w->cap_pairs < 8ull * n) ||
+                      (w->desc.max_manifolds == 0 && w->cap_man < 4ull * n) || w->rcap[1 - w->cur] < w->cap_man;
+    if (!keep_graphs || grow) w->drop_graphs();
     if (n > w->cap_n) AXM_TRY(alloc_bodies(w, n));
     if (w->desc.max_pairs == 0 && w->cap_pairs < 8ull * n) AXM_TRY(alloc_pairs(w, 8 * n));
     if (w->desc.max_manifolds == 0 && w->cap_man < 4ull * n) AXM_TRY(alloc_manifolds(w, 4 * n));
@@ -1239,6 +1259,10 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         return step_sync(w, dt, o, last);
     }
     AXM_TRY(ensure_res(w, iters));
+    // run totals at the start, for the strict status of the whole run
+    AXM_TRY(read_counters(w));
+    const uint32_t acc0_steps = w->h_cnt->acc_steps, acc0_reached = w->h_cnt->acc_reached;
+    const uint32_t acc0_gjk = w->h_cnt->acc_gjk_fail, acc0_epa = w->h_cnt->acc_epa_fail;
     uint32_t done = 0;
     if (!w->prev_valid) {
         // the first step after new bodies has no warm-start cache: run it eagerly
@@ -1355,7 +1379,11 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
         fill_stats(w, S);
         *last = S;
     }
-    return AXM_SUCCESS;
+    AXM_TRY(read_counters(w));
+    const Counters& h = *w->h_cnt;
+    const uint32_t ran = h.acc_steps - acc0_steps, reached = h.acc_reached - acc0_reached;
+    return strict_status(w, h.acc_gjk_fail - acc0_gjk, h.acc_epa_fail - acc0_epa,
+                         w->desc.residual_tol > 0.0f ? ran - reached : 0u);
 }
 
 axm_status axm_world_get_desc(axm_world w, axm_world_desc* out) {
@@ -1414,7 +1442,7 @@ axm_status axm_world_get_totals(axm_world w, axm_world_totals* out, int reset) {
     if (reset) {
         // the accumulators sit between steps_done's block and the spread rows
         const size_t off = offsetof(Counters, acc_steps);
-        const size_t len = offsetof(Counters, acc_local_contact_sweeps) + sizeof(unsigned long long) - off;
+        const size_t len = offsetof(Counters, acc_epa_fail) + sizeof(uint32_t) - off;
         launch_fill_u32(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(w->cnt) + off), 0u,
                         (uint32_t)(len / 4), w->stream);
         AXM_HIP(hipStreamSynchronize(w->stream));
@@ -1882,14 +1910,19 @@ axm_status axm_world_load(axm_world w, const char* path) {
     // the world keeps its device; everything else comes from the file
     d.device = w->desc.device;
     AXM_HIP(hipStreamSynchronize(w->stream));
+    // the captured steps stay valid when the file holds what they were captured with
+    const bool same = std::memcmp(&d, &w->desc, sizeof(d)) == 0 && n == w->n &&
+                      std::max(h.n_worlds, 1u) == w->n_worlds && h.cell == w->cell &&
+                      (h.use_filter != 0) == w->use_filter;
     w->desc = d;
-    AXM_TRY(install_bodies(w, n, std::max(h.n_worlds, 1u), h.cell, h.use_filter != 0, hb));
+    AXM_TRY(install_bodies(w, n, std::max(h.n_worlds, 1u), h.cell, h.use_filter != 0, hb, same));
     hipStream_t s = w->stream;
     AXM_HIP(hipMemcpy(&w->cnt->steps_done, &h.steps_done, 4, hipMemcpyHostToDevice));
     if (mp) {
         // the previous step's cache in the set the next step reads, and its lookup table
-        if (w->cap_man < mp) AXM_TRY(alloc_manifolds(w, mp));
         const int pv = 1 - w->cur;
+        if (w->cap_man < mp || w->rcap[pv] < std::max(w->cap_man, mp)) w->drop_graphs();
+        if (w->cap_man < mp) AXM_TRY(alloc_manifolds(w, mp));
         if (w->rcap[pv] < w->cap_man) AXM_TRY(alloc_rows(w, pv, w->cap_man));
         const SolverRows& R = w->R[pv];
         const size_t rc = w->rcap[pv];

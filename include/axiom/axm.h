@@ -115,6 +115,11 @@ typedef struct axm_world_desc {
     int32_t device;        /* HIP device ordinal */
     uint32_t solver_mode;  /* 0: one persistent sweep kernel ordered by per-body counters (when <= 64 colours);
                               1: one launch per colour per sweep */
+    uint32_t strict;       /* 1: a step (or run) that completes with a GJK or EPA failure returns
+                              GJKFailedToConverge / EPAFailedToConverge, and one whose sweeps stop at
+                              max_iterations above residual_tol returns ConstraintSolverFailure; the
+                              results are kept (the state has advanced).  0: such steps succeed and
+                              report it in their statistics only */
 } axm_world_desc;
 
 /* Host-side body arrays (structure of arrays).  Field set follows
@@ -329,6 +334,12 @@ axm_status axm_collide_pairs_noalloc(int32_t device, uint32_t n, const uint32_t*
  * pair (distance or depth, pA[3], pB[3], n[3]); status per pair: 0 separated,
  * 1 penetrating, 301 / 302 on failure (GJKFailedToConverge / EPAFailedToConverge,
  * include/axiom/core/error_code.hpp:37-38). */
+/* The same with the iteration caps of GJK and EPA given (0: the defaults, 32 and 48; at most
+ * 4096): a pair that needs more returns 301 / 302, as in the world step's narrowphase. */
+axm_status axm_gjk_epa_pairs_capped(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a,
+                                    const float* pos_a, const float* quat_a, const uint32_t* type_b,
+                                    const float* prm_b, const float* pos_b, const float* quat_b,
+                                    uint32_t gjk_max_iters, uint32_t epa_max_iters, float* out, int32_t* status);
 axm_status axm_gjk_epa_pairs(int32_t device, uint32_t n, const uint32_t* type_a, const float* prm_a, const float* pos_a,
                              const float* quat_a, const uint32_t* type_b, const float* prm_b, const float* pos_b,
                              const float* quat_b, float* out, int32_t* status);

@@ -102,6 +102,9 @@ def lib():
                                     u32p]
     L.orc_gjk_epa.restype = C.c_int
     L.orc_gjk_epa.argtypes = [C.c_int, C.c_uint32, f32p, f32p, f32p, C.c_uint32, f32p, f32p, f32p, f32p]
+    L.orc_gjk_epa_capped.restype = C.c_int
+    L.orc_gjk_epa_capped.argtypes = [C.c_int, C.c_uint32, f32p, f32p, f32p, C.c_uint32, f32p, f32p, f32p, C.c_int,
+                                     C.c_int, f32p]
     L.orc_mlp_nparam.restype = C.c_uint32
     L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p]
     L.orc_mlp_grad.restype = C.c_double
@@ -324,12 +327,12 @@ def collide(ta, pa, xa, qa, tb, pb, xb, qb, margin=0.02, fp64=False):
     return dict(npts=k, pos=pos[:k], normal=nrm[:k], depth=dep[:k], feature=fid[:k], status=status)
 
 
-def gjk_epa(ta, pa, xa, qa, tb, pb, xb, qb, fp64=False):
+def gjk_epa(ta, pa, xa, qa, tb, pb, xb, qb, fp64=False, gjk_iters=32, epa_iters=48):
     L = lib()
     f = lambda x: np.ascontiguousarray(x, np.float32)
     out = np.zeros(10, np.float32)
-    st = L.orc_gjk_epa(1 if fp64 else 0, ta, _p(f(pa)), _p(f(xa)), _p(f(qa)), tb, _p(f(pb)), _p(f(xb)), _p(f(qb)),
-                       _p(out))
+    st = L.orc_gjk_epa_capped(1 if fp64 else 0, ta, _p(f(pa)), _p(f(xa)), _p(f(qa)), tb, _p(f(pb)), _p(f(xb)),
+                              _p(f(qb)), gjk_iters, epa_iters, _p(out))
     return out, st
 
 

@@ -486,14 +486,14 @@ void orc_collide_plane(int fp64, uint32_t ta, const float* pa, const float* xa, 
 // GJK between the cores (box polytope / capsule segment / sphere centre) of two
 // shapes, followed by EPA when the cores intersect.  out: dist_or_depth,
 // pA[3], pB[3], n[3]; returns 0 separated, 1 penetrating (EPA ok), 301 / 302 on failure.
-int orc_gjk_epa(int fp64, uint32_t ta, const float* pa, const float* xa, const float* qa, uint32_t tb, const float* pb,
-                const float* xb, const float* qb, float* out) {
+int orc_gjk_epa_capped(int fp64, uint32_t ta, const float* pa, const float* xa, const float* qa, uint32_t tb,
+                       const float* pb, const float* xb, const float* qb, int gjk_iters, int epa_iters, float* out) {
     auto run = [&](auto tag) -> int {
         using T = decltype(tag);
         const Shape<T> A = mk_shape<T>(ta, pa, xa, qa), B = mk_shape<T>(tb, pb, xb, qb);
-        const GjkOut<T> g = gjk(A, B);
+        const GjkOut<T> g = gjk(A, B, gjk_iters);
         if (g.status == 1) {
-            const EpaOut<T> e = epa(A, B, g);
+            const EpaOut<T> e = epa(A, B, g, epa_iters);
             out[0] = (float)e.depth;
             out[1] = (float)e.pA.x; out[2] = (float)e.pA.y; out[3] = (float)e.pA.z;
             out[4] = (float)e.pB.x; out[5] = (float)e.pB.y; out[6] = (float)e.pB.z;
@@ -507,6 +507,10 @@ int orc_gjk_epa(int fp64, uint32_t ta, const float* pa, const float* xa, const f
         return g.status == 0 ? 0 : GEOM_GJK_FAIL;
     };
     return fp64 ? run(0.0) : run(0.0f);
+}
+int orc_gjk_epa(int fp64, uint32_t ta, const float* pa, const float* xa, const float* qa, uint32_t tb, const float* pb,
+                const float* xb, const float* qb, float* out) {
+    return orc_gjk_epa_capped(fp64, ta, pa, xa, qa, tb, pb, xb, qb, 32, 48, out);
 }
 
 // ---------------------------------------------------------------- warm-start MLP

@@ -169,3 +169,19 @@ def test_scene_config3_batch(axm):
     w0 = np.nonzero(b.world == 5)[0]
     s, _ = axm.scene(0, seed(3, 5 + 1), len(w0))
     assert np.array_equal(s.pos, b.pos[w0]) and np.array_equal(s.shape_params, b.shape_params[w0])
+
+
+def test_training_log_round_trip(tmp_path):
+    """The contact-impulse training log (warmstart.save_log / load_log): rows and description come
+    back exactly; rows of the wrong width are refused."""
+    import warmstart
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(1000, 24)).astype(np.float32)
+    Y = rng.normal(size=(1000, 3)).astype(np.float32)
+    p = str(tmp_path / "log.npz")
+    warmstart.save_log(p, X, Y, "seeded piles, steps 150-600")
+    X2, Y2, src = warmstart.load_log(p)
+    assert np.array_equal(X, X2) and np.array_equal(Y, Y2) and src == "seeded piles, steps 150-600"
+    import pytest
+    with pytest.raises(ValueError):
+        warmstart.save_log(p, X[:, :20], Y)

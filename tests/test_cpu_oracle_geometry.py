@@ -372,3 +372,29 @@ def test_epa_box_segment_core_matches_sat(orc):
         n += 1
         assert abs(out[0] - depth) <= 1e-5 * max(1.0, depth), (out[0], depth)
     assert n > 50
+
+
+def test_gjk_epa_iteration_caps(orc):
+    """The iteration caps: a cap below what a pair needs reports GJKFailedToConverge (301) or
+    EPAFailedToConverge (302), the reference's codes (include/axiom/core/error_code.hpp:37-38);
+    a cap at or above the iterations used changes nothing, bit for bit."""
+    rng = np.random.default_rng(41)
+    fails = {301: 0, 302: 0}
+    for _ in range(80):
+        ha, hb = rng.uniform(0.25, 0.75, 3), rng.uniform(0.25, 0.75, 3)
+        qa, qb = rand_quat(rng), rand_quat(rng)
+        xa = np.zeros(3)
+        xb = rng.uniform(-1.2, 1.2, 3)
+        out, st = orc.gjk_epa(BOX, ha, xa, qa, BOX, hb, xb, qb)
+        assert st in (0, 1)
+        big, stb = orc.gjk_epa(BOX, ha, xa, qa, BOX, hb, xb, qb, gjk_iters=200, epa_iters=200)
+        assert stb == st and np.array_equal(big, out)
+        _, s1 = orc.gjk_epa(BOX, ha, xa, qa, BOX, hb, xb, qb, gjk_iters=1)
+        assert s1 == 301  # one GJK iteration never decides a box pair
+        fails[301] += 1
+        if st == 1:
+            # one EPA expansion settles a pair only when the first polytope already holds the answer
+            _, se = orc.gjk_epa(BOX, ha, xa, qa, BOX, hb, xb, qb, epa_iters=1)
+            assert se in (1, 302)
+            fails[302] += se == 302
+    assert fails[302] > 10

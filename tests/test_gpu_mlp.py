@@ -166,6 +166,48 @@ def test_step_from_network_prediction(axm, orc, gpu):
     assert st.manifolds == len(m["a"])
 
 
+def test_pile_network_step_matches_oracle(axm, orc, gpu):
+    """A network trained on the GPU on contacts of the dense-pile configuration (6000 bodies,
+    converged impulses as targets) and applied on a held-out pile of that configuration: the step
+    starts from the projected prediction and matches the oracle solving the same manifolds from
+    the same warm start."""
+    import warmstart
+    xs, ys = [], []
+    for k in range(2):
+        b, d = axm.scene(1, seed(1, 20 + k), 6000)
+        X, Y = warmstart.collect(b, d, 160, 0, tol=1e-5)
+        xs.append(X[len(X) // 3:])
+        ys.append(Y[len(Y) // 3:])
+    net, losses = warmstart.train(np.concatenate(xs), np.concatenate(ys), epochs=3, batch=2048, lr=1e-3, clip=64.0)
+    assert losses[-1] < losses[0]
+    b, d = axm.scene(1, seed(1, 30), 6000)
+    d.warm_start = axm.WARM_MLP
+    w = axm.World(d)
+    w.set_bodies(b)
+    w.set_mlp(net)
+    for _ in range(150):
+        w.step(DT, 20)
+    g0 = w.get_bodies()
+    st = w.step(DT, 20)
+    m = w.manifolds()
+    lam0, lam1 = w.impulses()
+    fb = w.friction_bounds()
+    assert st.manifolds > 2000
+    ob = b.copy()
+    ob.pos, ob.quat, ob.lin_vel, ob.ang_vel = g0.pos, g0.quat, g0.lin_vel, g0.ang_vel
+    p = orc.params_from_desc(d)
+    p.warm_start = 1
+    ow = orc.OracleWorld(p)
+    ow.set_bodies(ob)
+    ow.step_given(DT, 20, m, warm=lam0, colours=w.colours()[0], bounds=fb)
+    g1 = w.get_bodies()
+    pos1, quat1, v1, w1 = ow.get_state()
+    assert rel_err(g1.lin_vel, v1) <= 1e-4
+    assert rel_err(g1.ang_vel, w1) <= 1e-4
+    w.close()
+    net.close()
+
+
 def _pile_state(axm, steps=60):
     b, d = axm.scene(0, seed(0), 1000)
     w = axm.World(d)

@@ -113,3 +113,30 @@ def test_network_file_resumes_training(axm, gpu, tmp_path):
     with pytest.raises(axm.AxmError) as e:
         net2.load(str(bad))
     assert e.value.code == axm.INVALID_PARAMETER
+
+
+def test_reload_into_the_same_world_replays_its_graphs(axm, gpu, tmp_path):
+    """Reloading a world's own snapshot keeps its captured step graphs (nothing they hold
+    changed): the replayed steps after the reload end bit for bit where the first ones did, also
+    with profiling (eagerly launched sweep kernels between two graphs) on."""
+    b, d = axm.scene(2, seed(2), 200000)
+    d.residual_tol = 1e-4
+    d.max_iterations = 300
+    w = axm.World(d)
+    w.set_bodies(b)
+    w.run(DT, 300, 12, graph=True)
+    path = str(tmp_path / "same.axm")
+    w.save(path)
+    outs = []
+    for prof in (False, True, True):
+        w.load(path)
+        w.profile(prof)
+        st = w.run(DT, 300, 15, graph=True)
+        if prof:
+            p = w.profile(False)
+            assert p.steps == 15 and p.bundle_launches == 15 and p.bundle_point_sweeps > 0
+        outs.append((st.iterations, _arrays(w)))
+    for it, arrs in outs[1:]:
+        assert it == outs[0][0]
+        _same(outs[0][1], arrs)
+    w.close()
