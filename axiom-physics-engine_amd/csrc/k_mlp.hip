@@ -1,9 +1,9 @@
 // Warm-start MLP kernels for gfx950: BF16 MFMA forward and backward with fp32
 // accumulation, a fixed-order reduction of per-block gradients, and Adam.
 //
-// MFMA shape: v_mfma_f32_16x16x16_bf16 (one wave computes a 16x16 fp32 tile
-// from 16x16 bf16 A and B tiles).  Lane l (r = l % 16, q = l / 16) holds
-//   A[r][4q + i], B[4q + i][r] (i = 0..3) and D[4q + i][r].
+// MFMA shape: gfx950's v_mfma_f32_16x16x32_bf16 (one wave computes a 16x16 fp32 tile
+// from a 16x32 bf16 A and a 32x16 bf16 B tile).  Lane l (r = l % 16, q = l / 16) holds
+//   A[r][8q + i], B[8q + i][r] (i = 0..7) and D[4q + i][r] (i = 0..3).
 // A 64-row block is one workgroup of 4 waves, 16 rows per wave; hidden
 // activations are staged through LDS between layers (row-major, bf16).
 // The same rounding points as the CPU oracle's bf16 emulation: inputs and
@@ -13,7 +13,8 @@
 
 namespace axm {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));   // bf16 bit patterns
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -25,18 +26,17 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ uint16_t relu_bf(float v) { return f2bf(v > 0.0f ? v : 0.0f); }
 
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b), c, 0,
+                                                   0, 0);
 }
-// four consecutive bf16 (8-byte aligned)
-__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *(const bf16x4*)p; }
-// four bf16 `stride` elements apart
-__device__ __forceinline__ bf16x4 ld4s(const uint16_t* p, int stride) {
-    bf16x4 r;
-    r.x = (short)p[0];
-    r.y = (short)p[stride];
-    r.z = (short)p[2 * stride];
-    r.w = (short)p[3 * stride];
+// eight consecutive bf16 (16-byte aligned)
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) { return *(const bf16x8*)p; }
+// eight bf16 `stride` elements apart
+__device__ __forceinline__ bf16x8 ld8s(const uint16_t* p, int stride) {
+    bf16x8 r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = (short)p[i * stride];
     return r;
 }
 __device__ __forceinline__ f32x4 splat(float v) {
@@ -78,28 +78,23 @@ __device__ __forceinline__ void load_weights(const MlpDev& P, uint16_t* sW1, uin
     if (threadIdx.x < 4u) sB[2 * MLP_H + threadIdx.x] = threadIdx.x < (uint32_t)MLP_OUT ? P.prm[MLP_OFF_B3 + threadIdx.x] : 0.0f;
 }
 
-// Input tile of one wave: A operands of the two 16-wide k steps (k >= 24 padded with 0).
-__device__ __forceinline__ void load_x(const float* __restrict__ X, uint32_t n, uint32_t row, int q, bf16x4& a0,
-                                       bf16x4& a1) {
+// Input tile of one wave: the A operand of the one 32-wide k step (k >= 24 padded with 0).
+__device__ __forceinline__ bf16x8 load_x(const float* __restrict__ X, uint32_t n, uint32_t row, int q) {
     const bool ok = row < n;
     const float* xr = X + (size_t)row * MLP_IN;
+    bf16x8 a;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k0 = 4 * q + i, k1 = 16 + 4 * q + i;
-        a0[i] = (short)(ok ? f2bf(xr[k0]) : (uint16_t)0);
-        a1[i] = (short)((ok && k1 < MLP_IN) ? f2bf(xr[k1]) : (uint16_t)0);
+    for (int i = 0; i < 8; ++i) {
+        const int k = 8 * q + i;
+        a[i] = (short)((ok && k < MLP_IN) ? f2bf(xr[k]) : (uint16_t)0);
     }
+    return a;
 }
 
 // Layer 1 of a 16-row tile: acc[j] = b1 + X W1^T for output tile j.
-__device__ __forceinline__ void layer1(const uint16_t* sW1, const float* sB, bf16x4 a0, bf16x4 a1, int r, int q,
-                                       f32x4 acc[8]) {
+__device__ __forceinline__ void layer1(const uint16_t* sW1, const float* sB, bf16x8 a, int r, int q, f32x4 acc[8]) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        acc[j] = splat(sB[16 * j + r]);
-        acc[j] = mfma16(a0, ld4(&sW1[(16 * j + r) * MLP_INP + 4 * q]), acc[j]);
-        acc[j] = mfma16(a1, ld4(&sW1[(16 * j + r) * MLP_INP + 16 + 4 * q]), acc[j]);
-    }
+    for (int j = 0; j < 8; ++j) acc[j] = mfma32(a, ld8(&sW1[(16 * j + r) * MLP_INP + 8 * q]), splat(sB[16 * j + r]));
 }
 // Layer 2 of a 16-row tile from the wave's activations act[16][128].
 __device__ __forceinline__ void layer2(const uint16_t* sW2, const float* sB, const uint16_t* act, int r, int q,
@@ -107,17 +102,17 @@ __device__ __forceinline__ void layer2(const uint16_t* sW2, const float* sB, con
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = splat(sB[MLP_H + 16 * j + r]);
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) {
-        const bf16x4 a = ld4(&act[r * MLP_H + 16 * kk + 4 * q]);
+    for (int kk = 0; kk < 4; ++kk) {
+        const bf16x8 a = ld8(&act[r * MLP_H + 32 * kk + 8 * q]);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[j] = mfma16(a, ld4(&sW2[(16 * j + r) * MLP_H + 16 * kk + 4 * q]), acc[j]);
+        for (int j = 0; j < 8; ++j) acc[j] = mfma32(a, ld8(&sW2[(16 * j + r) * MLP_H + 32 * kk + 8 * q]), acc[j]);
     }
 }
 __device__ __forceinline__ f32x4 layer3(const uint16_t* sW3, const float* sB, const uint16_t* act, int r, int q) {
     f32x4 y = splat(r < MLP_OUT ? sB[2 * MLP_H + r] : 0.0f);
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk)
-        y = mfma16(ld4(&act[r * MLP_H + 16 * kk + 4 * q]), ld4(&sW3[r * MLP_H + 16 * kk + 4 * q]), y);
+    for (int kk = 0; kk < 4; ++kk)
+        y = mfma32(ld8(&act[r * MLP_H + 32 * kk + 8 * q]), ld8(&sW3[r * MLP_H + 32 * kk + 8 * q]), y);
     return y;
 }
 // ReLU + bf16 of a 16x128 accumulator set into row-major act[16][128].
@@ -149,10 +144,9 @@ __global__ void __launch_bounds__(256) k_mlp_forward(MlpDev P, const float* __re
     uint16_t* act = sAct[wave];
     for (uint32_t base = blockIdx.x * MLP_ROWS_PER_BLOCK; base < n; base += gridDim.x * MLP_ROWS_PER_BLOCK) {
         const uint32_t r0 = base + 16u * wave;
-        bf16x4 a0, a1;
-        load_x(X, n, r0 + r, q, a0, a1);
+        const bf16x8 a = load_x(X, n, r0 + r, q);
         f32x4 acc[8];
-        layer1(sW1, sB, a0, a1, r, q, acc);
+        layer1(sW1, sB, a, r, q, acc);
         store_act(act, acc, r, q);
         __syncthreads();
         layer2(sW2, sB, act, r, q, acc);
@@ -208,17 +202,13 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev P, const float* __restr
     for (uint32_t base = blockIdx.x * MLP_ROWS_PER_BLOCK; base < n; base += gridDim.x * MLP_ROWS_PER_BLOCK) {
         const uint32_t r0 = base + 16u * wave;
         // ---- forward (keep x, h1, h2 in LDS)
-        bf16x4 a0, a1;
-        load_x(X, n, r0 + r, q, a0, a1);
+        const bf16x8 a = load_x(X, n, r0 + r, q);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            sX[(16 * wave + r) * MLP_INP + 4 * q + i] = (uint16_t)a0[i];
-            sX[(16 * wave + r) * MLP_INP + 16 + 4 * q + i] = (uint16_t)a1[i];
-        }
+        for (int i = 0; i < 8; ++i) sX[(16 * wave + r) * MLP_INP + 8 * q + i] = (uint16_t)a[i];
         f32x4 acc[8];
         uint16_t* h1w = sA1 + 16 * wave * MLP_H;
         uint16_t* h2w = sA2 + 16 * wave * MLP_H;
-        layer1(sW1, sB, a0, a1, r, q, acc);
+        layer1(sW1, sB, a, r, q, acc);
         store_act(h1w, acc, r, q);
         __syncthreads();
         layer2(sW2, sB, h1w, r, q, acc);
@@ -263,14 +253,14 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev P, const float* __restr
             for (int rr = 0; rr < MLP_ROWS_PER_BLOCK; ++rr) gb += bf2f(sA2[rr * MLP_H + tid]);
         // ---- layer 2: dW2[o][k] += sum_r d2[r][o] h1[r][k]; this wave owns o tiles 2w, 2w+1
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bf16x4 A0 = ld4s(&sA2[(16 * c + 4 * q) * MLP_H + 16 * (2 * wave) + r], MLP_H);
-            const bf16x4 A1 = ld4s(&sA2[(16 * c + 4 * q) * MLP_H + 16 * (2 * wave + 1) + r], MLP_H);
+        for (int c = 0; c < 2; ++c) {
+            const bf16x8 A0 = ld8s(&sA2[(32 * c + 8 * q) * MLP_H + 16 * (2 * wave) + r], MLP_H);
+            const bf16x8 A1 = ld8s(&sA2[(32 * c + 8 * q) * MLP_H + 16 * (2 * wave + 1) + r], MLP_H);
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
-                const bf16x4 Bm = ld4s(&sA1[(16 * c + 4 * q) * MLP_H + 16 * b + r], MLP_H);
-                gW2[0][b] = mfma16(A0, Bm, gW2[0][b]);
-                gW2[1][b] = mfma16(A1, Bm, gW2[1][b]);
+                const bf16x8 Bm = ld8s(&sA1[(32 * c + 8 * q) * MLP_H + 16 * b + r], MLP_H);
+                gW2[0][b] = mfma32(A0, Bm, gW2[0][b]);
+                gW2[1][b] = mfma32(A1, Bm, gW2[1][b]);
             }
         }
         // d1 (this wave's rows) = bf16(d2 W2), masked by relu'(h1) after every wave has read h1
@@ -278,10 +268,10 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev P, const float* __restr
 #pragma unroll
         for (int j = 0; j < 8; ++j) d1[j] = splat(0.0f);
 #pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-            const bf16x4 A = ld4(&sA2[(16 * wave + r) * MLP_H + 16 * kk + 4 * q]);
+        for (int kk = 0; kk < 4; ++kk) {
+            const bf16x8 A = ld8(&sA2[(16 * wave + r) * MLP_H + 32 * kk + 8 * q]);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) d1[j] = mfma16(A, ld4s(&sW2[(16 * kk + 4 * q) * MLP_H + 16 * j + r], MLP_H), d1[j]);
+            for (int j = 0; j < 8; ++j) d1[j] = mfma32(A, ld8s(&sW2[(32 * kk + 8 * q) * MLP_H + 16 * j + r], MLP_H), d1[j]);
         }
         __syncthreads();
 #pragma unroll
@@ -294,13 +284,13 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev P, const float* __restr
         __syncthreads();
         // ---- layer 1: dW1[o][k] += sum_r d1[r][o] x[r][k]
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < 2; ++c) {
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
-                const bf16x4 A = ld4s(&sA1[(16 * c + 4 * q) * MLP_H + 16 * (2 * wave + a) + r], MLP_H);
+                const bf16x8 A = ld8s(&sA1[(32 * c + 8 * q) * MLP_H + 16 * (2 * wave + a) + r], MLP_H);
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
-                    gW1[a][b] = mfma16(A, ld4s(&sX[(16 * c + 4 * q) * MLP_INP + 16 * b + r], MLP_INP), gW1[a][b]);
+                    gW1[a][b] = mfma32(A, ld8s(&sX[(32 * c + 8 * q) * MLP_INP + 16 * b + r], MLP_INP), gW1[a][b]);
             }
         }
         if (tid >= MLP_H) {

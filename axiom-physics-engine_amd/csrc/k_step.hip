@@ -2181,13 +2181,13 @@ __device__ __forceinline__ void local_done(Counters* cnt, uint32_t sweeps, float
 // Islands of at most ISLAND_LANE_MAX manifolds, bundled: the islands whose first slot lies
 // in window w (BUNDLE_SLOTS slots from the first small-island slot on) are solved by
 // workgroup w, one wave, one lane per slot (at most BUNDLE_SLOTS + ISLAND_LANE_MAX - 1 lanes).
-// Each sweep visits the bundle's colours in turn, every lane of that colour at once (within
-// an island a colour's slots share no dynamic body, and islands share none at all), with a
-// barrier between colours; then each island's residual is the maximum over its lanes, and an
-// island that is done stops while the others go on.
+// Each sweep runs phases: in phase p every island visits the slots of its p-th colour, all
+// at once (within an island a colour's slots share no dynamic body, and islands share none
+// at all), with a barrier between phases; then each island's residual is the maximum over
+// its lanes, and an island that is done stops while the others go on.
 //
 // Everything a sweep touches lives in LDS for the whole solve: each lane's Jacobian rows and
-// impulses (row r of point k at BUNDLE_LDS_ROW(k, r) * 64 + lane, row JROWS the impulses),
+// impulses (row r of point k at (k * BUNDLE_ROWS + r) * 64 + lane, row JROWS the impulses),
 // and each body's solver velocity in one home slot, the first (lane, side) of its island that
 // names it (dynamic bodies appear in one island only; a static partner keeps its own slot).
 // The visits are the arithmetic of solve_slot on those copies; velocities and impulses go
@@ -2215,6 +2215,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
     __shared__ uint32_t isl_res[2][64];  // per island (first slot - window start): residual bits, two sweeps
     __shared__ uint32_t isl_pts[64];
     __shared__ uint2 ab[64];
+    __shared__ uint32_t colr[64];
     __shared__ uint32_t kmax_w;
     if (cnt->sticky != 0u) return;
     const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob, lane = threadIdx.x;
@@ -2242,6 +2243,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
         const uint32_t np = min(h.z, 4u);
         const bool plane = (h.y & PLANE_BIT) != 0u;
         ab[lane] = make_uint2(h.x, h.y);
+        colr[lane] = mine ? colour : UNSET;
         isl_res[0][lane] = 0u;
         isl_res[1][lane] = 0u;
         isl_pts[lane] = 0u;
@@ -2264,10 +2266,13 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
             }
         }
         __syncthreads();
-        // home slot of each side: the first (lane, side) of the island naming the body
-        uint32_t home_a = lane, home_b = 64u + lane;
+        // home slot of each side: the first (lane, side) of the island naming the body; the
+        // lane's phase: the colours of its island below its own (an island's slots ascend by
+        // colour), so each island visits its own colours in turn, lanes of one phase at once
+        uint32_t home_a = lane, home_b = 64u + lane, phase = 0;
         if (mine) {
-            atomicMax(&kmax_w, colour);
+            for (uint32_t j = li + 1u; j <= lane; ++j) phase += colr[j] != colr[j - 1u] ? 1u : 0u;
+            atomicMax(&kmax_w, phase);
             atomicAdd(&isl_pts[li], np);
             auto home_of = [&](uint32_t body, uint32_t own) {
                 for (uint32_t j = li; j <= lane; ++j) {
@@ -2288,7 +2293,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
         for (uint32_t it = 0; it < max_sweeps; ++it) {
             if (!__syncthreads_or(active)) break;
             for (uint32_t c = 0; c <= kmax; ++c) {
-                if (active && colour == c) {
+                if (active && phase == c) {
                     SB A, Bd;
                     home_load(bl, home_a, A);
                     if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);

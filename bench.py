@@ -379,6 +379,23 @@ def tolerance_comparison(tol, max_sweeps, worlds):
     return out
 
 
+def mlp_forward_rate(rows, reps):
+    """Inference of the warm-start network on `rows` synthetic contact points, `reps` times
+    (the rocprof kernel time of k_mlp_forward gives the MFMA rate: 2 x (32x128 + 128x128 +
+    128x16) FLOP per row at the padded MFMA shapes, 2 x (24x128 + 128x128 + 128x3) useful)."""
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(rows, axm.MLP_INPUTS)).astype(np.float32)
+    net = axm.Mlp(0, seed=3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(X)
+    el = time.perf_counter() - t0
+    net.close()
+    print(json.dumps({"mlp_forward": {"rows": rows, "reps": reps, "seconds_with_copies": round(el, 4),
+                                      "flop_per_row_padded": 2 * (32 * 128 + 128 * 128 + 128 * 16),
+                                      "flop_per_row_useful": 2 * (24 * 128 + 128 * 128 + 128 * 3)}}), flush=True)
+
+
 def copy_calibration(mib, reps):
     """Plain device-to-device copies of a known size (read `mib` MiB, write `mib` MiB per
     dispatch), for scaling the HBM counters FETCH_SIZE / WRITE_SIZE of a counter run.
@@ -418,11 +435,16 @@ def main():
     ap.add_argument("--pile-steps", type=int, default=600)
     ap.add_argument("--no-tolerance", action="store_true", help="skip the plain vs learned run at residual <= tol")
     ap.add_argument("--tolerance-worlds", type=int, default=4096)
+    ap.add_argument("--mlp-forward", type=int, default=0, metavar="ROWS",
+                    help="only run the warm-start network's inference on ROWS synthetic rows (MFMA rate)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run plain copies of MIB MiB (counter scaling), then exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, 5)
+        return
+    if args.mlp_forward:
+        mlp_forward_rate(args.mlp_forward, 5)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build is single-GPU (every box has one MI355X); "
