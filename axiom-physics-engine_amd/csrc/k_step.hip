@@ -155,7 +155,7 @@ __device__ __forceinline__ uint32_t cell_bit(uint64_t h, uint32_t bmask) { retur
 
 __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
+    if (!t.mbits && i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
     if (t.mbits) {
         // the words the last step set (several bodies may name one word: all write zero)
         if (i < t.mlist_n) {
@@ -185,7 +185,10 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
     const uint64_t heads = __ballot(head);
     const uint32_t code = (uint32_t)(k & 0x3FFFFFFFull);
     if (t.mbits) {
-        if (s < n) t.mlist[s] = head ? code >> 5 : UNSET;
+        if (s < n) {
+            t.mlist[s] = head ? code >> 5 : UNSET;
+            t.crank[s] = head ? 1u : 0u;
+        }
         // the wave's cells are consecutive in Morton order and share bitmap words: one
         // atomic per distinct word, with the bits of every head in the wave that names it
         uint64_t todo = heads;
@@ -199,6 +202,7 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
             if (lane == lead) atomicOr(&t.mbits[word], b);
             todo &= ~__ballot(mine);
         }
+        return;  // cells are found by rank (k_cell_rank), no hash table
     }
     if (!head) return;
     __hip_atomic_store(reinterpret_cast<uint64_t*>(&t.ent[(uint32_t)mix64(k) & t.mask]), k, __ATOMIC_RELAXED,
@@ -207,6 +211,22 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
         const uint32_t bit = cell_bit(mix64(k), t.bmask);
         atomicOr(&t.bits[bit >> 5], 1u << (bit & 31u));
     }
+}
+// With the Morton bitmap (crank holds the exclusive scan of the cell heads): each cell's
+// first sorted position by rank, and each bitmap word's rank base (its first cell's rank).
+__global__ void __launch_bounds__(256) k_cell_rank(const uint64_t* __restrict__ skey, uint32_t n, CellTable t) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint64_t k = skey[s];
+    const uint64_t kp = s ? skey[s - 1u] : EMPTY_KEY;
+    const uint32_t r = t.crank[s];
+    const bool head = s == 0u || kp != k;
+    if (head) {
+        t.cstart[r] = s;
+        const uint32_t word = (uint32_t)(k & 0x3FFFFFFFull) >> 5;
+        if (s == 0u || ((uint32_t)(kp & 0x3FFFFFFFull) >> 5) != word) t.wrank[word] = r;
+    }
+    if (s == n - 1u) t.cstart[r + (head ? 1u : 0u)] = n;
 }
 __global__ void __launch_bounds__(256) k_cell_place(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
@@ -326,6 +346,27 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
         py[a] = part1by2((uint32_t)(oky[a] ? Y : 0)) << 1;
     }
     const uint32_t pz = part1by2((uint32_t)Z) << 2;
+    if (ct.mbits) {
+        // one large world: the nine occupancy words are read once (they stay in the cache);
+        // an occupied cell's range comes by rank from two arrays in Morton order
+        uint32_t wv[9];
+        uint32_t occ = 0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const uint32_t code = px[q % 3] | py[q / 3] | pz;
+            wv[q] = okx[q % 3] && oky[q / 3] ? ct.mbits[code >> 5] : 0u;
+            occ |= ((wv[q] >> (code & 31u)) & 1u) << q;
+        }
+        while (occ) {
+            const int q = __ffs(occ) - 1;
+            occ &= occ - 1u;
+            const uint32_t code = px[q % 3] | py[q / 3] | pz;
+            const uint32_t k = ct.wrank[code >> 5] + (uint32_t)__popc(wv[q] & ((1u << (code & 31u)) - 1u));
+            const uint32_t s0 = ct.cstart[k];
+            visit_range(make_uint2(s0, ct.cstart[k + 1u] - s0));
+        }
+        return;
+    }
     if constexpr (SPARSE) {
         // few bodies per occupied cell: the occupancy bits (which stay in the cache) are
         // read first, then only the occupied cells are hashed and probed, in order
@@ -2652,12 +2693,18 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
 }
 void launch_cell_clear(const CellTable& t, hipStream_t s) {
-    const uint32_t n = std::max(t.mask + 1u, t.mbits ? t.mlist_n : (t.bmask >> 5) + 1u);
+    const uint32_t n = t.mbits ? std::max(t.mlist_n, 1u) : std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
     hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
 }
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s) {
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, uint32_t* scan_tmp,
+                       hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
+    if (t.mbits) {
+        scan_exclusive(t.crank, t.crank, n, scan_tmp, s, nullptr, nullptr, false, &cnt->sticky);
+        hipLaunchKernelGGL(k_cell_rank, grid_for(n), dim3(256), 0, s, skey, n, t);
+        return;
+    }
     hipLaunchKernelGGL(k_cell_place, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,

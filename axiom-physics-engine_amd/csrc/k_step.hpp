@@ -59,7 +59,8 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // Empties the cell table and its occupancy bits.
 void launch_cell_clear(const CellTable& t, hipStream_t s);
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, hipStream_t s);
+void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, uint32_t* scan_tmp,
+                       hipStream_t s);
 // Pairs per (body, z slice) part, BP_PARTS * n counts in canonical order (stashing
 // the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
 // the scanned offsets.

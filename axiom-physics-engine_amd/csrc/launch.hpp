@@ -94,6 +94,13 @@ struct CellTable {
     uint32_t* mbits;  // Morton bitmap (2^25 words), or null
     uint32_t* mlist;  // per sorted body of the last step: the Morton word it set, or UNSET
     uint32_t mlist_n; // bodies of the last step (entries of mlist to clear)
+    // With the Morton bitmap, the cells are found by rank instead of by hash: the occupied
+    // cells in Morton order are the distinct keys of the sorted bodies, so cell c is number
+    // wrank[c >> 5] + (set bits of its word below c) and its bodies are sorted positions
+    // cstart[k] .. cstart[k + 1] - 1.
+    uint32_t* wrank;  // per bitmap word with a set bit: the number of occupied cells before it
+    uint32_t* cstart; // per occupied cell in Morton order: its first sorted position (+ n at the end)
+    uint32_t* crank;  // per sorted position: 1 at a cell's first body, then its exclusive scan
 };
 constexpr uint32_t MORTON_WORDS = 1u << 25;        // 2^30 cells / 32
 constexpr uint32_t MORTON_MIN_BODIES = 1u << 16;   // one world at least this large uses the Morton bitmap

@@ -228,6 +228,8 @@ void free_bodies(axm_world_s* w) {
     w->mem.release(w->ct.ent);
     w->mem.release(w->ct.bits);
     w->mem.release(w->ct.mlist);
+    w->mem.release(w->ct.cstart);
+    w->mem.release(w->ct.crank);
     w->mem.release(w->pair_cnt);
     w->mem.release(w->pair_off);
     w->mem.release(w->scan_tmp);
@@ -269,6 +271,8 @@ axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     AXM_TRY(w->mem.get(w->ct.bits, TB / 32u));
     w->ct.bmask = TB - 1;
     AXM_TRY(w->mem.get(w->ct.mlist, cap));
+    AXM_TRY(w->mem.get(w->ct.cstart, cap + 1));
+    AXM_TRY(w->mem.get(w->ct.crank, cap));
     AXM_TRY(w->mem.get(w->pair_cnt, (size_t)BP_PARTS * cap));
     AXM_TRY(w->mem.get(w->pair_off, (size_t)BP_PARTS * cap + 1));
     AXM_TRY(w->mem.get(w->scan_tmp, scan_tmp_words(BP_PARTS * cap)));
@@ -551,7 +555,7 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint64_t* skey = alt ? w->key_alt : w->key;
     const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_cell_clear(w->ct, s);
-    launch_cell_table(skey, n, w->ct, w->cnt, s);
+    launch_cell_table(skey, n, w->ct, w->cnt, w->scan_tmp, s);
     launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, w->cnt, s);
     scan_exclusive(w->pair_cnt, w->pair_off, BP_PARTS * n, w->scan_tmp, s, &w->cnt->pairs, nullptr, false,
                    &w->cnt->sticky);
@@ -1181,7 +1185,9 @@ axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, b
     // then per step only the words the previous step set)
     const bool morton = nw <= 1u && n >= MORTON_MIN_BODIES;
     if (morton && !w->ct.mbits) AXM_TRY(w->mem.get(w->ct.mbits, MORTON_WORDS));
+    if (morton && !w->ct.wrank) AXM_TRY(w->mem.get(w->ct.wrank, MORTON_WORDS));
     if (!morton && w->ct.mbits) w->mem.release(w->ct.mbits);
+    if (!morton && w->ct.wrank) w->mem.release(w->ct.wrank);
     w->ct.mlist_n = morton ? n : 0u;
     if (morton) {
         AXM_HIP(hipMemsetAsync(w->ct.mbits, 0, 4ull * MORTON_WORDS, s));
