@@ -2479,11 +2479,15 @@ __global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, Solv
 }
 
 // Previous-step lookup table for the warm start: (a, b) -> slot.
+// Each entry's table position goes to pos[slot], so that the next build of this table
+// clears only the entries it holds (cache_m of them) instead of the whole table.
 __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counters* cnt,
                                                      uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
-                                                     uint32_t mask, uint32_t* __restrict__ np_out) {
+                                                     uint32_t mask, uint32_t* __restrict__ np_out,
+                                                     uint32_t* __restrict__ pos, uint32_t set) {
     if (cnt->sticky != 0u) return;
     const uint32_t M = cnt->m_eff;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt->cache_m[set] = M;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
         np_out[s] = R.hdr[s].z;
         const uint4 h = R.hdr[s];
@@ -2495,6 +2499,7 @@ __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counter
                 atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
             if (prev == EMPTY_KEY || prev == k) {
                 slot[q] = s;
+                pos[s] = q;
                 placed = true;
             }
             q = (q + 1) & mask;
@@ -2507,7 +2512,8 @@ __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counter
 __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict__ okey,
                                                       const uint32_t* __restrict__ oslot, uint32_t on,
                                                       uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
-                                                      uint32_t mask, Counters* __restrict__ cnt) {
+                                                      uint32_t mask, Counters* __restrict__ cnt,
+                                                      uint32_t* __restrict__ pos) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= on) return;
     const uint64_t k = okey[i];
@@ -2519,6 +2525,7 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
             atomicCAS((unsigned long long*)&key[q], (unsigned long long)EMPTY_KEY, (unsigned long long)k);
         if (prev == EMPTY_KEY || prev == k) {
             slot[q] = v;
+            pos[v] = q;
             return;
         }
         q = (q + 1) & mask;
@@ -2866,19 +2873,23 @@ void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R,
                      uint32_t* out_bits, hipStream_t s) {
     hipLaunchKernelGGL(k_residual, grid_for(cap), dim3(256), 0, s, B, sp, R, cnt, cap, out_bits);
 }
-__global__ void k_cache_clear(uint64_t* __restrict__ key, uint32_t n, const Counters* cnt) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && cnt->sticky == 0u) key[i] = EMPTY_KEY;
+// Empties the entries the table holds (the positions its last build recorded).
+__global__ void k_cache_clear(uint64_t* __restrict__ key, const uint32_t* __restrict__ pos, const Counters* cnt,
+                              uint32_t set) {
+    if (cnt->sticky != 0u) return;
+    const uint32_t m = cnt->cache_m[set];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) key[pos[i]] = EMPTY_KEY;
 }
 
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
-                        uint32_t mask, uint32_t* np_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_cache_clear, grid_for(mask + 1), dim3(256), 0, s, key, mask + 1, cnt);
-    hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out);
+                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s) {
+    hipLaunchKernelGGL(k_cache_clear, grid_bounded(cap), dim3(256), 0, s, key, pos, cnt, set);
+    hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out, pos, set);
 }
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
-                         uint32_t mask, Counters* cnt, hipStream_t s) {
-    if (on) hipLaunchKernelGGL(k_cache_rehash, grid_for(on), dim3(256), 0, s, okey, oslot, on, key, slot, mask, cnt);
+                         uint32_t mask, Counters* cnt, uint32_t* pos, hipStream_t s) {
+    if (on)
+        hipLaunchKernelGGL(k_cache_rehash, grid_for(on), dim3(256), 0, s, okey, oslot, on, key, slot, mask, cnt, pos);
 }
 void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps,
                       hipStream_t s) {

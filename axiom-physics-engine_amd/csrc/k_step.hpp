@@ -137,11 +137,14 @@ void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,
                      uint32_t* out_bits, hipStream_t s);
+// Empties the (a, b) -> slot table of row set `set` (the entries its last build made, at
+// the positions pos holds) and fills it with the step's slots.
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
-                        uint32_t mask, uint32_t* np_out, hipStream_t s);
-// Rehashes the `on`-slot table (okey, oslot) into the empty table (key, slot, mask).
+                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s);
+// Rehashes the `on`-slot table (okey, oslot) into the empty table (key, slot, mask), with the
+// new positions in pos.
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
-                         uint32_t mask, Counters* cnt, hipStream_t s);
+                         uint32_t mask, Counters* cnt, uint32_t* pos, hipStream_t s);
 // sweeps == SWEEPS_ON_DEVICE: the count is read from Counters::sweeps_run.
 constexpr uint32_t SWEEPS_ON_DEVICE = 0xFFFFFFFFu;
 void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps, hipStream_t s);

@@ -157,6 +157,7 @@ struct Counters {
     unsigned long long pgs_bytes_total;  // sweep bytes of all completed steps (pgs_bytes x sweeps)
     unsigned long long sweeps_total;     // sweeps of all completed steps
     unsigned long long bundle_point_sweeps_total;  // bundle_point_sweeps of all completed steps
+    uint32_t cache_m[2];  // per row set: entries its (a, b) -> slot table holds (their positions: pc_pos)
     // ---- run totals since the last axm_world_get_totals reset (steps whose integration ran)
     uint32_t acc_steps;
     uint32_t acc_reached;  // steps that stopped at the residual tolerance

@@ -146,6 +146,7 @@ struct axm_world_s {
     uint32_t rcap[2] = {0, 0};
     uint32_t* np_slot[2] = {nullptr, nullptr};
     uint64_t* pc_key[2] = {nullptr, nullptr};
+    uint32_t* pc_pos[2] = {nullptr, nullptr};  // per slot: its entry's position in pc_key (see cache_m)
     uint32_t* pc_slot[2] = {nullptr, nullptr};
     uint32_t pc_mask[2] = {0, 0};
     int cur = 0;
@@ -325,6 +326,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(w->np_slot[set]);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
+    w->mem.release(w->pc_pos[set]);
     AXM_TRY(w->mem.get(R.hdr, cap));
     AXM_TRY(w->mem.get(R.J, (size_t)cap * 4 * JROWS));
     AXM_TRY(w->mem.get(R.lam, (size_t)cap * 4));
@@ -338,8 +340,10 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     AXM_TRY(w->mem.get(w->pc_key[set], T));
     AXM_TRY(w->mem.get(w->pc_slot[set], T));
+    AXM_TRY(w->mem.get(w->pc_pos[set], cap));
     // an empty table: no warm-start hit until a step has filled it
     if (hipMemset(w->pc_key[set], 0xFF, 8ull * T) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
+    if (w->cnt && hipMemset(&w->cnt->cache_m[set], 0, 4) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     if (hipMemset(w->np_slot[set], 0, 4ull * cap) != hipSuccess) return AXM_GPU_OPERATION_FAILED;
     w->pc_mask[set] = T - 1;
     w->rcap[set] = cap;
@@ -372,11 +376,13 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(dep, cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
     uint64_t* tkey = nullptr;
-    uint32_t* tslot = nullptr;
+    uint32_t *tslot = nullptr, *tpos = nullptr;
     AXM_TRY(w->mem.get(tkey, T));
     AXM_TRY(w->mem.get(tslot, T));
+    AXM_TRY(w->mem.get(tpos, cap));
     launch_fill_u64(tkey, EMPTY_KEY, T, w->stream);
-    launch_cache_rehash(w->pc_key[set], w->pc_slot[set], w->pc_mask[set] + 1, tkey, tslot, T - 1, w->cnt, w->stream);
+    launch_cache_rehash(w->pc_key[set], w->pc_slot[set], w->pc_mask[set] + 1, tkey, tslot, T - 1, w->cnt, tpos,
+                        w->stream);
     AXM_HIP(hipMemcpy2DAsync(lam, 16ull * cap, R.lam, 16ull * old, 16ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
     AXM_HIP(hipMemcpy2DAsync(fid, 4ull * cap, R.fid, 4ull * old, 4ull * old, 4, hipMemcpyDeviceToDevice, w->stream));
     AXM_HIP(hipMemcpyAsync(np, w->np_slot[set], 4ull * old, hipMemcpyDeviceToDevice, w->stream));
@@ -394,8 +400,10 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     w->mem.release(R.next);
     w->mem.release(w->pc_key[set]);
     w->mem.release(w->pc_slot[set]);
+    w->mem.release(w->pc_pos[set]);
     w->pc_key[set] = tkey;
     w->pc_slot[set] = tslot;
+    w->pc_pos[set] = tpos;
     w->pc_mask[set] = T - 1;
     R.lam = lam;
     R.fid = fid;
@@ -751,7 +759,8 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_iters = it;
     if (!post) return AXM_SUCCESS;
     w->last_dt = dt;
-    launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], s);
+    launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], w->pc_pos[c],
+                       (uint32_t)c, s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
     launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
     launch_islands(w->B, sp, n, w->cnt, s);
@@ -1181,6 +1190,9 @@ axm_status install_bodies(axm_world_s* w, uint32_t n, uint32_t nw, float cell, b
         }
     }
     AXM_HIP(hipMemsetAsync(w->cnt, 0, sizeof(Counters), s));
+    // the tables' entry counts are zero now: empty the tables in full
+    for (int set = 0; set < 2; ++set)
+        if (w->pc_key[set]) AXM_HIP(hipMemsetAsync(w->pc_key[set], 0xFF, 8ull * (w->pc_mask[set] + 1), s));
     // one large world: occupancy bits at the cells' Morton codes (cleared here in full,
     // then per step only the words the previous step set)
     const bool morton = nw <= 1u && n >= MORTON_MIN_BODIES;
@@ -1939,7 +1951,8 @@ axm_status axm_world_load(axm_world w, const char* path) {
         AXM_HIP(hipMemcpyAsync(R.hdr, hdr.data(), 16ull * mp, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(R.colour, col.data(), 4ull * mp, hipMemcpyHostToDevice, s));
         AXM_HIP(hipMemcpyAsync(&w->cnt->m_eff, &mp, 4, hipMemcpyHostToDevice, s));
-        launch_cache_build(R, w->cnt, w->rcap[pv], w->pc_key[pv], w->pc_slot[pv], w->pc_mask[pv], w->np_slot[pv], s);
+        launch_cache_build(R, w->cnt, w->rcap[pv], w->pc_key[pv], w->pc_slot[pv], w->pc_mask[pv], w->np_slot[pv],
+                           w->pc_pos[pv], (uint32_t)pv, s);
         AXM_HIP(hipStreamSynchronize(s));
         w->prev_valid = true;
     }
