@@ -165,6 +165,23 @@ def fp64_resolve(w, bodies, desc):
             "residual_fp32": float(st.residual), "residual_fp64": float(ost.residual)}
 
 
+def committed_traffic(kernel, cfg, n, args):
+    """HBM bytes per launch of `kernel` from the committed counter runs (FETCH_SIZE and WRITE_SIZE,
+    each scaled by what it reads for a plain copy of known size), when they were taken on this
+    exact workload; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "%s_scene%d_traffic.json" % (kernel, cfg))
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = {"config": cfg, "bodies": n, "steps": args.steps, "warmup": args.warmup, "residual_tol": args.tol,
+            "max_sweeps": args.max_iters, "check_every": max(1, args.tol_every)}
+    if t.get("kernel") != kernel or t.get("workload") != want:
+        return None
+    return {"bytes_per_launch": t["bytes_per_launch"], "source": "profiles/" + os.path.basename(path)}
+
+
 def headline(args):
     cfg = args.config
     bodies, desc = axm.scene(cfg, SEED_BASE + 1000 * cfg, args.bodies, args.worlds)
@@ -244,6 +261,10 @@ def headline(args):
         }
     dominant = max(kern, key=lambda k: kern[k][1])
     roof = roof_of(dominant)
+    traffic = committed_traffic(dominant, cfg, bodies.n, args)
+    if traffic:
+        roof["traffic"] = traffic["bytes_per_launch"]
+        roof["traffic_source"] = traffic["source"]
     # whole step: 304 N + 104 P + 304 C + 12 R C + 240 I C per step (P, C, R of the diagnostic step)
     dyn = int((bodies.inv_mass > 0).sum())
     b_step = (B_BODY * dyn + B_PAIR * diag.pairs + B_CONTACT * diag.contacts + B_ROUND * diag.colour_rounds *
