@@ -38,19 +38,23 @@ struct Man {
 };
 
 AXM_D void man_add(Man& m, f3 p, f3 n, float depth, uint32_t fid) {
-    // unrolled over the four slots with constant indices, so that the manifold stays
-    // in registers (an indexed write would place it in scratch memory)
-    if (m.np < 4) {
+    // every slot is rewritten with per-component selects (constant indices, no
+    // conditional stores), so that the manifold stays in registers: conditional stores
+    // get merged into one indexed store, which places the manifold in scratch memory
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k)
-            if (k == m.np) {
-                m.pt[k].p = p;
-                m.pt[k].n = n;
-                m.pt[k].depth = depth;
-                m.pt[k].fid = fid;
-            }
-        ++m.np;
+    for (uint32_t k = 0; k < 4; ++k) {
+        const bool w = k == m.np;
+        CPt& q = m.pt[k];
+        q.p.x = w ? p.x : q.p.x;
+        q.p.y = w ? p.y : q.p.y;
+        q.p.z = w ? p.z : q.p.z;
+        q.n.x = w ? n.x : q.n.x;
+        q.n.y = w ? n.y : q.n.y;
+        q.n.z = w ? n.z : q.n.z;
+        q.depth = w ? depth : q.depth;
+        q.fid = w ? fid : q.fid;
     }
+    m.np = m.np < 4u ? m.np + 1u : m.np;
 }
 
 AXM_D void sphere_sphere(Man& m, f3 cA, float rA, f3 cB, float rB, float margin, uint32_t fid) {
@@ -610,14 +614,44 @@ AXM_D float box_proj(const Shp& S, f3 L) {
 }
 AXM_D float hcomp(f3 h, int k) { return k == 0 ? h.x : (k == 1 ? h.y : h.z); }
 
+// Up to eight points held in registers: 8-wide vectors, whose run-time element
+// accesses the compiler expands into selects (an array indexed at run time would
+// live in scratch memory).
+typedef float vf8 __attribute__((ext_vector_type(8)));
+typedef int vi8 __attribute__((ext_vector_type(8)));
+struct Pts8 {
+    vf8 x, y, z;
+    AXM_D f3 get(int k) const { return f3{x[k], y[k], z[k]}; }
+    AXM_D void set(int k, f3 v) {
+        x[k] = v.x;
+        y[k] = v.y;
+        z[k] = v.z;
+    }
+};
+
+// A shape chosen at run time, by per-field selects (a reference chosen at run time
+// would keep both shapes in memory).
+AXM_D Shp select_shape(bool first, const Shp& S0, const Shp& S1) {
+    Shp r;
+    r.c = f3{first ? S0.c.x : S1.c.x, first ? S0.c.y : S1.c.y, first ? S0.c.z : S1.c.z};
+    r.R.r0 = f3{first ? S0.R.r0.x : S1.R.r0.x, first ? S0.R.r0.y : S1.R.r0.y, first ? S0.R.r0.z : S1.R.r0.z};
+    r.R.r1 = f3{first ? S0.R.r1.x : S1.R.r1.x, first ? S0.R.r1.y : S1.R.r1.y, first ? S0.R.r1.z : S1.R.r1.z};
+    r.R.r2 = f3{first ? S0.R.r2.x : S1.R.r2.x, first ? S0.R.r2.y : S1.R.r2.y, first ? S0.R.r2.z : S1.R.r2.z};
+    r.type = first ? S0.type : S1.type;
+    r.prm = f3{first ? S0.prm.x : S1.prm.x, first ? S0.prm.y : S1.prm.y, first ? S0.prm.z : S1.prm.z};
+    return r;
+}
+
 AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
     const f3 Tv = B.c - A.c;
-    float sFace[6];
+    vf8 sFace;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const f3 L = A.R.col(i);
         sFace[i] = fabsf(dot(Tv, L)) - (hcomp(A.prm, i) + box_proj(B, L));
         if (sFace[i] > margin) return;
     }
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const f3 L = B.R.col(i);
         sFace[3 + i] = fabsf(dot(Tv, L)) - (box_proj(A, L) + hcomp(B.prm, i));
@@ -626,7 +660,9 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
     float sEdge = 0.0f;
     int eBest = -1;
     f3 LEdge = {0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
             f3 L = cross(A.R.col(i), B.R.col(j));
             const float l2 = dot(L, L);
@@ -642,9 +678,11 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
         }
     }
     int fA = 0;
+#pragma unroll
     for (int i = 1; i < 3; ++i)
         if (sFace[i] > sFace[fA]) fA = i;
     int fB = 3;
+#pragma unroll
     for (int i = 4; i < 6; ++i)
         if (sFace[i] > sFace[fB]) fB = i;
     int face = fA;
@@ -655,6 +693,7 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
         if (dot(Tv, L) < 0.0f) L = -L;
         const int i = eBest / 3, j = eBest % 3;
         f3 pa = A.c, pb = B.c;
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (k != i) pa = pa + A.R.col(k) * (hcomp(A.prm, k) * sgn1(dot(A.R.col(k), L)));
             if (k != j) pb = pb + B.R.col(k) * (hcomp(B.prm, k) * (-sgn1(dot(B.R.col(k), L))));
@@ -667,13 +706,14 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
         return;
     }
     const bool refB = face >= 3;
-    const Shp& Rf = refB ? B : A;
-    const Shp& In = refB ? A : B;
+    const Shp Rf = select_shape(refB, B, A);
+    const Shp In = select_shape(refB, A, B);
     const int ri = refB ? face - 3 : face;
     f3 nRef = Rf.R.col(ri);
     if (dot(In.c - Rf.c, nRef) < 0.0f) nRef = -nRef;
     int ij = 0;
     float best = fabsf(dot(In.R.col(0), nRef));
+#pragma unroll
     for (int k = 1; k < 3; ++k) {
         const float d = fabsf(dot(In.R.col(k), nRef));
         if (d > best) { best = d; ij = k; }
@@ -684,48 +724,62 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
     const int iu = (ij + 1) % 3, iv = (ij + 2) % 3;
     const f3 u = In.R.col(iu) * hcomp(In.prm, iu);
     const f3 v = In.R.col(iv) * hcomp(In.prm, iv);
-    f3 poly[8], tmp[8];
+    // incident face clipped against the four side planes of the reference face
+    // (Sutherland-Hodgman)
+    Pts8 poly, tmp;
     int np = 4;
-    poly[0] = fc + u + v;
-    poly[1] = fc - u + v;
-    poly[2] = fc - u - v;
-    poly[3] = fc + u - v;
+    poly.set(0, fc + u + v);
+    poly.set(1, fc - u + v);
+    poly.set(2, fc - u - v);
+    poly.set(3, fc + u - v);
     const int ru = (ri + 1) % 3, rv = (ri + 2) % 3;
+#pragma unroll
     for (int p = 0; p < 4; ++p) {
         const f3 axp = Rf.R.col(p < 2 ? ru : rv);
         const float hp = hcomp(Rf.prm, p < 2 ? ru : rv);
         const f3 pn = (p & 1) ? -axp : axp;
         const float pd = dot(pn, Rf.c) + hp;
+        // at most eight vertices are kept: a convex face gains at most one per plane, and
+        // more can only come from rounding on a nearly degenerate face
         int nt = 0;
-        for (int k = 0; k < np; ++k) {
-            const f3 a = poly[(k + np - 1) % np], b = poly[k];
+        auto put = [&](f3 q) {
+            if (nt < 8) tmp.set(nt, q);
+            ++nt;
+        };
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= np) break;
+            const f3 a = poly.get(k == 0 ? np - 1 : k - 1), b = poly.get(k);
             const float da = dot(pn, a) - pd, db = dot(pn, b) - pd;
             if (da <= 0.0f) {
                 if (db <= 0.0f) {
-                    tmp[nt++] = b;
+                    put(b);
                 } else {
                     const float t = da / (da - db);
-                    tmp[nt++] = a + (b - a) * t;
+                    put(a + (b - a) * t);
                 }
             } else if (db <= 0.0f) {
                 const float t = da / (da - db);
-                tmp[nt++] = a + (b - a) * t;
-                tmp[nt++] = b;
+                put(a + (b - a) * t);
+                put(b);
             }
         }
-        np = nt;
-        for (int k = 0; k < np; ++k) poly[k] = tmp[k];
+        np = nt < 8 ? nt : 8;
+        poly = tmp;
         if (np == 0) return;
     }
     const float refD = dot(nRef, Rf.c) + hcomp(Rf.prm, ri);
-    f3 cp[8];
-    float cd[8];
-    int cid[8];
+    Pts8 cp;
+    vf8 cd;
+    vi8 cid;
     int nc = 0;
-    for (int k = 0; k < np; ++k) {
-        const float d = refD - dot(nRef, poly[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k >= np) break;
+        const f3 q = poly.get(k);
+        const float d = refD - dot(nRef, q);
         if (d > -margin) {
-            cp[nc] = poly[k];
+            cp.set(nc, q);
             cd[nc] = d;
             cid[nc] = k;
             ++nc;
@@ -737,56 +791,73 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
         pick = (1u << nc) - 1u;
     } else {
         int i0 = 0;
-        for (int k = 1; k < nc; ++k)
-            if (cd[k] > cd[i0]) i0 = k;
+#pragma unroll
+        for (int k = 1; k < 8; ++k)
+            if (k < nc && cd[k] > cd[i0]) i0 = k;
+        const f3 c0 = cp.get(i0);
         int i1 = -1;
         float dm = 0.0f;
-        for (int k = 0; k < nc; ++k) {
-            if (k == i0) continue;
-            const f3 dd = cp[k] - cp[i0];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= nc || k == i0) continue;
+            const f3 dd = cp.get(k) - c0;
             const float d2 = dot(dd, dd);
             if (i1 < 0 || d2 > dm) { dm = d2; i1 = k; }
         }
+        const f3 e01 = cp.get(i1) - c0;
         int i2 = -1, i3 = -1;
         float amax = 0.0f, amin = 0.0f;
-        for (int k = 0; k < nc; ++k) {
-            if (k == i0 || k == i1) continue;
-            const float a = dot(cross(cp[i1] - cp[i0], cp[k] - cp[i0]), nRef);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= nc || k == i0 || k == i1) continue;
+            const float a = dot(cross(e01, cp.get(k) - c0), nRef);
             if (i2 < 0 || a > amax) { amax = a; i2 = k; }
             if (i3 < 0 || a < amin) { amin = a; i3 = k; }
         }
         if (i3 == i2) {
-            for (int k = 0; k < nc; ++k)
-                if (k != i0 && k != i1 && k != i2) { i3 = k; break; }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (k < nc && k != i0 && k != i1 && k != i2) { i3 = k; break; }
         }
         pick = (1u << i0) | (1u << i1) | (1u << i2) | (1u << i3);
     }
     const uint32_t fbase = ((refB ? 1u : 0u) << 12) | ((uint32_t)ri << 8) | ((uint32_t)ij << 4);
-    for (int k = 0; k < nc; ++k) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k >= nc) break;
         if (!((pick >> k) & 1u)) continue;
         if (refB) {
-            man_add(m, cp[k], -nRef, cd[k], fbase | (uint32_t)cid[k]);
+            man_add(m, cp.get(k), -nRef, cd[k], fbase | (uint32_t)cid[k]);
         } else {
-            man_add(m, cp[k] + nRef * cd[k], nRef, cd[k], fbase | (uint32_t)cid[k]);
+            man_add(m, cp.get(k) + nRef * cd[k], nRef, cd[k], fbase | (uint32_t)cid[k]);
         }
     }
 }
 
-// Canonical-order dispatch (type(A) <= type(B)).
+// Canonical-order dispatch (type(A) <= type(B)).  TESTS selects which tests are compiled
+// in: COLLIDE_LIGHT (sphere-sphere, sphere-box, sphere-capsule, capsule-capsule),
+// COLLIDE_HEAVY (box-box face clipping, box-capsule GJK, which need several times the
+// registers and local arrays of the others), or both.
+constexpr int COLLIDE_LIGHT = 1, COLLIDE_HEAVY = 2, COLLIDE_ALL = 3;
+template <int TESTS = COLLIDE_ALL>
 AXM_D int collide_canon(Man& m, const Shp& A, const Shp& B, float margin, bool& need_epa) {
     need_epa = false;
     const uint32_t ta = A.type, tb = B.type;
-    if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) { sphere_sphere(m, A.c, A.prm.x, B.c, B.prm.x, margin, 0u); return GEOM_OK; }
-    if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) { sphere_box(m, A.c, A.prm.x, B, margin); return GEOM_OK; }
-    if (ta == SHAPE_SPHERE && tb == SHAPE_CAPSULE) {
-        f3 p0, p1;
-        capsule_segment(B, p0, p1);
-        sphere_sphere(m, A.c, A.prm.x, closest_on_segment(A.c, p0, p1), B.prm.x, margin, 0u);
-        return GEOM_OK;
+    if constexpr ((TESTS & COLLIDE_LIGHT) != 0) {
+        if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) { sphere_sphere(m, A.c, A.prm.x, B.c, B.prm.x, margin, 0u); return GEOM_OK; }
+        if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) { sphere_box(m, A.c, A.prm.x, B, margin); return GEOM_OK; }
+        if (ta == SHAPE_SPHERE && tb == SHAPE_CAPSULE) {
+            f3 p0, p1;
+            capsule_segment(B, p0, p1);
+            sphere_sphere(m, A.c, A.prm.x, closest_on_segment(A.c, p0, p1), B.prm.x, margin, 0u);
+            return GEOM_OK;
+        }
+        if (ta == SHAPE_CAPSULE && tb == SHAPE_CAPSULE) { capsule_capsule(m, A, B, margin); return GEOM_OK; }
     }
-    if (ta == SHAPE_BOX && tb == SHAPE_BOX) { box_box(m, A, B, margin); return GEOM_OK; }
-    if (ta == SHAPE_BOX && tb == SHAPE_CAPSULE) return box_capsule_gjk(m, A, B, margin, need_epa);
-    if (ta == SHAPE_CAPSULE && tb == SHAPE_CAPSULE) { capsule_capsule(m, A, B, margin); return GEOM_OK; }
+    if constexpr ((TESTS & COLLIDE_HEAVY) != 0) {
+        if (ta == SHAPE_BOX && tb == SHAPE_BOX) { box_box(m, A, B, margin); return GEOM_OK; }
+        if (ta == SHAPE_BOX && tb == SHAPE_CAPSULE) return box_capsule_gjk(m, A, B, margin, need_epa);
+    }
     return GEOM_OK;
 }
 

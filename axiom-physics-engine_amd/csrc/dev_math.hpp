@@ -33,11 +33,16 @@ struct q4 {
     float x, y, z, w;
 };
 
+// Column k (run-time) of the matrix with rows r0, r1, r2, by per-component selects on
+// values: all nine entries are read first, so that the choice never becomes a load at a
+// run-time offset (which would keep the matrix, and whatever holds it, in scratch memory).
+AXM_D f3 m3_col(f3 r0, f3 r1, f3 r2, int k) {
+    const bool k0 = k == 0, k1 = k == 1;
+    return f3{k0 ? r0.x : (k1 ? r0.y : r0.z), k0 ? r1.x : (k1 ? r1.y : r1.z), k0 ? r2.x : (k1 ? r2.y : r2.z)};
+}
 struct m3 {
     f3 r0, r1, r2;
-    AXM_D f3 col(int k) const {
-        return k == 0 ? f3{r0.x, r1.x, r2.x} : (k == 1 ? f3{r0.y, r1.y, r2.y} : f3{r0.z, r1.z, r2.z});
-    }
+    AXM_D f3 col(int k) const { return m3_col(r0, r1, r2, k); }
 };
 
 AXM_D m3 quat_to_mat(q4 q) {

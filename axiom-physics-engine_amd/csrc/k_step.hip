@@ -254,6 +254,21 @@ __global__ void __launch_bounds__(256) k_cell_place(const uint64_t* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- XCD-aware block order
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8),
+// and each XCD has its own L2.  Kernels whose neighbouring items share gathered data
+// and cost about the same (the pair kernels: neighbour cells of bodies in Morton order)
+// take their block index from here instead: each XCD gets one contiguous run of blocks,
+// so the shared lines stay in one L2.  Not for lists sorted by cost (pairs by kind,
+// manifolds by island), where contiguous runs would load some XCDs far more than
+// others.  A bijection of 0..gridDim.x-1, so a kernel that writes by item index is
+// unchanged.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t per = G >> 3, rem = G & 7u, x = b & 7u;
+    return x * per + min(x, rem) + (b >> 3);
+}
+
 // ---------------------------------------------------------------- pairs
 // Visits candidate partners of the body at sorted position t in the canonical
 // order: planes, then the 27 neighbour cells (dz, dy, dx in -1..1), then
@@ -427,7 +442,7 @@ __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, con
                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
                                                     const Counters* __restrict__ ctr) {
     if ((ctr->bp_sparse != 0u) != SPARSE) return;
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
@@ -447,7 +462,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                    Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
     if ((cnt->bp_sparse != 0u) != SPARSE) return;
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
     if (u == 0) {
         const uint32_t P = off[BP_PARTS * n];
         cnt->p_eff = P < cap ? P : cap;
@@ -555,19 +570,32 @@ __global__ void __launch_bounds__(256) k_pair_order(const uint32_t* __restrict__
     }
 }
 
+// The heavy tests (box-box, box-capsule) run in a kernel of their own, so that the
+// other tests keep a small register allocation and no scratch memory.  In the
+// kind-sorted order the heavy pairs are one contiguous range [h0, h1).
+constexpr uint32_t KIND_BOX_BOX = 1u * 4u + 1u, KIND_BOX_CAPSULE = 1u * 4u + 2u;
+static_assert(KIND_BOX_CAPSULE == KIND_BOX_BOX + 1u, "heavy kinds are adjacent");
+template <bool HEAVY>
 __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const uint32_t* __restrict__ pa,
                                                 const uint32_t* __restrict__ pb, uint32_t* __restrict__ p_np,
                                                 float4* __restrict__ p_pt, float4* __restrict__ p_nf,
                                                 uint32_t* __restrict__ epa_list, Counters* __restrict__ cnt,
-                                                const uint32_t* __restrict__ order) {
+                                                const uint32_t* __restrict__ order, const uint32_t* __restrict__ kinds) {
     const uint32_t P = cnt->p_eff;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+    uint32_t h0 = 0;
+    for (uint32_t k = 0; k < KIND_BOX_BOX; ++k) h0 += kinds[k];
+    uint32_t h1 = h0 + kinds[KIND_BOX_BOX] + kinds[KIND_BOX_CAPSULE];
+    h1 = min(h1, P);
+    h0 = min(h0, h1);
+    const uint32_t nh = h1 - h0, N = HEAVY ? nh : P - nh;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
+    const uint32_t t = HEAVY ? h0 + v : (v < h0 ? v : v + nh);
     const uint32_t p = order[t];
     const uint32_t a = pa[p], b = pb[p];
     Man m;
     m.np = 0;
     const Shp A = load_shape(B, a);
-    if (b & PLANE_BIT) {
+    if (!HEAVY && (b & PLANE_BIT)) {
         const float4 pl = sp.planes[b & ~PLANE_BIT];
         collide_plane(m, A, f3{pl.x, pl.y, pl.z}, pl.w, sp.margin);
     } else {
@@ -575,7 +603,9 @@ __global__ void __launch_bounds__(256) k_narrow(Bodies B, StepParams sp, const u
         bool need_epa = false;
         int st;
         const bool swap = A.type > Bs.type;
-        st = swap ? collide_canon(m, Bs, A, sp.margin, need_epa) : collide_canon(m, A, Bs, sp.margin, need_epa);
+        constexpr int TESTS = HEAVY ? COLLIDE_HEAVY : COLLIDE_LIGHT;
+        st = swap ? collide_canon<TESTS>(m, Bs, A, sp.margin, need_epa)
+                  : collide_canon<TESTS>(m, A, Bs, sp.margin, need_epa);
         if (need_epa) {
             const uint32_t slot = atomicAdd(&cnt->epa_count, 1u);
             epa_list[slot] = p;
@@ -2739,8 +2769,10 @@ void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, co
     // 256 workgroups: each adds its kind counts to the 16 shared counters once per pass
     hipLaunchKernelGGL(k_pair_kind, grid_bounded(cap_pairs, 256u), dim3(256), 0, s, B, pa, pb, p_kind, kinds, cnt);
     hipLaunchKernelGGL(k_pair_order, grid_bounded(cap_pairs, 256u), dim3(256), 0, s, p_kind, kinds, order, cnt);
-    hipLaunchKernelGGL(k_narrow, grid_bounded(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf, epa_list,
-                       cnt, order);
+    hipLaunchKernelGGL(k_narrow<false>, grid_bounded(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf,
+                       epa_list, cnt, order, kinds);
+    hipLaunchKernelGGL(k_narrow<true>, grid_bounded(cap_pairs), dim3(256), 0, s, B, sp, pa, pb, p_np, p_pt, p_nf,
+                       epa_list, cnt, order, kinds);
     // the deep pairs are few: a fixed grid strides over the queue
     hipLaunchKernelGGL(k_narrow_epa, dim3(epa_groups), dim3(64), 0, s, B, sp, pa, pb, epa_list, cap_pairs, p_np, p_pt,
                        p_nf, cnt);

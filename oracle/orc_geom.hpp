@@ -682,24 +682,30 @@ template <class T> inline void box_box(Man<T>& m, const Shape<T>& A, const Shape
         const T hp = hcomp(Rf.prm, p < 2 ? ru : rv);
         const V3<T> pn = (p & 1) ? -axp : axp;
         const T pd = dot(pn, Rf.c) + hp;
+        // at most eight vertices are kept: a convex face gains at most one per plane, and
+        // more can only come from rounding on a nearly degenerate face
         int nt = 0;
+        auto put = [&](const V3<T>& q) {
+            if (nt < 8) tmp[nt] = q;
+            ++nt;
+        };
         for (int k = 0; k < np; ++k) {
             const V3<T> a = poly[(k + np - 1) % np], b = poly[k];
             const T da = dot(pn, a) - pd, db = dot(pn, b) - pd;
             if (da <= T(0)) {
                 if (db <= T(0)) {
-                    tmp[nt++] = b;
+                    put(b);
                 } else {
                     const T t = da / (da - db);
-                    tmp[nt++] = a + (b - a) * t;
+                    put(a + (b - a) * t);
                 }
             } else if (db <= T(0)) {
                 const T t = da / (da - db);
-                tmp[nt++] = a + (b - a) * t;
-                tmp[nt++] = b;
+                put(a + (b - a) * t);
+                put(b);
             }
         }
-        np = nt;
+        np = nt < 8 ? nt : 8;
         for (int k = 0; k < np; ++k) poly[k] = tmp[k];
         if (np == 0) return;
     }
