@@ -643,19 +643,23 @@ AXM_D Shp select_shape(bool first, const Shp& S0, const Shp& S1) {
 }
 
 AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
+    // separating axes: the 15 tests are evaluated without a branch between them (a pair
+    // separated along any axis has no contact, whichever axis is found first), so that
+    // their arithmetic overlaps; the best edge axis is chosen in the same order
     const f3 Tv = B.c - A.c;
     vf8 sFace;
+    bool apart = false;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const f3 L = A.R.col(i);
         sFace[i] = fabsf(dot(Tv, L)) - (hcomp(A.prm, i) + box_proj(B, L));
-        if (sFace[i] > margin) return;
+        apart = apart || sFace[i] > margin;
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const f3 L = B.R.col(i);
         sFace[3 + i] = fabsf(dot(Tv, L)) - (box_proj(A, L) + hcomp(B.prm, i));
-        if (sFace[3 + i] > margin) return;
+        apart = apart || sFace[3 + i] > margin;
     }
     float sEdge = 0.0f;
     int eBest = -1;
@@ -666,17 +670,18 @@ AXM_D void box_box(Man& m, const Shp& A, const Shp& B, float margin) {
         for (int j = 0; j < 3; ++j) {
             f3 L = cross(A.R.col(i), B.R.col(j));
             const float l2 = dot(L, L);
-            if (l2 < 1e-6f) continue;
+            const bool use = !(l2 < 1e-6f);
             L = L * (1.0f / sqrtf(l2));
             const float s = fabsf(dot(Tv, L)) - (box_proj(A, L) + box_proj(B, L));
-            if (s > margin) return;
-            if (eBest < 0 || s > sEdge) {
+            apart = apart || (use && s > margin);
+            if (use && (eBest < 0 || s > sEdge)) {
                 sEdge = s;
                 eBest = i * 3 + j;
                 LEdge = L;
             }
         }
     }
+    if (apart) return;
     int fA = 0;
 #pragma unroll
     for (int i = 1; i < 3; ++i)

@@ -297,9 +297,14 @@ __device__ __forceinline__ bool should_collide(uint4 a, uint4 b) {
     return (a.z & b.y) != 0u && (a.y & b.z) != 0u;
 }
 
+// Broadphase modes: cells found by rank in the occupancy bitmap (one large world), or
+// through the hash table with few bodies per occupied cell (occupancy bits first) or
+// many.  Each mode is its own kernel build, with only the registers its path needs.
+constexpr int BP_RANK = 0, BP_SPARSE = 1, BP_DENSE = 2;
+
 // The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
 // in canonical order; the planes come with slice -1.
-template <bool SPARSE, class F>
+template <int MODE, class F>
 __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t,
                                             int dz, F&& f) {
@@ -361,7 +366,7 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
         py[a] = part1by2((uint32_t)(oky[a] ? Y : 0)) << 1;
     }
     const uint32_t pz = part1by2((uint32_t)Z) << 2;
-    if (ct.mbits) {
+    if constexpr (MODE == BP_RANK) {
         // one large world: the nine occupancy words are read once (they stay in the cache);
         // an occupied cell's range comes by rank from two arrays in Morton order
         uint32_t wv[9];
@@ -382,16 +387,15 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
         }
         return;
     }
-    if constexpr (SPARSE) {
+    if constexpr (MODE == BP_SPARSE) {
         // few bodies per occupied cell: the occupancy bits (which stay in the cache) are
         // read first, then only the occupied cells are hashed and probed, in order
         uint32_t occ = 0;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const uint32_t code = px[q % 3] | py[q / 3] | pz;
-            const uint32_t bit = ct.mbits ? code : cell_bit(mix64(wbits | code), ct.bmask);
-            const uint32_t* words = ct.mbits ? ct.mbits : ct.bits;
-            const bool on = okx[q % 3] && oky[q / 3] && ((words[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
+            const uint32_t bit = cell_bit(mix64(wbits | code), ct.bmask);
+            const bool on = okx[q % 3] && oky[q / 3] && ((ct.bits[bit >> 5] >> (bit & 31u)) & 1u) != 0u;
             occ |= on ? 1u << q : 0u;
         }
         while (occ) {
@@ -436,32 +440,32 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
 // first PAIR_STASH partners, so that the emit pass only re-visits the grid for the
 // few parts with more.  Three threads per body: the probes of a body's slices are
 // in flight together.
-template <bool SPARSE>
+template <int MODE>
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
                                                     const Counters* __restrict__ ctr) {
-    if ((ctr->bp_sparse != 0u) != SPARSE) return;
+    if (MODE != BP_RANK && (ctr->bp_sparse != 0u) != (MODE == BP_SPARSE)) return;
     const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
     if (u >= BP_PARTS * n) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
     uint32_t c = 0;
     uint32_t* st = stash + (size_t)u * PAIR_STASH;
-    visit_pairs<SPARSE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
+    visit_pairs<MODE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
         if (c < PAIR_STASH) st[c] = j;
         ++c;
     });
     cnt[u] = c;
 }
 
-template <bool SPARSE>
+template <int MODE>
 __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                    const uint32_t* __restrict__ off, uint32_t cap,
                                                    uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                    Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
-    if ((cnt->bp_sparse != 0u) != SPARSE) return;
+    if (MODE != BP_RANK && (cnt->bp_sparse != 0u) != (MODE == BP_SPARSE)) return;
     const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
     if (u == 0) {
         const uint32_t P = off[BP_PARTS * n];
@@ -482,7 +486,7 @@ __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, cons
         }
         return;
     }
-    visit_pairs<SPARSE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
+    visit_pairs<MODE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
         if (o < cap) {
             pa[o] = i;
             pb[o] = j;
@@ -2749,19 +2753,29 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
                        hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n, ctr);
-    hipLaunchKernelGGL(k_pair_count<true>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
+    if (ct.mbits) {
+        hipLaunchKernelGGL(k_pair_count<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n,
+                           cnt, stash, ctr);
+        return;
+    }
+    hipLaunchKernelGGL(k_pair_count<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
                        stash, ctr);
-    hipLaunchKernelGGL(k_pair_count<false>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
+    hipLaunchKernelGGL(k_pair_count<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
                        stash, ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
                       Counters* cnt, const uint32_t* stash, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_pair_emit<true>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
-                       pa, pb, cnt, stash);
-    hipLaunchKernelGGL(k_pair_emit<false>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off, cap,
-                       pa, pb, cnt, stash);
+    if (ct.mbits) {
+        hipLaunchKernelGGL(k_pair_emit<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
+                           cap, pa, pb, cnt, stash);
+        return;
+    }
+    hipLaunchKernelGGL(k_pair_emit<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
+                       cap, pa, pb, cnt, stash);
+    hipLaunchKernelGGL(k_pair_emit<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
+                       cap, pa, pb, cnt, stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,
