@@ -84,7 +84,6 @@ AXM_D s3 world_inv_inertia(const m3& R, f3 d) {
     s.yz = R.r1.x * d.x * R.r2.x + R.r1.y * d.y * R.r2.y + R.r1.z * d.z * R.r2.z;
     return s;
 }
-AXM_D s3 load_s3(float4 a, float4 b) { return s3{a.x, a.y, a.z, a.w, b.x, b.y}; }
 
 AXM_D void tangents(f3 n, f3& t1, f3& t2) {
     f3 t;

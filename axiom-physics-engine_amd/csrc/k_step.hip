@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
             else asleep = true;
         }
     }
-    const float4 x4 = B.x[i], q4v = B.q[i], v4 = B.vel[i], w4 = B.angv[i], ii = B.ii[i], sh = B.shape[i];
+    const float4 x4 = B.x[i], q4v = B.q[i], v4 = B.vel[i], w4 = B.angv[i], sh = B.shape[i];
     const uint32_t type = fbits(sh.w);
     // a sleeping body acts as a static one for the whole step
     const bool dyn = v4.w > 0.0f && !asleep;
@@ -80,9 +80,6 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
     B.sv[i] = f4(v, dyn ? v4.w : 0.0f);
     B.sw[i] = f4(w, 0.0f);
     const m3 R = quat_to_mat(q4v);
-    const s3 I = dyn ? world_inv_inertia(R, xyz(ii)) : s3{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    B.I0[i] = make_float4(I.xx, I.yy, I.zz, I.xy);
-    B.I1[i] = make_float4(I.xz, I.yz, 0.0f, 0.0f);
     const float mg = sp.margin;
     f3 e;
     if (type == SHAPE_SPHERE) {
@@ -1064,13 +1061,16 @@ struct SB {
     bool dyn;
 };
 
+// A body's solver state for the row preparation.  The world inverse inertia is formed
+// here, for the bodies of manifolds only (sv.w > 0 exactly when the body is dynamic and
+// awake this step), rather than stored for every body at the step start.
 __device__ __forceinline__ void load_sb(const Bodies& B, uint32_t i, SB& s, bool want_x) {
     const float4 v = B.sv[i], w = B.sw[i];
     s.v = xyz(v);
     s.im = v.w;
     s.w = xyz(w);
-    s.I = load_s3(B.I0[i], B.I1[i]);
     s.dyn = v.w > 0.0f;
+    s.I = s.dyn ? world_inv_inertia(quat_to_mat(B.q[i]), xyz(B.ii[i])) : s3{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (want_x) s.x = xyz(B.x[i]);
 }
 __device__ __forceinline__ void zero_sb(SB& s) {

@@ -6,7 +6,7 @@
 //   bodies   x      float4 (position, -)          q     float4 (x,y,z,w)
 //            vel    float4 (v, inverse mass)      angv  float4 (w, -)
 //            ii     float4 (body inverse inertia) shape float4 (params, type bits)
-//   per step sv/sw  float4 solver velocities,     I0/I1 float4 world inverse inertia
+//   per step sv/sw  float4 solver velocities
 //            amin/amax float4 AABB, key u64 + val u32 (cell key, body) and ping-pong copies
 //   pairs    pa/pb u32, per-pair contact staging (4 slots x 2 float4)
 //   manifolds (pair order)  m_ab uint2, m_np u32, c_pt/c_nf float4 [m*4+k]
@@ -51,8 +51,6 @@ struct Bodies {
     uint32_t* world_first;
     float4* sv;
     float4* sw;
-    float4* I0;
-    float4* I1;
     float4* amin;
     float4* amax;
     float4* smin;  // AABBs in sorted (cell) order, w = body index bits

@@ -207,7 +207,7 @@ namespace {
 // ------------------------------------------------------------------ buffers
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
-    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
+    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.amin, &B.amax,
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
     w->mem.release(w->pair_stash);
@@ -244,7 +244,7 @@ void free_bodies(axm_world_s* w) {
 axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     free_bodies(w);
     Bodies& B = w->B;
-    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.I0, &B.I1, &B.amin, &B.amax,
+    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.amin, &B.amax,
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * BP_PARTS * PAIR_STASH));
