@@ -325,9 +325,203 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const K* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- cooperative sort
+// For a count read on the device (usually far below the capacity the grid would be
+// sized for), one launch of at most CO_GROUPS workgroups, all resident, runs every
+// pass with one grid barrier per pass instead of three launches per pass.  Per-tile
+// digit counts live in three rotating buffers (tile-major: [tile * 256 + digit]): pass
+// p reads its counts from H[p % 3]; its scatter adds each key's next digit at the key's
+// destination tile into H[(p + 1) % 3]; and it clears H[(p + 2) % 3], which pass p - 1
+// read and pass p + 1 adds into.  A prologue writes pass 0's counts and clears the
+// buffer pass 0 adds into.  Each workgroup finds its tiles' offsets from the counts
+// themselves (digit totals over all tiles, then the tiles below its own).
+constexpr uint32_t CO_GROUPS = 64, CO_HDR = 4;
+
+size_t coop_tmp_words(uint32_t n) {
+    const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+    return CO_HDR + 3ull * 256u * (nb ? nb : 1u);
+}
+
+// Barrier over the grid (every workgroup resident): each wave's stores complete, then one
+// thread per workgroup arrives (agent-scope release) and waits for the last arrival
+// (agent-scope acquire); a bounded wait ORs STICKY_TIMEOUT into *err instead of hanging.
+// bar[0] counts arrivals and returns to 0; bar[1] is the generation.
+__device__ __forceinline__ void co_sync(uint32_t* bar, uint32_t* err) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t g = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (wall_clock64() - t0 > 200000000ull) {  // 2 s: report, do not hang
+                    if (err) atomicOr(err, STICKY_TIMEOUT);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    __syncthreads();
+}
+
+// Sum of counts[t * 256 + d] over tiles t0 <= t < t1, eight loads in flight at a time.
+__device__ __forceinline__ uint32_t sum_tiles(const uint32_t* __restrict__ h, uint32_t t0, uint32_t t1, uint32_t d) {
+    uint32_t s = 0;
+    uint32_t t = t0;
+    for (; t + 8u <= t1; t += 8u) {
+        uint32_t a[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) a[u] = h[(size_t)(t + u) * 256u + d];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; ++u) s += a[u];
+    }
+    uint32_t a[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8u; ++u) a[u] = t + u < t1 ? h[(size_t)(t + u) * 256u + d] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < 8u; ++u) s += a[u];
+    return s;
+}
+
+template <class K>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_coop(K* __restrict__ k0, uint32_t* __restrict__ v0,
+                                                        K* __restrict__ k1, uint32_t* __restrict__ v1, uint32_t cap,
+                                                        const uint32_t* __restrict__ n_dev, int passes,
+                                                        uint32_t* __restrict__ tmp, uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_off[256], s_cnt[256];
+    __shared__ uint32_t s_wc[RS_THREADS / 64][256];
+    const uint32_t n = eff_count(cap, n_dev);
+    const uint32_t nt = (n + RS_TILE - 1) / RS_TILE;
+    const uint32_t nbc = (cap + RS_TILE - 1) / RS_TILE;
+    uint32_t* bar = tmp;
+    uint32_t* const H0 = tmp + CO_HDR;
+    const size_t hsz = (size_t)256 * nbc;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    // prologue: pass 0's counts per tile, and the buffer pass 0 adds into cleared
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        s_cnt[tid] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < RS_ITEMS; ++it) {
+            const uint32_t i = t * RS_TILE + it * RS_THREADS + tid;
+            if (i < n) atomicAdd(&s_cnt[digit_of(k0[i], 0)], 1u);
+        }
+        __syncthreads();
+        H0[(size_t)t * 256u + tid] = s_cnt[tid];
+        H0[hsz + (size_t)t * 256u + tid] = 0u;
+        __syncthreads();
+    }
+    co_sync(bar, err);
+    K* ki = k0;
+    K* ko = k1;
+    uint32_t* vi = v0;
+    uint32_t* vo = v1;
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t* Hp = H0 + hsz * (uint32_t)(p % 3);
+        uint32_t* Hn = H0 + hsz * (uint32_t)((p + 1) % 3);
+        uint32_t* Hz = H0 + hsz * (uint32_t)((p + 2) % 3);
+        const bool more = p + 1 < passes;
+        const int shift = 8 * p;
+        // digit tid's place among all keys: the counts of the lower digits over all tiles
+        // the first tile's keys and values are loaded before the counts, so that the two
+        // rounds of loads overlap
+        K kk[RS_ITEMS];
+        uint32_t vv[RS_ITEMS];
+        auto load_tile = [&](uint32_t t) {
+#pragma unroll
+            for (int it = 0; it < RS_ITEMS; ++it) {
+                const uint32_t i = t * RS_TILE + it * RS_THREADS + tid;
+                kk[it] = i < n ? ki[i] : (K)0;
+                vv[it] = i < n ? vi[i] : 0u;
+            }
+        };
+        if (blockIdx.x < nt) load_tile(blockIdx.x);
+        // up to 32 tiles: all of digit tid's counts in registers from one round of loads,
+        // which give the total and the count below each of this workgroup's tiles
+        const bool few = nt <= 32u;
+        uint32_t c32[32];
+#pragma unroll
+        for (uint32_t u = 0; u < 32u; ++u) c32[u] = few && u < nt ? Hp[(size_t)u * 256u + tid] : 0u;
+        uint32_t tot = 0;
+        if (few) {
+#pragma unroll
+            for (uint32_t u = 0; u < 32u; ++u) tot += c32[u];
+        } else {
+            tot = sum_tiles(Hp, 0u, nt, tid);
+        }
+        uint32_t all;
+        const uint32_t base = block_excl_scan(tot, all);
+        uint32_t below = 0;  // count of digit tid in the tiles before the current one
+        uint32_t tb = 0;
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+            if (t != blockIdx.x) load_tile(t);
+            if (few) {
+                below = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < 32u; ++u) below += u < t ? c32[u] : 0u;
+            } else {
+                below += sum_tiles(Hp, tb, t, tid);
+                tb = t;
+            }
+            s_off[tid] = base + below;
+            if (more) Hz[(size_t)t * 256u + tid] = 0u;
+            // stable scatter, item by item in index order (as k_rs_scatter), adding each
+            // key's next digit at its destination tile
+#pragma unroll
+            for (int it = 0; it < RS_ITEMS; ++it) {
+                const uint32_t i = t * RS_TILE + it * RS_THREADS + tid;
+                const bool valid = i < n;
+                const K k = kk[it];
+                const uint32_t v = vv[it];
+                const uint32_t d = valid ? digit_of(k, shift) : 256u;
+#pragma unroll
+                for (int w = 0; w < RS_THREADS / 64; ++w) s_wc[w][tid] = 0;
+                uint64_t peers = __ballot(1);
+#pragma unroll
+                for (int b = 0; b < 9; ++b) {
+                    const uint64_t bal = __ballot((d >> b) & 1u);
+                    peers &= ((d >> b) & 1u) ? bal : ~bal;
+                }
+                const uint32_t rank = (uint32_t)__popcll(peers & lt);
+                __syncthreads();
+                if (valid && rank == 0) s_wc[wv][d] = (uint32_t)__popcll(peers);
+                __syncthreads();
+                uint32_t pos = 0;
+                if (valid) {
+                    pos = s_off[d] + rank;
+                    for (uint32_t w = 0; w < wv; ++w) pos += s_wc[w][d];
+                }
+                __syncthreads();
+                uint32_t add = 0;
+#pragma unroll
+                for (int w = 0; w < RS_THREADS / 64; ++w) add += s_wc[w][tid];
+                s_off[tid] += add;
+                if (valid && pos < n) {
+                    ko[pos] = k;
+                    vo[pos] = v;
+                    if (more) atomicAdd(&Hn[(size_t)(pos / RS_TILE) * 256u + digit_of(k, shift + 8)], 1u);
+                }
+            }
+            __syncthreads();
+        }
+        if (more) co_sync(bar, err);
+        K* tk = ki; ki = ko; ko = tk;
+        uint32_t* tv = vi; vi = vo; vo = tv;
+    }
+}
+
 size_t radix_tmp_words(uint32_t n) {
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
-    return (size_t)256 * nb + 2 + scan_tmp_words(256u * nb);
+    const size_t split = (size_t)256 * nb + 2 + scan_tmp_words(256u * nb);
+    const size_t coop = coop_tmp_words(n);
+    return split > coop ? split : coop;
 }
 
 template <class K>
@@ -335,6 +529,14 @@ static int radix_sort_impl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
                            uint32_t* tmp, hipStream_t s, const uint32_t* n_dev, uint32_t* err) {
     if (n == 0) return 0;
     const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+    if (n_dev) {
+        // the count is read on the device: one cooperative launch
+        const int passes = (end_bit + 7) / 8;
+        const uint32_t groups = nb < CO_GROUPS ? nb : CO_GROUPS;
+        hipLaunchKernelGGL(k_rs_coop<K>, dim3(groups), dim3(RS_THREADS), 0, s, keys, vals, keys_alt, vals_alt, n, n_dev,
+                           passes, tmp, err);
+        return passes & 1;
+    }
     uint32_t* hist = tmp;
     uint32_t* offs = tmp;  // scanned in place
     // the histogram length in use (256 x tiles with keys); the word before it takes the
