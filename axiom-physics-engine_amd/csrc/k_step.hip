@@ -1729,6 +1729,28 @@ __device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool
     return e;
 }
 
+// The same with the point's rows in registers (as solve_regs).
+__device__ __forceinline__ float point_residual_regs(const float4 (&Jr)[JROWS], bool plane, float keep, const SB& A,
+                                                     const SB& Bd, const float4 L) {
+    float mN, tg, mT1, maxf, mT2, x3;
+    const Dir Dn = dir_regs(Jr, 0, plane, mN, tg);
+    const Dir D1 = dir_regs(Jr, 1, plane, mT1, maxf);
+    const Dir D2 = dir_regs(Jr, 2, plane, mT2, x3);
+    const float kN = Jr[1].w, kT1 = Jr[6].w, kT2 = Jr[11].w;
+    float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
+    nl = nl > 0.0f ? nl : 0.0f;
+    float e = fabsf(nl - L.x) * kN;
+    float q1 = fmaf(-mT1, jvel(D1, A, Bd), keep * L.y);
+    q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
+    const float e1 = fabsf(q1 - L.y) * kT1;
+    float q2 = fmaf(-mT2, jvel(D2, A, Bd), keep * L.z);
+    q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
+    const float e2 = fabsf(q2 - L.z) * kT2;
+    if (e1 > e) e = e1;
+    if (e2 > e) e = e2;
+    return e;
+}
+
 // What a visit needs of its slot besides the rows: the slot, body indices and point count,
 // dependency words, the next visits of its bodies, inverse masses and the
 // velocity of a static partner (constant during the sweep).
@@ -2261,16 +2283,16 @@ __device__ __forceinline__ void local_done(Counters* cnt, uint32_t sweeps, float
 // at all), with a barrier between phases; then each island's residual is the maximum over
 // its lanes, and an island that is done stops while the others go on.
 //
-// Everything a sweep touches lives in LDS for the whole solve: each lane's Jacobian rows and
-// impulses (row r of point k at (k * BUNDLE_ROWS + r) * 64 + lane, row JROWS the impulses),
-// and each body's solver velocity in one home slot, the first (lane, side) of its island that
-// names it (dynamic bodies appear in one island only; a static partner keeps its own slot).
-// The visits are the arithmetic of solve_slot on those copies; velocities and impulses go
-// back to memory once, at the end.
+// Everything a sweep touches stays on chip for the whole solve: each lane's Jacobian rows
+// and impulses in its registers (a lane visits the same slot in every sweep; with one wave
+// per SIMD the rows fit the register file, accumulation registers included), and each
+// body's solver velocity in LDS, in one home slot: the first (lane, side) of its island
+// that names it (dynamic bodies appear in one island only; a static partner keeps its own
+// slot).  The visits are the arithmetic of solve_slot on those copies; velocities and
+// impulses go back to memory once, at the end.
 constexpr uint32_t BUNDLE_SLOTS = 56;
 static_assert(BUNDLE_SLOTS + ISLAND_LANE_MAX - 1u <= 64u, "a bundle fits one wave");
-constexpr uint32_t BUNDLE_ROWS = JROWS + 1u;  // per point: the Jacobian rows and the impulses
-constexpr uint32_t BUNDLE_BODY0 = 4u * BUNDLE_ROWS * 64u;  // first body slot (float4 units)
+constexpr uint32_t BUNDLE_BODY0 = 0u;  // first body slot (float4 units)
 constexpr uint32_t BUNDLE_LDS_BYTES = (BUNDLE_BODY0 + 4u * 64u) * 16u;
 __device__ __forceinline__ void home_load(const float4* lds, uint32_t h, SB& X) {
     const float4 v = lds[BUNDLE_BODY0 + h], w = lds[BUNDLE_BODY0 + 128u + h];
@@ -2297,7 +2319,6 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
     const uint32_t nwin = M > M0 ? (M - M0 + BUNDLE_SLOTS - 1u) / BUNDLE_SLOTS : 0u;
     const bool tol_mode = tol > 0.0f;
     const float keep = sp.keep;
-    float4* rows = bl + lane;
     for (uint32_t win = blockIdx.x; win < nwin; win += gridDim.x) {
         const uint32_t w0 = M0 + win * BUNDLE_SLOTS, s = w0 + lane;
         // this lane's slot belongs to the bundle when its island is a one-lane island whose
@@ -2323,8 +2344,13 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
         isl_res[1][lane] = 0u;
         isl_pts[lane] = 0u;
         if (lane == 0u) kmax_w = 0u;
+        // the slot's rows and impulses in registers
+        float4 Jr[4][JROWS];
+        float4 L[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) L[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (mine) {
-            // own copies of both bodies' velocities, the rows and the impulses
+            // own copies of both bodies' velocities
             const float4 va = B.sv[h.x], wa = B.sw[h.x];
             bl[BUNDLE_BODY0 + lane] = va;
             bl[BUNDLE_BODY0 + 128u + lane] = make_float4(wa.x, wa.y, wa.z, 0.0f);
@@ -2333,12 +2359,10 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                 bl[BUNDLE_BODY0 + 64u + lane] = vb;
                 bl[BUNDLE_BODY0 + 192u + lane] = make_float4(wb.x, wb.y, wb.z, 0.0f);
             }
-            for (uint32_t k = 0; k < np; ++k) {
-                const float4* src = R.J + (size_t)k * JROWS * cap + s;
+            load_rows(R, s, cap, np, plane, Jr);
 #pragma unroll
-                for (uint32_t r = 0; r < JROWS; ++r) rows[(k * BUNDLE_ROWS + r) * 64u] = src[(size_t)r * cap];
-                rows[(k * BUNDLE_ROWS + JROWS) * 64u] = R.lam[(size_t)k * cap + s];
-            }
+            for (uint32_t k = 0; k < 4; ++k)
+                if (k < np) L[k] = R.lam[(size_t)k * cap + s];
         }
         __syncthreads();
         // home slot of each side: the first (lane, side) of the island naming the body; the
@@ -2372,14 +2396,9 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                     SB A, Bd;
                     home_load(bl, home_a, A);
                     if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
-                    float4 L[4];
+                    solve_regs(Jr, np, plane, keep, it == 0u, A, Bd, L);
 #pragma unroll
-                    for (uint32_t k = 0; k < 4; ++k)
-                        if (k < np) L[k] = rows[(k * BUNDLE_ROWS + JROWS) * 64u];
-                    solve_rows(rows, 64u, (size_t)BUNDLE_ROWS * 64u, np, plane, keep, it == 0u, A, Bd, L);
-#pragma unroll
-                    for (uint32_t k = 0; k < 4; ++k)
-                        if (k < np) rows[(k * BUNDLE_ROWS + JROWS) * 64u] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
+                    for (uint32_t k = 0; k < 4; ++k) L[k].w = 0.0f;
                     if (A.dyn) home_store(bl, home_a, A);
                     if (Bd.dyn) home_store(bl, home_b, Bd);
                 }
@@ -2393,9 +2412,10 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                     home_load(bl, home_a, A);
                     if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
                     float r = 0.0f;
-                    for (uint32_t k = 0; k < np; ++k) {
-                        const float e = point_residual(rows + (size_t)k * BUNDLE_ROWS * 64u, 64u, plane, keep, A, Bd,
-                                                       rows[(k * BUNDLE_ROWS + JROWS) * 64u]);
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) {
+                        if (k >= np) break;
+                        const float e = point_residual_regs(Jr[k], plane, keep, A, Bd, L[k]);
                         if (e > r) r = e;
                     }
                     atomicMax(&cur[li], __float_as_uint(r));
@@ -2417,7 +2437,9 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
         __syncthreads();
         if (mine) {
             // impulses and the velocities of the bodies whose home this lane holds back to memory
-            for (uint32_t k = 0; k < np; ++k) R.lam[(size_t)k * cap + s] = rows[(k * BUNDLE_ROWS + JROWS) * 64u];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+                if (k < np) R.lam[(size_t)k * cap + s] = L[k];
             SB A;
             home_load(bl, lane, A);
             if (home_a == lane && A.dyn) {
