@@ -94,8 +94,10 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
         e.z = fabsf(R.r2.y) * sh.y + sh.x + mg;
     }
     const f3 c = xyz(x4);
-    B.amin[i] = f4(c - e, 0.0f);
-    B.amax[i] = f4(c + e, 0.0f);
+    // the AABB as one 32-byte record, with whether the body is dynamic this step, so that
+    // the gather in cell order reads one place per body
+    B.aabb[2u * i] = f4(c - e, 0.0f);
+    B.aabb[2u * i + 1u] = f4(c + e, dyn ? 1.0f : 0.0f);
     auto cc = [&](float p, float o) -> uint32_t {
         float f = (p - o) * sp.inv_cell;
         f = f > 0.0f ? f : 0.0f;
@@ -281,9 +283,9 @@ __global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* _
     if (t == 0) cnt->bp_sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n ? 1u : 0u;
     if (t >= n) return;
     const uint32_t i = sval[t];
-    const float4 mn = B.amin[i], mx = B.amax[i];
+    const float4 mn = B.aabb[2u * i], mx = B.aabb[2u * i + 1u];
     B.smin[t] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
-    B.smax[t] = make_float4(mx.x, mx.y, mx.z, B.sv[i].w > 0.0f ? 1.0f : 0.0f);
+    B.smax[t] = mx;
 }
 
 // Collision filter of two bodies (flags, category, mask, group): a shared nonzero group
@@ -1225,7 +1227,15 @@ __device__ __forceinline__ void prep_slot(const Bodies& B, const StepParams& sp,
             const float nb = hit ? Lp.x : est;
             J[8 * (size_t)cap] = f4(jia[1], mu * (nb > 0.0f ? nb : 0.0f));
         }
-        float4 L = warm_in ? warm_in[(size_t)m * 4 + k] : Lp;
+        // the given warm start, if any, else the cached one (chosen by value: a choice of
+        // address would place Lp in scratch memory)
+        float4 L = Lp;
+        if (warm_in) {
+            const float4 wv = warm_in[(size_t)m * 4 + k];
+            L.x = wv.x;
+            L.y = wv.y;
+            L.z = wv.z;
+        }
         L.w = 0.0f;
         R.lam[o] = L;
         R.lam0[o] = L;

@@ -7,7 +7,7 @@
 //            vel    float4 (v, inverse mass)      angv  float4 (w, -)
 //            ii     float4 (body inverse inertia) shape float4 (params, type bits)
 //   per step sv/sw  float4 solver velocities
-//            amin/amax float4 AABB, key u64 + val u32 (cell key, body) and ping-pong copies
+//            aabb 2 x float4 (min; max, dynamic), key u64 + val u32 (cell key, body) and ping-pong copies
 //   pairs    pa/pb u32, per-pair contact staging (4 slots x 2 float4)
 //   manifolds (pair order)  m_ab uint2, m_np u32, c_pt/c_nf float4 [m*4+k]
 //   solver rows (solve order, slot-major [k*cap + s]) R0..R3, lambda float4,
@@ -51,8 +51,7 @@ struct Bodies {
     uint32_t* world_first;
     float4* sv;
     float4* sw;
-    float4* amin;
-    float4* amax;
+    float4* aabb;  // per body [2i] (min, 0) and [2i+1] (max, 1 if dynamic this step)
     float4* smin;  // AABBs in sorted (cell) order, w = body index bits
     float4* smax;  // w = 1 for a dynamic body
     uint32_t* vslot;  // persistent sweep: per body and colour c < 64, the slot visiting it (slot << 1 | side)

@@ -207,9 +207,10 @@ namespace {
 // ------------------------------------------------------------------ buffers
 void free_bodies(axm_world_s* w) {
     Bodies& B = w->B;
-    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.amin, &B.amax,
+    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw,
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) w->mem.release(*p);
+    w->mem.release(B.aabb);
     w->mem.release(w->pair_stash);
     w->mem.release(B.vslot);
     w->mem.release(B.mat);
@@ -244,9 +245,10 @@ void free_bodies(axm_world_s* w) {
 axm_status alloc_bodies(axm_world_s* w, uint32_t cap) {
     free_bodies(w);
     Bodies& B = w->B;
-    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw, &B.amin, &B.amax,
+    float4** f4s[] = {&B.x, &B.q, &B.vel, &B.angv, &B.ii, &B.shape, &B.sv, &B.sw,
                       &B.smin, &B.smax, &w->pre_v, &w->pre_w};
     for (float4** p : f4s) AXM_TRY(w->mem.get(*p, cap));
+    AXM_TRY(w->mem.get(B.aabb, (size_t)cap * 2));
     AXM_TRY(w->mem.get(w->pair_stash, (size_t)cap * BP_PARTS * PAIR_STASH));
     AXM_TRY(w->mem.get(B.vslot, (size_t)cap * 64));
     AXM_TRY(w->mem.get(B.mat, cap));
