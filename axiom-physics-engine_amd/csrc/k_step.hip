@@ -435,47 +435,84 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     }
 }
 
+// Exclusive scan over a 256-thread workgroup (four waves); the workgroup's total in `total`.
+__device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= (uint32_t)o) inc += t;
+    }
+    if (lane == 63u) s_w[wv] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4u; ++w) {
+        const uint32_t x = s_w[w];
+        if (w < wv) base += x;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return base + inc - v;
+}
+
 // Counts the pairs of each (body, z slice) part u = 3 t + dz + 1 and keeps the
 // first PAIR_STASH partners, so that the emit pass only re-visits the grid for the
 // few parts with more.  Three threads per body: the probes of a body's slices are
 // in flight together.
+// The workgroup's total goes to bsum[block] (block in the XCD-aware order): the emit pass
+// forms each part's offset from the scanned totals and a scan within its workgroup.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
-                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ stash,
-                                                    const Counters* __restrict__ ctr) {
+                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
+                                                    uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
     if (MODE != BP_RANK && (ctr->bp_sparse != 0u) != (MODE == BP_SPARSE)) return;
-    const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
-    if (u >= BP_PARTS * n) return;
-    const uint32_t t = u / BP_PARTS;
-    const int dz = (int)(u - BP_PARTS * t) - 1;
+    const uint32_t lb = xcd_block();
+    const uint32_t u = lb * blockDim.x + threadIdx.x;
     uint32_t c = 0;
-    uint32_t* st = stash + (size_t)u * PAIR_STASH;
-    visit_pairs<MODE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
-        if (c < PAIR_STASH) st[c] = j;
-        ++c;
-    });
-    cnt[u] = c;
+    if (u < BP_PARTS * n) {
+        const uint32_t t = u / BP_PARTS;
+        const int dz = (int)(u - BP_PARTS * t) - 1;
+        uint32_t* st = stash + (size_t)u * PAIR_STASH;
+        visit_pairs<MODE>(B, sp, skey, sval, ct, t, dz, [&](uint32_t j) {
+            if (c < PAIR_STASH) st[c] = j;
+            ++c;
+        });
+        cnt[u] = c;
+    }
+    uint32_t tot;
+    (void)wg_excl_scan(c, tot);
+    if (threadIdx.x == 0u) bsum[lb] = tot;
 }
 
+// Part u's pairs start at boff[block] (the scanned workgroup totals; boff[blocks] is the
+// pair count) plus the counts of the parts before it in its workgroup.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
-                                                   const uint32_t* __restrict__ off, uint32_t cap,
-                                                   uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
+                                                   const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ boff,
+                                                   uint32_t cap, uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
                                                    Counters* __restrict__ cnt, const uint32_t* __restrict__ stash) {
     if (MODE != BP_RANK && (cnt->bp_sparse != 0u) != (MODE == BP_SPARSE)) return;
-    const uint32_t u = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t lb = xcd_block();
+    const uint32_t u = lb * blockDim.x + threadIdx.x;
+    const uint32_t parts = BP_PARTS * n;
     if (u == 0) {
-        const uint32_t P = off[BP_PARTS * n];
+        const uint32_t P = boff[(parts + 255u) / 256u];
         cnt->p_eff = P < cap ? P : cap;
         if (P > cap) atomicOr(&cnt->sticky, STICKY_PAIRS);
     }
-    if (u >= BP_PARTS * n) return;
+    const uint32_t c = u < parts ? pcnt[u] : 0u;
+    uint32_t tot;
+    const uint32_t ex = wg_excl_scan(c, tot);
+    if (u >= parts) return;
     const uint32_t t = u / BP_PARTS;
     const int dz = (int)(u - BP_PARTS * t) - 1;
-    uint32_t o = off[u];
-    const uint32_t c = off[u + 1] - o;
+    uint32_t o = boff[lb] + ex;
     const uint32_t i = sval[t];
     if (c <= PAIR_STASH) {
         const uint32_t* st = stash + (size_t)u * PAIR_STASH;
@@ -2781,33 +2818,33 @@ void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Cou
     hipLaunchKernelGGL(k_cell_place, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, Counters* ctr,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* bsum, uint32_t* stash, Counters* ctr,
                        hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n, ctr);
     if (ct.mbits) {
         hipLaunchKernelGGL(k_pair_count<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n,
-                           cnt, stash, ctr);
+                           cnt, bsum, stash, ctr);
         return;
     }
     hipLaunchKernelGGL(k_pair_count<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
-                       stash, ctr);
+                       bsum, stash, ctr);
     hipLaunchKernelGGL(k_pair_count<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
-                       stash, ctr);
+                       bsum, stash, ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                      const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
-                      Counters* cnt, const uint32_t* stash, hipStream_t s) {
+                      const CellTable& ct, uint32_t n, const uint32_t* pcnt, const uint32_t* boff, uint32_t cap,
+                      uint32_t* pa, uint32_t* pb, Counters* cnt, const uint32_t* stash, hipStream_t s) {
     if (!n) return;
     if (ct.mbits) {
-        hipLaunchKernelGGL(k_pair_emit<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
-                           cap, pa, pb, cnt, stash);
+        hipLaunchKernelGGL(k_pair_emit<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, pcnt,
+                           boff, cap, pa, pb, cnt, stash);
         return;
     }
-    hipLaunchKernelGGL(k_pair_emit<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
-                       cap, pa, pb, cnt, stash);
-    hipLaunchKernelGGL(k_pair_emit<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, off,
-                       cap, pa, pb, cnt, stash);
+    hipLaunchKernelGGL(k_pair_emit<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, pcnt,
+                       boff, cap, pa, pb, cnt, stash);
+    hipLaunchKernelGGL(k_pair_emit<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, pcnt,
+                       boff, cap, pa, pb, cnt, stash);
 }
 void launch_narrow(const Bodies& B, const StepParams& sp, const uint32_t* pa, const uint32_t* pb, uint32_t cap_pairs,
                    uint32_t* p_np, float4* p_pt, float4* p_nf, uint32_t* epa_list, uint32_t* p_kind, uint32_t* kinds,

@@ -64,12 +64,14 @@ void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Cou
 // Pairs per (body, z slice) part, BP_PARTS * n counts in canonical order (stashing
 // the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
 // the scanned offsets.
+// Pair counts per (body, z slice) part in cnt, and per 256-part workgroup in bsum (the
+// emit pass takes the exclusive scan of bsum, with the total at [workgroups], as boff).
 void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* stash, Counters* ctr,
+                       const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* bsum, uint32_t* stash, Counters* ctr,
                        hipStream_t s);
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
-                      const CellTable& ct, uint32_t n, const uint32_t* off, uint32_t cap, uint32_t* pa, uint32_t* pb,
-                      Counters* cnt, const uint32_t* stash, hipStream_t s);
+                      const CellTable& ct, uint32_t n, const uint32_t* pcnt, const uint32_t* boff, uint32_t cap,
+                      uint32_t* pa, uint32_t* pb, Counters* cnt, const uint32_t* stash, hipStream_t s);
 // Narrowphase of every pair: the pairs are first ordered by kind of test (p_kind,
 // the 32 counters `kinds` cleared at the step start, `order`), then tested; deep
 // box-capsule pairs queue for EPA, run by `epa_groups` workgroups of 64.

@@ -566,10 +566,15 @@ void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_cell_clear(w->ct, s);
     launch_cell_table(skey, n, w->ct, w->cnt, w->scan_tmp, s);
-    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, w->pair_stash, w->cnt, s);
-    scan_exclusive(w->pair_cnt, w->pair_off, BP_PARTS * n, w->scan_tmp, s, &w->cnt->pairs, nullptr, false,
+    // pair offsets in two levels: per-workgroup totals (256 parts each), scanned, then the
+    // emit pass scans within each workgroup
+    const uint32_t pwg = (BP_PARTS * n + 255u) / 256u;
+    uint32_t* bsum = w->pair_off;
+    uint32_t* boff = w->pair_off + pwg;
+    launch_pair_count(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, bsum, w->pair_stash, w->cnt, s);
+    scan_exclusive(bsum, boff, pwg, w->scan_tmp, s, &w->cnt->pairs, nullptr, false,
                    &w->cnt->sticky);
-    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_off, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
+    launch_pair_emit(w->B, sp, skey, sval, w->ct, n, w->pair_cnt, boff, w->cap_pairs, w->pa, w->pb, w->cnt, w->pair_stash,
                      s);
 }
 
