@@ -1608,67 +1608,81 @@ __global__ void __launch_bounds__(256) k_pgs(Bodies B, StepParams sp, SolverRows
 // between sweeps; only body velocities move between waves.
 //
 // Inboxes: during the launch a body's solver velocity travels from one visit of
-// the body to the next.  Each slot has an inbox of 16 64-bit words (one 128-byte
-// line) holding, for the slot's next visit, body a's velocity in words 0..5 and
-// body b's in words 8..13, each word the value bits | the body's visits
-// completed << 32.  A visit waits until its inbox words carry the visit
-// numbers it needs (a 64-bit word is single-copy atomic, so matching versions
-// mean matching values), solves, and writes each dynamic body's new velocity into
-// the inbox of that body's next visit (the manifold of the next colour in the
-// body's colour mask, or its first one for the next sweep) with plain stores
-// (three 16-byte ones where the inboxes lie within 2 GiB).  The last word of each
-// side (5 for a, 13 for b, in the last store a writer makes) is a hint: a poll
-// reads the two hints and loads the rest once both are in; the full load checks
-// every version itself.
+// the body to the next.  Each slot has an inbox of 8 64-bit words (64 bytes) holding,
+// for the slot's next visit, body a's velocity (v, w: 192 bits) in words 0..3 and
+// body b's in words 4..7, each word 48 value bits under a 16-bit version: the body's
+// visits completed, modulo 2^16.  An inbox only ever holds the version before the
+// one its reader waits for (deg visits earlier, deg <= 64) or that one, so 16 bits
+// never alias.  A visit waits until its inbox words carry the visit numbers it needs
+// (a 64-bit word is single-copy atomic, so matching versions mean matching values),
+// solves, and writes each dynamic body's new velocity into the inbox of that body's
+// next visit (the manifold of the next colour in the body's colour mask, or its first
+// one for the next sweep) with plain stores (two 16-byte ones where the inboxes lie
+// within 2 GiB).  The last word of each side (3 for a, 7 for b, in the last store a
+// writer makes) is a hint: a poll reads the two hints and loads the rest once both are
+// in; the full load checks every version itself.
 // A bounded wait sets STICKY_TIMEOUT instead of hanging.
-constexpr uint32_t IB_WORDS = 16;   // inbox words per slot
-constexpr uint32_t IB_HINT = 5;     // word 5 of each side (polled first), stored with word 4
+constexpr uint32_t IB_WORDS = INBOX_WORDS;  // inbox words per slot
+constexpr uint32_t IB_SIDE = 4;     // words per body
+constexpr uint32_t IB_HINT = 3;     // word 3 of each side (polled first), stored with word 2
 constexpr uint32_t OWN_WAVE = 64;   // threads per workgroup of the sweep kernel
 // residual check word: workgroups arrived in the low 16 bits, workgroups above the tolerance above them
 constexpr uint32_t CHECK_ABOVE = 1u << 16;
 constexpr uint32_t NO_VISIT = 0xFFFFFFFFu;  // version no visit count reaches
 
-__device__ __forceinline__ uint64_t vpack(float v, uint32_t ver) {
-    return ((uint64_t)ver << 32) | (uint64_t)__float_as_uint(v);
+// A velocity (v, w) as four inbox words under version ver (low 16 bits): the six
+// floats' 192 bits in 48-bit pieces, x0 | x1[0:16], x1[16:32] | x2, x3 | x4[0:16],
+// x4[16:32] | x5.
+__device__ __forceinline__ void vpack4(f3 v, f3 w, uint32_t ver, uint64_t (&o)[4]) {
+    const uint64_t V = (uint64_t)(ver & 0xFFFFu) << 48;
+    const uint32_t x0 = __float_as_uint(v.x), x1 = __float_as_uint(v.y), x2 = __float_as_uint(v.z);
+    const uint32_t x3 = __float_as_uint(w.x), x4 = __float_as_uint(w.y), x5 = __float_as_uint(w.z);
+    o[0] = V | ((uint64_t)(x1 & 0xFFFFu) << 32) | x0;
+    o[1] = V | ((uint64_t)x2 << 16) | (x1 >> 16);
+    o[2] = V | ((uint64_t)(x4 & 0xFFFFu) << 32) | x3;
+    o[3] = V | ((uint64_t)x5 << 16) | (x4 >> 16);
 }
-__device__ __forceinline__ float vval(uint64_t w) { return __uint_as_float((uint32_t)w); }
-__device__ __forceinline__ uint32_t vver(uint64_t w) { return (uint32_t)(w >> 32); }
+// The same words as (low, high) dword pairs: the version of a word is high >> 16.
+__device__ __forceinline__ uint32_t vver2(uint2 x) { return x.y >> 16; }
+__device__ __forceinline__ void vunpack4(uint2 a, uint2 b, uint2 c, uint2 d, f3& v, f3& w) {
+    v = f3{__uint_as_float(a.x), __uint_as_float((a.y & 0xFFFFu) | (b.x << 16)),
+           __uint_as_float((b.x >> 16) | (b.y << 16))};
+    w = f3{__uint_as_float(c.x), __uint_as_float((c.y & 0xFFFFu) | (d.x << 16)),
+           __uint_as_float((d.x >> 16) | (d.y << 16))};
+}
+__device__ __forceinline__ uint2 vsplit(uint64_t x) { return make_uint2((uint32_t)x, (uint32_t)(x >> 32)); }
 __device__ __forceinline__ uint64_t vload(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void vstore(uint64_t* p, uint64_t w) {
     __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Word k (0..5) of side `side` of the inbox of slot s.
+// Word k (0..3) of side `side` of the inbox of slot s.
 __device__ __forceinline__ size_t vin_word(uint32_t s, uint32_t side, uint32_t k) {
-    return (size_t)s * IB_WORDS + side * 8u + k;
+    return (size_t)s * IB_WORDS + side * IB_SIDE + k;
 }
 // Writes a velocity into side `side` of slot s's inbox, the hint word last.  With
-// `wide` (inboxes within 2 GiB) words 0-1, 2-3 and 4-5 go out as one 16-byte store
-// each, with the cache policy of a device-scope store: three writes instead of six.  Every
+// `wide` (inboxes within 2 GiB) words 0-1 and 2-3 go out as one 16-byte store each,
+// with the cache policy of a device-scope store: two writes instead of four.  Every
 // word carries its own version, so a reader that sees one half of a pair before the
 // other only waits.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void vin_store(uint64_t* inbox, uint32_t s, uint32_t side, uint32_t ver, f3 v, f3 w,
                                           bool wide) {
+    uint64_t o[4];
+    vpack4(v, w, ver, o);
     if (wide) {
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(inbox, 0, 0x7FFFFFFF, 0x00020000);
-        const uint32_t base = s * (IB_WORDS * 8u) + side * 64u;  // byte offset of word 0 of this side
-        const u32x4_t q0 = {__float_as_uint(v.x), ver, __float_as_uint(v.y), ver};
-        const u32x4_t q1 = {__float_as_uint(v.z), ver, __float_as_uint(w.x), ver};
+        const uint32_t base = s * (IB_WORDS * 8u) + side * (IB_SIDE * 8u);  // byte offset of word 0 of this side
+        const u32x4_t q0 = {(uint32_t)o[0], (uint32_t)(o[0] >> 32), (uint32_t)o[1], (uint32_t)(o[1] >> 32)};
+        const u32x4_t q1 = {(uint32_t)o[2], (uint32_t)(o[2] >> 32), (uint32_t)o[3], (uint32_t)(o[3] >> 32)};
         // aux: sc1, as the device-scope atomic store below
         __builtin_amdgcn_raw_buffer_store_b128(q0, rs, (int)base, 0, 16);
         __builtin_amdgcn_raw_buffer_store_b128(q1, rs, (int)(base + 16u), 0, 16);
-        const u32x4_t q2 = {__float_as_uint(w.y), ver, __float_as_uint(w.z), ver};
-        __builtin_amdgcn_raw_buffer_store_b128(q2, rs, (int)(base + 32u), 0, 16);
         return;
     }
-    vstore(inbox + vin_word(s, side, 0), vpack(v.x, ver));
-    vstore(inbox + vin_word(s, side, 1), vpack(v.y, ver));
-    vstore(inbox + vin_word(s, side, 2), vpack(v.z, ver));
-    vstore(inbox + vin_word(s, side, 3), vpack(w.x, ver));
-    vstore(inbox + vin_word(s, side, 4), vpack(w.y, ver));
-    vstore(inbox + vin_word(s, side, 5), vpack(w.z, ver));
+#pragma unroll
+    for (uint32_t k = 0; k < IB_SIDE; ++k) vstore(inbox + vin_word(s, side, k), o[k]);
 }
 // Words k and k + 1 of slot s's inbox in one coherent 16-byte access, as (value,
 // version) dword pairs; every word is checked against its own version, so a load
@@ -1715,16 +1729,16 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
         wb = xyz(B.sw[h.y]);
     }
     R.next[s] = nx;
-    // the whole inbox (words 0-5 and 8-13, (value, version) pairs as laid out by
-    // vin_word) in six 16-byte stores; the sweep kernel, a later launch, sees them after the
-    // launch boundary
+    // the whole inbox (words 0-3 and 4-7 as laid out by vin_word) in four 16-byte
+    // stores; the sweep kernel, a later launch, sees them after the launch boundary
+    uint64_t oa[4], ob[4];
+    vpack4(va, wa, ra, oa);
+    vpack4(vb, wb, rb, ob);
     uint4* q = reinterpret_cast<uint4*>(R.inbox + (size_t)s * IB_WORDS);
-    q[0] = make_uint4(fbits(va.x), ra, fbits(va.y), ra);
-    q[1] = make_uint4(fbits(va.z), ra, fbits(wa.x), ra);
-    q[2] = make_uint4(fbits(wa.y), ra, fbits(wa.z), ra);
-    q[4] = make_uint4(fbits(vb.x), rb, fbits(vb.y), rb);
-    q[5] = make_uint4(fbits(vb.z), rb, fbits(wb.x), rb);
-    q[6] = make_uint4(fbits(wb.y), rb, fbits(wb.z), rb);
+    q[0] = make_uint4((uint32_t)oa[0], (uint32_t)(oa[0] >> 32), (uint32_t)oa[1], (uint32_t)(oa[1] >> 32));
+    q[1] = make_uint4((uint32_t)oa[2], (uint32_t)(oa[2] >> 32), (uint32_t)oa[3], (uint32_t)(oa[3] >> 32));
+    q[2] = make_uint4((uint32_t)ob[0], (uint32_t)(ob[0] >> 32), (uint32_t)ob[1], (uint32_t)(ob[1] >> 32));
+    q[3] = make_uint4((uint32_t)ob[2], (uint32_t)(ob[2] >> 32), (uint32_t)ob[3], (uint32_t)(ob[3] >> 32));
     }
 }
 // After a launch: each dynamic body's final velocity is in the inbox of its first
@@ -1737,16 +1751,18 @@ __global__ void __launch_bounds__(256) k_flow_finish(Bodies B, SolverRows R, con
     const uint2 dp = R.dep[s];
     const uint64_t* ib = R.inbox;
     if (dp.x != UNSET && (dp.x & 0xFFu) == 0u) {
-        B.sv[h.x] = make_float4(vval(ib[vin_word(s, 0, 0)]), vval(ib[vin_word(s, 0, 1)]), vval(ib[vin_word(s, 0, 2)]),
-                                B.sv[h.x].w);
-        B.sw[h.x] = make_float4(vval(ib[vin_word(s, 0, 3)]), vval(ib[vin_word(s, 0, 4)]), vval(ib[vin_word(s, 0, 5)]),
-                                0.0f);
+        f3 v, w;
+        vunpack4(vsplit(ib[vin_word(s, 0, 0)]), vsplit(ib[vin_word(s, 0, 1)]), vsplit(ib[vin_word(s, 0, 2)]),
+                 vsplit(ib[vin_word(s, 0, 3)]), v, w);
+        B.sv[h.x] = f4(v, B.sv[h.x].w);
+        B.sw[h.x] = f4(w, 0.0f);
     }
     if (dp.y != UNSET && (dp.y & 0xFFu) == 0u) {
-        B.sv[h.y] = make_float4(vval(ib[vin_word(s, 1, 0)]), vval(ib[vin_word(s, 1, 1)]), vval(ib[vin_word(s, 1, 2)]),
-                                B.sv[h.y].w);
-        B.sw[h.y] = make_float4(vval(ib[vin_word(s, 1, 3)]), vval(ib[vin_word(s, 1, 4)]), vval(ib[vin_word(s, 1, 5)]),
-                                0.0f);
+        f3 v, w;
+        vunpack4(vsplit(ib[vin_word(s, 1, 0)]), vsplit(ib[vin_word(s, 1, 1)]), vsplit(ib[vin_word(s, 1, 2)]),
+                 vsplit(ib[vin_word(s, 1, 3)]), v, w);
+        B.sv[h.y] = f4(v, B.sv[h.y].w);
+        B.sw[h.y] = f4(w, 0.0f);
     }
     }
 }
@@ -1977,17 +1993,14 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
     // later sweep runs before the check's grid barrier, so the words stay put.
     auto final_vel = [&](uint32_t body, uint32_t dpw, uint32_t sweeps, SB& X) -> bool {
         const uint32_t v0 = B.vslot[(size_t)body * 64u + (uint32_t)__ffsll((unsigned long long)bmask[body]) - 1u];
-        const uint32_t want = sweeps * (dpw >> 8), vs = v0 >> 1, side = v0 & 1u;
+        const uint32_t want = (sweeps * (dpw >> 8)) & 0xFFFFu, vs = v0 >> 1, side = v0 & 1u;
         const uint64_t* ib = R.inbox;
         for (;;) {
-            const uint64_t a0 = vload(ib + vin_word(vs, side, 0)), a1 = vload(ib + vin_word(vs, side, 1)),
-                           a2 = vload(ib + vin_word(vs, side, 2)), a3 = vload(ib + vin_word(vs, side, 3)),
-                           a4 = vload(ib + vin_word(vs, side, 4)), a5 = vload(ib + vin_word(vs, side, 5));
-            const uint32_t d = (vver(a0) ^ want) | (vver(a1) ^ want) | (vver(a2) ^ want) | (vver(a3) ^ want) |
-                               (vver(a4) ^ want) | (vver(a5) ^ want);
+            const uint2 a0 = vsplit(vload(ib + vin_word(vs, side, 0))), a1 = vsplit(vload(ib + vin_word(vs, side, 1))),
+                        a2 = vsplit(vload(ib + vin_word(vs, side, 2))), a3 = vsplit(vload(ib + vin_word(vs, side, 3)));
+            const uint32_t d = (vver2(a0) ^ want) | (vver2(a1) ^ want) | (vver2(a2) ^ want) | (vver2(a3) ^ want);
             if (d == 0u) {
-                X.v = f3{vval(a0), vval(a1), vval(a2)};
-                X.w = f3{vval(a3), vval(a4), vval(a5)};
+                vunpack4(a0, a1, a2, a3, X.v, X.w);
                 return true;
             }
             if (wall_clock64() - t_start > t_limit) {
@@ -2118,42 +2131,43 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
                 // the two hint words first; the full load below runs only for lanes
                 // whose hint words are in
                 const uint64_t ha = vload(R.inbox + vin_word(s, 0, IB_HINT)), hb = vload(R.inbox + vin_word(s, 1, IB_HINT));
-                ready = ((vver(ha) == tA) | !A.dyn) & ((vver(hb) == tB) | !Bd.dyn);
+                ready = ((vver2(vsplit(ha)) == (tA & 0xFFFFu)) | !A.dyn) & ((vver2(vsplit(hb)) == (tB & 0xFFFFu)) | !Bd.dyn);
             }
             if (ready) {
-                // (value, version) of the six words of each side
-                uint2 a[6], b[6];
+                // (low, high) dwords of the four words of each side
+                uint2 a[4], b[4];
                 if (wide) {
-                    // 16-byte loads, three per side
+                    // 16-byte loads, two per side
                     const uint4 q0 = vin_quad(R.inbox, s, 0), q1 = vin_quad(R.inbox, s, 2),
-                                q2 = vin_quad(R.inbox, s, 4), q4 = vin_quad(R.inbox, s, 8),
-                                q5 = vin_quad(R.inbox, s, 10), q6 = vin_quad(R.inbox, s, 12);
-                    a[0] = make_uint2(q0.x, q0.y), a[1] = make_uint2(q0.z, q0.w), a[2] = make_uint2(q1.x, q1.y);
-                    a[3] = make_uint2(q1.z, q1.w), a[4] = make_uint2(q2.x, q2.y), a[5] = make_uint2(q2.z, q2.w);
-                    b[0] = make_uint2(q4.x, q4.y), b[1] = make_uint2(q4.z, q4.w), b[2] = make_uint2(q5.x, q5.y);
-                    b[3] = make_uint2(q5.z, q5.w), b[4] = make_uint2(q6.x, q6.y), b[5] = make_uint2(q6.z, q6.w);
+                                q2 = vin_quad(R.inbox, s, 4), q3 = vin_quad(R.inbox, s, 6);
+                    a[0] = make_uint2(q0.x, q0.y), a[1] = make_uint2(q0.z, q0.w);
+                    a[2] = make_uint2(q1.x, q1.y), a[3] = make_uint2(q1.z, q1.w);
+                    b[0] = make_uint2(q2.x, q2.y), b[1] = make_uint2(q2.z, q2.w);
+                    b[2] = make_uint2(q3.x, q3.y), b[3] = make_uint2(q3.z, q3.w);
                 } else {
                     const uint64_t* ib = R.inbox;
 #pragma unroll
-                    for (uint32_t k = 0; k < 6; ++k) {
-                        const uint64_t wa = vload(ib + vin_word(s, 0, k)), wb = vload(ib + vin_word(s, 1, k));
-                        a[k] = make_uint2((uint32_t)wa, vver(wa));
-                        b[k] = make_uint2((uint32_t)wb, vver(wb));
+                    for (uint32_t k = 0; k < IB_SIDE; ++k) {
+                        a[k] = vsplit(vload(ib + vin_word(s, 0, k)));
+                        b[k] = vsplit(vload(ib + vin_word(s, 1, k)));
                     }
                 }
                 // branch-free: every loaded word is consumed here, so no later instruction
                 // waits on the memory counter (which also counts the stores before it)
-                const uint32_t da = (a[0].y ^ tA) | (a[1].y ^ tA) | (a[2].y ^ tA) | (a[3].y ^ tA) |
-                                    (a[4].y ^ tA) | (a[5].y ^ tA);
-                const uint32_t db = (b[0].y ^ tB) | (b[1].y ^ tB) | (b[2].y ^ tB) | (b[3].y ^ tB) |
-                                    (b[4].y ^ tB) | (b[5].y ^ tB);
+                const uint32_t tA16 = tA & 0xFFFFu, tB16 = tB & 0xFFFFu;
+                const uint32_t da = (vver2(a[0]) ^ tA16) | (vver2(a[1]) ^ tA16) | (vver2(a[2]) ^ tA16) |
+                                    (vver2(a[3]) ^ tA16);
+                const uint32_t db = (vver2(b[0]) ^ tB16) | (vver2(b[1]) ^ tB16) | (vver2(b[2]) ^ tB16) |
+                                    (vver2(b[3]) ^ tB16);
                 ok = ((da == 0u) | !A.dyn) & ((db == 0u) | !Bd.dyn);
                 const bool ta = ok & A.dyn, tb = ok & Bd.dyn;
-                auto val = [](uint2 x) { return __uint_as_float(x.x); };
-                A.v = f3{ta ? val(a[0]) : A.v.x, ta ? val(a[1]) : A.v.y, ta ? val(a[2]) : A.v.z};
-                A.w = f3{ta ? val(a[3]) : A.w.x, ta ? val(a[4]) : A.w.y, ta ? val(a[5]) : A.w.z};
-                Bd.v = f3{tb ? val(b[0]) : Bd.v.x, tb ? val(b[1]) : Bd.v.y, tb ? val(b[2]) : Bd.v.z};
-                Bd.w = f3{tb ? val(b[3]) : Bd.w.x, tb ? val(b[4]) : Bd.w.y, tb ? val(b[5]) : Bd.w.z};
+                f3 av, aw, bv, bw;
+                vunpack4(a[0], a[1], a[2], a[3], av, aw);
+                vunpack4(b[0], b[1], b[2], b[3], bv, bw);
+                A.v = f3{ta ? av.x : A.v.x, ta ? av.y : A.v.y, ta ? av.z : A.v.z};
+                A.w = f3{ta ? aw.x : A.w.x, ta ? aw.y : A.w.y, ta ? aw.z : A.w.z};
+                Bd.v = f3{tb ? bv.x : Bd.v.x, tb ? bv.y : Bd.v.y, tb ? bv.z : Bd.v.z};
+                Bd.w = f3{tb ? bw.x : Bd.w.x, tb ? bw.y : Bd.w.y, tb ? bw.z : Bd.w.z};
             }
             if (ok) {
                 const bool first = sweep0 + sw == 0u;

@@ -15,6 +15,7 @@ namespace axm {
 // (m = effective mass, k = 1 / m).  lam = (lambda_n, lambda_t1, lambda_t2, -) at
 // k * cap + s, lam0 = its warm-start value, fid = feature id.
 constexpr uint32_t JROWS = 15;
+constexpr uint32_t INBOX_WORDS = 8;  // 64-bit inbox words per slot of the persistent sweep
 struct SolverRows {
     uint4* hdr;  // (a, b, npts, manifold) per slot
     float4* J;
@@ -23,7 +24,7 @@ struct SolverRows {
     uint32_t* fid;
     uint2* dep;  // per slot: (deg << 8 | rank) of body a and body b in the colour order, UNSET if not dynamic
     uint32_t* colour;  // per slot: the manifold's colour (kept by the next step's incremental colouring)
-    uint64_t* inbox;   // persistent sweep: per slot 16 words, velocities of body a (0..5) and b (8..13)
+    uint64_t* inbox;   // persistent sweep: per slot INBOX_WORDS words, velocities of body a (0..3) and b (4..7)
     uint2* next;       // persistent sweep: per slot, the next visit of body a and of b (slot << 1 | side)
 };
 // Steps between full recolourings (in between, persisting contacts keep their colour).

@@ -336,7 +336,7 @@ axm_status alloc_rows(axm_world_s* w, int set, uint32_t cap) {
     AXM_TRY(w->mem.get(R.fid, (size_t)cap * 4));
     AXM_TRY(w->mem.get(R.dep, cap));
     AXM_TRY(w->mem.get(R.colour, cap));
-    AXM_TRY(w->mem.get(R.inbox, (size_t)cap * 16));
+    AXM_TRY(w->mem.get(R.inbox, (size_t)cap * INBOX_WORDS));
     AXM_TRY(w->mem.get(R.next, cap));
     AXM_TRY(w->mem.get(w->np_slot[set], cap));
     const uint32_t T = pow2_at_least(2ull * cap + 16);
@@ -367,7 +367,7 @@ axm_status grow_rows_keep(axm_world_s* w, int set, uint32_t cap) {
     uint64_t* inbox = nullptr;
     uint2* next = nullptr;
     AXM_TRY(w->mem.get(colour, cap));
-    AXM_TRY(w->mem.get(inbox, (size_t)cap * 16));
+    AXM_TRY(w->mem.get(inbox, (size_t)cap * INBOX_WORDS));
     AXM_TRY(w->mem.get(next, cap));
     AXM_TRY(w->mem.get(lam, (size_t)cap * 4));
     AXM_TRY(w->mem.get(fid, (size_t)cap * 4));
