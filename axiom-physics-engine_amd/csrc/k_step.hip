@@ -1020,20 +1020,52 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
     }
     return cur;
 }
+// Root of x in the finished forest (after the union kernel, which no later kernel of the
+// step changes): the union's coherent loads, without its path-halving stores (in one giant
+// island every walk ends on the same root word, which stores would keep contended).
+__device__ __forceinline__ uint32_t uf_root(uint32_t* par, uint32_t x) {
+    uint32_t p;
+    while ((p = uf_load(par, x)) != x) x = p;
+    return x;
+}
 // Island (union-find root) of a manifold: that of its dynamic body, UNSET with none.
 __device__ __forceinline__ uint32_t man_root(const Bodies& B, uint2 ab) {
     const bool plane = (ab.y & PLANE_BIT) != 0u;
     const uint32_t d = B.sv[ab.x].w > 0.0f ? ab.x : (!plane && B.sv[ab.y].w > 0.0f ? ab.y : UNSET);
-    return d == UNSET ? UNSET : uf_find(B.upar, d);
+    return d == UNSET ? UNSET : uf_root(B.upar, d);
+}
+// Per-island atomics of a wave: the lanes whose island is that of the first lane taking
+// part combine their values into one atomic made by that lane; the others make their own.
+// In one giant island a wave makes one atomic instead of 64 on the same word.
+template <class Op>
+__device__ __forceinline__ void wave_by_island(bool on, uint32_t r, uint32_t v, uint32_t idle, Op op) {
+    const uint64_t act = __ballot(on);
+    if (act == 0ull) return;
+    const uint32_t lead = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+    const uint32_t lr = (uint32_t)__shfl((int)r, (int)lead, 64);
+    const bool same = on && r == lr;
+    uint32_t c = same ? v : idle;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c = op.combine(c, (uint32_t)__shfl_xor((int)c, o, 64));
+    if (__lane_id() == lead) op.apply(lr, c);
+    if (on && !same) op.apply(r, v);
 }
 // Manifolds per island (after the union kernel).
 __global__ void __launch_bounds__(256) k_isl_count(Bodies B, const uint2* __restrict__ m_ab, uint32_t cap,
                                                    const Counters* __restrict__ cnt) {
     if (cnt->sticky != 0u) return;
     const uint32_t M = min(cap, cnt->m_eff);
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        const uint32_t r = man_root(B, m_ab[m]);
-        if (r != UNSET) atomicAdd(B.icnt + r, 1u);
+    struct Add {
+        uint32_t* icnt;
+        __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a + b; }
+        __device__ void apply(uint32_t r, uint32_t c) const { atomicAdd(icnt + r, c); }
+    };
+    // every lane of a wave runs the same number of rounds (the wave-wide combine)
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t m0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); m0 < M; m0 += stride) {
+        const uint32_t m = m0 + __lane_id();
+        const uint32_t r = m < M ? man_root(B, m_ab[m]) : UNSET;
+        wave_by_island(r != UNSET, r, 1u, 0u, Add{B.icnt});
     }
 }
 // Colour sort input: values = manifold index, keys = colour, or with the island split
@@ -2746,19 +2778,29 @@ __global__ void __launch_bounds__(256) k_isl_union(Bodies B, const uint2* __rest
 // Labels, rest times and each island's smallest rest time; counts islands and sleeping bodies.
 __global__ void __launch_bounds__(256) k_isl_label(Bodies B, StepParams sp, uint32_t n, Counters* cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cnt->sticky != 0u || i >= n) return;
+    if (cnt->sticky != 0u) return;
     uint32_t isl = 0, slp = 0;
-    if (B.sv[i].w > 0.0f) {
-        uint32_t r = i, p;
-        while ((p = B.upar[r]) != r) r = p;  // after the union kernel: a finished forest
+    // each island's least rest time (rest times are >= 0: bit order = value order)
+    struct Min {
+        uint32_t* imin;
+        __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+        __device__ void apply(uint32_t r, uint32_t c) const { atomicMin(imin + r, c); }
+    };
+    const bool dyn = i < n && B.sv[i].w > 0.0f;
+    uint32_t r = UNSET, tb = 0u;
+    if (dyn) {
+        r = uf_root(B.upar, i);
         const float4 v = B.vel[i], w = B.angv[i];
         const bool still = v.x * v.x + v.y * v.y + v.z * v.z < sp.sleep_lin2 &&
                            w.x * w.x + w.y * w.y + w.z * w.z < sp.sleep_ang2;
         const float t = still ? __uint_as_float(B.slp[i].x) + sp.dt : 0.0f;
         B.slp[i] = make_uint2(__float_as_uint(t), r);
-        atomicMin(B.imin + r, __float_as_uint(t));  // rest times are >= 0: bit order = value order
+        tb = __float_as_uint(t);
         isl = r == i ? 1u : 0u;
-    } else if (B.vel[i].w > 0.0f) {
+    }
+    wave_by_island(dyn, r, tb, 0xFFFFFFFFu, Min{B.imin});
+    if (i >= n) return;
+    if (!dyn && B.vel[i].w > 0.0f) {
         // asleep (a static body has no inverse mass)
         const uint32_t y = B.slp[i].y;
         isl = (y & LABEL_MASK) == i ? 1u : 0u;
