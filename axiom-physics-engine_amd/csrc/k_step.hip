@@ -1419,6 +1419,18 @@ __device__ __forceinline__ void japply(const Dir& J, float lam, SB& A, SB& Bd) {
         Bd.w = f3{fmaf(lam, J.ib.x, Bd.w.x), fmaf(lam, J.ib.y, Bd.w.y), fmaf(lam, J.ib.z, Bd.w.z)};
     }
 }
+// japply without a branch per side, for visits whose lanes differ in which sides are
+// dynamic.  A side that is not dynamic has zero inverse mass and zero inverse-inertia
+// rows (its world inverse inertia is zero), so for a finite impulse its velocity changes
+// at most in the sign of a zero component; callers never store such a side back.
+__device__ __forceinline__ void japply_both(const Dir& J, float lam, SB& A, SB& Bd) {
+    const float sa = -(lam * A.im);
+    A.v = f3{fmaf(sa, J.d.x, A.v.x), fmaf(sa, J.d.y, A.v.y), fmaf(sa, J.d.z, A.v.z)};
+    A.w = f3{fmaf(-lam, J.ia.x, A.w.x), fmaf(-lam, J.ia.y, A.w.y), fmaf(-lam, J.ia.z, A.w.z)};
+    const float sb = lam * Bd.im;
+    Bd.v = f3{fmaf(sb, J.d.x, Bd.v.x), fmaf(sb, J.d.y, Bd.v.y), fmaf(sb, J.d.z, Bd.v.z)};
+    Bd.w = f3{fmaf(lam, J.ib.x, Bd.w.x), fmaf(lam, J.ib.y, Bd.w.y), fmaf(lam, J.ib.z, Bd.w.z)};
+}
 // Direction `dir` (0 n, 1 t1, 2 t2) of the point whose rows start at J, consecutive
 // rows `rs` float4 apart; the .w of the first row is returned in w0.  Against a plane
 // b and ib are zero and not read.
@@ -1504,8 +1516,10 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
 }
 // The same with every row of the visit already in registers (Jr[k][r]: row r of
 // point k), so that a visit whose rows are not staged waits for one batch of loads
-// instead of one per row group.  Same arithmetic, same results.
-__device__ __forceinline__ Dir dir_regs(const float4 (&Jr)[JROWS], int dir, bool plane, float& w0, float& w3) {
+// instead of one per row group.  Same arithmetic, same results.  The b-side rows of a
+// plane contact hold zeros (load_rows), the values the row loads above substitute, so
+// no lane selects between the two.
+__device__ __forceinline__ Dir dir_regs(const float4 (&Jr)[JROWS], int dir, float& w0, float& w3) {
     const float4 r0 = Jr[5 * dir], r1 = Jr[5 * dir + 1], r3 = Jr[5 * dir + 3];
     Dir D;
     D.d = xyz(r0);
@@ -1513,12 +1527,8 @@ __device__ __forceinline__ Dir dir_regs(const float4 (&Jr)[JROWS], int dir, bool
     D.ia = xyz(r3);
     w0 = r0.w;
     w3 = r3.w;
-    if (plane) {
-        D.b = D.ib = f3{0.0f, 0.0f, 0.0f};
-    } else {
-        D.b = xyz(Jr[5 * dir + 2]);
-        D.ib = xyz(Jr[5 * dir + 4]);
-    }
+    D.b = xyz(Jr[5 * dir + 2]);
+    D.ib = xyz(Jr[5 * dir + 4]);
     return D;
 }
 __device__ __forceinline__ void load_rows(const SolverRows& R, uint32_t s, uint32_t cap, uint32_t np, bool plane,
@@ -1534,19 +1544,19 @@ __device__ __forceinline__ void load_rows(const SolverRows& R, uint32_t s, uint3
         }
     }
 }
-__device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_t np, bool plane, float keep,
+__device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_t np, float keep,
                                            bool first, SB& A, SB& Bd, float4 L[4]) {
     float w0, w3;
     if (first) {
 #pragma unroll
         for (uint32_t k = 0; k < 4; ++k) {
             if (k >= np) break;
-            const Dir Dn = dir_regs(Jr[k], 0, plane, w0, w3);
-            const Dir D1 = dir_regs(Jr[k], 1, plane, w0, w3);
-            const Dir D2 = dir_regs(Jr[k], 2, plane, w0, w3);
-            japply(Dn, L[k].x, A, Bd);
-            japply(D1, L[k].y, A, Bd);
-            japply(D2, L[k].z, A, Bd);
+            const Dir Dn = dir_regs(Jr[k], 0, w0, w3);
+            const Dir D1 = dir_regs(Jr[k], 1, w0, w3);
+            const Dir D2 = dir_regs(Jr[k], 2, w0, w3);
+            japply_both(Dn, L[k].x, A, Bd);
+            japply_both(D1, L[k].y, A, Bd);
+            japply_both(D2, L[k].z, A, Bd);
         }
     }
 #pragma unroll
@@ -1555,33 +1565,33 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
         float maxf;  // the step's friction bound (row 8's .w)
         {
             float mT1;
-            const Dir D1 = dir_regs(Jr[k], 1, plane, mT1, maxf);
+            const Dir D1 = dir_regs(Jr[k], 1, mT1, maxf);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].y;
             L[k].y = nl;
-            japply(D1, d, A, Bd);
+            japply_both(D1, d, A, Bd);
         }
         {
             float mT2;
-            const Dir D2 = dir_regs(Jr[k], 2, plane, mT2, w3);
+            const Dir D2 = dir_regs(Jr[k], 2, mT2, w3);
             float nl = fmaf(-mT2, jvel(D2, A, Bd), keep * L[k].z);
             nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
             const float d = nl - L[k].z;
             L[k].z = nl;
-            japply(D2, d, A, Bd);
+            japply_both(D2, d, A, Bd);
         }
     }
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= np) break;
         float mN, tg;
-        const Dir Dn = dir_regs(Jr[k], 0, plane, mN, tg);
+        const Dir Dn = dir_regs(Jr[k], 0, mN, tg);
         float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L[k].x);
         nl = nl > 0.0f ? nl : 0.0f;
         const float d = nl - L[k].x;
         L[k].x = nl;
-        japply(Dn, d, A, Bd);
+        japply_both(Dn, d, A, Bd);
     }
 }
 // The same with the rows of solve slot s in the SoA row arrays (row r of point k at
@@ -1825,12 +1835,12 @@ __device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool
 }
 
 // The same with the point's rows in registers (as solve_regs).
-__device__ __forceinline__ float point_residual_regs(const float4 (&Jr)[JROWS], bool plane, float keep, const SB& A,
+__device__ __forceinline__ float point_residual_regs(const float4 (&Jr)[JROWS], float keep, const SB& A,
                                                      const SB& Bd, const float4 L) {
     float mN, tg, mT1, maxf, mT2, x3;
-    const Dir Dn = dir_regs(Jr, 0, plane, mN, tg);
-    const Dir D1 = dir_regs(Jr, 1, plane, mT1, maxf);
-    const Dir D2 = dir_regs(Jr, 2, plane, mT2, x3);
+    const Dir Dn = dir_regs(Jr, 0, mN, tg);
+    const Dir D1 = dir_regs(Jr, 1, mT1, maxf);
+    const Dir D2 = dir_regs(Jr, 2, mT2, x3);
     const float kN = Jr[1].w, kT1 = Jr[6].w, kT2 = Jr[11].w;
     float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
     nl = nl > 0.0f ? nl : 0.0f;
@@ -2221,7 +2231,7 @@ __global__ void __launch_bounds__(OWN_WAVE) k_pgs_own(Bodies B, StepParams sp, S
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) L[k] = R.lam[(size_t)k * cap + s];
-                    solve_regs(Jr, np, plane, keep, first, A, Bd, L);
+                    solve_regs(Jr, np, keep, first, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k)
                         if (k < np) R.lam[(size_t)k * cap + s] = make_float4(L[k].x, L[k].y, L[k].z, 0.0f);
@@ -2489,7 +2499,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                     SB A, Bd;
                     home_load(bl, home_a, A);
                     if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
-                    solve_regs(Jr, np, plane, keep, it == 0u, A, Bd, L);
+                    solve_regs(Jr, np, keep, it == 0u, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k) L[k].w = 0.0f;
                     if (A.dyn) home_store(bl, home_a, A);
@@ -2508,7 +2518,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k) {
                         if (k >= np) break;
-                        const float e = point_residual_regs(Jr[k], plane, keep, A, Bd, L[k]);
+                        const float e = point_residual_regs(Jr[k], keep, A, Bd, L[k]);
                         if (e > r) r = e;
                     }
                     atomicMax(&cur[li], __float_as_uint(r));
