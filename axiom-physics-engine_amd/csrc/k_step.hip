@@ -44,8 +44,15 @@ template <class T> __device__ __forceinline__ T spread_sum(const Counters* cnt, 
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
                                                uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
                                                uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
-                                               uint32_t* __restrict__ bdeg, const Counters* __restrict__ cnt) {
+                                               uint32_t* __restrict__ bdeg, const Counters* __restrict__ cnt,
+                                               CellTable ct) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // with the Morton bitmap: the words the last step set go back to zero (several bodies
+    // may name one word: all write zero); the cell table sets this step's after the sort
+    if (ct.mbits && i < ct.mlist_n) {
+        const uint32_t w = ct.mlist[i];
+        if (w != UNSET) ct.mbits[w] = 0u;
+    }
     if (i >= n) return;
     // the colouring's per-body words start from zero every step
     bmax0[i] = 0ull;
@@ -152,18 +159,11 @@ __device__ __forceinline__ bool ht_find_from(const CellTable& t, uint64_t k, uin
 // Occupancy bit of a cell hash (the high half of the hash; the slot takes the low bits).
 __device__ __forceinline__ uint32_t cell_bit(uint64_t h, uint32_t bmask) { return (uint32_t)(h >> 32) & bmask; }
 
+// The hash table and its occupancy bits (the Morton bitmap is cleared by k_begin).
 __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!t.mbits && i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
-    if (t.mbits) {
-        // the words the last step set (several bodies may name one word: all write zero)
-        if (i < t.mlist_n) {
-            const uint32_t w = t.mlist[i];
-            if (w != UNSET) t.mbits[w] = 0u;
-        }
-    } else if (i <= (t.bmask >> 5)) {
-        t.bits[i] = 0u;
-    }
+    if (i <= t.mask) t.ent[i] = make_uint4((uint32_t)EMPTY_KEY, (uint32_t)(EMPTY_KEY >> 32), 0u, 0u);
+    if (i <= (t.bmask >> 5)) t.bits[i] = 0u;
 }
 
 // The cell table is built in two passes, so that most cells need no atomic.  Pass 1
@@ -173,10 +173,19 @@ __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
 // its home slot writes its range there; the others insert by compare-and-swap from
 // the next slot on.  Slots claimed in pass 1 never change, so every key still lies
 // after its home with no empty slot between (the linear-probing invariant).
-__global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+// Each thread also gathers the AABB of sorted body s into smin / smax (body index in
+// smin.w), so that the candidates of a cell are contiguous and carry what the test needs.
+__global__ void __launch_bounds__(256) k_cell_table(Bodies B, const uint64_t* __restrict__ skey,
+                                                    const uint32_t* __restrict__ sval, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = __lane_id();
+    if (s < n) {
+        const uint32_t i = sval[s];
+        const float4 mn = B.aabb[2u * i], mx = B.aabb[2u * i + 1u];
+        B.smin[s] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
+        B.smax[s] = mx;
+    }
     // the first body of each cell inserts it; one count per wave
     const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
     const bool head = s < n && (s == 0 || skey[s - 1] != k);
@@ -213,8 +222,15 @@ __global__ void __launch_bounds__(256) k_cell_table(const uint64_t* __restrict__
 }
 // With the Morton bitmap (crank holds the exclusive scan of the cell heads): each cell's
 // first sorted position by rank, and each bitmap word's rank base (its first cell's rank).
-__global__ void __launch_bounds__(256) k_cell_rank(const uint64_t* __restrict__ skey, uint32_t n, CellTable t) {
+// Also decides the broadphase mode of the step (bp_sparse: fewer than four bodies per
+// occupied cell), as k_cell_place does in the hash mode.
+__device__ __forceinline__ void decide_mode(Counters* cnt, uint32_t n) {
+    cnt->bp_sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n ? 1u : 0u;
+}
+__global__ void __launch_bounds__(256) k_cell_rank(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+                                                   Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0u) decide_mode(cnt, n);
     if (s >= n) return;
     const uint64_t k = skey[s];
     const uint64_t kp = s ? skey[s - 1u] : EMPTY_KEY;
@@ -231,6 +247,7 @@ __global__ void __launch_bounds__(256) k_cell_place(const uint64_t* __restrict__
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = __lane_id();
+    if (s == 0u) decide_mode(cnt, n);
     const uint64_t k = s < n ? skey[s] : EMPTY_KEY;
     const bool head = s < n && (s == 0 || skey[s - 1] != k);
     const uint64_t heads = __ballot(head);
@@ -272,21 +289,11 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // Visits candidate partners of the body at sorted position t in the canonical
 // order: planes, then the 27 neighbour cells (dz, dy, dx in -1..1), then
 // ascending sorted position.  f(partner) is called for every emitted pair.
-// AABBs in sorted order: smin = (min, body index bits), smax = (max, 1 if dynamic),
-// so the candidates of a cell are contiguous and carry what the test needs.
-// Also decides the broadphase mode of the step: sparse when there are fewer than four
-// bodies per occupied cell (the pair kernels come in a sparse and a dense build, each
-// with only the registers its mode needs, and the one not chosen returns at once).
-__global__ void __launch_bounds__(256) k_sorted_aabb(Bodies B, const uint32_t* __restrict__ sval, uint32_t n,
-                                                     Counters* __restrict__ cnt) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) cnt->bp_sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n ? 1u : 0u;
-    if (t >= n) return;
-    const uint32_t i = sval[t];
-    const float4 mn = B.aabb[2u * i], mx = B.aabb[2u * i + 1u];
-    B.smin[t] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
-    B.smax[t] = mx;
-}
+// AABBs in sorted order (k_cell_table): smin = (min, body index bits), smax = (max,
+// 1 if dynamic).  The broadphase mode of the step (k_cell_rank / k_cell_place) is
+// sparse when there are fewer than four bodies per occupied cell (the pair kernels
+// come in a sparse and a dense build, each with only the registers its mode needs,
+// and the one not chosen returns at once).
 
 // Collision filter of two bodies (flags, category, mask, group): a shared nonzero group
 // decides (positive: always, negative: never), else each category must meet the other's mask.
@@ -1090,16 +1097,24 @@ __global__ void __launch_bounds__(256) k_col_sort_input(Bodies B, const uint2* _
 // The work list of the islands solved by one workgroup each, from the sorted keys: (first
 // slot, manifolds) of every small island of more than ISLAND_LANE_MAX manifolds; the smaller
 // ones are only counted (the bundle kernel finds them by their slots).
-__global__ void __launch_bounds__(256) k_isl_list(const Bodies B, const uint32_t* __restrict__ skey,
-                                                  uint2* __restrict__ list, uint32_t cap, Counters* cnt) {
-    if (cnt->sticky != 0u) return;
-    const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob;
-    for (uint32_t s = M0 + blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
-        const uint32_t k = skey[s];
-        if (s != M0 && ((skey[s - 1] ^ k) >> 8) == 0u) continue;
-        const uint32_t size = B.icnt[(k & ~LOCAL_KEY) >> 8];
-        if (size <= ISLAND_LANE_MAX) atomicAdd(&cnt->isl_lane_n, 1u);
-        else list[atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
+// (Run by the row preparation launch, thread gid of gsz, both whole waves; every lane
+// of a wave takes part in each round, so the one-lane islands are counted per wave.)
+__device__ __forceinline__ void isl_list_part(const Bodies& B, const uint32_t* __restrict__ skey,
+                                              uint2* __restrict__ list, uint32_t cap, Counters* cnt, uint32_t gid,
+                                              uint32_t gsz) {
+    const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob, lane = gid & 63u;
+    for (uint32_t base = M0 + gid - lane; base < M; base += gsz) {
+        const uint32_t s = base + lane;
+        bool start = false;
+        uint32_t size = 0;
+        if (s < M) {
+            const uint32_t k = skey[s];
+            start = s == M0 || ((skey[s - 1] ^ k) >> 8) != 0u;
+            if (start) size = B.icnt[(k & ~LOCAL_KEY) >> 8];
+        }
+        const uint64_t one = __ballot(start && size <= ISLAND_LANE_MAX);
+        if (lane == 0u && one) atomicAdd(&cnt->isl_lane_n, (uint32_t)__popcll(one));
+        if (start && size > ISLAND_LANE_MAX) list[atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
     }
 }
 
@@ -1339,12 +1354,21 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                                               const float4* __restrict__ c_nf, SolverRows R, PrevCache pc,
                                               const float4* __restrict__ warm_in, const uint32_t* __restrict__ col,
                                               const uint64_t* __restrict__ bmask, const uint32_t* __restrict__ bdeg,
-                                              Counters* __restrict__ cnt, float* __restrict__ feat) {
+                                              Counters* __restrict__ cnt, float* __restrict__ feat, PrepExtra X) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    for (uint32_t i = gid; i < X.zero_n; i += gsz) X.zero[i] = 0u;
+    if (cnt->sticky == 0u) {
+        if (X.ckey) {
+            const uint32_t m = cnt->cache_m[X.cset];
+            for (uint32_t i = gid; i < m; i += gsz) X.ckey[X.cpos[i]] = EMPTY_KEY;
+        }
+        if (X.list) isl_list_part(B, X.skey, X.list, cap, cnt, gid, gsz);
+    }
     uint32_t hits = 0;
     unsigned long long bytes = 0;
     // a failed step must not touch the impulse buffers
     const uint32_t M = cnt->sticky == 0u ? cnt->m_eff : 0u;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+    for (uint32_t s = gid; s < M; s += gsz) {
         uint32_t h = 0;
         unsigned long long b = 0;
         prep_slot(B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc, warm_in, col, bmask, bdeg, cnt, feat, s, h, b);
@@ -2457,11 +2481,14 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
             const float4 va = B.sv[h.x], wa = B.sw[h.x];
             bl[BUNDLE_BODY0 + lane] = va;
             bl[BUNDLE_BODY0 + 128u + lane] = make_float4(wa.x, wa.y, wa.z, 0.0f);
+            // a plane's side holds zeros (velocity, inverse mass): it loads as a static body
+            float4 vb = make_float4(0.0f, 0.0f, 0.0f, 0.0f), wb = vb;
             if (!plane) {
-                const float4 vb = B.sv[h.y], wb = B.sw[h.y];
-                bl[BUNDLE_BODY0 + 64u + lane] = vb;
-                bl[BUNDLE_BODY0 + 192u + lane] = make_float4(wb.x, wb.y, wb.z, 0.0f);
+                vb = B.sv[h.y];
+                wb = B.sw[h.y];
             }
+            bl[BUNDLE_BODY0 + 64u + lane] = vb;
+            bl[BUNDLE_BODY0 + 192u + lane] = make_float4(wb.x, wb.y, wb.z, 0.0f);
             load_rows(R, s, cap, np, plane, Jr);
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k)
@@ -2498,7 +2525,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                 if (active && phase == c) {
                     SB A, Bd;
                     home_load(bl, home_a, A);
-                    if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
+                    home_load(bl, home_b, Bd);
                     solve_regs(Jr, np, keep, it == 0u, A, Bd, L);
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k) L[k].w = 0.0f;
@@ -2513,7 +2540,7 @@ __global__ void __launch_bounds__(64) k_pgs_bundle(Bodies B, StepParams sp, Solv
                 if (active) {
                     SB A, Bd;
                     home_load(bl, home_a, A);
-                    if (plane) zero_sb(Bd); else home_load(bl, home_b, Bd);
+                    home_load(bl, home_b, Bd);
                     float r = 0.0f;
 #pragma unroll
                     for (uint32_t k = 0; k < 4; ++k) {
@@ -2856,8 +2883,12 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_begin, grid_for(n), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg, cnt);
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, const CellTable& ct,
+                  hipStream_t s) {
+    const uint32_t m = std::max(n, ct.mbits ? ct.mlist_n : 0u);
+    if (m)
+        hipLaunchKernelGGL(k_begin, grid_for(m), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg, cnt,
+                           ct);
 }
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u64, grid_for(n), dim3(256), 0, s, p, v, n);
@@ -2869,16 +2900,17 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota, grid_for(n), dim3(256), 0, s, p, n);
 }
 void launch_cell_clear(const CellTable& t, hipStream_t s) {
-    const uint32_t n = t.mbits ? std::max(t.mlist_n, 1u) : std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
+    if (t.mbits) return;  // k_begin cleared the bitmap words
+    const uint32_t n = std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
     hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
 }
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, uint32_t* scan_tmp,
-                       hipStream_t s) {
+void launch_cell_table(const Bodies& B, const uint64_t* skey, const uint32_t* sval, uint32_t n, const CellTable& t,
+                       Counters* cnt, uint32_t* scan_tmp, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
+    hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, B, skey, sval, n, t, cnt);
     if (t.mbits) {
         scan_exclusive(t.crank, t.crank, n, scan_tmp, s, nullptr, nullptr, false, &cnt->sticky);
-        hipLaunchKernelGGL(k_cell_rank, grid_for(n), dim3(256), 0, s, skey, n, t);
+        hipLaunchKernelGGL(k_cell_rank, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
         return;
     }
     hipLaunchKernelGGL(k_cell_place, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
@@ -2887,7 +2919,6 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* bsum, uint32_t* stash, Counters* ctr,
                        hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_sorted_aabb, grid_for(n), dim3(256), 0, s, B, sval, n, ctr);
     if (ct.mbits) {
         hipLaunchKernelGGL(k_pair_count<BP_RANK>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n,
                            cnt, bsum, stash, ctr);
@@ -2972,7 +3003,6 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-    hipLaunchKernelGGL(k_isl_list, grid_bounded(cap), dim3(256), 0, s, B, skey, list, cap, cnt);
     if (max_sweeps == 0u) return;
     hipLaunchKernelGGL(k_pgs_group, dim3(groups), dim3(256), 0, s, B, sp, R, cnt, cap, list, skey, tol, max_sweeps);
     const uint32_t windows = (cap + BUNDLE_SLOTS - 1u) / BUNDLE_SLOTS;
@@ -2984,10 +3014,10 @@ void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s) {
+                 const uint32_t* bdeg, Counters* cnt, float* feat, const PrepExtra& X, hipStream_t s) {
     const PrevCache pc = make_prev(pca);
     hipLaunchKernelGGL(k_prep, grid_bounded(cap), dim3(256), 0, s, B, sp, order, cap, m_ab, m_np, c_pt, c_nf, R, pc,
-                       warm_in, col, bmask, bdeg, cnt, feat);
+                       warm_in, col, bmask, bdeg, cnt, feat, X);
 }
 
 // Learned warm start: the network predicts the correction to the previous
@@ -3063,8 +3093,8 @@ __global__ void k_cache_clear(uint64_t* __restrict__ key, const uint32_t* __rest
 }
 
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
-                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s) {
-    hipLaunchKernelGGL(k_cache_clear, grid_bounded(cap), dim3(256), 0, s, key, pos, cnt, set);
+                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s, bool clear) {
+    if (clear) hipLaunchKernelGGL(k_cache_clear, grid_bounded(cap), dim3(256), 0, s, key, pos, cnt, set);
     hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out, pos, set);
 }
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,

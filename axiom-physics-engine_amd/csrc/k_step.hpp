@@ -47,7 +47,8 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 // Start of a step: velocities, world inertia, AABBs, cell keys; clears the
 // colouring's per-body words.
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, hipStream_t s);
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, const CellTable& ct,
+                  hipStream_t s);
 // Islands of the step, after the narrowphase: union-find over the manifolds between awake
 // dynamic bodies (root = smallest body index), and wake requests for sleeping islands an
 // awake body touched.
@@ -58,10 +59,11 @@ void launch_islands(const Bodies& B, const StepParams& sp, uint32_t n, Counters*
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
-// Empties the cell table and its occupancy bits.
+// Empties the cell hash table and its occupancy bits (nothing with the Morton bitmap,
+// whose words k_begin clears).
 void launch_cell_clear(const CellTable& t, hipStream_t s);
-void launch_cell_table(const uint64_t* skey, uint32_t n, const CellTable& t, Counters* cnt, uint32_t* scan_tmp,
-                       hipStream_t s);
+void launch_cell_table(const Bodies& B, const uint64_t* skey, const uint32_t* sval, uint32_t n, const CellTable& t,
+                       Counters* cnt, uint32_t* scan_tmp, hipStream_t s);
 // Pairs per (body, z slice) part, BP_PARTS * n counts in canonical order (stashing
 // the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
 // the scanned offsets.
@@ -94,15 +96,30 @@ void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* c
                            uint32_t* val, const Counters* cnt, uint32_t split,
                            hipStream_t s);
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);
-// The islands solved on their own (after the row preparation): work lists from the sorted
-// keys, then one lane or one workgroup per island; tol > 0 stops each at its own residual.
+// The islands solved on their own (after the row preparation, which built their work
+// list): one lane or one workgroup per island; tol > 0 stops each at its own residual.
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// Work the row preparation launch also does for the launches after it, each in a loop of
+// its own over the grid: the small islands' work list from the sorted keys (when list is
+// set; launch_pgs_local then builds none), clearing zero_n words (the sweep's residual
+// words), and emptying the (a, b) -> slot table of row set cset (the entries its last
+// build recorded, at the positions cpos holds; when ckey is set, launch_cache_build is
+// then called with clear = false).
+struct PrepExtra {
+    const uint32_t* skey = nullptr;
+    uint2* list = nullptr;
+    uint32_t* zero = nullptr;
+    uint32_t zero_n = 0;
+    uint64_t* ckey = nullptr;
+    const uint32_t* cpos = nullptr;
+    uint32_t cset = 0;
+};
 void launch_prep(const Bodies& B, const StepParams& sp, const uint32_t* order, uint32_t cap, const uint2* m_ab,
                  const uint32_t* m_np, const float4* c_pt, const float4* c_nf, const SolverRows& R,
                  const PrevCacheArgs& pca, const float4* warm_in, const uint32_t* col, const uint64_t* bmask,
-                 const uint32_t* bdeg, Counters* cnt, float* feat, hipStream_t s);
+                 const uint32_t* bdeg, Counters* cnt, float* feat, const PrepExtra& X, hipStream_t s);
 // Writes the learned warm start (network outputs Y, rows 4 * slot + point) into the impulse rows.
 void launch_mlp_apply(const SolverRows& R, const Counters* cnt, uint32_t cap, const float* Y, const StepParams& sp,
                       hipStream_t s);
@@ -141,9 +158,11 @@ void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,
                      uint32_t* out_bits, hipStream_t s);
 // Empties the (a, b) -> slot table of row set `set` (the entries its last build made, at
-// the positions pos holds) and fills it with the step's slots.
+// the positions pos holds; skipped with clear = false, when the row preparation did it)
+// and fills it with the step's slots.
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
-                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s);
+                        uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s,
+                        bool clear = true);
 // Rehashes the `on`-slot table (okey, oslot) into the empty table (key, slot, mask), with the
 // new positions in pos.
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,

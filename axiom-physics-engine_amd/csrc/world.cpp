@@ -558,14 +558,14 @@ struct StepOpts {
 void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
-    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, w->cnt, s);
+    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, w->cnt, w->ct, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s, nullptr,
                                    &w->cnt->sticky);
     const uint64_t* skey = alt ? w->key_alt : w->key;
     const uint32_t* sval = alt ? w->val_alt : w->val;
     launch_cell_clear(w->ct, s);
-    launch_cell_table(skey, n, w->ct, w->cnt, w->scan_tmp, s);
+    launch_cell_table(w->B, skey, sval, n, w->ct, w->cnt, w->scan_tmp, s);
     // pair offsets in two levels: per-workgroup totals (256 parts each), scanned, then the
     // emit pass scans within each workgroup
     const uint32_t pwg = (BP_PARTS * n + 255u) / 256u;
@@ -642,14 +642,24 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
+    // the same launch builds the small islands' work list, clears the residual words (the
+    // per-sweep residuals, then the check arrivals) and empties this row set's (a, b) -> slot
+    // table, which the step's cache build refills
+    PrepExtra X;
+    X.skey = scol;
+    X.list = split ? w->isl_list : nullptr;
+    X.zero = w->res_bits;
+    X.zero_n = (uint32_t)res_words(w->res_cap);
+    X.ckey = w->pc_key[c];
+    X.cpos = w->pc_pos[c];
+    X.cset = (uint32_t)c;
     launch_prep(w->B, sp, order, rc, w->m_ab, w->m_np, w->c_pt, w->c_nf, R, pca, o.warm_in, w->col, w->bmask,
-                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, s);
+                w->bdone, w->cnt, w->want_feat ? w->feat : nullptr, X, s);
     if (w->mlp && w->desc.warm_start == AXM_WARM_MLP && !o.warm_in && w->feat) {
         launch_mlp_forward(w->mlp->dev(), w->feat, 4u * w->cap_man, &w->cnt->m_eff, 4u, w->ymlp,
                            2u * (uint32_t)w->cu_count, s);
         launch_mlp_apply(R, w->cnt, rc, w->ymlp, sp, s);
     }
-    launch_fill_u32(w->res_bits, 0u, res_words(w->res_cap), s);  // residuals, then check arrivals
     return AXM_SUCCESS;
 }
 
@@ -766,8 +776,9 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_iters = it;
     if (!post) return AXM_SUCCESS;
     w->last_dt = dt;
+    // the step's row preparation emptied the table
     launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], w->pc_pos[c],
-                       (uint32_t)c, s);
+                       (uint32_t)c, s, false);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
     launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
     launch_islands(w->B, sp, n, w->cnt, s);
