@@ -375,19 +375,24 @@ __device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& s
     if constexpr (MODE == BP_RANK) {
         // one large world: the nine occupancy words are read once (they stay in the cache);
         // an occupied cell's range comes by rank from two arrays in Morton order
-        uint32_t wv[9];
-        uint32_t occ = 0;
+        // per cell: occupied, and the occupied cells of its word below it (5 bits each, six
+        // to a register, so the nine words need not stay in registers)
+        uint32_t occ = 0, below0 = 0, below1 = 0;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             const uint32_t code = px[q % 3] | py[q / 3] | pz;
-            wv[q] = okx[q % 3] && oky[q / 3] ? ct.mbits[code >> 5] : 0u;
-            occ |= ((wv[q] >> (code & 31u)) & 1u) << q;
+            const uint32_t wv = okx[q % 3] && oky[q / 3] ? ct.mbits[code >> 5] : 0u;
+            occ |= ((wv >> (code & 31u)) & 1u) << q;
+            const uint32_t b = (uint32_t)__popc(wv & ((1u << (code & 31u)) - 1u));
+            if (q < 6) below0 |= b << (5 * q);
+            else below1 |= b << (5 * (q - 6));
         }
         while (occ) {
             const int q = __ffs(occ) - 1;
             occ &= occ - 1u;
             const uint32_t code = px[q % 3] | py[q / 3] | pz;
-            const uint32_t k = ct.wrank[code >> 5] + (uint32_t)__popc(wv[q] & ((1u << (code & 31u)) - 1u));
+            const uint32_t below = (q < 6 ? below0 >> (5 * q) : below1 >> (5 * (q - 6))) & 31u;
+            const uint32_t k = ct.wrank[code >> 5] + below;
             const uint32_t s0 = ct.cstart[k];
             visit_range(make_uint2(s0, ct.cstart[k + 1u] - s0));
         }
@@ -473,10 +478,10 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t& total) {
 // The workgroup's total goes to bsum[block] (block in the XCD-aware order): the emit pass
 // forms each part's offset from the scanned totals and a scan within its workgroup.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
-                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
-                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
-                                                    uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
+__device__ __forceinline__ void pair_count_body(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
+                                                const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t n,
+                                                uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
+                                                uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
     if (MODE != BP_RANK && (ctr->bp_sparse != 0u) != (MODE == BP_SPARSE)) return;
     const uint32_t lb = xcd_block();
     const uint32_t u = lb * blockDim.x + threadIdx.x;
@@ -494,6 +499,21 @@ __global__ void __launch_bounds__(256) k_pair_count(Bodies B, StepParams sp, con
     uint32_t tot;
     (void)wg_excl_scan(c, tot);
     if (threadIdx.x == 0u) bsum[lb] = tot;
+}
+// The rank and sparse builds fit 64 registers without scratch, so eight waves per SIMD
+// hide more of the probe latency; the dense build keeps its registers.
+template <int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+             CellTable ct, uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
+             uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
+    pair_count_body<MODE>(B, sp, skey, sval, ct, n, cnt, bsum, stash, ctr);
+}
+__global__ void __launch_bounds__(256)
+k_pair_count_dense(Bodies B, StepParams sp, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+                   CellTable ct, uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
+                   uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
+    pair_count_body<BP_DENSE>(B, sp, skey, sval, ct, n, cnt, bsum, stash, ctr);
 }
 
 // Part u's pairs start at boff[block] (the scanned workgroup totals; boff[blocks] is the
@@ -2926,7 +2946,7 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
     }
     hipLaunchKernelGGL(k_pair_count<BP_SPARSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
                        bsum, stash, ctr);
-    hipLaunchKernelGGL(k_pair_count<BP_DENSE>, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
+    hipLaunchKernelGGL(k_pair_count_dense, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
                        bsum, stash, ctr);
 }
 void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
