@@ -40,13 +40,31 @@ template <class T> __device__ __forceinline__ T spread_sum(const Counters* cnt, 
     return s;
 }
 
+constexpr uint32_t PAIR_KINDS = 16;  // narrowphase test kinds (pair_kind)
+// The per-step counters back to zero (the threads of one workgroup).
+__device__ __forceinline__ void counters_reset(Counters* cnt, uint32_t* __restrict__ kinds) {
+    uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
+    if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
+    for (uint32_t k = threadIdx.x; k < SPREAD; k += blockDim.x) {
+        cnt->sp[k].islands = 0u;
+        cnt->sp[k].sleeping = 0u;
+        cnt->sp[k].contacts = 0u;
+        cnt->sp[k].warm_hits = 0u;
+        cnt->sp[k].cells = 0u;
+        cnt->sp[k].pgs_bytes = 0ull;
+    }
+    if (kinds && threadIdx.x < 2u * PAIR_KINDS) kinds[threadIdx.x] = 0u;
+}
 // ---------------------------------------------------------------- k_begin
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
                                                uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
                                                uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
-                                               uint32_t* __restrict__ bdeg, const Counters* __restrict__ cnt,
-                                               CellTable ct) {
+                                               uint32_t* __restrict__ bdeg, Counters* __restrict__ cnt,
+                                               uint32_t* __restrict__ kinds, CellTable ct) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the step's counters start from zero (the kernels that count run after this launch;
+    // steps_done, read below, is kept across steps)
+    if (blockIdx.x == 0) counters_reset(cnt, kinds);
     // with the Morton bitmap: the words the last step set go back to zero (several bodies
     // may name one word: all write zero); the cell table sets this step's after the sort
     if (ct.mbits && i < ct.mlist_n) {
@@ -582,8 +600,8 @@ __device__ __forceinline__ void store_pair(uint32_t p, const Man& m, uint32_t* _
 
 // Processing order of the narrowphase: pairs grouped by the kind of test (the two
 // shape types, or the shape type against a plane), so that a wave runs one test
-// rather than every test its lanes need.  Results stay indexed by pair.
-constexpr uint32_t PAIR_KINDS = 16;
+// rather than every test its lanes need.  Results stay indexed by pair (PAIR_KINDS
+// kinds, defined with the step counters).
 __device__ __forceinline__ uint32_t pair_kind(const Bodies& B, uint32_t a, uint32_t b) {
     const uint32_t ta = fbits(B.shape[a].w) & 3u;
     if (b & PLANE_BIT) return 12u + ta;
@@ -1117,24 +1135,42 @@ __global__ void __launch_bounds__(256) k_col_sort_input(Bodies B, const uint2* _
 // The work list of the islands solved by one workgroup each, from the sorted keys: (first
 // slot, manifolds) of every small island of more than ISLAND_LANE_MAX manifolds; the smaller
 // ones are only counted (the bundle kernel finds them by their slots).
-// (Run by the row preparation launch, thread gid of gsz, both whole waves; every lane
-// of a wave takes part in each round, so the one-lane islands are counted per wave.)
-__device__ __forceinline__ void isl_list_part(const Bodies& B, const uint32_t* __restrict__ skey,
-                                              uint2* __restrict__ list, uint32_t cap, Counters* cnt, uint32_t gid,
-                                              uint32_t gsz) {
-    const uint32_t M = min(cap, cnt->m_eff), M0 = cnt->m_glob, lane = gid & 63u;
-    for (uint32_t base = M0 + gid - lane; base < M; base += gsz) {
+// One pass over the sorted keys skey (thread gid of gsz, both whole waves; every lane of a
+// wave takes part in each round): with off, the colour offsets off[c] = first position whose
+// key has colour >= c (c = 0..K, a small island's key counting as colour K, since those
+// keys sort above every colour) and m_glob = off[K]; with list, the work list of the small
+// islands solved by one workgroup each (first slot, manifolds), the one-lane islands only
+// counted (per wave).  An island starts where its key differs from the one before it above
+// the colour bits.
+__device__ __forceinline__ void sorted_keys_part(const Bodies& B, const uint32_t* __restrict__ skey,
+                                                 uint32_t* __restrict__ off, uint2* __restrict__ list, uint32_t cap,
+                                                 Counters* cnt, uint32_t gid, uint32_t gsz) {
+    const uint32_t M = min(cap, cnt->m_eff), K = cnt->max_colour, lane = gid & 63u;
+    auto colour_of = [K](uint32_t k) { return (k & LOCAL_KEY) != 0u ? K : k; };
+    for (uint32_t base = gid - lane; base <= M; base += gsz) {
         const uint32_t s = base + lane;
         bool start = false;
         uint32_t size = 0;
-        if (s < M) {
-            const uint32_t k = skey[s];
-            start = s == M0 || ((skey[s - 1] ^ k) >> 8) != 0u;
-            if (start) size = B.icnt[(k & ~LOCAL_KEY) >> 8];
+        if (s <= M) {
+            const uint32_t k = s < M ? skey[s] : 0u, kp = s > 0u ? skey[s - 1u] : 0u;
+            if (off) {
+                // the colours from just above the previous key's to this key's (to K at the end)
+                const uint32_t c0 = s > 0u ? colour_of(kp) + 1u : 0u, c1 = s < M ? colour_of(k) : K;
+                for (uint32_t c = c0; c <= c1; ++c) {
+                    off[c] = s;
+                    if (c == K) cnt->m_glob = s;
+                }
+            }
+            if (list && s < M) {
+                start = (k & LOCAL_KEY) != 0u && (s == 0u || ((kp ^ k) >> 8) != 0u);
+                if (start) size = B.icnt[(k & ~LOCAL_KEY) >> 8];
+            }
         }
-        const uint64_t one = __ballot(start && size <= ISLAND_LANE_MAX);
-        if (lane == 0u && one) atomicAdd(&cnt->isl_lane_n, (uint32_t)__popcll(one));
-        if (start && size > ISLAND_LANE_MAX) list[atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
+        if (list) {
+            const uint64_t one = __ballot(start && size <= ISLAND_LANE_MAX);
+            if (lane == 0u && one) atomicAdd(&cnt->isl_lane_n, (uint32_t)__popcll(one));
+            if (start && size > ISLAND_LANE_MAX) list[atomicAdd(&cnt->isl_group_n, 1u)] = make_uint2(s, size);
+        }
     }
 }
 
@@ -1144,19 +1180,24 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
 }
 
 // colour offsets: off[c] = first sorted position with colour >= c (c = 0..K)
+__device__ __forceinline__ void col_offsets_part(const uint32_t* __restrict__ scol, uint32_t max_k,
+                                                 uint32_t* __restrict__ off, Counters* __restrict__ cnt, uint32_t gid,
+                                                 uint32_t gsz) {
+    const uint32_t K = cnt->max_colour, M = cnt->m_eff;
+    if (gid == 0 && K > max_k) atomicOr(&cnt->sticky, STICKY_COLOURS);
+    for (uint32_t c = gid; c <= K && c <= max_k; c += gsz) {
+        uint32_t lo = 0, hi = M;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (scol[mid] < c) lo = mid + 1; else hi = mid;
+        }
+        off[c] = lo;
+        if (c == K) cnt->m_glob = lo;
+    }
+}
 __global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t max_k, uint32_t* __restrict__ off,
                               Counters* __restrict__ cnt) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t K = cnt->max_colour, M = cnt->m_eff;
-    if (c == 0 && K > max_k) atomicOr(&cnt->sticky, STICKY_COLOURS);
-    if (c > K || c > max_k) return;
-    uint32_t lo = 0, hi = M;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (scol[mid] < c) lo = mid + 1; else hi = mid;
-    }
-    off[c] = lo;
-    if (c == K) cnt->m_glob = lo;  // the small islands' keys sort above every colour
+    col_offsets_part(scol, max_k, off, cnt, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------- solver
@@ -1377,17 +1418,22 @@ __global__ void __launch_bounds__(256) k_prep(Bodies B, StepParams sp, const uin
                                               Counters* __restrict__ cnt, float* __restrict__ feat, PrepExtra X) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
     for (uint32_t i = gid; i < X.zero_n; i += gsz) X.zero[i] = 0u;
-    if (cnt->sticky == 0u) {
+    // more colours than the colour offsets hold fail the step (as a sticky error, which
+    // this launch sets itself, so it reads the condition rather than the bit)
+    const bool colours_ok = cnt->max_colour <= X.max_k;
+    // a failed step must not touch the impulse buffers
+    const bool ok = cnt->sticky == 0u && colours_ok;
+    if (gid == 0 && !colours_ok) atomicOr(&cnt->sticky, STICKY_COLOURS);
+    if (ok) {
         if (X.ckey) {
             const uint32_t m = cnt->cache_m[X.cset];
             for (uint32_t i = gid; i < m; i += gsz) X.ckey[X.cpos[i]] = EMPTY_KEY;
         }
-        if (X.list) isl_list_part(B, X.skey, X.list, cap, cnt, gid, gsz);
+        if (X.col_off || X.list) sorted_keys_part(B, X.skey, X.col_off, X.list, cap, cnt, gid, gsz);
     }
     uint32_t hits = 0;
     unsigned long long bytes = 0;
-    // a failed step must not touch the impulse buffers
-    const uint32_t M = cnt->sticky == 0u ? cnt->m_eff : 0u;
+    const uint32_t M = ok ? cnt->m_eff : 0u;
     for (uint32_t s = gid; s < M; s += gsz) {
         uint32_t h = 0;
         unsigned long long b = 0;
@@ -2687,14 +2733,16 @@ __global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, Solv
 // Previous-step lookup table for the warm start: (a, b) -> slot.
 // Each entry's table position goes to pos[slot], so that the next build of this table
 // clears only the entries it holds (cache_m of them) instead of the whole table.
-__global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counters* cnt,
-                                                     uint64_t* __restrict__ key, uint32_t* __restrict__ slot,
-                                                     uint32_t mask, uint32_t* __restrict__ np_out,
-                                                     uint32_t* __restrict__ pos, uint32_t set) {
-    if (cnt->sticky != 0u) return;
+__device__ __forceinline__ void cache_build_part(const SolverRows& R, Counters* cnt, const CacheTable& C,
+                                                 uint32_t gid, uint32_t gsz) {
+    uint64_t* __restrict__ key = C.key;
+    uint32_t* __restrict__ slot = C.slot;
+    uint32_t* __restrict__ np_out = C.np_out;
+    uint32_t* __restrict__ pos = C.pos;
+    const uint32_t mask = C.mask;
     const uint32_t M = cnt->m_eff;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt->cache_m[set] = M;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+    if (gid == 0) cnt->cache_m[C.set] = M;
+    for (uint32_t s = gid; s < M; s += gsz) {
         np_out[s] = R.hdr[s].z;
         const uint4 h = R.hdr[s];
         const uint64_t k = ((uint64_t)h.x << 32) | h.y;
@@ -2712,6 +2760,10 @@ __global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counter
         }
         if (!placed) atomicOr(&cnt->sticky, STICKY_INTERNAL);  // table full: never expected (sized >= 2x the slots)
     }
+}
+__global__ void __launch_bounds__(256) k_cache_build(const SolverRows R, Counters* cnt, CacheTable C) {
+    if (cnt->sticky != 0u) return;
+    cache_build_part(R, cnt, C, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // Moves the entries of an (a, b) -> slot table into a larger empty one.
@@ -2742,26 +2794,16 @@ __global__ void __launch_bounds__(256) k_cache_rehash(const uint64_t* __restrict
 
 
 // Clears the per-step part of the control block.
-__global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ kinds) {
-    uint32_t* w = reinterpret_cast<uint32_t*>(cnt);
-    if (threadIdx.x < COUNTERS_RESET_WORDS) w[threadIdx.x] = 0u;
-    for (uint32_t k = threadIdx.x; k < SPREAD; k += blockDim.x) {
-        cnt->sp[k].islands = 0u;
-        cnt->sp[k].sleeping = 0u;
-        cnt->sp[k].contacts = 0u;
-        cnt->sp[k].warm_hits = 0u;
-        cnt->sp[k].cells = 0u;
-        cnt->sp[k].pgs_bytes = 0ull;
-    }
-    if (kinds && threadIdx.x < 2u * PAIR_KINDS) kinds[threadIdx.x] = 0u;
-}
+__global__ void k_counters_begin(Counters* cnt, uint32_t* __restrict__ kinds) { counters_reset(cnt, kinds); }
 
 // Integration is skipped while any sticky error is set, so an overflowing step
 // leaves the state exactly as it was.
 __global__ void __launch_bounds__(256) k_integrate(Bodies B, StepParams sp, uint32_t n, Counters* cnt,
-                                                   uint32_t sweeps) {
+                                                   uint32_t sweeps, SolverRows R, CacheTable C) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (cnt->sticky != 0u) return;
+    // the step's (a, b) -> slot table for the next step's warm start, in the same launch
+    if (C.key) cache_build_part(R, cnt, C, i, gridDim.x * blockDim.x);
     if (i == 0) {
         cnt->steps_done += 1u;
         // sweeps of the large islands, and of the step (the most any island ran)
@@ -2903,12 +2945,14 @@ void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, const CellTable& ct,
-                  hipStream_t s) {
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, Counters* cnt, uint32_t* kinds,
+                  const CellTable& ct, hipStream_t s) {
     const uint32_t m = std::max(n, ct.mbits ? ct.mlist_n : 0u);
     if (m)
         hipLaunchKernelGGL(k_begin, grid_for(m), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg, cnt,
-                           ct);
+                           kinds, ct);
+    else
+        launch_counters_begin(cnt, kinds, s);
 }
 void launch_fill_u64(uint64_t* p, uint64_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u64, grid_for(n), dim3(256), 0, s, p, v, n);
@@ -3115,7 +3159,8 @@ __global__ void k_cache_clear(uint64_t* __restrict__ key, const uint32_t* __rest
 void launch_cache_build(const SolverRows& R, Counters* cnt, uint32_t cap, uint64_t* key, uint32_t* slot,
                         uint32_t mask, uint32_t* np_out, uint32_t* pos, uint32_t set, hipStream_t s, bool clear) {
     if (clear) hipLaunchKernelGGL(k_cache_clear, grid_bounded(cap), dim3(256), 0, s, key, pos, cnt, set);
-    hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt, key, slot, mask, np_out, pos, set);
+    hipLaunchKernelGGL(k_cache_build, grid_bounded(cap), dim3(256), 0, s, R, cnt,
+                       CacheTable{key, slot, mask, np_out, pos, set});
 }
 void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t on, uint64_t* key, uint32_t* slot,
                          uint32_t mask, Counters* cnt, uint32_t* pos, hipStream_t s) {
@@ -3123,8 +3168,10 @@ void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t o
         hipLaunchKernelGGL(k_cache_rehash, grid_for(on), dim3(256), 0, s, okey, oslot, on, key, slot, mask, cnt, pos);
 }
 void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_integrate, grid_for(n), dim3(256), 0, s, B, sp, n, cnt, sweeps);
+                      hipStream_t s, const SolverRows* R, const CacheTable* C) {
+    const uint32_t g = std::max(n, 1u);
+    hipLaunchKernelGGL(k_integrate, grid_for(g), dim3(256), 0, s, B, sp, n, cnt, sweeps, R ? *R : SolverRows{},
+                       C ? *C : CacheTable{});
 }
 
 void launch_isl_union(const Bodies& B, uint32_t n, const uint2* m_ab, uint32_t cap_man, Counters* cnt,

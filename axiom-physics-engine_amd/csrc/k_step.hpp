@@ -46,9 +46,10 @@ struct PrevCacheArgs {
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 // Start of a step: velocities, world inertia, AABBs, cell keys; clears the
 // colouring's per-body words.
+// Also resets the step's counters (as launch_counters_begin).
 void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
-                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, const Counters* cnt, const CellTable& ct,
-                  hipStream_t s);
+                  uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, Counters* cnt, uint32_t* kinds,
+                  const CellTable& ct, hipStream_t s);
 // Islands of the step, after the narrowphase: union-find over the manifolds between awake
 // dynamic bodies (root = smallest body index), and wake requests for sleeping islands an
 // awake body touched.
@@ -102,13 +103,16 @@ void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Work the row preparation launch also does for the launches after it, each in a loop of
-// its own over the grid: the small islands' work list from the sorted keys (when list is
-// set; launch_pgs_local then builds none), clearing zero_n words (the sweep's residual
-// words), and emptying the (a, b) -> slot table of row set cset (the entries its last
-// build recorded, at the positions cpos holds; when ckey is set, launch_cache_build is
-// then called with clear = false).
+// its own over the grid: the colour offsets of the sorted keys skey (as launch_col_offsets,
+// when col_off is set), the small islands' work list from the same keys (when list is set;
+// launch_pgs_local then builds none), clearing zero_n words (the sweep's residual words),
+// and emptying the (a, b) -> slot table of row set cset (the entries its last build
+// recorded, at the positions cpos holds; when ckey is set, the table's build is then made
+// without clearing it).  More than max_k colours fail the step as launch_col_offsets does.
 struct PrepExtra {
     const uint32_t* skey = nullptr;
+    uint32_t* col_off = nullptr;
+    uint32_t max_k = 0xFFFFFFFFu;
     uint2* list = nullptr;
     uint32_t* zero = nullptr;
     uint32_t zero_n = 0;
@@ -157,6 +161,15 @@ void launch_pgs(const Bodies& B, const StepParams& sp, const SolverRows& R, uint
                 uint32_t first, hipStream_t s);
 void launch_residual(const Bodies& B, const StepParams& sp, const SolverRows& R, const Counters* cnt, uint32_t cap,
                      uint32_t* out_bits, hipStream_t s);
+// An (a, b) -> slot table of one row set, for its build.
+struct CacheTable {
+    uint64_t* key = nullptr;
+    uint32_t* slot = nullptr;
+    uint32_t mask = 0;
+    uint32_t* np_out = nullptr;
+    uint32_t* pos = nullptr;
+    uint32_t set = 0;
+};
 // Empties the (a, b) -> slot table of row set `set` (the entries its last build made, at
 // the positions pos holds; skipped with clear = false, when the row preparation did it)
 // and fills it with the step's slots.
@@ -169,6 +182,9 @@ void launch_cache_rehash(const uint64_t* okey, const uint32_t* oslot, uint32_t o
                          uint32_t mask, Counters* cnt, uint32_t* pos, hipStream_t s);
 // sweeps == SWEEPS_ON_DEVICE: the count is read from Counters::sweeps_run.
 constexpr uint32_t SWEEPS_ON_DEVICE = 0xFFFFFFFFu;
-void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps, hipStream_t s);
+// With R and C, the same launch also builds the (a, b) -> slot table C of the step's rows
+// (as launch_cache_build with clear = false).
+void launch_integrate(const Bodies& B, const StepParams& sp, uint32_t n, Counters* cnt, uint32_t sweeps, hipStream_t s,
+                      const SolverRows* R = nullptr, const CacheTable* C = nullptr);
 
 }  // namespace axm

@@ -558,7 +558,7 @@ struct StepOpts {
 void enqueue_broad(axm_world_s* w, const StepParams& sp) {
     const uint32_t n = w->n;
     hipStream_t s = w->stream;
-    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, w->cnt, w->ct, s);
+    launch_begin(w->B, sp, n, w->key, w->val, w->bmax0, w->bmax1, w->bmask, w->bdone, w->cnt, w->pkinds, w->ct, s);
     const int end_bit = 30 + bits_for(w->n_worlds > 1 ? w->n_worlds - 1 : 0);
     const int alt = radix_sort_u64(w->key, w->val, w->key_alt, w->val_alt, n, end_bit, w->sort_tmp, s, nullptr,
                                    &w->cnt->sticky);
@@ -604,8 +604,7 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const SolverRows& R = w->R[c];
     const uint32_t rc = w->rcap[c];
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[0], s));
-    launch_counters_begin(w->cnt, w->pkinds, s);
-    enqueue_broad(w, sp);
+    enqueue_broad(w, sp);  // its first launch resets the step's counters
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[1], s));
     if (o.flags & AXM_STEP_KEEP_STAGES) {
         AXM_HIP(hipMemcpyAsync(w->pre_v, w->B.sv, 16ull * n, hipMemcpyDeviceToDevice, s));
@@ -639,14 +638,15 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     const uint32_t* order = split ? w->cval : w->cval_alt;
     w->solve_order = order;
     w->solve_keys = scol;
-    launch_col_offsets(scol, MAX_SORT_COLOURS, w->col_off, w->cnt, s);
     // ---- solver
     if (w->desc.warm_start == AXM_WARM_NONE) pca.valid = 0;
-    // the same launch builds the small islands' work list, clears the residual words (the
-    // per-sweep residuals, then the check arrivals) and empties this row set's (a, b) -> slot
-    // table, which the step's cache build refills
+    // the same launch forms the colour offsets, builds the small islands' work list, clears
+    // the residual words (the per-sweep residuals, then the check arrivals) and empties this
+    // row set's (a, b) -> slot table, which the step's integration launch refills
     PrepExtra X;
     X.skey = scol;
+    X.col_off = w->col_off;
+    X.max_k = MAX_SORT_COLOURS;
     X.list = split ? w->isl_list : nullptr;
     X.zero = w->res_bits;
     X.zero_n = (uint32_t)res_words(w->res_cap);
@@ -776,11 +776,11 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_iters = it;
     if (!post) return AXM_SUCCESS;
     w->last_dt = dt;
-    // the step's row preparation emptied the table
-    launch_cache_build(R, w->cnt, rc, w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], w->pc_pos[c],
-                       (uint32_t)c, s, false);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[3], s));
-    launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s);
+    // the integration launch also builds the step's (a, b) -> slot table (the step's row
+    // preparation emptied it)
+    const CacheTable ctab{w->pc_key[c], w->pc_slot[c], w->pc_mask[c], w->np_slot[c], w->pc_pos[c], (uint32_t)c};
+    launch_integrate(w->B, sp, n, w->cnt, w->last_iters_dev ? SWEEPS_ON_DEVICE : it, s, &R, &ctab);
     launch_islands(w->B, sp, n, w->cnt, s);
     if (o.ev) AXM_HIP(hipEventRecord(o.ev[4], s));
     if (w->prof_on && !o.capture) {
