@@ -847,6 +847,75 @@ __device__ __forceinline__ bool prev_find(const PrevCache& c, uint64_t k, uint32
     return false;
 }
 
+// Union-find over body indices.  Every link points from a larger root to a smaller one
+// (compare-and-swap on the root), so the root of a finished component is its smallest
+// body index whatever the order of the links (lock-free union-find with path halving:
+// Jaiganesh & Burtscher 2018, ECL-CC).  The parents are shared between workgroups, so
+// they move through agent-scope atomics.
+__device__ __forceinline__ uint32_t uf_load(uint32_t* p, uint32_t x) {
+    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
+    uint32_t cur = uf_load(par, x);
+    if (cur != x) {
+        uint32_t prev = x, next;
+        while (cur > (next = uf_load(par, cur))) {
+            __hip_atomic_store(par + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            prev = cur;
+            cur = next;
+        }
+    }
+    return cur;
+}
+// Root of x in the finished forest (after the union kernel, which no later kernel of the
+// step changes): the union's coherent loads, without its path-halving stores (in one giant
+// island every walk ends on the same root word, which stores would keep contended).
+__device__ __forceinline__ uint32_t uf_root(uint32_t* par, uint32_t x) {
+    uint32_t p;
+    while ((p = uf_load(par, x)) != x) x = p;
+    return x;
+}
+// Island (union-find root) of a manifold: that of its dynamic body, UNSET with none.
+__device__ __forceinline__ uint32_t man_root(const Bodies& B, uint2 ab) {
+    const bool plane = (ab.y & PLANE_BIT) != 0u;
+    const uint32_t d = B.sv[ab.x].w > 0.0f ? ab.x : (!plane && B.sv[ab.y].w > 0.0f ? ab.y : UNSET);
+    return d == UNSET ? UNSET : uf_root(B.upar, d);
+}
+// Per-island atomics of a wave: the lanes whose island is that of the first lane taking
+// part combine their values into one atomic made by that lane; the others make their own.
+// In one giant island a wave makes one atomic instead of 64 on the same word.
+template <class Op>
+__device__ __forceinline__ void wave_by_island(bool on, uint32_t r, uint32_t v, uint32_t idle, Op op) {
+    const uint64_t act = __ballot(on);
+    if (act == 0ull) return;
+    const uint32_t lead = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+    const uint32_t lr = (uint32_t)__shfl((int)r, (int)lead, 64);
+    const bool same = on && r == lr;
+    uint32_t c = same ? v : idle;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c = op.combine(c, (uint32_t)__shfl_xor((int)c, o, 64));
+    if (__lane_id() == lead) op.apply(lr, c);
+    if (on && !same) op.apply(r, v);
+}
+// Manifolds per island (after the union kernel), run by the colouring's first launch
+// (k_colour_keep), every thread of its grid.
+__device__ __forceinline__ void isl_count_part(const Bodies& B, const uint2* __restrict__ m_ab, uint32_t cap,
+                                               const Counters* __restrict__ cnt) {
+    if (cnt->sticky != 0u) return;
+    const uint32_t M = min(cap, cnt->m_eff);
+    struct Add {
+        uint32_t* icnt;
+        __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a + b; }
+        __device__ void apply(uint32_t r, uint32_t c) const { atomicAdd(icnt + r, c); }
+    };
+    // every lane of a wave runs the same number of rounds (the wave-wide combine)
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t m0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); m0 < M; m0 += stride) {
+        const uint32_t m = m0 + __lane_id();
+        const uint32_t r = m < M ? man_root(B, m_ab[m]) : UNSET;
+        wave_by_island(r != UNSET, r, 1u, 0u, Add{B.icnt});
+    }
+}
 // Pass 1 (one thread per manifold, no barrier): degrees, priorities, the kept
 // colours of persisting contacts (with their mask bits) and, for every other
 // manifold, its round-0 tag at its dynamic bodies and an entry in the list of
@@ -859,7 +928,10 @@ __global__ void __launch_bounds__(256) k_colour_keep(Bodies B, const uint2* __re
                                                      uint32_t* __restrict__ col, uint64_t* __restrict__ prio,
                                                      uint64_t* __restrict__ bmax0, uint64_t* __restrict__ bmask,
                                                      uint32_t* __restrict__ bdeg, uint32_t* __restrict__ list,
-                                                     Counters* cnt, PrevCache pc, uint32_t recolour_period) {
+                                                     Counters* cnt, PrevCache pc, uint32_t recolour_period,
+                                                     uint32_t count_cap) {
+    // with the island split, the manifolds per island first (count_cap: the manifold capacity)
+    if (count_cap) isl_count_part(B, m_ab, count_cap, cnt);
     const uint32_t M = cnt->m_eff;
     const bool incremental = pc.valid != 0u && (cnt->steps_done % recolour_period) != 0u;
     // per workgroup, the largest kept colour and the list entries are gathered in LDS
@@ -1045,74 +1117,6 @@ __global__ void __launch_bounds__(256) k_colour(Bodies B, const uint2* __restric
     }
 }
 
-// Union-find over body indices.  Every link points from a larger root to a smaller one
-// (compare-and-swap on the root), so the root of a finished component is its smallest
-// body index whatever the order of the links (lock-free union-find with path halving:
-// Jaiganesh & Burtscher 2018, ECL-CC).  The parents are shared between workgroups, so
-// they move through agent-scope atomics.
-__device__ __forceinline__ uint32_t uf_load(uint32_t* p, uint32_t x) {
-    return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t uf_find(uint32_t* par, uint32_t x) {
-    uint32_t cur = uf_load(par, x);
-    if (cur != x) {
-        uint32_t prev = x, next;
-        while (cur > (next = uf_load(par, cur))) {
-            __hip_atomic_store(par + prev, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            prev = cur;
-            cur = next;
-        }
-    }
-    return cur;
-}
-// Root of x in the finished forest (after the union kernel, which no later kernel of the
-// step changes): the union's coherent loads, without its path-halving stores (in one giant
-// island every walk ends on the same root word, which stores would keep contended).
-__device__ __forceinline__ uint32_t uf_root(uint32_t* par, uint32_t x) {
-    uint32_t p;
-    while ((p = uf_load(par, x)) != x) x = p;
-    return x;
-}
-// Island (union-find root) of a manifold: that of its dynamic body, UNSET with none.
-__device__ __forceinline__ uint32_t man_root(const Bodies& B, uint2 ab) {
-    const bool plane = (ab.y & PLANE_BIT) != 0u;
-    const uint32_t d = B.sv[ab.x].w > 0.0f ? ab.x : (!plane && B.sv[ab.y].w > 0.0f ? ab.y : UNSET);
-    return d == UNSET ? UNSET : uf_root(B.upar, d);
-}
-// Per-island atomics of a wave: the lanes whose island is that of the first lane taking
-// part combine their values into one atomic made by that lane; the others make their own.
-// In one giant island a wave makes one atomic instead of 64 on the same word.
-template <class Op>
-__device__ __forceinline__ void wave_by_island(bool on, uint32_t r, uint32_t v, uint32_t idle, Op op) {
-    const uint64_t act = __ballot(on);
-    if (act == 0ull) return;
-    const uint32_t lead = (uint32_t)__ffsll((unsigned long long)act) - 1u;
-    const uint32_t lr = (uint32_t)__shfl((int)r, (int)lead, 64);
-    const bool same = on && r == lr;
-    uint32_t c = same ? v : idle;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) c = op.combine(c, (uint32_t)__shfl_xor((int)c, o, 64));
-    if (__lane_id() == lead) op.apply(lr, c);
-    if (on && !same) op.apply(r, v);
-}
-// Manifolds per island (after the union kernel).
-__global__ void __launch_bounds__(256) k_isl_count(Bodies B, const uint2* __restrict__ m_ab, uint32_t cap,
-                                                   const Counters* __restrict__ cnt) {
-    if (cnt->sticky != 0u) return;
-    const uint32_t M = min(cap, cnt->m_eff);
-    struct Add {
-        uint32_t* icnt;
-        __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a + b; }
-        __device__ void apply(uint32_t r, uint32_t c) const { atomicAdd(icnt + r, c); }
-    };
-    // every lane of a wave runs the same number of rounds (the wave-wide combine)
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t m0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); m0 < M; m0 += stride) {
-        const uint32_t m = m0 + __lane_id();
-        const uint32_t r = m < M ? man_root(B, m_ab[m]) : UNSET;
-        wave_by_island(r != UNSET, r, 1u, 0u, Add{B.icnt});
-    }
-}
 // Colour sort input: values = manifold index, keys = colour, or with the island split
 // LOCAL_KEY | root << 8 | colour for a manifold of an island of at most ISLAND_LOCAL_MAX
 // manifolds.  A stable sort then gives the solve order: the large islands' manifolds by
@@ -1836,12 +1840,14 @@ __device__ __forceinline__ uint32_t next_colour(unsigned long long mk, uint32_t 
 // Before a launch, per slot: the next visit of each dynamic body, and the inbox
 // words: a body's first visit (rank 0) gets the body's solver velocity with
 // version 0, every other inbox a version no visit waits for.
-__global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const uint64_t* __restrict__ bmask,
-                                                   const Counters* __restrict__ cnt) {
+// (Thread gid of gsz; also run by the small islands' workgroup kernel, k_pgs_group, whose
+// bodies and slots are disjoint from the large islands'.)
+__device__ __forceinline__ void flow_init_part(const Bodies& B, const SolverRows& R, const uint64_t* __restrict__ bmask,
+                                               const Counters* __restrict__ cnt, uint32_t gid, uint32_t gsz) {
     // the rows exist only when this step's preparation ran
     if (cnt->sticky != 0u) return;
     const uint32_t M = cnt->m_glob;  // the large islands' slots
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
+    for (uint32_t s = gid; s < M; s += gsz) {
     const uint4 h = R.hdr[s];
     const uint2 dp = R.dep[s];
     const uint32_t c = R.colour[s];
@@ -1872,6 +1878,10 @@ __global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const
     q[2] = make_uint4((uint32_t)ob[0], (uint32_t)(ob[0] >> 32), (uint32_t)ob[1], (uint32_t)(ob[1] >> 32));
     q[3] = make_uint4((uint32_t)ob[2], (uint32_t)(ob[2] >> 32), (uint32_t)ob[3], (uint32_t)(ob[3] >> 32));
     }
+}
+__global__ void __launch_bounds__(256) k_flow_init(Bodies B, SolverRows R, const uint64_t* __restrict__ bmask,
+                                                   const Counters* __restrict__ cnt) {
+    flow_init_part(B, R, bmask, cnt, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 // After a launch: each dynamic body's final velocity is in the inbox of its first
 // visit (the last visit of the last sweep passed it on there).
@@ -2681,8 +2691,12 @@ __device__ __forceinline__ void block_max_sum(uint32_t& mx, uint32_t& sum, uint3
 // velocities move through global memory, visible within the workgroup after the barrier).
 __global__ void __launch_bounds__(256) k_pgs_group(Bodies B, StepParams sp, SolverRows R, Counters* cnt, uint32_t cap,
                                                    const uint2* __restrict__ list, const uint32_t* __restrict__ skey,
-                                                   float tol, uint32_t max_sweeps) {
+                                                   float tol, uint32_t max_sweeps,
+                                                   const uint64_t* __restrict__ flow_bmask) {
     __shared__ uint32_t red[8];
+    // with flow_bmask, the large islands' sweep launch is prepared here (as k_flow_init)
+    if (flow_bmask) flow_init_part(B, R, flow_bmask, cnt, blockIdx.x * blockDim.x + threadIdx.x,
+                                   gridDim.x * blockDim.x);
     if (cnt->sticky != 0u) return;
     const uint32_t n = cnt->isl_group_n, t = threadIdx.x;
     const bool tol_mode = tol > 0.0f;
@@ -3047,10 +3061,10 @@ static PrevCache make_prev(const PrevCacheArgs& pca) {
 }
 void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_t* col, uint64_t* prio,
                    uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
-                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s) {
+                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, uint32_t split, hipStream_t s) {
     constexpr uint32_t SMALL = 16384;  // longest list coloured by one workgroup
     hipLaunchKernelGGL(k_colour_keep, grid_bounded(cap_man), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmask, bdeg, list,
-                       cnt, make_prev(prev), RECOLOUR_PERIOD);
+                       cnt, make_prev(prev), RECOLOUR_PERIOD, split ? cap_man : 0u);
     hipLaunchKernelGGL(k_colour_small, dim3(1), dim3(1024), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, cnt,
                        65000u, SMALL);
     hipLaunchKernelGGL(k_colour, dim3(blocks), dim3(256), 0, s, B, m_ab, col, prio, bmax0, bmax1, bmask, list, bar,
@@ -3058,7 +3072,6 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
 }
 void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
                            uint32_t* val, const Counters* cnt, uint32_t split, hipStream_t s) {
-    if (split) hipLaunchKernelGGL(k_isl_count, grid_bounded(cap), dim3(256), 0, s, B, m_ab, cap, cnt);
     hipLaunchKernelGGL(k_col_sort_input, grid_bounded(cap), dim3(256), 0, s, B, m_ab, col, cap, key, val, cnt, split);
 }
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s) {
@@ -3066,9 +3079,10 @@ void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Cou
 }
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
-                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const uint64_t* flow_bmask) {
     if (max_sweeps == 0u) return;
-    hipLaunchKernelGGL(k_pgs_group, dim3(groups), dim3(256), 0, s, B, sp, R, cnt, cap, list, skey, tol, max_sweeps);
+    hipLaunchKernelGGL(k_pgs_group, dim3(groups), dim3(256), 0, s, B, sp, R, cnt, cap, list, skey, tol, max_sweeps,
+                       flow_bmask);
     const uint32_t windows = (cap + BUNDLE_SLOTS - 1u) / BUNDLE_SLOTS;
     if (ev0) (void)hipEventRecord(ev0, s);
     hipLaunchKernelGGL(k_pgs_bundle, dim3(std::max(1u, std::min(windows, 16384u))), dim3(64), BUNDLE_LDS_BYTES, s, B,
@@ -3131,9 +3145,9 @@ int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes) {
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
                      uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
                      uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
-                     hipEvent_t ev0, hipEvent_t ev1, const SweepStop& stop) {
+                     hipEvent_t ev0, hipEvent_t ev1, const SweepStop& stop, bool inited) {
     if (!nsweeps) return;
-    hipLaunchKernelGGL(k_flow_init, grid_bounded(cap), dim3(256), 0, s, B, R, bmask, cnt);
+    if (!inited) hipLaunchKernelGGL(k_flow_init, grid_bounded(cap), dim3(256), 0, s, B, R, bmask, cnt);
     if (ev0) (void)hipEventRecord(ev0, s);
     hipLaunchKernelGGL(k_pgs_own, dim3(groups ? groups : 1u), dim3(OWN_WAVE), lds_bytes, s, B, sp, R, cnt, cap, sweep0,
                        nsweeps, lds_bytes / (uint32_t)sizeof(float4), spin_limit, tune, bmask, stop);

@@ -90,18 +90,23 @@ int colour_blocks_per_cu();
 // valid = a previous step exists) enables the incremental form between full recolourings.
 // Colouring: the keep pass, then Jones-Plassmann rounds over the manifolds not
 // kept (`list`), in one workgroup for a short list or on the whole grid.
+// With split, the first launch also counts the manifolds per island (icnt, after the union
+// kernel), which the colour sort input of the island split needs.
 void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_t* col, uint64_t* prio,
                    uint64_t* bmax0, uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, uint32_t* list, GridBar* bar,
-                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, hipStream_t s);
+                   Counters* cnt, uint32_t blocks, const PrevCacheArgs& prev, uint32_t split, hipStream_t s);
 void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
                            uint32_t* val, const Counters* cnt, uint32_t split,
                            hipStream_t s);
 void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);
 // The islands solved on their own (after the row preparation, which built their work
 // list): one lane or one workgroup per island; tol > 0 stops each at its own residual.
+// With flow_bmask (the colouring's per-body colour masks), its first launch also sets up
+// the large islands' sweep (as launch_pgs_flow does unless told it is inited).
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,
-                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                      const uint64_t* flow_bmask = nullptr);
 // Work the row preparation launch also does for the launches after it, each in a loop of
 // its own over the grid: the colour offsets of the sorted keys skey (as launch_col_offsets,
 // when col_off is set), the small islands' work list from the same keys (when list is set;
@@ -147,11 +152,14 @@ struct SweepStop {
 };
 // Persistent sweep: `groups` workgroups of one wave, each with `lds_bytes` of LDS for
 // its staged rows; all of them must be resident at once (see pgs_own_setup).  Events
-// ev0 / ev1, when given, are recorded around the sweep kernel itself.
+// ev0 / ev1, when given, are recorded around the sweep kernel itself.  With inited, the
+// inboxes and next visits were already set up by this step's launch_pgs_local (given
+// bmask as flow_bmask).
 void launch_pgs_flow(const Bodies& B, const StepParams& sp, const SolverRows& R, uint32_t cap, const uint64_t* bmask,
                      uint32_t sweep0, uint32_t nsweeps, Counters* cnt,
                      uint32_t spin_limit, uint32_t groups, uint32_t lds_bytes, uint32_t tune, hipStream_t s,
-                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const SweepStop& stop = SweepStop{});
+                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, const SweepStop& stop = SweepStop{},
+                     bool inited = false);
 // Raises the sweep kernel's dynamic-LDS limit to max_lds_bytes and returns how many of
 // its workgroups with lds_bytes each fit on one CU (0 on error).
 int pgs_own_setup(uint32_t max_lds_bytes, uint32_t lds_bytes);

@@ -627,9 +627,9 @@ axm_status enqueue_pre(axm_world_s* w, const StepParams& sp, const StepOpts& o) 
     // large ones, island by island)
     launch_isl_union(w->B, n, w->m_ab, w->cap_man, w->cnt, s);
     // bdone is the per-body manifold count of the colouring (cleared by k_begin)
-    launch_colour(w->B, w->m_ab, w->cap_man, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->clist,
-                  w->bar, w->cnt, w->colour_blocks, pca, s);
     const uint32_t split = island_split(w);
+    launch_colour(w->B, w->m_ab, w->cap_man, w->col, w->prio, w->bmax0, w->bmax1, w->bmask, w->bdone, w->clist,
+                  w->bar, w->cnt, w->colour_blocks, pca, split, s);
     launch_col_sort_input(w->B, w->m_ab, w->col, w->cap_man, w->ckey, w->cval, w->cnt, split, s);
     radix_sort_u32(w->ckey, w->cval, w->ckey_alt, w->cval_alt, w->cap_man, split ? 32 : 8, w->csort_tmp, s,
                    &w->cnt->m_eff, &w->cnt->sticky);
@@ -697,8 +697,10 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
     w->last_iters_dev = dev_checks;
     std::vector<uint32_t> off;
     if (pre) AXM_TRY(enqueue_pre(w, sp, o));
+    // the islands solved on their own run first; their launch also sets up the large
+    // islands' sweep launch (its inboxes and next visits)
+    const bool flow_inited = sweep && island_split(w) && !o.colour_launches;
     if (sweep && island_split(w)) {
-        // the islands solved on their own, before the large ones' sweeps
         hipEvent_t b0 = nullptr, b1 = nullptr;
         if (w->prof_on && !o.capture) {
             AXM_TRY(bundle_events(w, 0));
@@ -706,7 +708,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             b1 = w->bprof_ev[1];
         }
         launch_pgs_local(w->B, sp, R, w->cnt, rc, w->solve_keys, w->isl_list, w->desc.residual_tol, max_it,
-                         2u * (uint32_t)w->cu_count, s, b0, b1);
+                         2u * (uint32_t)w->cu_count, s, b0, b1, flow_inited ? w->bmask : nullptr);
     }
     const bool want_res = (o.flags & AXM_STEP_RESIDUAL) || w->desc.record_residual || tol_mode;
     const uint32_t every = std::max<uint32_t>(w->desc.residual_every, 1u);
@@ -727,7 +729,7 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
         AXM_TRY(prof_event(w, nev, o.capture, e0));
         AXM_TRY(prof_event(w, nev, o.capture, e1));
         launch_pgs_flow(w->B, sp, R, rc, w->bmask, 0, max_it, w->cnt, SWEEP_SPIN_LIMIT_US, w->sweep_groups,
-                        w->sweep_lds, w->sweep_tune, s, e0, e1, stop);
+                        w->sweep_lds, w->sweep_tune, s, e0, e1, stop, flow_inited);
         it = max_it;  // at most; the count the launch ran is read back with the counters
     }
     bool all_local = false;
@@ -753,7 +755,8 @@ axm_status enqueue_step(axm_world_s* w, float dt, const StepOpts& o) {
             AXM_TRY(prof_event(w, nev, o.capture, e0));
             AXM_TRY(prof_event(w, nev, o.capture, e1));
             launch_pgs_flow(w->B, sp, R, rc, w->bmask, it, seg, w->cnt, SWEEP_SPIN_LIMIT_US,
-                            w->sweep_groups, w->sweep_lds, w->sweep_tune, s, e0, e1);
+                            w->sweep_groups, w->sweep_lds, w->sweep_tune, s, e0, e1, SweepStop{},
+                            flow_inited && it == 0u);
         }
         it += seg;
         const bool last = it == max_it;
@@ -1352,10 +1355,10 @@ axm_status axm_world_run(axm_world w, float dt, uint32_t iters, uint32_t steps, 
                 if (split)
                     launch_pgs_local(w->B, sp, w->R[c], w->cnt, w->rcap[c], w->solve_keys, w->isl_list,
                                      w->desc.residual_tol, nsw, 2u * (uint32_t)w->cu_count, w->stream,
-                                     w->bprof_ev[2 * c], w->bprof_ev[2 * c + 1]);
+                                     w->bprof_ev[2 * c], w->bprof_ev[2 * c + 1], w->bmask);
                 launch_pgs_flow(w->B, sp, w->R[c], w->rcap[c], w->bmask, 0, nsw, w->cnt, SWEEP_SPIN_LIMIT_US,
                                 w->sweep_groups, w->sweep_lds, w->sweep_tune, w->stream, w->gprof_ev[2 * c],
-                                w->gprof_ev[2 * c + 1], checks ? stop : SweepStop{});
+                                w->gprof_ev[2 * c + 1], checks ? stop : SweepStop{}, split);
                 AXM_HIP(hipGraphLaunch(w->gpost[c], w->stream));
             } else {
                 AXM_HIP(hipGraphLaunch(w->gexec[c], w->stream));
