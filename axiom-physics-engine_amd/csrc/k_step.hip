@@ -57,7 +57,7 @@ __device__ __forceinline__ void counters_reset(Counters* cnt, uint32_t* __restri
 }
 // ---------------------------------------------------------------- k_begin
 __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t n, uint64_t* __restrict__ key,
-                                               uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
+                                               uint32_t* __restrict__ key32, uint32_t* __restrict__ val, uint64_t* __restrict__ bmax0,
                                                uint64_t* __restrict__ bmax1, uint64_t* __restrict__ bmask,
                                                uint32_t* __restrict__ bdeg, Counters* __restrict__ cnt,
                                                uint32_t* __restrict__ kinds, CellTable ct) {
@@ -130,7 +130,8 @@ __global__ void __launch_bounds__(256) k_begin(Bodies B, StepParams sp, uint32_t
         return (uint32_t)floorf(f);
     };
     const uint32_t cx = cc(c.x, sp.ox), cy = cc(c.y, sp.oy), cz = cc(c.z, sp.oz);
-    key[i] = ((uint64_t)B.world[i] << 30) | morton3(cx, cy, cz);
+    if (key32) key32[i] = morton3(cx, cy, cz);  // one world: the Morton code alone
+    else key[i] = ((uint64_t)B.world[i] << 30) | morton3(cx, cy, cz);
     val[i] = i;
 }
 
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(256) k_cell_clear(CellTable t) {
 // after its home with no empty slot between (the linear-probing invariant).
 // Each thread also gathers the AABB of sorted body s into smin / smax (body index in
 // smin.w), so that the candidates of a cell are contiguous and carry what the test needs.
-__global__ void __launch_bounds__(256) k_cell_table(Bodies B, const uint64_t* __restrict__ skey,
+__global__ void __launch_bounds__(256) k_cell_table(Bodies B, SortedKeys skey,
                                                     const uint32_t* __restrict__ sval, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(256) k_cell_table(Bodies B, const uint64_t* __
 __device__ __forceinline__ void decide_mode(Counters* cnt, uint32_t n) {
     cnt->bp_sparse = (uint64_t)spread_sum(cnt, &SpreadRow::cells) * 4u > n ? 1u : 0u;
 }
-__global__ void __launch_bounds__(256) k_cell_rank(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+__global__ void __launch_bounds__(256) k_cell_rank(SortedKeys skey, uint32_t n, CellTable t,
                                                    Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0u) decide_mode(cnt, n);
@@ -261,7 +262,7 @@ __global__ void __launch_bounds__(256) k_cell_rank(const uint64_t* __restrict__ 
     }
     if (s == n - 1u) t.cstart[r + (head ? 1u : 0u)] = n;
 }
-__global__ void __launch_bounds__(256) k_cell_place(const uint64_t* __restrict__ skey, uint32_t n, CellTable t,
+__global__ void __launch_bounds__(256) k_cell_place(SortedKeys skey, uint32_t n, CellTable t,
                                                     Counters* __restrict__ cnt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -329,10 +330,10 @@ constexpr int BP_RANK = 0, BP_SPARSE = 1, BP_DENSE = 2;
 // The candidates of sorted body t in z slice dz (-1, 0, 1) of its neighbour cells,
 // in canonical order; the planes come with slice -1.
 template <int MODE, class F>
-__device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
+__device__ __forceinline__ void visit_pairs(const Bodies& B, const StepParams& sp, const SortedKeys& skey,
                                             const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t t,
                                             int dz, F&& f) {
-    const uint64_t key = skey[t];
+    const uint64_t key = MODE == BP_RANK ? (uint64_t)skey.k32[t] : skey.k64[t];
     const float4 mn = B.smin[t], mx = B.smax[t];
     const uint32_t i = __float_as_uint(mn.w);
     const bool dyn_i = mx.w > 0.0f;
@@ -496,7 +497,7 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t& total) {
 // The workgroup's total goes to bsum[block] (block in the XCD-aware order): the emit pass
 // forms each part's offset from the scanned totals and a scan within its workgroup.
 template <int MODE>
-__device__ __forceinline__ void pair_count_body(const Bodies& B, const StepParams& sp, const uint64_t* __restrict__ skey,
+__device__ __forceinline__ void pair_count_body(const Bodies& B, const StepParams& sp, const SortedKeys& skey,
                                                 const uint32_t* __restrict__ sval, const CellTable& ct, uint32_t n,
                                                 uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
                                                 uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
@@ -522,13 +523,13 @@ __device__ __forceinline__ void pair_count_body(const Bodies& B, const StepParam
 // hide more of the probe latency; the dense build keeps its registers.
 template <int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
-k_pair_count(Bodies B, StepParams sp, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+k_pair_count(Bodies B, StepParams sp, SortedKeys skey, const uint32_t* __restrict__ sval,
              CellTable ct, uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
              uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
     pair_count_body<MODE>(B, sp, skey, sval, ct, n, cnt, bsum, stash, ctr);
 }
 __global__ void __launch_bounds__(256)
-k_pair_count_dense(Bodies B, StepParams sp, const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+k_pair_count_dense(Bodies B, StepParams sp, SortedKeys skey, const uint32_t* __restrict__ sval,
                    CellTable ct, uint32_t n, uint32_t* __restrict__ cnt, uint32_t* __restrict__ bsum,
                    uint32_t* __restrict__ stash, const Counters* __restrict__ ctr) {
     pair_count_body<BP_DENSE>(B, sp, skey, sval, ct, n, cnt, bsum, stash, ctr);
@@ -537,7 +538,7 @@ k_pair_count_dense(Bodies B, StepParams sp, const uint64_t* __restrict__ skey, c
 // Part u's pairs start at boff[block] (the scanned workgroup totals; boff[blocks] is the
 // pair count) plus the counts of the parts before it in its workgroup.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, const uint64_t* __restrict__ skey,
+__global__ void __launch_bounds__(256) k_pair_emit(Bodies B, StepParams sp, SortedKeys skey,
                                                    const uint32_t* __restrict__ sval, CellTable ct, uint32_t n,
                                                    const uint32_t* __restrict__ pcnt, const uint32_t* __restrict__ boff,
                                                    uint32_t cap, uint32_t* __restrict__ pa, uint32_t* __restrict__ pb,
@@ -2958,13 +2959,14 @@ static inline dim3 grid_bounded(uint32_t cap, uint32_t max_groups = 1024u) {
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s) {
     hipLaunchKernelGGL(k_counters_begin, dim3(1), dim3(64), 0, s, cnt, kinds);
 }
-void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* key32, uint32_t* val,
+                  uint64_t* bmax0,
                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, Counters* cnt, uint32_t* kinds,
                   const CellTable& ct, hipStream_t s) {
     const uint32_t m = std::max(n, ct.mbits ? ct.mlist_n : 0u);
     if (m)
-        hipLaunchKernelGGL(k_begin, grid_for(m), dim3(256), 0, s, B, sp, n, key, val, bmax0, bmax1, bmask, bdeg, cnt,
-                           kinds, ct);
+        hipLaunchKernelGGL(k_begin, grid_for(m), dim3(256), 0, s, B, sp, n, key, key32, val, bmax0, bmax1, bmask, bdeg,
+                           cnt, kinds, ct);
     else
         launch_counters_begin(cnt, kinds, s);
 }
@@ -2982,7 +2984,7 @@ void launch_cell_clear(const CellTable& t, hipStream_t s) {
     const uint32_t n = std::max(t.mask + 1u, (t.bmask >> 5) + 1u);
     hipLaunchKernelGGL(k_cell_clear, grid_for(n), dim3(256), 0, s, t);
 }
-void launch_cell_table(const Bodies& B, const uint64_t* skey, const uint32_t* sval, uint32_t n, const CellTable& t,
+void launch_cell_table(const Bodies& B, const SortedKeys& skey, const uint32_t* sval, uint32_t n, const CellTable& t,
                        Counters* cnt, uint32_t* scan_tmp, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_cell_table, grid_for(n), dim3(256), 0, s, B, skey, sval, n, t, cnt);
@@ -2993,7 +2995,7 @@ void launch_cell_table(const Bodies& B, const uint64_t* skey, const uint32_t* sv
     }
     hipLaunchKernelGGL(k_cell_place, grid_for(n), dim3(256), 0, s, skey, n, t, cnt);
 }
-void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+void launch_pair_count(const Bodies& B, const StepParams& sp, const SortedKeys& skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* bsum, uint32_t* stash, Counters* ctr,
                        hipStream_t s) {
     if (!n) return;
@@ -3007,7 +3009,7 @@ void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* sk
     hipLaunchKernelGGL(k_pair_count_dense, grid_for(BP_PARTS * n), dim3(256), 0, s, B, sp, skey, sval, ct, n, cnt,
                        bsum, stash, ctr);
 }
-void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+void launch_pair_emit(const Bodies& B, const StepParams& sp, const SortedKeys& skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* pcnt, const uint32_t* boff, uint32_t cap,
                       uint32_t* pa, uint32_t* pb, Counters* cnt, const uint32_t* stash, hipStream_t s) {
     if (!n) return;

@@ -46,8 +46,17 @@ struct PrevCacheArgs {
 void launch_counters_begin(Counters* cnt, uint32_t* kinds, hipStream_t s);
 // Start of a step: velocities, world inertia, AABBs, cell keys; clears the
 // colouring's per-body words.
-// Also resets the step's counters (as launch_counters_begin).
-void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* val, uint64_t* bmax0,
+// The sorted cell keys of the broadphase: 32-bit in the Morton rank mode (one world, where
+// a key is the cell's Morton code), else 64-bit (world << 30 | Morton code); one is set.
+struct SortedKeys {
+    const uint64_t* k64 = nullptr;
+    const uint32_t* k32 = nullptr;
+    __host__ __device__ uint64_t operator[](uint32_t s) const { return k32 ? (uint64_t)k32[s] : k64[s]; }
+};
+// Also resets the step's counters (as launch_counters_begin).  The keys go to key32 (the
+// Morton code) with the Morton bitmap, else to key.
+void launch_begin(const Bodies& B, const StepParams& sp, uint32_t n, uint64_t* key, uint32_t* key32, uint32_t* val,
+                  uint64_t* bmax0,
                   uint64_t* bmax1, uint64_t* bmask, uint32_t* bdeg, Counters* cnt, uint32_t* kinds,
                   const CellTable& ct, hipStream_t s);
 // Islands of the step, after the narrowphase: union-find over the manifolds between awake
@@ -63,17 +72,17 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);
 // Empties the cell hash table and its occupancy bits (nothing with the Morton bitmap,
 // whose words k_begin clears).
 void launch_cell_clear(const CellTable& t, hipStream_t s);
-void launch_cell_table(const Bodies& B, const uint64_t* skey, const uint32_t* sval, uint32_t n, const CellTable& t,
+void launch_cell_table(const Bodies& B, const SortedKeys& skey, const uint32_t* sval, uint32_t n, const CellTable& t,
                        Counters* cnt, uint32_t* scan_tmp, hipStream_t s);
 // Pairs per (body, z slice) part, BP_PARTS * n counts in canonical order (stashing
 // the first PAIR_STASH partners of each part), then the emit pass into pa / pb at
 // the scanned offsets.
 // Pair counts per (body, z slice) part in cnt, and per 256-part workgroup in bsum (the
 // emit pass takes the exclusive scan of bsum, with the total at [workgroups], as boff).
-void launch_pair_count(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+void launch_pair_count(const Bodies& B, const StepParams& sp, const SortedKeys& skey, const uint32_t* sval,
                        const CellTable& ct, uint32_t n, uint32_t* cnt, uint32_t* bsum, uint32_t* stash, Counters* ctr,
                        hipStream_t s);
-void launch_pair_emit(const Bodies& B, const StepParams& sp, const uint64_t* skey, const uint32_t* sval,
+void launch_pair_emit(const Bodies& B, const StepParams& sp, const SortedKeys& skey, const uint32_t* sval,
                       const CellTable& ct, uint32_t n, const uint32_t* pcnt, const uint32_t* boff, uint32_t cap,
                       uint32_t* pa, uint32_t* pb, Counters* cnt, const uint32_t* stash, hipStream_t s);
 // Narrowphase of every pair: the pairs are first ordered by kind of test (p_kind,
