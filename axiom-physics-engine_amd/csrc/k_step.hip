@@ -1609,6 +1609,12 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
         japply(Dn, d, A, Bd);
     }
 }
+// The projections of the register-row visits, one instruction each: x clamped to [-m, m]
+// (m >= 0) and x clamped below at zero.  For finite x they give the values of
+// x < -m ? -m : (x > m ? m : x) and x > 0 ? x : 0, except that a zero may come out with the
+// other sign; every later use of the impulse is a sum, product or comparison, so no later
+// value differs.
+__device__ __forceinline__ float clamp_sym(float x, float m) { return __builtin_amdgcn_fmed3f(x, -m, m); }
 // The same with every row of the visit already in registers (Jr[k][r]: row r of
 // point k), so that a visit whose rows are not staged waits for one batch of loads
 // instead of one per row group.  Same arithmetic, same results.  The b-side rows of a
@@ -1662,7 +1668,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
             float mT1;
             const Dir D1 = dir_regs(Jr[k], 1, mT1, maxf);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
-            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            nl = clamp_sym(nl, maxf);
             const float d = nl - L[k].y;
             L[k].y = nl;
             japply_both(D1, d, A, Bd);
@@ -1671,7 +1677,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
             float mT2;
             const Dir D2 = dir_regs(Jr[k], 2, mT2, w3);
             float nl = fmaf(-mT2, jvel(D2, A, Bd), keep * L[k].z);
-            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            nl = clamp_sym(nl, maxf);
             const float d = nl - L[k].z;
             L[k].z = nl;
             japply_both(D2, d, A, Bd);
@@ -1683,7 +1689,7 @@ __device__ __forceinline__ void solve_regs(const float4 (&Jr)[4][JROWS], uint32_
         float mN, tg;
         const Dir Dn = dir_regs(Jr[k], 0, mN, tg);
         float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L[k].x);
-        nl = nl > 0.0f ? nl : 0.0f;
+        nl = fmaxf(nl, 0.0f);
         const float d = nl - L[k].x;
         L[k].x = nl;
         japply_both(Dn, d, A, Bd);
@@ -1944,13 +1950,13 @@ __device__ __forceinline__ float point_residual_regs(const float4 (&Jr)[JROWS], 
     const Dir D2 = dir_regs(Jr, 2, mT2, x3);
     const float kN = Jr[1].w, kT1 = Jr[6].w, kT2 = Jr[11].w;
     float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
-    nl = nl > 0.0f ? nl : 0.0f;
+    nl = fmaxf(nl, 0.0f);
     float e = fabsf(nl - L.x) * kN;
     float q1 = fmaf(-mT1, jvel(D1, A, Bd), keep * L.y);
-    q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
+    q1 = clamp_sym(q1, maxf);
     const float e1 = fabsf(q1 - L.y) * kT1;
     float q2 = fmaf(-mT2, jvel(D2, A, Bd), keep * L.z);
-    q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
+    q2 = clamp_sym(q2, maxf);
     const float e2 = fabsf(q2 - L.z) * kT2;
     if (e1 > e) e = e1;
     if (e2 > e) e = e2;
