@@ -1553,6 +1553,12 @@ __device__ __forceinline__ Dir load_dir(const float4* J, size_t rs, int dir, boo
     return D;
 }
 
+// The projections of a visit, one instruction each: x clamped to [-m, m]
+// (m >= 0) and x clamped below at zero.  For finite x they give the values of
+// x < -m ? -m : (x > m ? m : x) and x > 0 ? x : 0, except that a zero may come out with the
+// other sign; every later use of the impulse is a sum, product or comparison, so no later
+// value differs.
+__device__ __forceinline__ float clamp_sym(float x, float m) { return __builtin_amdgcn_fmed3f(x, -m, m); }
 // One visit of a manifold on registers: warm start (first sweep only), friction
 // rows of every point, then normal rows.  Point k's rows start at J + k * ps, rows
 // `rs` apart; L[k] holds the point impulses in and out.
@@ -1567,9 +1573,9 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
             const Dir Dn = load_dir(P, rs, 0, plane, w0, w1, w2, w3, w4);
             const Dir D1 = load_dir(P, rs, 1, plane, w0, w1, w2, w3, w4);
             const Dir D2 = load_dir(P, rs, 2, plane, w0, w1, w2, w3, w4);
-            japply(Dn, L[k].x, A, Bd);
-            japply(D1, L[k].y, A, Bd);
-            japply(D2, L[k].z, A, Bd);
+            japply_both(Dn, L[k].x, A, Bd);
+            japply_both(D1, L[k].y, A, Bd);
+            japply_both(D2, L[k].z, A, Bd);
         }
     }
 #pragma unroll
@@ -1581,19 +1587,19 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
             float mT1;
             const Dir D1 = load_dir(P, rs, 1, plane, mT1, w1, w2, maxf, w4);
             float nl = fmaf(-mT1, jvel(D1, A, Bd), keep * L[k].y);
-            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            nl = clamp_sym(nl, maxf);
             const float d = nl - L[k].y;
             L[k].y = nl;
-            japply(D1, d, A, Bd);
+            japply_both(D1, d, A, Bd);
         }
         {
             float mT2;
             const Dir D2 = load_dir(P, rs, 2, plane, mT2, w1, w2, w3, w4);
             float nl = fmaf(-mT2, jvel(D2, A, Bd), keep * L[k].z);
-            nl = nl < -maxf ? -maxf : (nl > maxf ? maxf : nl);
+            nl = clamp_sym(nl, maxf);
             const float d = nl - L[k].z;
             L[k].z = nl;
-            japply(D2, d, A, Bd);
+            japply_both(D2, d, A, Bd);
         }
     }
 #pragma unroll
@@ -1603,18 +1609,12 @@ __device__ __forceinline__ void solve_rows(const float4* J, size_t rs, size_t ps
         float mN, x1, x2, tg, x4;
         const Dir Dn = load_dir(P, rs, 0, plane, mN, x1, x2, tg, x4);
         float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L[k].x);
-        nl = nl > 0.0f ? nl : 0.0f;
+        nl = fmaxf(nl, 0.0f);
         const float d = nl - L[k].x;
         L[k].x = nl;
-        japply(Dn, d, A, Bd);
+        japply_both(Dn, d, A, Bd);
     }
 }
-// The projections of the register-row visits, one instruction each: x clamped to [-m, m]
-// (m >= 0) and x clamped below at zero.  For finite x they give the values of
-// x < -m ? -m : (x > m ? m : x) and x > 0 ? x : 0, except that a zero may come out with the
-// other sign; every later use of the impulse is a sum, product or comparison, so no later
-// value differs.
-__device__ __forceinline__ float clamp_sym(float x, float m) { return __builtin_amdgcn_fmed3f(x, -m, m); }
 // The same with every row of the visit already in registers (Jr[k][r]: row r of
 // point k), so that a visit whose rows are not staged waits for one batch of loads
 // instead of one per row group.  Same arithmetic, same results.  The b-side rows of a
@@ -1928,13 +1928,13 @@ __device__ __forceinline__ float point_residual(const float4* J, size_t rs, bool
     const Dir D1 = load_dir(J, rs, 1, plane, mT1, kT1, x2, maxf, x4);
     const Dir D2 = load_dir(J, rs, 2, plane, mT2, kT2, x2, x3, x4);
     float nl = fmaf(mN, tg - jvel(Dn, A, Bd), keep * L.x);
-    nl = nl > 0.0f ? nl : 0.0f;
+    nl = fmaxf(nl, 0.0f);
     float e = fabsf(nl - L.x) * kN;
     float q1 = fmaf(-mT1, jvel(D1, A, Bd), keep * L.y);
-    q1 = q1 < -maxf ? -maxf : (q1 > maxf ? maxf : q1);
+    q1 = clamp_sym(q1, maxf);
     const float e1 = fabsf(q1 - L.y) * kT1;
     float q2 = fmaf(-mT2, jvel(D2, A, Bd), keep * L.z);
-    q2 = q2 < -maxf ? -maxf : (q2 > maxf ? maxf : q2);
+    q2 = clamp_sym(q2, maxf);
     const float e2 = fabsf(q2 - L.z) * kT2;
     if (e1 > e) e = e1;
     if (e2 > e) e = e2;
