@@ -1184,26 +1184,6 @@ __global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
     if (i < n) p[i] = i;
 }
 
-// colour offsets: off[c] = first sorted position with colour >= c (c = 0..K)
-__device__ __forceinline__ void col_offsets_part(const uint32_t* __restrict__ scol, uint32_t max_k,
-                                                 uint32_t* __restrict__ off, Counters* __restrict__ cnt, uint32_t gid,
-                                                 uint32_t gsz) {
-    const uint32_t K = cnt->max_colour, M = cnt->m_eff;
-    if (gid == 0 && K > max_k) atomicOr(&cnt->sticky, STICKY_COLOURS);
-    for (uint32_t c = gid; c <= K && c <= max_k; c += gsz) {
-        uint32_t lo = 0, hi = M;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (scol[mid] < c) lo = mid + 1; else hi = mid;
-        }
-        off[c] = lo;
-        if (c == K) cnt->m_glob = lo;
-    }
-}
-__global__ void k_col_offsets(const uint32_t* __restrict__ scol, uint32_t max_k, uint32_t* __restrict__ off,
-                              Counters* __restrict__ cnt) {
-    col_offsets_part(scol, max_k, off, cnt, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
 
 // ---------------------------------------------------------------- solver
 struct SB {
@@ -3081,9 +3061,6 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
 void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
                            uint32_t* val, const Counters* cnt, uint32_t split, hipStream_t s) {
     hipLaunchKernelGGL(k_col_sort_input, grid_bounded(cap), dim3(256), 0, s, B, m_ab, col, cap, key, val, cnt, split);
-}
-void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s) {
-    hipLaunchKernelGGL(k_col_offsets, grid_for(max_k + 1), dim3(256), 0, s, scol, max_k, off, cnt);
 }
 void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R, Counters* cnt, uint32_t cap,
                       const uint32_t* skey, uint2* list, float tol, uint32_t max_sweeps, uint32_t groups,

@@ -107,7 +107,6 @@ void launch_colour(const Bodies& B, const uint2* m_ab, uint32_t cap_man, uint32_
 void launch_col_sort_input(const Bodies& B, const uint2* m_ab, const uint32_t* col, uint32_t cap, uint32_t* key,
                            uint32_t* val, const Counters* cnt, uint32_t split,
                            hipStream_t s);
-void launch_col_offsets(const uint32_t* scol, uint32_t max_k, uint32_t* off, Counters* cnt, hipStream_t s);
 // The islands solved on their own (after the row preparation, which built their work
 // list): one lane or one workgroup per island; tol > 0 stops each at its own residual.
 // With flow_bmask (the colouring's per-body colour masks), its first launch also sets up
@@ -117,12 +116,13 @@ void launch_pgs_local(const Bodies& B, const StepParams& sp, const SolverRows& R
                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
                       const uint64_t* flow_bmask = nullptr);
 // Work the row preparation launch also does for the launches after it, each in a loop of
-// its own over the grid: the colour offsets of the sorted keys skey (as launch_col_offsets,
-// when col_off is set), the small islands' work list from the same keys (when list is set;
+// its own over the grid: the colour offsets of the sorted keys skey (col_off[c] = first
+// position whose key has colour >= c, c = 0..K, and m_glob = col_off[K], when col_off is
+// set), the small islands' work list from the same keys (when list is set;
 // launch_pgs_local then builds none), clearing zero_n words (the sweep's residual words),
 // and emptying the (a, b) -> slot table of row set cset (the entries its last build
 // recorded, at the positions cpos holds; when ckey is set, the table's build is then made
-// without clearing it).  More than max_k colours fail the step as launch_col_offsets does.
+// without clearing it).  More than max_k colours fail the step (STICKY_COLOURS).
 struct PrepExtra {
     const uint32_t* skey = nullptr;
     uint32_t* col_off = nullptr;
